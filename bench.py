@@ -1,0 +1,192 @@
+"""Benchmark: INSTA per-sample hot path on one MI355X.
+
+Workload (the full-size FP32 configuration): a 5,023-vertex / 9,976-triangle
+head mesh with 16 posed frames (device-built BVHs), hash grid L=16 F=2
+T=2^19 (N_min 16 -> N_max 2048), density MLP 32->64->16, colour MLP
+32->64->64->3, 128^3 occupancy bitfield per frame, 512x512 pinhole render
+with at most 1,024 steps per ray.  One step = one 512x512 frame rendered
+end to end (ray init, march, BVH deformation, encoding + MLPs, compositing),
+frames cycling through the 16 posed frames.  Inputs (meshes, parameters,
+occupancy) are resident in HBM before timing starts.
+
+Prints one JSON line: deformed samples/s (samples composited per second) as
+`value`, frames/s beside it, the dominant kernel's roofline position
+(measured with HIP events on the render stream), and the CPU oracle's
+throughput on a bounded pixel subset of the same frame.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+FP32_PEAK_TFLOPS = 157.3       # MI355X FP32 matrix/vector peak
+METRIC = "deformed samples/s and 512^2 frames/s, FP16-MFMA vs FP32; serving p50/p99 ms"
+
+
+def per_sample_costs(cfg, lay):
+    """Algorithmic work per sample (unpadded) for each kernel class."""
+    L, F = cfg.n_levels, cfg.n_features_per_level
+    w = cfg.n_neurons
+    flops = 2 * (lay.density_in * w + (cfg.density_hidden_layers - 1) * w * w + w * cfg.density_out)
+    flops += 2 * (lay.rgb_in * w + (cfg.rgb_hidden_layers - 1) * w * w + w * 3)
+    return {
+        # corner gathers L*8*F*4 B + canonical pos/dir in (32 B) + 4 outputs (16 B)
+        "network": dict(bytes=L * 8 * F * 4 + 32 + 16, flops=flops),
+        # posed pos + ray id in (16 B), ray direction (16 B), canonical pos/dir out (32 B)
+        "deform": dict(bytes=64, flops=0),
+        # network output + position per sample read (32 B)
+        "composite": dict(bytes=32, flops=0),
+        # emitted sample written (16 B)
+        "march": dict(bytes=16, flops=0),
+    }
+
+
+def setup(args):
+    import torch
+    from insta_amd import synth
+    from insta_amd.capi import Context, full_config, make_camera
+    cfg = full_config(max_frames=16)
+    av = synth.head_avatar(seed=1, n_frames=16)
+    ctx = Context(cfg)
+    ctx.set_canonical_mesh(av.canonical, av.faces, av.masks)
+    frames = torch.from_numpy(np.stack(av.frames)).cuda()
+    ctx.set_frames(0, frames)
+    params = synth.network_params(7, ctx.layout, cfg)
+    ctx.set_params(torch.from_numpy(params).cuda())
+    for f in range(16):
+        ctx.build_occupancy(f, 0.015)
+    rng = np.random.default_rng(2024)
+    cams = [make_camera(synth.random_camera(rng, 0.7), synth.focal_from_fov(args.res, 30.0)) for _ in range(16)]
+    torch.cuda.synchronize()
+    return ctx, cfg, av, params, cams
+
+
+def cpu_baseline(cfg, av, params, cam, res, budget_s):
+    """The oracle on the host cores for a bounded pixel subset of frame 0."""
+    from oracle.oracle import Oracle
+    orc = Oracle(cfg)
+    orc.set_canonical_mesh(av.canonical, av.faces, av.masks)
+    orc.set_frame(0, av.frames[0])
+    orc.set_params(params)
+    orc.build_occupancy(0, 0.015)
+    rng = np.random.default_rng(7)
+    n = 1024
+    pix = rng.permutation(res * res).astype(np.uint32)
+    t0 = time.perf_counter()
+    _, _, _, st = orc.render(cam, res, res, 0, 0, 0, pix[:n])
+    dt = time.perf_counter() - t0
+    samples, secs, used = int(st[0]), dt, n
+    # scale the pixel count so the timed run takes about budget_s
+    if dt < budget_s / 4:
+        n2 = int(min(res * res, n * budget_s / max(dt, 1e-3)))
+        t0 = time.perf_counter()
+        _, _, _, st = orc.render(cam, res, res, 0, 0, 0, pix[:n2])
+        secs = time.perf_counter() - t0
+        samples, used = int(st[0]), n2
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(value=samples / secs, unit="samples/s", cores=threads, kind="port",
+                sample=f"{used} random pixels of one {res}x{res} frame ({samples} samples, {secs:.1f} s)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=32)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--res", type=int, default=512)
+    ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds of CPU oracle timing (0: skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (every box has one GPU; "
+              "the path would shard by image tiles / avatars, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    ctx, cfg, av, params, cams = setup(args)
+    res = args.res
+    out = (torch.empty((res * res, 4), dtype=torch.float32, device="cuda"),
+           torch.empty(res * res, dtype=torch.float32, device="cuda"),
+           torch.empty(res * res, dtype=torch.int32, device="cuda"))
+
+    for s in range(args.warmup):
+        ctx.render(cams[s % 16], res, res, s % 16, s % 16, 0, out)
+    torch.cuda.synchronize()
+
+    ctx.set_profiling(True)
+    samples = 0
+    emitted = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        f = s % 16
+        ctx.render(cams[f], res, res, f, f, 0, out)
+        st = ctx.render_stats()
+        samples += st.n_samples
+        emitted += st.n_samples_emitted
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = ctx.profile()
+    ctx.set_profiling(False)
+
+    value = samples / elapsed
+    costs = per_sample_costs(cfg, ctx.layout)
+    # dominant kernel by summed event time
+    dom = max((k for k in prof if prof[k]["launches"]), key=lambda k: prof[k]["ms"])
+    p = prof[dom]
+    avg_s = p["ms"] / p["launches"] / 1e3
+    per_launch_items = p["items"] / p["launches"]
+    c = costs.get(dom, dict(bytes=0, flops=0))
+    if dom == "network":
+        achieved = c["flops"] * per_launch_items / avg_s / 1e12
+        roof = dict(bound="mfma", achieved=achieved, peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
+                    frac=achieved / FP32_PEAK_TFLOPS, traffic=None, kernel="network_soa_kernel<16,2>",
+                    per_unit=f"{c['flops']} FLOP/sample (unpadded), {c['bytes']} B/sample gathered+streamed",
+                    hbm_achieved_gbs=c["bytes"] * per_launch_items / avg_s / 1e9)
+    else:
+        achieved = c["bytes"] * per_launch_items / avg_s / 1e9
+        roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
+                    traffic=None, kernel=dom, per_unit=f"{c['bytes']} B/sample")
+    kernels = {k: dict(ms_per_frame=v["ms"] / args.steps, launches=v["launches"], items=v["items"])
+               for k, v in prof.items() if v["launches"]}
+
+    cpu = None
+    if args.cpu_budget > 0:
+        cpu = cpu_baseline(cfg, av, params, cams[0], res, args.cpu_budget)
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded ellipsoid head, Gaussian expression bases, random-init hash grid + MLPs)",
+        "config": {"workload": "full-size FP32 render: 9,976-tri head, 16 frames, L16 F2 T2^19, 128^3 grid, "
+                               f"{res}x{res}, <=1024 steps", "resolution": res, "frames": 16,
+                   "samples_per_frame": samples / args.steps, "emitted_per_frame": emitted / args.steps},
+        "frames_per_s": args.steps / elapsed,
+        "roofline": roof,
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,254 @@
+/*
+ * insta_capi.h -- C ABI of the MI355X-native INSTA per-sample hot path.
+ *
+ * One context (`insta_ctx`) holds one avatar: its canonical mesh, the posed
+ * frames with their device-built BVHs and per-triangle frames, the network
+ * parameters (hash grid + density/colour MLPs), the occupancy bitfields and
+ * the render workspace.  Every entry point is stream-ordered on the
+ * caller-given HIP stream (passed as `void*`, a `hipStream_t`), returns an
+ * `insta_status`, and never throws; `insta_last_error()` gives a message for
+ * the last failure on the calling thread.  Device pointers (`*_dev`) are
+ * caller-owned device memory; the context never allocates per call once its
+ * workspace has grown to the largest request seen (render workspace grows on
+ * demand, BVH/frame storage is sized at creation from `max_frames` and
+ * `max_triangles`).
+ *
+ * Coordinate layout of a sample ("NerfCoordinate", 11 + n_extra_dims floats,
+ * instant-ngp/include/neural-graphics-primitives/nerf.h:86-115):
+ *   [0..2] position (unit-cube world coordinates)   [3] dt (warped)
+ *   [4] holder = frame id (negative: skip)           [5..7] deformed position
+ *   [8..10] direction mapped to [0,1]^3 ((d+1)/2)    [11..] extra dims
+ */
+#ifndef INSTA_CAPI_H
+#define INSTA_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum insta_status {
+    INSTA_OK = 0,
+    INSTA_ERR_INVALID = 1,     /* bad argument or size */
+    INSTA_ERR_HIP = 2,         /* a HIP runtime call failed */
+    INSTA_ERR_OOM = 3,         /* device allocation failed */
+    INSTA_ERR_STATE = 4,       /* call order violated (e.g. render before mesh) */
+    INSTA_ERR_UNSUPPORTED = 5  /* configuration outside what the kernels handle */
+} insta_status;
+
+enum { INSTA_PRECISION_FP32 = 0, INSTA_PRECISION_FP16 = 1, INSTA_PRECISION_BF16 = 2 };
+
+/* Network/encoding/march configuration.  Field meanings follow the
+ * reference's JSON keys (configs/nerf/main.json:33-69) and testbed constants
+ * (instant-ngp/src/testbed_nerf.cu:48-68, testbed.h:669). */
+typedef struct insta_config {
+    /* multiresolution hash grid ("encoding": HashGrid) */
+    uint32_t n_levels;               /* L */
+    uint32_t n_features_per_level;   /* F in {1, 2, 4, 8} */
+    uint32_t log2_hashmap_size;      /* T = 2^log2_hashmap_size entries per level (cap) */
+    uint32_t base_resolution;        /* N_min */
+    float    per_level_scale;        /* b */
+    /* bias-free ReLU MLPs of width n_neurons ("network", "rgb_network") */
+    uint32_t n_neurons;              /* 64 */
+    uint32_t density_hidden_layers;  /* >= 1 */
+    uint32_t density_out;            /* 16: rgb input = density_out + SH coefficients */
+    uint32_t rgb_hidden_layers;      /* >= 1 */
+    uint32_t sh_degree;              /* 4 -> 16 coefficients */
+    uint32_t n_extra_dims;           /* appended to the density input */
+    float    density_logit_bias;     /* sigma = exp(logit + bias); 0 = reference */
+    /* occupancy grid and ray march (cone angle 0, one cascade) */
+    uint32_t grid_size;              /* G: occupancy cells per axis over [0,1]^3 */
+    float    dt;                     /* fixed step (reference: sqrt(3)/1024) */
+    uint32_t max_steps;              /* cap on composited samples per ray */
+    float    min_transmittance;      /* early termination (reference 0.01) */
+    uint32_t disable_interp;         /* 1: never blend neighbour frames */
+    uint32_t precision;              /* INSTA_PRECISION_* for tables/MLP */
+    /* capacities */
+    uint32_t max_frames;
+    uint32_t max_triangles;          /* <= INSTA_MAX_TRIANGLES */
+    uint32_t max_vertices;
+} insta_config;
+
+#define INSTA_MAX_TRIANGLES 16384u
+#define INSTA_MAX_LEVELS 32u
+
+/* Pinhole camera, instant-ngp convention: `matrix` is 3x4 row-major with
+ * columns right, down, forward, position (common_device.cuh:202-222). */
+typedef struct insta_camera {
+    float matrix[12];
+    float focal_px[2];
+    float screen_center[2];          /* (0.5, 0.5) = image centre */
+    uint32_t snap_to_pixel_centers;  /* 1: pixel centres, else ld jitter by spp */
+} insta_camera;
+
+typedef struct insta_render_stats {
+    uint64_t n_rays;           /* rays initialised (pixels) */
+    uint64_t n_rays_alive0;    /* rays that entered the render box */
+    uint64_t n_samples;        /* samples composited (the deformed-sample count) */
+    uint64_t n_samples_emitted;/* samples marched, deformed and evaluated */
+    uint32_t n_iterations;     /* march/deform/network/composite rounds */
+} insta_render_stats;
+
+/* Parameter layout: density MLP weights, colour MLP weights, hash tables.
+ * Weights are row-major [out][in]; the order mirrors NerfNetwork::set_params
+ * (instant-ngp/include/neural-graphics-primitives/nerf_network.h:492-525). */
+typedef struct insta_layout {
+    uint64_t n_params;
+    uint64_t n_mlp_params;
+    uint64_t n_grid_params;        /* sum over levels of level_size * F */
+    uint32_t density_in;           /* L*F + n_extra_dims */
+    uint32_t rgb_in;               /* density_out + (sh_degree)^2 */
+    uint32_t level_offset[INSTA_MAX_LEVELS];  /* entries before level l */
+    uint32_t level_size[INSTA_MAX_LEVELS];    /* entries in level l */
+    uint32_t level_resolution[INSTA_MAX_LEVELS];
+    float    level_scale[INSTA_MAX_LEVELS];
+    uint32_t level_dense[INSTA_MAX_LEVELS];   /* 1: dense index, 0: spatial hash */
+} insta_layout;
+
+const char* insta_last_error(void);
+const char* insta_version(void);
+
+/* Default configuration (the full-size workload shapes). */
+void insta_default_config(insta_config* cfg);
+
+/* Parameter/table layout for a configuration (host-only, no device). */
+insta_status insta_query_layout(const insta_config* cfg, insta_layout* out);
+
+typedef struct insta_ctx insta_ctx;
+
+/* Testbed::reset_network + rta::Core construction
+ * (instant-ngp/src/testbed.cu:2181-2412, src/core.cu:199-221): allocates the
+ * context on the current HIP device with capacities from `cfg`. */
+insta_status insta_create(const insta_config* cfg, insta_ctx** out);
+void insta_destroy(insta_ctx* ctx);
+
+/* Canonical mesh + region masks + edge adjacency
+ * (src/core.cu:283-304, src/tiny_mesh.cu:134-164, 316-371).  Host arrays:
+ * verts [nv*3] float (already offset into the unit cube), faces [nf*3] int32,
+ * mask_ids [nf] uint32 (FlameMasks values).  Also fixes the render box:
+ * canonical vertex AABB inflated by 0.15 * |diag|, clipped to [0,1]^3
+ * (tiny_mesh.cu:157-162, core.cu:289, testbed_nerf.cu:2675-2682). */
+insta_status insta_set_canonical_mesh(insta_ctx* ctx, const float* verts, uint32_t nv,
+                                      const int32_t* faces, uint32_t nf,
+                                      const uint32_t* mask_ids, void* stream);
+
+/* Posed frames: per-frame triangles, device BVH build and FP64-built
+ * per-triangle frames, replacing the per-frame TinyMesh + CPU
+ * TriangleBvh::build (src/core.cu:309-331, src/tiny_mesh.cu:134-178,
+ * instant-ngp/src/triangle_bvh.cu:548-631).  `verts_dev` holds n_frames
+ * consecutive [nv*3] float vertex arrays on the device (same topology as the
+ * canonical mesh); frames first_frame .. first_frame+n_frames-1 are rebuilt in
+ * one launch. */
+insta_status insta_set_frames(insta_ctx* ctx, uint32_t first_frame, uint32_t n_frames,
+                              const float* verts_dev, void* stream);
+
+/* Expression blend on the device: frame vertices = canonical + sum_k
+ * coeffs[k] * bases[k] (bases_dev [K][nv][3]), then the frame's BVH/frames are
+ * rebuilt as insta_set_frames does.  Serving path for per-request expressions. */
+insta_status insta_set_frame_expression(insta_ctx* ctx, uint32_t frame, const float* bases_dev,
+                                        uint32_t n_bases, const float* coeffs_dev, void* stream);
+
+/* Nearest triangle of each point on a frame's mesh (frame = -1: canonical),
+ * the query of `closest_triangle`
+ * (instant-ngp/include/neural-graphics-primitives/bvh.cuh:48-96).  points_dev
+ * [n*3]; ids_dev [n] receives the original face index (-1 if none);
+ * dist_dev [n] (nullable) receives the Euclidean distance. */
+insta_status insta_closest_triangles(insta_ctx* ctx, int32_t frame, const float* points_dev,
+                                     uint32_t n, int32_t* ids_dev, float* dist_dev, void* stream);
+
+/* Export a frame's BVH (frame = -1: canonical) for inspection: nodes_dev
+ * [n_nodes*8] floats (min xyz, begin, max xyz, end; begin/end are int32 bit
+ * patterns, -1 for inner nodes), order_dev [nf] = original face index of
+ * each BVH-order triangle.  Pass NULL pointers to query n_nodes only. */
+insta_status insta_export_bvh(insta_ctx* ctx, int32_t frame, float* nodes_dev, uint32_t* order_dev,
+                              uint32_t* n_nodes, void* stream);
+
+/* Posed -> canonical deformation of NerfCoordinate samples in place: the
+ * `warp` kernel behind Core::surface_closest_point and Core::render_deformed
+ * (src/core.cu:442-526, 581-637).  frame = -1 takes each sample's frame from
+ * its holder field; otherwise all samples use `frame`.  stride_floats =
+ * 11 + n_extra_dims. */
+insta_status insta_deform_samples(insta_ctx* ctx, float* coords_dev, uint32_t n,
+                                  uint32_t stride_floats, int32_t frame, void* stream);
+
+/* Occupancy bitfield from mesh distance: a cell (Morton order, G^3 cells over
+ * [0,1]^3) is occupied when its centre is closer than `threshold` to the mesh
+ * and the nearest triangle's mask id is not 0
+ * (instant-ngp/src/testbed_nerf.cu:364-389, 585-609).  frame = -1 uses the
+ * canonical mesh and fills the canonical grid slot, frame >= 0 the frame's
+ * own slot.  bitfield_dev (nullable) receives a copy of the G^3/8 bytes. */
+insta_status insta_build_occupancy(insta_ctx* ctx, int32_t frame, float threshold,
+                                   uint8_t* bitfield_dev, void* stream);
+
+/* Upload an occupancy bitfield (G^3/8 bytes, Morton order) into a slot. */
+insta_status insta_set_occupancy(insta_ctx* ctx, int32_t slot, const uint8_t* bitfield_dev,
+                                 void* stream);
+
+/* NerfNetwork::set_params (nerf_network.h:492-525): params_dev holds
+ * layout.n_params FP32 values in the declared order; the context keeps FP32
+ * master copies and, for reduced precision, converted tables/weights. */
+insta_status insta_set_params(insta_ctx* ctx, const float* params_dev, uint64_t n_params,
+                              void* stream);
+
+/* Hash-grid encoding forward (tiny-cuda-nn GridEncoding, absent from the
+ * reference tree; called from nerf_network.h:168-174): positions_dev [n*3]
+ * in [0,1]^3, features_dev [n * L*F] row-major (sample-major). */
+insta_status insta_encode(insta_ctx* ctx, const float* positions_dev, uint32_t n,
+                          float* features_dev, void* stream);
+
+/* Hash-grid backward: grad_tables_dev[layout.n_grid_params] +=
+ * sum over samples of dL/dfeature * trilinear weight (scatter-add).
+ * dL_dfeatures_dev [n * L*F] row-major. */
+insta_status insta_encode_backward(insta_ctx* ctx, const float* positions_dev, uint32_t n,
+                                   const float* dL_dfeatures_dev, float* grad_tables_dev,
+                                   void* stream);
+
+/* NerfNetwork::inference_mixed_precision_impl (nerf_network.h:159-232) on
+ * canonical NerfCoordinate samples: out_dev [n*4] = rgb logits (3) and the
+ * density logit (density-MLP output 0). */
+insta_status insta_network_inference(insta_ctx* ctx, const float* coords_dev, uint32_t n,
+                                     uint32_t stride_floats, float* out_dev, void* stream);
+
+/* Testbed::render_frame -> render_nerf -> NerfTracer::{init_rays_from_camera,
+ * trace} with the render_deformed hook (testbed_nerf.cu:2030-2244,
+ * 2320-2461; core.cu:346-348): renders `frame` through occupancy slot
+ * `occupancy` (-1 canonical) at sample index `spp`.  rgba_dev [w*h*4]
+ * (composited RGBA, 0 where the ray collected alpha <= 0.001),
+ * depth_dev [w*h] (nullable; 1e10 where alpha <= 0.2), steps_dev [w*h]
+ * (nullable; samples composited per pixel). */
+insta_status insta_render(insta_ctx* ctx, const insta_camera* cam, uint32_t width, uint32_t height,
+                          int32_t frame, int32_t occupancy, uint32_t spp, float* rgba_dev,
+                          float* depth_dev, uint32_t* steps_dev, void* stream);
+
+/* Counters of the last insta_render on this context. */
+insta_status insta_get_render_stats(insta_ctx* ctx, insta_render_stats* out);
+
+/* Per-kernel timing of insta_render, measured with HIP events recorded on the
+ * render stream around each kernel class.  No counterpart in the reference
+ * (whose only timers are EMA frame times, testbed.cu:1600, 2484-2512); it
+ * exists so throughput can be divided among kernels.  Classes: */
+enum {
+    INSTA_KERNEL_INIT = 0,      /* ray init + jitter + empty-space skip */
+    INSTA_KERNEL_MARCH = 1,     /* sample emission */
+    INSTA_KERNEL_DEFORM = 2,    /* BVH nearest triangle + frame transform */
+    INSTA_KERNEL_NETWORK = 3,   /* hash encoding + density/colour MLPs */
+    INSTA_KERNEL_COMPOSITE = 4, /* front-to-back compositing + compaction */
+    INSTA_KERNEL_FINALIZE = 5,  /* framebuffer write */
+    INSTA_KERNEL_CLASSES = 8
+};
+typedef struct insta_profile {
+    double ms[INSTA_KERNEL_CLASSES];         /* summed event time */
+    uint64_t launches[INSTA_KERNEL_CLASSES];
+    uint64_t items[INSTA_KERNEL_CLASSES];    /* rays or samples processed */
+} insta_profile;
+/* enable != 0 starts (and zeroes) accumulation; 0 stops it. */
+insta_status insta_set_profiling(insta_ctx* ctx, int enable);
+insta_status insta_get_profile(insta_ctx* ctx, insta_profile* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* INSTA_CAPI_H */

@@ -1,0 +1,329 @@
+"""ctypes binding of the C ABI in include/insta_capi.h.
+
+Device buffers are torch tensors on the current HIP device (PyTorch is only
+used for device memory and streams here); every call passes raw pointers
+through the C ABI, exactly as a C/C++ caller would.  Loading fails loudly if
+the HIP library has not been built: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libinsta_amd.so"
+
+INSTA_OK = 0
+STATUS_NAMES = {0: "OK", 1: "INVALID", 2: "HIP", 3: "OOM", 4: "STATE", 5: "UNSUPPORTED"}
+PRECISION_FP32, PRECISION_FP16, PRECISION_BF16 = 0, 1, 2
+MAX_LEVELS = 32
+MAX_TRIANGLES = 16384
+
+
+class InstaConfig(C.Structure):
+    _fields_ = [
+        ("n_levels", C.c_uint32), ("n_features_per_level", C.c_uint32), ("log2_hashmap_size", C.c_uint32),
+        ("base_resolution", C.c_uint32), ("per_level_scale", C.c_float),
+        ("n_neurons", C.c_uint32), ("density_hidden_layers", C.c_uint32), ("density_out", C.c_uint32),
+        ("rgb_hidden_layers", C.c_uint32), ("sh_degree", C.c_uint32), ("n_extra_dims", C.c_uint32),
+        ("density_logit_bias", C.c_float),
+        ("grid_size", C.c_uint32), ("dt", C.c_float), ("max_steps", C.c_uint32), ("min_transmittance", C.c_float),
+        ("disable_interp", C.c_uint32), ("precision", C.c_uint32),
+        ("max_frames", C.c_uint32), ("max_triangles", C.c_uint32), ("max_vertices", C.c_uint32),
+    ]
+
+
+class InstaCamera(C.Structure):
+    _fields_ = [("matrix", C.c_float * 12), ("focal_px", C.c_float * 2), ("screen_center", C.c_float * 2),
+                ("snap_to_pixel_centers", C.c_uint32)]
+
+
+class InstaRenderStats(C.Structure):
+    _fields_ = [("n_rays", C.c_uint64), ("n_rays_alive0", C.c_uint64), ("n_samples", C.c_uint64),
+                ("n_samples_emitted", C.c_uint64), ("n_iterations", C.c_uint32)]
+
+
+class InstaLayout(C.Structure):
+    _fields_ = [
+        ("n_params", C.c_uint64), ("n_mlp_params", C.c_uint64), ("n_grid_params", C.c_uint64),
+        ("density_in", C.c_uint32), ("rgb_in", C.c_uint32),
+        ("level_offset", C.c_uint32 * MAX_LEVELS), ("level_size", C.c_uint32 * MAX_LEVELS),
+        ("level_resolution", C.c_uint32 * MAX_LEVELS), ("level_scale", C.c_float * MAX_LEVELS),
+        ("level_dense", C.c_uint32 * MAX_LEVELS),
+    ]
+
+    def mlp_shapes(self, cfg: InstaConfig) -> list[tuple[int, int]]:
+        """(out, in) of every weight matrix in parameter order."""
+        w = cfg.n_neurons
+        shapes = [(w, self.density_in)] + [(w, w)] * (cfg.density_hidden_layers - 1) + [(cfg.density_out, w)]
+        shapes += [(w, self.rgb_in)] + [(w, w)] * (cfg.rgb_hidden_layers - 1) + [(3, w)]
+        return shapes
+
+
+KERNEL_CLASSES = ["init", "march", "deform", "network", "composite", "finalize"]
+
+
+class InstaProfile(C.Structure):
+    _fields_ = [("ms", C.c_double * 8), ("launches", C.c_uint64 * 8), ("items", C.c_uint64 * 8)]
+
+
+# the C ABI: (name, restype, argtypes); tests check this list against the header
+_VP = C.c_void_p
+SIGNATURES = [
+    ("insta_last_error", C.c_char_p, []),
+    ("insta_version", C.c_char_p, []),
+    ("insta_default_config", None, [C.POINTER(InstaConfig)]),
+    ("insta_query_layout", C.c_int, [C.POINTER(InstaConfig), C.POINTER(InstaLayout)]),
+    ("insta_create", C.c_int, [C.POINTER(InstaConfig), C.POINTER(_VP)]),
+    ("insta_destroy", None, [_VP]),
+    ("insta_set_canonical_mesh", C.c_int, [_VP, _VP, C.c_uint32, _VP, C.c_uint32, _VP, _VP]),
+    ("insta_set_frames", C.c_int, [_VP, C.c_uint32, C.c_uint32, _VP, _VP]),
+    ("insta_set_frame_expression", C.c_int, [_VP, C.c_uint32, _VP, C.c_uint32, _VP, _VP]),
+    ("insta_closest_triangles", C.c_int, [_VP, C.c_int32, _VP, C.c_uint32, _VP, _VP, _VP]),
+    ("insta_export_bvh", C.c_int, [_VP, C.c_int32, _VP, _VP, C.POINTER(C.c_uint32), _VP]),
+    ("insta_deform_samples", C.c_int, [_VP, _VP, C.c_uint32, C.c_uint32, C.c_int32, _VP]),
+    ("insta_build_occupancy", C.c_int, [_VP, C.c_int32, C.c_float, _VP, _VP]),
+    ("insta_set_occupancy", C.c_int, [_VP, C.c_int32, _VP, _VP]),
+    ("insta_set_params", C.c_int, [_VP, _VP, C.c_uint64, _VP]),
+    ("insta_encode", C.c_int, [_VP, _VP, C.c_uint32, _VP, _VP]),
+    ("insta_encode_backward", C.c_int, [_VP, _VP, C.c_uint32, _VP, _VP, _VP]),
+    ("insta_network_inference", C.c_int, [_VP, _VP, C.c_uint32, C.c_uint32, _VP, _VP]),
+    ("insta_render", C.c_int, [_VP, C.POINTER(InstaCamera), C.c_uint32, C.c_uint32, C.c_int32, C.c_int32,
+                               C.c_uint32, _VP, _VP, _VP, _VP]),
+    ("insta_get_render_stats", C.c_int, [_VP, C.POINTER(InstaRenderStats)]),
+    ("insta_set_profiling", C.c_int, [_VP, C.c_int]),
+    ("insta_get_profile", C.c_int, [_VP, C.POINTER(InstaProfile)]),
+]
+
+_lib = None
+
+
+def load_library(path: Path | str = LIB_PATH) -> C.CDLL:
+    """Load the HIP library; raises if it is missing (build it first)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = Path(path)
+    if not p.exists():
+        raise RuntimeError(f"{p} is missing: run `python -c 'import __graft_entry__ as g; g.build()'` first")
+    lib = C.CDLL(str(p))
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class InstaError(RuntimeError):
+    def __init__(self, status: int, where: str, message: str):
+        super().__init__(f"{where}: {STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+def check(status: int, where: str) -> None:
+    if status != INSTA_OK:
+        raise InstaError(status, where, load_library().insta_last_error().decode())
+
+
+# ----------------------------------------------------------------------------
+# workload configurations
+# ----------------------------------------------------------------------------
+
+def default_config() -> InstaConfig:
+    cfg = InstaConfig()
+    load_library().insta_default_config(C.byref(cfg))
+    return cfg
+
+
+def make_config(**kw) -> InstaConfig:
+    """Full default config with overrides; fields are set verbatim."""
+    cfg = InstaConfig()
+    base = dict(n_levels=16, n_features_per_level=2, log2_hashmap_size=19, base_resolution=16,
+                per_level_scale=math.exp(math.log(2048.0 / 16.0) / 15.0), n_neurons=64,
+                density_hidden_layers=1, density_out=16, rgb_hidden_layers=2, sh_degree=4, n_extra_dims=0,
+                density_logit_bias=1.0, grid_size=128, dt=1.73205080757 / 1024.0, max_steps=1024,
+                min_transmittance=0.01, disable_interp=0, precision=PRECISION_FP32, max_frames=16,
+                max_triangles=MAX_TRIANGLES, max_vertices=1 << 16)
+    base.update(kw)
+    for k, v in base.items():
+        setattr(cfg, k, v)
+    return cfg
+
+
+def small_config(**kw) -> InstaConfig:
+    """Small workload: L=8 F=2 T=2^14 N_min=16 b=1.5, 64^3 grid, step 1/256,
+    at most 256 steps, MLP 16->64->16 / 32->64->64->3."""
+    base = dict(n_levels=8, log2_hashmap_size=14, per_level_scale=1.5, grid_size=64, dt=1.0 / 256.0,
+                max_steps=256, max_frames=4, max_triangles=4096, max_vertices=4096)
+    base.update(kw)
+    return make_config(**base)
+
+
+def full_config(**kw) -> InstaConfig:
+    """Full-size workload: L=16 F=2 T=2^19, N_min=16 -> N_max=2048, 128^3
+    grid, step sqrt(3)/1024, at most 1,024 steps, 16 frames."""
+    return make_config(**kw)
+
+
+def query_layout(cfg: InstaConfig) -> InstaLayout:
+    lay = InstaLayout()
+    check(load_library().insta_query_layout(C.byref(cfg), C.byref(lay)), "insta_query_layout")
+    return lay
+
+
+def make_camera(matrix: np.ndarray, focal: float, snap: bool = True, center=(0.5, 0.5)) -> InstaCamera:
+    cam = InstaCamera()
+    m = np.asarray(matrix, dtype=np.float32).reshape(3, 4)
+    for i, v in enumerate(m.ravel()):
+        cam.matrix[i] = float(v)
+    cam.focal_px[0] = cam.focal_px[1] = float(focal)
+    cam.screen_center[0], cam.screen_center[1] = center
+    cam.snap_to_pixel_centers = 1 if snap else 0
+    return cam
+
+
+# ----------------------------------------------------------------------------
+# context wrapper (device buffers are torch tensors)
+# ----------------------------------------------------------------------------
+
+def _ptr(t) -> int | None:
+    return None if t is None else t.data_ptr()
+
+
+class Context:
+    """One avatar on the current HIP device."""
+
+    def __init__(self, cfg: InstaConfig):
+        import torch
+        self.torch = torch
+        self.lib = load_library()
+        self.cfg = cfg
+        self.layout = query_layout(cfg)
+        h = C.c_void_p()
+        check(self.lib.insta_create(C.byref(cfg), C.byref(h)), "insta_create")
+        self.h = h
+        self.nv = self.nf = 0
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.insta_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def stream(self):
+        return C.c_void_p(self.torch.cuda.current_stream().cuda_stream)
+
+    def dev(self, a, dtype=None):
+        t = self.torch.as_tensor(np.ascontiguousarray(a))
+        if dtype is not None:
+            t = t.to(dtype)
+        return t.cuda()
+
+    # --- mesh -------------------------------------------------------------
+    def set_canonical_mesh(self, verts: np.ndarray, faces: np.ndarray, masks: np.ndarray) -> None:
+        v = np.ascontiguousarray(verts, dtype=np.float32)
+        f = np.ascontiguousarray(faces, dtype=np.int32)
+        m = np.ascontiguousarray(masks, dtype=np.uint32)
+        self.nv, self.nf = len(v), len(f)
+        check(self.lib.insta_set_canonical_mesh(self.h, v.ctypes.data, len(v), f.ctypes.data, len(f),
+                                                m.ctypes.data, self.stream()), "insta_set_canonical_mesh")
+
+    def set_frames(self, first: int, frames_dev) -> None:
+        n = frames_dev.shape[0] if frames_dev.dim() == 3 else 1
+        check(self.lib.insta_set_frames(self.h, first, n, _ptr(frames_dev.contiguous()), self.stream()),
+              "insta_set_frames")
+
+    def set_frame_expression(self, frame: int, bases_dev, coeffs_dev) -> None:
+        check(self.lib.insta_set_frame_expression(self.h, frame, _ptr(bases_dev), bases_dev.shape[0],
+                                                  _ptr(coeffs_dev), self.stream()), "insta_set_frame_expression")
+
+    def closest(self, frame: int, pts_dev):
+        n = pts_dev.shape[0]
+        ids = self.torch.empty(n, dtype=self.torch.int32, device=pts_dev.device)
+        dist = self.torch.empty(n, dtype=self.torch.float32, device=pts_dev.device)
+        check(self.lib.insta_closest_triangles(self.h, frame, _ptr(pts_dev), n, _ptr(ids), _ptr(dist),
+                                               self.stream()), "insta_closest_triangles")
+        return ids, dist
+
+    def export_bvh(self, frame: int):
+        nn = C.c_uint32()
+        check(self.lib.insta_export_bvh(self.h, frame, None, None, C.byref(nn), self.stream()), "insta_export_bvh")
+        nodes = self.torch.empty(nn.value * 8, dtype=self.torch.float32, device="cuda")
+        order = self.torch.empty(self.nf, dtype=self.torch.int32, device="cuda")
+        check(self.lib.insta_export_bvh(self.h, frame, _ptr(nodes), _ptr(order), C.byref(nn), self.stream()),
+              "insta_export_bvh")
+        return nodes, order
+
+    def deform_samples(self, coords_dev, frame: int = -1) -> None:
+        n, stride = coords_dev.shape
+        check(self.lib.insta_deform_samples(self.h, _ptr(coords_dev), n, stride, frame, self.stream()),
+              "insta_deform_samples")
+
+    def build_occupancy(self, frame: int, threshold: float):
+        G = self.cfg.grid_size
+        bits = self.torch.empty(G * G * G // 8, dtype=self.torch.uint8, device="cuda")
+        check(self.lib.insta_build_occupancy(self.h, frame, threshold, _ptr(bits), self.stream()),
+              "insta_build_occupancy")
+        return bits
+
+    def set_occupancy(self, slot: int, bits_dev) -> None:
+        check(self.lib.insta_set_occupancy(self.h, slot, _ptr(bits_dev), self.stream()), "insta_set_occupancy")
+
+    # --- network ----------------------------------------------------------
+    def set_params(self, params_dev) -> None:
+        check(self.lib.insta_set_params(self.h, _ptr(params_dev), params_dev.numel(), self.stream()),
+              "insta_set_params")
+
+    def encode(self, pos_dev):
+        n = pos_dev.shape[0]
+        out = self.torch.empty((n, self.cfg.n_levels * self.cfg.n_features_per_level), dtype=self.torch.float32,
+                               device=pos_dev.device)
+        check(self.lib.insta_encode(self.h, _ptr(pos_dev), n, _ptr(out), self.stream()), "insta_encode")
+        return out
+
+    def encode_backward(self, pos_dev, dL_dev, grad_dev) -> None:
+        check(self.lib.insta_encode_backward(self.h, _ptr(pos_dev), pos_dev.shape[0], _ptr(dL_dev), _ptr(grad_dev),
+                                             self.stream()), "insta_encode_backward")
+
+    def network_inference(self, coords_dev):
+        n, stride = coords_dev.shape
+        out = self.torch.empty((n, 4), dtype=self.torch.float32, device=coords_dev.device)
+        check(self.lib.insta_network_inference(self.h, _ptr(coords_dev), n, stride, _ptr(out), self.stream()),
+              "insta_network_inference")
+        return out
+
+    # --- render -----------------------------------------------------------
+    def render(self, cam: InstaCamera, width: int, height: int, frame: int, occupancy: int, spp: int = 0,
+               out=None):
+        torch = self.torch
+        if out is None:
+            out = (torch.empty((height * width, 4), dtype=torch.float32, device="cuda"),
+                   torch.empty(height * width, dtype=torch.float32, device="cuda"),
+                   torch.empty(height * width, dtype=torch.int32, device="cuda"))
+        rgba, depth, steps = out
+        check(self.lib.insta_render(self.h, C.byref(cam), width, height, frame, occupancy, spp, _ptr(rgba),
+                                    _ptr(depth), _ptr(steps), self.stream()), "insta_render")
+        return out
+
+    def set_profiling(self, on: bool) -> None:
+        check(self.lib.insta_set_profiling(self.h, 1 if on else 0), "insta_set_profiling")
+
+    def profile(self) -> dict:
+        p = InstaProfile()
+        check(self.lib.insta_get_profile(self.h, C.byref(p)), "insta_get_profile")
+        return {name: dict(ms=p.ms[i], launches=p.launches[i], items=p.items[i])
+                for i, name in enumerate(KERNEL_CLASSES)}
+
+    def render_stats(self) -> InstaRenderStats:
+        st = InstaRenderStats()
+        check(self.lib.insta_get_render_stats(self.h, C.byref(st)), "insta_get_render_stats")
+        return st

@@ -1,0 +1,335 @@
+// bvh.hip -- per-frame mesh preparation on the device: triangles, BVH, frames.
+//
+// Replaces the reference's per-frame host work (src/tiny_mesh.cu:134-178:
+// build_triangles + TriangleBvh::build on the CPU, then an H2D copy) with one
+// workgroup per frame that builds everything in LDS:
+//   1. centroids (float, (a+b+c)/3) and their AABB (workgroup reduction);
+//   2. keys = (30-bit Morton code of the quantised centroid) << 32 | face id;
+//   3. a bitonic sort of the keys in LDS (<= 16,384 keys = 128 KiB, gfx950's
+//      160 KiB LDS holds them in one workgroup);
+//   4. BVH-order triangle records, leaf boxes over 4^k count-balanced leaves,
+//      inner boxes bottom-up over the implicit 4-ary heap;
+//   5. per-triangle frames tri2projection (include/rta/affine.cuh:26-39) and
+//      their inverses, built and inverted in FP64, stored FP32.
+// Several frames build concurrently, one per CU.
+#include "context.hpp"
+
+namespace insta {
+namespace {
+
+constexpr uint32_t kBuildThreads = 1024;
+
+struct BuildArgs {
+    const float* verts;        // [n][nv*3] input vertices for the slots being built
+    const float* canon_verts;  // canonical vertices (slot 0 store) for area ratios
+    const int32_t* faces;
+    const uint32_t* masks;
+    float* verts_store;
+    float4* nodes;
+    float4* btris;
+    float4* proj;
+    float4* proj_inv;
+    float4* cen;
+    uint32_t first_slot, nv, nf, k, leaf_base, node_cap, tri_cap, vert_cap;
+};
+
+struct D3 {
+    double x, y, z;
+};
+__device__ inline D3 dsub(D3 a, D3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ inline double ddot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ inline D3 dcross(D3 a, D3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+__device__ inline D3 dnormalized(D3 a) {
+    double s = ddot(a, a);
+    if (s > 0.0) {
+        double n = sqrt(s);
+        return {a.x / n, a.y / n, a.z / n};
+    }
+    return a;
+}
+
+__device__ inline V3 load_v(const float* V, int32_t i) { return v3(V[3 * i], V[3 * i + 1], V[3 * i + 2]); }
+
+__device__ inline uint32_t quantise(float c, float lo, float e) {
+    if (!(e > 0.0f)) return 0u;
+    float q = (c - lo) / e * 1024.0f;
+    if (!(q > 0.0f)) return 0u;
+    uint32_t u = (uint32_t)q;
+    return u > 1023u ? 1023u : u;
+}
+
+__device__ inline float wave_min(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ inline float wave_max(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+
+__global__ __launch_bounds__(kBuildThreads) void build_slot_kernel(BuildArgs a) {
+    __shared__ uint64_t keys[INSTA_MAX_TRIANGLES];
+    __shared__ float red[6][kBuildThreads / 64];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t slot = a.first_slot + blockIdx.x;
+    const uint32_t nf = a.nf;
+    const float* V = a.verts + (size_t)blockIdx.x * a.nv * 3;
+    float* VS = a.verts_store + (size_t)slot * a.vert_cap * 3;
+    float4* cen = a.cen + (size_t)slot * a.tri_cap;
+    float4* btris = a.btris + (size_t)slot * a.tri_cap * 3;
+    float4* nodes = a.nodes + (size_t)slot * a.node_cap * 2;
+    float4* proj = a.proj + (size_t)slot * a.tri_cap * 3;
+    float4* proj_inv = a.proj_inv + (size_t)slot * a.tri_cap * 3;
+
+    for (uint32_t i = tid; i < a.nv * 3; i += kBuildThreads) VS[i] = V[i];
+
+    // 1. centroids and their bounds
+    V3 lo = v3(INFINITY, INFINITY, INFINITY), hi = v3(-INFINITY, -INFINITY, -INFINITY);
+    for (uint32_t i = tid; i < nf; i += kBuildThreads) {
+        const int32_t* f = a.faces + 3 * i;
+        V3 A = load_v(V, f[0]), B = load_v(V, f[1]), C = load_v(V, f[2]);
+        V3 c = v3((A.x + B.x + C.x) / 3.0f, (A.y + B.y + C.y) / 3.0f, (A.z + B.z + C.z) / 3.0f);
+        cen[i] = make_float4(c.x, c.y, c.z, 0.0f);
+        lo = vmin(lo, c);
+        hi = vmax(hi, c);
+    }
+    const uint32_t wave = tid / 64, lane = tid % 64;
+    {
+        float r[6] = {wave_min(lo.x), wave_min(lo.y), wave_min(lo.z), wave_max(hi.x), wave_max(hi.y), wave_max(hi.z)};
+        if (lane == 0)
+            for (int q = 0; q < 6; ++q) red[q][wave] = r[q];
+    }
+    __syncthreads();
+    V3 cmin = v3(INFINITY, INFINITY, INFINITY), cmax = v3(-INFINITY, -INFINITY, -INFINITY);
+    for (uint32_t w = 0; w < kBuildThreads / 64; ++w) {
+        cmin = vmin(cmin, v3(red[0][w], red[1][w], red[2][w]));
+        cmax = vmax(cmax, v3(red[3][w], red[4][w], red[5][w]));
+    }
+    const V3 ext = cmax - cmin;
+
+    // 2. Morton keys (padding keys sort last)
+    uint32_t npad = 1;
+    while (npad < nf) npad <<= 1;
+    for (uint32_t i = tid; i < npad; i += kBuildThreads) {
+        uint64_t key = ~0ull;
+        if (i < nf) {
+            const int32_t* f = a.faces + 3 * i;
+            V3 A = load_v(V, f[0]), B = load_v(V, f[1]), C = load_v(V, f[2]);
+            V3 c = v3((A.x + B.x + C.x) / 3.0f, (A.y + B.y + C.y) / 3.0f, (A.z + B.z + C.z) / 3.0f);
+            uint32_t m = morton3(quantise(c.x, cmin.x, ext.x), quantise(c.y, cmin.y, ext.y), quantise(c.z, cmin.z, ext.z));
+            key = ((uint64_t)m << 32) | i;
+        }
+        keys[i] = key;
+    }
+    __syncthreads();
+
+    // 3. bitonic sort, ascending
+    for (uint32_t size = 2; size <= npad; size <<= 1) {
+        for (uint32_t j = size >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = tid; i < npad / 2; i += kBuildThreads) {
+                const uint32_t x = ((i & ~(j - 1)) << 1) | (i & (j - 1));
+                const uint32_t y = x + j;
+                const bool up = (x & size) == 0;
+                const uint64_t kx = keys[x], ky = keys[y];
+                if ((kx > ky) == up) {
+                    keys[x] = ky;
+                    keys[y] = kx;
+                }
+            }
+            __syncthreads();
+        }
+    }
+
+    // 4. BVH-order triangle records
+    for (uint32_t j = tid; j < nf; j += kBuildThreads) {
+        const uint32_t o = (uint32_t)(keys[j] & 0xffffffffu);
+        const int32_t* f = a.faces + 3 * o;
+        V3 A = load_v(V, f[0]), B = load_v(V, f[1]), C = load_v(V, f[2]);
+        btris[3 * j + 0] = make_float4(A.x, A.y, A.z, B.x);
+        btris[3 * j + 1] = make_float4(B.y, B.z, C.x, C.y);
+        btris[3 * j + 2] = make_float4(C.z, __uint_as_float(o), __uint_as_float(a.masks[o]), 0.0f);
+    }
+    // leaf boxes
+    const uint32_t n_leaves = 1u << (2 * a.k);
+    for (uint32_t j = tid; j < n_leaves; j += kBuildThreads) {
+        const uint32_t b = (uint32_t)(((uint64_t)j * nf) >> (2 * a.k));
+        const uint32_t e = (uint32_t)(((uint64_t)(j + 1) * nf) >> (2 * a.k));
+        V3 blo = v3(INFINITY, INFINITY, INFINITY), bhi = v3(-INFINITY, -INFINITY, -INFINITY);
+        for (uint32_t t = b; t < e; ++t) {
+            const int32_t* f = a.faces + 3 * (uint32_t)(keys[t] & 0xffffffffu);
+            V3 A = load_v(V, f[0]), B = load_v(V, f[1]), C = load_v(V, f[2]);
+            blo = vmin(blo, vmin(A, vmin(B, C)));
+            bhi = vmax(bhi, vmax(A, vmax(B, C)));
+        }
+        const uint32_t node = a.leaf_base + j;
+        nodes[2 * node] = make_float4(blo.x, blo.y, blo.z, __int_as_float((int32_t)b));
+        nodes[2 * node + 1] = make_float4(bhi.x, bhi.y, bhi.z, __int_as_float((int32_t)e));
+    }
+    __syncthreads();
+    // inner boxes, one heap level at a time
+    for (int d = (int)a.k - 1; d >= 0; --d) {
+        const uint32_t count = 1u << (2 * d);
+        const uint32_t base = (count - 1) / 3;
+        for (uint32_t p = tid; p < count; p += kBuildThreads) {
+            const uint32_t node = base + p;
+            V3 blo = v3(INFINITY, INFINITY, INFINITY), bhi = v3(-INFINITY, -INFINITY, -INFINITY);
+            for (uint32_t c = 0; c < 4; ++c) {
+                const uint32_t ch = 4 * node + 1 + c;
+                blo = vmin(blo, xyz(nodes[2 * ch]));
+                bhi = vmax(bhi, xyz(nodes[2 * ch + 1]));
+            }
+            nodes[2 * node] = make_float4(blo.x, blo.y, blo.z, __int_as_float(-1));
+            nodes[2 * node + 1] = make_float4(bhi.x, bhi.y, bhi.z, __int_as_float(-1));
+        }
+        __syncthreads();
+    }
+
+    // 5. per-triangle frames in FP64 (original face order)
+    for (uint32_t i = tid; i < nf; i += kBuildThreads) {
+        const int32_t* f = a.faces + 3 * i;
+        D3 A = {V[3 * f[0]], V[3 * f[0] + 1], V[3 * f[0] + 2]};
+        D3 B = {V[3 * f[1]], V[3 * f[1] + 1], V[3 * f[1] + 2]};
+        D3 C = {V[3 * f[2]], V[3 * f[2] + 1], V[3 * f[2] + 2]};
+        double s = 1.0;
+        if (slot != 0) {
+            const float* CV = a.canon_verts;
+            D3 cA = {CV[3 * f[0]], CV[3 * f[0] + 1], CV[3 * f[0] + 2]};
+            D3 cB = {CV[3 * f[1]], CV[3 * f[1] + 1], CV[3 * f[1] + 2]};
+            D3 cC = {CV[3 * f[2]], CV[3 * f[2] + 1], CV[3 * f[2] + 2]};
+            D3 cr = dcross(dsub(B, A), dsub(C, A)), ccr = dcross(dsub(cB, cA), dsub(cC, cA));
+            s = (0.5 * sqrt(ddot(cr, cr))) / (0.5 * sqrt(ddot(ccr, ccr)));
+        }
+        // Triangle::tbn (triangle.cuh:43-57): N = |(b-a) x (c-a)|, B = |(b-a) x N|, T = |B x N|
+        D3 t = dsub(B, A);
+        D3 n = dnormalized(dcross(t, dsub(C, A)));
+        D3 bi = dnormalized(dcross(t, n));
+        D3 ta = dnormalized(dcross(bi, n));
+        const double sc = (a.masks[i] == 1u) ? s : 1.0;
+        double m[12] = {ta.x * sc, bi.x * sc, n.x * sc, A.x, ta.y * sc, bi.y * sc, n.y * sc, A.y,
+                        ta.z * sc, bi.z * sc, n.z * sc, A.z};
+        // affine inverse by cofactors
+        double c00 = m[5] * m[10] - m[6] * m[9], c01 = m[2] * m[9] - m[1] * m[10], c02 = m[1] * m[6] - m[2] * m[5];
+        double c10 = m[6] * m[8] - m[4] * m[10], c11 = m[0] * m[10] - m[2] * m[8], c12 = m[2] * m[4] - m[0] * m[6];
+        double c20 = m[4] * m[9] - m[5] * m[8], c21 = m[1] * m[8] - m[0] * m[9], c22 = m[0] * m[5] - m[1] * m[4];
+        double det = m[0] * c00 + m[1] * c10 + m[2] * c20;
+        double id = 1.0 / det;
+        double inv[9] = {c00 * id, c01 * id, c02 * id, c10 * id, c11 * id, c12 * id, c20 * id, c21 * id, c22 * id};
+        double it[3];
+        for (int r = 0; r < 3; ++r) it[r] = -(inv[3 * r] * m[3] + inv[3 * r + 1] * m[7] + inv[3 * r + 2] * m[11]);
+        for (int r = 0; r < 3; ++r) {
+            proj[3 * i + r] = make_float4((float)m[4 * r], (float)m[4 * r + 1], (float)m[4 * r + 2], (float)m[4 * r + 3]);
+            proj_inv[3 * i + r] = make_float4((float)inv[3 * r], (float)inv[3 * r + 1], (float)inv[3 * r + 2], (float)it[r]);
+        }
+    }
+}
+
+__global__ void blend_expression_kernel(const float* __restrict__ canon, const float* __restrict__ bases,
+                                        const float* __restrict__ coeffs, uint32_t n_bases, uint32_t n, float* out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float acc = canon[i];
+    for (uint32_t k = 0; k < n_bases; ++k) acc = acc + coeffs[k] * bases[(size_t)k * n + i];
+    out[i] = acc;
+}
+
+constexpr uint32_t kQueryThreads = 256;
+
+__global__ __launch_bounds__(kQueryThreads) void closest_kernel(MeshView m, const float* __restrict__ pts, uint32_t n,
+                                                               int32_t* __restrict__ ids, float* __restrict__ dist) {
+    __shared__ uint16_t stack[kStackDepth * kQueryThreads];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    V3 p = v3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+    Hit h = bvh_closest(m, p, stack + threadIdx.x, kQueryThreads);
+    ids[i] = h.idx < 0 ? -1 : (int32_t)h.orig;
+    if (dist) dist[i] = sqrtf(h.dist_sq);
+}
+
+// mark_flame_density_grid + grid_to_bitfield (testbed_nerf.cu:364-389, 585-609):
+// one thread per byte of 8 Morton-consecutive cells.
+__global__ __launch_bounds__(kQueryThreads) void occupancy_kernel(MeshView m, uint32_t G, float threshold,
+                                                                 uint8_t* __restrict__ bits, uint32_t n_bytes) {
+    __shared__ uint16_t stack[kStackDepth * kQueryThreads];
+    const uint32_t byte = blockIdx.x * blockDim.x + threadIdx.x;
+    if (byte >= n_bytes) return;
+    const float g = (float)G;
+    uint32_t v = 0;
+    for (uint32_t j = 0; j < 8; ++j) {
+        const uint32_t idx = byte * 8 + j;
+        const uint32_t x = compact3(idx), y = compact3(idx >> 1), z = compact3(idx >> 2);
+        V3 pos = v3((((float)x + 0.5f) / g - 0.5f) * 1.0f + 0.5f, (((float)y + 0.5f) / g - 0.5f) * 1.0f + 0.5f,
+                    (((float)z + 0.5f) / g - 0.5f) * 1.0f + 0.5f);
+        Hit h = bvh_closest(m, pos, stack + threadIdx.x, kQueryThreads);
+        if (h.idx < 0) continue;
+        const float dist = sqrtf(h.dist_sq);
+        const uint32_t mask = __float_as_uint(m.btris[3 * h.idx + 2].z);
+        if (dist < threshold && mask != 0u) v |= 1u << j;
+    }
+    bits[byte] = (uint8_t)v;
+}
+
+}  // namespace
+
+insta_status launch_build_slots(insta_ctx* ctx, uint32_t first_slot, uint32_t n_slots, const float* verts,
+                                hipStream_t stream) {
+    BuildArgs a;
+    a.verts = verts;
+    a.canon_verts = ctx->d_verts;
+    a.faces = ctx->d_faces;
+    a.masks = ctx->d_masks;
+    a.verts_store = ctx->d_verts;
+    a.nodes = ctx->d_nodes;
+    a.btris = ctx->d_btris;
+    a.proj = ctx->d_proj;
+    a.proj_inv = ctx->d_proj_inv;
+    a.cen = ctx->d_cen;
+    a.first_slot = first_slot;
+    a.nv = ctx->nv;
+    a.nf = ctx->nf;
+    a.k = ctx->bvh_k;
+    a.leaf_base = ctx->leaf_base;
+    a.node_cap = ctx->node_cap;
+    a.tri_cap = ctx->cfg.max_triangles;
+    a.vert_cap = ctx->cfg.max_vertices;
+    if (ctx->nf == 0 || ctx->nf > INSTA_MAX_TRIANGLES || first_slot + n_slots > ctx->n_slots) {
+        set_error("build: mesh size or slot range out of bounds");
+        return INSTA_ERR_INVALID;
+    }
+    hipLaunchKernelGGL(build_slot_kernel, dim3(n_slots), dim3(kBuildThreads), 0, stream, a);
+    INSTA_HIP(hipGetLastError());
+    return INSTA_OK;
+}
+
+insta_status launch_blend_expression(insta_ctx* ctx, uint32_t slot, const float* bases, uint32_t n_bases,
+                                     const float* coeffs, hipStream_t stream) {
+    const uint32_t n = ctx->nv * 3;
+    float* tmp = static_cast<float*>(ctx->d_scratch);
+    if (ctx->scratch_bytes < n * sizeof(float)) {
+        set_error("blend: scratch too small");
+        return INSTA_ERR_STATE;
+    }
+    hipLaunchKernelGGL(blend_expression_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, ctx->d_verts, bases,
+                       coeffs, n_bases, n, tmp);
+    INSTA_HIP(hipGetLastError());
+    return launch_build_slots(ctx, slot, 1, tmp, stream);
+}
+
+insta_status launch_closest(insta_ctx* ctx, uint32_t slot, const float* pts, uint32_t n, int32_t* ids, float* dist,
+                            hipStream_t stream) {
+    if (n == 0) return INSTA_OK;
+    hipLaunchKernelGGL(closest_kernel, dim3((n + kQueryThreads - 1) / kQueryThreads), dim3(kQueryThreads), 0, stream,
+                       ctx->mesh(slot), pts, n, ids, dist);
+    INSTA_HIP(hipGetLastError());
+    return INSTA_OK;
+}
+
+insta_status launch_occupancy(insta_ctx* ctx, uint32_t slot, float threshold, hipStream_t stream) {
+    const uint32_t G = ctx->cfg.grid_size;
+    const uint32_t n_bytes = G * G * G / 8;
+    hipLaunchKernelGGL(occupancy_kernel, dim3((n_bytes + kQueryThreads - 1) / kQueryThreads), dim3(kQueryThreads), 0,
+                       stream, ctx->mesh(slot), G, threshold, ctx->grid(slot), n_bytes);
+    INSTA_HIP(hipGetLastError());
+    return INSTA_OK;
+}
+
+}  // namespace insta

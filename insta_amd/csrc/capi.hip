@@ -1,0 +1,557 @@
+// capi.hip -- the extern "C" boundary declared in include/insta_capi.h.
+//
+// Host-side orchestration only: validation, allocation at creation time,
+// adjacency/render-box set-up from the canonical mesh, and dispatch to the
+// kernel launchers.  No exception crosses the ABI; failures return a status
+// and leave a message for insta_last_error().
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "context.hpp"
+
+namespace insta {
+
+insta_status render(insta_ctx* ctx, const insta_camera* cam, uint32_t width, uint32_t height, int32_t frame,
+                    int32_t occupancy, uint32_t spp, float* rgba, float* depth, uint32_t* steps, hipStream_t stream);
+void free_workspace(insta_ctx* ctx);
+insta_status set_profiling(insta_ctx* ctx, bool on);
+
+namespace {
+thread_local std::string g_error;
+
+__global__ void set_flags_kernel(uint8_t* flags, uint32_t first, uint32_t n, uint8_t value) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flags[first + i] = value;
+}
+
+__global__ void export_order_kernel(const float4* __restrict__ btris, uint32_t nf, uint32_t* __restrict__ order) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nf) order[i] = __float_as_uint(btris[3 * i + 2].y);
+}
+
+uint32_t bvh_levels(uint32_t n) {
+    uint32_t k = 0;
+    while ((uint64_t(1) << (2 * (k + 1))) <= n) ++k;
+    return k;
+}
+
+// Hash-grid and MLP parameter layout.  The per-level scale is evaluated in
+// double and rounded to float once, so the kernels and any CPU check see the
+// same float scale.
+void compute_layout(const insta_config& c, insta_layout& L) {
+    std::memset(&L, 0, sizeof(L));
+    const uint64_t T = uint64_t(1) << c.log2_hashmap_size;
+    uint64_t off = 0;
+    for (uint32_t l = 0; l < c.n_levels; ++l) {
+        const double s = std::exp2(l * std::log2((double)c.per_level_scale)) * c.base_resolution - 1.0;
+        const float sf = (float)s;
+        const uint32_t res = (uint32_t)std::ceil(sf) + 1u;
+        const uint64_t verts = uint64_t(res) * res * res;
+        const bool dense = verts <= T;
+        const uint64_t size = dense ? ((verts + 7) / 8) * 8 : T;
+        L.level_scale[l] = sf;
+        L.level_resolution[l] = res;
+        L.level_dense[l] = dense ? 1u : 0u;
+        L.level_offset[l] = (uint32_t)off;
+        L.level_size[l] = (uint32_t)size;
+        off += size;
+    }
+    L.n_grid_params = off * c.n_features_per_level;
+    L.density_in = c.n_levels * c.n_features_per_level + c.n_extra_dims;
+    L.rgb_in = c.density_out + c.sh_degree * c.sh_degree;
+    const uint64_t w = c.n_neurons;
+    uint64_t n_mlp = w * L.density_in + (uint64_t)(c.density_hidden_layers - 1) * w * w + c.density_out * w;
+    n_mlp += w * L.rgb_in + (uint64_t)(c.rgb_hidden_layers - 1) * w * w + 3 * w;
+    L.n_mlp_params = n_mlp;
+    L.n_params = n_mlp + L.n_grid_params;
+}
+
+bool is_pow2(uint32_t v) { return v && !(v & (v - 1)); }
+
+insta_status validate(const insta_config& c) {
+    auto bad = [](const char* m) {
+        set_error(m);
+        return INSTA_ERR_INVALID;
+    };
+    if (c.n_levels == 0 || c.n_levels > INSTA_MAX_LEVELS) return bad("n_levels must be in [1, 32]");
+    if (!(c.n_features_per_level == 1 || c.n_features_per_level == 2 || c.n_features_per_level == 4 ||
+          c.n_features_per_level == 8))
+        return bad("n_features_per_level must be 1, 2, 4 or 8");
+    if (c.log2_hashmap_size < 4 || c.log2_hashmap_size > 26) return bad("log2_hashmap_size must be in [4, 26]");
+    if (c.base_resolution < 2 || !(c.per_level_scale >= 1.0f)) return bad("base_resolution >= 2, per_level_scale >= 1");
+    if (c.n_neurons == 0 || c.n_neurons > 256 || c.density_out == 0 || c.density_out > 48)
+        return bad("n_neurons in [1, 256], density_out in [1, 48]");
+    if (c.density_hidden_layers == 0 || c.rgb_hidden_layers == 0) return bad("at least one hidden layer per MLP");
+    if (c.sh_degree != 4) return bad("sh_degree must be 4");
+    if (c.n_extra_dims > 64) return bad("n_extra_dims must be <= 64");
+    if (!is_pow2(c.grid_size) || c.grid_size < 8 || c.grid_size > 1024) return bad("grid_size: power of two in [8, 1024]");
+    if (!(c.dt > 0.0f) || c.max_steps == 0) return bad("dt > 0 and max_steps >= 1");
+    if (!(c.min_transmittance >= 0.0f && c.min_transmittance < 1.0f)) return bad("min_transmittance in [0, 1)");
+    if (c.max_frames == 0 || c.max_frames > 4096) return bad("max_frames in [1, 4096]");
+    if (c.max_triangles == 0 || c.max_triangles > INSTA_MAX_TRIANGLES) return bad("max_triangles in [1, 16384]");
+    if (c.max_vertices == 0 || c.max_vertices > (1u << 24)) return bad("max_vertices in [1, 2^24]");
+    if (c.precision > INSTA_PRECISION_BF16) return bad("unknown precision");
+    return INSTA_OK;
+}
+
+#define INSTA_CHECK_CTX(ctx)                       \
+    if (!(ctx)) {                                  \
+        insta::set_error("null context");          \
+        return INSTA_ERR_INVALID;                  \
+    }
+
+insta_status mark_built(insta_ctx* ctx, uint32_t first, uint32_t n, uint8_t v, hipStream_t stream) {
+    hipLaunchKernelGGL(set_flags_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, ctx->d_built, first, n, v);
+    INSTA_HIP(hipGetLastError());
+    for (uint32_t i = 0; i < n; ++i) ctx->slot_built[first + i] = v;
+    return INSTA_OK;
+}
+
+bool slot_of(const insta_ctx* ctx, int32_t frame, uint32_t& slot) {
+    if (frame < -1 || frame >= (int32_t)ctx->cfg.max_frames) return false;
+    slot = (uint32_t)(frame + 1);
+    return ctx->slot_built[slot] != 0;
+}
+
+}  // namespace
+
+void set_error(const std::string& msg) { g_error = msg; }
+
+insta_status hip_fail(hipError_t e, const char* what) {
+    g_error = std::string(what) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? INSTA_ERR_OOM : INSTA_ERR_HIP;
+}
+
+}  // namespace insta
+
+using insta::set_error;
+
+extern "C" {
+
+const char* insta_last_error(void) { return insta::g_error.c_str(); }
+
+const char* insta_version(void) { return "insta-amd 0.1 (gfx950)"; }
+
+void insta_default_config(insta_config* c) {
+    std::memset(c, 0, sizeof(*c));
+    c->n_levels = 16;
+    c->n_features_per_level = 2;
+    c->log2_hashmap_size = 19;
+    c->base_resolution = 16;
+    c->per_level_scale = (float)std::exp(std::log(2048.0 / 16.0) / 15.0);
+    c->n_neurons = 64;
+    c->density_hidden_layers = 1;
+    c->density_out = 16;
+    c->rgb_hidden_layers = 2;
+    c->sh_degree = 4;
+    c->n_extra_dims = 0;
+    c->density_logit_bias = 1.0f;
+    c->grid_size = 128;
+    c->dt = 1.73205080757f / 1024.0f;
+    c->max_steps = 1024;
+    c->min_transmittance = 0.01f;
+    c->disable_interp = 0;
+    c->precision = INSTA_PRECISION_FP32;
+    c->max_frames = 16;
+    c->max_triangles = INSTA_MAX_TRIANGLES;
+    c->max_vertices = 1u << 16;
+}
+
+insta_status insta_query_layout(const insta_config* cfg, insta_layout* out) {
+    if (!cfg || !out) {
+        set_error("null argument");
+        return INSTA_ERR_INVALID;
+    }
+    insta_status st = insta::validate(*cfg);
+    if (st != INSTA_OK) return st;
+    insta::compute_layout(*cfg, *out);
+    return INSTA_OK;
+}
+
+insta_status insta_create(const insta_config* cfg, insta_ctx** out) {
+    if (!cfg || !out) {
+        set_error("null argument");
+        return INSTA_ERR_INVALID;
+    }
+    *out = nullptr;
+    insta_status st = insta::validate(*cfg);
+    if (st != INSTA_OK) return st;
+    insta_ctx* ctx = new (std::nothrow) insta_ctx();
+    if (!ctx) {
+        set_error("host allocation failed");
+        return INSTA_ERR_OOM;
+    }
+    ctx->cfg = *cfg;
+    insta::compute_layout(ctx->cfg, ctx->layout);
+    if (hipGetDevice(&ctx->device) != hipSuccess) {
+        delete ctx;
+        set_error("no HIP device");
+        return INSTA_ERR_HIP;
+    }
+    const uint32_t kmax = insta::bvh_levels(cfg->max_triangles);
+    ctx->node_cap = ((1u << (2 * (kmax + 1))) - 1) / 3;
+    ctx->n_slots = cfg->max_frames + 1;
+    ctx->slot_built.assign(ctx->n_slots, 0);
+    ctx->grid_built.assign(ctx->n_slots, 0);
+    const size_t S = ctx->n_slots, T = cfg->max_triangles, G = cfg->grid_size;
+    ctx->scratch_bytes = std::max<size_t>((size_t)cfg->max_vertices * 3 * sizeof(float), 1 << 16) + 4096;
+    bool ok = true;
+    auto alloc = [&](void** p, size_t bytes) {
+        if (ok && hipMalloc(p, bytes) != hipSuccess) ok = false;
+    };
+    alloc((void**)&ctx->d_faces, T * 3 * sizeof(int32_t));
+    alloc((void**)&ctx->d_masks, T * sizeof(uint32_t));
+    alloc((void**)&ctx->d_adj, T * sizeof(int4));
+    alloc((void**)&ctx->d_verts, S * cfg->max_vertices * 3 * sizeof(float));
+    alloc((void**)&ctx->d_nodes, S * ctx->node_cap * 2 * sizeof(float4));
+    alloc((void**)&ctx->d_btris, S * T * 3 * sizeof(float4));
+    alloc((void**)&ctx->d_proj, S * T * 3 * sizeof(float4));
+    alloc((void**)&ctx->d_proj_inv, S * T * 3 * sizeof(float4));
+    alloc((void**)&ctx->d_cen, S * T * sizeof(float4));
+    alloc((void**)&ctx->d_grids, S * (G * G * G / 8));
+    alloc((void**)&ctx->d_built, S);
+    alloc((void**)&ctx->d_params, ctx->layout.n_params * sizeof(float));
+    alloc((void**)&ctx->d_counters, 16 * sizeof(uint32_t));
+    alloc(&ctx->d_scratch, ctx->scratch_bytes);
+    if (ok && hipHostMalloc((void**)&ctx->h_counters, 16 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
+        ok = false;
+    if (ok && hipMemset(ctx->d_built, 0, S) != hipSuccess) ok = false;
+    if (ok && hipMemset(ctx->d_grids, 0, S * (G * G * G / 8)) != hipSuccess) ok = false;
+    if (!ok) {
+        insta_destroy(ctx);
+        set_error("device allocation failed in insta_create");
+        return INSTA_ERR_OOM;
+    }
+    *out = ctx;
+    return INSTA_OK;
+}
+
+void insta_destroy(insta_ctx* ctx) {
+    if (!ctx) return;
+    insta::free_workspace(ctx);
+    using insta::dev_free;
+    dev_free(ctx->d_faces); dev_free(ctx->d_masks); dev_free(ctx->d_adj); dev_free(ctx->d_verts);
+    dev_free(ctx->d_nodes); dev_free(ctx->d_btris); dev_free(ctx->d_proj); dev_free(ctx->d_proj_inv);
+    dev_free(ctx->d_cen); dev_free(ctx->d_grids); dev_free(ctx->d_built); dev_free(ctx->d_params);
+    dev_free(ctx->d_counters); dev_free(ctx->d_scratch);
+    if (ctx->h_counters) (void)hipHostFree(ctx->h_counters);
+    delete ctx;
+}
+
+insta_status insta_set_canonical_mesh(insta_ctx* ctx, const float* verts, uint32_t nv, const int32_t* faces,
+                                      uint32_t nf, const uint32_t* mask_ids, void* stream_) {
+    INSTA_CHECK_CTX(ctx);
+    hipStream_t stream = (hipStream_t)stream_;
+    if (!verts || !faces || !mask_ids || nv == 0 || nf == 0) {
+        set_error("set_canonical_mesh: empty mesh");
+        return INSTA_ERR_INVALID;
+    }
+    if (nv > ctx->cfg.max_vertices || nf > ctx->cfg.max_triangles) {
+        set_error("set_canonical_mesh: mesh exceeds max_vertices / max_triangles");
+        return INSTA_ERR_INVALID;
+    }
+    for (uint64_t i = 0; i < (uint64_t)nf * 3; ++i) {
+        if (faces[i] < 0 || (uint32_t)faces[i] >= nv) {
+            set_error("set_canonical_mesh: face index out of range");
+            return INSTA_ERR_INVALID;
+        }
+    }
+    // edge adjacency (src/tiny_mesh.cu:316-371): faces sharing an edge,
+    // ascending face id, self removed, -1 padded
+    std::vector<std::vector<int32_t>> vf(nv);
+    for (uint32_t i = 0; i < nf; ++i)
+        for (int k = 0; k < 3; ++k) vf[faces[3 * i + k]].push_back((int32_t)i);
+    std::vector<int4> adj(nf);
+    for (uint32_t i = 0; i < nf; ++i) {
+        std::vector<int32_t> ids;
+        const int32_t v[3] = {faces[3 * i], faces[3 * i + 1], faces[3 * i + 2]};
+        const int pairs[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+        for (const auto& pr : pairs) {
+            const auto& A = vf[v[pr[0]]];
+            const auto& B = vf[v[pr[1]]];
+            for (int32_t f : A)
+                if (std::find(B.begin(), B.end(), f) != B.end()) ids.push_back(f);
+        }
+        std::sort(ids.begin(), ids.end());
+        ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+        int32_t nb[3] = {-1, -1, -1};
+        int k = 0;
+        for (int32_t f : ids) {
+            if (f == (int32_t)i) continue;
+            if (k < 3) nb[k] = f;
+            ++k;
+        }
+        adj[i] = make_int4(nb[0], nb[1], nb[2], -1);
+    }
+    // render box: vertex AABB inflated by 0.15 |diag| (tiny_mesh.cu:157-162),
+    // clipped to the unit cube (testbed_nerf.cu:2675-2682)
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = 0; i < nv; ++i)
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = std::fmin(lo[d], verts[3 * i + d]);
+            hi[d] = std::fmax(hi[d], verts[3 * i + d]);
+        }
+    const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    const float amount = std::sqrt(dx * dx + dy * dy + dz * dz) * 0.15f;
+    for (int d = 0; d < 3; ++d) {
+        ctx->render_lo[d] = std::fmax(lo[d] - amount, 0.0f);
+        ctx->render_hi[d] = std::fmin(hi[d] + amount, 1.0f);
+    }
+    ctx->nv = nv;
+    ctx->nf = nf;
+    ctx->bvh_k = insta::bvh_levels(nf);
+    ctx->leaf_base = ((1u << (2 * ctx->bvh_k)) - 1) / 3;
+    ctx->n_nodes = ctx->leaf_base + (1u << (2 * ctx->bvh_k));
+    insta_status st = insta::mark_built(ctx, 0, ctx->n_slots, 0, stream);
+    if (st != INSTA_OK) return st;
+    std::fill(ctx->grid_built.begin(), ctx->grid_built.end(), 0);
+    INSTA_HIP(hipMemcpyAsync(ctx->d_faces, faces, (size_t)nf * 3 * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+    INSTA_HIP(hipMemcpyAsync(ctx->d_masks, mask_ids, (size_t)nf * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    INSTA_HIP(hipMemcpyAsync(ctx->d_adj, adj.data(), (size_t)nf * sizeof(int4), hipMemcpyHostToDevice, stream));
+    float* tmp = static_cast<float*>(ctx->d_scratch);
+    INSTA_HIP(hipMemcpyAsync(tmp, verts, (size_t)nv * 3 * sizeof(float), hipMemcpyHostToDevice, stream));
+    st = insta::launch_build_slots(ctx, 0, 1, tmp, stream);
+    if (st != INSTA_OK) return st;
+    st = insta::mark_built(ctx, 0, 1, 1, stream);
+    if (st != INSTA_OK) return st;
+    // host arrays above were staged by the copies; wait so they may be freed
+    INSTA_HIP(hipStreamSynchronize(stream));
+    return INSTA_OK;
+}
+
+insta_status insta_set_frames(insta_ctx* ctx, uint32_t first_frame, uint32_t n_frames, const float* verts_dev,
+                              void* stream_) {
+    INSTA_CHECK_CTX(ctx);
+    hipStream_t stream = (hipStream_t)stream_;
+    if (!ctx->slot_built[0]) {
+        set_error("set_frames: canonical mesh not set");
+        return INSTA_ERR_STATE;
+    }
+    if (!verts_dev || n_frames == 0 || (uint64_t)first_frame + n_frames > ctx->cfg.max_frames) {
+        set_error("set_frames: frame range outside [0, max_frames)");
+        return INSTA_ERR_INVALID;
+    }
+    insta_status st = insta::launch_build_slots(ctx, first_frame + 1, n_frames, verts_dev, stream);
+    if (st != INSTA_OK) return st;
+    for (uint32_t f = 0; f < n_frames; ++f) ctx->grid_built[first_frame + 1 + f] = 0;
+    return insta::mark_built(ctx, first_frame + 1, n_frames, 1, stream);
+}
+
+insta_status insta_set_frame_expression(insta_ctx* ctx, uint32_t frame, const float* bases_dev, uint32_t n_bases,
+                                        const float* coeffs_dev, void* stream_) {
+    INSTA_CHECK_CTX(ctx);
+    hipStream_t stream = (hipStream_t)stream_;
+    if (!ctx->slot_built[0]) {
+        set_error("set_frame_expression: canonical mesh not set");
+        return INSTA_ERR_STATE;
+    }
+    if (frame >= ctx->cfg.max_frames || !bases_dev || !coeffs_dev) {
+        set_error("set_frame_expression: bad frame or null pointer");
+        return INSTA_ERR_INVALID;
+    }
+    insta_status st = insta::launch_blend_expression(ctx, frame + 1, bases_dev, n_bases, coeffs_dev, stream);
+    if (st != INSTA_OK) return st;
+    ctx->grid_built[frame + 1] = 0;
+    return insta::mark_built(ctx, frame + 1, 1, 1, stream);
+}
+
+insta_status insta_closest_triangles(insta_ctx* ctx, int32_t frame, const float* points_dev, uint32_t n,
+                                     int32_t* ids_dev, float* dist_dev, void* stream) {
+    INSTA_CHECK_CTX(ctx);
+    uint32_t slot;
+    if (!insta::slot_of(ctx, frame, slot)) {
+        set_error("closest_triangles: frame not built");
+        return INSTA_ERR_STATE;
+    }
+    if (n && (!points_dev || !ids_dev)) {
+        set_error("closest_triangles: null pointer");
+        return INSTA_ERR_INVALID;
+    }
+    return insta::launch_closest(ctx, slot, points_dev, n, ids_dev, dist_dev, (hipStream_t)stream);
+}
+
+insta_status insta_export_bvh(insta_ctx* ctx, int32_t frame, float* nodes_dev, uint32_t* order_dev,
+                              uint32_t* n_nodes, void* stream_) {
+    INSTA_CHECK_CTX(ctx);
+    hipStream_t stream = (hipStream_t)stream_;
+    uint32_t slot;
+    if (!insta::slot_of(ctx, frame, slot)) {
+        set_error("export_bvh: frame not built");
+        return INSTA_ERR_STATE;
+    }
+    if (n_nodes) *n_nodes = ctx->n_nodes;
+    const insta::MeshView m = ctx->mesh(slot);
+    if (nodes_dev)
+        INSTA_HIP(hipMemcpyAsync(nodes_dev, m.nodes, (size_t)ctx->n_nodes * 2 * sizeof(float4),
+                                 hipMemcpyDeviceToDevice, stream));
+    if (order_dev) {
+        hipLaunchKernelGGL(insta::export_order_kernel, dim3((ctx->nf + 255) / 256), dim3(256), 0, stream, m.btris,
+                           ctx->nf, order_dev);
+        INSTA_HIP(hipGetLastError());
+    }
+    return INSTA_OK;
+}
+
+insta_status insta_deform_samples(insta_ctx* ctx, float* coords_dev, uint32_t n, uint32_t stride_floats,
+                                  int32_t frame, void* stream) {
+    INSTA_CHECK_CTX(ctx);
+    if (!ctx->slot_built[0]) {
+        set_error("deform_samples: canonical mesh not set");
+        return INSTA_ERR_STATE;
+    }
+    if (stride_floats < 11 + ctx->cfg.n_extra_dims || (n && !coords_dev)) {
+        set_error("deform_samples: stride must be >= 11 + n_extra_dims");
+        return INSTA_ERR_INVALID;
+    }
+    if (frame >= (int32_t)ctx->cfg.max_frames || frame < -1) {
+        set_error("deform_samples: frame out of range");
+        return INSTA_ERR_INVALID;
+    }
+    return insta::launch_deform_aos(ctx, coords_dev, n, stride_floats, frame, (hipStream_t)stream);
+}
+
+insta_status insta_build_occupancy(insta_ctx* ctx, int32_t frame, float threshold, uint8_t* bitfield_dev,
+                                   void* stream_) {
+    INSTA_CHECK_CTX(ctx);
+    hipStream_t stream = (hipStream_t)stream_;
+    uint32_t slot;
+    if (!insta::slot_of(ctx, frame, slot)) {
+        set_error("build_occupancy: frame not built");
+        return INSTA_ERR_STATE;
+    }
+    insta_status st = insta::launch_occupancy(ctx, slot, threshold, stream);
+    if (st != INSTA_OK) return st;
+    ctx->grid_built[slot] = 1;
+    const size_t G = ctx->cfg.grid_size;
+    if (bitfield_dev)
+        INSTA_HIP(hipMemcpyAsync(bitfield_dev, ctx->grid(slot), G * G * G / 8, hipMemcpyDeviceToDevice, stream));
+    return INSTA_OK;
+}
+
+insta_status insta_set_occupancy(insta_ctx* ctx, int32_t slot_arg, const uint8_t* bitfield_dev, void* stream) {
+    INSTA_CHECK_CTX(ctx);
+    if (slot_arg < -1 || slot_arg >= (int32_t)ctx->cfg.max_frames || !bitfield_dev) {
+        set_error("set_occupancy: bad slot or null pointer");
+        return INSTA_ERR_INVALID;
+    }
+    const uint32_t slot = (uint32_t)(slot_arg + 1);
+    const size_t G = ctx->cfg.grid_size;
+    INSTA_HIP(hipMemcpyAsync(ctx->grid(slot), bitfield_dev, G * G * G / 8, hipMemcpyDeviceToDevice,
+                             (hipStream_t)stream));
+    ctx->grid_built[slot] = 1;
+    return INSTA_OK;
+}
+
+insta_status insta_set_params(insta_ctx* ctx, const float* params_dev, uint64_t n_params, void* stream) {
+    INSTA_CHECK_CTX(ctx);
+    if (!params_dev || n_params != ctx->layout.n_params) {
+        set_error("set_params: n_params does not match the layout");
+        return INSTA_ERR_INVALID;
+    }
+    INSTA_HIP(hipMemcpyAsync(ctx->d_params, params_dev, n_params * sizeof(float), hipMemcpyDeviceToDevice,
+                             (hipStream_t)stream));
+    ctx->params_set = true;
+    return INSTA_OK;
+}
+
+insta_status insta_encode(insta_ctx* ctx, const float* positions_dev, uint32_t n, float* features_dev, void* stream) {
+    INSTA_CHECK_CTX(ctx);
+    if (!ctx->params_set) {
+        set_error("encode: parameters not set");
+        return INSTA_ERR_STATE;
+    }
+    if (n && (!positions_dev || !features_dev)) {
+        set_error("encode: null pointer");
+        return INSTA_ERR_INVALID;
+    }
+    return insta::launch_encode(ctx, positions_dev, n, features_dev, (hipStream_t)stream);
+}
+
+insta_status insta_encode_backward(insta_ctx* ctx, const float* positions_dev, uint32_t n,
+                                   const float* dL_dfeatures_dev, float* grad_tables_dev, void* stream) {
+    INSTA_CHECK_CTX(ctx);
+    if (n && (!positions_dev || !dL_dfeatures_dev || !grad_tables_dev)) {
+        set_error("encode_backward: null pointer");
+        return INSTA_ERR_INVALID;
+    }
+    return insta::launch_encode_backward(ctx, positions_dev, n, dL_dfeatures_dev, grad_tables_dev,
+                                         (hipStream_t)stream);
+}
+
+insta_status insta_network_inference(insta_ctx* ctx, const float* coords_dev, uint32_t n, uint32_t stride_floats,
+                                     float* out_dev, void* stream) {
+    INSTA_CHECK_CTX(ctx);
+    if (!ctx->params_set) {
+        set_error("network_inference: parameters not set");
+        return INSTA_ERR_STATE;
+    }
+    if (!insta::network_supported(ctx->cfg)) {
+        set_error("network_inference: configuration not supported by the fused MLP kernels");
+        return INSTA_ERR_UNSUPPORTED;
+    }
+    if (stride_floats < 11 + ctx->cfg.n_extra_dims || (n && (!coords_dev || !out_dev))) {
+        set_error("network_inference: bad stride or null pointer");
+        return INSTA_ERR_INVALID;
+    }
+    return insta::launch_network_aos(ctx, coords_dev, n, stride_floats, out_dev, (hipStream_t)stream);
+}
+
+insta_status insta_render(insta_ctx* ctx, const insta_camera* cam, uint32_t width, uint32_t height, int32_t frame,
+                          int32_t occupancy, uint32_t spp, float* rgba_dev, float* depth_dev, uint32_t* steps_dev,
+                          void* stream) {
+    INSTA_CHECK_CTX(ctx);
+    if (!cam || !rgba_dev || width == 0 || height == 0 || (uint64_t)width * height > (1ull << 26)) {
+        set_error("render: bad camera, output pointer or resolution");
+        return INSTA_ERR_INVALID;
+    }
+    if (!ctx->params_set) {
+        set_error("render: parameters not set");
+        return INSTA_ERR_STATE;
+    }
+    if (!insta::network_supported(ctx->cfg)) {
+        set_error("render: configuration not supported by the fused MLP kernels");
+        return INSTA_ERR_UNSUPPORTED;
+    }
+    uint32_t slot;
+    if (!insta::slot_of(ctx, frame, slot)) {
+        set_error("render: frame not built");
+        return INSTA_ERR_STATE;
+    }
+    if (occupancy < -1 || occupancy >= (int32_t)ctx->cfg.max_frames || !ctx->grid_built[occupancy + 1]) {
+        set_error("render: occupancy slot not built");
+        return INSTA_ERR_STATE;
+    }
+    return insta::render(ctx, cam, width, height, frame, occupancy, spp, rgba_dev, depth_dev, steps_dev,
+                         (hipStream_t)stream);
+}
+
+insta_status insta_get_render_stats(insta_ctx* ctx, insta_render_stats* out) {
+    INSTA_CHECK_CTX(ctx);
+    if (!out) {
+        set_error("get_render_stats: null pointer");
+        return INSTA_ERR_INVALID;
+    }
+    *out = ctx->last_stats;
+    return INSTA_OK;
+}
+
+insta_status insta_set_profiling(insta_ctx* ctx, int enable) {
+    INSTA_CHECK_CTX(ctx);
+    return insta::set_profiling(ctx, enable != 0);
+}
+
+insta_status insta_get_profile(insta_ctx* ctx, insta_profile* out) {
+    INSTA_CHECK_CTX(ctx);
+    if (!out) {
+        set_error("get_profile: null pointer");
+        return INSTA_ERR_INVALID;
+    }
+    *out = ctx->prof;
+    return INSTA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,176 @@
+// common.hpp -- device math shared by the INSTA hot-path kernels (gfx950).
+//
+// All sources are compiled with -ffp-contract=off: each float operation below
+// rounds once, in the order written.  The geometric predicates (triangle and
+// box distances, Morton codes, march stepping) must give bit-identical results
+// on identical inputs, so their operation order is part of the contract.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/insta_capi.h"
+
+#define INSTA_DEV __device__ __forceinline__
+#define INSTA_HD __host__ __device__ __forceinline__
+
+namespace insta {
+
+struct V3 {
+    float x, y, z;
+};
+INSTA_HD V3 v3(float x, float y, float z) { return {x, y, z}; }
+INSTA_HD V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+INSTA_HD V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+INSTA_HD V3 operator*(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+INSTA_HD float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+INSTA_HD V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+INSTA_HD float sqnorm(V3 a) { return dot(a, a); }
+INSTA_DEV float norm(V3 a) { return sqrtf(sqnorm(a)); }
+INSTA_DEV V3 vmin(V3 a, V3 b) { return {fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)}; }
+INSTA_DEV V3 vmax(V3 a, V3 b) { return {fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)}; }
+INSTA_DEV float signf(float x) { return copysignf(1.0f, x); }
+INSTA_DEV float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
+INSTA_DEV V3 xyz(float4 a) { return {a.x, a.y, a.z}; }
+
+// Point-triangle squared distance (Inigo Quilez's formulation, as in
+// instant-ngp/include/neural-graphics-primitives/triangle.cuh:88-107).
+INSTA_DEV float tri_distance_sq(V3 a, V3 b, V3 c, V3 pos) {
+    V3 v21 = b - a; V3 p1 = pos - a;
+    V3 v32 = c - b; V3 p2 = pos - b;
+    V3 v13 = a - c; V3 p3 = pos - c;
+    V3 nor = cross(v21, v13);
+    float s = signf(dot(cross(v21, nor), p1)) + signf(dot(cross(v32, nor), p2)) + signf(dot(cross(v13, nor), p3));
+    if (s < 2.0f) {
+        float e1 = sqnorm(v21 * clampf(dot(v21, p1) / sqnorm(v21), 0.0f, 1.0f) - p1);
+        float e2 = sqnorm(v32 * clampf(dot(v32, p2) / sqnorm(v32), 0.0f, 1.0f) - p2);
+        float e3 = sqnorm(v13 * clampf(dot(v13, p3) / sqnorm(v13), 0.0f, 1.0f) - p3);
+        return fminf(fminf(e1, e2), e3);
+    }
+    float d = dot(nor, p1);
+    return d * d / sqnorm(nor);
+}
+
+// Point-AABB squared distance (bounding_box.cuh:228-230).
+INSTA_DEV float box_distance_sq(float4 lo, float4 hi, V3 p) {
+    float dx = fmaxf(fmaxf(lo.x - p.x, p.x - hi.x), 0.0f);
+    float dy = fmaxf(fmaxf(lo.y - p.y, p.y - hi.y), 0.0f);
+    float dz = fmaxf(fmaxf(lo.z - p.z, p.z - hi.z), 0.0f);
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// 10-bit-per-axis Morton interleave: bit i of x/y/z -> bit 3i/3i+1/3i+2.
+INSTA_HD uint32_t spread3(uint32_t v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+INSTA_HD uint32_t compact3(uint32_t v) {
+    v &= 0x09249249u;
+    v = (v | (v >> 2)) & 0x030c30c3u;
+    v = (v | (v >> 4)) & 0x0300f00fu;
+    v = (v | (v >> 8)) & 0x030000ffu;
+    v = (v | (v >> 16)) & 0x000003ffu;
+    return v;
+}
+INSTA_HD uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) { return spread3(x) | (spread3(y) << 1) | (spread3(z) << 2); }
+
+// ---------------------------------------------------------------------------
+// Mesh storage.  Slot 0 is the canonical mesh, slot f+1 posed frame f.  All
+// slots share the topology, so the BVH shape (levels, node count) is common.
+//   nodes  [slot][n_nodes][2] float4: (min xyz, begin) (max xyz, end); begin/
+//          end are int bit patterns of the BVH-order triangle range (leaves)
+//   btris  [slot][nf][3] float4: (a xyz, b.x) (b.yz, c.xy) (c.z, orig id,
+//          mask id, 0) in BVH order; id/mask are uint bit patterns
+//   proj / proj_inv [slot][nf][3] float4: rows of the 3x4 frame [A | a]
+//   cen    [slot][nf] float4: float centroid xyz (original face order)
+// ---------------------------------------------------------------------------
+struct MeshView {
+    const float4* nodes;
+    const float4* btris;
+    const float4* proj;
+    const float4* proj_inv;
+    const float4* cen;
+    uint32_t nf, leaf_base, k;
+};
+
+struct Hit {
+    int32_t idx;       // BVH-order index, -1 if none
+    uint32_t orig;     // original face id
+    float dist_sq;
+};
+
+// Nearest triangle (bvh.cuh:48-96): far-to-near ordered DFS over the implicit
+// 4-ary heap, children pushed when their box distance is within a few ulp of
+// the best so far (so the result equals the brute-force argmin),
+// leaf triangles ranked by (dist_sq, original id).  `stack` points at this
+// thread's column of a [depth][blockDim] uint16 LDS array.
+INSTA_DEV Hit bvh_closest(const MeshView& m, V3 p, uint16_t* stack, uint32_t stride) {
+    int sp = 0;
+    stack[0] = 0;
+    sp = 1;
+    float best = 3.40282347e38f;
+    int32_t best_idx = -1;
+    uint32_t best_orig = 0xffffffffu;
+    const uint32_t shift = 2u * m.k;
+    while (sp > 0) {
+        --sp;
+        const uint32_t node = stack[sp * stride];
+        if (node >= m.leaf_base) {
+            const uint32_t j = node - m.leaf_base;
+            const uint32_t b = (uint32_t)(((uint64_t)j * m.nf) >> shift);
+            const uint32_t e = (uint32_t)(((uint64_t)(j + 1) * m.nf) >> shift);
+            for (uint32_t t = b; t < e; ++t) {
+                const float4 r0 = m.btris[3 * t + 0];
+                const float4 r1 = m.btris[3 * t + 1];
+                const float4 r2 = m.btris[3 * t + 2];
+                const float d = tri_distance_sq(v3(r0.x, r0.y, r0.z), v3(r0.w, r1.x, r1.y), v3(r1.z, r1.w, r2.x), p);
+                const uint32_t o = __float_as_uint(r2.y);
+                if (d < best || (d == best && o < best_orig)) {
+                    best = d;
+                    best_idx = (int32_t)t;
+                    best_orig = o;
+                }
+            }
+        } else {
+            const uint32_t first = 4u * node + 1u;
+            float cd[4];
+            uint32_t ci[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                ci[c] = first + c;
+                cd[c] = box_distance_sq(m.nodes[2 * (first + c)], m.nodes[2 * (first + c) + 1], p);
+            }
+#define INSTA_CAS(i, j)                          \
+    if (cd[i] < cd[j]) {                         \
+        float td = cd[i]; cd[i] = cd[j]; cd[j] = td; \
+        uint32_t ti = ci[i]; ci[i] = ci[j]; ci[j] = ti; \
+    }
+            INSTA_CAS(0, 2) INSTA_CAS(1, 3) INSTA_CAS(0, 1) INSTA_CAS(2, 3) INSTA_CAS(1, 2)
+#undef INSTA_CAS
+            // conservative prune (relative 2^-18 + absolute 2^-26): rounding
+            // can put a box a few ulp beyond the best triangle it contains
+            const float bound = best + best * 0x1p-18f + 0x1p-26f;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                if (cd[c] <= bound) {
+                    stack[sp * stride] = (uint16_t)ci[c];
+                    ++sp;
+                }
+            }
+        }
+    }
+    Hit h;
+    h.idx = best_idx;
+    h.orig = best_orig;
+    h.dist_sq = best_idx < 0 ? 0.0f : best;
+    return h;
+}
+
+// Maximum DFS stack depth: at most 3 pending siblings per level plus one.
+constexpr uint32_t kStackDepth = 24;
+
+}  // namespace insta

@@ -1,0 +1,143 @@
+// context.hpp -- host-side state of one avatar context and the launchers that
+// each kernel translation unit provides.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+struct insta_ctx {
+    insta_config cfg{};
+    insta_layout layout{};
+    int device = 0;
+
+    // topology (shared by all slots)
+    uint32_t nv = 0, nf = 0;
+    uint32_t bvh_k = 0, leaf_base = 0, n_nodes = 0;
+    uint32_t node_cap = 0;          // nodes reserved per slot (for max_triangles)
+    uint32_t n_slots = 0;           // max_frames + 1
+    std::vector<uint8_t> slot_built;
+    std::vector<uint8_t> grid_built;
+
+    int32_t* d_faces = nullptr;     // [nf*3]
+    uint32_t* d_masks = nullptr;    // [nf]
+    int4* d_adj = nullptr;          // [nf] (3 edge neighbours, -1 padded)
+    float* d_verts = nullptr;       // [n_slots][max_vertices*3]
+    float4* d_nodes = nullptr;      // [n_slots][node_cap*2]
+    float4* d_btris = nullptr;      // [n_slots][max_triangles*3]
+    float4* d_proj = nullptr;       // [n_slots][max_triangles*3]
+    float4* d_proj_inv = nullptr;   // [n_slots][max_triangles*3]
+    float4* d_cen = nullptr;        // [n_slots][max_triangles]
+    uint8_t* d_grids = nullptr;     // [n_slots][G^3/8]
+    uint8_t* d_built = nullptr;     // [n_slots] 1 once a slot's mesh is built
+    float render_lo[3] = {0, 0, 0}, render_hi[3] = {1, 1, 1};
+
+    // parameters
+    float* d_params = nullptr;      // FP32 master copy [n_params]
+    bool params_set = false;
+
+    // render workspace (grown on demand)
+    uint32_t ws_pixels = 0;
+    uint64_t ws_samples = 0;
+    float4* r_o = nullptr;          // origin xyz, t
+    float4* r_d = nullptr;          // dir xyz, max_weight
+    float4* r_rgba = nullptr;
+    float* r_depth = nullptr;
+    uint32_t* r_steps = nullptr;    // samples composited
+    uint32_t* r_flags = nullptr;    // bit0 alive, bit1 exited box in the last march
+    uint32_t* alive[2] = {nullptr, nullptr};
+    uint32_t* r_off = nullptr;      // per alive-list entry: first sample
+    uint32_t* r_cnt = nullptr;      // per alive-list entry: samples in this round
+    float4* s_pos = nullptr;        // posed xyz, ray id bits
+    float4* s_cpos = nullptr;       // canonical xyz
+    float4* s_cdir = nullptr;       // canonical direction (warped to [0,1])
+    float4* s_out = nullptr;        // rgb logits, density logit
+    uint32_t* d_counters = nullptr; // [8]: alive counts, sample counter, stats
+    uint32_t* h_counters = nullptr; // pinned mirror
+    insta_render_stats last_stats{};
+
+    // optional per-kernel-class event timing of insta_render
+    bool profiling = false;
+    hipEvent_t prof_ev[2 * INSTA_KERNEL_CLASSES] = {};
+    bool prof_pending[INSTA_KERNEL_CLASSES] = {};
+    uint64_t prof_pending_items[INSTA_KERNEL_CLASSES] = {};
+    insta_profile prof{};
+
+    // misc scratch for AoS entry points
+    void* d_scratch = nullptr;
+    size_t scratch_bytes = 0;
+
+    insta_ctx() = default;
+    insta_ctx(const insta_ctx&) = delete;
+    insta_ctx& operator=(const insta_ctx&) = delete;
+
+    insta::MeshView mesh(uint32_t slot) const {
+        insta::MeshView m;
+        const size_t T = cfg.max_triangles;
+        m.nodes = d_nodes + (size_t)slot * node_cap * 2;
+        m.btris = d_btris + (size_t)slot * T * 3;
+        m.proj = d_proj + (size_t)slot * T * 3;
+        m.proj_inv = d_proj_inv + (size_t)slot * T * 3;
+        m.cen = d_cen + (size_t)slot * T;
+        m.nf = nf;
+        m.leaf_base = leaf_base;
+        m.k = bvh_k;
+        return m;
+    }
+    uint8_t* grid(uint32_t slot) const {
+        const size_t G = cfg.grid_size;
+        return d_grids + (size_t)slot * (G * G * G / 8);
+    }
+};
+
+namespace insta {
+
+template <class T>
+inline void dev_free(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+void set_error(const std::string& msg);
+insta_status hip_fail(hipError_t e, const char* what);
+
+#define INSTA_HIP(call)                                              \
+    do {                                                             \
+        hipError_t _e = (call);                                      \
+        if (_e != hipSuccess) return insta::hip_fail(_e, #call);     \
+    } while (0)
+
+// Hash-grid level constants passed by value to kernels.
+struct GridParams {
+    float scale[INSTA_MAX_LEVELS];
+    uint32_t res[INSTA_MAX_LEVELS];
+    uint32_t offset[INSTA_MAX_LEVELS];
+    uint32_t size[INSTA_MAX_LEVELS];
+    uint32_t dense_mask;
+};
+GridParams grid_params(const insta_ctx* ctx);
+
+// launchers (one per translation unit)
+insta_status launch_build_slots(insta_ctx* ctx, uint32_t first_slot, uint32_t n_slots, const float* verts,
+                                hipStream_t stream);
+insta_status launch_blend_expression(insta_ctx* ctx, uint32_t slot, const float* bases, uint32_t n_bases,
+                                     const float* coeffs, hipStream_t stream);
+insta_status launch_closest(insta_ctx* ctx, uint32_t slot, const float* pts, uint32_t n, int32_t* ids, float* dist,
+                            hipStream_t stream);
+insta_status launch_deform_aos(insta_ctx* ctx, float* coords, uint32_t n, uint32_t stride, int32_t frame,
+                               hipStream_t stream);
+insta_status launch_deform_soa(insta_ctx* ctx, int32_t frame, const uint32_t* n_dev, uint32_t n_max,
+                               hipStream_t stream);
+insta_status launch_occupancy(insta_ctx* ctx, uint32_t slot, float threshold, hipStream_t stream);
+insta_status launch_encode(insta_ctx* ctx, const float* pos, uint32_t n, float* out, hipStream_t stream);
+insta_status launch_encode_backward(insta_ctx* ctx, const float* pos, uint32_t n, const float* dL, float* grad,
+                                    hipStream_t stream);
+insta_status launch_network_aos(insta_ctx* ctx, const float* coords, uint32_t n, uint32_t stride, float* out,
+                                hipStream_t stream);
+insta_status launch_network_soa(insta_ctx* ctx, const uint32_t* n_dev, uint32_t n_max, hipStream_t stream);
+bool network_supported(const insta_config& cfg);
+
+}  // namespace insta

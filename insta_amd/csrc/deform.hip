@@ -1,0 +1,227 @@
+// deform.hip -- posed -> canonical sample deformation (the `warp` kernel of
+// src/core.cu:442-526 behind Core::surface_closest_point / render_deformed).
+//
+// Per sample: nearest triangle on the frame's BVH, then either the
+// triangle's precomputed inverse frame (most regions) or, on the face region
+// (mask 2), an exp-weighted blend of the triangle's and its edge neighbours'
+// frames inverted per sample (include/rta/affine.cuh:45-100), followed by the
+// canonical triangle's (blended) frame.  Directions go through the 3x3 parts
+// and are renormalised.
+#include "context.hpp"
+
+namespace insta {
+namespace {
+
+constexpr uint32_t kDeformThreads = 256;
+
+struct MeshSet {
+    const float4* nodes;
+    const float4* btris;
+    const float4* proj;
+    const float4* proj_inv;
+    const float4* cen;
+    const int4* adj;
+    const uint8_t* built;      // [n_slots]
+    size_t node_stride, tri_stride, cen_stride;  // float4 units per slot
+    uint32_t nf, leaf_base, k, n_slots, disable_interp;
+
+    INSTA_DEV MeshView view(uint32_t slot) const {
+        MeshView m;
+        m.nodes = nodes + slot * node_stride;
+        m.btris = btris + slot * tri_stride;
+        m.proj = proj + slot * tri_stride;
+        m.proj_inv = proj_inv + slot * tri_stride;
+        m.cen = cen + slot * cen_stride;
+        m.nf = nf;
+        m.leaf_base = leaf_base;
+        m.k = k;
+        return m;
+    }
+};
+
+MeshSet mesh_set(const insta_ctx* ctx, const uint8_t* built_dev) {
+    MeshSet s;
+    s.nodes = ctx->d_nodes;
+    s.btris = ctx->d_btris;
+    s.proj = ctx->d_proj;
+    s.proj_inv = ctx->d_proj_inv;
+    s.cen = ctx->d_cen;
+    s.adj = ctx->d_adj;
+    s.built = built_dev;
+    s.node_stride = (size_t)ctx->node_cap * 2;
+    s.tri_stride = (size_t)ctx->cfg.max_triangles * 3;
+    s.cen_stride = ctx->cfg.max_triangles;
+    s.nf = ctx->nf;
+    s.leaf_base = ctx->leaf_base;
+    s.k = ctx->bvh_k;
+    s.n_slots = ctx->n_slots;
+    s.disable_interp = ctx->cfg.disable_interp;
+    return s;
+}
+
+INSTA_DEV void load_rows(const float4* rows, uint32_t id, float m[12]) {
+    const float4 r0 = rows[3 * id], r1 = rows[3 * id + 1], r2 = rows[3 * id + 2];
+    m[0] = r0.x; m[1] = r0.y; m[2] = r0.z; m[3] = r0.w;
+    m[4] = r1.x; m[5] = r1.y; m[6] = r1.z; m[7] = r1.w;
+    m[8] = r2.x; m[9] = r2.y; m[10] = r2.z; m[11] = r2.w;
+}
+
+// interpolate_transformation without the final inverse
+INSTA_DEV void blend_frames(const MeshView& m, const int4 adj, uint32_t id, V3 p, float out[12]) {
+    const int32_t nb[3] = {adj.x, adj.y, adj.z};
+    float e[3];
+    float sum = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        e[k] = 0.0f;
+        if (nb[k] >= 0) {
+            e[k] = __expf(-4.0f * norm(xyz(m.cen[nb[k]]) - p));
+            sum += e[k];
+        }
+    }
+    const float es = __expf(-1.0f * norm(xyz(m.cen[id]) - p));
+    sum += es;
+#pragma unroll
+    for (int i = 0; i < 12; ++i) out[i] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (nb[k] >= 0) {
+            const float w = e[k] / sum;
+            float P[12];
+            load_rows(m.proj, (uint32_t)nb[k], P);
+#pragma unroll
+            for (int i = 0; i < 12; ++i) out[i] = out[i] + w * P[i];
+        }
+    }
+    const float w = es / sum;
+    float P[12];
+    load_rows(m.proj, id, P);
+#pragma unroll
+    for (int i = 0; i < 12; ++i) out[i] = out[i] + w * P[i];
+}
+
+INSTA_DEV void invert_affine(const float m[12], float out[12]) {
+    const float a00 = m[0], a01 = m[1], a02 = m[2];
+    const float a10 = m[4], a11 = m[5], a12 = m[6];
+    const float a20 = m[8], a21 = m[9], a22 = m[10];
+    float c00 = a11 * a22 - a12 * a21, c01 = a02 * a21 - a01 * a22, c02 = a01 * a12 - a02 * a11;
+    float c10 = a12 * a20 - a10 * a22, c11 = a00 * a22 - a02 * a20, c12 = a02 * a10 - a00 * a12;
+    float c20 = a10 * a21 - a11 * a20, c21 = a01 * a20 - a00 * a21, c22 = a00 * a11 - a01 * a10;
+    float det = a00 * c00 + a01 * c10 + a02 * c20;
+    float id = 1.0f / det;
+    float inv[9] = {c00 * id, c01 * id, c02 * id, c10 * id, c11 * id, c12 * id, c20 * id, c21 * id, c22 * id};
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        out[r * 4 + 0] = inv[r * 3 + 0];
+        out[r * 4 + 1] = inv[r * 3 + 1];
+        out[r * 4 + 2] = inv[r * 3 + 2];
+        out[r * 4 + 3] = -(inv[r * 3 + 0] * m[3] + inv[r * 3 + 1] * m[7] + inv[r * 3 + 2] * m[11]);
+    }
+}
+
+INSTA_DEV V3 apply_point(const float m[12], V3 p) {
+    return v3(m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3], m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7],
+              m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11]);
+}
+INSTA_DEV V3 apply_dir(const float m[12], V3 d) {
+    return v3(m[0] * d.x + m[1] * d.y + m[2] * d.z, m[4] * d.x + m[5] * d.y + m[6] * d.z,
+              m[8] * d.x + m[9] * d.y + m[10] * d.z);
+}
+
+// Body of `warp` for one sample; false leaves the sample untouched.
+INSTA_DEV bool warp_one(const MeshSet& ms, uint32_t frame, V3 p, V3 dir_w, V3& out_p, V3& out_d, uint32_t& mask,
+                        uint16_t* stack, uint32_t stride) {
+    if (!(p.x >= 0.0f && p.x <= 1.0f && p.y >= 0.0f && p.y <= 1.0f && p.z >= 0.0f && p.z <= 1.0f)) return false;
+    const MeshView fr = ms.view(frame + 1);
+    const Hit h = bvh_closest(fr, p, stack, stride);
+    if (h.idx < 0) return false;
+    const uint32_t id = h.orig;
+    mask = __float_as_uint(fr.btris[3 * h.idx + 2].z);
+    const MeshView cn = ms.view(0);
+    float W[12], C[12];
+    if (mask == 2u && !ms.disable_interp) {
+        const int4 adj = ms.adj[id];
+        float Wb[12];
+        blend_frames(fr, adj, id, p, Wb);
+        invert_affine(Wb, W);
+        blend_frames(cn, adj, id, p, C);
+    } else {
+        load_rows(fr.proj_inv, id, W);
+        load_rows(cn.proj, id, C);
+    }
+    out_p = apply_point(C, apply_point(W, p));
+    const V3 d = v3(dir_w.x * 2.0f - 1.0f, dir_w.y * 2.0f - 1.0f, dir_w.z * 2.0f - 1.0f);
+    V3 dc = apply_dir(C, apply_dir(W, d));
+    const float n = norm(dc);
+    dc = v3(dc.x / n, dc.y / n, dc.z / n);
+    out_d = v3((dc.x + 1.0f) * 0.5f, (dc.y + 1.0f) * 0.5f, (dc.z + 1.0f) * 0.5f);
+    return true;
+}
+
+// NerfCoordinate samples in place (stride floats per sample)
+__global__ __launch_bounds__(kDeformThreads) void deform_aos_kernel(MeshSet ms, float* __restrict__ coords, uint32_t n,
+                                                                    uint32_t stride, int32_t frame_arg,
+                                                                    uint32_t n_extra) {
+    __shared__ uint16_t stack[kStackDepth * kDeformThreads];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float* c = coords + (size_t)i * stride;
+    const float holder = c[4];
+    if (holder < 0.0f) return;
+    const uint32_t frame = frame_arg < 0 ? (uint32_t)holder : (uint32_t)frame_arg;
+    if (frame >= ms.n_slots - 1 || !ms.built[frame + 1]) return;
+    V3 op, od;
+    uint32_t mask = 0;
+    if (!warp_one(ms, frame, v3(c[0], c[1], c[2]), v3(c[8], c[9], c[10]), op, od, mask, stack + threadIdx.x,
+                  kDeformThreads))
+        return;
+    c[0] = op.x; c[1] = op.y; c[2] = op.z;
+    c[8] = od.x; c[9] = od.y; c[10] = od.z;
+    if (mask != 7u)
+        for (uint32_t m = 0; m < n_extra; ++m) c[stride - n_extra + m] = 1.0f;
+}
+
+// Render samples (SoA): posed position + ray id in s_pos, ray direction from
+// r_d; canonical position / warped direction to s_cpos / s_cdir.
+__global__ __launch_bounds__(kDeformThreads) void deform_soa_kernel(MeshSet ms, int32_t frame,
+                                                                    const uint32_t* __restrict__ n_dev,
+                                                                    const float4* __restrict__ s_pos,
+                                                                    const float4* __restrict__ r_d,
+                                                                    float4* __restrict__ s_cpos,
+                                                                    float4* __restrict__ s_cdir) {
+    __shared__ uint16_t stack[kStackDepth * kDeformThreads];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *n_dev) return;
+    const float4 sp = s_pos[i];
+    const float4 rd = r_d[__float_as_uint(sp.w)];
+    const V3 p = v3(sp.x, sp.y, sp.z);
+    const V3 dir_w = v3((rd.x + 1.0f) * 0.5f, (rd.y + 1.0f) * 0.5f, (rd.z + 1.0f) * 0.5f);
+    V3 op = p, od = dir_w;
+    uint32_t mask = 0;
+    if (frame >= 0) warp_one(ms, (uint32_t)frame, p, dir_w, op, od, mask, stack + threadIdx.x, kDeformThreads);
+    s_cpos[i] = make_float4(op.x, op.y, op.z, __uint_as_float(mask));
+    s_cdir[i] = make_float4(od.x, od.y, od.z, 0.0f);
+}
+
+}  // namespace
+
+insta_status launch_deform_aos(insta_ctx* ctx, float* coords, uint32_t n, uint32_t stride, int32_t frame,
+                               hipStream_t stream) {
+    if (n == 0) return INSTA_OK;
+    hipLaunchKernelGGL(deform_aos_kernel, dim3((n + kDeformThreads - 1) / kDeformThreads), dim3(kDeformThreads), 0,
+                       stream, mesh_set(ctx, ctx->d_built), coords, n, stride, frame, ctx->cfg.n_extra_dims);
+    INSTA_HIP(hipGetLastError());
+    return INSTA_OK;
+}
+
+insta_status launch_deform_soa(insta_ctx* ctx, int32_t frame, const uint32_t* n_dev, uint32_t n_max,
+                               hipStream_t stream) {
+    if (n_max == 0) return INSTA_OK;
+    hipLaunchKernelGGL(deform_soa_kernel, dim3((n_max + kDeformThreads - 1) / kDeformThreads), dim3(kDeformThreads),
+                       0, stream, mesh_set(ctx, ctx->d_built), frame, n_dev, ctx->s_pos, ctx->r_d, ctx->s_cpos,
+                       ctx->s_cdir);
+    INSTA_HIP(hipGetLastError());
+    return INSTA_OK;
+}
+
+}  // namespace insta

@@ -1,0 +1,341 @@
+// network.hip -- hash-grid encoding (forward, backward scatter-add) and the
+// fused density/colour MLP evaluation, FP32 path.
+//
+// Composition follows NerfNetwork::inference_mixed_precision_impl
+// (instant-ngp/include/neural-graphics-primitives/nerf_network.h:159-232):
+//   features = HashGrid(pos) [| extra dims]
+//   d        = density MLP(features)            (16 outputs, bias-free, ReLU hidden)
+//   rgb      = colour MLP([d | SH_deg4(dir)])   (3 outputs)
+//   output   = (rgb logits, d[0] = density logit)
+//
+// Hash grid convention (the reference delegates it to tiny-cuda-nn, absent
+// from its tree; Instant-NGP, Mueller et al. 2022): level l uses scale s_l =
+// b^l * N_min - 1 and resolution ceil(s_l) + 1; position x maps to
+// x * s_l + 0.5, whose floor/fraction give the cell and trilinear weights;
+// dense levels index x + y r + z r^2, hashed levels
+// (x * 1 ^ y * 2654435761 ^ z * 805459861) mod T.
+#include "context.hpp"
+
+namespace insta {
+namespace {
+
+constexpr uint32_t kNetThreads = 128;
+constexpr uint32_t kEncThreads = 256;
+
+template <int F>
+INSTA_DEV void encode_level(const float* __restrict__ table, const GridParams& gp, uint32_t l, V3 x, float acc[F]) {
+    const float sc = gp.scale[l];
+    const float px = x.x * sc + 0.5f, py = x.y * sc + 0.5f, pz = x.z * sc + 0.5f;
+    const float fx = floorf(px), fy = floorf(py), fz = floorf(pz);
+    const float frac[3] = {px - fx, py - fy, pz - fz};
+    const uint32_t g[3] = {(uint32_t)(int32_t)fx, (uint32_t)(int32_t)fy, (uint32_t)(int32_t)fz};
+    const bool dense = (gp.dense_mask >> l) & 1u;
+    const uint32_t res = gp.res[l], size = gp.size[l];
+    const float* base = table + (size_t)gp.offset[l] * F;
+#pragma unroll
+    for (int f = 0; f < F; ++f) acc[f] = 0.0f;
+#pragma unroll
+    for (uint32_t c = 0; c < 8; ++c) {
+        float w = 1.0f;
+        uint32_t cg[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const bool up = (c >> d) & 1u;
+            w = w * (up ? frac[d] : 1.0f - frac[d]);
+            cg[d] = g[d] + (up ? 1u : 0u);
+        }
+        uint32_t idx;
+        if (dense) {
+            idx = cg[0] + cg[1] * res + cg[2] * (res * res);
+            if (idx >= size) idx %= size;
+        } else {
+            idx = (cg[0] ^ (cg[1] * 2654435761u) ^ (cg[2] * 805459861u)) & (size - 1u);
+        }
+        const float* e = base + (size_t)idx * F;
+        if constexpr (F == 2) {
+            const float2 v = *reinterpret_cast<const float2*>(e);
+            acc[0] = acc[0] + w * v.x;
+            acc[1] = acc[1] + w * v.y;
+        } else if constexpr (F == 4) {
+            const float4 v = *reinterpret_cast<const float4*>(e);
+            acc[0] = acc[0] + w * v.x;
+            acc[1] = acc[1] + w * v.y;
+            acc[2] = acc[2] + w * v.z;
+            acc[3] = acc[3] + w * v.w;
+        } else {
+#pragma unroll
+            for (int f = 0; f < F; ++f) acc[f] = acc[f] + w * e[f];
+        }
+    }
+}
+
+template <int F>
+__global__ __launch_bounds__(kEncThreads) void encode_kernel(const float* __restrict__ table, GridParams gp,
+                                                             uint32_t n_levels, const float* __restrict__ pos,
+                                                             uint32_t n, float* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const V3 x = v3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+    float* o = out + (size_t)i * n_levels * F;
+    for (uint32_t l = 0; l < n_levels; ++l) {
+        float acc[F];
+        encode_level<F>(table, gp, l, x, acc);
+#pragma unroll
+        for (int f = 0; f < F; ++f) o[l * F + f] = acc[f];
+    }
+}
+
+// Backward: each sample scatters dL/dfeature * weight into its 8 corners per
+// level with hardware FP32 atomics (global_atomic_add_f32).
+template <int F>
+__global__ __launch_bounds__(kEncThreads) void encode_backward_kernel(GridParams gp, uint32_t n_levels,
+                                                                      const float* __restrict__ pos, uint32_t n,
+                                                                      const float* __restrict__ dL,
+                                                                      float* __restrict__ grad) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float xs[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    for (uint32_t l = 0; l < n_levels; ++l) {
+        const float sc = gp.scale[l];
+        float frac[3];
+        uint32_t g[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const float p = xs[d] * sc + 0.5f;
+            const float fl = floorf(p);
+            frac[d] = p - fl;
+            g[d] = (uint32_t)(int32_t)fl;
+        }
+        float gl[F];
+#pragma unroll
+        for (int f = 0; f < F; ++f) gl[f] = dL[((size_t)i * n_levels + l) * F + f];
+        const bool dense = (gp.dense_mask >> l) & 1u;
+        const uint32_t res = gp.res[l], size = gp.size[l];
+        float* base = grad + (size_t)gp.offset[l] * F;
+#pragma unroll
+        for (uint32_t c = 0; c < 8; ++c) {
+            float w = 1.0f;
+            uint32_t cg[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const bool up = (c >> d) & 1u;
+                w = w * (up ? frac[d] : 1.0f - frac[d]);
+                cg[d] = g[d] + (up ? 1u : 0u);
+            }
+            uint32_t idx;
+            if (dense) {
+                idx = cg[0] + cg[1] * res + cg[2] * (res * res);
+                if (idx >= size) idx %= size;
+            } else {
+                idx = (cg[0] ^ (cg[1] * 2654435761u) ^ (cg[2] * 805459861u)) & (size - 1u);
+            }
+#pragma unroll
+            for (int f = 0; f < F; ++f) atomicAdd(base + (size_t)idx * F + f, w * gl[f]);
+        }
+    }
+}
+
+// real SH bands 0..3 of a direction in [-1,1]^3 (see the oracle for the
+// normalisation constants' closed forms)
+INSTA_DEV void sh16(float x, float y, float z, float* o) {
+    const float xy = x * y, xz = x * z, yz = y * z, x2 = x * x, y2 = y * y, z2 = z * z;
+    o[0] = 0.28209479177387814f;
+    o[1] = -0.48860251190291987f * y;
+    o[2] = 0.48860251190291987f * z;
+    o[3] = -0.48860251190291987f * x;
+    o[4] = 1.0925484305920792f * xy;
+    o[5] = -1.0925484305920792f * yz;
+    o[6] = 0.94617469575755997f * z2 - 0.31539156525251999f;
+    o[7] = -1.0925484305920792f * xz;
+    o[8] = 0.54627421529603959f * x2 - 0.54627421529603959f * y2;
+    o[9] = 0.59004358992664352f * y * (-3.0f * x2 + y2);
+    o[10] = 2.8906114426405538f * xy * z;
+    o[11] = 0.45704579946446572f * y * (1.0f - 5.0f * z2);
+    o[12] = 0.3731763325901154f * z * (5.0f * z2 - 3.0f);
+    o[13] = 0.45704579946446572f * x * (1.0f - 5.0f * z2);
+    o[14] = 1.4453057213202769f * z * (x2 - y2);
+    o[15] = 0.59004358992664352f * x * (-x2 + 3.0f * y2);
+}
+
+// bias-free dense layer, row-major W[out][in]; weights are wave-uniform and
+// reach the FMAs as scalar operands
+template <int NIN, int NOUT, bool RELU>
+INSTA_DEV void dense(const float* __restrict__ W, const float* x, float* y) {
+#pragma unroll
+    for (int o = 0; o < NOUT; ++o) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int i = 0; i < NIN; ++i) acc = __builtin_fmaf(W[o * NIN + i], x[i], acc);
+        y[o] = RELU ? (acc > 0.0f ? acc : 0.0f) : acc;
+    }
+}
+
+struct NetArgs {
+    const float* params;   // MLP weights followed by the hash tables
+    const float* table;
+    GridParams gp;
+    uint32_t density_hidden, rgb_hidden;
+};
+
+constexpr int kWidth = 64;
+constexpr int kDensityOut = 16;
+constexpr int kRgbIn = kDensityOut + 16;
+
+template <int L, int F>
+INSTA_DEV void network_one(const NetArgs& a, V3 pos, V3 dir_w, float out[4]) {
+    constexpr int DIN = L * F;
+    float x[DIN];
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        float acc[F];
+        encode_level<F>(a.table, a.gp, (uint32_t)l, pos, acc);
+#pragma unroll
+        for (int f = 0; f < F; ++f) x[l * F + f] = acc[f];
+    }
+    float h[kWidth], g[kWidth];
+    const float* W = a.params;
+    dense<DIN, kWidth, true>(W, x, h);
+    W += kWidth * DIN;
+    for (uint32_t k = 1; k < a.density_hidden; ++k) {
+        dense<kWidth, kWidth, true>(W, h, g);
+#pragma unroll
+        for (int i = 0; i < kWidth; ++i) h[i] = g[i];
+        W += kWidth * kWidth;
+    }
+    float r[kRgbIn];
+    dense<kWidth, kDensityOut, false>(W, h, r);
+    W += kDensityOut * kWidth;
+    sh16(dir_w.x * 2.0f - 1.0f, dir_w.y * 2.0f - 1.0f, dir_w.z * 2.0f - 1.0f, r + kDensityOut);
+    dense<kRgbIn, kWidth, true>(W, r, h);
+    W += kWidth * kRgbIn;
+    for (uint32_t k = 1; k < a.rgb_hidden; ++k) {
+        dense<kWidth, kWidth, true>(W, h, g);
+#pragma unroll
+        for (int i = 0; i < kWidth; ++i) h[i] = g[i];
+        W += kWidth * kWidth;
+    }
+    float rgb[3];
+    dense<kWidth, 3, false>(W, h, rgb);
+    out[0] = rgb[0];
+    out[1] = rgb[1];
+    out[2] = rgb[2];
+    out[3] = r[0];
+}
+
+template <int L, int F>
+__global__ __launch_bounds__(kNetThreads) void network_aos_kernel(NetArgs a, const float* __restrict__ coords,
+                                                                  uint32_t n, uint32_t stride,
+                                                                  float* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* c = coords + (size_t)i * stride;
+    float o[4];
+    network_one<L, F>(a, v3(c[0], c[1], c[2]), v3(c[8], c[9], c[10]), o);
+    reinterpret_cast<float4*>(out)[i] = make_float4(o[0], o[1], o[2], o[3]);
+}
+
+template <int L, int F>
+__global__ __launch_bounds__(kNetThreads) void network_soa_kernel(NetArgs a, const uint32_t* __restrict__ n_dev,
+                                                                  const float4* __restrict__ cpos,
+                                                                  const float4* __restrict__ cdir,
+                                                                  float4* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *n_dev) return;
+    const float4 p = cpos[i], d = cdir[i];
+    float o[4];
+    network_one<L, F>(a, v3(p.x, p.y, p.z), v3(d.x, d.y, d.z), o);
+    out[i] = make_float4(o[0], o[1], o[2], o[3]);
+}
+
+NetArgs net_args(const insta_ctx* ctx) {
+    NetArgs a;
+    a.params = ctx->d_params;
+    a.table = ctx->d_params + ctx->layout.n_mlp_params;
+    a.gp = grid_params(ctx);
+    a.density_hidden = ctx->cfg.density_hidden_layers;
+    a.rgb_hidden = ctx->cfg.rgb_hidden_layers;
+    return a;
+}
+
+}  // namespace
+
+GridParams grid_params(const insta_ctx* ctx) {
+    GridParams gp{};
+    for (uint32_t l = 0; l < ctx->cfg.n_levels; ++l) {
+        gp.scale[l] = ctx->layout.level_scale[l];
+        gp.res[l] = ctx->layout.level_resolution[l];
+        gp.offset[l] = ctx->layout.level_offset[l];
+        gp.size[l] = ctx->layout.level_size[l];
+        if (ctx->layout.level_dense[l]) gp.dense_mask |= 1u << l;
+    }
+    return gp;
+}
+
+bool network_supported(const insta_config& c) {
+    const bool shape = c.n_neurons == 64 && c.density_out == 16 && c.sh_degree == 4 && c.n_extra_dims == 0 &&
+                       c.density_hidden_layers >= 1 && c.rgb_hidden_layers >= 1;
+    const bool grid = c.n_features_per_level == 2 && (c.n_levels == 8 || c.n_levels == 16);
+    return shape && grid;
+}
+
+#define INSTA_DISPATCH_LF(L_, F_, KERNEL, ...)                                             \
+    if (ctx->cfg.n_levels == L_ && ctx->cfg.n_features_per_level == F_) {                   \
+        hipLaunchKernelGGL((KERNEL<L_, F_>), __VA_ARGS__);                                  \
+        INSTA_HIP(hipGetLastError());                                                       \
+        return INSTA_OK;                                                                    \
+    }
+
+insta_status launch_network_aos(insta_ctx* ctx, const float* coords, uint32_t n, uint32_t stride, float* out,
+                                hipStream_t stream) {
+    if (n == 0) return INSTA_OK;
+    const NetArgs a = net_args(ctx);
+    const dim3 grid((n + kNetThreads - 1) / kNetThreads), block(kNetThreads);
+    INSTA_DISPATCH_LF(8, 2, network_aos_kernel, grid, block, 0, stream, a, coords, n, stride, out)
+    INSTA_DISPATCH_LF(16, 2, network_aos_kernel, grid, block, 0, stream, a, coords, n, stride, out)
+    set_error("network: unsupported (n_levels, n_features_per_level)");
+    return INSTA_ERR_UNSUPPORTED;
+}
+
+insta_status launch_network_soa(insta_ctx* ctx, const uint32_t* n_dev, uint32_t n_max, hipStream_t stream) {
+    if (n_max == 0) return INSTA_OK;
+    const NetArgs a = net_args(ctx);
+    const dim3 grid((n_max + kNetThreads - 1) / kNetThreads), block(kNetThreads);
+    INSTA_DISPATCH_LF(8, 2, network_soa_kernel, grid, block, 0, stream, a, n_dev, ctx->s_cpos, ctx->s_cdir, ctx->s_out)
+    INSTA_DISPATCH_LF(16, 2, network_soa_kernel, grid, block, 0, stream, a, n_dev, ctx->s_cpos, ctx->s_cdir, ctx->s_out)
+    set_error("network: unsupported (n_levels, n_features_per_level)");
+    return INSTA_ERR_UNSUPPORTED;
+}
+
+insta_status launch_encode(insta_ctx* ctx, const float* pos, uint32_t n, float* out, hipStream_t stream) {
+    if (n == 0) return INSTA_OK;
+    const float* table = ctx->d_params + ctx->layout.n_mlp_params;
+    const GridParams gp = grid_params(ctx);
+    const dim3 grid((n + kEncThreads - 1) / kEncThreads), block(kEncThreads);
+    switch (ctx->cfg.n_features_per_level) {
+        case 1: hipLaunchKernelGGL(encode_kernel<1>, grid, block, 0, stream, table, gp, ctx->cfg.n_levels, pos, n, out); break;
+        case 2: hipLaunchKernelGGL(encode_kernel<2>, grid, block, 0, stream, table, gp, ctx->cfg.n_levels, pos, n, out); break;
+        case 4: hipLaunchKernelGGL(encode_kernel<4>, grid, block, 0, stream, table, gp, ctx->cfg.n_levels, pos, n, out); break;
+        case 8: hipLaunchKernelGGL(encode_kernel<8>, grid, block, 0, stream, table, gp, ctx->cfg.n_levels, pos, n, out); break;
+        default: set_error("encode: n_features_per_level must be 1, 2, 4 or 8"); return INSTA_ERR_UNSUPPORTED;
+    }
+    INSTA_HIP(hipGetLastError());
+    return INSTA_OK;
+}
+
+insta_status launch_encode_backward(insta_ctx* ctx, const float* pos, uint32_t n, const float* dL, float* grad,
+                                    hipStream_t stream) {
+    if (n == 0) return INSTA_OK;
+    const GridParams gp = grid_params(ctx);
+    const dim3 grid((n + kEncThreads - 1) / kEncThreads), block(kEncThreads);
+    switch (ctx->cfg.n_features_per_level) {
+        case 1: hipLaunchKernelGGL(encode_backward_kernel<1>, grid, block, 0, stream, gp, ctx->cfg.n_levels, pos, n, dL, grad); break;
+        case 2: hipLaunchKernelGGL(encode_backward_kernel<2>, grid, block, 0, stream, gp, ctx->cfg.n_levels, pos, n, dL, grad); break;
+        case 4: hipLaunchKernelGGL(encode_backward_kernel<4>, grid, block, 0, stream, gp, ctx->cfg.n_levels, pos, n, dL, grad); break;
+        case 8: hipLaunchKernelGGL(encode_backward_kernel<8>, grid, block, 0, stream, gp, ctx->cfg.n_levels, pos, n, dL, grad); break;
+        default: set_error("encode: n_features_per_level must be 1, 2, 4 or 8"); return INSTA_ERR_UNSUPPORTED;
+    }
+    INSTA_HIP(hipGetLastError());
+    return INSTA_OK;
+}
+
+}  // namespace insta

@@ -1,0 +1,550 @@
+// render.hip -- ray init, occupancy-grid march and front-to-back composite.
+//
+// Restates NerfTracer::init_rays_from_camera + trace (instant-ngp/src/
+// testbed_nerf.cu:2030-2244) for a single cascade and cone angle 0 (the
+// reference's unit-cube case, testbed_nerf.cu:2684-2691), with this schedule:
+//   init      one thread per pixel: pinhole ray, render-box entry, one-step
+//             ld jitter and empty-space skip; live rays appended to a list
+//   repeat    march: each live ray emits up to K samples (K sized so all fit
+//                    the sample buffer), counted first, then written
+//                    contiguously at a wave-aggregated atomic offset
+//             deform: posed -> canonical per sample (deform.hip)
+//             network: encode + MLPs per sample (network.hip)
+//             composite: per ray over its samples; rays that neither hit
+//                    the transmittance threshold, left the box nor reached
+//                    max_steps are appended to the next live list
+//   finalize  one thread per pixel writes RGBA/depth/step counts.
+// Each ray's samples depend only on its own t sequence, so the result does
+// not depend on K or on the order rays are processed in.  The host reads one
+// counter (the live-ray count) per round.
+#include "context.hpp"
+
+namespace insta {
+namespace {
+
+constexpr uint32_t kRayThreads = 256;
+
+enum : uint32_t {
+    kCntAliveA = 0,
+    kCntAliveB = 1,
+    kCntSamples = 2,
+    kCntAlive0 = 3,
+    kCntComposited = 4,
+    kCntEmitted = 5,
+    kNumCounters = 16
+};
+
+struct RenderArgs {
+    float M[12];
+    float cx, cy, fx, fy;
+    float off_x, off_y;
+    uint32_t width, height, spp, G, max_steps;
+    float dt, min_transmittance, density_bias;
+    float lo[3], hi[3];
+    const uint8_t* grid;
+};
+
+// ---- Sobol / Owen scrambling (random_val.cuh:159-288) ---------------------
+INSTA_DEV uint32_t lk_permutation(uint32_t x, uint32_t seed) {
+    x += seed;
+    x ^= x * 0x6c50b47cu;
+    x ^= x * 0xb82f1e52u;
+    x ^= x * 0xc7afe638u;
+    x ^= x * 0x8d22f6e6u;
+    return x;
+}
+INSTA_DEV uint32_t owen_scramble(uint32_t x, uint32_t seed) {
+    return __builtin_bitreverse32(lk_permutation(__builtin_bitreverse32(x), seed));
+}
+INSTA_DEV uint32_t hash_combine(uint32_t seed, uint32_t v) { return seed ^ (v + (seed << 6) + (seed >> 2)); }
+// dimension 0 of the Sobol sequence is the bit-reversed index
+INSTA_DEV float ld_random_val0(uint32_t index, uint32_t seed) {
+    const float S = (float)(1.0 / 4294967296.0);
+    index = owen_scramble(index, seed);
+    return (float)owen_scramble(__builtin_bitreverse32(index), hash_combine(seed, 0u)) * S;
+}
+
+// ---- march helpers (testbed_nerf.cu:186-208, 313-337) --------------------
+INSTA_DEV bool box_contains(const RenderArgs& a, V3 p) {
+    return p.x >= a.lo[0] && p.x <= a.hi[0] && p.y >= a.lo[1] && p.y <= a.hi[1] && p.z >= a.lo[2] && p.z <= a.hi[2];
+}
+
+INSTA_DEV bool occupied(const uint8_t* __restrict__ grid, uint32_t G, V3 pos) {
+    const float g = (float)G;
+    const float px = ((pos.x - 0.5f) * 1.0f + 0.5f) * g;
+    const float py = ((pos.y - 0.5f) * 1.0f + 0.5f) * g;
+    const float pz = ((pos.z - 0.5f) * 1.0f + 0.5f) * g;
+    int ix = (int)px, iy = (int)py, iz = (int)pz;
+    const int hi = (int)G - 1;
+    ix = ix < 0 ? 0 : (ix > hi ? hi : ix);
+    iy = iy < 0 ? 0 : (iy > hi ? hi : iy);
+    iz = iz < 0 ? 0 : (iz > hi ? hi : iz);
+    const uint32_t idx = morton3((uint32_t)ix, (uint32_t)iy, (uint32_t)iz);
+    return (grid[idx >> 3] >> (idx & 7u)) & 1u;
+}
+
+INSTA_DEV float advance_to_next_voxel(float t, float dt, V3 pos, V3 dir, V3 idir, uint32_t res) {
+    const float r = (float)res;
+    const V3 p = v3(r * pos.x, r * pos.y, r * pos.z);
+    const float tx = (floorf(p.x + 0.5f + 0.5f * signf(dir.x)) - p.x) * idir.x;
+    const float ty = (floorf(p.y + 0.5f + 0.5f * signf(dir.y)) - p.y) * idir.y;
+    const float tz = (floorf(p.z + 0.5f + 0.5f * signf(dir.z)) - p.z) * idir.z;
+    const float tn = fminf(fminf(tx, ty), tz);
+    const float t_target = t + fmaxf(tn / r, 0.0f);
+    do {
+        t += dt;
+    } while (t < t_target);
+    return t;
+}
+
+// Walk from t to the next occupied position inside the box.  Returns false
+// if the ray leaves the box first.
+INSTA_DEV bool skip_empty(const RenderArgs& a, V3 o, V3 d, V3 idir, float& t, V3& pos) {
+    while (true) {
+        pos = o + d * t;
+        if (!box_contains(a, pos)) return false;
+        if (occupied(a.grid, a.G, pos)) return true;
+        t = advance_to_next_voxel(t, a.dt, pos, d, idir, a.G);
+    }
+}
+
+INSTA_DEV float box_ray_tmin(const RenderArgs& a, V3 o, V3 d) {
+    float tmin = (a.lo[0] - o.x) / d.x, tmax = (a.hi[0] - o.x) / d.x;
+    if (tmin > tmax) { float s = tmin; tmin = tmax; tmax = s; }
+    float tymin = (a.lo[1] - o.y) / d.y, tymax = (a.hi[1] - o.y) / d.y;
+    if (tymin > tymax) { float s = tymin; tymin = tymax; tymax = s; }
+    if (tmin > tymax || tymin > tmax) return 3.40282347e38f;
+    if (tymin > tmin) tmin = tymin;
+    if (tymax < tmax) tmax = tymax;
+    float tzmin = (a.lo[2] - o.z) / d.z, tzmax = (a.hi[2] - o.z) / d.z;
+    if (tzmin > tzmax) { float s = tzmin; tzmin = tzmax; tzmax = s; }
+    if (tmin > tzmax || tzmin > tmax) return 3.40282347e38f;
+    if (tzmin > tmin) tmin = tzmin;
+    return tmin;
+}
+
+// wave-aggregated append: returns this lane's slot (only meaningful if `want`)
+INSTA_DEV uint32_t wave_append(uint32_t* counter, bool want) {
+    const uint64_t mask = __ballot(want);
+    if (mask == 0) return 0;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    const uint64_t below = mask & ((1ull << lane) - 1ull);
+    return base + (uint32_t)__popcll(below);
+}
+
+// wave-wide exclusive prefix sum of `v` plus one atomic reservation
+INSTA_DEV uint32_t wave_reserve(uint32_t* counter, uint32_t v) {
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t incl = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t up = __shfl_up(incl, o);
+        if (lane >= (uint32_t)o) incl += up;
+    }
+    const uint32_t total = __shfl(incl, 63);
+    uint32_t base = 0;
+    if (lane == 63 && total > 0) base = atomicAdd(counter, total);
+    base = __shfl(base, 63);
+    return base + incl - v;
+}
+
+__global__ __launch_bounds__(kRayThreads) void init_rays_kernel(RenderArgs a, float4* __restrict__ r_o,
+                                                                float4* __restrict__ r_d, float4* __restrict__ r_rgba,
+                                                                float* __restrict__ r_depth,
+                                                                uint32_t* __restrict__ r_steps,
+                                                                uint32_t* __restrict__ alive,
+                                                                uint32_t* __restrict__ counters) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = a.width * a.height;
+    bool live = false;
+    if (i < n) {
+        const uint32_t x = i % a.width, y = i / a.width;
+        // pixel_to_ray, pinhole (common_device.cuh:260-315)
+        const float u = ((float)x + a.off_x) / (float)a.width;
+        const float v = ((float)y + a.off_y) / (float)a.height;
+        const V3 d0 = v3((u - a.cx) * (float)a.width / a.fx, (v - a.cy) * (float)a.height / a.fy, 1.0f);
+        V3 d = v3(a.M[0] * d0.x + a.M[1] * d0.y + a.M[2] * d0.z, a.M[4] * d0.x + a.M[5] * d0.y + a.M[6] * d0.z,
+                  a.M[8] * d0.x + a.M[9] * d0.y + a.M[10] * d0.z);
+        const float dn = norm(d);
+        d = v3(d.x / dn, d.y / dn, d.z / dn);
+        const V3 o = v3(a.M[3], a.M[7], a.M[11]);
+        float t = fmaxf(box_ray_tmin(a, o, d), 0.05f) + 1e-6f;
+        if (box_contains(a, o + d * t)) {
+            const V3 idir = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+            t += ld_random_val0(a.spp, i * 786433u) * a.dt;  // advance_pos_nerf (testbed_nerf.cu:663-664)
+            V3 pos;
+            live = skip_empty(a, o, d, idir, t, pos);
+        }
+        r_o[i] = make_float4(o.x, o.y, o.z, t);
+        r_d[i] = make_float4(d.x, d.y, d.z, 0.0f);
+        r_rgba[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        r_depth[i] = 0.0f;
+        r_steps[i] = 0;
+    }
+    const uint32_t slot = wave_append(&counters[kCntAliveA], live);
+    if (live) alive[slot] = i;
+    // counters[kCntAlive0] is derived from the first live count on the host
+}
+
+__global__ __launch_bounds__(kRayThreads) void march_kernel(RenderArgs a, const uint32_t* __restrict__ alive,
+                                                            uint32_t n_alive, uint32_t K, float4* __restrict__ r_o,
+                                                            const float4* __restrict__ r_d,
+                                                            const uint32_t* __restrict__ r_steps,
+                                                            uint32_t* __restrict__ r_flags,
+                                                            uint32_t* __restrict__ r_off, uint32_t* __restrict__ r_cnt,
+                                                            float4* __restrict__ s_pos,
+                                                            uint32_t* __restrict__ counters) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = k < n_alive;
+    uint32_t r = 0, count = 0;
+    bool exited = false;
+    V3 o = v3(0, 0, 0), d = v3(0, 0, 1), idir = v3(0, 0, 1);
+    float t0 = 0.0f;
+    if (valid) {
+        r = alive[k];
+        const float4 ro = r_o[r], rd = r_d[r];
+        o = v3(ro.x, ro.y, ro.z);
+        d = v3(rd.x, rd.y, rd.z);
+        idir = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+        t0 = ro.w;
+        const uint32_t done = r_steps[r];
+        const uint32_t budget = a.max_steps > done ? a.max_steps - done : 0u;
+        const uint32_t kmax = K < budget ? K : budget;
+        float t = t0;
+        V3 pos;
+        while (count < kmax) {
+            if (!skip_empty(a, o, d, idir, t, pos)) {
+                exited = true;
+                break;
+            }
+            ++count;
+            t += a.dt;
+        }
+    }
+    const uint32_t base = wave_reserve(&counters[kCntSamples], count);
+    if (!valid) return;
+    float t = t0;
+    V3 pos;
+    for (uint32_t j = 0; j < count; ++j) {
+        skip_empty(a, o, d, idir, t, pos);
+        s_pos[base + j] = make_float4(pos.x, pos.y, pos.z, __uint_as_float(r));
+        t += a.dt;
+    }
+    r_off[k] = base;
+    r_cnt[k] = count;
+    r_o[r].w = t;
+    r_flags[r] = exited ? 2u : 0u;
+}
+
+__global__ __launch_bounds__(kRayThreads) void composite_kernel(RenderArgs a, const uint32_t* __restrict__ alive,
+                                                                uint32_t n_alive, const uint32_t* __restrict__ r_off,
+                                                                const uint32_t* __restrict__ r_cnt,
+                                                                const float4* __restrict__ s_pos,
+                                                                const float4* __restrict__ s_out,
+                                                                float4* __restrict__ r_d, float4* __restrict__ r_rgba,
+                                                                float* __restrict__ r_depth,
+                                                                uint32_t* __restrict__ r_steps,
+                                                                const uint32_t* __restrict__ r_flags,
+                                                                uint32_t* __restrict__ next_alive,
+                                                                uint32_t* __restrict__ counters, uint32_t next_slot) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    bool keep = false;
+    uint32_t r = 0, used = 0;
+    if (k < n_alive) {
+        r = alive[k];
+        const uint32_t off = r_off[k], cnt = r_cnt[k];
+        float4 acc = r_rgba[r];
+        float4 rd = r_d[r];
+        float depth = r_depth[r];
+        uint32_t n = r_steps[r];
+        const V3 cam_pos = v3(a.M[3], a.M[7], a.M[11]);
+        const V3 cam_fwd = v3(a.M[2], a.M[6], a.M[10]);
+        bool terminated = false;
+        for (uint32_t j = 0; j < cnt; ++j) {
+            const float4 out = s_out[off + j];
+            const float4 sp = s_pos[off + j];
+            // composite_kernel_nerf (testbed_nerf.cu:853-999), render mode Shade
+            const float T = 1.0f - acc.w;
+            const float sigma = expf(out.w + a.density_bias);
+            const float alpha = 1.0f - expf(-sigma * a.dt);
+            const float weight = alpha * T;
+            acc.x = acc.x + (1.0f / (1.0f + expf(-out.x))) * weight;
+            acc.y = acc.y + (1.0f / (1.0f + expf(-out.y))) * weight;
+            acc.z = acc.z + (1.0f / (1.0f + expf(-out.z))) * weight;
+            acc.w = acc.w + weight;
+            if (weight > rd.w) {
+                rd.w = weight;
+                depth = dot(cam_fwd, v3(sp.x, sp.y, sp.z) - cam_pos);
+            }
+            ++n;
+            ++used;
+            if (acc.w > 1.0f - a.min_transmittance) {
+                const float s = acc.w;
+                acc.x /= s; acc.y /= s; acc.z /= s; acc.w /= s;
+                terminated = true;
+                break;
+            }
+        }
+        r_rgba[r] = acc;
+        r_d[r] = rd;
+        r_depth[r] = depth;
+        r_steps[r] = n;
+        keep = !terminated && !(r_flags[r] & 2u) && n < a.max_steps;
+    }
+    const uint32_t slot = wave_append(&counters[next_slot], keep);
+    if (keep) next_alive[slot] = r;
+    // statistics: composited samples this round
+    uint32_t total = used;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) total += __shfl_xor(total, o);
+    if ((threadIdx.x & 63u) == 0 && total) atomicAdd(&counters[kCntComposited], total);
+}
+
+__global__ void finalize_kernel(uint32_t n, const float4* __restrict__ r_rgba, const float* __restrict__ r_depth,
+                                const uint32_t* __restrict__ r_steps, float4* __restrict__ rgba,
+                                float* __restrict__ depth, uint32_t* __restrict__ steps) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 acc = r_rgba[i];
+    rgba[i] = acc.w > 0.001f ? acc : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (depth) depth[i] = acc.w > 0.2f ? r_depth[i] : 1e10f;
+    if (steps) steps[i] = r_steps[i];
+}
+
+__global__ void clear_counters_kernel(uint32_t* counters, uint32_t first, uint32_t n) {
+    const uint32_t i = threadIdx.x;
+    if (i < n) counters[first + i] = 0;
+}
+
+// ---- host side of the Sobol pixel offset (random_val.cuh:272-322) --------
+uint32_t h_bitrev(uint32_t x) {
+    uint32_t r = 0;
+    for (int i = 0; i < 32; ++i) r |= ((x >> i) & 1u) << (31 - i);
+    return r;
+}
+uint32_t h_lk(uint32_t x, uint32_t seed) {
+    x += seed;
+    x ^= x * 0x6c50b47cu;
+    x ^= x * 0xb82f1e52u;
+    x ^= x * 0xc7afe638u;
+    x ^= x * 0x8d22f6e6u;
+    return x;
+}
+uint32_t h_owen(uint32_t x, uint32_t seed) { return h_bitrev(h_lk(h_bitrev(x), seed)); }
+uint32_t h_sobol(uint32_t index, uint32_t dim) {
+    if (dim == 0) return h_bitrev(index);
+    uint32_t v = 0x80000000u, x = 0;
+    for (uint32_t bit = 0; bit < 32; ++bit) {
+        if ((index >> bit) & 1u) x ^= v;
+        v ^= v >> 1;
+    }
+    return x;
+}
+void h_ld_2d(uint32_t index, uint32_t seed, float out[2]) {
+    const float S = (float)(1.0 / 4294967296.0);
+    index = h_owen(index, seed);
+    for (uint32_t i = 0; i < 2; ++i) out[i] = (float)h_owen(h_sobol(index, i), seed ^ (i + (seed << 6) + (seed >> 2))) * S;
+}
+void pixel_offset(uint32_t spp, float out[2]) {
+    float a[2], b[2];
+    h_ld_2d(0, 0xdeadbeefu, a);
+    h_ld_2d(spp, 0xdeadbeefu, b);
+    for (int i = 0; i < 2; ++i) {
+        const float o = 0.5f - a[i] + b[i];
+        out[i] = o - floorf(o);
+    }
+}
+
+void prof_begin(insta_ctx* ctx, int cls, hipStream_t stream) {
+    if (ctx->profiling) (void)hipEventRecord(ctx->prof_ev[2 * cls], stream);
+}
+void prof_end(insta_ctx* ctx, int cls, uint64_t items, hipStream_t stream) {
+    if (!ctx->profiling) return;
+    (void)hipEventRecord(ctx->prof_ev[2 * cls + 1], stream);
+    ctx->prof_pending[cls] = true;
+    ctx->prof_pending_items[cls] = items;
+}
+// after a stream synchronisation: fold the recorded intervals in
+void prof_collect(insta_ctx* ctx) {
+    if (!ctx->profiling) return;
+    for (int c = 0; c < INSTA_KERNEL_CLASSES; ++c) {
+        if (!ctx->prof_pending[c]) continue;
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, ctx->prof_ev[2 * c], ctx->prof_ev[2 * c + 1]) == hipSuccess) {
+            ctx->prof.ms[c] += ms;
+            ctx->prof.launches[c] += 1;
+            ctx->prof.items[c] += ctx->prof_pending_items[c];
+        }
+        ctx->prof_pending[c] = false;
+    }
+}
+
+insta_status ensure_workspace(insta_ctx* ctx, uint32_t n_pixels) {
+    if (n_pixels <= ctx->ws_pixels) return INSTA_OK;
+    dev_free(ctx->r_o); dev_free(ctx->r_d); dev_free(ctx->r_rgba); dev_free(ctx->r_depth);
+    dev_free(ctx->r_steps); dev_free(ctx->r_flags); dev_free(ctx->alive[0]); dev_free(ctx->alive[1]);
+    dev_free(ctx->r_off); dev_free(ctx->r_cnt);
+    dev_free(ctx->s_pos); dev_free(ctx->s_cpos); dev_free(ctx->s_cdir); dev_free(ctx->s_out);
+    ctx->ws_pixels = 0;
+    const size_t n = n_pixels;
+    // sample buffer: 128 samples per pixel, between 1M and 64M samples (4 GiB)
+    uint64_t cap = (uint64_t)n * 128;
+    if (cap < (1u << 20)) cap = 1u << 20;
+    if (cap > (64ull << 20)) cap = 64ull << 20;
+#define INSTA_ALLOC(ptr, bytes)                                           \
+    if (hipMalloc(reinterpret_cast<void**>(&(ptr)), (bytes)) != hipSuccess) { \
+        set_error("render workspace allocation failed");                 \
+        return INSTA_ERR_OOM;                                             \
+    }
+    INSTA_ALLOC(ctx->r_o, n * sizeof(float4));
+    INSTA_ALLOC(ctx->r_d, n * sizeof(float4));
+    INSTA_ALLOC(ctx->r_rgba, n * sizeof(float4));
+    INSTA_ALLOC(ctx->r_depth, n * sizeof(float));
+    INSTA_ALLOC(ctx->r_steps, n * sizeof(uint32_t));
+    INSTA_ALLOC(ctx->r_flags, n * sizeof(uint32_t));
+    INSTA_ALLOC(ctx->alive[0], n * sizeof(uint32_t));
+    INSTA_ALLOC(ctx->alive[1], n * sizeof(uint32_t));
+    INSTA_ALLOC(ctx->r_off, n * sizeof(uint32_t));
+    INSTA_ALLOC(ctx->r_cnt, n * sizeof(uint32_t));
+    INSTA_ALLOC(ctx->s_pos, cap * sizeof(float4));
+    INSTA_ALLOC(ctx->s_cpos, cap * sizeof(float4));
+    INSTA_ALLOC(ctx->s_cdir, cap * sizeof(float4));
+    INSTA_ALLOC(ctx->s_out, cap * sizeof(float4));
+#undef INSTA_ALLOC
+    ctx->ws_pixels = n_pixels;
+    ctx->ws_samples = cap;
+    return INSTA_OK;
+}
+
+}  // namespace
+
+insta_status render(insta_ctx* ctx, const insta_camera* cam, uint32_t width, uint32_t height, int32_t frame,
+                    int32_t occupancy, uint32_t spp, float* rgba, float* depth, uint32_t* steps, hipStream_t stream) {
+    const uint32_t n_pix = width * height;
+    insta_status st = ensure_workspace(ctx, n_pix);
+    if (st != INSTA_OK) return st;
+    RenderArgs a;
+    for (int i = 0; i < 12; ++i) a.M[i] = cam->matrix[i];
+    a.cx = cam->screen_center[0];
+    a.cy = cam->screen_center[1];
+    a.fx = cam->focal_px[0];
+    a.fy = cam->focal_px[1];
+    float off[2];
+    pixel_offset(cam->snap_to_pixel_centers ? 0u : spp, off);
+    a.off_x = off[0];
+    a.off_y = off[1];
+    a.width = width;
+    a.height = height;
+    a.spp = spp;
+    a.G = ctx->cfg.grid_size;
+    a.max_steps = ctx->cfg.max_steps;
+    a.dt = ctx->cfg.dt;
+    a.min_transmittance = ctx->cfg.min_transmittance;
+    a.density_bias = ctx->cfg.density_logit_bias;
+    for (int i = 0; i < 3; ++i) {
+        a.lo[i] = ctx->render_lo[i];
+        a.hi[i] = ctx->render_hi[i];
+    }
+    a.grid = ctx->grid(occupancy < 0 ? 0u : (uint32_t)occupancy + 1u);
+    uint32_t* cnt = ctx->d_counters;
+    hipLaunchKernelGGL(clear_counters_kernel, dim3(1), dim3(64), 0, stream, cnt, 0u, (uint32_t)kNumCounters);
+    prof_begin(ctx, INSTA_KERNEL_INIT, stream);
+    hipLaunchKernelGGL(init_rays_kernel, dim3((n_pix + kRayThreads - 1) / kRayThreads), dim3(kRayThreads), 0, stream, a,
+                       ctx->r_o, ctx->r_d, ctx->r_rgba, ctx->r_depth, ctx->r_steps, ctx->alive[0], cnt);
+    INSTA_HIP(hipGetLastError());
+    prof_end(ctx, INSTA_KERNEL_INIT, n_pix, stream);
+    INSTA_HIP(hipMemcpyAsync(ctx->h_counters, cnt, kNumCounters * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    INSTA_HIP(hipStreamSynchronize(stream));
+    prof_collect(ctx);
+    uint32_t n_alive = ctx->h_counters[kCntAliveA];
+    ctx->last_stats = insta_render_stats{};
+    ctx->last_stats.n_rays = n_pix;
+    ctx->last_stats.n_rays_alive0 = n_alive;
+    uint64_t emitted = 0;
+    uint32_t cur = 0, iters = 0;
+    while (n_alive > 0) {
+        uint64_t K = ctx->ws_samples / n_alive;
+        if (K > ctx->cfg.max_steps) K = ctx->cfg.max_steps;
+        if (K < 1) K = 1;
+        const uint32_t next = 1u - cur;
+        const uint32_t next_slot = next == 0 ? kCntAliveA : kCntAliveB;
+        hipLaunchKernelGGL(clear_counters_kernel, dim3(1), dim3(64), 0, stream, cnt, (uint32_t)next_slot, 1u);
+        hipLaunchKernelGGL(clear_counters_kernel, dim3(1), dim3(64), 0, stream, cnt, (uint32_t)kCntSamples, 1u);
+        const dim3 rgrid((n_alive + kRayThreads - 1) / kRayThreads);
+        prof_begin(ctx, INSTA_KERNEL_MARCH, stream);
+        hipLaunchKernelGGL(march_kernel, rgrid, dim3(kRayThreads), 0, stream, a, ctx->alive[cur], n_alive, (uint32_t)K,
+                           ctx->r_o, ctx->r_d, ctx->r_steps, ctx->r_flags, ctx->r_off, ctx->r_cnt, ctx->s_pos, cnt);
+        INSTA_HIP(hipGetLastError());
+        prof_end(ctx, INSTA_KERNEL_MARCH, n_alive, stream);
+        const uint64_t n_max64 = (uint64_t)n_alive * K;
+        const uint32_t n_max = (uint32_t)(n_max64 < ctx->ws_samples ? n_max64 : ctx->ws_samples);
+        prof_begin(ctx, INSTA_KERNEL_DEFORM, stream);
+        st = launch_deform_soa(ctx, frame, cnt + kCntSamples, n_max, stream);
+        if (st != INSTA_OK) return st;
+        prof_end(ctx, INSTA_KERNEL_DEFORM, 0, stream);
+        prof_begin(ctx, INSTA_KERNEL_NETWORK, stream);
+        st = launch_network_soa(ctx, cnt + kCntSamples, n_max, stream);
+        if (st != INSTA_OK) return st;
+        prof_end(ctx, INSTA_KERNEL_NETWORK, 0, stream);
+        prof_begin(ctx, INSTA_KERNEL_COMPOSITE, stream);
+        hipLaunchKernelGGL(composite_kernel, rgrid, dim3(kRayThreads), 0, stream, a, ctx->alive[cur], n_alive,
+                           ctx->r_off, ctx->r_cnt, ctx->s_pos, ctx->s_out, ctx->r_d, ctx->r_rgba, ctx->r_depth,
+                           ctx->r_steps, ctx->r_flags, ctx->alive[next], cnt, next_slot);
+        INSTA_HIP(hipGetLastError());
+        prof_end(ctx, INSTA_KERNEL_COMPOSITE, n_alive, stream);
+        INSTA_HIP(hipMemcpyAsync(ctx->h_counters, cnt, kNumCounters * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        INSTA_HIP(hipStreamSynchronize(stream));
+        // sample counts are known only now; attribute them to deform/network
+        ctx->prof_pending_items[INSTA_KERNEL_DEFORM] = ctx->h_counters[kCntSamples];
+        ctx->prof_pending_items[INSTA_KERNEL_NETWORK] = ctx->h_counters[kCntSamples];
+        prof_collect(ctx);
+        emitted += ctx->h_counters[kCntSamples];
+        n_alive = ctx->h_counters[next_slot];
+        cur = next;
+        ++iters;
+    }
+    ctx->last_stats.n_samples = ctx->h_counters[kCntComposited];
+    ctx->last_stats.n_samples_emitted = emitted;
+    ctx->last_stats.n_iterations = iters;
+    prof_begin(ctx, INSTA_KERNEL_FINALIZE, stream);
+    hipLaunchKernelGGL(finalize_kernel, dim3((n_pix + kRayThreads - 1) / kRayThreads), dim3(kRayThreads), 0, stream,
+                       n_pix, ctx->r_rgba, ctx->r_depth, ctx->r_steps, reinterpret_cast<float4*>(rgba), depth, steps);
+    INSTA_HIP(hipGetLastError());
+    prof_end(ctx, INSTA_KERNEL_FINALIZE, n_pix, stream);
+    if (ctx->profiling) {
+        INSTA_HIP(hipStreamSynchronize(stream));
+        prof_collect(ctx);
+    }
+    return INSTA_OK;
+}
+
+insta_status set_profiling(insta_ctx* ctx, bool on) {
+    if (on && !ctx->prof_ev[0]) {
+        for (int i = 0; i < 2 * INSTA_KERNEL_CLASSES; ++i) INSTA_HIP(hipEventCreate(&ctx->prof_ev[i]));
+    }
+    ctx->profiling = on;
+    if (on) {
+        ctx->prof = insta_profile{};
+        for (int c = 0; c < INSTA_KERNEL_CLASSES; ++c) ctx->prof_pending[c] = false;
+    }
+    return INSTA_OK;
+}
+
+void free_workspace(insta_ctx* ctx) {
+    for (int i = 0; i < 2 * INSTA_KERNEL_CLASSES; ++i)
+        if (ctx->prof_ev[i]) {
+            (void)hipEventDestroy(ctx->prof_ev[i]);
+            ctx->prof_ev[i] = nullptr;
+        }
+    dev_free(ctx->r_o); dev_free(ctx->r_d); dev_free(ctx->r_rgba); dev_free(ctx->r_depth);
+    dev_free(ctx->r_steps); dev_free(ctx->r_flags); dev_free(ctx->alive[0]); dev_free(ctx->alive[1]);
+    dev_free(ctx->r_off); dev_free(ctx->r_cnt);
+    dev_free(ctx->s_pos); dev_free(ctx->s_cpos); dev_free(ctx->s_cdir); dev_free(ctx->s_out);
+    ctx->ws_pixels = 0;
+}
+
+}  // namespace insta

@@ -1,0 +1,1164 @@
+// oracle.cpp -- scalar CPU restatement of INSTA's per-sample hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the parity oracle and the reported
+// CPU baseline: tests/, the package's smoke() check and bench.py's cpu_baseline
+// leg load it; nothing in the product path (insta_amd/) may call it.  It is
+// compiled with -O3 -fopenmp -ffp-contract=off (no -ffast-math) so that every
+// float operation rounds exactly once, in the order written, like the HIP
+// kernels, which are compiled the same way.  Integer/index results (nearest
+// triangle ids, hash-table indices, occupancy bits, march step counts) must
+// then match the GPU bit for bit on identical inputs.
+//
+// Each function names the reference code it restates (file:line under the
+// reference tree).  Where the reference calls into tiny-cuda-nn (an empty
+// submodule in the reference tree: hash grid, MLPs, spherical harmonics), the
+// published Instant-NGP algorithm (Mueller et al. 2022) is restated and the
+// declared convention is written next to the code.
+//
+// Parity pinning (what checks each part; tests/test_oracle_*.py):
+//   distance_sq / closest     brute force over all triangles, closed forms
+//   bvh build                 invariants (cover, nesting, leaf ranges)
+//   frames / inverse          orthonormality, M * M^-1 = I, rigid motion
+//   warp                      identity pose => identity map; rigid pose =>
+//                             exact rigid inverse; FP64 cross-check
+//   occupancy                 brute-force distances, Morton round trip
+//   ld_random_val / sobol     integer KATs from the recurrence and bit tricks
+//   march                     empty grid => 0 steps; full grid => closed form
+//   composite                 constant sigma closed form
+//   hash encode               vertex => table entry; dense bijection; weights
+//   MLP / SH                  one-hot closed forms; SH orthonormality
+#include "../include/insta_capi.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// small vector math (float, evaluation order fixed)
+// ---------------------------------------------------------------------------
+struct V3 {
+    float x, y, z;
+};
+inline V3 v3(float x, float y, float z) { return {x, y, z}; }
+inline V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 operator*(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 cross(V3 a, V3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+inline float sqnorm(V3 a) { return dot(a, a); }
+inline float norm(V3 a) { return std::sqrt(sqnorm(a)); }
+inline V3 vmin(V3 a, V3 b) { return {std::fmin(a.x, b.x), std::fmin(a.y, b.y), std::fmin(a.z, b.z)}; }
+inline V3 vmax(V3 a, V3 b) { return {std::fmax(a.x, b.x), std::fmax(a.y, b.y), std::fmax(a.z, b.z)}; }
+inline float signf(float x) { return std::copysign(1.0f, x); }  // common.h:191-193
+inline float clampf(float v, float lo, float hi) { return std::fmin(std::fmax(v, lo), hi); }
+
+struct D3 {
+    double x, y, z;
+};
+inline D3 operator-(D3 a, D3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline D3 operator*(D3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+inline double dotd(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline D3 crossd(D3 a, D3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+// Eigen::normalized(): unchanged when the norm is zero.
+inline D3 normalized(D3 a) {
+    double s = dotd(a, a);
+    if (s > 0.0) {
+        double n = std::sqrt(s);
+        return {a.x / n, a.y / n, a.z / n};
+    }
+    return a;
+}
+
+// ---------------------------------------------------------------------------
+// triangle distance (triangle.cuh:88-107) and box distance (bounding_box.cuh:228-230)
+// ---------------------------------------------------------------------------
+struct Tri {
+    V3 a, b, c;
+};
+
+inline float tri_distance_sq(const Tri& t, V3 pos) {
+    V3 v21 = t.b - t.a; V3 p1 = pos - t.a;
+    V3 v32 = t.c - t.b; V3 p2 = pos - t.b;
+    V3 v13 = t.a - t.c; V3 p3 = pos - t.c;
+    V3 nor = cross(v21, v13);
+    float s = signf(dot(cross(v21, nor), p1)) + signf(dot(cross(v32, nor), p2)) +
+              signf(dot(cross(v13, nor), p3));
+    if (s < 2.0f) {
+        float e1 = sqnorm(v21 * clampf(dot(v21, p1) / sqnorm(v21), 0.0f, 1.0f) - p1);
+        float e2 = sqnorm(v32 * clampf(dot(v32, p2) / sqnorm(v32), 0.0f, 1.0f) - p2);
+        float e3 = sqnorm(v13 * clampf(dot(v13, p3) / sqnorm(v13), 0.0f, 1.0f) - p3);
+        return std::fmin(std::fmin(e1, e2), e3);
+    }
+    float d = dot(nor, p1);
+    return d * d / sqnorm(nor);
+}
+
+inline float box_distance_sq(V3 lo, V3 hi, V3 p) {
+    float dx = std::fmax(std::fmax(lo.x - p.x, p.x - hi.x), 0.0f);
+    float dy = std::fmax(std::fmax(lo.y - p.y, p.y - hi.y), 0.0f);
+    float dz = std::fmax(std::fmax(lo.z - p.z, p.z - hi.z), 0.0f);
+    return dx * dx + dy * dy + dz * dz;
+}
+
+inline V3 centroid(const Tri& t) {  // triangle.cuh:220-222
+    return v3((t.a.x + t.b.x + t.c.x) / 3.0f, (t.a.y + t.b.y + t.c.y) / 3.0f,
+              (t.a.z + t.b.z + t.c.z) / 3.0f);
+}
+
+// ---------------------------------------------------------------------------
+// Morton codes (tcnn morton3D / morton3D_invert, used at testbed_nerf.cu:325,
+// 375-377): bit i of x/y/z goes to bit 3i/3i+1/3i+2.
+// ---------------------------------------------------------------------------
+inline uint32_t spread3(uint32_t v) {
+    uint32_t r = 0;
+    for (uint32_t i = 0; i < 10; ++i) r |= ((v >> i) & 1u) << (3 * i);
+    return r;
+}
+inline uint32_t compact3(uint32_t v) {
+    uint32_t r = 0;
+    for (uint32_t i = 0; i < 10; ++i) r |= ((v >> (3 * i)) & 1u) << i;
+    return r;
+}
+inline uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) {
+    return spread3(x) | (spread3(y) << 1) | (spread3(z) << 2);
+}
+
+// ---------------------------------------------------------------------------
+// BVH.  Declared layout (deviation from the reference's nth_element median
+// split, triangle_bvh.cu:548-631, whose element order is implementation
+// defined): triangles sorted by (30-bit Morton code of the centroid in the
+// centroid AABB, original face id); 4^k leaves for the largest k with
+// 4^k <= n, leaf j holding sorted triangles [j*n >> 2k, (j+1)*n >> 2k); an
+// implicit complete 4-ary heap above them (children of node i are 4i+1..4i+4).
+// Traversal (bvh.cuh:48-96): children sorted far-to-near by box distance with
+// a 4-input sorting network, pushed when their distance is <= the best so far
+// plus a margin of a few ulp (so the result is the brute-force argmin);
+// a leaf triangle replaces the best when (dist_sq, original id) is
+// lexicographically smaller -- a layout-independent tie-break instead of the
+// reference's "last visited wins" (bvh.cuh:64).
+// ---------------------------------------------------------------------------
+struct Bvh {
+    uint32_t n = 0, k = 0, leaf_base = 0, n_nodes = 0;
+    std::vector<V3> lo, hi;
+    std::vector<int32_t> begin, end;
+    std::vector<Tri> tris;          // BVH order
+    std::vector<uint32_t> orig;     // BVH order -> original face id
+};
+
+uint32_t bvh_levels(uint32_t n) {
+    uint32_t k = 0;
+    while ((uint64_t(1) << (2 * (k + 1))) <= n) ++k;
+    return k;
+}
+
+void bvh_build(Bvh& b, const std::vector<Tri>& tris) {
+    const uint32_t n = (uint32_t)tris.size();
+    b.n = n;
+    b.k = bvh_levels(n);
+    const uint32_t n_leaves = 1u << (2 * b.k);
+    b.leaf_base = (n_leaves - 1) / 3;
+    b.n_nodes = b.leaf_base + n_leaves;
+    std::vector<V3> cen(n);
+    V3 cmin = v3(INFINITY, INFINITY, INFINITY), cmax = v3(-INFINITY, -INFINITY, -INFINITY);
+    for (uint32_t i = 0; i < n; ++i) {
+        cen[i] = centroid(tris[i]);
+        cmin = vmin(cmin, cen[i]);
+        cmax = vmax(cmax, cen[i]);
+    }
+    V3 ext = cmax - cmin;
+    auto quant = [](float c, float lo, float e) -> uint32_t {
+        if (!(e > 0.0f)) return 0u;
+        float q = (c - lo) / e * 1024.0f;
+        if (!(q > 0.0f)) return 0u;
+        uint32_t u = (uint32_t)q;
+        return u > 1023u ? 1023u : u;
+    };
+    std::vector<uint64_t> keys(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t m = morton3(quant(cen[i].x, cmin.x, ext.x), quant(cen[i].y, cmin.y, ext.y),
+                             quant(cen[i].z, cmin.z, ext.z));
+        keys[i] = (uint64_t(m) << 32) | i;
+    }
+    std::sort(keys.begin(), keys.end());
+    b.tris.resize(n);
+    b.orig.resize(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        b.orig[i] = (uint32_t)(keys[i] & 0xffffffffu);
+        b.tris[i] = tris[b.orig[i]];
+    }
+    b.lo.assign(b.n_nodes, v3(0, 0, 0));
+    b.hi.assign(b.n_nodes, v3(0, 0, 0));
+    b.begin.assign(b.n_nodes, -1);
+    b.end.assign(b.n_nodes, -1);
+    for (uint32_t j = 0; j < n_leaves; ++j) {
+        uint32_t s = (uint32_t)((uint64_t(j) * n) >> (2 * b.k));
+        uint32_t e = (uint32_t)((uint64_t(j + 1) * n) >> (2 * b.k));
+        V3 lo = v3(INFINITY, INFINITY, INFINITY), hi = v3(-INFINITY, -INFINITY, -INFINITY);
+        for (uint32_t t = s; t < e; ++t) {
+            lo = vmin(lo, vmin(b.tris[t].a, vmin(b.tris[t].b, b.tris[t].c)));
+            hi = vmax(hi, vmax(b.tris[t].a, vmax(b.tris[t].b, b.tris[t].c)));
+        }
+        uint32_t node = b.leaf_base + j;
+        b.lo[node] = lo;
+        b.hi[node] = hi;
+        b.begin[node] = (int32_t)s;
+        b.end[node] = (int32_t)e;
+    }
+    for (int64_t node = (int64_t)b.leaf_base - 1; node >= 0; --node) {
+        V3 lo = v3(INFINITY, INFINITY, INFINITY), hi = v3(-INFINITY, -INFINITY, -INFINITY);
+        for (uint32_t c = 0; c < 4; ++c) {
+            lo = vmin(lo, b.lo[4 * node + 1 + c]);
+            hi = vmax(hi, b.hi[4 * node + 1 + c]);
+        }
+        b.lo[node] = lo;
+        b.hi[node] = hi;
+    }
+}
+
+struct Hit {
+    int32_t idx;      // BVH-order index, -1 if none
+    float dist_sq;
+};
+
+Hit bvh_closest(const Bvh& b, V3 p) {
+    int32_t stack[64];
+    int sp = 0;
+    stack[sp++] = 0;
+    float best = FLT_MAX;
+    int32_t best_idx = -1;
+    uint32_t best_orig = 0xffffffffu;
+    while (sp > 0) {
+        int32_t node = stack[--sp];
+        if ((uint32_t)node >= b.leaf_base) {
+            for (int32_t t = b.begin[node]; t < b.end[node]; ++t) {
+                float d = tri_distance_sq(b.tris[t], p);
+                uint32_t o = b.orig[t];
+                if (d < best || (d == best && o < best_orig)) {
+                    best = d;
+                    best_idx = t;
+                    best_orig = o;
+                }
+            }
+        } else {
+            float cd[4];
+            int32_t ci[4];
+            for (int c = 0; c < 4; ++c) {
+                ci[c] = 4 * node + 1 + c;
+                cd[c] = box_distance_sq(b.lo[ci[c]], b.hi[ci[c]], p);
+            }
+            auto cas = [&](int i, int j) {  // larger distance first
+                if (cd[i] < cd[j]) {
+                    std::swap(cd[i], cd[j]);
+                    std::swap(ci[i], ci[j]);
+                }
+            };
+            cas(0, 2); cas(1, 3); cas(0, 1); cas(2, 3); cas(1, 2);
+            // conservative prune: a box whose rounded distance exceeds the
+            // best rounded triangle distance by a few ulp may still hold the
+            // argmin, so keep boxes within a relative 2^-18 + absolute 2^-26
+            const float bound = best + best * 0x1p-18f + 0x1p-26f;
+            for (int c = 0; c < 4; ++c)
+                if (cd[c] <= bound) stack[sp++] = ci[c];
+        }
+    }
+    return {best_idx, best_idx < 0 ? 0.0f : best};
+}
+
+// ---------------------------------------------------------------------------
+// per-triangle frames, built and inverted in FP64 (north-star precision
+// choice); restates Triangle::tbn (triangle.cuh:43-57) and tri2projection
+// (include/rta/affine.cuh:26-39).  Stored as row-major 3x4 float [A | t].
+// ---------------------------------------------------------------------------
+struct Frame12 {
+    float m[12];
+};
+
+void build_projection(const D3& a, const D3& b, const D3& c, bool scaled, double s, double out[12]) {
+    D3 t = b - a;
+    D3 bb = c - a;
+    D3 n = normalized(crossd(t, bb));
+    D3 bi = normalized(crossd(t, n));
+    D3 ta = normalized(crossd(bi, n));
+    double f = scaled ? s : 1.0;
+    D3 cols[3] = {ta, bi, n};
+    for (int r = 0; r < 3; ++r) {
+        for (int col = 0; col < 3; ++col) {
+            double v = r == 0 ? cols[col].x : (r == 1 ? cols[col].y : cols[col].z);
+            out[r * 4 + col] = v * f;
+        }
+    }
+    out[3] = a.x;
+    out[7] = a.y;
+    out[11] = a.z;
+}
+
+void invert_affine_d(const double m[12], double out[12]) {
+    const double a00 = m[0], a01 = m[1], a02 = m[2];
+    const double a10 = m[4], a11 = m[5], a12 = m[6];
+    const double a20 = m[8], a21 = m[9], a22 = m[10];
+    double c00 = a11 * a22 - a12 * a21, c01 = a02 * a21 - a01 * a22, c02 = a01 * a12 - a02 * a11;
+    double c10 = a12 * a20 - a10 * a22, c11 = a00 * a22 - a02 * a20, c12 = a02 * a10 - a00 * a12;
+    double c20 = a10 * a21 - a11 * a20, c21 = a01 * a20 - a00 * a21, c22 = a00 * a11 - a01 * a10;
+    double det = a00 * c00 + a01 * c10 + a02 * c20;
+    double id = 1.0 / det;
+    double inv[9] = {c00 * id, c01 * id, c02 * id, c10 * id, c11 * id, c12 * id, c20 * id, c21 * id, c22 * id};
+    for (int r = 0; r < 3; ++r) {
+        out[r * 4 + 0] = inv[r * 3 + 0];
+        out[r * 4 + 1] = inv[r * 3 + 1];
+        out[r * 4 + 2] = inv[r * 3 + 2];
+        out[r * 4 + 3] = -(inv[r * 3 + 0] * m[3] + inv[r * 3 + 1] * m[7] + inv[r * 3 + 2] * m[11]);
+    }
+}
+
+// float affine inverse for the per-sample blended frame; the HIP kernel uses
+// the same cofactor expansion in the same order.
+void invert_affine_f(const float m[12], float out[12]) {
+    const float a00 = m[0], a01 = m[1], a02 = m[2];
+    const float a10 = m[4], a11 = m[5], a12 = m[6];
+    const float a20 = m[8], a21 = m[9], a22 = m[10];
+    float c00 = a11 * a22 - a12 * a21, c01 = a02 * a21 - a01 * a22, c02 = a01 * a12 - a02 * a11;
+    float c10 = a12 * a20 - a10 * a22, c11 = a00 * a22 - a02 * a20, c12 = a02 * a10 - a00 * a12;
+    float c20 = a10 * a21 - a11 * a20, c21 = a01 * a20 - a00 * a21, c22 = a00 * a11 - a01 * a10;
+    float det = a00 * c00 + a01 * c10 + a02 * c20;
+    float id = 1.0f / det;
+    float inv[9] = {c00 * id, c01 * id, c02 * id, c10 * id, c11 * id, c12 * id, c20 * id, c21 * id, c22 * id};
+    for (int r = 0; r < 3; ++r) {
+        out[r * 4 + 0] = inv[r * 3 + 0];
+        out[r * 4 + 1] = inv[r * 3 + 1];
+        out[r * 4 + 2] = inv[r * 3 + 2];
+        out[r * 4 + 3] = -(inv[r * 3 + 0] * m[3] + inv[r * 3 + 1] * m[7] + inv[r * 3 + 2] * m[11]);
+    }
+}
+
+inline V3 apply_point(const float m[12], V3 p) {
+    return v3(m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3], m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7],
+              m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11]);
+}
+inline V3 apply_dir(const float m[12], V3 d) {
+    return v3(m[0] * d.x + m[1] * d.y + m[2] * d.z, m[4] * d.x + m[5] * d.y + m[6] * d.z,
+              m[8] * d.x + m[9] * d.y + m[10] * d.z);
+}
+
+struct MeshFrame {
+    bool built = false;
+    std::vector<Tri> tris;        // original order
+    std::vector<V3> cen;          // original order, float centroids
+    std::vector<Frame12> proj;    // tri2projection
+    std::vector<Frame12> proj_inv;
+    Bvh bvh;
+};
+
+// ---------------------------------------------------------------------------
+// hash grid layout (declared convention; tiny-cuda-nn GridEncoding is absent
+// from the reference tree):  level l has scale s_l = b^l * N_min - 1 (double,
+// rounded to float once on the host), resolution r_l = ceil(s_l) + 1, and
+// r_l^3 vertices; it is dense (index x + y r + z r^2) when r_l^3 <= T, with
+// size rounded up to a multiple of 8, else hashed into T entries with
+// h = (x * 1) ^ (y * 2654435761) ^ (z * 805459861) mod T (Instant-NGP eq. 4).
+// ---------------------------------------------------------------------------
+void compute_layout(const insta_config& c, insta_layout& L) {
+    std::memset(&L, 0, sizeof(L));
+    const uint64_t T = uint64_t(1) << c.log2_hashmap_size;
+    uint64_t off = 0;
+    for (uint32_t l = 0; l < c.n_levels; ++l) {
+        double s = std::exp2(l * std::log2((double)c.per_level_scale)) * c.base_resolution - 1.0;
+        float sf = (float)s;
+        uint32_t res = (uint32_t)std::ceil(sf) + 1u;
+        uint64_t verts = uint64_t(res) * res * res;
+        bool dense = verts <= T;
+        uint64_t size = dense ? ((verts + 7) / 8) * 8 : T;
+        L.level_scale[l] = sf;
+        L.level_resolution[l] = res;
+        L.level_dense[l] = dense ? 1u : 0u;
+        L.level_offset[l] = (uint32_t)off;
+        L.level_size[l] = (uint32_t)size;
+        off += size;
+    }
+    L.n_grid_params = off * c.n_features_per_level;
+    L.density_in = c.n_levels * c.n_features_per_level + c.n_extra_dims;
+    L.rgb_in = c.density_out + c.sh_degree * c.sh_degree;
+    uint64_t w = c.n_neurons;
+    uint64_t n_mlp = w * L.density_in + (uint64_t)(c.density_hidden_layers - 1) * w * w + c.density_out * w;
+    n_mlp += w * L.rgb_in + (uint64_t)(c.rgb_hidden_layers - 1) * w * w + 3 * w;
+    L.n_mlp_params = n_mlp;
+    L.n_params = n_mlp + L.n_grid_params;
+}
+
+inline uint32_t grid_index(const insta_layout& L, uint32_t l, uint32_t x, uint32_t y, uint32_t z) {
+    const uint32_t size = L.level_size[l];
+    if (L.level_dense[l]) {
+        const uint32_t r = L.level_resolution[l];
+        return (x + y * r + z * (r * r)) % size;
+    }
+    return (x ^ (y * 2654435761u) ^ (z * 805459861u)) % size;
+}
+
+// ---------------------------------------------------------------------------
+// Sobol + Owen scrambling (random_val.cuh:159-288), after Burley 2020,
+// "Practical Hash-based Owen Scrambling" (JCGT 9(4)).  Dimension 0 is the
+// van der Corput sequence (bit reversal); dimension 1 uses direction numbers
+// v_0 = 2^31, v_k = v_{k-1} ^ (v_{k-1} >> 1).
+// ---------------------------------------------------------------------------
+inline uint32_t reverse_bits(uint32_t x) {
+    uint32_t r = 0;
+    for (int i = 0; i < 32; ++i) r |= ((x >> i) & 1u) << (31 - i);
+    return r;
+}
+inline uint32_t sobol_dim(uint32_t index, uint32_t dim) {
+    if (dim == 0) return reverse_bits(index);
+    uint32_t v = 0x80000000u, x = 0;
+    for (uint32_t bit = 0; bit < 32; ++bit) {
+        if ((index >> bit) & 1u) x ^= v;
+        v ^= v >> 1;
+    }
+    return x;
+}
+inline uint32_t hash_combine(uint32_t seed, uint32_t v) { return seed ^ (v + (seed << 6) + (seed >> 2)); }
+inline uint32_t lk_permutation(uint32_t x, uint32_t seed) {
+    x += seed;
+    x ^= x * 0x6c50b47cu;
+    x ^= x * 0xb82f1e52u;
+    x ^= x * 0xc7afe638u;
+    x ^= x * 0x8d22f6e6u;
+    return x;
+}
+inline uint32_t owen_scramble(uint32_t x, uint32_t seed) {
+    return reverse_bits(lk_permutation(reverse_bits(x), seed));
+}
+inline float ld_random_val(uint32_t index, uint32_t seed, uint32_t dim) {
+    const float S = (float)(1.0 / 4294967296.0);
+    index = owen_scramble(index, seed);
+    return (float)owen_scramble(sobol_dim(index, dim), hash_combine(seed, dim)) * S;
+}
+inline void ld_random_val_2d(uint32_t index, uint32_t seed, float out[2]) {
+    const float S = (float)(1.0 / 4294967296.0);
+    index = owen_scramble(index, seed);
+    for (uint32_t i = 0; i < 2; ++i) out[i] = (float)owen_scramble(sobol_dim(index, i), hash_combine(seed, i)) * S;
+}
+// random_val.cuh:317-322
+inline void ld_random_pixel_offset(uint32_t spp, float out[2]) {
+    float a[2], b[2];
+    ld_random_val_2d(0, 0xdeadbeefu, a);
+    ld_random_val_2d(spp, 0xdeadbeefu, b);
+    for (int i = 0; i < 2; ++i) {
+        float o = 0.5f - a[i] + b[i];
+        out[i] = o - std::floor(o);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// real spherical harmonics, degree 4 = bands 0..3 (16 coefficients) of the
+// direction mapped from [0,1]^3 to [-1,1]^3 (tcnn SphericalHarmonics encoding,
+// configs/nerf/main.json:48-62).  Constants are the orthonormal real SH
+// normalisations: 1/(2 sqrt(pi)), sqrt(3/(4 pi)), sqrt(15/pi)/2, sqrt(5/pi)/4,
+// sqrt(15/pi)/4, sqrt(35/(2 pi))/4, sqrt(105/pi)/2, sqrt(21/(2 pi))/4,
+// sqrt(7/pi)/4, sqrt(105/pi)/4.
+// ---------------------------------------------------------------------------
+void sh16(float x, float y, float z, float* o) {
+    const float xy = x * y, xz = x * z, yz = y * z, x2 = x * x, y2 = y * y, z2 = z * z;
+    o[0] = 0.28209479177387814f;
+    o[1] = -0.48860251190291987f * y;
+    o[2] = 0.48860251190291987f * z;
+    o[3] = -0.48860251190291987f * x;
+    o[4] = 1.0925484305920792f * xy;
+    o[5] = -1.0925484305920792f * yz;
+    o[6] = 0.94617469575755997f * z2 - 0.31539156525251999f;
+    o[7] = -1.0925484305920792f * xz;
+    o[8] = 0.54627421529603959f * x2 - 0.54627421529603959f * y2;
+    o[9] = 0.59004358992664352f * y * (-3.0f * x2 + y2);
+    o[10] = 2.8906114426405538f * xy * z;
+    o[11] = 0.45704579946446572f * y * (1.0f - 5.0f * z2);
+    o[12] = 0.3731763325901154f * z * (5.0f * z2 - 3.0f);
+    o[13] = 0.45704579946446572f * x * (1.0f - 5.0f * z2);
+    o[14] = 1.4453057213202769f * z * (x2 - y2);
+    o[15] = 0.59004358992664352f * x * (-x2 + 3.0f * y2);
+}
+
+inline float logistic(float x) { return 1.0f / (1.0f + std::exp(-x)); }
+
+}  // namespace
+
+// ===========================================================================
+// scene (mirror of insta_ctx)
+// ===========================================================================
+struct orc_scene {
+    insta_config cfg{};
+    insta_layout layout{};
+    uint32_t nv = 0, nf = 0;
+    std::vector<int32_t> faces;
+    std::vector<uint32_t> masks;
+    std::vector<int32_t> adjacency;   // [nf*3]
+    MeshFrame canon;
+    std::vector<MeshFrame> frames;
+    V3 render_lo{}, render_hi{};
+    std::vector<float> params;
+    std::vector<std::vector<uint8_t>> grids;   // slot 0 = canonical, slot f+1 = frame f
+    int n_threads = 0;
+};
+
+namespace {
+
+void fill_frame(orc_scene* s, MeshFrame& mf, const float* verts, const MeshFrame* canon) {
+    const uint32_t nf = s->nf;
+    mf.tris.resize(nf);
+    mf.cen.resize(nf);
+    mf.proj.resize(nf);
+    mf.proj_inv.resize(nf);
+    for (uint32_t i = 0; i < nf; ++i) {
+        const int32_t* f = &s->faces[3 * i];
+        Tri t;
+        t.a = v3(verts[3 * f[0]], verts[3 * f[0] + 1], verts[3 * f[0] + 2]);
+        t.b = v3(verts[3 * f[1]], verts[3 * f[1] + 1], verts[3 * f[1] + 2]);
+        t.c = v3(verts[3 * f[2]], verts[3 * f[2] + 1], verts[3 * f[2] + 2]);
+        mf.tris[i] = t;
+        mf.cen[i] = centroid(t);
+        D3 a = {t.a.x, t.a.y, t.a.z}, b = {t.b.x, t.b.y, t.b.z}, c = {t.c.x, t.c.y, t.c.z};
+        double s_ratio = 1.0;
+        if (canon) {
+            const Tri& ct = canon->tris[i];
+            D3 ca = {ct.a.x, ct.a.y, ct.a.z}, cb = {ct.b.x, ct.b.y, ct.b.z}, cc = {ct.c.x, ct.c.y, ct.c.z};
+            D3 cr = crossd(b - a, c - a), ccr = crossd(cb - ca, cc - ca);
+            double area = 0.5 * std::sqrt(dotd(cr, cr)), carea = 0.5 * std::sqrt(dotd(ccr, ccr));
+            s_ratio = area / carea;
+        }
+        double P[12], Pi[12];
+        build_projection(a, b, c, s->masks[i] == 1u, s_ratio, P);
+        invert_affine_d(P, Pi);
+        for (int k = 0; k < 12; ++k) {
+            mf.proj[i].m[k] = (float)P[k];
+            mf.proj_inv[i].m[k] = (float)Pi[k];
+        }
+    }
+    bvh_build(mf.bvh, mf.tris);
+    mf.built = true;
+}
+
+// interpolate_transformation (include/rta/affine.cuh:45-100) without the
+// final inverse: exp-weighted blend of the triangle's own frame (factor 1)
+// and its edge neighbours' frames (factor 4), weights normalised.
+void blend_frames(const orc_scene* s, const MeshFrame& mf, uint32_t id, V3 p, float out[12]) {
+    const int32_t* adj = &s->adjacency[3 * id];
+    float sum = 0.0f;
+    for (int k = 0; k < 3; ++k)
+        if (adj[k] >= 0) sum += std::exp(-4.0f * norm(mf.cen[adj[k]] - p));
+    sum += std::exp(-1.0f * norm(mf.cen[id] - p));
+    for (int i = 0; i < 12; ++i) out[i] = 0.0f;
+    for (int k = 0; k < 3; ++k) {
+        if (adj[k] >= 0) {
+            float a = std::exp(-4.0f * norm(mf.cen[adj[k]] - p));
+            float w = a / sum;
+            const float* m = mf.proj[adj[k]].m;
+            for (int i = 0; i < 12; ++i) out[i] = out[i] + w * m[i];
+        }
+    }
+    float a = std::exp(-1.0f * norm(mf.cen[id] - p));
+    float w = a / sum;
+    const float* m = mf.proj[id].m;
+    for (int i = 0; i < 12; ++i) out[i] = out[i] + w * m[i];
+}
+
+// The per-sample body of `warp` (src/core.cu:470-519).  Returns false when the
+// sample is left untouched.
+bool warp_one(const orc_scene* s, const MeshFrame& fr, V3 p, V3 dir_w, V3& out_p, V3& out_d, uint32_t& mask) {
+    if (!(p.x >= 0.0f && p.x <= 1.0f && p.y >= 0.0f && p.y <= 1.0f && p.z >= 0.0f && p.z <= 1.0f)) return false;
+    Hit h = bvh_closest(fr.bvh, p);
+    if (h.idx < 0) return false;
+    const uint32_t id = fr.bvh.orig[h.idx];
+    mask = s->masks[id];
+    float W[12], C[12];
+    if (mask == 2u && !s->cfg.disable_interp) {
+        float Wb[12];
+        blend_frames(s, fr, id, p, Wb);
+        invert_affine_f(Wb, W);
+        blend_frames(s, s->canon, id, p, C);
+    } else {
+        std::memcpy(W, fr.proj_inv[id].m, sizeof(W));
+        std::memcpy(C, s->canon.proj[id].m, sizeof(C));
+    }
+    out_p = apply_point(C, apply_point(W, p));
+    V3 d = v3(dir_w.x * 2.0f - 1.0f, dir_w.y * 2.0f - 1.0f, dir_w.z * 2.0f - 1.0f);
+    V3 dc = apply_dir(C, apply_dir(W, d));
+    float n = norm(dc);
+    dc = v3(dc.x / n, dc.y / n, dc.z / n);
+    out_d = v3((dc.x + 1.0f) * 0.5f, (dc.y + 1.0f) * 0.5f, (dc.z + 1.0f) * 0.5f);
+    return true;
+}
+
+const MeshFrame* frame_ptr(const orc_scene* s, int32_t frame) {
+    if (frame < 0) return s->canon.built ? &s->canon : nullptr;
+    if ((uint32_t)frame >= s->frames.size() || !s->frames[frame].built) return nullptr;
+    return &s->frames[frame];
+}
+
+// hash-grid forward for one position (see the declared convention above)
+void encode_one(const orc_scene* s, V3 x, float* out) {
+    const insta_layout& L = s->layout;
+    const uint32_t F = s->cfg.n_features_per_level;
+    const float* table = s->params.data() + L.n_mlp_params;
+    const float xs[3] = {x.x, x.y, x.z};
+    for (uint32_t l = 0; l < s->cfg.n_levels; ++l) {
+        const float sc = L.level_scale[l];
+        float frac[3];
+        uint32_t g[3];
+        for (int d = 0; d < 3; ++d) {
+            float pos = xs[d] * sc + 0.5f;
+            float fl = std::floor(pos);
+            frac[d] = pos - fl;
+            g[d] = (uint32_t)(int32_t)fl;
+        }
+        float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (uint32_t c = 0; c < 8; ++c) {
+            float w = 1.0f;
+            uint32_t cg[3];
+            for (int d = 0; d < 3; ++d) {
+                bool up = (c >> d) & 1u;
+                w = w * (up ? frac[d] : 1.0f - frac[d]);
+                cg[d] = g[d] + (up ? 1u : 0u);
+            }
+            uint32_t idx = grid_index(L, l, cg[0], cg[1], cg[2]);
+            const float* e = table + (uint64_t(L.level_offset[l]) + idx) * F;
+            for (uint32_t f = 0; f < F; ++f) acc[f] = acc[f] + w * e[f];
+        }
+        for (uint32_t f = 0; f < F; ++f) out[l * F + f] = acc[f];
+    }
+}
+
+// bias-free dense layer, row-major W[out][in], sequential accumulation
+void dense(const float* W, const float* x, uint32_t n_in, uint32_t n_out, bool relu, float* y) {
+    for (uint32_t o = 0; o < n_out; ++o) {
+        float acc = 0.0f;
+        const float* w = W + (uint64_t)o * n_in;
+        for (uint32_t i = 0; i < n_in; ++i) acc = acc + w[i] * x[i];
+        y[o] = relu ? (acc > 0.0f ? acc : 0.0f) : acc;
+    }
+}
+
+// NerfNetwork::inference_mixed_precision_impl (nerf_network.h:159-232) for
+// one sample: encoding [+ extra dims] -> density MLP -> [density out | SH]
+// -> rgb MLP.  out4 = rgb logits, density logit (density output 0).
+void network_one(const orc_scene* s, V3 pos, V3 dir_w, const float* extra, float* out4) {
+    const insta_config& c = s->cfg;
+    const insta_layout& L = s->layout;
+    const uint32_t W = c.n_neurons;
+    float din[INSTA_MAX_LEVELS * 8 + 64];
+    float h0[256], h1[256], rgb_in[64];
+    encode_one(s, pos, din);
+    const uint32_t enc = c.n_levels * c.n_features_per_level;
+    for (uint32_t e = 0; e < c.n_extra_dims; ++e) din[enc + e] = extra ? extra[e] : 0.0f;
+    const float* p = s->params.data();
+    dense(p, din, L.density_in, W, true, h0);
+    p += (uint64_t)W * L.density_in;
+    for (uint32_t k = 1; k < c.density_hidden_layers; ++k) {
+        dense(p, h0, W, W, true, h1);
+        std::memcpy(h0, h1, W * sizeof(float));
+        p += (uint64_t)W * W;
+    }
+    dense(p, h0, W, c.density_out, false, rgb_in);
+    p += (uint64_t)c.density_out * W;
+    float dx = dir_w.x * 2.0f - 1.0f, dy = dir_w.y * 2.0f - 1.0f, dz = dir_w.z * 2.0f - 1.0f;
+    sh16(dx, dy, dz, rgb_in + c.density_out);
+    dense(p, rgb_in, L.rgb_in, W, true, h0);
+    p += (uint64_t)W * L.rgb_in;
+    for (uint32_t k = 1; k < c.rgb_hidden_layers; ++k) {
+        dense(p, h0, W, W, true, h1);
+        std::memcpy(h0, h1, W * sizeof(float));
+        p += (uint64_t)W * W;
+    }
+    float rgb[3];
+    dense(p, h0, W, 3, false, rgb);
+    out4[0] = rgb[0];
+    out4[1] = rgb[1];
+    out4[2] = rgb[2];
+    out4[3] = rgb_in[0];
+}
+
+// ---------------------------------------------------------------------------
+// occupancy + march helpers (testbed_nerf.cu:186-208, 313-337, 634-686, 722-795)
+// ---------------------------------------------------------------------------
+inline bool occupied(const uint8_t* grid, uint32_t G, V3 pos) {
+    // cascaded_grid_idx_at with mip 0: (pos - 0.5) * 1 + 0.5, scaled by G, truncated
+    float px = ((pos.x - 0.5f) * 1.0f + 0.5f) * (float)G;
+    float py = ((pos.y - 0.5f) * 1.0f + 0.5f) * (float)G;
+    float pz = ((pos.z - 0.5f) * 1.0f + 0.5f) * (float)G;
+    int ix = (int)px, iy = (int)py, iz = (int)pz;
+    const int hi = (int)G - 1;
+    ix = ix < 0 ? 0 : (ix > hi ? hi : ix);
+    iy = iy < 0 ? 0 : (iy > hi ? hi : iy);
+    iz = iz < 0 ? 0 : (iz > hi ? hi : iz);
+    uint32_t idx = morton3((uint32_t)ix, (uint32_t)iy, (uint32_t)iz);
+    return (grid[idx / 8] >> (idx % 8)) & 1u;
+}
+
+inline bool box_contains(V3 lo, V3 hi, V3 p) {
+    return p.x >= lo.x && p.x <= hi.x && p.y >= lo.y && p.y <= hi.y && p.z >= lo.z && p.z <= hi.z;
+}
+
+inline float distance_to_next_voxel(V3 pos, V3 dir, V3 idir, uint32_t res) {
+    const float r = (float)res;
+    V3 p = v3(r * pos.x, r * pos.y, r * pos.z);
+    float tx = (std::floor(p.x + 0.5f + 0.5f * signf(dir.x)) - p.x) * idir.x;
+    float ty = (std::floor(p.y + 0.5f + 0.5f * signf(dir.y)) - p.y) * idir.y;
+    float tz = (std::floor(p.z + 0.5f + 0.5f * signf(dir.z)) - p.z) * idir.z;
+    float t = std::fmin(std::fmin(tx, ty), tz);
+    return std::fmax(t / r, 0.0f);
+}
+
+inline float advance_to_next_voxel(float t, float dt, V3 pos, V3 dir, V3 idir, uint32_t res) {
+    float t_target = t + distance_to_next_voxel(pos, dir, idir, res);
+    do {
+        t += dt;
+    } while (t < t_target);
+    return t;
+}
+
+// BoundingBox::ray_intersect (bounding_box.cuh:163-210), returns tmin
+inline float box_ray_tmin(V3 lo, V3 hi, V3 o, V3 d) {
+    float tmin = (lo.x - o.x) / d.x, tmax = (hi.x - o.x) / d.x;
+    if (tmin > tmax) std::swap(tmin, tmax);
+    float tymin = (lo.y - o.y) / d.y, tymax = (hi.y - o.y) / d.y;
+    if (tymin > tymax) std::swap(tymin, tymax);
+    if (tmin > tymax || tymin > tmax) return FLT_MAX;
+    if (tymin > tmin) tmin = tymin;
+    if (tymax < tmax) tmax = tymax;
+    float tzmin = (lo.z - o.z) / d.z, tzmax = (hi.z - o.z) / d.z;
+    if (tzmin > tzmax) std::swap(tzmin, tzmax);
+    if (tmin > tzmax || tzmin > tmax) return FLT_MAX;
+    if (tzmin > tmin) tmin = tzmin;
+    return tmin;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C API
+// ===========================================================================
+extern "C" {
+
+uint32_t orc_abi_version(void) { return 1u; }
+
+orc_scene* orc_create(const insta_config* cfg) {
+    if (!cfg || cfg->n_levels == 0 || cfg->n_levels > INSTA_MAX_LEVELS) return nullptr;
+    orc_scene* s = new orc_scene();
+    s->cfg = *cfg;
+    compute_layout(s->cfg, s->layout);
+    s->frames.resize(cfg->max_frames);
+    s->grids.resize(cfg->max_frames + 1);
+    return s;
+}
+
+void orc_destroy(orc_scene* s) { delete s; }
+
+void orc_set_threads(orc_scene* s, int n) { s->n_threads = n; }
+
+int orc_query_layout(const insta_config* cfg, insta_layout* out) {
+    compute_layout(*cfg, *out);
+    return 0;
+}
+
+int orc_set_canonical_mesh(orc_scene* s, const float* verts, uint32_t nv, const int32_t* faces, uint32_t nf,
+                           const uint32_t* masks) {
+    s->nv = nv;
+    s->nf = nf;
+    s->faces.assign(faces, faces + 3 * (size_t)nf);
+    s->masks.assign(masks, masks + nf);
+    // get_triangle_3_neighbours (src/tiny_mesh.cu:316-371): faces sharing an
+    // edge, ascending face id, self removed, -1 padded
+    std::vector<std::vector<int32_t>> vf(nv);
+    for (uint32_t i = 0; i < nf; ++i)
+        for (int k = 0; k < 3; ++k) vf[faces[3 * i + k]].push_back((int32_t)i);
+    s->adjacency.assign(3 * (size_t)nf, -1);
+    for (uint32_t i = 0; i < nf; ++i) {
+        std::vector<int32_t> ids;
+        const int32_t v[3] = {faces[3 * i], faces[3 * i + 1], faces[3 * i + 2]};
+        const int pairs[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+        for (auto& pr : pairs) {
+            const auto& A = vf[v[pr[0]]];
+            const auto& B = vf[v[pr[1]]];
+            for (int32_t f : A)
+                if (std::find(B.begin(), B.end(), f) != B.end()) ids.push_back(f);
+        }
+        std::sort(ids.begin(), ids.end());
+        ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+        int k = 0;
+        for (int32_t f : ids) {
+            if (f == (int32_t)i) continue;
+            if (k < 3) s->adjacency[3 * i + k] = f;
+            ++k;
+        }
+    }
+    fill_frame(s, s->canon, verts, nullptr);
+    // render box: vertex AABB inflated by 0.15 |diag| (tiny_mesh.cu:157-162),
+    // intersected with the unit cube (testbed_nerf.cu:2675-2682)
+    V3 lo = v3(INFINITY, INFINITY, INFINITY), hi = v3(-INFINITY, -INFINITY, -INFINITY);
+    for (uint32_t i = 0; i < nv; ++i) {
+        V3 p = v3(verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]);
+        lo = vmin(lo, p);
+        hi = vmax(hi, p);
+    }
+    float amount = norm(hi - lo) * 0.15f;
+    lo = lo - v3(amount, amount, amount);
+    hi = hi + v3(amount, amount, amount);
+    s->render_lo = vmax(lo, v3(0.0f, 0.0f, 0.0f));
+    s->render_hi = vmin(hi, v3(1.0f, 1.0f, 1.0f));
+    return 0;
+}
+
+int orc_set_frame_vertices(orc_scene* s, uint32_t frame, const float* verts) {
+    if (frame >= s->frames.size() || !s->canon.built) return 1;
+    fill_frame(s, s->frames[frame], verts, &s->canon);
+    return 0;
+}
+
+void orc_render_box(const orc_scene* s, float* lo_hi) {
+    lo_hi[0] = s->render_lo.x; lo_hi[1] = s->render_lo.y; lo_hi[2] = s->render_lo.z;
+    lo_hi[3] = s->render_hi.x; lo_hi[4] = s->render_hi.y; lo_hi[5] = s->render_hi.z;
+}
+
+void orc_adjacency(const orc_scene* s, int32_t* out) {
+    std::memcpy(out, s->adjacency.data(), s->adjacency.size() * sizeof(int32_t));
+}
+
+// per-triangle frames of a frame (frame = -1: canonical), [nf*12] each
+int orc_frames(const orc_scene* s, int32_t frame, float* proj, float* proj_inv, float* centroids) {
+    const MeshFrame* fr = frame_ptr(s, frame);
+    if (!fr) return 1;
+    for (uint32_t i = 0; i < s->nf; ++i) {
+        if (proj) std::memcpy(proj + 12 * i, fr->proj[i].m, 48);
+        if (proj_inv) std::memcpy(proj_inv + 12 * i, fr->proj_inv[i].m, 48);
+        if (centroids) {
+            centroids[3 * i] = fr->cen[i].x;
+            centroids[3 * i + 1] = fr->cen[i].y;
+            centroids[3 * i + 2] = fr->cen[i].z;
+        }
+    }
+    return 0;
+}
+
+uint32_t orc_bvh_n_nodes(const orc_scene* s, int32_t frame) {
+    const MeshFrame* fr = frame_ptr(s, frame);
+    return fr ? fr->bvh.n_nodes : 0u;
+}
+
+int orc_export_bvh(const orc_scene* s, int32_t frame, float* nodes, uint32_t* order) {
+    const MeshFrame* fr = frame_ptr(s, frame);
+    if (!fr) return 1;
+    const Bvh& b = fr->bvh;
+    for (uint32_t i = 0; i < b.n_nodes; ++i) {
+        float* o = nodes + 8 * i;
+        o[0] = b.lo[i].x; o[1] = b.lo[i].y; o[2] = b.lo[i].z;
+        std::memcpy(&o[3], &b.begin[i], 4);
+        o[4] = b.hi[i].x; o[5] = b.hi[i].y; o[6] = b.hi[i].z;
+        std::memcpy(&o[7], &b.end[i], 4);
+    }
+    if (order) std::memcpy(order, b.orig.data(), b.n * sizeof(uint32_t));
+    return 0;
+}
+
+int orc_closest_triangles(const orc_scene* s, int32_t frame, const float* pts, uint32_t n, int32_t* ids, float* dist) {
+    const MeshFrame* fr = frame_ptr(s, frame);
+    if (!fr) return 1;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        Hit h = bvh_closest(fr->bvh, v3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]));
+        ids[i] = h.idx < 0 ? -1 : (int32_t)fr->bvh.orig[h.idx];
+        if (dist) dist[i] = std::sqrt(h.dist_sq);
+    }
+    return 0;
+}
+
+// brute force over all triangles with the same (dist_sq, id) order: the
+// pinning check of the BVH traversal
+int orc_closest_triangles_brute(const orc_scene* s, int32_t frame, const float* pts, uint32_t n, int32_t* ids,
+                                float* dist_sq) {
+    const MeshFrame* fr = frame_ptr(s, frame);
+    if (!fr) return 1;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        V3 p = v3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+        float best = FLT_MAX;
+        int32_t bi = -1;
+        for (uint32_t t = 0; t < s->nf; ++t) {
+            float d = tri_distance_sq(fr->tris[t], p);
+            if (d < best) {
+                best = d;
+                bi = (int32_t)t;
+            }
+        }
+        ids[i] = bi;
+        dist_sq[i] = best;
+    }
+    return 0;
+}
+
+float orc_triangle_distance_sq(const float* tri9, const float* p) {
+    Tri t{v3(tri9[0], tri9[1], tri9[2]), v3(tri9[3], tri9[4], tri9[5]), v3(tri9[6], tri9[7], tri9[8])};
+    return tri_distance_sq(t, v3(p[0], p[1], p[2]));
+}
+
+int orc_deform_samples(const orc_scene* s, float* coords, uint32_t n, uint32_t stride, int32_t frame) {
+    const uint32_t n_extra = s->cfg.n_extra_dims;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        float* c = coords + (uint64_t)i * stride;
+        if (c[4] < 0.0f) continue;
+        uint32_t f = frame < 0 ? (uint32_t)c[4] : (uint32_t)frame;
+        if (f >= s->frames.size() || !s->frames[f].built) continue;
+        V3 op, od;
+        uint32_t mask = 0;
+        if (!warp_one(s, s->frames[f], v3(c[0], c[1], c[2]), v3(c[8], c[9], c[10]), op, od, mask)) continue;
+        c[0] = op.x; c[1] = op.y; c[2] = op.z;
+        c[8] = od.x; c[9] = od.y; c[10] = od.z;
+        if (mask != 7u)
+            for (uint32_t m = 0; m < n_extra; ++m) c[stride - n_extra + m] = 1.0f;
+    }
+    return 0;
+}
+
+int orc_build_occupancy(orc_scene* s, int32_t frame, float threshold, uint8_t* out) {
+    const MeshFrame* fr = frame_ptr(s, frame);
+    if (!fr) return 1;
+    const uint32_t G = s->cfg.grid_size;
+    const uint32_t n_cells = G * G * G;
+    std::vector<uint8_t> bits(n_cells / 8, 0);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t byte = 0; byte < (int64_t)(n_cells / 8); ++byte) {
+        uint8_t v = 0;
+        for (uint32_t j = 0; j < 8; ++j) {
+            uint32_t idx = (uint32_t)byte * 8 + j;
+            uint32_t x = compact3(idx), y = compact3(idx >> 1), z = compact3(idx >> 2);
+            V3 pos = v3((((float)x + 0.5f) / (float)G - 0.5f) * 1.0f + 0.5f,
+                        (((float)y + 0.5f) / (float)G - 0.5f) * 1.0f + 0.5f,
+                        (((float)z + 0.5f) / (float)G - 0.5f) * 1.0f + 0.5f);
+            Hit h = bvh_closest(fr->bvh, pos);
+            if (h.idx < 0) continue;
+            float dist = std::sqrt(h.dist_sq);
+            if (dist < threshold && s->masks[fr->bvh.orig[h.idx]] != 0u) v |= (uint8_t)(1u << j);
+        }
+        bits[byte] = v;
+    }
+    s->grids[frame < 0 ? 0 : frame + 1] = bits;
+    if (out) std::memcpy(out, bits.data(), bits.size());
+    return 0;
+}
+
+int orc_set_occupancy(orc_scene* s, int32_t slot, const uint8_t* bits) {
+    const uint32_t G = s->cfg.grid_size;
+    size_t nb = (size_t)G * G * G / 8;
+    s->grids[slot < 0 ? 0 : slot + 1].assign(bits, bits + nb);
+    return 0;
+}
+
+int orc_set_params(orc_scene* s, const float* params, uint64_t n) {
+    if (n != s->layout.n_params) return 1;
+    s->params.assign(params, params + n);
+    return 0;
+}
+
+int orc_encode(const orc_scene* s, const float* pos, uint32_t n, float* out) {
+    const uint32_t w = s->cfg.n_levels * s->cfg.n_features_per_level;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) encode_one(s, v3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]), out + i * w);
+    return 0;
+}
+
+// corner indices of every level (bit-exact KAT surface): idx_out [n][L][8]
+int orc_encode_indices(const orc_scene* s, const float* pos, uint32_t n, uint32_t* idx_out) {
+    const insta_layout& L = s->layout;
+    const uint32_t nl = s->cfg.n_levels;
+    for (uint32_t i = 0; i < n; ++i) {
+        const float xs[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+        for (uint32_t l = 0; l < nl; ++l) {
+            uint32_t g[3];
+            for (int d = 0; d < 3; ++d) g[d] = (uint32_t)(int32_t)std::floor(xs[d] * L.level_scale[l] + 0.5f);
+            for (uint32_t c = 0; c < 8; ++c)
+                idx_out[((uint64_t)i * nl + l) * 8 + c] =
+                    grid_index(L, l, g[0] + (c & 1u), g[1] + ((c >> 1) & 1u), g[2] + ((c >> 2) & 1u));
+        }
+    }
+    return 0;
+}
+
+int orc_encode_backward(const orc_scene* s, const float* pos, uint32_t n, const float* dL, double* grad) {
+    const insta_layout& L = s->layout;
+    const uint32_t F = s->cfg.n_features_per_level, nl = s->cfg.n_levels;
+    for (uint32_t i = 0; i < n; ++i) {
+        const float xs[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+        for (uint32_t l = 0; l < nl; ++l) {
+            float frac[3];
+            uint32_t g[3];
+            for (int d = 0; d < 3; ++d) {
+                float p = xs[d] * L.level_scale[l] + 0.5f;
+                float fl = std::floor(p);
+                frac[d] = p - fl;
+                g[d] = (uint32_t)(int32_t)fl;
+            }
+            for (uint32_t c = 0; c < 8; ++c) {
+                float w = 1.0f;
+                for (int d = 0; d < 3; ++d) w = w * (((c >> d) & 1u) ? frac[d] : 1.0f - frac[d]);
+                uint32_t idx = grid_index(L, l, g[0] + (c & 1u), g[1] + ((c >> 1) & 1u), g[2] + ((c >> 2) & 1u));
+                for (uint32_t f = 0; f < F; ++f)
+                    grad[(uint64_t(L.level_offset[l]) + idx) * F + f] += (double)w * (double)dL[((uint64_t)i * nl + l) * F + f];
+            }
+        }
+    }
+    return 0;
+}
+
+int orc_sh16(const float* dirs_w, uint32_t n, float* out) {
+    for (uint32_t i = 0; i < n; ++i)
+        sh16(dirs_w[3 * i] * 2.0f - 1.0f, dirs_w[3 * i + 1] * 2.0f - 1.0f, dirs_w[3 * i + 2] * 2.0f - 1.0f, out + 16 * i);
+    return 0;
+}
+
+int orc_network_inference(const orc_scene* s, const float* coords, uint32_t n, uint32_t stride, float* out) {
+    const uint32_t n_extra = s->cfg.n_extra_dims;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        const float* c = coords + (uint64_t)i * stride;
+        network_one(s, v3(c[0], c[1], c[2]), v3(c[8], c[9], c[10]), n_extra ? c + stride - n_extra : nullptr,
+                    out + 4 * i);
+    }
+    return 0;
+}
+
+float orc_ld_random_val(uint32_t index, uint32_t seed, uint32_t dim) { return ld_random_val(index, seed, dim); }
+uint32_t orc_sobol(uint32_t index, uint32_t dim) { return sobol_dim(index, dim); }
+uint32_t orc_owen_scramble(uint32_t x, uint32_t seed) { return owen_scramble(x, seed); }
+uint32_t orc_morton3(uint32_t x, uint32_t y, uint32_t z) { return morton3(x, y, z); }
+uint32_t orc_morton3_invert(uint32_t v) { return compact3(v); }
+void orc_pixel_offset(uint32_t spp, float* out2) { ld_random_pixel_offset(spp, out2); }
+
+// Full render of one frame: init_rays_with_payload_kernel_nerf + advance_pos_nerf
+// (testbed_nerf.cu:1854-1963, 634-686), then per ray the sequential body of
+// generate_next_nerf_network_inputs + warp + network + composite_kernel_nerf
+// (722-795, core.cu:442-526, 797-1009).  Sample-by-sample per ray, which gives
+// the same result as the reference's compaction schedule because each ray's
+// sample sequence does not depend on it.  rgba [w*h*4], depth [w*h],
+// steps [w*h], stats_out[0] = samples composited, [1] = rays alive after init.
+int orc_render(const orc_scene* s, const insta_camera* cam, uint32_t width, uint32_t height, int32_t frame,
+               int32_t occupancy, uint32_t spp, const uint32_t* pixel_subset, uint32_t n_subset, float* rgba,
+               float* depth, uint32_t* steps, uint64_t* stats_out) {
+    const MeshFrame* fr = frame_ptr(s, frame);
+    if (!fr || s->params.empty()) return 1;
+    const std::vector<uint8_t>& grid = s->grids[occupancy < 0 ? 0 : occupancy + 1];
+    if (grid.empty()) return 2;
+    const insta_config& c = s->cfg;
+    const uint32_t G = c.grid_size;
+    const float DT = c.dt;
+    const V3 lo = s->render_lo, hi = s->render_hi;
+    const float* M = cam->matrix;
+    const V3 cam_pos = v3(M[3], M[7], M[11]);
+    const V3 cam_fwd = v3(M[2], M[6], M[10]);
+    float offset[2];
+    ld_random_pixel_offset(cam->snap_to_pixel_centers ? 0u : spp, offset);
+    const uint32_t n_pix = pixel_subset ? n_subset : width * height;
+    uint64_t total_samples = 0, alive0 = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : total_samples, alive0)
+    for (int64_t q = 0; q < (int64_t)n_pix; ++q) {
+        const uint32_t i = pixel_subset ? pixel_subset[q] : (uint32_t)q;
+        const uint32_t x = i % width, y = i / width;
+        float* out = rgba + 4 * (uint64_t)q;
+        out[0] = out[1] = out[2] = out[3] = 0.0f;
+        if (depth) depth[q] = 1e10f;
+        if (steps) steps[q] = 0;
+        // pixel_to_ray, pinhole (common_device.cuh:260-315)
+        float u = ((float)x + offset[0]) / (float)width;
+        float v = ((float)y + offset[1]) / (float)height;
+        V3 d0 = v3((u - cam->screen_center[0]) * (float)width / cam->focal_px[0],
+                   (v - cam->screen_center[1]) * (float)height / cam->focal_px[1], 1.0f);
+        V3 d = v3(M[0] * d0.x + M[1] * d0.y + M[2] * d0.z, M[4] * d0.x + M[5] * d0.y + M[6] * d0.z,
+                  M[8] * d0.x + M[9] * d0.y + M[10] * d0.z);
+        float dn = norm(d);
+        d = v3(d.x / dn, d.y / dn, d.z / dn);
+        const V3 o = cam_pos;
+        float t = std::fmax(box_ray_tmin(lo, hi, o, d), 0.05f) + 1e-6f;
+        if (!box_contains(lo, hi, o + d * t)) continue;
+        const V3 idir = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+        // advance_pos_nerf: jitter by one step, skip empty space
+        t += ld_random_val(spp, i * 786433u, 0) * DT;
+        bool alive = true;
+        while (true) {
+            V3 pos = o + d * t;
+            if (!box_contains(lo, hi, pos)) {
+                alive = false;
+                break;
+            }
+            if (occupied(grid.data(), G, pos)) break;
+            t = advance_to_next_voxel(t, DT, pos, d, idir, G);
+        }
+        if (!alive) continue;
+        alive0 += 1;
+        float acc[4] = {0, 0, 0, 0};
+        float local_depth = 0.0f, max_weight = 0.0f;
+        uint32_t n = 0;
+        const V3 dir_w = v3((d.x + 1.0f) * 0.5f, (d.y + 1.0f) * 0.5f, (d.z + 1.0f) * 0.5f);
+        while (n < c.max_steps) {
+            V3 pos;
+            bool inside = true;
+            while (true) {
+                pos = o + d * t;
+                if (!box_contains(lo, hi, pos)) {
+                    inside = false;
+                    break;
+                }
+                if (occupied(grid.data(), G, pos)) break;
+                t = advance_to_next_voxel(t, DT, pos, d, idir, G);
+            }
+            if (!inside) break;
+            // deform (warp) then network on the canonical sample
+            V3 cp = pos, cd = dir_w;
+            uint32_t mask = 0;
+            warp_one(s, *fr, pos, dir_w, cp, cd, mask);
+            float net[4];
+            float ones[64];
+            for (uint32_t e = 0; e < c.n_extra_dims && e < 64; ++e) ones[e] = (mask != 7u) ? 1.0f : 0.0f;
+            network_one(s, cp, cd, c.n_extra_dims ? ones : nullptr, net);
+            // composite_kernel_nerf body
+            float T = 1.0f - acc[3];
+            float sigma = std::exp(net[3] + c.density_logit_bias);
+            float alpha = 1.0f - std::exp(-sigma * DT);
+            float weight = alpha * T;
+            acc[0] = acc[0] + logistic(net[0]) * weight;
+            acc[1] = acc[1] + logistic(net[1]) * weight;
+            acc[2] = acc[2] + logistic(net[2]) * weight;
+            acc[3] = acc[3] + weight;
+            if (weight > max_weight) {
+                max_weight = weight;
+                local_depth = dot(cam_fwd, pos - cam_pos);
+            }
+            ++n;
+            t += DT;
+            if (acc[3] > 1.0f - c.min_transmittance) {
+                float a = acc[3];
+                acc[0] /= a; acc[1] /= a; acc[2] /= a; acc[3] /= a;
+                break;
+            }
+        }
+        total_samples += n;
+        if (steps) steps[q] = n;
+        if (acc[3] > 0.001f) {
+            out[0] = acc[0]; out[1] = acc[1]; out[2] = acc[2]; out[3] = acc[3];
+        }
+        if (depth && acc[3] > 0.2f) depth[q] = local_depth;
+    }
+    if (stats_out) {
+        stats_out[0] = total_samples;
+        stats_out[1] = alive0;
+    }
+    return 0;
+}
+
+}  // extern "C"

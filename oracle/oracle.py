@@ -1,0 +1,243 @@
+"""ctypes binding of the CPU oracle (oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the package's smoke() check and
+bench.py's cpu_baseline leg, never by the product path.  All buffers are
+numpy arrays on the host.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from insta_amd.capi import InstaCamera, InstaConfig, InstaLayout
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+_VP = C.c_void_p
+_SIGS = [
+    ("orc_abi_version", C.c_uint32, []),
+    ("orc_create", _VP, [C.POINTER(InstaConfig)]),
+    ("orc_destroy", None, [_VP]),
+    ("orc_set_threads", None, [_VP, C.c_int]),
+    ("orc_query_layout", C.c_int, [C.POINTER(InstaConfig), C.POINTER(InstaLayout)]),
+    ("orc_set_canonical_mesh", C.c_int, [_VP, _VP, C.c_uint32, _VP, C.c_uint32, _VP]),
+    ("orc_set_frame_vertices", C.c_int, [_VP, C.c_uint32, _VP]),
+    ("orc_render_box", None, [_VP, _VP]),
+    ("orc_adjacency", None, [_VP, _VP]),
+    ("orc_frames", C.c_int, [_VP, C.c_int32, _VP, _VP, _VP]),
+    ("orc_bvh_n_nodes", C.c_uint32, [_VP, C.c_int32]),
+    ("orc_export_bvh", C.c_int, [_VP, C.c_int32, _VP, _VP]),
+    ("orc_closest_triangles", C.c_int, [_VP, C.c_int32, _VP, C.c_uint32, _VP, _VP]),
+    ("orc_closest_triangles_brute", C.c_int, [_VP, C.c_int32, _VP, C.c_uint32, _VP, _VP]),
+    ("orc_triangle_distance_sq", C.c_float, [_VP, _VP]),
+    ("orc_deform_samples", C.c_int, [_VP, _VP, C.c_uint32, C.c_uint32, C.c_int32]),
+    ("orc_build_occupancy", C.c_int, [_VP, C.c_int32, C.c_float, _VP]),
+    ("orc_set_occupancy", C.c_int, [_VP, C.c_int32, _VP]),
+    ("orc_set_params", C.c_int, [_VP, _VP, C.c_uint64]),
+    ("orc_encode", C.c_int, [_VP, _VP, C.c_uint32, _VP]),
+    ("orc_encode_indices", C.c_int, [_VP, _VP, C.c_uint32, _VP]),
+    ("orc_encode_backward", C.c_int, [_VP, _VP, C.c_uint32, _VP, _VP]),
+    ("orc_sh16", C.c_int, [_VP, C.c_uint32, _VP]),
+    ("orc_network_inference", C.c_int, [_VP, _VP, C.c_uint32, C.c_uint32, _VP]),
+    ("orc_ld_random_val", C.c_float, [C.c_uint32, C.c_uint32, C.c_uint32]),
+    ("orc_sobol", C.c_uint32, [C.c_uint32, C.c_uint32]),
+    ("orc_owen_scramble", C.c_uint32, [C.c_uint32, C.c_uint32]),
+    ("orc_morton3", C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32]),
+    ("orc_morton3_invert", C.c_uint32, [C.c_uint32]),
+    ("orc_pixel_offset", None, [C.c_uint32, _VP]),
+    ("orc_render", C.c_int, [_VP, C.POINTER(InstaCamera), C.c_uint32, C.c_uint32, C.c_int32, C.c_int32,
+                             C.c_uint32, _VP, C.c_uint32, _VP, _VP, _VP, _VP]),
+]
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle`")
+        L = C.CDLL(str(LIB_PATH))
+        for name, res, args in _SIGS:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data
+
+
+def _c(a, dtype) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+class Oracle:
+    """CPU mirror of an insta_ctx: same call sequence, host arrays."""
+
+    def __init__(self, cfg: InstaConfig):
+        self.L = lib()
+        self.cfg = cfg
+        self.h = self.L.orc_create(C.byref(cfg))
+        if not self.h:
+            raise ValueError("oracle rejected the configuration")
+        self.layout = InstaLayout()
+        self.L.orc_query_layout(C.byref(cfg), C.byref(self.layout))
+        self.nf = 0
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.orc_destroy(self.h)
+            self.h = None
+
+    def set_threads(self, n: int) -> None:
+        self.L.orc_set_threads(self.h, n)
+
+    def set_canonical_mesh(self, verts, faces, masks) -> None:
+        v, f, m = _c(verts, np.float32), _c(faces, np.int32), _c(masks, np.uint32)
+        self.nf = len(f)
+        assert self.L.orc_set_canonical_mesh(self.h, _p(v), len(v), _p(f), len(f), _p(m)) == 0
+
+    def set_frame(self, frame: int, verts) -> None:
+        v = _c(verts, np.float32)
+        assert self.L.orc_set_frame_vertices(self.h, frame, _p(v)) == 0
+
+    def render_box(self) -> np.ndarray:
+        out = np.zeros(6, np.float32)
+        self.L.orc_render_box(self.h, _p(out))
+        return out
+
+    def adjacency(self) -> np.ndarray:
+        out = np.zeros((self.nf, 3), np.int32)
+        self.L.orc_adjacency(self.h, _p(out))
+        return out
+
+    def frames(self, frame: int):
+        P = np.zeros((self.nf, 3, 4), np.float32)
+        Pi = np.zeros((self.nf, 3, 4), np.float32)
+        cen = np.zeros((self.nf, 3), np.float32)
+        assert self.L.orc_frames(self.h, frame, _p(P), _p(Pi), _p(cen)) == 0
+        return P, Pi, cen
+
+    def export_bvh(self, frame: int):
+        nn = self.L.orc_bvh_n_nodes(self.h, frame)
+        nodes = np.zeros(nn * 8, np.float32)
+        order = np.zeros(self.nf, np.uint32)
+        assert self.L.orc_export_bvh(self.h, frame, _p(nodes), _p(order)) == 0
+        return nodes, order
+
+    def closest(self, frame: int, pts):
+        p = _c(pts, np.float32)
+        ids = np.zeros(len(p), np.int32)
+        dist = np.zeros(len(p), np.float32)
+        assert self.L.orc_closest_triangles(self.h, frame, _p(p), len(p), _p(ids), _p(dist)) == 0
+        return ids, dist
+
+    def closest_brute(self, frame: int, pts):
+        p = _c(pts, np.float32)
+        ids = np.zeros(len(p), np.int32)
+        d2 = np.zeros(len(p), np.float32)
+        assert self.L.orc_closest_triangles_brute(self.h, frame, _p(p), len(p), _p(ids), _p(d2)) == 0
+        return ids, d2
+
+    def deform(self, coords, frame: int = -1) -> np.ndarray:
+        c = _c(coords, np.float32).copy()
+        n, stride = c.shape
+        assert self.L.orc_deform_samples(self.h, _p(c), n, stride, frame) == 0
+        return c
+
+    def build_occupancy(self, frame: int, threshold: float) -> np.ndarray:
+        G = self.cfg.grid_size
+        out = np.zeros(G ** 3 // 8, np.uint8)
+        assert self.L.orc_build_occupancy(self.h, frame, threshold, _p(out)) == 0
+        return out
+
+    def set_occupancy(self, slot: int, bits) -> None:
+        b = _c(bits, np.uint8)
+        assert self.L.orc_set_occupancy(self.h, slot, _p(b)) == 0
+
+    def set_params(self, params) -> None:
+        p = _c(params, np.float32)
+        assert self.L.orc_set_params(self.h, _p(p), p.size) == 0
+
+    def encode(self, pos) -> np.ndarray:
+        p = _c(pos, np.float32)
+        out = np.zeros((len(p), self.cfg.n_levels * self.cfg.n_features_per_level), np.float32)
+        assert self.L.orc_encode(self.h, _p(p), len(p), _p(out)) == 0
+        return out
+
+    def encode_indices(self, pos) -> np.ndarray:
+        p = _c(pos, np.float32)
+        out = np.zeros((len(p), self.cfg.n_levels, 8), np.uint32)
+        assert self.L.orc_encode_indices(self.h, _p(p), len(p), _p(out)) == 0
+        return out
+
+    def encode_backward(self, pos, dL) -> np.ndarray:
+        p = _c(pos, np.float32)
+        d = _c(dL, np.float32)
+        g = np.zeros(self.layout.n_grid_params, np.float64)
+        assert self.L.orc_encode_backward(self.h, _p(p), len(p), _p(d), _p(g)) == 0
+        return g
+
+    def network(self, coords) -> np.ndarray:
+        c = _c(coords, np.float32)
+        out = np.zeros((len(c), 4), np.float32)
+        assert self.L.orc_network_inference(self.h, _p(c), len(c), c.shape[1], _p(out)) == 0
+        return out
+
+    def render(self, cam: InstaCamera, width: int, height: int, frame: int, occupancy: int, spp: int = 0,
+               pixels=None):
+        sub = None if pixels is None else _c(pixels, np.uint32)
+        n = width * height if sub is None else len(sub)
+        rgba = np.zeros((n, 4), np.float32)
+        depth = np.zeros(n, np.float32)
+        steps = np.zeros(n, np.uint32)
+        stats = np.zeros(2, np.uint64)
+        rc = self.L.orc_render(self.h, C.byref(cam), width, height, frame, occupancy, spp, _p(sub),
+                               0 if sub is None else len(sub), _p(rgba), _p(depth), _p(steps), _p(stats))
+        assert rc == 0, rc
+        return rgba, depth, steps, stats
+
+
+def sh16(dirs_w) -> np.ndarray:
+    d = _c(dirs_w, np.float32)
+    out = np.zeros((len(d), 16), np.float32)
+    lib().orc_sh16(_p(d), len(d), _p(out))
+    return out
+
+
+def ld_random_val(index: int, seed: int, dim: int = 0) -> float:
+    return lib().orc_ld_random_val(index, seed, dim)
+
+
+def sobol(index: int, dim: int) -> int:
+    return lib().orc_sobol(index, dim)
+
+
+def owen_scramble(x: int, seed: int) -> int:
+    return lib().orc_owen_scramble(x, seed)
+
+
+def morton3(x: int, y: int, z: int) -> int:
+    return lib().orc_morton3(x, y, z)
+
+
+def morton3_invert(v: int) -> int:
+    return lib().orc_morton3_invert(v)
+
+
+def pixel_offset(spp: int) -> np.ndarray:
+    out = np.zeros(2, np.float32)
+    lib().orc_pixel_offset(spp, _p(out))
+    return out
+
+
+def triangle_distance_sq(tri9, p) -> float:
+    t = _c(tri9, np.float32)
+    q = _c(p, np.float32)
+    return lib().orc_triangle_distance_sq(_p(t), _p(q))

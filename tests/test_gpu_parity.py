@@ -1,0 +1,296 @@
+"""GPU parity: the HIP library through the C ABI against the CPU oracle on
+identical seeded inputs.
+
+Bar: nearest-triangle ids, BVH layout, occupancy bits, hash-grid features and
+march step counts are bit-exact; FP32 positions, directions, network outputs,
+composited pixels and depths are within 1e-5 relative (relative to
+|ref| + mean|ref|, so entries near zero are judged on the array's scale).
+"""
+import numpy as np
+import pytest
+
+from insta_amd import synth
+from insta_amd.capi import full_config, make_camera, small_config
+from tests.conftest import assert_rel_close, ensure_oracle
+
+pytestmark = pytest.mark.gpu
+RTOL = 1e-5
+
+
+def _torch():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def _scene(cfg, avatar, params_seed=7):
+    """The same avatar in the HIP library and in the oracle."""
+    torch = _torch()
+    ensure_oracle()
+    from insta_amd.capi import Context
+    from oracle.oracle import Oracle
+    ctx = Context(cfg)
+    orc = Oracle(cfg)
+    ctx.set_canonical_mesh(avatar.canonical, avatar.faces, avatar.masks)
+    orc.set_canonical_mesh(avatar.canonical, avatar.faces, avatar.masks)
+    frames = np.stack(avatar.frames).astype(np.float32)
+    ctx.set_frames(0, torch.from_numpy(frames).cuda())
+    for f, v in enumerate(avatar.frames):
+        orc.set_frame(f, v)
+    params = synth.network_params(params_seed, ctx.layout, cfg)
+    ctx.set_params(torch.from_numpy(params).cuda())
+    orc.set_params(params)
+    torch.cuda.synchronize()
+    return ctx, orc
+
+
+@pytest.fixture(scope="module")
+def small(small_avatar):
+    return _scene(small_config(), small_avatar)
+
+
+@pytest.fixture(scope="module")
+def head(head_avatar):
+    return _scene(full_config(max_frames=4), head_avatar)
+
+
+def _points(rng, n, lo=0.0, hi=1.0):
+    return rng.uniform(lo, hi, (n, 3)).astype(np.float32)
+
+
+def _near_surface(verts, faces, n, rng, spread=0.02):
+    tri = verts[faces].astype(np.float64)
+    k = rng.integers(0, len(faces), n)
+    bary = rng.dirichlet([1, 1, 1], n)
+    p = np.einsum("nk,nkd->nd", bary, tri[k])
+    return (p + spread * rng.standard_normal((n, 3))).astype(np.float32)
+
+
+def _coords(pos, dirs_unit, frame, stride=11):
+    c = np.zeros((len(pos), stride), np.float32)
+    c[:, 0:3] = pos
+    c[:, 4] = frame
+    c[:, 5:8] = pos
+    c[:, 8:11] = (dirs_unit + 1.0) * 0.5
+    return c
+
+
+# ---------------------------------------------------------------- BVH
+@pytest.mark.parametrize("frame", [-1, 0, 1])
+def test_closest_ids_bit_exact_small(small, small_avatar, frame):
+    torch = _torch()
+    ctx, orc = small
+    rng = np.random.default_rng(100 + frame)
+    pts = np.concatenate([_points(rng, 60000), _near_surface(small_avatar.canonical, small_avatar.faces, 60000, rng)])
+    ids, dist = ctx.closest(frame, torch.from_numpy(pts).cuda())
+    rid, rdist = orc.closest(frame, pts)
+    assert np.array_equal(ids.cpu().numpy(), rid)
+    assert np.array_equal(dist.cpu().numpy(), rdist)
+
+
+def test_closest_ids_bit_exact_head(head, head_avatar):
+    torch = _torch()
+    ctx, orc = head
+    rng = np.random.default_rng(5)
+    pts = np.concatenate([_points(rng, 50000, 0.3, 0.7),
+                          _near_surface(head_avatar.frames[0], head_avatar.faces, 50000, rng, 0.01)])
+    for frame in (-1, 0, 3):
+        ids, dist = ctx.closest(frame, torch.from_numpy(pts).cuda())
+        rid, rdist = orc.closest(frame, pts)
+        assert np.array_equal(ids.cpu().numpy(), rid), frame
+        assert np.array_equal(dist.cpu().numpy(), rdist), frame
+
+
+@pytest.mark.parametrize("which", ["small", "head"])
+def test_device_bvh_equals_oracle_bvh(which, request):
+    ctx, orc = request.getfixturevalue(which)
+    for frame in (-1, 0):
+        nodes, order = ctx.export_bvh(frame)
+        rn, ro = orc.export_bvh(frame)
+        assert np.array_equal(order.cpu().numpy().view(np.uint32), ro)
+        assert np.array_equal(nodes.cpu().numpy().view(np.uint32), rn.view(np.uint32))
+
+
+def test_expression_rebuild_matches_host_blend(head, head_avatar):
+    torch = _torch()
+    ctx, orc = head
+    rng = np.random.default_rng(21)
+    coeffs = rng.standard_normal(head_avatar.bases.shape[0]).astype(np.float32)
+    ctx.set_frame_expression(2, torch.from_numpy(head_avatar.bases).cuda(), torch.from_numpy(coeffs).cuda())
+    # the same float32 accumulation order on the host
+    v = head_avatar.canonical.astype(np.float32).copy()
+    for k in range(len(coeffs)):
+        v = (v + coeffs[k] * head_avatar.bases[k]).astype(np.float32)
+    orc.set_frame(2, v)
+    pts = _near_surface(v, head_avatar.faces, 40000, rng, 0.01)
+    ids, dist = ctx.closest(2, torch.from_numpy(pts).cuda())
+    rid, rdist = orc.closest(2, pts)
+    assert np.array_equal(ids.cpu().numpy(), rid)
+    assert np.array_equal(dist.cpu().numpy(), rdist)
+    # restore frame 2 for later tests
+    ctx.set_frames(2, torch.from_numpy(head_avatar.frames[2][None]).cuda())
+    orc.set_frame(2, head_avatar.frames[2])
+
+
+# ---------------------------------------------------------------- deform
+@pytest.mark.parametrize("which", ["small", "head"])
+def test_deform_matches_oracle(which, request):
+    torch = _torch()
+    ctx, orc = request.getfixturevalue(which)
+    av = request.getfixturevalue("small_avatar" if which == "small" else "head_avatar")
+    rng = np.random.default_rng(31)
+    n = 80000
+    nfr = len(av.frames)
+    frame = rng.integers(0, nfr, n)
+    pos = np.concatenate([_near_surface(av.frames[f], av.faces, n // nfr + 1, rng, 0.015) for f in range(nfr)])[:n]
+    frame = np.repeat(np.arange(nfr), n // nfr + 1)[:n]
+    d = rng.standard_normal((n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    c = _coords(pos, d, 0)
+    c[:, 4] = frame
+    c[::97, 4] = -1.0                    # skipped
+    c[::89, 0] = 1.25                    # outside the unit cube: skipped
+    g = torch.from_numpy(c).cuda()
+    ctx.deform_samples(g, -1)
+    ref = orc.deform(c, -1)
+    got = g.cpu().numpy()
+    skipped = (c[:, 4] < 0) | (c[:, 0] > 1)
+    assert np.array_equal(got[skipped], c[skipped])
+    assert_rel_close(got[:, 0:3], ref[:, 0:3], RTOL, "canonical position")
+    assert_rel_close(got[:, 8:11], ref[:, 8:11], RTOL, "canonical direction")
+    assert np.array_equal(got[:, [3, 4, 5, 6, 7]], ref[:, [3, 4, 5, 6, 7]])
+    # fixed-frame mode ignores the holder's frame id
+    g2 = torch.from_numpy(c).cuda()
+    ctx.deform_samples(g2, 0)
+    assert_rel_close(g2.cpu().numpy()[:, 0:3], orc.deform(c, 0)[:, 0:3], RTOL, "fixed frame")
+
+
+# ---------------------------------------------------------------- occupancy
+@pytest.mark.parametrize("which,thr", [("small", 0.05), ("head", 0.015)])
+def test_occupancy_bit_exact(which, thr, request):
+    ctx, orc = request.getfixturevalue(which)
+    for frame in (-1, 0):
+        bits = ctx.build_occupancy(frame, thr).cpu().numpy()
+        ref = orc.build_occupancy(frame, thr)
+        assert np.array_equal(bits, ref), frame
+        assert np.unpackbits(ref).sum() > 1000
+
+
+# ---------------------------------------------------------------- network
+@pytest.mark.parametrize("which", ["small", "head"])
+def test_encode_bit_exact(which, request):
+    torch = _torch()
+    ctx, orc = request.getfixturevalue(which)
+    rng = np.random.default_rng(41)
+    x = _points(rng, 100000, 0.02, 0.98)
+    got = ctx.encode(torch.from_numpy(x).cuda()).cpu().numpy()
+    ref = orc.encode(x)
+    assert np.array_equal(got, ref)
+
+
+def test_encode_backward_matches_oracle(small):
+    torch = _torch()
+    ctx, orc = small
+    rng = np.random.default_rng(43)
+    x = _points(rng, 20000, 0.02, 0.98)
+    dL = rng.standard_normal((20000, ctx.cfg.n_levels * ctx.cfg.n_features_per_level)).astype(np.float32)
+    grad = torch.zeros(ctx.layout.n_grid_params, dtype=torch.float32, device="cuda")
+    ctx.encode_backward(torch.from_numpy(x).cuda(), torch.from_numpy(dL).cuda(), grad)
+    ref = orc.encode_backward(x, dL)
+    got = grad.cpu().numpy().astype(np.float64)
+    touched = ref != 0
+    assert touched.sum() > 10000
+    scale = np.abs(ref[touched]).mean()
+    assert np.abs(got - ref).max() <= RTOL * (np.abs(ref).max() + scale)
+
+
+@pytest.mark.parametrize("which", ["small", "head"])
+def test_network_matches_oracle(which, request):
+    torch = _torch()
+    ctx, orc = request.getfixturevalue(which)
+    rng = np.random.default_rng(51)
+    n = 60000
+    pos = _points(rng, n, 0.05, 0.95)
+    d = rng.standard_normal((n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    c = _coords(pos, d, 0)
+    got = ctx.network_inference(torch.from_numpy(c).cuda()).cpu().numpy()
+    ref = orc.network(c)
+    assert_rel_close(got, ref, RTOL, "network outputs")
+
+
+# ---------------------------------------------------------------- render
+def _render_pair(ctx, orc, cam, w, h, frame, occ, pixels=None, spp=0):
+    rgba, depth, steps = ctx.render(cam, w, h, frame, occ, spp)
+    rgba, depth, steps = rgba.cpu().numpy(), depth.cpu().numpy(), steps.cpu().numpy()
+    r_rgba, r_depth, r_steps, stats = orc.render(cam, w, h, frame, occ, spp, pixels)
+    if pixels is not None:
+        rgba, depth, steps = rgba[pixels], depth[pixels], steps[pixels]
+    return (rgba, depth, steps.astype(np.uint32)), (r_rgba, r_depth, r_steps), stats
+
+
+def test_render_small_geometry_exact(small_avatar):
+    """No early termination: per-pixel step counts are pure march geometry."""
+    ctx, orc = _scene(small_config(min_transmittance=0.0), small_avatar)
+    ctx.build_occupancy(0, 0.05)
+    orc.build_occupancy(0, 0.05)
+    cam = make_camera(synth.look_at_camera(np.array([0.3, 0.2, 1.0]), 1.0), synth.focal_from_fov(64, 30.0))
+    (rgba, depth, steps), (r_rgba, r_depth, r_steps), stats = _render_pair(ctx, orc, cam, 64, 64, 0, 0)
+    assert np.array_equal(steps, r_steps)
+    assert steps.sum() == stats[0] and steps.sum() > 10000
+    assert_rel_close(rgba, r_rgba, RTOL, "rgba")
+    hit = r_depth < 1e9
+    assert np.array_equal(depth >= 1e9, ~hit)
+    assert_rel_close(depth[hit], r_depth[hit], RTOL, "depth")
+    st = ctx.render_stats()
+    assert st.n_samples == stats[0] and st.n_rays_alive0 == stats[1]
+
+
+def test_render_small_default(small):
+    ctx, orc = small
+    ctx.build_occupancy(0, 0.05)
+    orc.build_occupancy(0, 0.05)
+    for k, spp in enumerate((0, 3)):
+        cam = make_camera(synth.look_at_camera(np.array([0.1, -0.3, 1.0 - k]), 1.0), synth.focal_from_fov(64, 30.0),
+                          snap=(spp == 0))
+        (rgba, depth, steps), (r_rgba, r_depth, r_steps), _ = _render_pair(ctx, orc, cam, 64, 64, 0, 0, spp=spp)
+        same = steps == r_steps
+        # a ray whose accumulated alpha lands within rounding of the
+        # termination threshold may stop one sample apart
+        assert same.mean() >= 0.995
+        assert_rel_close(rgba[same], r_rgba[same], RTOL, "rgba")
+
+
+def test_render_full_size_subset(head, head_avatar):
+    """512x512 full-size render; a bounded subset of pixels is checked
+    against the oracle, every pixel against the properties below."""
+    ctx, orc = head
+    ctx.build_occupancy(0, 0.015)
+    orc.build_occupancy(0, 0.015)
+    cam = make_camera(synth.look_at_camera(np.array([0.2, 0.1, 1.0]), 0.7), synth.focal_from_fov(512, 30.0))
+    rng = np.random.default_rng(61)
+    pixels = rng.choice(512 * 512, 3000, replace=False).astype(np.uint32)
+    (rgba, depth, steps), (r_rgba, r_depth, r_steps), _ = _render_pair(ctx, orc, cam, 512, 512, 0, 0, pixels)
+    assert np.array_equal(steps, r_steps)
+    assert_rel_close(rgba, r_rgba, RTOL, "rgba")
+    hit = r_depth < 1e9
+    assert_rel_close(depth[hit], r_depth[hit], RTOL, "depth")
+    full, _, fsteps = ctx.render(cam, 512, 512, 0, 0)
+    full = full.cpu().numpy()
+    st = ctx.render_stats()
+    assert st.n_samples == int(fsteps.cpu().numpy().astype(np.int64).sum())
+    assert (full >= 0).all() and (full[:, 3] <= 1.0 + 1e-6).all()
+    assert (full[:, :3] <= full[:, 3:4] + 1e-6).all()
+
+
+def test_render_is_deterministic(head):
+    torch = _torch()
+    ctx, orc = head
+    ctx.build_occupancy(1, 0.015)
+    cam = make_camera(synth.look_at_camera(np.array([-0.3, 0.2, 1.0]), 0.7), synth.focal_from_fov(256, 30.0))
+    a = [t.clone() for t in ctx.render(cam, 256, 256, 1, 1)]
+    b = ctx.render(cam, 256, 256, 1, 1)
+    torch.cuda.synchronize()
+    for x, y in zip(a, b):
+        assert torch.equal(x, y)
