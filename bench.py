@@ -106,6 +106,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--res", type=int, default=512)
     ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds of CPU oracle timing (0: skip)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run one 256 MiB device copy first (reference kernel for rocprof byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (every box has one GPU; "
@@ -113,6 +115,12 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.calibrate_copy:
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     ctx, cfg, av, params, cams = setup(args)
     res = args.res
     out = (torch.empty((res * res, 4), dtype=torch.float32, device="cuda"),
