@@ -215,6 +215,8 @@ insta_status insta_create(const insta_config* cfg, insta_ctx** out) {
     alloc((void**)&ctx->d_grids, S * (G * G * G / 8));
     alloc((void**)&ctx->d_built, S);
     alloc((void**)&ctx->d_params, ctx->layout.n_params * sizeof(float));
+    alloc(&ctx->d_wfrag, 64 * 1024 * sizeof(float));
+    if (cfg->precision != INSTA_PRECISION_FP32) alloc(&ctx->d_table_h, ctx->layout.n_grid_params * 2);
     alloc((void**)&ctx->d_counters, 16 * sizeof(uint32_t));
     alloc(&ctx->d_scratch, ctx->scratch_bytes);
     if (ok && hipHostMalloc((void**)&ctx->h_counters, 16 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
@@ -237,7 +239,7 @@ void insta_destroy(insta_ctx* ctx) {
     dev_free(ctx->d_faces); dev_free(ctx->d_masks); dev_free(ctx->d_adj); dev_free(ctx->d_verts);
     dev_free(ctx->d_nodes); dev_free(ctx->d_btris); dev_free(ctx->d_proj); dev_free(ctx->d_proj_inv);
     dev_free(ctx->d_cen); dev_free(ctx->d_grids); dev_free(ctx->d_built); dev_free(ctx->d_params);
-    dev_free(ctx->d_counters); dev_free(ctx->d_scratch);
+    dev_free(ctx->d_counters); dev_free(ctx->d_scratch); dev_free(ctx->d_wfrag); dev_free(ctx->d_table_h);
     if (ctx->h_counters) (void)hipHostFree(ctx->h_counters);
     delete ctx;
 }
@@ -454,6 +456,8 @@ insta_status insta_set_params(insta_ctx* ctx, const float* params_dev, uint64_t 
     }
     INSTA_HIP(hipMemcpyAsync(ctx->d_params, params_dev, n_params * sizeof(float), hipMemcpyDeviceToDevice,
                              (hipStream_t)stream));
+    insta_status st = insta::prepare_network(ctx, (hipStream_t)stream);
+    if (st != INSTA_OK) return st;
     ctx->params_set = true;
     return INSTA_OK;
 }
@@ -489,13 +493,15 @@ insta_status insta_network_inference(insta_ctx* ctx, const float* coords_dev, ui
         set_error("network_inference: parameters not set");
         return INSTA_ERR_STATE;
     }
-    if (!insta::network_supported(ctx->cfg)) {
-        set_error("network_inference: configuration not supported by the fused MLP kernels");
-        return INSTA_ERR_UNSUPPORTED;
-    }
     if (stride_floats < 11 + ctx->cfg.n_extra_dims || (n && (!coords_dev || !out_dev))) {
         set_error("network_inference: bad stride or null pointer");
         return INSTA_ERR_INVALID;
+    }
+    if (insta::mfma_network_supported(ctx->cfg))
+        return insta::launch_network_mfma_aos(ctx, coords_dev, n, stride_floats, out_dev, (hipStream_t)stream);
+    if (!insta::network_supported(ctx->cfg) || ctx->cfg.precision != INSTA_PRECISION_FP32) {
+        set_error("network_inference: configuration not supported by the fused MLP kernels");
+        return INSTA_ERR_UNSUPPORTED;
     }
     return insta::launch_network_aos(ctx, coords_dev, n, stride_floats, out_dev, (hipStream_t)stream);
 }
@@ -512,7 +518,8 @@ insta_status insta_render(insta_ctx* ctx, const insta_camera* cam, uint32_t widt
         set_error("render: parameters not set");
         return INSTA_ERR_STATE;
     }
-    if (!insta::network_supported(ctx->cfg)) {
+    if (!insta::mfma_network_supported(ctx->cfg) &&
+        (!insta::network_supported(ctx->cfg) || ctx->cfg.precision != INSTA_PRECISION_FP32)) {
         set_error("render: configuration not supported by the fused MLP kernels");
         return INSTA_ERR_UNSUPPORTED;
     }

@@ -37,6 +37,8 @@ struct insta_ctx {
 
     // parameters
     float* d_params = nullptr;      // FP32 master copy [n_params]
+    void* d_wfrag = nullptr;        // MLP weights arranged as MFMA A fragments
+    void* d_table_h = nullptr;      // FP16 copy of the hash tables (reduced precision)
     bool params_set = false;
 
     // render workspace (grown on demand)
@@ -139,5 +141,10 @@ insta_status launch_network_aos(insta_ctx* ctx, const float* coords, uint32_t n,
                                 hipStream_t stream);
 insta_status launch_network_soa(insta_ctx* ctx, const uint32_t* n_dev, uint32_t n_max, hipStream_t stream);
 bool network_supported(const insta_config& cfg);
+bool mfma_network_supported(const insta_config& cfg);
+insta_status prepare_network(insta_ctx* ctx, hipStream_t stream);
+insta_status launch_network_mfma_soa(insta_ctx* ctx, const uint32_t* n_dev, uint32_t n_max, hipStream_t stream);
+insta_status launch_network_mfma_aos(insta_ctx* ctx, const float* coords, uint32_t n, uint32_t stride, float* out,
+                                     hipStream_t stream);
 
 }  // namespace insta

@@ -294,3 +294,55 @@ def test_render_is_deterministic(head):
     torch.cuda.synchronize()
     for x, y in zip(a, b):
         assert torch.equal(x, y)
+
+
+# ---------------------------------------------------------------- other shapes and precisions
+def test_network_fallback_shape_matches_oracle(small_avatar):
+    """Two density hidden layers: not an MFMA shape, served by the scalar kernel."""
+    torch = _torch()
+    ctx, orc = _scene(small_config(density_hidden_layers=2), small_avatar)
+    rng = np.random.default_rng(52)
+    pos = _points(rng, 20000, 0.05, 0.95)
+    d = rng.standard_normal((20000, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    c = _coords(pos, d, 0)
+    got = ctx.network_inference(torch.from_numpy(c).cuda()).cpu().numpy()
+    assert_rel_close(got, orc.network(c), RTOL, "network (scalar kernel)")
+
+
+def _psnr(a, b):
+    mse = float(np.mean((np.asarray(a, np.float64) - np.asarray(b, np.float64)) ** 2))
+    return float("inf") if mse == 0 else 10 * np.log10(1.0 / mse)
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 3e-3), (2, 2e-2)])
+def test_network_reduced_precision(head_avatar, prec, tol):
+    """FP16 tables + FP16/BF16 MFMA MLPs with FP32 accumulation against the
+    FP32 oracle: error bounded by the input rounding (2^-11 / 2^-8)."""
+    torch = _torch()
+    ctx, orc = _scene(full_config(max_frames=4, precision=prec), head_avatar)
+    rng = np.random.default_rng(53)
+    n = 40000
+    pos = _points(rng, n, 0.05, 0.95)
+    d = rng.standard_normal((n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    c = _coords(pos, d, 0)
+    got = ctx.network_inference(torch.from_numpy(c).cuda()).cpu().numpy().astype(np.float64)
+    ref = orc.network(c).astype(np.float64)
+    scale = np.abs(ref).mean(axis=0)
+    err = (np.abs(got - ref) / scale).max()
+    print(f"precision {prec}: max error / mean |ref| = {err:.2e}")
+    assert err < tol
+
+
+@pytest.mark.parametrize("prec", [1, 2])
+def test_render_reduced_precision_psnr(head_avatar, prec):
+    ctx, orc = _scene(full_config(max_frames=4, precision=prec), head_avatar)
+    ctx.build_occupancy(0, 0.015)
+    orc.build_occupancy(0, 0.015)
+    cam = make_camera(synth.look_at_camera(np.array([0.2, 0.1, 1.0]), 0.7), synth.focal_from_fov(128, 30.0))
+    (rgba, depth, steps), (r_rgba, r_depth, r_steps), _ = _render_pair(ctx, orc, cam, 128, 128, 0, 0)
+    assert np.array_equal(steps, r_steps)
+    p = _psnr(rgba[:, :3], r_rgba[:, :3])
+    print(f"precision {prec}: PSNR vs FP32 oracle {p:.1f} dB")
+    assert p >= 45.0
