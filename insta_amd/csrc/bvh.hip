@@ -2,13 +2,16 @@
 //
 // Replaces the reference's per-frame host work (src/tiny_mesh.cu:134-178:
 // build_triangles + TriangleBvh::build on the CPU, then an H2D copy) with one
-// workgroup per frame that builds everything in LDS:
+// workgroup per frame:
 //   1. centroids (float, (a+b+c)/3) and their AABB (workgroup reduction);
-//   2. keys = (30-bit Morton code of the quantised centroid) << 32 | face id;
-//   3. a bitonic sort of the keys in LDS (<= 16,384 keys = 128 KiB, gfx950's
-//      160 KiB LDS holds them in one workgroup);
-//   4. BVH-order triangle records, leaf boxes over 4^k count-balanced leaves,
-//      inner boxes bottom-up over the implicit 4-ary heap;
+//   2. per axis, keys (18-bit quantised centroid << 14 | face id) sorted by an
+//      LDS bitonic sort (<= 16,384 keys), giving three presorted id lists;
+//   3. for every binary depth of the implicit 4-ary heap, each segment picks
+//      its longest-extent axis and all three lists are stable-partitioned at
+//      the segment's count median (block-wide scans) -- a median-split tree
+//      like triangle_bvh.cu:548-631, but exactly reproducible;
+//   4. BVH-order triangle records, leaf boxes over the 4^k count-balanced
+//      leaves, inner boxes bottom-up;
 //   5. per-triangle frames tri2projection (include/rta/affine.cuh:26-39) and
 //      their inverses, built and inverted in FP64, stored FP32.
 // Several frames build concurrently, one per CU.
@@ -30,6 +33,8 @@ struct BuildArgs {
     float4* proj;
     float4* proj_inv;
     float4* cen;
+    uint16_t* lists;           // [slot][2 buffers][3 axes][tri_cap] build scratch
+    uint32_t* qcoord;          // [slot][3][tri_cap] quantised centroids
     uint32_t first_slot, nv, nf, k, leaf_base, node_cap, tri_cap, vert_cap;
 };
 
@@ -50,12 +55,19 @@ __device__ inline D3 dnormalized(D3 a) {
 
 __device__ inline V3 load_v(const float* V, int32_t i) { return v3(V[3 * i], V[3 * i + 1], V[3 * i + 2]); }
 
-__device__ inline uint32_t quantise(float c, float lo, float e) {
+__device__ inline uint32_t quantise18(float c, float lo, float e) {
     if (!(e > 0.0f)) return 0u;
-    float q = (c - lo) / e * 1024.0f;
+    float q = (c - lo) / e * 262144.0f;
     if (!(q > 0.0f)) return 0u;
     uint32_t u = (uint32_t)q;
-    return u > 1023u ? 1023u : u;
+    return u > 262143u ? 262143u : u;
+}
+
+__device__ inline uint32_t seg_bound(uint64_t j, uint32_t n, uint32_t s) { return (uint32_t)((j * n) >> s); }
+
+// first segment at depth s that contains position pos
+__device__ inline uint32_t seg_of(uint32_t pos, uint32_t n, uint32_t s) {
+    return (uint32_t)(((((uint64_t)pos + 1) << s) - 1) / n);
 }
 
 __device__ inline float wave_min(float v) {
@@ -68,7 +80,7 @@ __device__ inline float wave_max(float v) {
 }
 
 __global__ __launch_bounds__(kBuildThreads) void build_slot_kernel(BuildArgs a) {
-    __shared__ uint64_t keys[INSTA_MAX_TRIANGLES];
+    __shared__ uint32_t lds[INSTA_MAX_TRIANGLES];   // 64 KiB: sort keys, then split scratch
     __shared__ float red[6][kBuildThreads / 64];
     const uint32_t tid = threadIdx.x;
     const uint32_t slot = a.first_slot + blockIdx.x;
@@ -107,42 +119,129 @@ __global__ __launch_bounds__(kBuildThreads) void build_slot_kernel(BuildArgs a) 
     }
     const V3 ext = cmax - cmin;
 
-    // 2. Morton keys (padding keys sort last)
+    // 2. three presorted id lists (quantised centroid, id)
     uint32_t npad = 1;
     while (npad < nf) npad <<= 1;
-    for (uint32_t i = tid; i < npad; i += kBuildThreads) {
-        uint64_t key = ~0ull;
-        if (i < nf) {
-            const int32_t* f = a.faces + 3 * i;
-            V3 A = load_v(V, f[0]), B = load_v(V, f[1]), C = load_v(V, f[2]);
-            V3 c = v3((A.x + B.x + C.x) / 3.0f, (A.y + B.y + C.y) / 3.0f, (A.z + B.z + C.z) / 3.0f);
-            uint32_t m = morton3(quantise(c.x, cmin.x, ext.x), quantise(c.y, cmin.y, ext.y), quantise(c.z, cmin.z, ext.z));
-            key = ((uint64_t)m << 32) | i;
+    uint32_t* keys = reinterpret_cast<uint32_t*>(lds);
+    uint16_t* lists = a.lists + (size_t)slot * 6 * a.tri_cap;     // [buffer][axis][tri_cap]
+    uint32_t* qc = a.qcoord + (size_t)slot * 3 * a.tri_cap;
+    for (uint32_t ax = 0; ax < 3; ++ax) {
+        const float clo = ax == 0 ? cmin.x : (ax == 1 ? cmin.y : cmin.z);
+        const float cex = ax == 0 ? ext.x : (ax == 1 ? ext.y : ext.z);
+        for (uint32_t i = tid; i < npad; i += kBuildThreads) {
+            uint32_t key = 0xffffffffu;
+            if (i < nf) {
+                const float4 c = cen[i];
+                const uint32_t q = quantise18(ax == 0 ? c.x : (ax == 1 ? c.y : c.z), clo, cex);
+                qc[ax * a.tri_cap + i] = q;
+                key = (q << 14) | i;
+            }
+            keys[i] = key;
         }
-        keys[i] = key;
+        __syncthreads();
+        for (uint32_t size = 2; size <= npad; size <<= 1) {
+            for (uint32_t j = size >> 1; j > 0; j >>= 1) {
+                for (uint32_t i = tid; i < npad / 2; i += kBuildThreads) {
+                    const uint32_t x = ((i & ~(j - 1)) << 1) | (i & (j - 1));
+                    const uint32_t y = x + j;
+                    const bool up = (x & size) == 0;
+                    const uint32_t kx = keys[x], ky = keys[y];
+                    if ((kx > ky) == up) {
+                        keys[x] = ky;
+                        keys[y] = kx;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (uint32_t i = tid; i < nf; i += kBuildThreads) lists[ax * a.tri_cap + i] = (uint16_t)(keys[i] & 0x3fffu);
+        __syncthreads();
     }
-    __syncthreads();
 
-    // 3. bitonic sort, ascending
-    for (uint32_t size = 2; size <= npad; size <<= 1) {
-        for (uint32_t j = size >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = tid; i < npad / 2; i += kBuildThreads) {
-                const uint32_t x = ((i & ~(j - 1)) << 1) | (i & (j - 1));
-                const uint32_t y = x + j;
-                const bool up = (x & size) == 0;
-                const uint64_t kx = keys[x], ky = keys[y];
-                if ((kx > ky) == up) {
-                    keys[x] = ky;
-                    keys[y] = kx;
+    // 3. median splits, one binary depth at a time.  LDS reuse: exclusive
+    // left-counts (u16), right flags per face (u8), chosen axis per segment.
+    uint16_t* excl = reinterpret_cast<uint16_t*>(lds);
+    uint8_t* right = reinterpret_cast<uint8_t*>(lds) + 2 * INSTA_MAX_TRIANGLES;
+    uint8_t* seg_axis = reinterpret_cast<uint8_t*>(lds) + 3 * INSTA_MAX_TRIANGLES;
+    constexpr uint32_t kPer = INSTA_MAX_TRIANGLES / kBuildThreads;  // positions per thread
+    const uint32_t p0 = tid * kPer;
+    uint32_t cur = 0;
+    for (uint32_t s = 0; s < 2 * a.k; ++s) {
+        const uint16_t* L = lists + (size_t)cur * 3 * a.tri_cap;
+        uint16_t* Ln = lists + (size_t)(cur ^ 1u) * 3 * a.tri_cap;
+        const uint32_t n_seg = 1u << s;
+        for (uint32_t j = tid; j < n_seg; j += kBuildThreads) {
+            const uint32_t b = seg_bound(j, nf, s), e = seg_bound(j + 1, nf, s);
+            uint32_t best_ax = 0, best_ext = 0;
+            for (uint32_t ax = 0; ax < 3; ++ax) {
+                const uint16_t* La = L + ax * a.tri_cap;
+                const uint32_t ea = qc[ax * a.tri_cap + La[e - 1]] - qc[ax * a.tri_cap + La[b]];
+                if (ax == 0 || ea > best_ext) {
+                    best_ext = ea;
+                    best_ax = ax;
+                }
+            }
+            seg_axis[j] = (uint8_t)best_ax;
+        }
+        __syncthreads();
+        if (p0 < nf) {
+            uint32_t j = seg_of(p0, nf, s);
+            uint32_t e = seg_bound(j + 1, nf, s);
+            for (uint32_t pos = p0; pos < p0 + kPer && pos < nf; ++pos) {
+                while (pos >= e) {
+                    ++j;
+                    e = seg_bound(j + 1, nf, s);
+                }
+                const uint32_t mid = seg_bound(2 * j + 1, nf, s + 1);
+                const uint32_t id = L[seg_axis[j] * a.tri_cap + pos];
+                right[id] = pos >= mid ? 1 : 0;
+            }
+        }
+        __syncthreads();
+        for (uint32_t ax = 0; ax < 3; ++ax) {
+            const uint16_t* La = L + ax * a.tri_cap;
+            uint16_t* Lna = Ln + ax * a.tri_cap;
+            // block-wide exclusive scan of left flags in this list's order
+            uint32_t cnt = 0;
+            for (uint32_t pos = p0; pos < p0 + kPer && pos < nf; ++pos) cnt += right[La[pos]] ? 0u : 1u;
+            uint32_t incl = cnt;
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t up = __shfl_up(incl, o);
+                if (lane >= (uint32_t)o) incl += up;
+            }
+            if (lane == 63) red[0][wave] = __uint_as_float(incl);
+            __syncthreads();
+            uint32_t run = incl - cnt;
+            for (uint32_t w = 0; w < wave; ++w) run += __float_as_uint(red[0][w]);
+            for (uint32_t pos = p0; pos < p0 + kPer && pos < nf; ++pos) {
+                excl[pos] = (uint16_t)run;
+                run += right[La[pos]] ? 0u : 1u;
+            }
+            __syncthreads();
+            if (p0 < nf) {
+                uint32_t j = seg_of(p0, nf, s);
+                uint32_t e = seg_bound(j + 1, nf, s);
+                for (uint32_t pos = p0; pos < p0 + kPer && pos < nf; ++pos) {
+                    while (pos >= e) {
+                        ++j;
+                        e = seg_bound(j + 1, nf, s);
+                    }
+                    const uint32_t b = seg_bound(j, nf, s), mid = seg_bound(2 * j + 1, nf, s + 1);
+                    const uint32_t id = La[pos];
+                    const uint32_t left_before = (uint32_t)excl[pos] - (uint32_t)excl[b];
+                    const uint32_t np = right[id] ? mid + (pos - b) - left_before : b + left_before;
+                    Lna[np] = (uint16_t)id;
                 }
             }
             __syncthreads();
         }
+        cur ^= 1u;
     }
+    const uint16_t* order = lists + (size_t)cur * 3 * a.tri_cap;  // axis-0 list: BVH order
 
     // 4. BVH-order triangle records
     for (uint32_t j = tid; j < nf; j += kBuildThreads) {
-        const uint32_t o = (uint32_t)(keys[j] & 0xffffffffu);
+        const uint32_t o = order[j];
         const int32_t* f = a.faces + 3 * o;
         V3 A = load_v(V, f[0]), B = load_v(V, f[1]), C = load_v(V, f[2]);
         btris[3 * j + 0] = make_float4(A.x, A.y, A.z, B.x);
@@ -156,7 +255,7 @@ __global__ __launch_bounds__(kBuildThreads) void build_slot_kernel(BuildArgs a) 
         const uint32_t e = (uint32_t)(((uint64_t)(j + 1) * nf) >> (2 * a.k));
         V3 blo = v3(INFINITY, INFINITY, INFINITY), bhi = v3(-INFINITY, -INFINITY, -INFINITY);
         for (uint32_t t = b; t < e; ++t) {
-            const int32_t* f = a.faces + 3 * (uint32_t)(keys[t] & 0xffffffffu);
+            const int32_t* f = a.faces + 3 * (uint32_t)order[t];
             V3 A = load_v(V, f[0]), B = load_v(V, f[1]), C = load_v(V, f[2]);
             blo = vmin(blo, vmin(A, vmin(B, C)));
             bhi = vmax(bhi, vmax(A, vmax(B, C)));
@@ -253,13 +352,16 @@ __global__ __launch_bounds__(kQueryThreads) void occupancy_kernel(MeshView m, ui
     const uint32_t byte = blockIdx.x * blockDim.x + threadIdx.x;
     if (byte >= n_bytes) return;
     const float g = (float)G;
+    // cells farther than the threshold are never occupied: bound the query
+    // a little above threshold^2 so sqrt(bound) >= threshold after rounding
+    const float bound = threshold * threshold * (1.0f + 0x1p-10f);
     uint32_t v = 0;
     for (uint32_t j = 0; j < 8; ++j) {
         const uint32_t idx = byte * 8 + j;
         const uint32_t x = compact3(idx), y = compact3(idx >> 1), z = compact3(idx >> 2);
         V3 pos = v3((((float)x + 0.5f) / g - 0.5f) * 1.0f + 0.5f, (((float)y + 0.5f) / g - 0.5f) * 1.0f + 0.5f,
                     (((float)z + 0.5f) / g - 0.5f) * 1.0f + 0.5f);
-        Hit h = bvh_closest(m, pos, stack + threadIdx.x, kQueryThreads);
+        Hit h = bvh_closest(m, pos, stack + threadIdx.x, kQueryThreads, -1, bound);
         if (h.idx < 0) continue;
         const float dist = sqrtf(h.dist_sq);
         const uint32_t mask = __float_as_uint(m.btris[3 * h.idx + 2].z);
@@ -283,6 +385,8 @@ insta_status launch_build_slots(insta_ctx* ctx, uint32_t first_slot, uint32_t n_
     a.proj = ctx->d_proj;
     a.proj_inv = ctx->d_proj_inv;
     a.cen = ctx->d_cen;
+    a.lists = ctx->d_build_lists;
+    a.qcoord = ctx->d_build_q;
     a.first_slot = first_slot;
     a.nv = ctx->nv;
     a.nf = ctx->nf;

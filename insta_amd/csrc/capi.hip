@@ -214,6 +214,8 @@ insta_status insta_create(const insta_config* cfg, insta_ctx** out) {
     alloc((void**)&ctx->d_cen, S * T * sizeof(float4));
     alloc((void**)&ctx->d_grids, S * (G * G * G / 8));
     alloc((void**)&ctx->d_built, S);
+    alloc((void**)&ctx->d_build_lists, S * 6 * T * sizeof(uint16_t));
+    alloc((void**)&ctx->d_build_q, S * 3 * T * sizeof(uint32_t));
     alloc((void**)&ctx->d_params, ctx->layout.n_params * sizeof(float));
     alloc(&ctx->d_wfrag, 64 * 1024 * sizeof(float));
     if (cfg->precision != INSTA_PRECISION_FP32) alloc(&ctx->d_table_h, ctx->layout.n_grid_params * 2);
@@ -239,6 +241,7 @@ void insta_destroy(insta_ctx* ctx) {
     dev_free(ctx->d_faces); dev_free(ctx->d_masks); dev_free(ctx->d_adj); dev_free(ctx->d_verts);
     dev_free(ctx->d_nodes); dev_free(ctx->d_btris); dev_free(ctx->d_proj); dev_free(ctx->d_proj_inv);
     dev_free(ctx->d_cen); dev_free(ctx->d_grids); dev_free(ctx->d_built); dev_free(ctx->d_params);
+    dev_free(ctx->d_build_lists); dev_free(ctx->d_build_q);
     dev_free(ctx->d_counters); dev_free(ctx->d_scratch); dev_free(ctx->d_wfrag); dev_free(ctx->d_table_h);
     if (ctx->h_counters) (void)hipHostFree(ctx->h_counters);
     delete ctx;

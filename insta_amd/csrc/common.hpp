@@ -108,13 +108,26 @@ struct Hit {
 // the best so far (so the result equals the brute-force argmin),
 // leaf triangles ranked by (dist_sq, original id).  `stack` points at this
 // thread's column of a [depth][blockDim] uint16 LDS array.
-INSTA_DEV Hit bvh_closest(const MeshView& m, V3 p, uint16_t* stack, uint32_t stride) {
+INSTA_DEV Hit bvh_closest(const MeshView& m, V3 p, uint16_t* stack, uint32_t stride, int32_t hint = -1,
+                          float max_dist_sq = 3.40282347e38f) {
     int sp = 0;
     stack[0] = 0;
     sp = 1;
-    float best = 3.40282347e38f;
+    // max_dist_sq (bvh.cuh:48): only triangles closer than this are reported
+    float best = max_dist_sq;
     int32_t best_idx = -1;
     uint32_t best_orig = 0xffffffffu;
+    if (hint >= 0) {
+        // a known triangle (e.g. the previous sample's nearest) bounds the
+        // search from the start; it competes under the same (dist, id) order,
+        // so the result is unchanged, only fewer boxes survive the prune
+        const float4 r0 = m.btris[3 * hint + 0];
+        const float4 r1 = m.btris[3 * hint + 1];
+        const float4 r2 = m.btris[3 * hint + 2];
+        best = tri_distance_sq(v3(r0.x, r0.y, r0.z), v3(r0.w, r1.x, r1.y), v3(r1.z, r1.w, r2.x), p);
+        best_idx = hint;
+        best_orig = __float_as_uint(r2.y);
+    }
     const uint32_t shift = 2u * m.k;
     while (sp > 0) {
         --sp;

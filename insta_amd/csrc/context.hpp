@@ -33,6 +33,8 @@ struct insta_ctx {
     float4* d_cen = nullptr;        // [n_slots][max_triangles]
     uint8_t* d_grids = nullptr;     // [n_slots][G^3/8]
     uint8_t* d_built = nullptr;     // [n_slots] 1 once a slot's mesh is built
+    uint16_t* d_build_lists = nullptr;  // [n_slots][2][3][max_triangles] BVH build scratch
+    uint32_t* d_build_q = nullptr;      // [n_slots][3][max_triangles]
     float render_lo[3] = {0, 0, 0}, render_hi[3] = {1, 1, 1};
 
     // parameters

@@ -130,10 +130,11 @@ INSTA_DEV V3 apply_dir(const float m[12], V3 d) {
 
 // Body of `warp` for one sample; false leaves the sample untouched.
 INSTA_DEV bool warp_one(const MeshSet& ms, uint32_t frame, V3 p, V3 dir_w, V3& out_p, V3& out_d, uint32_t& mask,
-                        uint16_t* stack, uint32_t stride) {
+                        uint16_t* stack, uint32_t stride, int32_t* hint = nullptr) {
     if (!(p.x >= 0.0f && p.x <= 1.0f && p.y >= 0.0f && p.y <= 1.0f && p.z >= 0.0f && p.z <= 1.0f)) return false;
     const MeshView fr = ms.view(frame + 1);
-    const Hit h = bvh_closest(fr, p, stack, stride);
+    const Hit h = bvh_closest(fr, p, stack, stride, hint ? *hint : -1);
+    if (hint) *hint = h.idx;
     if (h.idx < 0) return false;
     const uint32_t id = h.orig;
     mask = __float_as_uint(fr.btris[3 * h.idx + 2].z);
@@ -183,6 +184,11 @@ __global__ __launch_bounds__(kDeformThreads) void deform_aos_kernel(MeshSet ms, 
 
 // Render samples (SoA): posed position + ray id in s_pos, ray direction from
 // r_d; canonical position / warped direction to s_cpos / s_cdir.
+// Each thread deforms kChunk consecutive samples (consecutive samples of a
+// ray lie dt apart), seeding every query with the previous one's nearest
+// triangle.
+constexpr uint32_t kChunk = 4;
+
 __global__ __launch_bounds__(kDeformThreads) void deform_soa_kernel(MeshSet ms, int32_t frame,
                                                                     const uint32_t* __restrict__ n_dev,
                                                                     const float4* __restrict__ s_pos,
@@ -190,17 +196,23 @@ __global__ __launch_bounds__(kDeformThreads) void deform_soa_kernel(MeshSet ms, 
                                                                     float4* __restrict__ s_cpos,
                                                                     float4* __restrict__ s_cdir) {
     __shared__ uint16_t stack[kStackDepth * kDeformThreads];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *n_dev) return;
-    const float4 sp = s_pos[i];
-    const float4 rd = r_d[__float_as_uint(sp.w)];
-    const V3 p = v3(sp.x, sp.y, sp.z);
-    const V3 dir_w = v3((rd.x + 1.0f) * 0.5f, (rd.y + 1.0f) * 0.5f, (rd.z + 1.0f) * 0.5f);
-    V3 op = p, od = dir_w;
-    uint32_t mask = 0;
-    if (frame >= 0) warp_one(ms, (uint32_t)frame, p, dir_w, op, od, mask, stack + threadIdx.x, kDeformThreads);
-    s_cpos[i] = make_float4(op.x, op.y, op.z, __uint_as_float(mask));
-    s_cdir[i] = make_float4(od.x, od.y, od.z, 0.0f);
+    const uint32_t n = *n_dev;
+    const uint32_t first = (blockIdx.x * blockDim.x + threadIdx.x) * kChunk;
+    int32_t hint = -1;
+    for (uint32_t j = 0; j < kChunk; ++j) {
+        const uint32_t i = first + j;
+        if (i >= n) return;
+        const float4 sp = s_pos[i];
+        const float4 rd = r_d[__float_as_uint(sp.w)];
+        const V3 p = v3(sp.x, sp.y, sp.z);
+        const V3 dir_w = v3((rd.x + 1.0f) * 0.5f, (rd.y + 1.0f) * 0.5f, (rd.z + 1.0f) * 0.5f);
+        V3 op = p, od = dir_w;
+        uint32_t mask = 0;
+        if (frame >= 0)
+            warp_one(ms, (uint32_t)frame, p, dir_w, op, od, mask, stack + threadIdx.x, kDeformThreads, &hint);
+        s_cpos[i] = make_float4(op.x, op.y, op.z, __uint_as_float(mask));
+        s_cdir[i] = make_float4(od.x, od.y, od.z, 0.0f);
+    }
 }
 
 }  // namespace
@@ -217,7 +229,8 @@ insta_status launch_deform_aos(insta_ctx* ctx, float* coords, uint32_t n, uint32
 insta_status launch_deform_soa(insta_ctx* ctx, int32_t frame, const uint32_t* n_dev, uint32_t n_max,
                                hipStream_t stream) {
     if (n_max == 0) return INSTA_OK;
-    hipLaunchKernelGGL(deform_soa_kernel, dim3((n_max + kDeformThreads - 1) / kDeformThreads), dim3(kDeformThreads),
+    const uint32_t threads = (n_max + kChunk - 1) / kChunk;
+    hipLaunchKernelGGL(deform_soa_kernel, dim3((threads + kDeformThreads - 1) / kDeformThreads), dim3(kDeformThreads),
                        0, stream, mesh_set(ctx, ctx->d_built), frame, n_dev, ctx->s_pos, ctx->r_d, ctx->s_cpos,
                        ctx->s_cdir);
     INSTA_HIP(hipGetLastError());

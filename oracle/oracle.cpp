@@ -137,12 +137,17 @@ inline uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) {
 }
 
 // ---------------------------------------------------------------------------
-// BVH.  Declared layout (deviation from the reference's nth_element median
-// split, triangle_bvh.cu:548-631, whose element order is implementation
-// defined): triangles sorted by (30-bit Morton code of the centroid in the
-// centroid AABB, original face id); 4^k leaves for the largest k with
-// 4^k <= n, leaf j holding sorted triangles [j*n >> 2k, (j+1)*n >> 2k); an
-// implicit complete 4-ary heap above them (children of node i are 4i+1..4i+4).
+// BVH.  Declared layout: an implicit complete 4-ary heap (children of node i
+// are 4i+1..4i+4) over 4^k count-balanced leaves, k the largest with
+// 4^k <= n; at binary depth s the triangles are split into segments
+// [floor(j n / 2^s), floor((j+1) n / 2^s)).  Like the reference's build
+// (instant-ngp/src/triangle_bvh.cu:548-631) every segment is split at its
+// count median along one axis, but the axis is the longest extent of the
+// segment's quantised centroids instead of the largest variance, and ties are
+// broken by face id, so the layout is an exact function of the input (the
+// reference's std::nth_element order is implementation-defined).  Centroids
+// are quantised to 18 bits over the centroid AABB; the three per-axis orders
+// (q, id) are sorted once and stable-partitioned at every depth.
 // Traversal (bvh.cuh:48-96): children sorted far-to-near by box distance with
 // a 4-input sorting network, pushed when their distance is <= the best so far
 // plus a margin of a few ulp (so the result is the brute-force argmin);
@@ -164,13 +169,19 @@ uint32_t bvh_levels(uint32_t n) {
     return k;
 }
 
-void bvh_build(Bvh& b, const std::vector<Tri>& tris) {
+inline uint32_t quantise18(float c, float lo, float e) {
+    if (!(e > 0.0f)) return 0u;
+    float q = (c - lo) / e * 262144.0f;
+    if (!(q > 0.0f)) return 0u;
+    uint32_t u = (uint32_t)q;
+    return u > 262143u ? 262143u : u;
+}
+
+// segment boundaries at binary depth s
+inline uint32_t seg_bound(uint64_t j, uint32_t n, uint32_t s) { return (uint32_t)((j * n) >> s); }
+
+void bvh_order(const std::vector<Tri>& tris, std::vector<uint32_t>& order, uint32_t k) {
     const uint32_t n = (uint32_t)tris.size();
-    b.n = n;
-    b.k = bvh_levels(n);
-    const uint32_t n_leaves = 1u << (2 * b.k);
-    b.leaf_base = (n_leaves - 1) / 3;
-    b.n_nodes = b.leaf_base + n_leaves;
     std::vector<V3> cen(n);
     V3 cmin = v3(INFINITY, INFINITY, INFINITY), cmax = v3(-INFINITY, -INFINITY, -INFINITY);
     for (uint32_t i = 0; i < n; ++i) {
@@ -178,27 +189,65 @@ void bvh_build(Bvh& b, const std::vector<Tri>& tris) {
         cmin = vmin(cmin, cen[i]);
         cmax = vmax(cmax, cen[i]);
     }
-    V3 ext = cmax - cmin;
-    auto quant = [](float c, float lo, float e) -> uint32_t {
-        if (!(e > 0.0f)) return 0u;
-        float q = (c - lo) / e * 1024.0f;
-        if (!(q > 0.0f)) return 0u;
-        uint32_t u = (uint32_t)q;
-        return u > 1023u ? 1023u : u;
-    };
-    std::vector<uint64_t> keys(n);
-    for (uint32_t i = 0; i < n; ++i) {
-        uint32_t m = morton3(quant(cen[i].x, cmin.x, ext.x), quant(cen[i].y, cmin.y, ext.y),
-                             quant(cen[i].z, cmin.z, ext.z));
-        keys[i] = (uint64_t(m) << 32) | i;
+    const V3 ext = cmax - cmin;
+    std::vector<uint32_t> q[3];
+    std::vector<uint32_t> list[3];
+    for (int a = 0; a < 3; ++a) {
+        q[a].resize(n);
+        std::vector<uint32_t> keys(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            const float c = a == 0 ? cen[i].x : (a == 1 ? cen[i].y : cen[i].z);
+            const float lo = a == 0 ? cmin.x : (a == 1 ? cmin.y : cmin.z);
+            const float e = a == 0 ? ext.x : (a == 1 ? ext.y : ext.z);
+            q[a][i] = quantise18(c, lo, e);
+            keys[i] = (q[a][i] << 14) | i;
+        }
+        std::sort(keys.begin(), keys.end());
+        list[a].resize(n);
+        for (uint32_t i = 0; i < n; ++i) list[a][i] = keys[i] & 0x3fffu;
     }
-    std::sort(keys.begin(), keys.end());
+    std::vector<uint8_t> right(n);
+    std::vector<uint32_t> tmp(n);
+    for (uint32_t s = 0; s < 2 * k; ++s) {
+        const uint32_t n_seg = 1u << s;
+        for (uint32_t j = 0; j < n_seg; ++j) {
+            const uint32_t b = seg_bound(j, n, s), e = seg_bound(j + 1, n, s);
+            const uint32_t mid = seg_bound(2 * j + 1, n, s + 1);
+            uint32_t best_axis = 0, best_ext = 0;
+            for (uint32_t a = 0; a < 3; ++a) {
+                const uint32_t ea = q[a][list[a][e - 1]] - q[a][list[a][b]];
+                if (a == 0 || ea > best_ext) {
+                    best_ext = ea;
+                    best_axis = a;
+                }
+            }
+            for (uint32_t pos = b; pos < e; ++pos) right[list[best_axis][pos]] = pos >= mid ? 1 : 0;
+        }
+        for (int a = 0; a < 3; ++a) {
+            for (uint32_t j = 0; j < n_seg; ++j) {
+                const uint32_t b = seg_bound(j, n, s), e = seg_bound(j + 1, n, s);
+                uint32_t l = b, r = seg_bound(2 * j + 1, n, s + 1);
+                for (uint32_t pos = b; pos < e; ++pos) {
+                    const uint32_t id = list[a][pos];
+                    tmp[right[id] ? r++ : l++] = id;
+                }
+            }
+            list[a].swap(tmp);
+        }
+    }
+    order = list[0];
+}
+
+void bvh_build(Bvh& b, const std::vector<Tri>& tris) {
+    const uint32_t n = (uint32_t)tris.size();
+    b.n = n;
+    b.k = bvh_levels(n);
+    const uint32_t n_leaves = 1u << (2 * b.k);
+    b.leaf_base = (n_leaves - 1) / 3;
+    b.n_nodes = b.leaf_base + n_leaves;
+    bvh_order(tris, b.orig, b.k);
     b.tris.resize(n);
-    b.orig.resize(n);
-    for (uint32_t i = 0; i < n; ++i) {
-        b.orig[i] = (uint32_t)(keys[i] & 0xffffffffu);
-        b.tris[i] = tris[b.orig[i]];
-    }
+    for (uint32_t i = 0; i < n; ++i) b.tris[i] = tris[b.orig[i]];
     b.lo.assign(b.n_nodes, v3(0, 0, 0));
     b.hi.assign(b.n_nodes, v3(0, 0, 0));
     b.begin.assign(b.n_nodes, -1);
@@ -358,6 +407,7 @@ struct MeshFrame {
     std::vector<V3> cen;          // original order, float centroids
     std::vector<Frame12> proj;    // tri2projection
     std::vector<Frame12> proj_inv;
+    std::vector<double> proj_d;   // [nf*12] unrounded FP64 frames (FP64 oracle)
     Bvh bvh;
 };
 
@@ -517,6 +567,7 @@ void fill_frame(orc_scene* s, MeshFrame& mf, const float* verts, const MeshFrame
     mf.cen.resize(nf);
     mf.proj.resize(nf);
     mf.proj_inv.resize(nf);
+    mf.proj_d.resize(12 * (size_t)nf);
     for (uint32_t i = 0; i < nf; ++i) {
         const int32_t* f = &s->faces[3 * i];
         Tri t;
@@ -540,6 +591,7 @@ void fill_frame(orc_scene* s, MeshFrame& mf, const float* verts, const MeshFrame
         for (int k = 0; k < 12; ++k) {
             mf.proj[i].m[k] = (float)P[k];
             mf.proj_inv[i].m[k] = (float)Pi[k];
+            mf.proj_d[12 * (size_t)i + k] = P[k];
         }
     }
     bvh_build(mf.bvh, mf.tris);
@@ -738,6 +790,197 @@ inline float box_ray_tmin(V3 lo, V3 hi, V3 o, V3 d) {
     if (tmin > tzmax || tzmin > tmax) return FLT_MAX;
     if (tzmin > tmin) tmin = tzmin;
     return tmin;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 restatement of the per-sample math (deform, encode, MLPs, composite),
+// used as the accuracy reference for the FP32/FP16/BF16 renders.  The march
+// itself (sample positions) stays FP32 so all paths see the same samples.
+// ---------------------------------------------------------------------------
+struct D3v {
+    double x, y, z;
+};
+inline D3v dv(V3 a) { return {a.x, a.y, a.z}; }
+inline D3v operator-(D3v a, D3v b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline D3v operator*(D3v a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+inline double ddot(D3v a, D3v b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline D3v dcross(D3v a, D3v b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+
+double tri_distance_sq_d(const Tri& t, D3v pos) {
+    D3v a = dv(t.a), b = dv(t.b), c = dv(t.c);
+    D3v v21 = b - a, p1 = pos - a, v32 = c - b, p2 = pos - b, v13 = a - c, p3 = pos - c;
+    D3v nor = dcross(v21, v13);
+    double sg = std::copysign(1.0, ddot(dcross(v21, nor), p1)) + std::copysign(1.0, ddot(dcross(v32, nor), p2)) +
+                std::copysign(1.0, ddot(dcross(v13, nor), p3));
+    if (sg < 2.0) {
+        auto edge = [](D3v v, D3v p) {
+            double t = std::fmin(std::fmax(ddot(v, p) / ddot(v, v), 0.0), 1.0);
+            D3v d = v * t - p;
+            return ddot(d, d);
+        };
+        return std::fmin(std::fmin(edge(v21, p1), edge(v32, p2)), edge(v13, p3));
+    }
+    double d = ddot(nor, p1);
+    return d * d / ddot(nor, nor);
+}
+
+// nearest triangle in FP64 over the same BVH (boxes are exact float bounds)
+int32_t bvh_closest_d(const Bvh& b, D3v p) {
+    int32_t stack[64];
+    int sp = 0;
+    stack[sp++] = 0;
+    double best = 1e300;
+    int32_t bi = -1;
+    uint32_t bo = 0xffffffffu;
+    while (sp > 0) {
+        int32_t node = stack[--sp];
+        if ((uint32_t)node >= b.leaf_base) {
+            for (int32_t t = b.begin[node]; t < b.end[node]; ++t) {
+                double d = tri_distance_sq_d(b.tris[t], p);
+                if (d < best || (d == best && b.orig[t] < bo)) {
+                    best = d;
+                    bi = t;
+                    bo = b.orig[t];
+                }
+            }
+        } else {
+            for (int c = 0; c < 4; ++c) {
+                const int32_t ch = 4 * node + 1 + c;
+                double dx = std::fmax(std::fmax(b.lo[ch].x - p.x, p.x - b.hi[ch].x), 0.0);
+                double dy = std::fmax(std::fmax(b.lo[ch].y - p.y, p.y - b.hi[ch].y), 0.0);
+                double dz = std::fmax(std::fmax(b.lo[ch].z - p.z, p.z - b.hi[ch].z), 0.0);
+                if (dx * dx + dy * dy + dz * dz <= best * (1.0 + 1e-12)) stack[sp++] = ch;
+            }
+        }
+    }
+    return bi;
+}
+
+void blend_frames_d(const orc_scene* s, const MeshFrame& mf, uint32_t id, D3v p, double out[12]) {
+    const int32_t* adj = &s->adjacency[3 * id];
+    auto cen = [&](uint32_t t) { return dv(mf.cen[t]); };
+    auto nrm = [](D3v a) { return std::sqrt(ddot(a, a)); };
+    double sum = 0.0;
+    for (int k = 0; k < 3; ++k)
+        if (adj[k] >= 0) sum += std::exp(-4.0 * nrm(cen(adj[k]) - p));
+    sum += std::exp(-1.0 * nrm(cen(id) - p));
+    for (int i = 0; i < 12; ++i) out[i] = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        if (adj[k] >= 0) {
+            double w = std::exp(-4.0 * nrm(cen(adj[k]) - p)) / sum;
+            for (int i = 0; i < 12; ++i) out[i] += w * mf.proj_d[12 * (size_t)adj[k] + i];
+        }
+    }
+    double w = std::exp(-1.0 * nrm(cen(id) - p)) / sum;
+    for (int i = 0; i < 12; ++i) out[i] += w * mf.proj_d[12 * (size_t)id + i];
+}
+
+inline D3v apply_point_d(const double m[12], D3v p) {
+    return {m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3], m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7],
+            m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11]};
+}
+inline D3v apply_dir_d(const double m[12], D3v d) {
+    return {m[0] * d.x + m[1] * d.y + m[2] * d.z, m[4] * d.x + m[5] * d.y + m[6] * d.z,
+            m[8] * d.x + m[9] * d.y + m[10] * d.z};
+}
+
+bool warp_one_d(const orc_scene* s, const MeshFrame& fr, D3v p, D3v dir, D3v& op, D3v& od, uint32_t& mask) {
+    if (!(p.x >= 0.0 && p.x <= 1.0 && p.y >= 0.0 && p.y <= 1.0 && p.z >= 0.0 && p.z <= 1.0)) return false;
+    int32_t idx = bvh_closest_d(fr.bvh, p);
+    if (idx < 0) return false;
+    const uint32_t id = fr.bvh.orig[idx];
+    mask = s->masks[id];
+    double W[12], C[12];
+    if (mask == 2u && !s->cfg.disable_interp) {
+        double Wb[12];
+        blend_frames_d(s, fr, id, p, Wb);
+        invert_affine_d(Wb, W);
+        blend_frames_d(s, s->canon, id, p, C);
+    } else {
+        invert_affine_d(&fr.proj_d[12 * (size_t)id], W);
+        std::memcpy(C, &s->canon.proj_d[12 * (size_t)id], sizeof(C));
+    }
+    op = apply_point_d(C, apply_point_d(W, p));
+    D3v dc = apply_dir_d(C, apply_dir_d(W, dir));
+    od = dc * (1.0 / std::sqrt(ddot(dc, dc)));
+    return true;
+}
+
+void encode_one_d(const orc_scene* s, D3v x, double* out) {
+    const insta_layout& L = s->layout;
+    const uint32_t F = s->cfg.n_features_per_level;
+    const float* table = s->params.data() + L.n_mlp_params;
+    const double xs[3] = {x.x, x.y, x.z};
+    for (uint32_t l = 0; l < s->cfg.n_levels; ++l) {
+        double frac[3];
+        uint32_t g[3];
+        for (int d = 0; d < 3; ++d) {
+            double pos = xs[d] * (double)L.level_scale[l] + 0.5;
+            double fl = std::floor(pos);
+            frac[d] = pos - fl;
+            g[d] = (uint32_t)(int32_t)fl;
+        }
+        for (uint32_t f = 0; f < F; ++f) out[l * F + f] = 0.0;
+        for (uint32_t c = 0; c < 8; ++c) {
+            double w = 1.0;
+            for (int d = 0; d < 3; ++d) w *= ((c >> d) & 1u) ? frac[d] : 1.0 - frac[d];
+            uint32_t idx = grid_index(L, l, g[0] + (c & 1u), g[1] + ((c >> 1) & 1u), g[2] + ((c >> 2) & 1u));
+            const float* e = table + (uint64_t(L.level_offset[l]) + idx) * F;
+            for (uint32_t f = 0; f < F; ++f) out[l * F + f] += w * (double)e[f];
+        }
+    }
+}
+
+void dense_d(const float* W, const double* x, uint32_t n_in, uint32_t n_out, bool relu, double* y) {
+    for (uint32_t o = 0; o < n_out; ++o) {
+        double acc = 0.0;
+        for (uint32_t i = 0; i < n_in; ++i) acc += (double)W[(uint64_t)o * n_in + i] * x[i];
+        y[o] = relu ? (acc > 0.0 ? acc : 0.0) : acc;
+    }
+}
+
+void network_one_d(const orc_scene* s, D3v pos, D3v dir, double* out4) {
+    const insta_config& c = s->cfg;
+    const insta_layout& L = s->layout;
+    const uint32_t W = c.n_neurons;
+    double din[INSTA_MAX_LEVELS * 8 + 64] = {0};
+    double h0[256], h1[256], rgb_in[64];
+    encode_one_d(s, pos, din);
+    const float* p = s->params.data();
+    dense_d(p, din, L.density_in, W, true, h0);
+    p += (uint64_t)W * L.density_in;
+    for (uint32_t k = 1; k < c.density_hidden_layers; ++k) {
+        dense_d(p, h0, W, W, true, h1);
+        std::memcpy(h0, h1, W * sizeof(double));
+        p += (uint64_t)W * W;
+    }
+    dense_d(p, h0, W, c.density_out, false, rgb_in);
+    p += (uint64_t)c.density_out * W;
+    const double x = dir.x, y = dir.y, z = dir.z;
+    double* o = rgb_in + c.density_out;
+    const double pi = 3.14159265358979323846;
+    const double c0 = 0.5 / std::sqrt(pi), c1 = std::sqrt(3.0 / (4 * pi)), c2 = 0.5 * std::sqrt(15.0 / pi),
+                 c3 = 0.25 * std::sqrt(5.0 / pi), c4 = 0.25 * std::sqrt(15.0 / pi), c5 = 0.25 * std::sqrt(35.0 / (2 * pi)),
+                 c6 = 0.5 * std::sqrt(105.0 / pi), c7 = 0.25 * std::sqrt(21.0 / (2 * pi)), c8 = 0.25 * std::sqrt(7.0 / pi),
+                 c9 = 0.25 * std::sqrt(105.0 / pi);
+    o[0] = c0; o[1] = -c1 * y; o[2] = c1 * z; o[3] = -c1 * x;
+    o[4] = c2 * x * y; o[5] = -c2 * y * z; o[6] = c3 * (3 * z * z - 1); o[7] = -c2 * x * z;
+    o[8] = c4 * (x * x - y * y); o[9] = c5 * y * (3 * x * x - y * y) * -1.0; o[10] = c6 * x * y * z;
+    o[11] = c7 * y * (1 - 5 * z * z); o[12] = c8 * z * (5 * z * z - 3); o[13] = c7 * x * (1 - 5 * z * z);
+    o[14] = c9 * z * (x * x - y * y); o[15] = c5 * x * (-x * x + 3 * y * y);
+    dense_d(p, rgb_in, L.rgb_in, W, true, h0);
+    p += (uint64_t)W * L.rgb_in;
+    for (uint32_t k = 1; k < c.rgb_hidden_layers; ++k) {
+        dense_d(p, h0, W, W, true, h1);
+        std::memcpy(h0, h1, W * sizeof(double));
+        p += (uint64_t)W * W;
+    }
+    double rgb[3];
+    dense_d(p, h0, W, 3, false, rgb);
+    out4[0] = rgb[0];
+    out4[1] = rgb[1];
+    out4[2] = rgb[2];
+    out4[3] = rgb_in[0];
 }
 
 }  // namespace
@@ -1157,6 +1400,94 @@ int orc_render(const orc_scene* s, const insta_camera* cam, uint32_t width, uint
     if (stats_out) {
         stats_out[0] = total_samples;
         stats_out[1] = alive0;
+    }
+    return 0;
+}
+
+// FP64 reference render: the FP32 march of orc_render (identical samples),
+// then deformation, encoding, MLPs and compositing in double.  rgba [n*4].
+int orc_render_fp64(const orc_scene* s, const insta_camera* cam, uint32_t width, uint32_t height, int32_t frame,
+                    int32_t occupancy, uint32_t spp, const uint32_t* pixel_subset, uint32_t n_subset, double* rgba) {
+    const MeshFrame* fr = frame_ptr(s, frame);
+    if (!fr || s->params.empty()) return 1;
+    const std::vector<uint8_t>& grid = s->grids[occupancy < 0 ? 0 : occupancy + 1];
+    if (grid.empty()) return 2;
+    const insta_config& c = s->cfg;
+    const uint32_t G = c.grid_size;
+    const float DT = c.dt;
+    const V3 lo = s->render_lo, hi = s->render_hi;
+    const float* M = cam->matrix;
+    const V3 cam_pos = v3(M[3], M[7], M[11]);
+    float offset[2];
+    ld_random_pixel_offset(cam->snap_to_pixel_centers ? 0u : spp, offset);
+    const uint32_t n_pix = pixel_subset ? n_subset : width * height;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t q = 0; q < (int64_t)n_pix; ++q) {
+        const uint32_t i = pixel_subset ? pixel_subset[q] : (uint32_t)q;
+        const uint32_t x = i % width, y = i / width;
+        double* out = rgba + 4 * (uint64_t)q;
+        out[0] = out[1] = out[2] = out[3] = 0.0;
+        float u = ((float)x + offset[0]) / (float)width;
+        float v = ((float)y + offset[1]) / (float)height;
+        V3 d0 = v3((u - cam->screen_center[0]) * (float)width / cam->focal_px[0],
+                   (v - cam->screen_center[1]) * (float)height / cam->focal_px[1], 1.0f);
+        V3 d = v3(M[0] * d0.x + M[1] * d0.y + M[2] * d0.z, M[4] * d0.x + M[5] * d0.y + M[6] * d0.z,
+                  M[8] * d0.x + M[9] * d0.y + M[10] * d0.z);
+        float dn = norm(d);
+        d = v3(d.x / dn, d.y / dn, d.z / dn);
+        const V3 o = cam_pos;
+        float t = std::fmax(box_ray_tmin(lo, hi, o, d), 0.05f) + 1e-6f;
+        if (!box_contains(lo, hi, o + d * t)) continue;
+        const V3 idir = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+        t += ld_random_val(spp, i * 786433u, 0) * DT;
+        bool alive = true;
+        while (true) {
+            V3 pos = o + d * t;
+            if (!box_contains(lo, hi, pos)) {
+                alive = false;
+                break;
+            }
+            if (occupied(grid.data(), G, pos)) break;
+            t = advance_to_next_voxel(t, DT, pos, d, idir, G);
+        }
+        if (!alive) continue;
+        double acc[4] = {0, 0, 0, 0};
+        uint32_t n = 0;
+        const D3v dird = {d.x, d.y, d.z};
+        while (n < c.max_steps) {
+            V3 pos;
+            bool inside = true;
+            while (true) {
+                pos = o + d * t;
+                if (!box_contains(lo, hi, pos)) {
+                    inside = false;
+                    break;
+                }
+                if (occupied(grid.data(), G, pos)) break;
+                t = advance_to_next_voxel(t, DT, pos, d, idir, G);
+            }
+            if (!inside) break;
+            D3v cp = dv(pos), cd = dird;
+            uint32_t mask = 0;
+            warp_one_d(s, *fr, dv(pos), dird, cp, cd, mask);
+            double net[4];
+            network_one_d(s, cp, cd, net);
+            const double T = 1.0 - acc[3];
+            const double sigma = std::exp(net[3] + (double)c.density_logit_bias);
+            const double alpha = 1.0 - std::exp(-sigma * (double)DT);
+            const double weight = alpha * T;
+            for (int k = 0; k < 3; ++k) acc[k] += weight / (1.0 + std::exp(-net[k]));
+            acc[3] += weight;
+            ++n;
+            t += DT;
+            if (acc[3] > 1.0 - (double)c.min_transmittance) {
+                const double a = acc[3];
+                for (int k = 0; k < 4; ++k) acc[k] /= a;
+                break;
+            }
+        }
+        if (acc[3] > 0.001)
+            for (int k = 0; k < 4; ++k) out[k] = acc[k];
     }
     return 0;
 }

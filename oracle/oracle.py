@@ -50,6 +50,8 @@ _SIGS = [
     ("orc_pixel_offset", None, [C.c_uint32, _VP]),
     ("orc_render", C.c_int, [_VP, C.POINTER(InstaCamera), C.c_uint32, C.c_uint32, C.c_int32, C.c_int32,
                              C.c_uint32, _VP, C.c_uint32, _VP, _VP, _VP, _VP]),
+    ("orc_render_fp64", C.c_int, [_VP, C.POINTER(InstaCamera), C.c_uint32, C.c_uint32, C.c_int32, C.c_int32,
+                                  C.c_uint32, _VP, C.c_uint32, _VP]),
 ]
 
 _lib = None
@@ -202,6 +204,18 @@ class Oracle:
                                0 if sub is None else len(sub), _p(rgba), _p(depth), _p(steps), _p(stats))
         assert rc == 0, rc
         return rgba, depth, steps, stats
+
+
+def render_fp64(orc: Oracle, cam: InstaCamera, width: int, height: int, frame: int, occupancy: int, spp: int = 0,
+                pixels=None) -> np.ndarray:
+    """FP64 reference render (same samples as the FP32 march): rgba [n, 4] float64."""
+    sub = None if pixels is None else _c(pixels, np.uint32)
+    n = width * height if sub is None else len(sub)
+    rgba = np.zeros((n, 4), np.float64)
+    rc = orc.L.orc_render_fp64(orc.h, C.byref(cam), width, height, frame, occupancy, spp, _p(sub),
+                               0 if sub is None else len(sub), _p(rgba))
+    assert rc == 0, rc
+    return rgba
 
 
 def sh16(dirs_w) -> np.ndarray:
