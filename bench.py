@@ -52,12 +52,13 @@ def per_sample_costs(cfg, lay):
     }
 
 
-def setup(args):
+def setup(args, precision=0, av=None):
     import torch
     from insta_amd import synth
     from insta_amd.capi import Context, full_config, make_camera
-    cfg = full_config(max_frames=16)
-    av = synth.head_avatar(seed=1, n_frames=16)
+    cfg = full_config(max_frames=16, precision=precision)
+    if av is None:
+        av = synth.head_avatar(seed=1, n_frames=16)
     ctx = Context(cfg)
     ctx.set_canonical_mesh(av.canonical, av.faces, av.masks)
     frames = torch.from_numpy(np.stack(av.frames)).cuda()
@@ -70,6 +71,53 @@ def setup(args):
     cams = [make_camera(synth.random_camera(rng, 0.7), synth.focal_from_fov(args.res, 30.0)) for _ in range(16)]
     torch.cuda.synchronize()
     return ctx, cfg, av, params, cams
+
+
+def timed_renders(ctx, cams, res, steps, warmup, out):
+    """Render `steps` frames (after `warmup`), cycling the 16 posed frames;
+    returns (seconds, samples composited, samples emitted, kernel profile)."""
+    import torch
+    for s in range(warmup):
+        ctx.render(cams[s % 16], res, res, s % 16, s % 16, 0, out)
+    torch.cuda.synchronize()
+    ctx.set_profiling(True)
+    samples = emitted = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        f = s % 16
+        ctx.render(cams[f], res, res, f, f, 0, out)
+        st = ctx.render_stats()
+        samples += st.n_samples
+        emitted += st.n_samples_emitted
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = ctx.profile()
+    ctx.set_profiling(False)
+    return elapsed, samples, emitted, prof
+
+
+def accuracy_vs_fp64(ctxs, cfg, av, params, cam, res, n_pix):
+    """Per-pixel error of each precision's render against the FP64 oracle on
+    a fixed random subset of pixels of frame 0."""
+    from oracle.oracle import Oracle, render_fp64
+    orc = Oracle(cfg)
+    orc.set_canonical_mesh(av.canonical, av.faces, av.masks)
+    orc.set_frame(0, av.frames[0])
+    orc.set_params(params)
+    orc.build_occupancy(0, 0.015)
+    pix = np.random.default_rng(11).choice(res * res, n_pix, replace=False).astype(np.uint32)
+    ref = render_fp64(orc, cam, res, res, 0, 0, 0, pix)
+    out = {}
+    for name, ctx in ctxs.items():
+        rgba, _, _ = ctx.render(cam, res, res, 0, 0, 0)
+        got = rgba.cpu().numpy().astype(np.float64)[pix]
+        err = got[:, :3] - ref[:, :3]
+        mse = float(np.mean(err ** 2))
+        out[name] = dict(max_abs=float(np.abs(err).max()), rms=math.sqrt(mse),
+                         psnr_db=(float("inf") if mse == 0 else 10 * math.log10(1.0 / mse)))
+    out["pixels"] = n_pix
+    return out
 
 
 def cpu_baseline(cfg, av, params, cam, res, budget_s):
@@ -106,6 +154,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--res", type=int, default=512)
     ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds of CPU oracle timing (0: skip)")
+    ap.add_argument("--skip-forms", action="store_true", help="only the FP32 path")
+    ap.add_argument("--accuracy-pixels", type=int, default=4096, help="pixels compared with the FP64 oracle")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 256 MiB device copy first (reference kernel for rocprof byte counters)")
     args = ap.parse_args()
@@ -115,41 +165,23 @@ def main():
         sys.exit(2)
 
     import torch
+    from insta_amd.capi import PRECISION_NAMES
     if args.calibrate_copy:
         src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    ctx, cfg, av, params, cams = setup(args)
     res = args.res
     out = (torch.empty((res * res, 4), dtype=torch.float32, device="cuda"),
            torch.empty(res * res, dtype=torch.float32, device="cuda"),
            torch.empty(res * res, dtype=torch.int32, device="cuda"))
 
-    for s in range(args.warmup):
-        ctx.render(cams[s % 16], res, res, s % 16, s % 16, 0, out)
-    torch.cuda.synchronize()
-
-    ctx.set_profiling(True)
-    samples = 0
-    emitted = 0
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for s in range(args.steps):
-        f = s % 16
-        ctx.render(cams[f], res, res, f, f, 0, out)
-        st = ctx.render_stats()
-        samples += st.n_samples
-        emitted += st.n_samples_emitted
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    prof = ctx.profile()
-    ctx.set_profiling(False)
-
+    # headline: the FP32 path (FP32 MFMA network)
+    ctx, cfg, av, params, cams = setup(args, 0)
+    elapsed, samples, emitted, prof = timed_renders(ctx, cams, res, args.steps, args.warmup, out)
     value = samples / elapsed
     costs = per_sample_costs(cfg, ctx.layout)
-    # dominant kernel by summed event time
     dom = max((k for k in prof if prof[k]["launches"]), key=lambda k: prof[k]["ms"])
     p = prof[dom]
     avg_s = p["ms"] / p["launches"] / 1e3
@@ -158,15 +190,30 @@ def main():
     if dom == "network":
         achieved = c["flops"] * per_launch_items / avg_s / 1e12
         roof = dict(bound="mfma", achieved=achieved, peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
-                    frac=achieved / FP32_PEAK_TFLOPS, traffic=None, kernel="network_soa_kernel<16,2>",
-                    per_unit=f"{c['flops']} FLOP/sample (unpadded), {c['bytes']} B/sample gathered+streamed",
-                    hbm_achieved_gbs=c["bytes"] * per_launch_items / avg_s / 1e9)
+                    frac=achieved / FP32_PEAK_TFLOPS, traffic=None, kernel="network_f32_kernel<16>",
+                    per_unit=f"{c['flops']} FLOP/sample (unpadded), {c['bytes']} B/sample gathered+streamed")
     else:
         achieved = c["bytes"] * per_launch_items / avg_s / 1e9
         roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                    traffic=None, kernel=dom, per_unit=f"{c['bytes']} B/sample")
+                    traffic=None, kernel=f"{dom}_soa_kernel", per_unit=f"{c['bytes']} B/sample streamed")
     kernels = {k: dict(ms_per_frame=v["ms"] / args.steps, launches=v["launches"], items=v["items"])
                for k, v in prof.items() if v["launches"]}
+
+    # precision forms: same workload, other network arithmetic
+    forms = {"fp32": dict(frames_per_s=args.steps / elapsed, samples_per_s=value,
+                          network_ms_per_frame=prof["network"]["ms"] / args.steps)}
+    ctxs = {"fp32": ctx}
+    if not args.skip_forms:
+        for prec in (3, 1, 2):
+            c2, _, _, _, cams2 = setup(args, prec, av)
+            e2, s2, _, p2 = timed_renders(c2, cams2, res, args.steps, args.warmup, out)
+            forms[PRECISION_NAMES[prec]] = dict(frames_per_s=args.steps / e2, samples_per_s=s2 / e2,
+                                                network_ms_per_frame=p2["network"]["ms"] / args.steps)
+            ctxs[PRECISION_NAMES[prec]] = c2
+        forms["fp16_vs_fp32_frames"] = forms["fp16"]["frames_per_s"] / forms["fp32"]["frames_per_s"]
+        forms["fp16_vs_fp32_scalar_frames"] = forms["fp16"]["frames_per_s"] / forms["fp32_scalar"]["frames_per_s"]
+        if args.accuracy_pixels > 0:
+            forms["accuracy_vs_fp64"] = accuracy_vs_fp64(ctxs, cfg, av, params, cams[0], res, args.accuracy_pixels)
 
     cpu = None
     if args.cpu_budget > 0:
@@ -189,6 +236,7 @@ def main():
                                f"{res}x{res}, <=1024 steps", "resolution": res, "frames": 16,
                    "samples_per_frame": samples / args.steps, "emitted_per_frame": emitted / args.steps},
         "frames_per_s": args.steps / elapsed,
+        "precision_forms": forms,
         "roofline": roof,
         "kernels": kernels,
         "cpu_baseline": cpu,

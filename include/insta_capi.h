@@ -38,7 +38,15 @@ typedef enum insta_status {
     INSTA_ERR_UNSUPPORTED = 5  /* configuration outside what the kernels handle */
 } insta_status;
 
-enum { INSTA_PRECISION_FP32 = 0, INSTA_PRECISION_FP16 = 1, INSTA_PRECISION_BF16 = 2 };
+/* Arithmetic of the network: FP32 MFMA (v_mfma_f32_16x16x4f32), FP16 or BF16
+ * MFMA with FP32 accumulation and FP16 hash tables, or FP32 per-sample FMA
+ * chains without matrix cores (the plain FP32 path). */
+enum {
+    INSTA_PRECISION_FP32 = 0,
+    INSTA_PRECISION_FP16 = 1,
+    INSTA_PRECISION_BF16 = 2,
+    INSTA_PRECISION_FP32_SCALAR = 3
+};
 
 /* Network/encoding/march configuration.  Field meanings follow the
  * reference's JSON keys (configs/nerf/main.json:33-69) and testbed constants

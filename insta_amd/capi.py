@@ -18,7 +18,8 @@ LIB_PATH = PKG / "libinsta_amd.so"
 
 INSTA_OK = 0
 STATUS_NAMES = {0: "OK", 1: "INVALID", 2: "HIP", 3: "OOM", 4: "STATE", 5: "UNSUPPORTED"}
-PRECISION_FP32, PRECISION_FP16, PRECISION_BF16 = 0, 1, 2
+PRECISION_FP32, PRECISION_FP16, PRECISION_BF16, PRECISION_FP32_SCALAR = 0, 1, 2, 3
+PRECISION_NAMES = {0: "fp32", 1: "fp16", 2: "bf16", 3: "fp32_scalar"}
 MAX_LEVELS = 32
 MAX_TRIANGLES = 16384
 

@@ -94,7 +94,7 @@ insta_status validate(const insta_config& c) {
     if (c.max_frames == 0 || c.max_frames > 4096) return bad("max_frames in [1, 4096]");
     if (c.max_triangles == 0 || c.max_triangles > INSTA_MAX_TRIANGLES) return bad("max_triangles in [1, 16384]");
     if (c.max_vertices == 0 || c.max_vertices > (1u << 24)) return bad("max_vertices in [1, 2^24]");
-    if (c.precision > INSTA_PRECISION_BF16) return bad("unknown precision");
+    if (c.precision > INSTA_PRECISION_FP32_SCALAR) return bad("unknown precision");
     return INSTA_OK;
 }
 
@@ -218,7 +218,8 @@ insta_status insta_create(const insta_config* cfg, insta_ctx** out) {
     alloc((void**)&ctx->d_build_q, S * 3 * T * sizeof(uint32_t));
     alloc((void**)&ctx->d_params, ctx->layout.n_params * sizeof(float));
     alloc(&ctx->d_wfrag, 64 * 1024 * sizeof(float));
-    if (cfg->precision != INSTA_PRECISION_FP32) alloc(&ctx->d_table_h, ctx->layout.n_grid_params * 2);
+    if (cfg->precision == INSTA_PRECISION_FP16 || cfg->precision == INSTA_PRECISION_BF16)
+        alloc(&ctx->d_table_h, ctx->layout.n_grid_params * 2);
     alloc((void**)&ctx->d_counters, 16 * sizeof(uint32_t));
     alloc(&ctx->d_scratch, ctx->scratch_bytes);
     if (ok && hipHostMalloc((void**)&ctx->h_counters, 16 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
@@ -502,7 +503,7 @@ insta_status insta_network_inference(insta_ctx* ctx, const float* coords_dev, ui
     }
     if (insta::mfma_network_supported(ctx->cfg))
         return insta::launch_network_mfma_aos(ctx, coords_dev, n, stride_floats, out_dev, (hipStream_t)stream);
-    if (!insta::network_supported(ctx->cfg) || ctx->cfg.precision != INSTA_PRECISION_FP32) {
+    if (!insta::network_supported(ctx->cfg) || !insta::scalar_precision(ctx->cfg)) {
         set_error("network_inference: configuration not supported by the fused MLP kernels");
         return INSTA_ERR_UNSUPPORTED;
     }
@@ -522,7 +523,7 @@ insta_status insta_render(insta_ctx* ctx, const insta_camera* cam, uint32_t widt
         return INSTA_ERR_STATE;
     }
     if (!insta::mfma_network_supported(ctx->cfg) &&
-        (!insta::network_supported(ctx->cfg) || ctx->cfg.precision != INSTA_PRECISION_FP32)) {
+        (!insta::network_supported(ctx->cfg) || !insta::scalar_precision(ctx->cfg))) {
         set_error("render: configuration not supported by the fused MLP kernels");
         return INSTA_ERR_UNSUPPORTED;
     }

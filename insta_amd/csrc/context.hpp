@@ -144,6 +144,10 @@ insta_status launch_network_aos(insta_ctx* ctx, const float* coords, uint32_t n,
 insta_status launch_network_soa(insta_ctx* ctx, const uint32_t* n_dev, uint32_t n_max, hipStream_t stream);
 bool network_supported(const insta_config& cfg);
 bool mfma_network_supported(const insta_config& cfg);
+// FP32 arithmetic that the per-sample (scalar FMA) MLP kernel implements
+inline bool scalar_precision(const insta_config& c) {
+    return c.precision == INSTA_PRECISION_FP32 || c.precision == INSTA_PRECISION_FP32_SCALAR;
+}
 insta_status prepare_network(insta_ctx* ctx, hipStream_t stream);
 insta_status launch_network_mfma_soa(insta_ctx* ctx, const uint32_t* n_dev, uint32_t n_max, hipStream_t stream);
 insta_status launch_network_mfma_aos(insta_ctx* ctx, const float* coords, uint32_t n, uint32_t stride, float* out,

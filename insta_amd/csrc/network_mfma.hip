@@ -439,7 +439,7 @@ uint32_t grid_for(uint32_t n_max) {
 }  // namespace
 
 bool mfma_network_supported(const insta_config& c) {
-    return c.n_neurons == 64 && c.density_out == 16 && c.sh_degree == 4 && c.n_extra_dims == 0 &&
+    return c.precision != INSTA_PRECISION_FP32_SCALAR && c.n_neurons == 64 && c.density_out == 16 && c.sh_degree == 4 && c.n_extra_dims == 0 &&
            c.density_hidden_layers == 1 && c.rgb_hidden_layers == 2 && c.n_features_per_level == 2 &&
            (c.n_levels == 8 || c.n_levels == 16);
 }
