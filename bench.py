@@ -81,7 +81,7 @@ def timed_renders(ctx, cams, res, steps, warmup, out):
         ctx.render(cams[s % 16], res, res, s % 16, s % 16, 0, out)
     torch.cuda.synchronize()
     ctx.set_profiling(True)
-    samples = emitted = 0
+    samples = emitted = rounds = rays0 = 0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s in range(steps):
@@ -90,9 +90,13 @@ def timed_renders(ctx, cams, res, steps, warmup, out):
         st = ctx.render_stats()
         samples += st.n_samples
         emitted += st.n_samples_emitted
+        rounds += st.n_iterations
+        rays0 += st.n_rays_alive0
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     prof = ctx.profile()
+    prof["_rounds_per_frame"] = rounds / steps
+    prof["_live_rays_per_frame"] = rays0 / steps
     ctx.set_profiling(False)
     return elapsed, samples, emitted, prof
 
@@ -180,6 +184,7 @@ def main():
     # headline: the FP32 path (FP32 MFMA network)
     ctx, cfg, av, params, cams = setup(args, 0)
     elapsed, samples, emitted, prof = timed_renders(ctx, cams, res, args.steps, args.warmup, out)
+    rounds, live_rays = prof.pop("_rounds_per_frame"), prof.pop("_live_rays_per_frame")
     value = samples / elapsed
     costs = per_sample_costs(cfg, ctx.layout)
     dom = max((k for k in prof if prof[k]["launches"]), key=lambda k: prof[k]["ms"])
@@ -207,6 +212,7 @@ def main():
         for prec in (3, 1, 2):
             c2, _, _, _, cams2 = setup(args, prec, av)
             e2, s2, _, p2 = timed_renders(c2, cams2, res, args.steps, args.warmup, out)
+            p2.pop("_rounds_per_frame"), p2.pop("_live_rays_per_frame")
             forms[PRECISION_NAMES[prec]] = dict(frames_per_s=args.steps / e2, samples_per_s=s2 / e2,
                                                 network_ms_per_frame=p2["network"]["ms"] / args.steps)
             ctxs[PRECISION_NAMES[prec]] = c2
@@ -234,7 +240,8 @@ def main():
         "data": "synthetic (seeded ellipsoid head, Gaussian expression bases, random-init hash grid + MLPs)",
         "config": {"workload": "full-size FP32 render: 9,976-tri head, 16 frames, L16 F2 T2^19, 128^3 grid, "
                                f"{res}x{res}, <=1024 steps", "resolution": res, "frames": 16,
-                   "samples_per_frame": samples / args.steps, "emitted_per_frame": emitted / args.steps},
+                   "samples_per_frame": samples / args.steps, "emitted_per_frame": emitted / args.steps,
+                   "live_rays_per_frame": live_rays, "march_rounds_per_frame": rounds},
         "frames_per_s": args.steps / elapsed,
         "precision_forms": forms,
         "roofline": roof,

@@ -335,7 +335,7 @@ constexpr uint32_t kQueryThreads = 256;
 
 __global__ __launch_bounds__(kQueryThreads) void closest_kernel(MeshView m, const float* __restrict__ pts, uint32_t n,
                                                                int32_t* __restrict__ ids, float* __restrict__ dist) {
-    __shared__ uint16_t stack[kStackDepth * kQueryThreads];
+    extern __shared__ uint32_t stack[];  // [stack_depth(k)][kQueryThreads]
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     V3 p = v3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
@@ -348,7 +348,7 @@ __global__ __launch_bounds__(kQueryThreads) void closest_kernel(MeshView m, cons
 // one thread per byte of 8 Morton-consecutive cells.
 __global__ __launch_bounds__(kQueryThreads) void occupancy_kernel(MeshView m, uint32_t G, float threshold,
                                                                  uint8_t* __restrict__ bits, uint32_t n_bytes) {
-    __shared__ uint16_t stack[kStackDepth * kQueryThreads];
+    extern __shared__ uint32_t stack[];  // [stack_depth(k)][kQueryThreads]
     const uint32_t byte = blockIdx.x * blockDim.x + threadIdx.x;
     if (byte >= n_bytes) return;
     const float g = (float)G;
@@ -370,7 +370,32 @@ __global__ __launch_bounds__(kQueryThreads) void occupancy_kernel(MeshView m, ui
     bits[byte] = (uint8_t)v;
 }
 
+// Linear-order copy of a Morton-order bitfield: bit x + G(y + G z) of the
+// output is bit morton3(x, y, z) of the input.  One thread per output byte.
+__global__ void grid_linear_kernel(const uint8_t* __restrict__ morton_bits, uint32_t G, uint8_t* __restrict__ lin,
+                                   uint32_t n_bytes) {
+    const uint32_t byte = blockIdx.x * blockDim.x + threadIdx.x;
+    if (byte >= n_bytes) return;
+    uint32_t v = 0;
+    for (uint32_t b = 0; b < 8; ++b) {
+        const uint32_t cell = byte * 8 + b;
+        const uint32_t x = cell % G, y = (cell / G) % G, z = cell / (G * G);
+        const uint32_t m = morton3(x, y, z);
+        v |= ((morton_bits[m >> 3] >> (m & 7u)) & 1u) << b;
+    }
+    lin[byte] = (uint8_t)v;
+}
+
 }  // namespace
+
+insta_status launch_grid_linear(insta_ctx* ctx, uint32_t slot, hipStream_t stream) {
+    const uint32_t G = ctx->cfg.grid_size;
+    const uint32_t n_bytes = G * G * G / 8;
+    hipLaunchKernelGGL(grid_linear_kernel, dim3((n_bytes + 255) / 256), dim3(256), 0, stream, ctx->grid(slot), G,
+                       ctx->grid_lin(slot), n_bytes);
+    INSTA_HIP(hipGetLastError());
+    return INSTA_OK;
+}
 
 insta_status launch_build_slots(insta_ctx* ctx, uint32_t first_slot, uint32_t n_slots, const float* verts,
                                 hipStream_t stream) {
@@ -421,8 +446,9 @@ insta_status launch_blend_expression(insta_ctx* ctx, uint32_t slot, const float*
 insta_status launch_closest(insta_ctx* ctx, uint32_t slot, const float* pts, uint32_t n, int32_t* ids, float* dist,
                             hipStream_t stream) {
     if (n == 0) return INSTA_OK;
-    hipLaunchKernelGGL(closest_kernel, dim3((n + kQueryThreads - 1) / kQueryThreads), dim3(kQueryThreads), 0, stream,
-                       ctx->mesh(slot), pts, n, ids, dist);
+    const size_t shmem = (size_t)stack_depth(ctx->bvh_k) * kQueryThreads * sizeof(uint32_t);
+    hipLaunchKernelGGL(closest_kernel, dim3((n + kQueryThreads - 1) / kQueryThreads), dim3(kQueryThreads), shmem,
+                       stream, ctx->mesh(slot), pts, n, ids, dist);
     INSTA_HIP(hipGetLastError());
     return INSTA_OK;
 }
@@ -430,8 +456,9 @@ insta_status launch_closest(insta_ctx* ctx, uint32_t slot, const float* pts, uin
 insta_status launch_occupancy(insta_ctx* ctx, uint32_t slot, float threshold, hipStream_t stream) {
     const uint32_t G = ctx->cfg.grid_size;
     const uint32_t n_bytes = G * G * G / 8;
-    hipLaunchKernelGGL(occupancy_kernel, dim3((n_bytes + kQueryThreads - 1) / kQueryThreads), dim3(kQueryThreads), 0,
-                       stream, ctx->mesh(slot), G, threshold, ctx->grid(slot), n_bytes);
+    const size_t shmem = (size_t)stack_depth(ctx->bvh_k) * kQueryThreads * sizeof(uint32_t);
+    hipLaunchKernelGGL(occupancy_kernel, dim3((n_bytes + kQueryThreads - 1) / kQueryThreads), dim3(kQueryThreads),
+                       shmem, stream, ctx->mesh(slot), G, threshold, ctx->grid(slot), n_bytes);
     INSTA_HIP(hipGetLastError());
     return INSTA_OK;
 }

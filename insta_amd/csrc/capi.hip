@@ -213,6 +213,7 @@ insta_status insta_create(const insta_config* cfg, insta_ctx** out) {
     alloc((void**)&ctx->d_proj_inv, S * T * 3 * sizeof(float4));
     alloc((void**)&ctx->d_cen, S * T * sizeof(float4));
     alloc((void**)&ctx->d_grids, S * (G * G * G / 8));
+    alloc((void**)&ctx->d_grids_lin, S * (G * G * G / 8));
     alloc((void**)&ctx->d_built, S);
     alloc((void**)&ctx->d_build_lists, S * 6 * T * sizeof(uint16_t));
     alloc((void**)&ctx->d_build_q, S * 3 * T * sizeof(uint32_t));
@@ -226,6 +227,7 @@ insta_status insta_create(const insta_config* cfg, insta_ctx** out) {
         ok = false;
     if (ok && hipMemset(ctx->d_built, 0, S) != hipSuccess) ok = false;
     if (ok && hipMemset(ctx->d_grids, 0, S * (G * G * G / 8)) != hipSuccess) ok = false;
+    if (ok && hipMemset(ctx->d_grids_lin, 0, S * (G * G * G / 8)) != hipSuccess) ok = false;
     if (!ok) {
         insta_destroy(ctx);
         set_error("device allocation failed in insta_create");
@@ -241,7 +243,7 @@ void insta_destroy(insta_ctx* ctx) {
     using insta::dev_free;
     dev_free(ctx->d_faces); dev_free(ctx->d_masks); dev_free(ctx->d_adj); dev_free(ctx->d_verts);
     dev_free(ctx->d_nodes); dev_free(ctx->d_btris); dev_free(ctx->d_proj); dev_free(ctx->d_proj_inv);
-    dev_free(ctx->d_cen); dev_free(ctx->d_grids); dev_free(ctx->d_built); dev_free(ctx->d_params);
+    dev_free(ctx->d_cen); dev_free(ctx->d_grids); dev_free(ctx->d_grids_lin); dev_free(ctx->d_built); dev_free(ctx->d_params);
     dev_free(ctx->d_build_lists); dev_free(ctx->d_build_q);
     dev_free(ctx->d_counters); dev_free(ctx->d_scratch); dev_free(ctx->d_wfrag); dev_free(ctx->d_table_h);
     if (ctx->h_counters) (void)hipHostFree(ctx->h_counters);
@@ -431,6 +433,8 @@ insta_status insta_build_occupancy(insta_ctx* ctx, int32_t frame, float threshol
     }
     insta_status st = insta::launch_occupancy(ctx, slot, threshold, stream);
     if (st != INSTA_OK) return st;
+    st = insta::launch_grid_linear(ctx, slot, stream);
+    if (st != INSTA_OK) return st;
     ctx->grid_built[slot] = 1;
     const size_t G = ctx->cfg.grid_size;
     if (bitfield_dev)
@@ -448,6 +452,8 @@ insta_status insta_set_occupancy(insta_ctx* ctx, int32_t slot_arg, const uint8_t
     const size_t G = ctx->cfg.grid_size;
     INSTA_HIP(hipMemcpyAsync(ctx->grid(slot), bitfield_dev, G * G * G / 8, hipMemcpyDeviceToDevice,
                              (hipStream_t)stream));
+    const insta_status st = insta::launch_grid_linear(ctx, slot, (hipStream_t)stream);
+    if (st != INSTA_OK) return st;
     ctx->grid_built[slot] = 1;
     return INSTA_OK;
 }

@@ -103,16 +103,38 @@ struct Hit {
     float dist_sq;
 };
 
-// Nearest triangle (bvh.cuh:48-96): far-to-near ordered DFS over the implicit
-// 4-ary heap, children pushed when their box distance is within a few ulp of
-// the best so far (so the result equals the brute-force argmin),
-// leaf triangles ranked by (dist_sq, original id).  `stack` points at this
-// thread's column of a [depth][blockDim] uint16 LDS array.
-INSTA_DEV Hit bvh_closest(const MeshView& m, V3 p, uint16_t* stack, uint32_t stride, int32_t hint = -1,
+// Nearest triangle (bvh.cuh:48-96), exact under the (dist_sq, original id)
+// order: the result equals the brute-force argmin, whatever the visiting
+// order, because a subtree is skipped only when its box lies beyond the best
+// triangle found so far by more than a rounding margin.
+//
+// Traversal (implicit 4-ary heap): descend into the nearest surviving child,
+// pushing the other survivors far-to-near; at a leaf rank its triangles, then
+// pop until an entry whose box distance is still within the (possibly
+// tightened) bound.  Inner-node descents and leaf tests run as two loops so
+// the lanes of a wave execute the same code path together.
+//
+// `stack` points at this thread's column of a [stack_depth(k)][stride] uint32
+// LDS array.  An entry packs the child's box distance, truncated to its upper
+// 16 bits (never above the true distance, so the pop-time check stays
+// conservative), with the 16-bit node index.
+constexpr uint32_t kNoNode = 0xffffffffu;
+
+// conservative prune bound (relative 2^-18 + absolute 2^-26): rounding can
+// put a box a few ulp beyond the best triangle it contains
+INSTA_DEV float prune_bound(float best) { return best + best * 0x1p-18f + 0x1p-26f; }
+
+INSTA_DEV uint32_t pop_live(const uint32_t* stack, int& sp, uint32_t stride, float bound) {
+    while (sp > 0) {
+        --sp;
+        const uint32_t w = stack[sp * stride];
+        if (__uint_as_float(w & 0xffff0000u) <= bound) return w & 0xffffu;
+    }
+    return kNoNode;
+}
+
+INSTA_DEV Hit bvh_closest(const MeshView& m, V3 p, uint32_t* stack, uint32_t stride, int32_t hint = -1,
                           float max_dist_sq = 3.40282347e38f) {
-    int sp = 0;
-    stack[0] = 0;
-    sp = 1;
     // max_dist_sq (bvh.cuh:48): only triangles closer than this are reported
     float best = max_dist_sq;
     int32_t best_idx = -1;
@@ -128,27 +150,12 @@ INSTA_DEV Hit bvh_closest(const MeshView& m, V3 p, uint16_t* stack, uint32_t str
         best_idx = hint;
         best_orig = __float_as_uint(r2.y);
     }
+    float bound = prune_bound(best);
     const uint32_t shift = 2u * m.k;
-    while (sp > 0) {
-        --sp;
-        const uint32_t node = stack[sp * stride];
-        if (node >= m.leaf_base) {
-            const uint32_t j = node - m.leaf_base;
-            const uint32_t b = (uint32_t)(((uint64_t)j * m.nf) >> shift);
-            const uint32_t e = (uint32_t)(((uint64_t)(j + 1) * m.nf) >> shift);
-            for (uint32_t t = b; t < e; ++t) {
-                const float4 r0 = m.btris[3 * t + 0];
-                const float4 r1 = m.btris[3 * t + 1];
-                const float4 r2 = m.btris[3 * t + 2];
-                const float d = tri_distance_sq(v3(r0.x, r0.y, r0.z), v3(r0.w, r1.x, r1.y), v3(r1.z, r1.w, r2.x), p);
-                const uint32_t o = __float_as_uint(r2.y);
-                if (d < best || (d == best && o < best_orig)) {
-                    best = d;
-                    best_idx = (int32_t)t;
-                    best_orig = o;
-                }
-            }
-        } else {
+    int sp = 0;
+    uint32_t node = 0;
+    for (;;) {
+        while (node < m.leaf_base) {
             const uint32_t first = 4u * node + 1u;
             float cd[4];
             uint32_t ci[4];
@@ -157,24 +164,48 @@ INSTA_DEV Hit bvh_closest(const MeshView& m, V3 p, uint16_t* stack, uint32_t str
                 ci[c] = first + c;
                 cd[c] = box_distance_sq(m.nodes[2 * (first + c)], m.nodes[2 * (first + c) + 1], p);
             }
-#define INSTA_CAS(i, j)                          \
-    if (cd[i] < cd[j]) {                         \
+            // sorting network, farthest first (bvh.cuh:8-46)
+#define INSTA_CAS(i, j)                              \
+    if (cd[i] < cd[j]) {                             \
         float td = cd[i]; cd[i] = cd[j]; cd[j] = td; \
         uint32_t ti = ci[i]; ci[i] = ci[j]; ci[j] = ti; \
     }
             INSTA_CAS(0, 2) INSTA_CAS(1, 3) INSTA_CAS(0, 1) INSTA_CAS(2, 3) INSTA_CAS(1, 2)
 #undef INSTA_CAS
-            // conservative prune (relative 2^-18 + absolute 2^-26): rounding
-            // can put a box a few ulp beyond the best triangle it contains
-            const float bound = best + best * 0x1p-18f + 0x1p-26f;
+            uint32_t next = kNoNode;
+            float nd = 0.0f;
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 if (cd[c] <= bound) {
-                    stack[sp * stride] = (uint16_t)ci[c];
-                    ++sp;
+                    if (next != kNoNode) {
+                        stack[sp * stride] = (__float_as_uint(nd) & 0xffff0000u) | next;
+                        ++sp;
+                    }
+                    next = ci[c];
+                    nd = cd[c];
                 }
             }
+            node = next != kNoNode ? next : pop_live(stack, sp, stride, bound);
         }
+        if (node == kNoNode) break;
+        const uint32_t j = node - m.leaf_base;
+        const uint32_t b = (uint32_t)(((uint64_t)j * m.nf) >> shift);
+        const uint32_t e = (uint32_t)(((uint64_t)(j + 1) * m.nf) >> shift);
+        for (uint32_t t = b; t < e; ++t) {
+            const float4 r0 = m.btris[3 * t + 0];
+            const float4 r1 = m.btris[3 * t + 1];
+            const float4 r2 = m.btris[3 * t + 2];
+            const float d = tri_distance_sq(v3(r0.x, r0.y, r0.z), v3(r0.w, r1.x, r1.y), v3(r1.z, r1.w, r2.x), p);
+            const uint32_t o = __float_as_uint(r2.y);
+            if (d < best || (d == best && o < best_orig)) {
+                best = d;
+                best_idx = (int32_t)t;
+                best_orig = o;
+            }
+        }
+        bound = prune_bound(best);
+        node = pop_live(stack, sp, stride, bound);
+        if (node == kNoNode) break;
     }
     Hit h;
     h.idx = best_idx;
@@ -183,7 +214,10 @@ INSTA_DEV Hit bvh_closest(const MeshView& m, V3 p, uint16_t* stack, uint32_t str
     return h;
 }
 
-// Maximum DFS stack depth: at most 3 pending siblings per level plus one.
-constexpr uint32_t kStackDepth = 24;
+// Stack entries a query can hold: at most 3 pending siblings per inner level
+// of a tree with k inner levels (k <= 7 for 16384 triangles).  Kernels keep
+// the stack in dynamic LDS sized by this, so shallower trees leave room for
+// more resident waves.
+INSTA_HD uint32_t stack_depth(uint32_t k) { return k ? 3u * k : 1u; }
 
 }  // namespace insta

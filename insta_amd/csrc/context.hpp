@@ -31,7 +31,8 @@ struct insta_ctx {
     float4* d_proj = nullptr;       // [n_slots][max_triangles*3]
     float4* d_proj_inv = nullptr;   // [n_slots][max_triangles*3]
     float4* d_cen = nullptr;        // [n_slots][max_triangles]
-    uint8_t* d_grids = nullptr;     // [n_slots][G^3/8]
+    uint8_t* d_grids = nullptr;     // [n_slots][G^3/8] Morton-order bitfields (the reference layout)
+    uint8_t* d_grids_lin = nullptr; // [n_slots][G^3/8] same bits, linear x + G(y + G z) order (march)
     uint8_t* d_built = nullptr;     // [n_slots] 1 once a slot's mesh is built
     uint16_t* d_build_lists = nullptr;  // [n_slots][2][3][max_triangles] BVH build scratch
     uint32_t* d_build_q = nullptr;      // [n_slots][3][max_triangles]
@@ -59,6 +60,7 @@ struct insta_ctx {
     float4* s_cpos = nullptr;       // canonical xyz
     float4* s_cdir = nullptr;       // canonical direction (warped to [0,1])
     float4* s_out = nullptr;        // rgb logits, density logit
+    int32_t* s_hit = nullptr;       // per sample slot: march t values, then nearest triangle (BVH order)
     uint32_t* d_counters = nullptr; // [8]: alive counts, sample counter, stats
     uint32_t* h_counters = nullptr; // pinned mirror
     insta_render_stats last_stats{};
@@ -94,6 +96,10 @@ struct insta_ctx {
     uint8_t* grid(uint32_t slot) const {
         const size_t G = cfg.grid_size;
         return d_grids + (size_t)slot * (G * G * G / 8);
+    }
+    uint8_t* grid_lin(uint32_t slot) const {
+        const size_t G = cfg.grid_size;
+        return d_grids_lin + (size_t)slot * (G * G * G / 8);
     }
 };
 
@@ -136,6 +142,7 @@ insta_status launch_deform_aos(insta_ctx* ctx, float* coords, uint32_t n, uint32
 insta_status launch_deform_soa(insta_ctx* ctx, int32_t frame, const uint32_t* n_dev, uint32_t n_max,
                                hipStream_t stream);
 insta_status launch_occupancy(insta_ctx* ctx, uint32_t slot, float threshold, hipStream_t stream);
+insta_status launch_grid_linear(insta_ctx* ctx, uint32_t slot, hipStream_t stream);
 insta_status launch_encode(insta_ctx* ctx, const float* pos, uint32_t n, float* out, hipStream_t stream);
 insta_status launch_encode_backward(insta_ctx* ctx, const float* pos, uint32_t n, const float* dL, float* grad,
                                     hipStream_t stream);

@@ -25,7 +25,7 @@ struct MeshSet {
     size_t node_stride, tri_stride, cen_stride;  // float4 units per slot
     uint32_t nf, leaf_base, k, n_slots, disable_interp;
 
-    INSTA_DEV MeshView view(uint32_t slot) const {
+    INSTA_HD MeshView view(uint32_t slot) const {
         MeshView m;
         m.nodes = nodes + slot * node_stride;
         m.btris = btris + slot * tri_stride;
@@ -128,16 +128,18 @@ INSTA_DEV V3 apply_dir(const float m[12], V3 d) {
               m[8] * d.x + m[9] * d.y + m[10] * d.z);
 }
 
-// Body of `warp` for one sample; false leaves the sample untouched.
-INSTA_DEV bool warp_one(const MeshSet& ms, uint32_t frame, V3 p, V3 dir_w, V3& out_p, V3& out_d, uint32_t& mask,
-                        uint16_t* stack, uint32_t stride, int32_t* hint = nullptr) {
-    if (!(p.x >= 0.0f && p.x <= 1.0f && p.y >= 0.0f && p.y <= 1.0f && p.z >= 0.0f && p.z <= 1.0f)) return false;
+INSTA_DEV bool in_unit_box(V3 p) {
+    return p.x >= 0.0f && p.x <= 1.0f && p.y >= 0.0f && p.y <= 1.0f && p.z >= 0.0f && p.z <= 1.0f;
+}
+
+// The transform part of `warp` for a sample whose nearest triangle (BVH-order
+// index `idx` on the frame's BVH) is known.
+INSTA_DEV void warp_from_hit(const MeshSet& ms, uint32_t frame, int32_t idx, V3 p, V3 dir_w, V3& out_p, V3& out_d,
+                             uint32_t& mask) {
     const MeshView fr = ms.view(frame + 1);
-    const Hit h = bvh_closest(fr, p, stack, stride, hint ? *hint : -1);
-    if (hint) *hint = h.idx;
-    if (h.idx < 0) return false;
-    const uint32_t id = h.orig;
-    mask = __float_as_uint(fr.btris[3 * h.idx + 2].z);
+    const float4 r2 = fr.btris[3 * idx + 2];
+    const uint32_t id = __float_as_uint(r2.y);
+    mask = __float_as_uint(r2.z);
     const MeshView cn = ms.view(0);
     float W[12], C[12];
     if (mask == 2u && !ms.disable_interp) {
@@ -156,6 +158,15 @@ INSTA_DEV bool warp_one(const MeshSet& ms, uint32_t frame, V3 p, V3 dir_w, V3& o
     const float n = norm(dc);
     dc = v3(dc.x / n, dc.y / n, dc.z / n);
     out_d = v3((dc.x + 1.0f) * 0.5f, (dc.y + 1.0f) * 0.5f, (dc.z + 1.0f) * 0.5f);
+}
+
+// Body of `warp` for one sample; false leaves the sample untouched.
+INSTA_DEV bool warp_one(const MeshSet& ms, uint32_t frame, V3 p, V3 dir_w, V3& out_p, V3& out_d, uint32_t& mask,
+                        uint32_t* stack, uint32_t stride) {
+    if (!in_unit_box(p)) return false;
+    const Hit h = bvh_closest(ms.view(frame + 1), p, stack, stride);
+    if (h.idx < 0) return false;
+    warp_from_hit(ms, frame, h.idx, p, dir_w, out_p, out_d, mask);
     return true;
 }
 
@@ -163,7 +174,7 @@ INSTA_DEV bool warp_one(const MeshSet& ms, uint32_t frame, V3 p, V3 dir_w, V3& o
 __global__ __launch_bounds__(kDeformThreads) void deform_aos_kernel(MeshSet ms, float* __restrict__ coords, uint32_t n,
                                                                     uint32_t stride, int32_t frame_arg,
                                                                     uint32_t n_extra) {
-    __shared__ uint16_t stack[kStackDepth * kDeformThreads];
+    extern __shared__ uint32_t stack[];  // [stack_depth(k)][kDeformThreads]
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float* c = coords + (size_t)i * stride;
@@ -182,37 +193,68 @@ __global__ __launch_bounds__(kDeformThreads) void deform_aos_kernel(MeshSet ms, 
         for (uint32_t m = 0; m < n_extra; ++m) c[stride - n_extra + m] = 1.0f;
 }
 
-// Render samples (SoA): posed position + ray id in s_pos, ray direction from
-// r_d; canonical position / warped direction to s_cpos / s_cdir.
-// Each thread deforms kChunk consecutive samples (consecutive samples of a
-// ray lie dt apart), seeding every query with the previous one's nearest
-// triangle.
-constexpr uint32_t kChunk = 4;
+// Render samples (SoA) in two kernels:
+//  hits:  nearest triangle per posed sample (s_pos) into s_hit; each thread
+//         takes kChunk consecutive samples (consecutive samples of a ray lie
+//         dt apart) and seeds every query with the previous one's triangle.
+//         Kept apart from the transforms so the traversal runs at low
+//         register count, i.e. with many waves resident to cover its chain
+//         of dependent node loads.
+//  apply: per sample, the frame transforms of `warp` from the hit; writes
+//         canonical position (+ mask bits) to s_cpos and the warped
+//         direction to s_cdir.
+constexpr uint32_t kChunk = 2;
+constexpr uint32_t kMaxBlocks = 8192;
 
-__global__ __launch_bounds__(kDeformThreads) void deform_soa_kernel(MeshSet ms, int32_t frame,
-                                                                    const uint32_t* __restrict__ n_dev,
-                                                                    const float4* __restrict__ s_pos,
-                                                                    const float4* __restrict__ r_d,
-                                                                    float4* __restrict__ s_cpos,
-                                                                    float4* __restrict__ s_cdir) {
-    __shared__ uint16_t stack[kStackDepth * kDeformThreads];
+__global__ __launch_bounds__(kDeformThreads) void deform_hits_kernel(MeshView fr, const uint32_t* __restrict__ n_dev,
+                                                                     const float4* __restrict__ s_pos,
+                                                                     int32_t* __restrict__ s_hit) {
+    extern __shared__ uint32_t stack[];  // [stack_depth(k)][kDeformThreads]
     const uint32_t n = *n_dev;
-    const uint32_t first = (blockIdx.x * blockDim.x + threadIdx.x) * kChunk;
-    int32_t hint = -1;
-    for (uint32_t j = 0; j < kChunk; ++j) {
-        const uint32_t i = first + j;
-        if (i >= n) return;
+    const uint32_t step = gridDim.x * kDeformThreads * kChunk;
+    for (uint32_t first = (blockIdx.x * kDeformThreads + threadIdx.x) * kChunk; first < n; first += step) {
+        int32_t hint = -1;
+#pragma unroll
+        for (uint32_t j = 0; j < kChunk; ++j) {
+            const uint32_t i = first + j;
+            if (i >= n) break;
+            const float4 sp = s_pos[i];
+            const V3 p = v3(sp.x, sp.y, sp.z);
+            int32_t h = -1;
+            if (in_unit_box(p)) {
+                h = bvh_closest(fr, p, stack + threadIdx.x, kDeformThreads, hint).idx;
+                hint = h;
+            }
+            s_hit[i] = h;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kDeformThreads) void deform_apply_kernel(MeshSet ms, int32_t frame,
+                                                                      const uint32_t* __restrict__ n_dev,
+                                                                      const float4* __restrict__ s_pos,
+                                                                      const float4* __restrict__ r_d,
+                                                                      const int32_t* __restrict__ s_hit,
+                                                                      float4* __restrict__ s_cpos,
+                                                                      float4* __restrict__ s_cdir) {
+    const uint32_t n = *n_dev;
+    for (uint32_t i = blockIdx.x * kDeformThreads + threadIdx.x; i < n; i += gridDim.x * kDeformThreads) {
         const float4 sp = s_pos[i];
         const float4 rd = r_d[__float_as_uint(sp.w)];
         const V3 p = v3(sp.x, sp.y, sp.z);
         const V3 dir_w = v3((rd.x + 1.0f) * 0.5f, (rd.y + 1.0f) * 0.5f, (rd.z + 1.0f) * 0.5f);
         V3 op = p, od = dir_w;
         uint32_t mask = 0;
-        if (frame >= 0)
-            warp_one(ms, (uint32_t)frame, p, dir_w, op, od, mask, stack + threadIdx.x, kDeformThreads, &hint);
+        const int32_t h = frame >= 0 ? s_hit[i] : -1;
+        if (h >= 0) warp_from_hit(ms, (uint32_t)frame, h, p, dir_w, op, od, mask);
         s_cpos[i] = make_float4(op.x, op.y, op.z, __uint_as_float(mask));
         s_cdir[i] = make_float4(od.x, od.y, od.z, 0.0f);
     }
+}
+
+uint32_t blocks_for(uint64_t items) {
+    const uint64_t b = (items + kDeformThreads - 1) / kDeformThreads;
+    return (uint32_t)(b < kMaxBlocks ? (b ? b : 1) : kMaxBlocks);
 }
 
 }  // namespace
@@ -220,7 +262,8 @@ __global__ __launch_bounds__(kDeformThreads) void deform_soa_kernel(MeshSet ms, 
 insta_status launch_deform_aos(insta_ctx* ctx, float* coords, uint32_t n, uint32_t stride, int32_t frame,
                                hipStream_t stream) {
     if (n == 0) return INSTA_OK;
-    hipLaunchKernelGGL(deform_aos_kernel, dim3((n + kDeformThreads - 1) / kDeformThreads), dim3(kDeformThreads), 0,
+    const size_t shmem = (size_t)stack_depth(ctx->bvh_k) * kDeformThreads * sizeof(uint32_t);
+    hipLaunchKernelGGL(deform_aos_kernel, dim3((n + kDeformThreads - 1) / kDeformThreads), dim3(kDeformThreads), shmem,
                        stream, mesh_set(ctx, ctx->d_built), coords, n, stride, frame, ctx->cfg.n_extra_dims);
     INSTA_HIP(hipGetLastError());
     return INSTA_OK;
@@ -229,10 +272,15 @@ insta_status launch_deform_aos(insta_ctx* ctx, float* coords, uint32_t n, uint32
 insta_status launch_deform_soa(insta_ctx* ctx, int32_t frame, const uint32_t* n_dev, uint32_t n_max,
                                hipStream_t stream) {
     if (n_max == 0) return INSTA_OK;
-    const uint32_t threads = (n_max + kChunk - 1) / kChunk;
-    hipLaunchKernelGGL(deform_soa_kernel, dim3((threads + kDeformThreads - 1) / kDeformThreads), dim3(kDeformThreads),
-                       0, stream, mesh_set(ctx, ctx->d_built), frame, n_dev, ctx->s_pos, ctx->r_d, ctx->s_cpos,
-                       ctx->s_cdir);
+    const MeshSet ms = mesh_set(ctx, ctx->d_built);
+    if (frame >= 0) {
+        const size_t shmem = (size_t)stack_depth(ctx->bvh_k) * kDeformThreads * sizeof(uint32_t);
+        hipLaunchKernelGGL(deform_hits_kernel, dim3(blocks_for((n_max + kChunk - 1) / kChunk)), dim3(kDeformThreads),
+                           shmem, stream, ms.view((uint32_t)frame + 1), n_dev, ctx->s_pos, ctx->s_hit);
+        INSTA_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(deform_apply_kernel, dim3(blocks_for(n_max)), dim3(kDeformThreads), 0, stream, ms, frame, n_dev,
+                       ctx->s_pos, ctx->r_d, ctx->s_hit, ctx->s_cpos, ctx->s_cdir);
     INSTA_HIP(hipGetLastError());
     return INSTA_OK;
 }

@@ -69,6 +69,8 @@ INSTA_DEV bool box_contains(const RenderArgs& a, V3 p) {
     return p.x >= a.lo[0] && p.x <= a.hi[0] && p.y >= a.lo[1] && p.y <= a.hi[1] && p.z >= a.lo[2] && p.z <= a.hi[2];
 }
 
+// density_grid_occupied_at for one cascade (testbed_nerf.cu:313-337), read
+// from the linear-order copy of the Morton bitfield (same bits).
 INSTA_DEV bool occupied(const uint8_t* __restrict__ grid, uint32_t G, V3 pos) {
     const float g = (float)G;
     const float px = ((pos.x - 0.5f) * 1.0f + 0.5f) * g;
@@ -79,7 +81,7 @@ INSTA_DEV bool occupied(const uint8_t* __restrict__ grid, uint32_t G, V3 pos) {
     ix = ix < 0 ? 0 : (ix > hi ? hi : ix);
     iy = iy < 0 ? 0 : (iy > hi ? hi : iy);
     iz = iz < 0 ? 0 : (iz > hi ? hi : iz);
-    const uint32_t idx = morton3((uint32_t)ix, (uint32_t)iy, (uint32_t)iz);
+    const uint32_t idx = (uint32_t)ix + G * ((uint32_t)iy + G * (uint32_t)iz);
     return (grid[idx >> 3] >> (idx & 7u)) & 1u;
 }
 
@@ -190,49 +192,52 @@ __global__ __launch_bounds__(kRayThreads) void init_rays_kernel(RenderArgs a, fl
     // counters[kCntAlive0] is derived from the first live count on the host
 }
 
+// One walk per ray: the t of each emitted sample goes to step-major scratch
+// (t_buf[j * n_alive + k], coalesced across the wave; K * n_alive fits the
+// sample buffer), then the wave reserves contiguous output space and each
+// lane writes its samples from the stored t values (pos = o + d t exactly as
+// during the walk).
 __global__ __launch_bounds__(kRayThreads) void march_kernel(RenderArgs a, const uint32_t* __restrict__ alive,
                                                             uint32_t n_alive, uint32_t K, float4* __restrict__ r_o,
                                                             const float4* __restrict__ r_d,
                                                             const uint32_t* __restrict__ r_steps,
                                                             uint32_t* __restrict__ r_flags,
                                                             uint32_t* __restrict__ r_off, uint32_t* __restrict__ r_cnt,
-                                                            float4* __restrict__ s_pos,
+                                                            float* __restrict__ t_buf, float4* __restrict__ s_pos,
                                                             uint32_t* __restrict__ counters) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = k < n_alive;
     uint32_t r = 0, count = 0;
     bool exited = false;
-    V3 o = v3(0, 0, 0), d = v3(0, 0, 1), idir = v3(0, 0, 1);
-    float t0 = 0.0f;
+    V3 o = v3(0, 0, 0), d = v3(0, 0, 1);
+    float t = 0.0f;
     if (valid) {
         r = alive[k];
         const float4 ro = r_o[r], rd = r_d[r];
         o = v3(ro.x, ro.y, ro.z);
         d = v3(rd.x, rd.y, rd.z);
-        idir = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-        t0 = ro.w;
+        const V3 idir = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+        t = ro.w;
         const uint32_t done = r_steps[r];
         const uint32_t budget = a.max_steps > done ? a.max_steps - done : 0u;
         const uint32_t kmax = K < budget ? K : budget;
-        float t = t0;
         V3 pos;
         while (count < kmax) {
             if (!skip_empty(a, o, d, idir, t, pos)) {
                 exited = true;
                 break;
             }
+            t_buf[(size_t)count * n_alive + k] = t;
             ++count;
             t += a.dt;
         }
     }
     const uint32_t base = wave_reserve(&counters[kCntSamples], count);
     if (!valid) return;
-    float t = t0;
-    V3 pos;
     for (uint32_t j = 0; j < count; ++j) {
-        skip_empty(a, o, d, idir, t, pos);
+        const float tj = t_buf[(size_t)j * n_alive + k];
+        const V3 pos = o + d * tj;
         s_pos[base + j] = make_float4(pos.x, pos.y, pos.z, __uint_as_float(r));
-        t += a.dt;
     }
     r_off[k] = base;
     r_cnt[k] = count;
@@ -389,6 +394,7 @@ insta_status ensure_workspace(insta_ctx* ctx, uint32_t n_pixels) {
     dev_free(ctx->r_steps); dev_free(ctx->r_flags); dev_free(ctx->alive[0]); dev_free(ctx->alive[1]);
     dev_free(ctx->r_off); dev_free(ctx->r_cnt);
     dev_free(ctx->s_pos); dev_free(ctx->s_cpos); dev_free(ctx->s_cdir); dev_free(ctx->s_out);
+    dev_free(ctx->s_hit);
     ctx->ws_pixels = 0;
     const size_t n = n_pixels;
     // sample buffer: 128 samples per pixel, between 1M and 64M samples (4 GiB)
@@ -414,6 +420,7 @@ insta_status ensure_workspace(insta_ctx* ctx, uint32_t n_pixels) {
     INSTA_ALLOC(ctx->s_cpos, cap * sizeof(float4));
     INSTA_ALLOC(ctx->s_cdir, cap * sizeof(float4));
     INSTA_ALLOC(ctx->s_out, cap * sizeof(float4));
+    INSTA_ALLOC(ctx->s_hit, cap * sizeof(int32_t));
 #undef INSTA_ALLOC
     ctx->ws_pixels = n_pixels;
     ctx->ws_samples = cap;
@@ -449,7 +456,7 @@ insta_status render(insta_ctx* ctx, const insta_camera* cam, uint32_t width, uin
         a.lo[i] = ctx->render_lo[i];
         a.hi[i] = ctx->render_hi[i];
     }
-    a.grid = ctx->grid(occupancy < 0 ? 0u : (uint32_t)occupancy + 1u);
+    a.grid = ctx->grid_lin(occupancy < 0 ? 0u : (uint32_t)occupancy + 1u);
     uint32_t* cnt = ctx->d_counters;
     hipLaunchKernelGGL(clear_counters_kernel, dim3(1), dim3(64), 0, stream, cnt, 0u, (uint32_t)kNumCounters);
     prof_begin(ctx, INSTA_KERNEL_INIT, stream);
@@ -477,7 +484,8 @@ insta_status render(insta_ctx* ctx, const insta_camera* cam, uint32_t width, uin
         const dim3 rgrid((n_alive + kRayThreads - 1) / kRayThreads);
         prof_begin(ctx, INSTA_KERNEL_MARCH, stream);
         hipLaunchKernelGGL(march_kernel, rgrid, dim3(kRayThreads), 0, stream, a, ctx->alive[cur], n_alive, (uint32_t)K,
-                           ctx->r_o, ctx->r_d, ctx->r_steps, ctx->r_flags, ctx->r_off, ctx->r_cnt, ctx->s_pos, cnt);
+                           ctx->r_o, ctx->r_d, ctx->r_steps, ctx->r_flags, ctx->r_off, ctx->r_cnt,
+                           reinterpret_cast<float*>(ctx->s_hit), ctx->s_pos, cnt);
         INSTA_HIP(hipGetLastError());
         prof_end(ctx, INSTA_KERNEL_MARCH, n_alive, stream);
         const uint64_t n_max64 = (uint64_t)n_alive * K;
@@ -545,6 +553,7 @@ void free_workspace(insta_ctx* ctx) {
     dev_free(ctx->r_steps); dev_free(ctx->r_flags); dev_free(ctx->alive[0]); dev_free(ctx->alive[1]);
     dev_free(ctx->r_off); dev_free(ctx->r_cnt);
     dev_free(ctx->s_pos); dev_free(ctx->s_cpos); dev_free(ctx->s_cdir); dev_free(ctx->s_out);
+    dev_free(ctx->s_hit);
     ctx->ws_pixels = 0;
 }
 
