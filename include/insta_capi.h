@@ -233,6 +233,64 @@ insta_status insta_render(insta_ctx* ctx, const insta_camera* cam, uint32_t widt
 /* Counters of the last insta_render on this context. */
 insta_status insta_get_render_stats(insta_ctx* ctx, insta_render_stats* out);
 
+/* ---- training step --------------------------------------------------- */
+
+/* Training set: n_images RGBA float images, rgba_dev [n][height][width][4]
+ * (colour in [0,1]; alpha 1 = opaque, else composited over the random
+ * background), image i seen by cameras[i] (host array) through posed frame
+ * frames[i] (host array; that frame's mesh and occupancy slot must be built
+ * before training).  The context keeps its own copy of the images.  Stands
+ * for the NeRF dataset load + Testbed::load_training_data
+ * (instant-ngp/src/nerf_loader.cu:405-540, testbed.cu:2951-2980). */
+insta_status insta_set_training_set(insta_ctx* ctx, uint32_t n_images, uint32_t width, uint32_t height,
+                                    const float* rgba_dev, const insta_camera* cameras,
+                                    const int32_t* frames, void* stream);
+
+typedef struct insta_train_config {
+    uint32_t n_rays;          /* rays this step; 0 = adaptive count (target_samples) */
+    uint32_t target_samples;  /* samples per step the adaptive ray count aims at */
+    uint32_t max_samples;     /* cap on samples marched per step */
+    float    learning_rate;   /* Adam */
+    float    beta1, beta2, epsilon;
+    float    loss_scale;      /* gradients are computed scaled by this, then unscaled */
+    uint32_t grid_grad_fp16;  /* 1: hash-grid gradients accumulated in FP16 (packed atomics) */
+    uint32_t apply_update;    /* 0: gradients only, parameters unchanged */
+    uint32_t seed;            /* PCG32 seed of the ray/background stream */
+} insta_train_config;
+
+typedef struct insta_train_stats {
+    uint32_t n_rays;          /* rays generated */
+    uint32_t n_rays_used;     /* rays that collected samples (and fit the cap) */
+    uint32_t n_samples;       /* samples marched, deformed and evaluated */
+    uint32_t n_samples_used;  /* samples composited before transmittance < 1e-4 */
+    float    loss;            /* mean Huber loss per ray (unscaled) */
+    uint32_t step;            /* optimizer steps taken so far */
+} insta_train_stats;
+
+/* Defaults: adaptive rays aiming at 2^18 samples, Adam (0.9, 0.99, 1e-15) at
+ * learning rate 1e-2, FP32 grid gradients, loss scale 1, seed 1337. */
+void insta_default_train_config(insta_train_config* tc);
+
+/* One training step, Testbed::train_nerf_step (testbed_nerf.cu:3176-3400):
+ * rays from random training pixels (generate_training_samples_nerf,
+ * 1105-1285) marched through their frame's occupancy grid, deformed to
+ * canonical space (the warp hook, core.cu:442-526), network forward, loss and
+ * dL/doutput (compute_loss_kernel_train_nerf, 1305-1643: Huber/5 at 0.1,
+ * product transmittance, random background), hand-written backward through
+ * both MLPs and the hash grid (nerf_network.h:296-383), Adam.  Reads one
+ * block of counters back to the host (the adaptive ray count, stats). */
+insta_status insta_train_step(insta_ctx* ctx, const insta_train_config* tc, insta_train_stats* stats,
+                              void* stream);
+
+/* dL/dparams of the last step (loss scale removed), layout.n_params floats. */
+insta_status insta_get_gradients(insta_ctx* ctx, float* grads_dev, void* stream);
+
+/* Current FP32 parameters (layout.n_params), e.g. after training steps. */
+insta_status insta_get_params(insta_ctx* ctx, float* params_dev, void* stream);
+
+/* Zero the Adam moments and step count. */
+insta_status insta_reset_optimizer(insta_ctx* ctx);
+
 /* Per-kernel timing of insta_render, measured with HIP events recorded on the
  * render stream around each kernel class.  No counterpart in the reference
  * (whose only timers are EMA frame times, testbed.cu:1600, 2484-2512); it

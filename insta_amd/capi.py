@@ -67,6 +67,18 @@ class InstaLayout(C.Structure):
 KERNEL_CLASSES = ["init", "march", "deform", "network", "composite", "finalize"]
 
 
+class InstaTrainConfig(C.Structure):
+    _fields_ = [("n_rays", C.c_uint32), ("target_samples", C.c_uint32), ("max_samples", C.c_uint32),
+                ("learning_rate", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("epsilon", C.c_float),
+                ("loss_scale", C.c_float), ("grid_grad_fp16", C.c_uint32), ("apply_update", C.c_uint32),
+                ("seed", C.c_uint32)]
+
+
+class InstaTrainStats(C.Structure):
+    _fields_ = [("n_rays", C.c_uint32), ("n_rays_used", C.c_uint32), ("n_samples", C.c_uint32),
+                ("n_samples_used", C.c_uint32), ("loss", C.c_float), ("step", C.c_uint32)]
+
+
 class InstaProfile(C.Structure):
     _fields_ = [("ms", C.c_double * 8), ("launches", C.c_uint64 * 8), ("items", C.c_uint64 * 8)]
 
@@ -97,6 +109,12 @@ SIGNATURES = [
     ("insta_get_render_stats", C.c_int, [_VP, C.POINTER(InstaRenderStats)]),
     ("insta_set_profiling", C.c_int, [_VP, C.c_int]),
     ("insta_get_profile", C.c_int, [_VP, C.POINTER(InstaProfile)]),
+    ("insta_set_training_set", C.c_int, [_VP, C.c_uint32, C.c_uint32, C.c_uint32, _VP, _VP, _VP, _VP]),
+    ("insta_default_train_config", None, [C.POINTER(InstaTrainConfig)]),
+    ("insta_train_step", C.c_int, [_VP, C.POINTER(InstaTrainConfig), C.POINTER(InstaTrainStats), _VP]),
+    ("insta_get_gradients", C.c_int, [_VP, _VP, _VP]),
+    ("insta_get_params", C.c_int, [_VP, _VP, _VP]),
+    ("insta_reset_optimizer", C.c_int, [_VP]),
 ]
 
 _lib = None
@@ -324,7 +342,42 @@ class Context:
         return {name: dict(ms=p.ms[i], launches=p.launches[i], items=p.items[i])
                 for i, name in enumerate(KERNEL_CLASSES)}
 
+    # --- training ---------------------------------------------------------
+    def set_training_set(self, images_dev, cams, frames) -> None:
+        """images_dev [n, h, w, 4] float32 on the device; cams: InstaCamera per image; frames: ints."""
+        n, h, w, _ = images_dev.shape
+        arr = (InstaCamera * n)(*cams)
+        fr = np.ascontiguousarray(frames, dtype=np.int32)
+        check(self.lib.insta_set_training_set(self.h, n, w, h, _ptr(images_dev.contiguous()), C.addressof(arr),
+                                              fr.ctypes.data, self.stream()), "insta_set_training_set")
+
+    def train_step(self, tc: "InstaTrainConfig") -> InstaTrainStats:
+        st = InstaTrainStats()
+        check(self.lib.insta_train_step(self.h, C.byref(tc), C.byref(st), self.stream()), "insta_train_step")
+        return st
+
+    def gradients(self):
+        g = self.torch.empty(self.layout.n_params, dtype=self.torch.float32, device="cuda")
+        check(self.lib.insta_get_gradients(self.h, _ptr(g), self.stream()), "insta_get_gradients")
+        return g
+
+    def params(self):
+        p = self.torch.empty(self.layout.n_params, dtype=self.torch.float32, device="cuda")
+        check(self.lib.insta_get_params(self.h, _ptr(p), self.stream()), "insta_get_params")
+        return p
+
+    def reset_optimizer(self) -> None:
+        check(self.lib.insta_reset_optimizer(self.h), "insta_reset_optimizer")
+
     def render_stats(self) -> InstaRenderStats:
         st = InstaRenderStats()
         check(self.lib.insta_get_render_stats(self.h, C.byref(st)), "insta_get_render_stats")
         return st
+
+
+def default_train_config(**kw) -> InstaTrainConfig:
+    tc = InstaTrainConfig()
+    load_library().insta_default_train_config(C.byref(tc))
+    for k, v in kw.items():
+        setattr(tc, k, v)
+    return tc

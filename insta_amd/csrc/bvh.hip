@@ -361,7 +361,7 @@ __global__ __launch_bounds__(kQueryThreads) void occupancy_kernel(MeshView m, ui
         const uint32_t x = compact3(idx), y = compact3(idx >> 1), z = compact3(idx >> 2);
         V3 pos = v3((((float)x + 0.5f) / g - 0.5f) * 1.0f + 0.5f, (((float)y + 0.5f) / g - 0.5f) * 1.0f + 0.5f,
                     (((float)z + 0.5f) / g - 0.5f) * 1.0f + 0.5f);
-        Hit h = bvh_closest(m, pos, stack + threadIdx.x, kQueryThreads, -1, bound);
+        Hit h = bvh_closest(m, pos, stack + threadIdx.x, kQueryThreads, bound);
         if (h.idx < 0) continue;
         const float dist = sqrtf(h.dist_sq);
         const uint32_t mask = __float_as_uint(m.btris[3 * h.idx + 2].z);

@@ -240,6 +240,7 @@ insta_status insta_create(const insta_config* cfg, insta_ctx** out) {
 void insta_destroy(insta_ctx* ctx) {
     if (!ctx) return;
     insta::free_workspace(ctx);
+    insta::free_train(ctx);
     using insta::dev_free;
     dev_free(ctx->d_faces); dev_free(ctx->d_masks); dev_free(ctx->d_adj); dev_free(ctx->d_verts);
     dev_free(ctx->d_nodes); dev_free(ctx->d_btris); dev_free(ctx->d_proj); dev_free(ctx->d_proj_inv);
@@ -572,3 +573,98 @@ insta_status insta_get_profile(insta_ctx* ctx, insta_profile* out) {
 }
 
 }  // extern "C"
+
+void insta_default_train_config(insta_train_config* tc) {
+    if (!tc) return;
+    tc->n_rays = 0;
+    tc->target_samples = 1u << 18;
+    tc->max_samples = 1u << 20;
+    tc->learning_rate = 1e-2f;
+    tc->beta1 = 0.9f;
+    tc->beta2 = 0.99f;
+    tc->epsilon = 1e-15f;
+    tc->loss_scale = 1.0f;
+    tc->grid_grad_fp16 = 0;
+    tc->apply_update = 1;
+    tc->seed = 1337u;
+}
+
+insta_status insta_set_training_set(insta_ctx* ctx, uint32_t n_images, uint32_t width, uint32_t height,
+                                    const float* rgba_dev, const insta_camera* cameras, const int32_t* frames,
+                                    void* stream) {
+    INSTA_CHECK_CTX(ctx);
+    if (n_images == 0 || width == 0 || height == 0 || !rgba_dev || !cameras || !frames ||
+        (uint64_t)n_images * width * height > (1ull << 32)) {
+        set_error("set_training_set: bad sizes or null pointer");
+        return INSTA_ERR_INVALID;
+    }
+    for (uint32_t i = 0; i < n_images; ++i) {
+        const int32_t f = frames[i];
+        if (f < 0 || f >= (int32_t)ctx->cfg.max_frames || !ctx->slot_built[f + 1] || !ctx->grid_built[f + 1]) {
+            set_error("set_training_set: an image refers to a frame whose mesh or occupancy grid is not built");
+            return INSTA_ERR_STATE;
+        }
+        if (!(cameras[i].focal_px[0] > 0.0f) || !(cameras[i].focal_px[1] > 0.0f)) {
+            set_error("set_training_set: focal lengths must be positive");
+            return INSTA_ERR_INVALID;
+        }
+    }
+    if (!insta::train_supported(ctx->cfg)) {
+        set_error("set_training_set: training supports width-64 MLPs with 1 density / 2 colour hidden layers, "
+                  "F = 2, L in {8, 16}, no extra dims");
+        return INSTA_ERR_UNSUPPORTED;
+    }
+    return insta::set_training_set(ctx, n_images, width, height, rgba_dev, cameras, frames, (hipStream_t)stream);
+}
+
+insta_status insta_train_step(insta_ctx* ctx, const insta_train_config* tc, insta_train_stats* stats, void* stream) {
+    INSTA_CHECK_CTX(ctx);
+    if (!tc || tc->max_samples == 0 || tc->max_samples > (1u << 26) || tc->n_rays > (1u << 24) ||
+        !(tc->loss_scale > 0.0f) || !(tc->learning_rate >= 0.0f) || (tc->n_rays == 0 && tc->target_samples == 0)) {
+        set_error("train_step: bad training configuration");
+        return INSTA_ERR_INVALID;
+    }
+    if (!insta::train_has_set(ctx)) {
+        set_error("train_step: no training set");
+        return INSTA_ERR_STATE;
+    }
+    if (!ctx->params_set) {
+        set_error("train_step: parameters not set");
+        return INSTA_ERR_STATE;
+    }
+    return insta::train_step(ctx, *tc, stats, (hipStream_t)stream);
+}
+
+insta_status insta_get_gradients(insta_ctx* ctx, float* grads_dev, void* stream) {
+    INSTA_CHECK_CTX(ctx);
+    if (!grads_dev) {
+        set_error("get_gradients: null pointer");
+        return INSTA_ERR_INVALID;
+    }
+    if (!insta::train_has_gradients(ctx)) {
+        set_error("get_gradients: no training step has run");
+        return INSTA_ERR_STATE;
+    }
+    return insta::get_gradients(ctx, grads_dev, (hipStream_t)stream);
+}
+
+insta_status insta_get_params(insta_ctx* ctx, float* params_dev, void* stream) {
+    INSTA_CHECK_CTX(ctx);
+    if (!params_dev) {
+        set_error("get_params: null pointer");
+        return INSTA_ERR_INVALID;
+    }
+    if (!ctx->params_set) {
+        set_error("get_params: parameters not set");
+        return INSTA_ERR_STATE;
+    }
+    INSTA_HIP(hipMemcpyAsync(params_dev, ctx->d_params, ctx->layout.n_params * sizeof(float),
+                             hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    return INSTA_OK;
+}
+
+insta_status insta_reset_optimizer(insta_ctx* ctx) {
+    INSTA_CHECK_CTX(ctx);
+    if (!ctx->train) return INSTA_OK;
+    return insta::reset_optimizer(ctx);
+}

@@ -78,6 +78,51 @@ INSTA_HD uint32_t compact3(uint32_t v) {
 }
 INSTA_HD uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) { return spread3(x) | (spread3(y) << 1) | (spread3(z) << 2); }
 
+// PCG32 (O'Neill 2014, the pcg32 generator of Jakob's pcg32.h that
+// instant-ngp uses as default_rng_t): 64-bit LCG state, XSH-RR output,
+// O(log n) jump-ahead.  Host and device produce the same stream.
+struct Pcg32 {
+    static constexpr uint64_t kMult = 0x5851f42d4c957f2dull;
+    static constexpr uint64_t kDefaultState = 0x853c49e6748fea9bull;
+    static constexpr uint64_t kDefaultStream = 0xda3e39cb94b95bdbull;
+    uint64_t state, inc;
+
+    INSTA_HD static Pcg32 seeded(uint64_t initstate, uint64_t initseq = kDefaultStream) {
+        Pcg32 r{0u, (initseq << 1u) | 1u};
+        r.next_uint();
+        r.state += initstate;
+        r.next_uint();
+        return r;
+    }
+    INSTA_HD uint32_t next_uint() {
+        const uint64_t old = state;
+        state = old * kMult + inc;
+        const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        const uint32_t rot = (uint32_t)(old >> 59u);
+        return (xs >> rot) | (xs << ((~rot + 1u) & 31u));
+    }
+    // uniform in [0, 1): 23 random mantissa bits
+    INSTA_HD float next_float() {
+        const uint32_t u = (next_uint() >> 9) | 0x3f800000u;
+        float f;
+        __builtin_memcpy(&f, &u, 4);
+        return f - 1.0f;
+    }
+    INSTA_HD void advance(uint64_t delta) {
+        uint64_t cur_mult = kMult, cur_plus = inc, acc_mult = 1u, acc_plus = 0u;
+        while (delta > 0) {
+            if (delta & 1u) {
+                acc_mult *= cur_mult;
+                acc_plus = acc_plus * cur_mult + cur_plus;
+            }
+            cur_plus = (cur_mult + 1u) * cur_plus;
+            cur_mult *= cur_mult;
+            delta /= 2u;
+        }
+        state = acc_mult * state + acc_plus;
+    }
+};
+
 // ---------------------------------------------------------------------------
 // Mesh storage.  Slot 0 is the canonical mesh, slot f+1 posed frame f.  All
 // slots share the topology, so the BVH shape (levels, node count) is common.
@@ -133,23 +178,12 @@ INSTA_DEV uint32_t pop_live(const uint32_t* stack, int& sp, uint32_t stride, flo
     return kNoNode;
 }
 
-INSTA_DEV Hit bvh_closest(const MeshView& m, V3 p, uint32_t* stack, uint32_t stride, int32_t hint = -1,
+INSTA_DEV Hit bvh_closest(const MeshView& m, V3 p, uint32_t* stack, uint32_t stride,
                           float max_dist_sq = 3.40282347e38f) {
     // max_dist_sq (bvh.cuh:48): only triangles closer than this are reported
     float best = max_dist_sq;
     int32_t best_idx = -1;
     uint32_t best_orig = 0xffffffffu;
-    if (hint >= 0) {
-        // a known triangle (e.g. the previous sample's nearest) bounds the
-        // search from the start; it competes under the same (dist, id) order,
-        // so the result is unchanged, only fewer boxes survive the prune
-        const float4 r0 = m.btris[3 * hint + 0];
-        const float4 r1 = m.btris[3 * hint + 1];
-        const float4 r2 = m.btris[3 * hint + 2];
-        best = tri_distance_sq(v3(r0.x, r0.y, r0.z), v3(r0.w, r1.x, r1.y), v3(r1.z, r1.w, r2.x), p);
-        best_idx = hint;
-        best_orig = __float_as_uint(r2.y);
-    }
     float bound = prune_bound(best);
     const uint32_t shift = 2u * m.k;
     int sp = 0;

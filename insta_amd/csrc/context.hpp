@@ -9,6 +9,10 @@
 
 #include "common.hpp"
 
+namespace insta {
+struct TrainState;
+}
+
 struct insta_ctx {
     insta_config cfg{};
     insta_layout layout{};
@@ -64,6 +68,9 @@ struct insta_ctx {
     uint32_t* d_counters = nullptr; // [8]: alive counts, sample counter, stats
     uint32_t* h_counters = nullptr; // pinned mirror
     insta_render_stats last_stats{};
+
+    // training state (training set, workspace, gradients, Adam moments)
+    insta::TrainState* train = nullptr;
 
     // optional per-kernel-class event timing of insta_render
     bool profiling = false;
@@ -141,8 +148,23 @@ insta_status launch_deform_aos(insta_ctx* ctx, float* coords, uint32_t n, uint32
                                hipStream_t stream);
 insta_status launch_deform_soa(insta_ctx* ctx, int32_t frame, const uint32_t* n_dev, uint32_t n_max,
                                hipStream_t stream);
+// samples s_pos (xyz, ray id bits) with ray directions r_d; frame per ray
+// from ray_frame when non-null
+insta_status launch_deform_samples(insta_ctx* ctx, int32_t frame, const int32_t* ray_frame, const uint32_t* n_dev,
+                                   uint32_t n_max, const float4* s_pos, const float4* r_d, int32_t* s_hit,
+                                   float4* s_cpos, float4* s_cdir, hipStream_t stream);
 insta_status launch_occupancy(insta_ctx* ctx, uint32_t slot, float threshold, hipStream_t stream);
 insta_status launch_grid_linear(insta_ctx* ctx, uint32_t slot, hipStream_t stream);
+// training (train.hip)
+bool train_supported(const insta_config& cfg);
+void free_train(insta_ctx* ctx);
+insta_status set_training_set(insta_ctx* ctx, uint32_t n_images, uint32_t width, uint32_t height, const float* rgba,
+                              const insta_camera* cams, const int32_t* frames, hipStream_t stream);
+insta_status train_step(insta_ctx* ctx, const insta_train_config& tc, insta_train_stats* stats, hipStream_t stream);
+bool train_has_gradients(const insta_ctx* ctx);
+bool train_has_set(const insta_ctx* ctx);
+insta_status get_gradients(insta_ctx* ctx, float* out, hipStream_t stream);
+insta_status reset_optimizer(insta_ctx* ctx);
 insta_status launch_encode(insta_ctx* ctx, const float* pos, uint32_t n, float* out, hipStream_t stream);
 insta_status launch_encode_backward(insta_ctx* ctx, const float* pos, uint32_t n, const float* dL, float* grad,
                                     hipStream_t stream);

@@ -194,43 +194,45 @@ __global__ __launch_bounds__(kDeformThreads) void deform_aos_kernel(MeshSet ms, 
 }
 
 // Render samples (SoA) in two kernels:
-//  hits:  nearest triangle per posed sample (s_pos) into s_hit; each thread
-//         takes kChunk consecutive samples (consecutive samples of a ray lie
-//         dt apart) and seeds every query with the previous one's triangle.
-//         Kept apart from the transforms so the traversal runs at low
-//         register count, i.e. with many waves resident to cover its chain
-//         of dependent node loads.
+//  hits:  nearest triangle per posed sample (s_pos) into s_hit, one sample
+//         per lane: neighbouring lanes take neighbouring samples of a ray,
+//         so a wave's traversals follow nearly the same node sequence.  Kept
+//         apart from the transforms so the traversal runs at low register
+//         count, i.e. with many waves resident to cover its chain of
+//         dependent node loads.  (Seeding a query with the previous sample's
+//         triangle measured slower: the descent reaches a tight bound on its
+//         first leaf anyway, and chunking samples per lane breaks the
+//         coherence above.)
 //  apply: per sample, the frame transforms of `warp` from the hit; writes
 //         canonical position (+ mask bits) to s_cpos and the warped
 //         direction to s_cdir.
-constexpr uint32_t kChunk = 2;
 constexpr uint32_t kMaxBlocks = 8192;
 
-__global__ __launch_bounds__(kDeformThreads) void deform_hits_kernel(MeshView fr, const uint32_t* __restrict__ n_dev,
+// Frame of a sample: the per-ray table when given (training batches mix
+// frames), else the call's frame; -1 leaves the sample undeformed.
+INSTA_DEV int32_t sample_frame(int32_t frame, const int32_t* __restrict__ ray_frame, float4 sp) {
+    return ray_frame ? ray_frame[__float_as_uint(sp.w)] : frame;
+}
+
+__global__ __launch_bounds__(kDeformThreads) void deform_hits_kernel(MeshSet ms, int32_t frame,
+                                                                     const int32_t* __restrict__ ray_frame,
+                                                                     const uint32_t* __restrict__ n_dev,
                                                                      const float4* __restrict__ s_pos,
                                                                      int32_t* __restrict__ s_hit) {
     extern __shared__ uint32_t stack[];  // [stack_depth(k)][kDeformThreads]
     const uint32_t n = *n_dev;
-    const uint32_t step = gridDim.x * kDeformThreads * kChunk;
-    for (uint32_t first = (blockIdx.x * kDeformThreads + threadIdx.x) * kChunk; first < n; first += step) {
-        int32_t hint = -1;
-#pragma unroll
-        for (uint32_t j = 0; j < kChunk; ++j) {
-            const uint32_t i = first + j;
-            if (i >= n) break;
-            const float4 sp = s_pos[i];
-            const V3 p = v3(sp.x, sp.y, sp.z);
-            int32_t h = -1;
-            if (in_unit_box(p)) {
-                h = bvh_closest(fr, p, stack + threadIdx.x, kDeformThreads, hint).idx;
-                hint = h;
-            }
-            s_hit[i] = h;
-        }
+    for (uint32_t i = blockIdx.x * kDeformThreads + threadIdx.x; i < n; i += gridDim.x * kDeformThreads) {
+        const float4 sp = s_pos[i];
+        const V3 p = v3(sp.x, sp.y, sp.z);
+        const int32_t f = sample_frame(frame, ray_frame, sp);
+        s_hit[i] = f >= 0 && in_unit_box(p)
+                       ? bvh_closest(ms.view((uint32_t)f + 1), p, stack + threadIdx.x, kDeformThreads).idx
+                       : -1;
     }
 }
 
 __global__ __launch_bounds__(kDeformThreads) void deform_apply_kernel(MeshSet ms, int32_t frame,
+                                                                      const int32_t* __restrict__ ray_frame,
                                                                       const uint32_t* __restrict__ n_dev,
                                                                       const float4* __restrict__ s_pos,
                                                                       const float4* __restrict__ r_d,
@@ -245,8 +247,9 @@ __global__ __launch_bounds__(kDeformThreads) void deform_apply_kernel(MeshSet ms
         const V3 dir_w = v3((rd.x + 1.0f) * 0.5f, (rd.y + 1.0f) * 0.5f, (rd.z + 1.0f) * 0.5f);
         V3 op = p, od = dir_w;
         uint32_t mask = 0;
-        const int32_t h = frame >= 0 ? s_hit[i] : -1;
-        if (h >= 0) warp_from_hit(ms, (uint32_t)frame, h, p, dir_w, op, od, mask);
+        const int32_t f = sample_frame(frame, ray_frame, sp);
+        const int32_t h = f >= 0 ? s_hit[i] : -1;
+        if (h >= 0) warp_from_hit(ms, (uint32_t)f, h, p, dir_w, op, od, mask);
         s_cpos[i] = make_float4(op.x, op.y, op.z, __uint_as_float(mask));
         s_cdir[i] = make_float4(od.x, od.y, od.z, 0.0f);
     }
@@ -269,20 +272,27 @@ insta_status launch_deform_aos(insta_ctx* ctx, float* coords, uint32_t n, uint32
     return INSTA_OK;
 }
 
-insta_status launch_deform_soa(insta_ctx* ctx, int32_t frame, const uint32_t* n_dev, uint32_t n_max,
-                               hipStream_t stream) {
+insta_status launch_deform_samples(insta_ctx* ctx, int32_t frame, const int32_t* ray_frame, const uint32_t* n_dev,
+                                   uint32_t n_max, const float4* s_pos, const float4* r_d, int32_t* s_hit,
+                                   float4* s_cpos, float4* s_cdir, hipStream_t stream) {
     if (n_max == 0) return INSTA_OK;
     const MeshSet ms = mesh_set(ctx, ctx->d_built);
-    if (frame >= 0) {
+    if (frame >= 0 || ray_frame) {
         const size_t shmem = (size_t)stack_depth(ctx->bvh_k) * kDeformThreads * sizeof(uint32_t);
-        hipLaunchKernelGGL(deform_hits_kernel, dim3(blocks_for((n_max + kChunk - 1) / kChunk)), dim3(kDeformThreads),
-                           shmem, stream, ms.view((uint32_t)frame + 1), n_dev, ctx->s_pos, ctx->s_hit);
+        hipLaunchKernelGGL(deform_hits_kernel, dim3(blocks_for(n_max)), dim3(kDeformThreads), shmem, stream, ms, frame,
+                           ray_frame, n_dev, s_pos, s_hit);
         INSTA_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(deform_apply_kernel, dim3(blocks_for(n_max)), dim3(kDeformThreads), 0, stream, ms, frame, n_dev,
-                       ctx->s_pos, ctx->r_d, ctx->s_hit, ctx->s_cpos, ctx->s_cdir);
+    hipLaunchKernelGGL(deform_apply_kernel, dim3(blocks_for(n_max)), dim3(kDeformThreads), 0, stream, ms, frame,
+                       ray_frame, n_dev, s_pos, r_d, s_hit, s_cpos, s_cdir);
     INSTA_HIP(hipGetLastError());
     return INSTA_OK;
+}
+
+insta_status launch_deform_soa(insta_ctx* ctx, int32_t frame, const uint32_t* n_dev, uint32_t n_max,
+                               hipStream_t stream) {
+    return launch_deform_samples(ctx, frame, nullptr, n_dev, n_max, ctx->s_pos, ctx->r_d, ctx->s_hit, ctx->s_cpos,
+                                 ctx->s_cdir, stream);
 }
 
 }  // namespace insta

@@ -1,0 +1,805 @@
+// train.hip -- one training step of the avatar network on the device.
+//
+// Restates Testbed::train_nerf_step (instant-ngp/src/testbed_nerf.cu:
+// 3176-3400) for the INSTA path, as this sequence of kernels on one stream:
+//   rays      generate_training_samples_nerf (1105-1285): per ray a training
+//             image ((i + rays_total) * n_images / n_rays, 1082-1103) and a
+//             PCG32 stream advanced by 8 draws per ray (1143): pixel xy, a
+//             draw the reference spends on motion blur, the start jitter;
+//             march through the image frame's occupancy grid from
+//             max(t_box, 0) + dt * jitter, at most max_steps samples; a ray
+//             whose sample range would cross the cap is dropped (1239-1242)
+//   deform    posed -> canonical with the frame of each ray (core.cu:442-526)
+//   forward   hash grid + density MLP + colour MLP per sample (the FP32 path
+//             of the inference kernels), keeping every activation the
+//             backward needs, feature-major [feature][sample]
+//   loss      compute_loss_kernel_train_nerf (1305-1643): product
+//             transmittance T *= 1 - alpha until T < 1e-4, random background
+//             (the same PCG32 stream: draws 2-4 of the ray), Huber(0.1) / 5
+//             per channel, dL/d(rgb logit) = w * dL/drgb * s(1 - s),
+//             dL/d(density logit) = sigma * dt * dL/drgb . (T rgb - suffix),
+//             scaled by loss_scale / n_rays
+//   backward  per sample through both MLPs (ReLU masks from the stored
+//             activations) into dL/dencoding, then scattered into the hash
+//             tables: FP32 atomics, or packed FP16 atomics (one
+//             global_atomic_pk_add_f16 per corner and level, F = 2)
+//   wgrad     weight gradients dW = sum_n delta_n x_n^T as per-chunk partial
+//             products (LDS tiles, 4x4 register blocks), then a fixed-order
+//             sum over chunks: deterministic MLP gradients
+//   adam      Kingma & Ba with bias correction over all parameters, after
+//             removing the loss scale
+// Declared simplifications against the reference: no loss compaction to a
+// fixed batch (every sample is back-propagated, those past the termination
+// with zero gradient), no density/colour regularisers, no error map, no
+// extra dims, background not passed through sRGB<->linear.
+#include <hip/hip_fp16.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "context.hpp"
+#include "encoding.hpp"
+#include "march.hpp"
+
+namespace insta {
+
+struct TrainImage {
+    float M[12];
+    float fx, fy, cx, cy;
+    int32_t frame;
+    uint32_t pad[3];
+};
+
+enum : uint32_t {
+    kTRays = 0,      // rays kept (ray list length)
+    kTSamples = 1,   // samples reserved
+    kTLimit = 2,     // first sample of the ray that crossed the cap
+    kTValid = 3,     // samples marched into the buffers
+    kTUsed = 4,      // samples composited before termination
+    kTLoss = 5,      // float bits: sum of per-ray loss / n_rays
+    kTCounters = 8
+};
+
+constexpr int kW = 64;
+constexpr int kDOut = 16;
+constexpr int kCin = 32;             // density outputs + 16 SH
+constexpr uint32_t kTrainThreads = 128;
+constexpr uint32_t kChunk = 1024;    // samples per weight-gradient partial
+constexpr uint32_t kRaySeedStride = 8;
+
+// activation rows (feature-major, row stride = sample capacity)
+template <int E>
+struct Act {
+    static constexpr int ENC = 0, H1 = E, DENS = E + kW, SH = E + kW + kDOut, H2 = E + kW + kCin,
+                         H3 = H2 + kW, N = H3 + kW;
+};
+// delta rows
+struct Dact {
+    static constexpr int DRGB = 0, DH3 = 4, DH2 = DH3 + kW, DDENS = DH2 + kW, DH1 = DDENS + kDOut, N = DH1 + kW;
+};
+// parameter offsets (density MLP, colour MLP), row-major [out][in]
+template <int E>
+struct Off {
+    static constexpr uint32_t W0 = 0, W1 = W0 + kW * E, W2 = W1 + kDOut * kW, W3 = W2 + kW * kCin, W4 = W3 + kW * kW,
+                              N = W4 + 3 * kW;
+};
+
+struct TrainState {
+    uint32_t n_images = 0, width = 0, height = 0;
+    float4* images = nullptr;
+    TrainImage* meta = nullptr;
+    std::vector<int32_t> frames;
+    // workspace
+    uint32_t ray_cap = 0, sample_cap = 0;
+    float4* ray_d = nullptr;
+    int32_t* ray_frame = nullptr;
+    uint32_t *ray_img = nullptr, *ray_pix = nullptr, *ray_src = nullptr, *ray_base = nullptr, *ray_cnt = nullptr;
+    float4 *s_pos = nullptr, *s_cpos = nullptr, *s_cdir = nullptr, *s_out = nullptr, *s_dout = nullptr;
+    int32_t* s_hit = nullptr;
+    float *act = nullptr, *dact = nullptr, *part = nullptr;
+    // optimizer
+    float *grad = nullptr, *adam_m = nullptr, *adam_v = nullptr;
+    __half2* grad_h = nullptr;
+    uint32_t* counters = nullptr;
+    uint32_t* h_counters = nullptr;
+    uint64_t rays_total = 0;
+    uint32_t rays_per_batch = 1u << 12;
+    uint32_t step = 0;
+    bool have_grad = false;
+};
+
+namespace {
+
+__global__ void train_clear_kernel(uint32_t* counters, uint32_t limit) {
+    const uint32_t i = threadIdx.x;
+    if (i < kTCounters) counters[i] = i == kTLimit ? limit : 0u;
+}
+
+__global__ void fill_zero_kernel(uint32_t* p, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+__global__ void train_count_kernel(uint32_t* counters) {
+    if (threadIdx.x == 0) counters[kTValid] = min(counters[kTSamples], counters[kTLimit]);
+}
+
+struct RayArgs {
+    RenderArgs box;               // render box, G, dt, max_steps (grid set per ray)
+    const TrainImage* meta;
+    const uint8_t* grids_lin;     // [slot][G^3/8]
+    size_t grid_bytes;
+    uint32_t n_images, width, height, max_samples;
+};
+
+INSTA_DEV bool train_ray(const RayArgs& a, uint32_t i, uint32_t n_rays, uint64_t rays_total, Pcg32 rng,
+                         uint32_t& img, uint32_t& pix, V3& o, V3& d, float& t0, RenderArgs& ra) {
+    img = (uint32_t)((((uint64_t)i + rays_total) * a.n_images / n_rays) % a.n_images);
+    rng.advance((uint64_t)i * kRaySeedStride);
+    const float x = rng.next_float();
+    const float y = rng.next_float();
+    (void)rng.next_float();  // motion-blur time in the reference
+    const float jitter = rng.next_float();
+    const uint32_t px = min((uint32_t)(x * (float)a.width), a.width - 1u);
+    const uint32_t py = min((uint32_t)(y * (float)a.height), a.height - 1u);
+    pix = px + py * a.width;
+    const TrainImage& c = a.meta[img];
+    const V3 d0 = v3((x - c.cx) * (float)a.width / c.fx, (y - c.cy) * (float)a.height / c.fy, 1.0f);
+    d = v3(c.M[0] * d0.x + c.M[1] * d0.y + c.M[2] * d0.z, c.M[4] * d0.x + c.M[5] * d0.y + c.M[6] * d0.z,
+           c.M[8] * d0.x + c.M[9] * d0.y + c.M[10] * d0.z);
+    const float dn = norm(d);
+    d = v3(d.x / dn, d.y / dn, d.z / dn);
+    o = v3(c.M[3], c.M[7], c.M[11]);
+    ra = a.box;
+    ra.grid = a.grids_lin + (size_t)(c.frame + 1) * a.grid_bytes;
+    const float tb = box_ray_tmin(ra, o, d);
+    if (tb == 3.40282347e38f) return false;
+    t0 = fmaxf(tb, 0.0f) + a.box.dt * jitter;
+    return true;
+}
+
+__global__ __launch_bounds__(256) void train_rays_kernel(RayArgs a, uint32_t n_rays, uint64_t rays_total,
+                                                         Pcg32 rng, float4* __restrict__ ray_d,
+                                                         int32_t* __restrict__ ray_frame,
+                                                         uint32_t* __restrict__ ray_img,
+                                                         uint32_t* __restrict__ ray_pix,
+                                                         uint32_t* __restrict__ ray_src,
+                                                         uint32_t* __restrict__ ray_base,
+                                                         uint32_t* __restrict__ ray_cnt,
+                                                         float4* __restrict__ s_pos,
+                                                         uint32_t* __restrict__ counters) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t img = 0, pix = 0, count = 0;
+    V3 o = v3(0, 0, 0), d = v3(0, 0, 1), idir = v3(0, 0, 1);
+    float t0 = 0.0f;
+    RenderArgs ra = a.box;
+    if (i < n_rays && train_ray(a, i, n_rays, rays_total, rng, img, pix, o, d, t0, ra)) {
+        idir = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+        float t = t0;
+        V3 pos;
+        while (count < a.box.max_steps && skip_empty(ra, o, d, idir, t, pos)) {
+            ++count;
+            t += a.box.dt;
+        }
+    }
+    const uint32_t base = wave_reserve(&counters[kTSamples], count);
+    const bool fits = base + count <= a.max_samples;
+    if (count > 0 && !fits && base < a.max_samples) atomicMin(&counters[kTLimit], base);
+    const bool keep = count > 0 && fits;
+    const uint32_t r = wave_append(&counters[kTRays], keep);
+    if (!keep) return;
+    ray_d[r] = make_float4(d.x, d.y, d.z, 0.0f);
+    ray_frame[r] = a.meta[img].frame;
+    ray_img[r] = img;
+    ray_pix[r] = pix;
+    ray_src[r] = i;
+    ray_base[r] = base;
+    ray_cnt[r] = count;
+    float t = t0;
+    V3 pos;
+    for (uint32_t j = 0; j < count; ++j) {
+        skip_empty(ra, o, d, idir, t, pos);
+        s_pos[base + j] = make_float4(pos.x, pos.y, pos.z, __uint_as_float(r));
+        t += a.box.dt;
+    }
+}
+
+template <int NIN, int NOUT>
+INSTA_DEV void store_rows(float* __restrict__ act, uint32_t ld, int row0, uint32_t i, const float* v) {
+#pragma unroll
+    for (int k = 0; k < NOUT; ++k) act[(size_t)(row0 + k) * ld + i] = v[k];
+}
+
+// Forward with stored activations; the arithmetic is that of the plain FP32
+// inference kernel (same FMA chains), so outputs match it bit for bit.
+template <int L>
+__global__ __launch_bounds__(kTrainThreads) void train_forward_kernel(const float* __restrict__ params,
+                                                                      const float* __restrict__ table, GridParams gp,
+                                                                      const uint32_t* __restrict__ n_dev,
+                                                                      const float4* __restrict__ cpos,
+                                                                      const float4* __restrict__ cdir,
+                                                                      float* __restrict__ act, uint32_t ld,
+                                                                      float4* __restrict__ out) {
+    constexpr int E = 2 * L;
+    using A = Act<E>;
+    using O = Off<E>;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *n_dev) return;
+    const float4 p4 = cpos[i], d4 = cdir[i];
+    const V3 x = v3(p4.x, p4.y, p4.z);
+    float enc[E];
+#pragma unroll
+    for (int l = 0; l < L; ++l) encode_level<2>(table, gp, (uint32_t)l, x, enc + 2 * l);
+    store_rows<E, E>(act, ld, A::ENC, i, enc);
+    float h[kW], g[kW], r[kCin], rgb[3];
+    dense<E, kW, true>(params + O::W0, enc, h);
+    store_rows<kW, kW>(act, ld, A::H1, i, h);
+    dense<kW, kDOut, false>(params + O::W1, h, r);
+    sh16(d4.x * 2.0f - 1.0f, d4.y * 2.0f - 1.0f, d4.z * 2.0f - 1.0f, r + kDOut);
+    store_rows<kCin, kCin>(act, ld, A::DENS, i, r);
+    dense<kCin, kW, true>(params + O::W2, r, h);
+    store_rows<kW, kW>(act, ld, A::H2, i, h);
+    dense<kW, kW, true>(params + O::W3, h, g);
+    store_rows<kW, kW>(act, ld, A::H3, i, g);
+    dense<kW, 3, false>(params + O::W4, g, rgb);
+    out[i] = make_float4(rgb[0], rgb[1], rgb[2], r[0]);
+}
+
+struct LossArgs {
+    const float4* images;
+    uint32_t width, height;
+    float dt, density_bias, loss_scale_per_ray, inv_rays;
+};
+
+INSTA_DEV float sigmoid(float x) { return 1.0f / (1.0f + expf(-x)); }
+
+__global__ __launch_bounds__(256) void train_loss_kernel(LossArgs a, Pcg32 rng0, const uint32_t* __restrict__ ray_src,
+                                                         const uint32_t* __restrict__ ray_img,
+                                                         const uint32_t* __restrict__ ray_pix,
+                                                         const uint32_t* __restrict__ ray_base,
+                                                         const uint32_t* __restrict__ ray_cnt,
+                                                         const float4* __restrict__ s_out,
+                                                         float4* __restrict__ dout, uint32_t* __restrict__ counters) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    float ray_loss = 0.0f;
+    uint32_t used = 0;
+    if (r < counters[kTRays]) {
+        const uint32_t base = ray_base[r], cnt = ray_cnt[r];
+        Pcg32 rng = rng0;
+        rng.advance((uint64_t)ray_src[r] * kRaySeedStride);
+        (void)rng.next_float();
+        (void)rng.next_float();
+        const float bg[3] = {rng.next_float(), rng.next_float(), rng.next_float()};
+        const float4 tex = a.images[(size_t)ray_img[r] * a.width * a.height + ray_pix[r]];
+        const float target[3] = {tex.x + (1.0f - tex.w) * bg[0], tex.y + (1.0f - tex.w) * bg[1],
+                                 tex.z + (1.0f - tex.w) * bg[2]};
+        float T = 1.0f;
+        float acc[3] = {0.0f, 0.0f, 0.0f};
+        for (; used < cnt; ++used) {
+            if (T < 1e-4f) break;
+            const float4 o = s_out[base + used];
+            const float sigma = expf(o.w + a.density_bias);
+            const float alpha = 1.0f - expf(-sigma * a.dt);
+            const float w = alpha * T;
+            acc[0] = acc[0] + w * sigmoid(o.x);
+            acc[1] = acc[1] + w * sigmoid(o.y);
+            acc[2] = acc[2] + w * sigmoid(o.z);
+            T = T * (1.0f - alpha);
+        }
+        if (used == cnt) {
+            for (int c = 0; c < 3; ++c) acc[c] = acc[c] + T * bg[c];
+        }
+        // Huber(0.1) / 5 per channel (testbed_nerf.cu:141-157, 1299)
+        float grad[3];
+        for (int c = 0; c < 3; ++c) {
+            const float diff = acc[c] - target[c];
+            const float ad = fabsf(diff);
+            const float l = ad > 0.1f ? ad - 0.05f : (0.5f / 0.1f) * diff * diff;
+            grad[c] = (ad > 0.1f ? (diff > 0.0f ? 1.0f : -1.0f) : diff / 0.1f) / 5.0f;
+            ray_loss += l / 5.0f;
+        }
+        ray_loss = ray_loss / 3.0f * a.inv_rays;
+        const float ls = a.loss_scale_per_ray;
+        T = 1.0f;
+        float acc2[3] = {0.0f, 0.0f, 0.0f};
+        for (uint32_t j = 0; j < used; ++j) {
+            const float4 o = s_out[base + j];
+            const float s[3] = {sigmoid(o.x), sigmoid(o.y), sigmoid(o.z)};
+            const float sigma = expf(o.w + a.density_bias);
+            const float alpha = 1.0f - expf(-sigma * a.dt);
+            const float w = alpha * T;
+            for (int c = 0; c < 3; ++c) acc2[c] = acc2[c] + w * s[c];
+            T = T * (1.0f - alpha);
+            float dsig = 0.0f;
+            float drgb[3];
+            for (int c = 0; c < 3; ++c) {
+                drgb[c] = ls * (w * grad[c] * (s[c] * (1.0f - s[c])));
+                dsig = dsig + grad[c] * (T * s[c] - (acc[c] - acc2[c]));
+            }
+            dout[base + j] = make_float4(drgb[0], drgb[1], drgb[2], ls * (sigma * (a.dt * dsig)));
+        }
+        for (uint32_t j = used; j < cnt; ++j) dout[base + j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    // statistics
+    uint32_t u = used;
+    float lsum = ray_loss;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        u += __shfl_xor(u, off);
+        lsum += __shfl_xor(lsum, off);
+    }
+    if ((threadIdx.x & 63u) == 0) {
+        if (u) atomicAdd(&counters[kTUsed], u);
+        if (lsum != 0.0f) atomicAdd(reinterpret_cast<float*>(&counters[kTLoss]), lsum);
+    }
+}
+
+// Backward of one sample: deltas of both MLPs (stored for the weight
+// gradients) and the hash-grid scatter.
+template <int L, bool H16>
+__global__ __launch_bounds__(kTrainThreads) void train_backward_kernel(const float* __restrict__ params, GridParams gp,
+                                                                       const uint32_t* __restrict__ n_dev,
+                                                                       const float4* __restrict__ cpos,
+                                                                       const float* __restrict__ act,
+                                                                       const float4* __restrict__ dout,
+                                                                       float* __restrict__ dact, uint32_t ld,
+                                                                       float* __restrict__ grad_tab,
+                                                                       __half2* __restrict__ grad_tab_h) {
+    constexpr int E = 2 * L;
+    using A = Act<E>;
+    using O = Off<E>;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *n_dev) return;
+    const float4 d4 = dout[i];
+    const float drgb[3] = {d4.x, d4.y, d4.z};
+    store_rows<3, 3>(dact, ld, Dact::DRGB, i, drgb);
+    const float* W4 = params + O::W4;
+    const float* W3 = params + O::W3;
+    const float* W2 = params + O::W2;
+    const float* W1 = params + O::W1;
+    const float* W0 = params + O::W0;
+    float dh3[kW];
+#pragma unroll
+    for (int k = 0; k < kW; ++k) {
+        const float s = W4[k] * drgb[0] + W4[kW + k] * drgb[1] + W4[2 * kW + k] * drgb[2];
+        dh3[k] = act[(size_t)(A::H3 + k) * ld + i] > 0.0f ? s : 0.0f;
+    }
+    store_rows<kW, kW>(dact, ld, Dact::DH3, i, dh3);
+    float dh2[kW];
+#pragma unroll
+    for (int k = 0; k < kW; ++k) {
+        float s = 0.0f;
+#pragma unroll
+        for (int m = 0; m < kW; ++m) s = __builtin_fmaf(W3[m * kW + k], dh3[m], s);
+        dh2[k] = act[(size_t)(A::H2 + k) * ld + i] > 0.0f ? s : 0.0f;
+    }
+    store_rows<kW, kW>(dact, ld, Dact::DH2, i, dh2);
+    float dd[kDOut];
+#pragma unroll
+    for (int q = 0; q < kDOut; ++q) {
+        float s = 0.0f;
+#pragma unroll
+        for (int m = 0; m < kW; ++m) s = __builtin_fmaf(W2[m * kCin + q], dh2[m], s);
+        dd[q] = s;
+    }
+    dd[0] = dd[0] + d4.w;  // density logit = density-MLP output 0
+    store_rows<kDOut, kDOut>(dact, ld, Dact::DDENS, i, dd);
+    float dh1[kW];
+#pragma unroll
+    for (int k = 0; k < kW; ++k) {
+        float s = 0.0f;
+#pragma unroll
+        for (int q = 0; q < kDOut; ++q) s = __builtin_fmaf(W1[q * kW + k], dd[q], s);
+        dh1[k] = act[(size_t)(A::H1 + k) * ld + i] > 0.0f ? s : 0.0f;
+    }
+    store_rows<kW, kW>(dact, ld, Dact::DH1, i, dh1);
+    float denc[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        float s = 0.0f;
+#pragma unroll
+        for (int k = 0; k < kW; ++k) s = __builtin_fmaf(W0[k * E + e], dh1[k], s);
+        denc[e] = s;
+    }
+    // hash-grid scatter (the corner/weight arithmetic of encode_level)
+    const float4 p4 = cpos[i];
+    const float xs[3] = {p4.x, p4.y, p4.z};
+#pragma unroll 1
+    for (int l = 0; l < L; ++l) {
+        const float sc = gp.scale[l];
+        float frac[3];
+        uint32_t g[3];
+#pragma unroll
+        for (int dd3 = 0; dd3 < 3; ++dd3) {
+            const float pp = xs[dd3] * sc + 0.5f;
+            const float fl = floorf(pp);
+            frac[dd3] = pp - fl;
+            g[dd3] = (uint32_t)(int32_t)fl;
+        }
+        const bool dense_l = (gp.dense_mask >> l) & 1u;
+        const uint32_t res = gp.res[l], size = gp.size[l], off = gp.offset[l];
+#pragma unroll
+        for (uint32_t c = 0; c < 8; ++c) {
+            float w = 1.0f;
+            uint32_t cg[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const bool up = (c >> k) & 1u;
+                w = w * (up ? frac[k] : 1.0f - frac[k]);
+                cg[k] = g[k] + (up ? 1u : 0u);
+            }
+            const uint32_t idx = off + grid_index(cg, dense_l, res, size);
+            if constexpr (H16) {
+                unsafeAtomicAdd(grad_tab_h + idx, __floats2half2_rn(w * denc[2 * l], w * denc[2 * l + 1]));
+            } else {
+                atomicAdd(grad_tab + 2 * (size_t)idx, w * denc[2 * l]);
+                atomicAdd(grad_tab + 2 * (size_t)idx + 1, w * denc[2 * l + 1]);
+            }
+        }
+    }
+}
+
+struct WProduct {
+    uint32_t d_row, x_row, O, I, out;
+};
+struct WArgs {
+    WProduct p[5];
+};
+
+// partial[c][out + o*I + i] = sum over samples n of chunk c of
+// dact[d_row + o][n] * act[x_row + i][n]
+__global__ __launch_bounds__(256) void wgrad_kernel(WArgs wa, const uint32_t* __restrict__ n_dev,
+                                                    const float* __restrict__ act, const float* __restrict__ dact,
+                                                    uint32_t ld, float* __restrict__ part, uint32_t n_mlp) {
+    __shared__ float Ds[64][33];
+    __shared__ float Xs[64][33];
+    const WProduct pr = wa.p[blockIdx.y];
+    const uint32_t n = *n_dev;
+    const uint32_t s0 = blockIdx.x * kChunk;
+    if (s0 >= n) return;
+    const uint32_t s1 = min(s0 + kChunk, n);
+    const uint32_t t = threadIdx.x, to = t / 16u, ti = t % 16u;
+    float acc[4][4] = {};
+    for (uint32_t s = s0; s < s1; s += 32) {
+        for (uint32_t e = t; e < 64u * 32u; e += 256u) {
+            const uint32_t row = e / 32u, col = e % 32u, smp = s + col;
+            Ds[row][col] = (row < pr.O && smp < s1) ? dact[(size_t)(pr.d_row + row) * ld + smp] : 0.0f;
+            Xs[row][col] = (row < pr.I && smp < s1) ? act[(size_t)(pr.x_row + row) * ld + smp] : 0.0f;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (uint32_t col = 0; col < 32u; ++col) {
+            float dv[4], xv[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                dv[q] = Ds[to + 16u * q][col];
+                xv[q] = Xs[ti + 16u * q][col];
+            }
+#pragma unroll
+            for (int a2 = 0; a2 < 4; ++a2)
+#pragma unroll
+                for (int b2 = 0; b2 < 4; ++b2) acc[a2][b2] = __builtin_fmaf(dv[a2], xv[b2], acc[a2][b2]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a2 = 0; a2 < 4; ++a2)
+#pragma unroll
+        for (int b2 = 0; b2 < 4; ++b2) {
+            const uint32_t o = to + 16u * a2, ii = ti + 16u * b2;
+            if (o < pr.O && ii < pr.I) part[(size_t)blockIdx.x * n_mlp + pr.out + o * pr.I + ii] = acc[a2][b2];
+        }
+}
+
+__global__ void wgrad_reduce_kernel(const float* __restrict__ part, const uint32_t* __restrict__ n_dev,
+                                    uint32_t n_mlp, float* __restrict__ grad) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_mlp) return;
+    const uint32_t nc = (*n_dev + kChunk - 1) / kChunk;
+    float s = 0.0f;
+    for (uint32_t c = 0; c < nc; ++c) s += part[(size_t)c * n_mlp + p];
+    grad[p] = s;
+}
+
+// Unscale, optionally step Adam.  grad[] receives dL/dparam (unscaled).
+__global__ void adam_kernel(float* __restrict__ params, float* __restrict__ grad, const __half2* __restrict__ grad_h,
+                            float* __restrict__ m, float* __restrict__ v, uint64_t n_params, uint64_t n_mlp,
+                            float inv_scale, float lr, float b1, float b2, float eps, float bc1, float bc2,
+                            uint32_t apply) {
+    for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < n_params;
+         p += (uint64_t)gridDim.x * blockDim.x) {
+        float g;
+        if (grad_h && p >= n_mlp) {
+            const uint64_t q = p - n_mlp;
+            const __half2 h = grad_h[q >> 1];
+            g = (q & 1u) ? __high2float(h) : __low2float(h);
+        } else {
+            g = grad[p];
+        }
+        g = g * inv_scale;
+        grad[p] = g;
+        if (apply) {
+            const float mm = b1 * m[p] + (1.0f - b1) * g;
+            const float vv = b2 * v[p] + (1.0f - b2) * (g * g);
+            m[p] = mm;
+            v[p] = vv;
+            params[p] = params[p] - lr * (mm / bc1) / (sqrtf(vv / bc2) + eps);
+        }
+    }
+}
+
+template <class T>
+bool train_alloc(T*& p, size_t bytes) {
+    return hipMalloc(reinterpret_cast<void**>(&p), bytes) == hipSuccess;
+}
+
+void free_workspace_train(TrainState* ts) {
+    dev_free(ts->ray_d); dev_free(ts->ray_frame); dev_free(ts->ray_img); dev_free(ts->ray_pix);
+    dev_free(ts->ray_src); dev_free(ts->ray_base); dev_free(ts->ray_cnt);
+    dev_free(ts->s_pos); dev_free(ts->s_cpos); dev_free(ts->s_cdir); dev_free(ts->s_out); dev_free(ts->s_dout);
+    dev_free(ts->s_hit); dev_free(ts->act); dev_free(ts->dact); dev_free(ts->part);
+    ts->ray_cap = ts->sample_cap = 0;
+}
+
+insta_status ensure_train_workspace(insta_ctx* ctx, TrainState* ts, uint32_t n_rays, uint32_t n_samples) {
+    if (n_rays <= ts->ray_cap && n_samples <= ts->sample_cap) return INSTA_OK;
+    const uint32_t rc = std::max(n_rays, ts->ray_cap), sc = std::max(n_samples, ts->sample_cap);
+    free_workspace_train(ts);
+    const int E = (int)(ctx->cfg.n_levels * 2);
+    const size_t act_rows = (size_t)E + 224, n_mlp = ctx->layout.n_mlp_params;
+    bool ok = true;
+    ok = ok && train_alloc(ts->ray_d, rc * sizeof(float4)) && train_alloc(ts->ray_frame, rc * sizeof(int32_t));
+    ok = ok && train_alloc(ts->ray_img, rc * 4) && train_alloc(ts->ray_pix, rc * 4) && train_alloc(ts->ray_src, rc * 4);
+    ok = ok && train_alloc(ts->ray_base, rc * 4) && train_alloc(ts->ray_cnt, rc * 4);
+    ok = ok && train_alloc(ts->s_pos, sc * sizeof(float4)) && train_alloc(ts->s_cpos, sc * sizeof(float4));
+    ok = ok && train_alloc(ts->s_cdir, sc * sizeof(float4)) && train_alloc(ts->s_out, sc * sizeof(float4));
+    ok = ok && train_alloc(ts->s_dout, sc * sizeof(float4)) && train_alloc(ts->s_hit, sc * sizeof(int32_t));
+    ok = ok && train_alloc(ts->act, act_rows * sc * sizeof(float));
+    ok = ok && train_alloc(ts->dact, (size_t)Dact::N * sc * sizeof(float));
+    ok = ok && train_alloc(ts->part, ((sc + kChunk - 1) / kChunk) * n_mlp * sizeof(float));
+    if (!ok) {
+        free_workspace_train(ts);
+        set_error("training workspace allocation failed");
+        return INSTA_ERR_OOM;
+    }
+    ts->ray_cap = rc;
+    ts->sample_cap = sc;
+    return INSTA_OK;
+}
+
+template <int L>
+insta_status launch_train_network(insta_ctx* ctx, TrainState* ts, hipStream_t stream) {
+    const uint32_t cap = ts->sample_cap;
+    const uint32_t* n_dev = ts->counters + kTValid;
+    const float* table = ctx->d_params + ctx->layout.n_mlp_params;
+    const GridParams gp = grid_params(ctx);
+    const dim3 sgrid((cap + kTrainThreads - 1) / kTrainThreads);
+    hipLaunchKernelGGL(train_forward_kernel<L>, sgrid, dim3(kTrainThreads), 0, stream, ctx->d_params, table, gp, n_dev,
+                       ts->s_cpos, ts->s_cdir, ts->act, cap, ts->s_out);
+    INSTA_HIP(hipGetLastError());
+    return INSTA_OK;
+}
+
+template <int L>
+insta_status launch_train_backward(insta_ctx* ctx, TrainState* ts, const insta_train_config& tc, hipStream_t stream) {
+    constexpr int E = 2 * L;
+    using A = Act<E>;
+    using O = Off<E>;
+    const uint32_t cap = ts->sample_cap;
+    const uint32_t* n_dev = ts->counters + kTValid;
+    const GridParams gp = grid_params(ctx);
+    const uint32_t n_mlp = (uint32_t)ctx->layout.n_mlp_params;
+    const dim3 sgrid((cap + kTrainThreads - 1) / kTrainThreads);
+    float* grad_tab = ts->grad + n_mlp;
+    if (tc.grid_grad_fp16)
+        hipLaunchKernelGGL((train_backward_kernel<L, true>), sgrid, dim3(kTrainThreads), 0, stream, ctx->d_params, gp,
+                           n_dev, ts->s_cpos, ts->act, ts->s_dout, ts->dact, cap, grad_tab, ts->grad_h);
+    else
+        hipLaunchKernelGGL((train_backward_kernel<L, false>), sgrid, dim3(kTrainThreads), 0, stream, ctx->d_params, gp,
+                           n_dev, ts->s_cpos, ts->act, ts->s_dout, ts->dact, cap, grad_tab, ts->grad_h);
+    INSTA_HIP(hipGetLastError());
+    WArgs wa;
+    wa.p[0] = WProduct{Dact::DH1, A::ENC, kW, E, O::W0};
+    wa.p[1] = WProduct{Dact::DDENS, A::H1, kDOut, kW, O::W1};
+    wa.p[2] = WProduct{Dact::DH2, A::DENS, kW, kCin, O::W2};
+    wa.p[3] = WProduct{Dact::DH3, A::H2, kW, kW, O::W3};
+    wa.p[4] = WProduct{Dact::DRGB, A::H3, 3, kW, O::W4};
+    const uint32_t n_chunks = (cap + kChunk - 1) / kChunk;
+    hipLaunchKernelGGL(wgrad_kernel, dim3(n_chunks, 5), dim3(256), 0, stream, wa, n_dev, ts->act, ts->dact, cap,
+                       ts->part, n_mlp);
+    INSTA_HIP(hipGetLastError());
+    hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((n_mlp + 255) / 256), dim3(256), 0, stream, ts->part, n_dev, n_mlp,
+                       ts->grad);
+    INSTA_HIP(hipGetLastError());
+    return INSTA_OK;
+}
+
+}  // namespace
+
+bool train_supported(const insta_config& c) {
+    return c.n_neurons == 64 && c.density_out == 16 && c.sh_degree == 4 && c.n_extra_dims == 0 &&
+           c.density_hidden_layers == 1 && c.rgb_hidden_layers == 2 && c.n_features_per_level == 2 &&
+           (c.n_levels == 8 || c.n_levels == 16);
+}
+
+void free_train(insta_ctx* ctx) {
+    TrainState* ts = ctx->train;
+    if (!ts) return;
+    free_workspace_train(ts);
+    dev_free(ts->images); dev_free(ts->meta); dev_free(ts->grad); dev_free(ts->grad_h);
+    dev_free(ts->adam_m); dev_free(ts->adam_v); dev_free(ts->counters);
+    if (ts->h_counters) (void)hipHostFree(ts->h_counters);
+    delete ts;
+    ctx->train = nullptr;
+}
+
+insta_status set_training_set(insta_ctx* ctx, uint32_t n_images, uint32_t width, uint32_t height, const float* rgba,
+                              const insta_camera* cams, const int32_t* frames, hipStream_t stream) {
+    if (!ctx->train) {
+        TrainState* ts = new (std::nothrow) TrainState();
+        if (!ts) {
+            set_error("host allocation failed");
+            return INSTA_ERR_OOM;
+        }
+        ctx->train = ts;
+        const uint64_t np = ctx->layout.n_params;
+        bool ok = train_alloc(ts->grad, np * sizeof(float)) && train_alloc(ts->adam_m, np * sizeof(float)) &&
+                  train_alloc(ts->adam_v, np * sizeof(float)) &&
+                  train_alloc(ts->grad_h, (ctx->layout.n_grid_params / 2 + 1) * sizeof(__half2)) &&
+                  train_alloc(ts->counters, kTCounters * sizeof(uint32_t)) &&
+                  hipHostMalloc((void**)&ts->h_counters, kTCounters * sizeof(uint32_t), hipHostMallocDefault) ==
+                      hipSuccess;
+        if (!ok) {
+            free_train(ctx);
+            set_error("training state allocation failed");
+            return INSTA_ERR_OOM;
+        }
+        INSTA_HIP(hipMemsetAsync(ts->adam_m, 0, np * sizeof(float), stream));
+        INSTA_HIP(hipMemsetAsync(ts->adam_v, 0, np * sizeof(float), stream));
+    }
+    TrainState* ts = ctx->train;
+    dev_free(ts->images);
+    dev_free(ts->meta);
+    const size_t px = (size_t)n_images * width * height;
+    if (!train_alloc(ts->images, px * sizeof(float4)) || !train_alloc(ts->meta, n_images * sizeof(TrainImage))) {
+        set_error("training image allocation failed");
+        return INSTA_ERR_OOM;
+    }
+    std::vector<TrainImage> meta(n_images);
+    for (uint32_t i = 0; i < n_images; ++i) {
+        TrainImage& m = meta[i];
+        for (int k = 0; k < 12; ++k) m.M[k] = cams[i].matrix[k];
+        m.fx = cams[i].focal_px[0];
+        m.fy = cams[i].focal_px[1];
+        m.cx = cams[i].screen_center[0];
+        m.cy = cams[i].screen_center[1];
+        m.frame = frames[i];
+        m.pad[0] = m.pad[1] = m.pad[2] = 0;
+    }
+    INSTA_HIP(hipMemcpyAsync(ts->images, rgba, px * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+    INSTA_HIP(hipMemcpyAsync(ts->meta, meta.data(), n_images * sizeof(TrainImage), hipMemcpyHostToDevice, stream));
+    INSTA_HIP(hipStreamSynchronize(stream));  // `meta` is pageable host memory
+    ts->frames.assign(frames, frames + n_images);
+    ts->n_images = n_images;
+    ts->width = width;
+    ts->height = height;
+    ts->rays_total = 0;
+    return INSTA_OK;
+}
+
+insta_status train_step(insta_ctx* ctx, const insta_train_config& tc, insta_train_stats* stats, hipStream_t stream) {
+    TrainState* ts = ctx->train;
+    const uint32_t n_rays = tc.n_rays ? tc.n_rays : ts->rays_per_batch;
+    insta_status st = ensure_train_workspace(ctx, ts, n_rays, tc.max_samples);
+    if (st != INSTA_OK) return st;
+    const uint64_t np = ctx->layout.n_params, n_mlp = ctx->layout.n_mlp_params;
+    const uint32_t cap = ts->sample_cap;
+    // zero gradients with a fill kernel (capturable, unlike memset nodes)
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(1024), dim3(256), 0, stream, reinterpret_cast<uint32_t*>(ts->grad), np);
+    if (tc.grid_grad_fp16)
+        hipLaunchKernelGGL(fill_zero_kernel, dim3(1024), dim3(256), 0, stream,
+                           reinterpret_cast<uint32_t*>(ts->grad_h), ctx->layout.n_grid_params / 2);
+    hipLaunchKernelGGL(train_clear_kernel, dim3(1), dim3(64), 0, stream, ts->counters, cap);
+    INSTA_HIP(hipGetLastError());
+
+    Pcg32 rng = Pcg32::seeded(tc.seed);
+    rng.advance(ts->rays_total * kRaySeedStride);
+    RayArgs ra{};
+    ra.box.G = ctx->cfg.grid_size;
+    ra.box.dt = ctx->cfg.dt;
+    ra.box.max_steps = ctx->cfg.max_steps;
+    for (int k = 0; k < 3; ++k) {
+        ra.box.lo[k] = ctx->render_lo[k];
+        ra.box.hi[k] = ctx->render_hi[k];
+    }
+    ra.meta = ts->meta;
+    ra.grids_lin = ctx->d_grids_lin;
+    ra.grid_bytes = (size_t)ctx->cfg.grid_size * ctx->cfg.grid_size * ctx->cfg.grid_size / 8;
+    ra.n_images = ts->n_images;
+    ra.width = ts->width;
+    ra.height = ts->height;
+    ra.max_samples = cap;
+    hipLaunchKernelGGL(train_rays_kernel, dim3((n_rays + 255) / 256), dim3(256), 0, stream, ra, n_rays, ts->rays_total,
+                       rng, ts->ray_d, ts->ray_frame, ts->ray_img, ts->ray_pix, ts->ray_src, ts->ray_base, ts->ray_cnt,
+                       ts->s_pos, ts->counters);
+    INSTA_HIP(hipGetLastError());
+    hipLaunchKernelGGL(train_count_kernel, dim3(1), dim3(64), 0, stream, ts->counters);
+    st = launch_deform_samples(ctx, -1, ts->ray_frame, ts->counters + kTValid, cap, ts->s_pos, ts->ray_d, ts->s_hit,
+                               ts->s_cpos, ts->s_cdir, stream);
+    if (st != INSTA_OK) return st;
+    st = ctx->cfg.n_levels == 8 ? launch_train_network<8>(ctx, ts, stream)
+                                : launch_train_network<16>(ctx, ts, stream);
+    if (st != INSTA_OK) return st;
+    LossArgs la;
+    la.images = ts->images;
+    la.width = ts->width;
+    la.height = ts->height;
+    la.dt = ctx->cfg.dt;
+    la.density_bias = ctx->cfg.density_logit_bias;
+    la.loss_scale_per_ray = tc.loss_scale / (float)n_rays;
+    la.inv_rays = 1.0f / (float)n_rays;
+    hipLaunchKernelGGL(train_loss_kernel, dim3((n_rays + 255) / 256), dim3(256), 0, stream, la, rng, ts->ray_src,
+                       ts->ray_img, ts->ray_pix, ts->ray_base, ts->ray_cnt, ts->s_out, ts->s_dout, ts->counters);
+    INSTA_HIP(hipGetLastError());
+    st = ctx->cfg.n_levels == 8 ? launch_train_backward<8>(ctx, ts, tc, stream)
+                                : launch_train_backward<16>(ctx, ts, tc, stream);
+    if (st != INSTA_OK) return st;
+    const uint32_t t = ts->step + (tc.apply_update ? 1u : 0u);
+    const float bc1 = tc.apply_update ? 1.0f - (float)std::pow((double)tc.beta1, (double)t) : 1.0f;
+    const float bc2 = tc.apply_update ? 1.0f - (float)std::pow((double)tc.beta2, (double)t) : 1.0f;
+    hipLaunchKernelGGL(adam_kernel, dim3(2048), dim3(256), 0, stream, ctx->d_params, ts->grad,
+                       tc.grid_grad_fp16 ? ts->grad_h : nullptr, ts->adam_m, ts->adam_v, np, n_mlp,
+                       1.0f / tc.loss_scale, tc.learning_rate, tc.beta1, tc.beta2, tc.epsilon, bc1, bc2,
+                       tc.apply_update);
+    INSTA_HIP(hipGetLastError());
+    if (tc.apply_update) {
+        st = prepare_network(ctx, stream);
+        if (st != INSTA_OK) return st;
+    }
+    INSTA_HIP(hipMemcpyAsync(ts->h_counters, ts->counters, kTCounters * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                             stream));
+    INSTA_HIP(hipStreamSynchronize(stream));
+    const uint32_t* hc = ts->h_counters;
+    if (stats) {
+        stats->n_rays = n_rays;
+        stats->n_rays_used = hc[kTRays];
+        stats->n_samples = hc[kTValid];
+        stats->n_samples_used = hc[kTUsed];
+        float loss;
+        std::memcpy(&loss, &hc[kTLoss], 4);
+        stats->loss = loss;
+    }
+    ts->rays_total += n_rays;
+    ts->have_grad = true;
+    if (tc.apply_update) ts->step = t;
+    if (stats) stats->step = ts->step;
+    // adaptive ray count (testbed_nerf.cu:3402-3410): aim at target_samples
+    const uint32_t measured = std::max(hc[kTUsed], 1u);
+    double next = (double)n_rays * tc.target_samples / measured;
+    next = std::min(std::max(next, 256.0), (double)(1u << 22));
+    ts->rays_per_batch = ((uint32_t)next + 255u) & ~255u;
+    return INSTA_OK;
+}
+
+bool train_has_gradients(const insta_ctx* ctx) { return ctx->train && ctx->train->have_grad; }
+bool train_has_set(const insta_ctx* ctx) { return ctx->train && ctx->train->n_images > 0; }
+
+insta_status get_gradients(insta_ctx* ctx, float* out, hipStream_t stream) {
+    INSTA_HIP(hipMemcpyAsync(out, ctx->train->grad, ctx->layout.n_params * sizeof(float), hipMemcpyDeviceToDevice,
+                             stream));
+    return INSTA_OK;
+}
+
+insta_status reset_optimizer(insta_ctx* ctx) {
+    TrainState* ts = ctx->train;
+    const uint64_t np = ctx->layout.n_params;
+    INSTA_HIP(hipMemset(ts->adam_m, 0, np * sizeof(float)));
+    INSTA_HIP(hipMemset(ts->adam_v, 0, np * sizeof(float)));
+    INSTA_HIP(hipDeviceSynchronize());
+    ts->step = 0;
+    return INSTA_OK;
+}
+
+}  // namespace insta

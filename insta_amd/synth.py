@@ -275,6 +275,32 @@ def focal_from_fov(height: int, fov_deg: float) -> float:
 
 
 # ----------------------------------------------------------------------------
+# training targets
+# ----------------------------------------------------------------------------
+
+def smooth_noise_images(seed: int, n: int, height: int, width: int, n_waves: int = 6) -> np.ndarray:
+    """Training targets: RGBA float32 [n, height, width, 4], alpha 1, each
+    colour channel a seeded smooth field in [0, 1]:
+    0.5 + 0.5 * sum_k a_k sin(2 pi f_k u + p_k) cos(2 pi g_k v + q_k) / sum_k |a_k|
+    over pixel-centre coordinates u, v in (0, 1), with 1-4 cycles per image."""
+    rng = _rng(seed)
+    u = (np.arange(width) + 0.5) / width
+    v = (np.arange(height) + 0.5) / height
+    out = np.ones((n, height, width, 4), dtype=np.float32)
+    for i in range(n):
+        for c in range(3):
+            a = rng.uniform(0.2, 1.0, n_waves)
+            f = rng.uniform(1.0, 4.0, n_waves)
+            g = rng.uniform(1.0, 4.0, n_waves)
+            ph = rng.uniform(0.0, 2 * math.pi, (2, n_waves))
+            sx = np.sin(2 * math.pi * f[:, None] * u[None, :] + ph[0][:, None])   # [K, W]
+            cy = np.cos(2 * math.pi * g[:, None] * v[None, :] + ph[1][:, None])   # [K, H]
+            field = np.einsum("k,kh,kw->hw", a, cy, sx) / a.sum()
+            out[i, :, :, c] = (0.5 + 0.5 * field).astype(np.float32)
+    return out
+
+
+# ----------------------------------------------------------------------------
 # network parameters (declared order: density MLP, colour MLP, hash tables)
 # ----------------------------------------------------------------------------
 

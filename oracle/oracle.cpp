@@ -557,6 +557,11 @@ struct orc_scene {
     std::vector<float> params;
     std::vector<std::vector<uint8_t>> grids;   // slot 0 = canonical, slot f+1 = frame f
     int n_threads = 0;
+    // training set (orc_set_training_set)
+    uint32_t tr_n = 0, tr_w = 0, tr_h = 0;
+    std::vector<float> tr_images;              // [n][h][w][4]
+    std::vector<insta_camera> tr_cams;
+    std::vector<int32_t> tr_frames;
 };
 
 namespace {
@@ -1488,6 +1493,376 @@ int orc_render_fp64(const orc_scene* s, const insta_camera* cam, uint32_t width,
         }
         if (acc[3] > 0.001)
             for (int k = 0; k < 4; ++k) out[k] = acc[k];
+    }
+    return 0;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Training step (Testbed::train_nerf_step, instant-ngp/src/testbed_nerf.cu:
+// 3176-3400): the FP32 ray generation, march and deformation of the device
+// kernels (identical canonical samples), then network, loss and the backward
+// pass in FP64.  Gradients are those of the objective
+//   L = (1 / n_rays) * sum_rays sum_channels Huber_0.1(rgb - target) / 5,
+// i.e. the per-channel gradient of compute_loss_kernel_train_nerf
+// (1305-1643) without the loss scale; the reported loss is the per-ray
+// channel mean, as the reference reports it.
+// ===========================================================================
+namespace {
+
+// PCG32 (O'Neill 2014; pcg32.h by W. Jakob, instant-ngp's default_rng_t)
+struct Pcg32 {
+    uint64_t state, inc;
+    static Pcg32 seeded(uint64_t initstate, uint64_t initseq = 0xda3e39cb94b95bdbull) {
+        Pcg32 r{0u, (initseq << 1u) | 1u};
+        r.next_uint();
+        r.state += initstate;
+        r.next_uint();
+        return r;
+    }
+    uint32_t next_uint() {
+        const uint64_t old = state;
+        state = old * 0x5851f42d4c957f2dull + inc;
+        const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        const uint32_t rot = (uint32_t)(old >> 59u);
+        return (xs >> rot) | (xs << ((~rot + 1u) & 31u));
+    }
+    float next_float() {
+        const uint32_t u = (next_uint() >> 9) | 0x3f800000u;
+        float f;
+        std::memcpy(&f, &u, 4);
+        return f - 1.0f;
+    }
+    void advance(uint64_t delta) {
+        uint64_t cur_mult = 0x5851f42d4c957f2dull, cur_plus = inc, acc_mult = 1u, acc_plus = 0u;
+        while (delta > 0) {
+            if (delta & 1u) {
+                acc_mult *= cur_mult;
+                acc_plus = acc_plus * cur_mult + cur_plus;
+            }
+            cur_plus = (cur_mult + 1u) * cur_plus;
+            cur_mult *= cur_mult;
+            delta /= 2u;
+        }
+        state = acc_mult * state + acc_plus;
+    }
+};
+
+constexpr uint32_t kRaySeedStride = 8;  // N_MAX_RANDOM_SAMPLES_PER_RAY (testbed_nerf.cu:60)
+
+struct TrainRay {
+    uint32_t src, img, pix;
+    V3 o, d;
+    std::vector<V3> samples;
+    float bg[3];
+};
+
+// generate_training_samples_nerf for ray i (testbed_nerf.cu:1137-1285)
+bool train_ray(const orc_scene* s, uint32_t i, uint32_t n_rays, uint64_t rays_total, const Pcg32& rng0,
+               TrainRay& r) {
+    const uint32_t n_img = s->tr_n, W = s->tr_w, H = s->tr_h;
+    r.src = i;
+    r.img = (uint32_t)((((uint64_t)i + rays_total) * n_img / n_rays) % n_img);
+    Pcg32 rng = rng0;
+    rng.advance((uint64_t)i * kRaySeedStride);
+    const float x = rng.next_float();
+    const float y = rng.next_float();
+    (void)rng.next_float();  // motion-blur time
+    const float jitter = rng.next_float();
+    const uint32_t px = std::min((uint32_t)(x * (float)W), W - 1u);
+    const uint32_t py = std::min((uint32_t)(y * (float)H), H - 1u);
+    r.pix = px + py * W;
+    // background: draws 2..4 of the same stream (testbed_nerf.cu:1408-1419)
+    Pcg32 rb = rng0;
+    rb.advance((uint64_t)i * kRaySeedStride);
+    (void)rb.next_float();
+    (void)rb.next_float();
+    for (int c = 0; c < 3; ++c) r.bg[c] = rb.next_float();
+    const insta_camera& cam = s->tr_cams[r.img];
+    const float* M = cam.matrix;
+    V3 d0 = v3((x - cam.screen_center[0]) * (float)W / cam.focal_px[0],
+               (y - cam.screen_center[1]) * (float)H / cam.focal_px[1], 1.0f);
+    V3 d = v3(M[0] * d0.x + M[1] * d0.y + M[2] * d0.z, M[4] * d0.x + M[5] * d0.y + M[6] * d0.z,
+              M[8] * d0.x + M[9] * d0.y + M[10] * d0.z);
+    const float dn = norm(d);
+    d = v3(d.x / dn, d.y / dn, d.z / dn);
+    const V3 o = v3(M[3], M[7], M[11]);
+    r.o = o;
+    r.d = d;
+    r.samples.clear();
+    const float tb = box_ray_tmin(s->render_lo, s->render_hi, o, d);
+    if (tb == FLT_MAX) return false;
+    const std::vector<uint8_t>& grid = s->grids[s->tr_frames[r.img] + 1];
+    const uint32_t G = s->cfg.grid_size;
+    const float DT = s->cfg.dt;
+    const V3 idir = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    float t = std::fmax(tb, 0.0f) + DT * jitter;
+    while (r.samples.size() < s->cfg.max_steps) {
+        V3 pos;
+        bool inside = true;
+        while (true) {
+            pos = o + d * t;
+            if (!box_contains(s->render_lo, s->render_hi, pos)) {
+                inside = false;
+                break;
+            }
+            if (occupied(grid.data(), G, pos)) break;
+            t = advance_to_next_voxel(t, DT, pos, d, idir, G);
+        }
+        if (!inside) break;
+        r.samples.push_back(pos);
+        t += DT;
+    }
+    return !r.samples.empty();
+}
+
+struct SampleAct {
+    double enc[64], h1[64], cin[32], h2[64], h3[64], out[4];
+    D3v pos;
+};
+
+// forward of the supported shape (1 density / 2 colour hidden layers, width
+// 64, F = 2) in FP64, keeping the activations
+void forward_d(const orc_scene* s, D3v pos, D3v dir, SampleAct& a) {
+    const uint32_t E = s->cfg.n_levels * 2;
+    a.pos = pos;
+    encode_one_d(s, pos, a.enc);
+    const float* p = s->params.data();
+    dense_d(p, a.enc, E, 64, true, a.h1);
+    p += 64 * E;
+    dense_d(p, a.h1, 64, 16, false, a.cin);
+    p += 16 * 64;
+    // SH of the direction (the closed forms of network_one_d)
+    const double x = dir.x, y = dir.y, z = dir.z;
+    double* o = a.cin + 16;
+    const double pi = 3.14159265358979323846;
+    const double c0 = 0.5 / std::sqrt(pi), c1 = std::sqrt(3.0 / (4 * pi)), c2 = 0.5 * std::sqrt(15.0 / pi),
+                 c3 = 0.25 * std::sqrt(5.0 / pi), c4 = 0.25 * std::sqrt(15.0 / pi), c5 = 0.25 * std::sqrt(35.0 / (2 * pi)),
+                 c6 = 0.5 * std::sqrt(105.0 / pi), c7 = 0.25 * std::sqrt(21.0 / (2 * pi)), c8 = 0.25 * std::sqrt(7.0 / pi),
+                 c9 = 0.25 * std::sqrt(105.0 / pi);
+    o[0] = c0; o[1] = -c1 * y; o[2] = c1 * z; o[3] = -c1 * x;
+    o[4] = c2 * x * y; o[5] = -c2 * y * z; o[6] = c3 * (3 * z * z - 1); o[7] = -c2 * x * z;
+    o[8] = c4 * (x * x - y * y); o[9] = c5 * y * (3 * x * x - y * y) * -1.0; o[10] = c6 * x * y * z;
+    o[11] = c7 * y * (1 - 5 * z * z); o[12] = c8 * z * (5 * z * z - 3); o[13] = c7 * x * (1 - 5 * z * z);
+    o[14] = c9 * z * (x * x - y * y); o[15] = c5 * x * (-x * x + 3 * y * y);
+    dense_d(p, a.cin, 32, 64, true, a.h2);
+    p += 64 * 32;
+    dense_d(p, a.h2, 64, 64, true, a.h3);
+    p += 64 * 64;
+    dense_d(p, a.h3, 64, 3, false, a.out);
+    a.out[3] = a.cin[0];
+}
+
+// backward of one sample: accumulates dL/dparams into g (FP64)
+void backward_d(const orc_scene* s, const SampleAct& a, const double dout[4], double* g) {
+    const uint32_t E = s->cfg.n_levels * 2;
+    const uint64_t W0 = 0, W1 = W0 + 64 * E, W2 = W1 + 16 * 64, W3 = W2 + 64 * 32, W4 = W3 + 64 * 64;
+    const float* P = s->params.data();
+    double dh3[64], dh2[64], dd[16], dh1[64], denc[64];
+    for (int c = 0; c < 3; ++c)
+        for (int k = 0; k < 64; ++k) g[W4 + c * 64 + k] += dout[c] * a.h3[k];
+    for (int k = 0; k < 64; ++k) {
+        double acc = 0.0;
+        for (int c = 0; c < 3; ++c) acc += (double)P[W4 + c * 64 + k] * dout[c];
+        dh3[k] = a.h3[k] > 0.0 ? acc : 0.0;
+    }
+    for (int m = 0; m < 64; ++m)
+        for (int k = 0; k < 64; ++k) g[W3 + m * 64 + k] += dh3[m] * a.h2[k];
+    for (int k = 0; k < 64; ++k) {
+        double acc = 0.0;
+        for (int m = 0; m < 64; ++m) acc += (double)P[W3 + m * 64 + k] * dh3[m];
+        dh2[k] = a.h2[k] > 0.0 ? acc : 0.0;
+    }
+    for (int m = 0; m < 64; ++m)
+        for (int q = 0; q < 32; ++q) g[W2 + m * 32 + q] += dh2[m] * a.cin[q];
+    for (int q = 0; q < 16; ++q) {
+        double acc = 0.0;
+        for (int m = 0; m < 64; ++m) acc += (double)P[W2 + m * 32 + q] * dh2[m];
+        dd[q] = acc;
+    }
+    dd[0] += dout[3];
+    for (int q = 0; q < 16; ++q)
+        for (int k = 0; k < 64; ++k) g[W1 + q * 64 + k] += dd[q] * a.h1[k];
+    for (int k = 0; k < 64; ++k) {
+        double acc = 0.0;
+        for (int q = 0; q < 16; ++q) acc += (double)P[W1 + q * 64 + k] * dd[q];
+        dh1[k] = a.h1[k] > 0.0 ? acc : 0.0;
+    }
+    for (int k = 0; k < 64; ++k)
+        for (uint32_t e = 0; e < E; ++e) g[W0 + k * E + e] += dh1[k] * a.enc[e];
+    for (uint32_t e = 0; e < E; ++e) {
+        double acc = 0.0;
+        for (int k = 0; k < 64; ++k) acc += (double)P[W0 + k * E + e] * dh1[k];
+        denc[e] = acc;
+    }
+    // hash-grid scatter with the FP64 trilinear weights of encode_one_d
+    const insta_layout& L = s->layout;
+    double* gt = g + L.n_mlp_params;
+    const double xs[3] = {a.pos.x, a.pos.y, a.pos.z};
+    for (uint32_t l = 0; l < s->cfg.n_levels; ++l) {
+        double frac[3];
+        uint32_t gi[3];
+        for (int d = 0; d < 3; ++d) {
+            double pp = xs[d] * (double)L.level_scale[l] + 0.5;
+            double fl = std::floor(pp);
+            frac[d] = pp - fl;
+            gi[d] = (uint32_t)(int32_t)fl;
+        }
+        for (uint32_t c = 0; c < 8; ++c) {
+            double w = 1.0;
+            for (int d = 0; d < 3; ++d) w *= ((c >> d) & 1u) ? frac[d] : 1.0 - frac[d];
+            uint32_t idx = grid_index(L, l, gi[0] + (c & 1u), gi[1] + ((c >> 1) & 1u), gi[2] + ((c >> 2) & 1u));
+            const uint64_t e = (uint64_t(L.level_offset[l]) + idx) * 2;
+            gt[e] += w * denc[2 * l];
+            gt[e + 1] += w * denc[2 * l + 1];
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_set_training_set(orc_scene* s, uint32_t n_images, uint32_t width, uint32_t height, const float* rgba,
+                         const insta_camera* cams, const int32_t* frames) {
+    for (uint32_t i = 0; i < n_images; ++i)
+        if (frames[i] < 0 || (uint32_t)frames[i] >= s->frames.size() || !s->frames[frames[i]].built ||
+            s->grids[frames[i] + 1].empty())
+            return 1;
+    s->tr_n = n_images;
+    s->tr_w = width;
+    s->tr_h = height;
+    s->tr_images.assign(rgba, rgba + (size_t)n_images * width * height * 4);
+    s->tr_cams.assign(cams, cams + n_images);
+    s->tr_frames.assign(frames, frames + n_images);
+    return 0;
+}
+
+void orc_pcg32_draws(uint64_t initstate, uint64_t initseq, uint32_t n, uint32_t* out) {
+    Pcg32 r = Pcg32::seeded(initstate, initseq);
+    for (uint32_t i = 0; i < n; ++i) out[i] = r.next_uint();
+}
+
+uint32_t orc_pcg32_next(uint64_t initstate, uint64_t initseq, uint64_t skip) {
+    Pcg32 r = Pcg32::seeded(initstate, initseq);
+    r.advance(skip);
+    return r.next_uint();
+}
+
+// One training batch: rays i < n_rays of the stream seeded by `seed` and
+// advanced by rays_total * 8 (the device's per-step base).  Every ray with
+// samples is used (choose max_steps * n_rays within the device's sample cap
+// when comparing).  grads [n_params] (nullable) receives dL/dparams in FP64;
+// loss[0] = reported loss, loss[1] = the objective L above;
+// counts = {rays used, samples, samples composited}.
+int orc_train_grads(const orc_scene* s, uint32_t n_rays, uint32_t seed, uint64_t rays_total, double* grads,
+                    double* loss, uint32_t* counts) {
+    const insta_config& c = s->cfg;
+    if (s->tr_n == 0 || s->params.empty() || c.n_features_per_level != 2 || c.n_neurons != 64 ||
+        c.density_hidden_layers != 1 || c.rgb_hidden_layers != 2 || c.density_out != 16 || c.n_extra_dims != 0)
+        return 1;
+    Pcg32 rng0 = Pcg32::seeded(seed);
+    rng0.advance(rays_total * kRaySeedStride);
+    const uint64_t np = s->layout.n_params;
+    const double dt = (double)c.dt, bias = (double)c.density_logit_bias, inv_rays = 1.0 / (double)n_rays;
+    int nt = 1;
+#ifdef _OPENMP
+    nt = s->n_threads > 0 ? s->n_threads : omp_get_max_threads();
+#endif
+    std::vector<std::vector<double>> tg(grads ? nt : 0, std::vector<double>(grads ? np : 0, 0.0));
+    double rep = 0.0, obj = 0.0;
+    uint64_t rays_used = 0, n_samples = 0, n_used = 0;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nt) reduction(+ : rep, obj, rays_used, n_samples, n_used)
+    for (int64_t i = 0; i < (int64_t)n_rays; ++i) {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        TrainRay r;
+        if (!train_ray(s, (uint32_t)i, n_rays, rays_total, rng0, r)) continue;
+        rays_used += 1;
+        const uint32_t cnt = (uint32_t)r.samples.size();
+        n_samples += cnt;
+        const MeshFrame& fr = s->frames[s->tr_frames[r.img]];
+        const V3 dir_w = v3((r.d.x + 1.0f) * 0.5f, (r.d.y + 1.0f) * 0.5f, (r.d.z + 1.0f) * 0.5f);
+        std::vector<SampleAct> act(cnt);
+        for (uint32_t j = 0; j < cnt; ++j) {
+            // FP32 deformation (the canonical samples the device path sees:
+            // the nearest triangle is a discontinuous choice, so an FP64
+            // deformation would move near-tie samples to another triangle's
+            // frame); network, loss and backward in FP64 from there
+            V3 cp = r.samples[j], cd = dir_w;
+            uint32_t mask = 0;
+            warp_one(s, fr, r.samples[j], dir_w, cp, cd, mask);
+            const D3v dird = {(double)cd.x * 2.0 - 1.0, (double)cd.y * 2.0 - 1.0, (double)cd.z * 2.0 - 1.0};
+            forward_d(s, dv(cp), dird, act[j]);
+        }
+        const float* tex = &s->tr_images[((size_t)r.img * s->tr_w * s->tr_h + r.pix) * 4];
+        double target[3], bg[3];
+        for (int k = 0; k < 3; ++k) {
+            bg[k] = r.bg[k];
+            target[k] = (double)tex[k] + (1.0 - (double)tex[3]) * bg[k];
+        }
+        double T = 1.0, acc[3] = {0, 0, 0};
+        uint32_t used = 0;
+        for (; used < cnt; ++used) {
+            if (T < 1e-4) break;
+            const double* o = act[used].out;
+            const double sigma = std::exp(o[3] + bias);
+            const double alpha = 1.0 - std::exp(-sigma * dt);
+            const double w = alpha * T;
+            for (int k = 0; k < 3; ++k) acc[k] += w / (1.0 + std::exp(-o[k]));
+            T *= 1.0 - alpha;
+        }
+        n_used += used;
+        if (used == cnt)
+            for (int k = 0; k < 3; ++k) acc[k] += T * bg[k];
+        double grad[3], lsum = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            const double diff = acc[k] - target[k], ad = std::fabs(diff);
+            const double l = ad > 0.1 ? ad - 0.05 : (0.5 / 0.1) * diff * diff;
+            grad[k] = (ad > 0.1 ? (diff > 0 ? 1.0 : -1.0) : diff / 0.1) / 5.0;
+            lsum += l / 5.0;
+        }
+        rep += lsum / 3.0 * inv_rays;
+        obj += lsum * inv_rays;
+        if (!grads) continue;
+        T = 1.0;
+        double acc2[3] = {0, 0, 0};
+        for (uint32_t j = 0; j < used; ++j) {
+            const double* o = act[j].out;
+            double sg[3];
+            for (int k = 0; k < 3; ++k) sg[k] = 1.0 / (1.0 + std::exp(-o[k]));
+            const double sigma = std::exp(o[3] + bias);
+            const double alpha = 1.0 - std::exp(-sigma * dt);
+            const double w = alpha * T;
+            for (int k = 0; k < 3; ++k) acc2[k] += w * sg[k];
+            T *= 1.0 - alpha;
+            double dout[4], dsig = 0.0;
+            for (int k = 0; k < 3; ++k) {
+                dout[k] = inv_rays * w * grad[k] * sg[k] * (1.0 - sg[k]);
+                dsig += grad[k] * (T * sg[k] - (acc[k] - acc2[k]));
+            }
+            dout[3] = inv_rays * sigma * dt * dsig;
+            backward_d(s, act[j], dout, tg[tid].data());
+        }
+    }
+    if (grads) {
+        for (uint64_t p2 = 0; p2 < np; ++p2) {
+            double sum = 0.0;
+            for (int t = 0; t < nt; ++t) sum += tg[t][p2];
+            grads[p2] = sum;
+        }
+    }
+    if (loss) {
+        loss[0] = rep;
+        loss[1] = obj;
+    }
+    if (counts) {
+        counts[0] = (uint32_t)rays_used;
+        counts[1] = (uint32_t)n_samples;
+        counts[2] = (uint32_t)n_used;
     }
     return 0;
 }

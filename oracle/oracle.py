@@ -52,6 +52,10 @@ _SIGS = [
                              C.c_uint32, _VP, C.c_uint32, _VP, _VP, _VP, _VP]),
     ("orc_render_fp64", C.c_int, [_VP, C.POINTER(InstaCamera), C.c_uint32, C.c_uint32, C.c_int32, C.c_int32,
                                   C.c_uint32, _VP, C.c_uint32, _VP]),
+    ("orc_set_training_set", C.c_int, [_VP, C.c_uint32, C.c_uint32, C.c_uint32, _VP, _VP, _VP]),
+    ("orc_pcg32_next", C.c_uint32, [C.c_uint64, C.c_uint64, C.c_uint64]),
+    ("orc_pcg32_draws", None, [C.c_uint64, C.c_uint64, C.c_uint32, _VP]),
+    ("orc_train_grads", C.c_int, [_VP, C.c_uint32, C.c_uint32, C.c_uint64, _VP, _VP, _VP]),
 ]
 
 _lib = None
@@ -204,6 +208,38 @@ class Oracle:
                                0 if sub is None else len(sub), _p(rgba), _p(depth), _p(steps), _p(stats))
         assert rc == 0, rc
         return rgba, depth, steps, stats
+
+
+def set_training_set(orc: Oracle, images, cams, frames) -> None:
+    """images [n, h, w, 4] float32 (host), cams: list of InstaCamera, frames: ints."""
+    im = _c(images, np.float32)
+    n, h, w, _ = im.shape
+    arr = (InstaCamera * n)(*cams)
+    fr = _c(frames, np.int32)
+    assert orc.L.orc_set_training_set(orc.h, n, w, h, _p(im), C.addressof(arr), _p(fr)) == 0
+
+
+def train_grads(orc: Oracle, n_rays: int, seed: int, rays_total: int = 0, want_grads: bool = True):
+    """FP64 training batch: (grads [n_params] float64 or None, (reported loss, objective),
+    (rays used, samples, samples composited))."""
+    g = np.zeros(orc.layout.n_params, np.float64) if want_grads else None
+    loss = np.zeros(2, np.float64)
+    counts = np.zeros(3, np.uint32)
+    rc = orc.L.orc_train_grads(orc.h, n_rays, seed, rays_total, _p(g), _p(loss), _p(counts))
+    assert rc == 0, rc
+    return g, (float(loss[0]), float(loss[1])), tuple(int(c) for c in counts)
+
+
+def pcg32_draws(initstate: int, initseq: int, n: int) -> np.ndarray:
+    """First n outputs of PCG32 seeded (initstate, initseq), drawn one by one."""
+    out = np.zeros(n, np.uint32)
+    lib().orc_pcg32_draws(initstate, initseq, n, _p(out))
+    return out
+
+
+def pcg32_next(initstate: int, initseq: int, skip: int = 0) -> int:
+    """Output of PCG32 seeded (initstate, initseq) after `skip` draws."""
+    return lib().orc_pcg32_next(initstate, initseq, skip)
 
 
 def render_fp64(orc: Oracle, cam: InstaCamera, width: int, height: int, frame: int, occupancy: int, spp: int = 0,
