@@ -125,30 +125,112 @@ def accuracy_vs_fp64(ctxs, cfg, av, params, cam, res, n_pix):
 
 
 def cpu_baseline(cfg, av, params, cam, res, budget_s):
-    """The oracle on the host cores for a bounded pixel subset of frame 0."""
+    """The oracle on the host cores: whole 512x512 frames of posed frame 0 at
+    successive sample indices (new jitter each), until about budget_s."""
     from oracle.oracle import Oracle
     orc = Oracle(cfg)
     orc.set_canonical_mesh(av.canonical, av.faces, av.masks)
     orc.set_frame(0, av.frames[0])
     orc.set_params(params)
     orc.build_occupancy(0, 0.015)
-    rng = np.random.default_rng(7)
-    n = 1024
-    pix = rng.permutation(res * res).astype(np.uint32)
-    t0 = time.perf_counter()
-    _, _, _, st = orc.render(cam, res, res, 0, 0, 0, pix[:n])
-    dt = time.perf_counter() - t0
-    samples, secs, used = int(st[0]), dt, n
-    # scale the pixel count so the timed run takes about budget_s
-    if dt < budget_s / 4:
-        n2 = int(min(res * res, n * budget_s / max(dt, 1e-3)))
+    samples, secs, frames = 0, 0.0, 0
+    while secs < budget_s:
         t0 = time.perf_counter()
-        _, _, _, st = orc.render(cam, res, res, 0, 0, 0, pix[:n2])
-        secs = time.perf_counter() - t0
-        samples, used = int(st[0]), n2
+        _, _, _, st = orc.render(cam, res, res, 0, 0, frames)
+        secs += time.perf_counter() - t0
+        samples += int(st[0])
+        frames += 1
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return dict(value=samples / secs, unit="samples/s", cores=threads, kind="port",
-                sample=f"{used} random pixels of one {res}x{res} frame ({samples} samples, {secs:.1f} s)")
+                sample=f"{frames} full {res}x{res} frames of posed frame 0 (spp 0..{frames - 1}), "
+                       f"{samples} samples in {secs:.1f} s", frames_per_s=frames / secs)
+
+
+def training_bench(args):
+    """Training step at the small shapes (L=8, T=2^14, 64^3 grid, 2,048-triangle
+    sphere): rays uniform over 64 posed frames seen by 64 random 512x512
+    cameras with smooth-noise targets, a 2^18-sample training batch per step
+    (adaptive ray count; marched samples are evaluated, the composited ones
+    compacted into the batch), Adam.  FP32 and FP16 grid-gradient accumulation (loss scale
+    128); gradient error of one batch against the FP64 oracle."""
+    import torch
+    from insta_amd import synth
+    from insta_amd.capi import Context, default_train_config, make_camera, small_config
+    n_frames, res = 64, 512
+    cfg = small_config(max_frames=n_frames)
+    av = synth.small_sphere_avatar(seed=0, n_frames=n_frames)
+    rng = np.random.default_rng(21)
+    cams = [make_camera(synth.random_camera(rng, 1.0, front_bias=False), synth.focal_from_fov(res, 30.0))
+            for _ in range(n_frames)]
+    images = synth.smooth_noise_images(5, n_frames, res, res)
+    images_dev = torch.from_numpy(images).cuda()
+    frames_dev = torch.from_numpy(np.stack(av.frames)).cuda()
+
+    def make_ctx():
+        ctx = Context(cfg)
+        ctx.set_canonical_mesh(av.canonical, av.faces, av.masks)
+        ctx.set_frames(0, frames_dev)
+        params = synth.network_params(7, ctx.layout, cfg)
+        ctx.set_params(torch.from_numpy(params).cuda())
+        for f in range(n_frames):
+            ctx.build_occupancy(f, 0.05)
+        ctx.set_training_set(images_dev, cams, list(range(n_frames)))
+        return ctx, params
+
+    out = {}
+    for name, kw in (("fp32_grid_grads", {}), ("fp16_grid_grads", dict(grid_grad_fp16=1, loss_scale=128.0))):
+        ctx, _ = make_ctx()
+        tc = default_train_config(seed=3, **kw)
+        first = None
+        for _ in range(args.train_warmup):
+            st = ctx.train_step(tc)
+            first = st.loss if first is None else first
+        torch.cuda.synchronize()
+        samples = rays = batch = 0
+        t0 = time.perf_counter()
+        for _ in range(args.train_steps):
+            st = ctx.train_step(tc)
+            samples += st.n_samples
+            rays += st.n_rays
+            batch += min(st.n_samples_used, tc.target_samples)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        k = args.train_steps
+        out[name] = dict(steps_per_s=k / dt, ms_per_step=dt / k * 1e3, batch_samples_per_step=batch / k,
+                         marched_samples_per_step=samples / k, rays_per_step=rays / k,
+                         batch_samples_per_s=batch / dt, loss_first=first, loss_last=st.loss)
+        del ctx
+
+    # gradient error of one batch against the FP64 oracle (same rays and samples;
+    # a fresh context per variant so both see the batch of step 0)
+    from oracle import oracle as O
+    n_rays = 4096
+    grads = {}
+    for name, kw in (("fp32", {}), ("fp16_grid", dict(grid_grad_fp16=1, loss_scale=128.0))):
+        ctx, params = make_ctx()
+        tc = default_train_config(n_rays=n_rays, max_samples=n_rays * cfg.max_steps,
+                                  target_samples=n_rays * cfg.max_steps, apply_update=0, seed=17, **kw)
+        ctx.train_step(tc)
+        grads[name] = ctx.gradients().cpu().numpy().astype(np.float64)
+        del ctx
+    orc = O.Oracle(cfg)
+    orc.set_canonical_mesh(av.canonical, av.faces, av.masks)
+    for f in range(n_frames):
+        orc.set_frame(f, av.frames[f])
+        orc.build_occupancy(f, 0.05)
+    orc.set_params(params)
+    O.set_training_set(orc, images, cams, list(range(n_frames)))
+    g64, _, counts = O.train_grads(orc, n_rays, 17)
+    lay = orc.layout
+    b = np.cumsum([0] + [o * i for (o, i) in lay.mlp_shapes(cfg)] + [lay.n_grid_params])
+    names = ["density_W0", "density_W1", "colour_W0", "colour_W1", "colour_W2", "hash_tables"]
+    rel = lambda a, r: float(np.linalg.norm(a - r) / (np.linalg.norm(r) + 1e-30))
+    out["grad_rel_l2_vs_fp64"] = {"batch": f"{n_rays} rays, {counts[1]} samples"}
+    for name, g in grads.items():
+        out["grad_rel_l2_vs_fp64"][name] = {n: rel(g[b0:b1], g64[b0:b1]) for n, b0, b1 in zip(names, b[:-1], b[1:])}
+    out["workload"] = ("small shapes (L8 T2^14, 64^3 grid), 64 posed frames x 512x512 random cameras, "
+                       "~2^18 samples/step, Adam(0.9, 0.99, lr 1e-2)")
+    return out
 
 
 def main():
@@ -160,6 +242,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds of CPU oracle timing (0: skip)")
     ap.add_argument("--skip-forms", action="store_true", help="only the FP32 path")
     ap.add_argument("--accuracy-pixels", type=int, default=4096, help="pixels compared with the FP64 oracle")
+    ap.add_argument("--skip-train", action="store_true", help="skip the training-step measurement")
+    ap.add_argument("--train-steps", type=int, default=40)
+    ap.add_argument("--train-warmup", type=int, default=8)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 256 MiB device copy first (reference kernel for rocprof byte counters)")
     args = ap.parse_args()
@@ -221,6 +306,8 @@ def main():
         if args.accuracy_pixels > 0:
             forms["accuracy_vs_fp64"] = accuracy_vs_fp64(ctxs, cfg, av, params, cams[0], res, args.accuracy_pixels)
 
+    train = None if args.skip_train else training_bench(args)
+
     cpu = None
     if args.cpu_budget > 0:
         cpu = cpu_baseline(cfg, av, params, cams[0], res, args.cpu_budget)
@@ -247,6 +334,7 @@ def main():
         "roofline": roof,
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "training": train,
     }
     print(json.dumps(line))
 

@@ -247,8 +247,8 @@ insta_status insta_set_training_set(insta_ctx* ctx, uint32_t n_images, uint32_t 
                                     const int32_t* frames, void* stream);
 
 typedef struct insta_train_config {
-    uint32_t n_rays;          /* rays this step; 0 = adaptive count (target_samples) */
-    uint32_t target_samples;  /* samples per step the adaptive ray count aims at */
+    uint32_t n_rays;          /* rays this step; 0 = adaptive count (aims at target_samples) */
+    uint32_t target_samples;  /* training batch: composited samples compacted per step (cap) */
     uint32_t max_samples;     /* cap on samples marched per step */
     float    learning_rate;   /* Adam */
     float    beta1, beta2, epsilon;
@@ -267,18 +267,21 @@ typedef struct insta_train_stats {
     uint32_t step;            /* optimizer steps taken so far */
 } insta_train_stats;
 
-/* Defaults: adaptive rays aiming at 2^18 samples, Adam (0.9, 0.99, 1e-15) at
- * learning rate 1e-2, FP32 grid gradients, loss scale 1, seed 1337. */
+/* Defaults: adaptive rays aiming at a 2^18-sample batch, 2^22 marched samples
+ * at most, Adam (0.9, 0.99, 1e-15) at learning rate 1e-2, FP32 grid
+ * gradients, loss scale 1, seed 1337. */
 void insta_default_train_config(insta_train_config* tc);
 
 /* One training step, Testbed::train_nerf_step (testbed_nerf.cu:3176-3400):
  * rays from random training pixels (generate_training_samples_nerf,
  * 1105-1285) marched through their frame's occupancy grid, deformed to
- * canonical space (the warp hook, core.cu:442-526), network forward, loss and
- * dL/doutput (compute_loss_kernel_train_nerf, 1305-1643: Huber/5 at 0.1,
- * product transmittance, random background), hand-written backward through
- * both MLPs and the hash grid (nerf_network.h:296-383), Adam.  Reads one
- * block of counters back to the host (the adaptive ray count, stats). */
+ * canonical space (the warp hook, core.cu:442-526), network inference, loss
+ * and dL/doutput (compute_loss_kernel_train_nerf, 1305-1643: Huber/5 at 0.1,
+ * product transmittance, random background) with the composited samples
+ * compacted into the training batch, FP32 forward and hand-written backward
+ * through both MLPs and the hash grid on that batch (nerf_network.h:296-383),
+ * Adam.  Reads one block of counters back to the host (the adaptive ray
+ * count, stats). */
 insta_status insta_train_step(insta_ctx* ctx, const insta_train_config* tc, insta_train_stats* stats,
                               void* stream);
 

@@ -578,7 +578,7 @@ void insta_default_train_config(insta_train_config* tc) {
     if (!tc) return;
     tc->n_rays = 0;
     tc->target_samples = 1u << 18;
-    tc->max_samples = 1u << 20;
+    tc->max_samples = 1u << 22;
     tc->learning_rate = 1e-2f;
     tc->beta1 = 0.9f;
     tc->beta2 = 0.99f;
@@ -620,7 +620,8 @@ insta_status insta_set_training_set(insta_ctx* ctx, uint32_t n_images, uint32_t 
 insta_status insta_train_step(insta_ctx* ctx, const insta_train_config* tc, insta_train_stats* stats, void* stream) {
     INSTA_CHECK_CTX(ctx);
     if (!tc || tc->max_samples == 0 || tc->max_samples > (1u << 26) || tc->n_rays > (1u << 24) ||
-        !(tc->loss_scale > 0.0f) || !(tc->learning_rate >= 0.0f) || (tc->n_rays == 0 && tc->target_samples == 0)) {
+        tc->target_samples == 0 || tc->target_samples > (1u << 24) || !(tc->loss_scale > 0.0f) ||
+        !(tc->learning_rate >= 0.0f)) {
         set_error("train_step: bad training configuration");
         return INSTA_ERR_INVALID;
     }

@@ -170,7 +170,10 @@ insta_status launch_encode_backward(insta_ctx* ctx, const float* pos, uint32_t n
                                     hipStream_t stream);
 insta_status launch_network_aos(insta_ctx* ctx, const float* coords, uint32_t n, uint32_t stride, float* out,
                                 hipStream_t stream);
-insta_status launch_network_soa(insta_ctx* ctx, const uint32_t* n_dev, uint32_t n_max, hipStream_t stream);
+insta_status launch_network_soa(insta_ctx* ctx, const uint32_t* n_dev, uint32_t n_max, const float4* cpos,
+                                const float4* cdir, float4* out, hipStream_t stream);
+insta_status launch_network_samples(insta_ctx* ctx, const uint32_t* n_dev, uint32_t n_max, const float4* cpos,
+                                    const float4* cdir, float4* out, hipStream_t stream);
 bool network_supported(const insta_config& cfg);
 bool mfma_network_supported(const insta_config& cfg);
 // FP32 arithmetic that the per-sample (scalar FMA) MLP kernel implements
@@ -178,7 +181,8 @@ inline bool scalar_precision(const insta_config& c) {
     return c.precision == INSTA_PRECISION_FP32 || c.precision == INSTA_PRECISION_FP32_SCALAR;
 }
 insta_status prepare_network(insta_ctx* ctx, hipStream_t stream);
-insta_status launch_network_mfma_soa(insta_ctx* ctx, const uint32_t* n_dev, uint32_t n_max, hipStream_t stream);
+insta_status launch_network_mfma_soa(insta_ctx* ctx, const uint32_t* n_dev, uint32_t n_max, const float4* cpos,
+                                     const float4* cdir, float4* out, hipStream_t stream);
 insta_status launch_network_mfma_aos(insta_ctx* ctx, const float* coords, uint32_t n, uint32_t stride, float* out,
                                      hipStream_t stream);
 

@@ -203,12 +203,13 @@ insta_status launch_network_aos(insta_ctx* ctx, const float* coords, uint32_t n,
     return INSTA_ERR_UNSUPPORTED;
 }
 
-insta_status launch_network_soa(insta_ctx* ctx, const uint32_t* n_dev, uint32_t n_max, hipStream_t stream) {
+insta_status launch_network_soa(insta_ctx* ctx, const uint32_t* n_dev, uint32_t n_max, const float4* cpos,
+                                const float4* cdir, float4* out, hipStream_t stream) {
     if (n_max == 0) return INSTA_OK;
     const NetArgs a = net_args(ctx);
     const dim3 grid((n_max + kNetThreads - 1) / kNetThreads), block(kNetThreads);
-    INSTA_DISPATCH_LF(8, 2, network_soa_kernel, grid, block, 0, stream, a, n_dev, ctx->s_cpos, ctx->s_cdir, ctx->s_out)
-    INSTA_DISPATCH_LF(16, 2, network_soa_kernel, grid, block, 0, stream, a, n_dev, ctx->s_cpos, ctx->s_cdir, ctx->s_out)
+    INSTA_DISPATCH_LF(8, 2, network_soa_kernel, grid, block, 0, stream, a, n_dev, cpos, cdir, out)
+    INSTA_DISPATCH_LF(16, 2, network_soa_kernel, grid, block, 0, stream, a, n_dev, cpos, cdir, out)
     set_error("network: unsupported (n_levels, n_features_per_level)");
     return INSTA_ERR_UNSUPPORTED;
 }
@@ -243,6 +244,13 @@ insta_status launch_encode_backward(insta_ctx* ctx, const float* pos, uint32_t n
     }
     INSTA_HIP(hipGetLastError());
     return INSTA_OK;
+}
+
+// canonical samples (cpos, cdir) -> out through the context's inference path
+insta_status launch_network_samples(insta_ctx* ctx, const uint32_t* n_dev, uint32_t n_max, const float4* cpos,
+                                    const float4* cdir, float4* out, hipStream_t stream) {
+    return mfma_network_supported(ctx->cfg) ? launch_network_mfma_soa(ctx, n_dev, n_max, cpos, cdir, out, stream)
+                                            : launch_network_soa(ctx, n_dev, n_max, cpos, cdir, out, stream);
 }
 
 }  // namespace insta

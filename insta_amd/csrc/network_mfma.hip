@@ -499,9 +499,10 @@ static insta_status launch_mfma(insta_ctx* ctx, Src src, uint32_t n_max, hipStre
     return INSTA_OK;
 }
 
-insta_status launch_network_mfma_soa(insta_ctx* ctx, const uint32_t* n_dev, uint32_t n_max, hipStream_t stream) {
+insta_status launch_network_mfma_soa(insta_ctx* ctx, const uint32_t* n_dev, uint32_t n_max, const float4* cpos,
+                                     const float4* cdir, float4* out, hipStream_t stream) {
     if (n_max == 0) return INSTA_OK;
-    SoaSrc s{ctx->s_cpos, ctx->s_cdir, ctx->s_out, n_dev};
+    SoaSrc s{cpos, cdir, out, n_dev};
     return launch_mfma(ctx, s, n_max, stream);
 }
 

@@ -376,8 +376,7 @@ insta_status render(insta_ctx* ctx, const insta_camera* cam, uint32_t width, uin
         if (st != INSTA_OK) return st;
         prof_end(ctx, INSTA_KERNEL_DEFORM, 0, stream);
         prof_begin(ctx, INSTA_KERNEL_NETWORK, stream);
-        st = mfma_network_supported(ctx->cfg) ? launch_network_mfma_soa(ctx, cnt + kCntSamples, n_max, stream)
-                                              : launch_network_soa(ctx, cnt + kCntSamples, n_max, stream);
+        st = launch_network_samples(ctx, cnt + kCntSamples, n_max, ctx->s_cpos, ctx->s_cdir, ctx->s_out, stream);
         if (st != INSTA_OK) return st;
         prof_end(ctx, INSTA_KERNEL_NETWORK, 0, stream);
         prof_begin(ctx, INSTA_KERNEL_COMPOSITE, stream);

@@ -10,28 +10,29 @@
 //             max(t_box, 0) + dt * jitter, at most max_steps samples; a ray
 //             whose sample range would cross the cap is dropped (1239-1242)
 //   deform    posed -> canonical with the frame of each ray (core.cu:442-526)
-//   forward   hash grid + density MLP + colour MLP per sample (the FP32 path
-//             of the inference kernels), keeping every activation the
-//             backward needs, feature-major [feature][sample]
+//   infer     the context's inference network (MFMA) on every marched sample
 //   loss      compute_loss_kernel_train_nerf (1305-1643): product
 //             transmittance T *= 1 - alpha until T < 1e-4, random background
 //             (the same PCG32 stream: draws 2-4 of the ray), Huber(0.1) / 5
 //             per channel, dL/d(rgb logit) = w * dL/drgb * s(1 - s),
 //             dL/d(density logit) = sigma * dt * dL/drgb . (T rgb - suffix),
-//             scaled by loss_scale / n_rays
-//   backward  per sample through both MLPs (ReLU masks from the stored
-//             activations) into dL/dencoding, then scattered into the hash
-//             tables: FP32 atomics, or packed FP16 atomics (one
+//             scaled by loss_scale / n_rays; the composited samples are
+//             compacted into a batch of at most target_samples (1543-1560)
+//   forward   on the compacted batch: hash grid + both MLPs in FP32 (the
+//             plain path's FMA chains), keeping every activation the backward
+//             needs, feature-major [feature][sample]
+//   backward  per compacted sample through both MLPs (ReLU masks from the
+//             stored activations) into dL/dencoding, then scattered into the
+//             hash tables: FP32 atomics, or packed FP16 atomics (one
 //             global_atomic_pk_add_f16 per corner and level, F = 2)
 //   wgrad     weight gradients dW = sum_n delta_n x_n^T as per-chunk partial
 //             products (LDS tiles, 4x4 register blocks), then a fixed-order
 //             sum over chunks: deterministic MLP gradients
 //   adam      Kingma & Ba with bias correction over all parameters, after
 //             removing the loss scale
-// Declared simplifications against the reference: no loss compaction to a
-// fixed batch (every sample is back-propagated, those past the termination
-// with zero gradient), no density/colour regularisers, no error map, no
-// extra dims, background not passed through sRGB<->linear.
+// Declared simplifications against the reference: no density/colour
+// regularisers, no error map, no extra dims, background not passed through
+// sRGB<->linear, rays never masked out.
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
@@ -59,6 +60,8 @@ enum : uint32_t {
     kTValid = 3,     // samples marched into the buffers
     kTUsed = 4,      // samples composited before termination
     kTLoss = 5,      // float bits: sum of per-ray loss / n_rays
+    kTCompactRes = 6,  // compacted samples reserved
+    kTCompactN = 7,    // compacted samples kept (<= compact capacity)
     kTCounters = 8
 };
 
@@ -92,12 +95,14 @@ struct TrainState {
     TrainImage* meta = nullptr;
     std::vector<int32_t> frames;
     // workspace
-    uint32_t ray_cap = 0, sample_cap = 0;
+    uint32_t ray_cap = 0, sample_cap = 0, compact_cap = 0;
     float4* ray_d = nullptr;
     int32_t* ray_frame = nullptr;
     uint32_t *ray_img = nullptr, *ray_pix = nullptr, *ray_src = nullptr, *ray_base = nullptr, *ray_cnt = nullptr;
     float4 *s_pos = nullptr, *s_cpos = nullptr, *s_cdir = nullptr, *s_out = nullptr, *s_dout = nullptr;
     int32_t* s_hit = nullptr;
+    uint32_t* c_idx = nullptr;     // compacted batch: sample index
+    float4* c_dout = nullptr;      // compacted batch: dL/d(rgb logits, density logit)
     float *act = nullptr, *dact = nullptr, *part = nullptr;
     // optimizer
     float *grad = nullptr, *adam_m = nullptr, *adam_v = nullptr;
@@ -124,6 +129,10 @@ __global__ void fill_zero_kernel(uint32_t* p, uint64_t n) {
 
 __global__ void train_count_kernel(uint32_t* counters) {
     if (threadIdx.x == 0) counters[kTValid] = min(counters[kTSamples], counters[kTLimit]);
+}
+
+__global__ void train_compact_count_kernel(uint32_t* counters, uint32_t cap) {
+    if (threadIdx.x == 0) counters[kTCompactN] = min(counters[kTCompactRes], cap);
 }
 
 struct RayArgs {
@@ -212,28 +221,30 @@ INSTA_DEV void store_rows(float* __restrict__ act, uint32_t ld, int row0, uint32
     for (int k = 0; k < NOUT; ++k) act[(size_t)(row0 + k) * ld + i] = v[k];
 }
 
-// Forward with stored activations; the arithmetic is that of the plain FP32
-// inference kernel (same FMA chains), so outputs match it bit for bit.
+// Forward with stored activations over the compacted batch; the arithmetic
+// is that of the plain FP32 inference kernel (same FMA chains).  The output
+// layer is not needed: dL/doutput comes from the loss kernel.
 template <int L>
 __global__ __launch_bounds__(kTrainThreads) void train_forward_kernel(const float* __restrict__ params,
                                                                       const float* __restrict__ table, GridParams gp,
                                                                       const uint32_t* __restrict__ n_dev,
+                                                                      const uint32_t* __restrict__ c_idx,
                                                                       const float4* __restrict__ cpos,
                                                                       const float4* __restrict__ cdir,
-                                                                      float* __restrict__ act, uint32_t ld,
-                                                                      float4* __restrict__ out) {
+                                                                      float* __restrict__ act, uint32_t ld) {
     constexpr int E = 2 * L;
     using A = Act<E>;
     using O = Off<E>;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= *n_dev) return;
-    const float4 p4 = cpos[i], d4 = cdir[i];
+    const uint32_t smp = c_idx[i];
+    const float4 p4 = cpos[smp], d4 = cdir[smp];
     const V3 x = v3(p4.x, p4.y, p4.z);
     float enc[E];
 #pragma unroll
     for (int l = 0; l < L; ++l) encode_level<2>(table, gp, (uint32_t)l, x, enc + 2 * l);
     store_rows<E, E>(act, ld, A::ENC, i, enc);
-    float h[kW], g[kW], r[kCin], rgb[3];
+    float h[kW], g[kW], r[kCin];
     dense<E, kW, true>(params + O::W0, enc, h);
     store_rows<kW, kW>(act, ld, A::H1, i, h);
     dense<kW, kDOut, false>(params + O::W1, h, r);
@@ -243,8 +254,6 @@ __global__ __launch_bounds__(kTrainThreads) void train_forward_kernel(const floa
     store_rows<kW, kW>(act, ld, A::H2, i, h);
     dense<kW, kW, true>(params + O::W3, h, g);
     store_rows<kW, kW>(act, ld, A::H3, i, g);
-    dense<kW, 3, false>(params + O::W4, g, rgb);
-    out[i] = make_float4(rgb[0], rgb[1], rgb[2], r[0]);
 }
 
 struct LossArgs {
@@ -261,12 +270,16 @@ __global__ __launch_bounds__(256) void train_loss_kernel(LossArgs a, Pcg32 rng0,
                                                          const uint32_t* __restrict__ ray_base,
                                                          const uint32_t* __restrict__ ray_cnt,
                                                          const float4* __restrict__ s_out,
-                                                         float4* __restrict__ dout, uint32_t* __restrict__ counters) {
+                                                         uint32_t* __restrict__ c_idx, float4* __restrict__ c_dout,
+                                                         uint32_t compact_cap, uint32_t* __restrict__ counters) {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = r < counters[kTRays];
     float ray_loss = 0.0f;
-    uint32_t used = 0;
-    if (r < counters[kTRays]) {
-        const uint32_t base = ray_base[r], cnt = ray_cnt[r];
+    uint32_t used = 0, base = 0;
+    float acc[3] = {0.0f, 0.0f, 0.0f}, grad[3] = {0.0f, 0.0f, 0.0f};
+    if (valid) {
+        base = ray_base[r];
+        const uint32_t cnt = ray_cnt[r];
         Pcg32 rng = rng0;
         rng.advance((uint64_t)ray_src[r] * kRaySeedStride);
         (void)rng.next_float();
@@ -276,7 +289,6 @@ __global__ __launch_bounds__(256) void train_loss_kernel(LossArgs a, Pcg32 rng0,
         const float target[3] = {tex.x + (1.0f - tex.w) * bg[0], tex.y + (1.0f - tex.w) * bg[1],
                                  tex.z + (1.0f - tex.w) * bg[2]};
         float T = 1.0f;
-        float acc[3] = {0.0f, 0.0f, 0.0f};
         for (; used < cnt; ++used) {
             if (T < 1e-4f) break;
             const float4 o = s_out[base + used];
@@ -292,7 +304,6 @@ __global__ __launch_bounds__(256) void train_loss_kernel(LossArgs a, Pcg32 rng0,
             for (int c = 0; c < 3; ++c) acc[c] = acc[c] + T * bg[c];
         }
         // Huber(0.1) / 5 per channel (testbed_nerf.cu:141-157, 1299)
-        float grad[3];
         for (int c = 0; c < 3; ++c) {
             const float diff = acc[c] - target[c];
             const float ad = fabsf(diff);
@@ -301,26 +312,31 @@ __global__ __launch_bounds__(256) void train_loss_kernel(LossArgs a, Pcg32 rng0,
             ray_loss += l / 5.0f;
         }
         ray_loss = ray_loss / 3.0f * a.inv_rays;
+    }
+    // compaction of the composited samples into the training batch
+    const uint32_t cb = wave_reserve(&counters[kTCompactRes], used);
+    const uint32_t n_c = cb < compact_cap ? min(used, compact_cap - cb) : 0u;
+    if (valid) {
         const float ls = a.loss_scale_per_ray;
-        T = 1.0f;
+        float T = 1.0f;
         float acc2[3] = {0.0f, 0.0f, 0.0f};
-        for (uint32_t j = 0; j < used; ++j) {
+        for (uint32_t j = 0; j < n_c; ++j) {
             const float4 o = s_out[base + j];
-            const float s[3] = {sigmoid(o.x), sigmoid(o.y), sigmoid(o.z)};
+            const float sg[3] = {sigmoid(o.x), sigmoid(o.y), sigmoid(o.z)};
             const float sigma = expf(o.w + a.density_bias);
             const float alpha = 1.0f - expf(-sigma * a.dt);
             const float w = alpha * T;
-            for (int c = 0; c < 3; ++c) acc2[c] = acc2[c] + w * s[c];
+            for (int c = 0; c < 3; ++c) acc2[c] = acc2[c] + w * sg[c];
             T = T * (1.0f - alpha);
             float dsig = 0.0f;
             float drgb[3];
             for (int c = 0; c < 3; ++c) {
-                drgb[c] = ls * (w * grad[c] * (s[c] * (1.0f - s[c])));
-                dsig = dsig + grad[c] * (T * s[c] - (acc[c] - acc2[c]));
+                drgb[c] = ls * (w * grad[c] * (sg[c] * (1.0f - sg[c])));
+                dsig = dsig + grad[c] * (T * sg[c] - (acc[c] - acc2[c]));
             }
-            dout[base + j] = make_float4(drgb[0], drgb[1], drgb[2], ls * (sigma * (a.dt * dsig)));
+            c_idx[cb + j] = base + j;
+            c_dout[cb + j] = make_float4(drgb[0], drgb[1], drgb[2], ls * (sigma * (a.dt * dsig)));
         }
-        for (uint32_t j = used; j < cnt; ++j) dout[base + j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     // statistics
     uint32_t u = used;
@@ -341,6 +357,7 @@ __global__ __launch_bounds__(256) void train_loss_kernel(LossArgs a, Pcg32 rng0,
 template <int L, bool H16>
 __global__ __launch_bounds__(kTrainThreads) void train_backward_kernel(const float* __restrict__ params, GridParams gp,
                                                                        const uint32_t* __restrict__ n_dev,
+                                                                       const uint32_t* __restrict__ c_idx,
                                                                        const float4* __restrict__ cpos,
                                                                        const float* __restrict__ act,
                                                                        const float4* __restrict__ dout,
@@ -404,7 +421,7 @@ __global__ __launch_bounds__(kTrainThreads) void train_backward_kernel(const flo
         denc[e] = s;
     }
     // hash-grid scatter (the corner/weight arithmetic of encode_level)
-    const float4 p4 = cpos[i];
+    const float4 p4 = cpos[c_idx[i]];
     const float xs[3] = {p4.x, p4.y, p4.z};
 #pragma unroll 1
     for (int l = 0; l < L; ++l) {
@@ -538,14 +555,18 @@ bool train_alloc(T*& p, size_t bytes) {
 void free_workspace_train(TrainState* ts) {
     dev_free(ts->ray_d); dev_free(ts->ray_frame); dev_free(ts->ray_img); dev_free(ts->ray_pix);
     dev_free(ts->ray_src); dev_free(ts->ray_base); dev_free(ts->ray_cnt);
-    dev_free(ts->s_pos); dev_free(ts->s_cpos); dev_free(ts->s_cdir); dev_free(ts->s_out); dev_free(ts->s_dout);
-    dev_free(ts->s_hit); dev_free(ts->act); dev_free(ts->dact); dev_free(ts->part);
-    ts->ray_cap = ts->sample_cap = 0;
+    dev_free(ts->s_pos); dev_free(ts->s_cpos); dev_free(ts->s_cdir); dev_free(ts->s_out); dev_free(ts->s_hit);
+    dev_free(ts->c_idx); dev_free(ts->c_dout); dev_free(ts->act); dev_free(ts->dact); dev_free(ts->part);
+    ts->ray_cap = ts->sample_cap = ts->compact_cap = 0;
 }
 
-insta_status ensure_train_workspace(insta_ctx* ctx, TrainState* ts, uint32_t n_rays, uint32_t n_samples) {
-    if (n_rays <= ts->ray_cap && n_samples <= ts->sample_cap) return INSTA_OK;
+// rays, marched samples and the compacted batch grow independently to the
+// largest request seen
+insta_status ensure_train_workspace(insta_ctx* ctx, TrainState* ts, uint32_t n_rays, uint32_t n_samples,
+                                    uint32_t n_compact) {
+    if (n_rays <= ts->ray_cap && n_samples <= ts->sample_cap && n_compact <= ts->compact_cap) return INSTA_OK;
     const uint32_t rc = std::max(n_rays, ts->ray_cap), sc = std::max(n_samples, ts->sample_cap);
+    const uint32_t cc = std::max(n_compact, ts->compact_cap);
     free_workspace_train(ts);
     const int E = (int)(ctx->cfg.n_levels * 2);
     const size_t act_rows = (size_t)E + 224, n_mlp = ctx->layout.n_mlp_params;
@@ -555,10 +576,11 @@ insta_status ensure_train_workspace(insta_ctx* ctx, TrainState* ts, uint32_t n_r
     ok = ok && train_alloc(ts->ray_base, rc * 4) && train_alloc(ts->ray_cnt, rc * 4);
     ok = ok && train_alloc(ts->s_pos, sc * sizeof(float4)) && train_alloc(ts->s_cpos, sc * sizeof(float4));
     ok = ok && train_alloc(ts->s_cdir, sc * sizeof(float4)) && train_alloc(ts->s_out, sc * sizeof(float4));
-    ok = ok && train_alloc(ts->s_dout, sc * sizeof(float4)) && train_alloc(ts->s_hit, sc * sizeof(int32_t));
-    ok = ok && train_alloc(ts->act, act_rows * sc * sizeof(float));
-    ok = ok && train_alloc(ts->dact, (size_t)Dact::N * sc * sizeof(float));
-    ok = ok && train_alloc(ts->part, ((sc + kChunk - 1) / kChunk) * n_mlp * sizeof(float));
+    ok = ok && train_alloc(ts->s_hit, sc * sizeof(int32_t));
+    ok = ok && train_alloc(ts->c_idx, cc * sizeof(uint32_t)) && train_alloc(ts->c_dout, cc * sizeof(float4));
+    ok = ok && train_alloc(ts->act, act_rows * cc * sizeof(float));
+    ok = ok && train_alloc(ts->dact, (size_t)Dact::N * cc * sizeof(float));
+    ok = ok && train_alloc(ts->part, ((cc + kChunk - 1) / kChunk) * n_mlp * sizeof(float));
     if (!ok) {
         free_workspace_train(ts);
         set_error("training workspace allocation failed");
@@ -566,18 +588,19 @@ insta_status ensure_train_workspace(insta_ctx* ctx, TrainState* ts, uint32_t n_r
     }
     ts->ray_cap = rc;
     ts->sample_cap = sc;
+    ts->compact_cap = cc;
     return INSTA_OK;
 }
 
 template <int L>
-insta_status launch_train_network(insta_ctx* ctx, TrainState* ts, hipStream_t stream) {
-    const uint32_t cap = ts->sample_cap;
-    const uint32_t* n_dev = ts->counters + kTValid;
+insta_status launch_train_forward(insta_ctx* ctx, TrainState* ts, hipStream_t stream) {
+    const uint32_t cap = ts->compact_cap;
+    const uint32_t* n_dev = ts->counters + kTCompactN;
     const float* table = ctx->d_params + ctx->layout.n_mlp_params;
     const GridParams gp = grid_params(ctx);
     const dim3 sgrid((cap + kTrainThreads - 1) / kTrainThreads);
     hipLaunchKernelGGL(train_forward_kernel<L>, sgrid, dim3(kTrainThreads), 0, stream, ctx->d_params, table, gp, n_dev,
-                       ts->s_cpos, ts->s_cdir, ts->act, cap, ts->s_out);
+                       ts->c_idx, ts->s_cpos, ts->s_cdir, ts->act, cap);
     INSTA_HIP(hipGetLastError());
     return INSTA_OK;
 }
@@ -587,18 +610,18 @@ insta_status launch_train_backward(insta_ctx* ctx, TrainState* ts, const insta_t
     constexpr int E = 2 * L;
     using A = Act<E>;
     using O = Off<E>;
-    const uint32_t cap = ts->sample_cap;
-    const uint32_t* n_dev = ts->counters + kTValid;
+    const uint32_t cap = ts->compact_cap;
+    const uint32_t* n_dev = ts->counters + kTCompactN;
     const GridParams gp = grid_params(ctx);
     const uint32_t n_mlp = (uint32_t)ctx->layout.n_mlp_params;
     const dim3 sgrid((cap + kTrainThreads - 1) / kTrainThreads);
     float* grad_tab = ts->grad + n_mlp;
     if (tc.grid_grad_fp16)
         hipLaunchKernelGGL((train_backward_kernel<L, true>), sgrid, dim3(kTrainThreads), 0, stream, ctx->d_params, gp,
-                           n_dev, ts->s_cpos, ts->act, ts->s_dout, ts->dact, cap, grad_tab, ts->grad_h);
+                           n_dev, ts->c_idx, ts->s_cpos, ts->act, ts->c_dout, ts->dact, cap, grad_tab, ts->grad_h);
     else
         hipLaunchKernelGGL((train_backward_kernel<L, false>), sgrid, dim3(kTrainThreads), 0, stream, ctx->d_params, gp,
-                           n_dev, ts->s_cpos, ts->act, ts->s_dout, ts->dact, cap, grad_tab, ts->grad_h);
+                           n_dev, ts->c_idx, ts->s_cpos, ts->act, ts->c_dout, ts->dact, cap, grad_tab, ts->grad_h);
     INSTA_HIP(hipGetLastError());
     WArgs wa;
     wa.p[0] = WProduct{Dact::DH1, A::ENC, kW, E, O::W0};
@@ -692,7 +715,7 @@ insta_status set_training_set(insta_ctx* ctx, uint32_t n_images, uint32_t width,
 insta_status train_step(insta_ctx* ctx, const insta_train_config& tc, insta_train_stats* stats, hipStream_t stream) {
     TrainState* ts = ctx->train;
     const uint32_t n_rays = tc.n_rays ? tc.n_rays : ts->rays_per_batch;
-    insta_status st = ensure_train_workspace(ctx, ts, n_rays, tc.max_samples);
+    insta_status st = ensure_train_workspace(ctx, ts, n_rays, tc.max_samples, tc.target_samples);
     if (st != INSTA_OK) return st;
     const uint64_t np = ctx->layout.n_params, n_mlp = ctx->layout.n_mlp_params;
     const uint32_t cap = ts->sample_cap;
@@ -729,8 +752,7 @@ insta_status train_step(insta_ctx* ctx, const insta_train_config& tc, insta_trai
     st = launch_deform_samples(ctx, -1, ts->ray_frame, ts->counters + kTValid, cap, ts->s_pos, ts->ray_d, ts->s_hit,
                                ts->s_cpos, ts->s_cdir, stream);
     if (st != INSTA_OK) return st;
-    st = ctx->cfg.n_levels == 8 ? launch_train_network<8>(ctx, ts, stream)
-                                : launch_train_network<16>(ctx, ts, stream);
+    st = launch_network_samples(ctx, ts->counters + kTValid, cap, ts->s_cpos, ts->s_cdir, ts->s_out, stream);
     if (st != INSTA_OK) return st;
     LossArgs la;
     la.images = ts->images;
@@ -741,8 +763,12 @@ insta_status train_step(insta_ctx* ctx, const insta_train_config& tc, insta_trai
     la.loss_scale_per_ray = tc.loss_scale / (float)n_rays;
     la.inv_rays = 1.0f / (float)n_rays;
     hipLaunchKernelGGL(train_loss_kernel, dim3((n_rays + 255) / 256), dim3(256), 0, stream, la, rng, ts->ray_src,
-                       ts->ray_img, ts->ray_pix, ts->ray_base, ts->ray_cnt, ts->s_out, ts->s_dout, ts->counters);
+                       ts->ray_img, ts->ray_pix, ts->ray_base, ts->ray_cnt, ts->s_out, ts->c_idx, ts->c_dout,
+                       ts->compact_cap, ts->counters);
     INSTA_HIP(hipGetLastError());
+    hipLaunchKernelGGL(train_compact_count_kernel, dim3(1), dim3(64), 0, stream, ts->counters, ts->compact_cap);
+    st = ctx->cfg.n_levels == 8 ? launch_train_forward<8>(ctx, ts, stream) : launch_train_forward<16>(ctx, ts, stream);
+    if (st != INSTA_OK) return st;
     st = ctx->cfg.n_levels == 8 ? launch_train_backward<8>(ctx, ts, tc, stream)
                                 : launch_train_backward<16>(ctx, ts, tc, stream);
     if (st != INSTA_OK) return st;
@@ -775,11 +801,12 @@ insta_status train_step(insta_ctx* ctx, const insta_train_config& tc, insta_trai
     ts->have_grad = true;
     if (tc.apply_update) ts->step = t;
     if (stats) stats->step = ts->step;
-    // adaptive ray count (testbed_nerf.cu:3402-3410): aim at target_samples
-    const uint32_t measured = std::max(hc[kTUsed], 1u);
+    // adaptive ray count (testbed_nerf.cu:3402-3410): rays * target /
+    // compacted batch, rounded up to a multiple of 128, at most 2^18 rays
+    const uint32_t measured = std::max(std::min(hc[kTUsed], ts->compact_cap), 1u);
     double next = (double)n_rays * tc.target_samples / measured;
-    next = std::min(std::max(next, 256.0), (double)(1u << 22));
-    ts->rays_per_batch = ((uint32_t)next + 255u) & ~255u;
+    next = std::min(std::max(next, 128.0), (double)(1u << 18));
+    ts->rays_per_batch = ((uint32_t)next + 127u) & ~127u;
     return INSTA_OK;
 }
 
