@@ -233,6 +233,108 @@ def training_bench(args):
     return out
 
 
+def serving_bench(args):
+    """Serving form: 8 resident avatars at the small shapes (different seeds for
+    mesh, bases, MLP and tables), FP16 MFMA network, Poisson arrivals at several
+    rates; each request = avatar, 50 expression coefficients (device BVH
+    rebuild + occupancy), random camera, resolution 128/256/512/1024^2 with
+    weights 0.3/0.3/0.3/0.1.  Against sequential plain-FP32 rendering: the
+    same requests' service times on the FP32 per-sample (scalar) path, queued
+    one at a time under the same arrivals."""
+    import torch
+    from insta_amd import synth
+    from insta_amd.capi import (PRECISION_FP16, PRECISION_FP32_SCALAR, Context, Server, default_server_config,
+                                make_camera, small_config)
+    n_av, n_slots = 8, args.serve_slots
+    resolutions, weights = np.array([128, 256, 512, 1024]), np.array([0.3, 0.3, 0.3, 0.1])
+
+    def avatars(precision):
+        out = []
+        for a in range(n_av):
+            av = synth.small_sphere_avatar(seed=300 + a, n_frames=0, n_bases=50)
+            cfg = small_config(max_frames=max(n_slots, 1), precision=precision)
+            ctx = Context(cfg)
+            ctx.set_canonical_mesh(av.canonical, av.faces, av.masks)
+            ctx.set_params(torch.from_numpy(synth.network_params(400 + a, ctx.layout, cfg)).cuda())
+            out.append((ctx, torch.from_numpy(av.bases).cuda()))
+        torch.cuda.synchronize()
+        return out
+
+    def requests(seed, n):
+        rng = np.random.default_rng(seed)
+        reqs = []
+        for _ in range(n):
+            res = int(rng.choice(resolutions, p=weights))
+            cam = make_camera(synth.random_camera(rng, 1.0, front_bias=False), synth.focal_from_fov(res, 30.0))
+            reqs.append((int(rng.integers(n_av)), rng.normal(0.0, 1.0, 50).astype(np.float32), cam, res))
+        return reqs
+
+    def pct(x, q):
+        return float(np.percentile(np.asarray(x) * 1e3, q)) if len(x) else None
+
+    fp16 = avatars(PRECISION_FP16)
+    server = Server(default_server_config(n_slots=n_slots, hbm_budget_bytes=args.serve_budget_gib << 30))
+    ids = [server.add_avatar(ctx, bases) for ctx, bases in fp16]
+    outs = {r: [torch.empty((r * r, 4), dtype=torch.float32, device="cuda") for _ in range(n_slots * 4)]
+            for r in resolutions.tolist()}
+    # warm-up: graphs captured, workspaces touched
+    for k, (a, c, cam, res) in enumerate(requests(1, 2 * n_slots)):
+        server.submit(ids[a], c, cam, res, res, outs[res][k % len(outs[res])])
+    while server.poll(wait=True):
+        pass
+    out = {"avatars": n_av, "slots": n_slots, "budget_gib": args.serve_budget_gib, "rates": {}}
+    for rate in args.serve_rates:
+        reqs = requests(int(rate) + 7, args.serve_requests)
+        arrivals = np.cumsum(np.random.default_rng(int(rate)).exponential(1.0 / rate, len(reqs)))
+        t0 = server.clock() + 0.05
+        results, k = [], 0
+        while len(results) < len(reqs):
+            now = server.clock()
+            while k < len(reqs) and now >= t0 + arrivals[k]:
+                a, c, cam, res = reqs[k]
+                server.submit(ids[a], c, cam, res, res, outs[res][k % len(outs[res])], tag=k)
+                k += 1
+            done = server.poll(wait=(k == len(reqs)))
+            results += done
+            if not done and k < len(reqs):
+                time.sleep(min(0.0005, max(0.0, t0 + arrivals[k] - server.clock())))
+        lat = {r.user_tag: r.t_done - r.t_submit for r in results}
+        by_res = {int(res): [lat[i] for i, q in enumerate(reqs) if q[3] == res] for res in resolutions.tolist()}
+        span = max(r.t_done for r in results) - min(r.t_submit for r in results)
+        out["rates"][str(rate)] = dict(
+            requests=len(reqs), frames_per_s=len(reqs) / span, p50_ms=pct(list(lat.values()), 50),
+            p99_ms=pct(list(lat.values()), 99),
+            p99_ms_by_res={str(r): pct(v, 99) for r, v in by_res.items()},
+            samples_per_s=sum(r.n_samples for r in results) / span)
+    server.close()
+
+    # sequential plain FP32: service time of each request alone on the FP32 per-sample path
+    fp32 = avatars(PRECISION_FP32_SCALAR)
+    frame = 0
+    svc = {}
+    for rate in args.serve_rates:
+        reqs = requests(int(rate) + 7, args.serve_requests)
+        times = []
+        for (a, c, cam, res) in reqs:
+            ctx, bases = fp32[a]
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            ctx.set_frame_expression(frame, bases, torch.from_numpy(c).cuda())
+            ctx.build_occupancy(frame, 0.05)
+            ctx.render(cam, res, res, frame, frame, 0, (outs[res][0], None, None))
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t)
+        arrivals = np.cumsum(np.random.default_rng(int(rate)).exponential(1.0 / rate, len(reqs)))
+        finish, lat = 0.0, []
+        for t_arr, s in zip(arrivals, times):
+            finish = max(finish, t_arr) + s
+            lat.append(finish - t_arr)
+        svc[str(rate)] = dict(frames_per_s=len(reqs) / (finish - arrivals[0]), p50_ms=pct(lat, 50),
+                              p99_ms=pct(lat, 99), mean_service_ms=float(np.mean(times) * 1e3))
+    out["sequential_fp32"] = svc
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -243,6 +345,11 @@ def main():
     ap.add_argument("--skip-forms", action="store_true", help="only the FP32 path")
     ap.add_argument("--accuracy-pixels", type=int, default=4096, help="pixels compared with the FP64 oracle")
     ap.add_argument("--skip-train", action="store_true", help="skip the training-step measurement")
+    ap.add_argument("--skip-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[20.0, 50.0, 100.0, 200.0])
+    ap.add_argument("--serve-requests", type=int, default=120)
+    ap.add_argument("--serve-slots", type=int, default=6)
+    ap.add_argument("--serve-budget-gib", type=int, default=24)
     ap.add_argument("--train-steps", type=int, default=40)
     ap.add_argument("--train-warmup", type=int, default=8)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -307,6 +414,7 @@ def main():
             forms["accuracy_vs_fp64"] = accuracy_vs_fp64(ctxs, cfg, av, params, cams[0], res, args.accuracy_pixels)
 
     train = None if args.skip_train else training_bench(args)
+    serve = None if args.skip_serve else serving_bench(args)
 
     cpu = None
     if args.cpu_budget > 0:
@@ -335,6 +443,7 @@ def main():
         "kernels": kernels,
         "cpu_baseline": cpu,
         "training": train,
+        "serving": serve,
     }
     print(json.dumps(line))
 

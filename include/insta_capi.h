@@ -294,6 +294,63 @@ insta_status insta_get_params(insta_ctx* ctx, float* params_dev, void* stream);
 /* Zero the Adam moments and step count. */
 insta_status insta_reset_optimizer(insta_ctx* ctx);
 
+/* ---- serving -------------------------------------------------------- */
+
+/* Request loop over several resident avatars on one device.  No counterpart
+ * in the reference, which renders one frame at a time (Testbed::render_frame,
+ * instant-ngp/src/testbed.cu:2613); each request runs the reference's render
+ * path for a new expression: insta_set_frame_expression (device BVH),
+ * insta_build_occupancy, insta_render.  n_slots requests are in flight at
+ * once (a HIP stream and host thread each, overlapping on the device); the
+ * FIFO queue head is admitted when a slot is idle and its render workspace
+ * fits the HBM budget (one device arena, reserved at creation).  A request's
+ * result depends only on its inputs, alone or in a mix. */
+typedef struct insta_server insta_server;
+
+typedef struct insta_server_config {
+    uint32_t n_slots;              /* requests in flight at once */
+    uint64_t hbm_budget_bytes;     /* device arena for the in-flight render workspaces */
+    uint32_t samples_per_pixel;    /* sample buffer per pixel (more march rounds beyond it) */
+    float    occupancy_threshold;  /* per-request occupancy grid: distance to the posed mesh */
+    uint32_t use_graphs;           /* 1: per-request preparation replayed from captured hipGraphs */
+} insta_server_config;
+
+typedef struct insta_request {
+    uint32_t avatar;               /* id from insta_server_add_avatar */
+    const float* coeffs;           /* host [n_bases] expression coefficients, copied at submit */
+    insta_camera camera;
+    uint32_t width, height;
+    float* rgba_dev;               /* caller-owned [w*h*4] output, written when the request completes */
+    uint64_t user_tag;
+} insta_request;
+
+typedef struct insta_result {
+    uint64_t ticket, user_tag;
+    double t_submit, t_admit, t_done;  /* seconds on the server clock */
+    uint64_t n_samples;            /* samples composited */
+    int32_t status;                /* insta_status of the request */
+    uint32_t slot;                 /* worker that ran it */
+} insta_result;
+
+/* 6 slots, 24 GiB budget, 64 samples per pixel, occupancy threshold 0.05, graphs on. */
+void insta_default_server_config(insta_server_config* cfg);
+insta_status insta_server_create(const insta_server_config* cfg, insta_server** out);
+/* Register an avatar context (caller-owned, outliving the server; canonical
+ * mesh and parameters set, max_frames >= n_slots: worker i uses frame slot i)
+ * with its expression bases bases_dev [n_bases][nv][3].  Before any submit. */
+insta_status insta_server_add_avatar(insta_server* srv, insta_ctx* ctx, const float* bases_dev,
+                                     uint32_t n_bases, uint32_t* avatar_id);
+/* Queue a request (never blocks on the device). */
+insta_status insta_server_submit(insta_server* srv, const insta_request* req, uint64_t* ticket);
+/* Take up to max_results finished requests; wait != 0 blocks until one is
+ * finished or nothing is pending. */
+insta_status insta_server_poll(insta_server* srv, insta_result* out, uint32_t max_results,
+                               uint32_t* n_out, int wait);
+/* Seconds since the server was created (the clock of insta_result). */
+double insta_server_clock(insta_server* srv);
+/* Finishes every queued request, then releases the server. */
+void insta_server_destroy(insta_server* srv);
+
 /* Per-kernel timing of insta_render, measured with HIP events recorded on the
  * render stream around each kernel class.  No counterpart in the reference
  * (whose only timers are EMA frame times, testbed.cu:1600, 2484-2512); it

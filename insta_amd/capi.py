@@ -79,6 +79,21 @@ class InstaTrainStats(C.Structure):
                 ("n_samples_used", C.c_uint32), ("loss", C.c_float), ("step", C.c_uint32)]
 
 
+class InstaServerConfig(C.Structure):
+    _fields_ = [("n_slots", C.c_uint32), ("hbm_budget_bytes", C.c_uint64), ("samples_per_pixel", C.c_uint32),
+                ("occupancy_threshold", C.c_float), ("use_graphs", C.c_uint32)]
+
+
+class InstaRequest(C.Structure):
+    _fields_ = [("avatar", C.c_uint32), ("coeffs", C.c_void_p), ("camera", InstaCamera), ("width", C.c_uint32),
+                ("height", C.c_uint32), ("rgba_dev", C.c_void_p), ("user_tag", C.c_uint64)]
+
+
+class InstaResult(C.Structure):
+    _fields_ = [("ticket", C.c_uint64), ("user_tag", C.c_uint64), ("t_submit", C.c_double), ("t_admit", C.c_double),
+                ("t_done", C.c_double), ("n_samples", C.c_uint64), ("status", C.c_int32), ("slot", C.c_uint32)]
+
+
 class InstaProfile(C.Structure):
     _fields_ = [("ms", C.c_double * 8), ("launches", C.c_uint64 * 8), ("items", C.c_uint64 * 8)]
 
@@ -115,6 +130,13 @@ SIGNATURES = [
     ("insta_get_gradients", C.c_int, [_VP, _VP, _VP]),
     ("insta_get_params", C.c_int, [_VP, _VP, _VP]),
     ("insta_reset_optimizer", C.c_int, [_VP]),
+    ("insta_default_server_config", None, [C.POINTER(InstaServerConfig)]),
+    ("insta_server_create", C.c_int, [C.POINTER(InstaServerConfig), C.POINTER(_VP)]),
+    ("insta_server_add_avatar", C.c_int, [_VP, _VP, _VP, C.c_uint32, C.POINTER(C.c_uint32)]),
+    ("insta_server_submit", C.c_int, [_VP, C.POINTER(InstaRequest), C.POINTER(C.c_uint64)]),
+    ("insta_server_poll", C.c_int, [_VP, C.POINTER(InstaResult), C.c_uint32, C.POINTER(C.c_uint32), C.c_int]),
+    ("insta_server_clock", C.c_double, [_VP]),
+    ("insta_server_destroy", None, [_VP]),
 ]
 
 _lib = None
@@ -381,3 +403,57 @@ def default_train_config(**kw) -> InstaTrainConfig:
     for k, v in kw.items():
         setattr(tc, k, v)
     return tc
+
+
+def default_server_config(**kw) -> InstaServerConfig:
+    c = InstaServerConfig()
+    load_library().insta_default_server_config(C.byref(c))
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+class Server:
+    """Request loop over resident avatars (insta_server_*)."""
+
+    def __init__(self, cfg: InstaServerConfig | None = None):
+        self.lib = load_library()
+        self.cfg = cfg if cfg is not None else default_server_config()
+        h = C.c_void_p()
+        check(self.lib.insta_server_create(C.byref(self.cfg), C.byref(h)), "insta_server_create")
+        self.h = h
+        self._keep = {}     # host coefficient arrays stay alive until submit copied them
+
+    def add_avatar(self, ctx: Context, bases_dev) -> int:
+        aid = C.c_uint32()
+        check(self.lib.insta_server_add_avatar(self.h, ctx.h, _ptr(bases_dev), bases_dev.shape[0], C.byref(aid)),
+              "insta_server_add_avatar")
+        return aid.value
+
+    def submit(self, avatar: int, coeffs: np.ndarray, cam: InstaCamera, width: int, height: int, rgba_dev,
+               tag: int = 0) -> int:
+        c = np.ascontiguousarray(coeffs, dtype=np.float32)
+        req = InstaRequest(avatar, c.ctypes.data, cam, width, height, _ptr(rgba_dev), tag)
+        t = C.c_uint64()
+        check(self.lib.insta_server_submit(self.h, C.byref(req), C.byref(t)), "insta_server_submit")
+        return t.value
+
+    def poll(self, wait: bool = False, max_results: int = 256) -> list:
+        buf = (InstaResult * max_results)()
+        n = C.c_uint32()
+        check(self.lib.insta_server_poll(self.h, buf, max_results, C.byref(n), 1 if wait else 0), "insta_server_poll")
+        return [buf[i] for i in range(n.value)]
+
+    def clock(self) -> float:
+        return self.lib.insta_server_clock(self.h)
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.insta_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -431,16 +431,15 @@ insta_status launch_build_slots(insta_ctx* ctx, uint32_t first_slot, uint32_t n_
 
 insta_status launch_blend_expression(insta_ctx* ctx, uint32_t slot, const float* bases, uint32_t n_bases,
                                      const float* coeffs, hipStream_t stream) {
+    // blended straight into the slot's vertex store (the build's vertex copy
+    // is then in place), so concurrent expressions on other slots never share
+    // a buffer
     const uint32_t n = ctx->nv * 3;
-    float* tmp = static_cast<float*>(ctx->d_scratch);
-    if (ctx->scratch_bytes < n * sizeof(float)) {
-        set_error("blend: scratch too small");
-        return INSTA_ERR_STATE;
-    }
+    float* dst = ctx->d_verts + (size_t)slot * ctx->cfg.max_vertices * 3;
     hipLaunchKernelGGL(blend_expression_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, ctx->d_verts, bases,
-                       coeffs, n_bases, n, tmp);
+                       coeffs, n_bases, n, dst);
     INSTA_HIP(hipGetLastError());
-    return launch_build_slots(ctx, slot, 1, tmp, stream);
+    return launch_build_slots(ctx, slot, 1, dst, stream);
 }
 
 insta_status launch_closest(insta_ctx* ctx, uint32_t slot, const float* pts, uint32_t n, int32_t* ids, float* dist,

@@ -15,10 +15,6 @@
 
 namespace insta {
 
-insta_status render(insta_ctx* ctx, const insta_camera* cam, uint32_t width, uint32_t height, int32_t frame,
-                    int32_t occupancy, uint32_t spp, float* rgba, float* depth, uint32_t* steps, hipStream_t stream);
-void free_workspace(insta_ctx* ctx);
-insta_status set_profiling(insta_ctx* ctx, bool on);
 
 namespace {
 thread_local std::string g_error;
@@ -110,6 +106,18 @@ insta_status mark_built(insta_ctx* ctx, uint32_t first, uint32_t n, uint8_t v, h
     for (uint32_t i = 0; i < n; ++i) ctx->slot_built[first + i] = v;
     return INSTA_OK;
 }
+
+}  // namespace
+
+// device side of mark_built only (capturable into a graph)
+insta_status launch_set_built(insta_ctx* ctx, uint32_t first, uint32_t n, hipStream_t stream) {
+    hipLaunchKernelGGL(set_flags_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, ctx->d_built, first, n,
+                       (uint8_t)1);
+    INSTA_HIP(hipGetLastError());
+    return INSTA_OK;
+}
+
+namespace {
 
 bool slot_of(const insta_ctx* ctx, int32_t frame, uint32_t& slot) {
     if (frame < -1 || frame >= (int32_t)ctx->cfg.max_frames) return false;
@@ -221,10 +229,7 @@ insta_status insta_create(const insta_config* cfg, insta_ctx** out) {
     alloc(&ctx->d_wfrag, 64 * 1024 * sizeof(float));
     if (cfg->precision == INSTA_PRECISION_FP16 || cfg->precision == INSTA_PRECISION_BF16)
         alloc(&ctx->d_table_h, ctx->layout.n_grid_params * 2);
-    alloc((void**)&ctx->d_counters, 16 * sizeof(uint32_t));
     alloc(&ctx->d_scratch, ctx->scratch_bytes);
-    if (ok && hipHostMalloc((void**)&ctx->h_counters, 16 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
-        ok = false;
     if (ok && hipMemset(ctx->d_built, 0, S) != hipSuccess) ok = false;
     if (ok && hipMemset(ctx->d_grids, 0, S * (G * G * G / 8)) != hipSuccess) ok = false;
     if (ok && hipMemset(ctx->d_grids_lin, 0, S * (G * G * G / 8)) != hipSuccess) ok = false;
@@ -246,8 +251,7 @@ void insta_destroy(insta_ctx* ctx) {
     dev_free(ctx->d_nodes); dev_free(ctx->d_btris); dev_free(ctx->d_proj); dev_free(ctx->d_proj_inv);
     dev_free(ctx->d_cen); dev_free(ctx->d_grids); dev_free(ctx->d_grids_lin); dev_free(ctx->d_built); dev_free(ctx->d_params);
     dev_free(ctx->d_build_lists); dev_free(ctx->d_build_q);
-    dev_free(ctx->d_counters); dev_free(ctx->d_scratch); dev_free(ctx->d_wfrag); dev_free(ctx->d_table_h);
-    if (ctx->h_counters) (void)hipHostFree(ctx->h_counters);
+    dev_free(ctx->d_scratch); dev_free(ctx->d_wfrag); dev_free(ctx->d_table_h);
     delete ctx;
 }
 
@@ -543,7 +547,7 @@ insta_status insta_render(insta_ctx* ctx, const insta_camera* cam, uint32_t widt
         set_error("render: occupancy slot not built");
         return INSTA_ERR_STATE;
     }
-    return insta::render(ctx, cam, width, height, frame, occupancy, spp, rgba_dev, depth_dev, steps_dev,
+    return insta::render(ctx, ctx->ws, cam, width, height, frame, occupancy, spp, rgba_dev, depth_dev, steps_dev,
                          (hipStream_t)stream);
 }
 
@@ -553,7 +557,7 @@ insta_status insta_get_render_stats(insta_ctx* ctx, insta_render_stats* out) {
         set_error("get_render_stats: null pointer");
         return INSTA_ERR_INVALID;
     }
-    *out = ctx->last_stats;
+    *out = ctx->ws.stats;
     return INSTA_OK;
 }
 

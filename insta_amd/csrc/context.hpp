@@ -11,6 +11,39 @@
 
 namespace insta {
 struct TrainState;
+
+// Buffers of one render in flight: per pixel ray state, per sample the
+// march/deform/network stages, and the round counters.
+struct RenderWS {
+    uint32_t pixels = 0;            // capacity in pixels
+    uint64_t samples = 0;           // sample-buffer capacity
+    uint32_t samples_per_pixel = 128;  // sample-buffer sizing (more rounds when exceeded)
+    float4* r_o = nullptr;          // origin xyz, t
+    float4* r_d = nullptr;          // dir xyz, max_weight
+    float4* r_rgba = nullptr;
+    float* r_depth = nullptr;
+    uint32_t* r_steps = nullptr;    // samples composited
+    uint32_t* r_flags = nullptr;    // bit1: exited the box in the last march
+    uint32_t* alive[2] = {nullptr, nullptr};
+    uint32_t* r_off = nullptr;      // per alive-list entry: first sample
+    uint32_t* r_cnt = nullptr;      // per alive-list entry: samples in this round
+    float4* s_pos = nullptr;        // posed xyz, ray id bits
+    float4* s_cpos = nullptr;       // canonical xyz
+    float4* s_cdir = nullptr;       // canonical direction (warped to [0,1])
+    float4* s_out = nullptr;        // rgb logits, density logit
+    int32_t* s_hit = nullptr;       // per sample slot: march t values, then nearest triangle (BVH order)
+    uint32_t* d_counters = nullptr; // [16]: alive counts, sample counter, stats
+    uint32_t* h_counters = nullptr; // pinned mirror
+    insta_render_stats stats{};
+    uint64_t bytes = 0;             // device bytes held
+    bool external = false;          // buffers carved from caller memory (bind_render_ws)
+};
+// device bytes a workspace for n_pixels needs (with spp samples per pixel)
+uint64_t render_ws_bytes(uint32_t n_pixels, uint32_t samples_per_pixel);
+void release_render_ws(RenderWS& ws);
+// point ws's buffers into [base, base + render_ws_bytes(n_pixels, spp)) (no
+// allocation; the counters must already exist)
+void bind_render_ws(RenderWS& ws, char* base, uint32_t n_pixels, uint32_t samples_per_pixel);
 }
 
 struct insta_ctx {
@@ -48,26 +81,9 @@ struct insta_ctx {
     void* d_table_h = nullptr;      // FP16 copy of the hash tables (reduced precision)
     bool params_set = false;
 
-    // render workspace (grown on demand)
-    uint32_t ws_pixels = 0;
-    uint64_t ws_samples = 0;
-    float4* r_o = nullptr;          // origin xyz, t
-    float4* r_d = nullptr;          // dir xyz, max_weight
-    float4* r_rgba = nullptr;
-    float* r_depth = nullptr;
-    uint32_t* r_steps = nullptr;    // samples composited
-    uint32_t* r_flags = nullptr;    // bit0 alive, bit1 exited box in the last march
-    uint32_t* alive[2] = {nullptr, nullptr};
-    uint32_t* r_off = nullptr;      // per alive-list entry: first sample
-    uint32_t* r_cnt = nullptr;      // per alive-list entry: samples in this round
-    float4* s_pos = nullptr;        // posed xyz, ray id bits
-    float4* s_cpos = nullptr;       // canonical xyz
-    float4* s_cdir = nullptr;       // canonical direction (warped to [0,1])
-    float4* s_out = nullptr;        // rgb logits, density logit
-    int32_t* s_hit = nullptr;       // per sample slot: march t values, then nearest triangle (BVH order)
-    uint32_t* d_counters = nullptr; // [8]: alive counts, sample counter, stats
-    uint32_t* h_counters = nullptr; // pinned mirror
-    insta_render_stats last_stats{};
+    // render workspace of insta_render (grown on demand); the server keeps
+    // one per slot
+    insta::RenderWS ws;
 
     // training state (training set, workspace, gradients, Adam moments)
     insta::TrainState* train = nullptr;
@@ -146,7 +162,7 @@ insta_status launch_closest(insta_ctx* ctx, uint32_t slot, const float* pts, uin
                             hipStream_t stream);
 insta_status launch_deform_aos(insta_ctx* ctx, float* coords, uint32_t n, uint32_t stride, int32_t frame,
                                hipStream_t stream);
-insta_status launch_deform_soa(insta_ctx* ctx, int32_t frame, const uint32_t* n_dev, uint32_t n_max,
+insta_status launch_deform_soa(insta_ctx* ctx, RenderWS& ws, int32_t frame, const uint32_t* n_dev, uint32_t n_max,
                                hipStream_t stream);
 // samples s_pos (xyz, ray id bits) with ray directions r_d; frame per ray
 // from ray_frame when non-null
@@ -155,6 +171,13 @@ insta_status launch_deform_samples(insta_ctx* ctx, int32_t frame, const int32_t*
                                    float4* s_cpos, float4* s_cdir, hipStream_t stream);
 insta_status launch_occupancy(insta_ctx* ctx, uint32_t slot, float threshold, hipStream_t stream);
 insta_status launch_grid_linear(insta_ctx* ctx, uint32_t slot, hipStream_t stream);
+insta_status launch_set_built(insta_ctx* ctx, uint32_t first, uint32_t n, hipStream_t stream);
+// render pipeline (render.hip) on an explicit workspace
+insta_status render(insta_ctx* ctx, RenderWS& ws, const insta_camera* cam, uint32_t width, uint32_t height,
+                    int32_t frame, int32_t occupancy, uint32_t spp, float* rgba, float* depth, uint32_t* steps,
+                    hipStream_t stream);
+void free_workspace(insta_ctx* ctx);
+insta_status set_profiling(insta_ctx* ctx, bool on);
 // training (train.hip)
 bool train_supported(const insta_config& cfg);
 void free_train(insta_ctx* ctx);

@@ -289,10 +289,10 @@ insta_status launch_deform_samples(insta_ctx* ctx, int32_t frame, const int32_t*
     return INSTA_OK;
 }
 
-insta_status launch_deform_soa(insta_ctx* ctx, int32_t frame, const uint32_t* n_dev, uint32_t n_max,
+insta_status launch_deform_soa(insta_ctx* ctx, RenderWS& ws, int32_t frame, const uint32_t* n_dev, uint32_t n_max,
                                hipStream_t stream) {
-    return launch_deform_samples(ctx, frame, nullptr, n_dev, n_max, ctx->s_pos, ctx->r_d, ctx->s_hit, ctx->s_cpos,
-                                 ctx->s_cdir, stream);
+    return launch_deform_samples(ctx, frame, nullptr, n_dev, n_max, ws.s_pos, ws.r_d, ws.s_hit, ws.s_cpos, ws.s_cdir,
+                                 stream);
 }
 
 }  // namespace insta

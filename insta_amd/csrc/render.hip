@@ -17,6 +17,8 @@
 // Each ray's samples depend only on its own t sequence, so the result does
 // not depend on K or on the order rays are processed in.  The host reads one
 // counter (the live-ray count) per round.
+#include <type_traits>
+
 #include "context.hpp"
 #include "march.hpp"
 
@@ -245,18 +247,18 @@ void pixel_offset(uint32_t spp, float out[2]) {
     }
 }
 
-void prof_begin(insta_ctx* ctx, int cls, hipStream_t stream) {
-    if (ctx->profiling) (void)hipEventRecord(ctx->prof_ev[2 * cls], stream);
+void prof_begin(insta_ctx* ctx, bool on, int cls, hipStream_t stream) {
+    if (on) (void)hipEventRecord(ctx->prof_ev[2 * cls], stream);
 }
-void prof_end(insta_ctx* ctx, int cls, uint64_t items, hipStream_t stream) {
-    if (!ctx->profiling) return;
+void prof_end(insta_ctx* ctx, bool on, int cls, uint64_t items, hipStream_t stream) {
+    if (!on) return;
     (void)hipEventRecord(ctx->prof_ev[2 * cls + 1], stream);
     ctx->prof_pending[cls] = true;
     ctx->prof_pending_items[cls] = items;
 }
 // after a stream synchronisation: fold the recorded intervals in
-void prof_collect(insta_ctx* ctx) {
-    if (!ctx->profiling) return;
+void prof_collect(insta_ctx* ctx, bool on) {
+    if (!on) return;
     for (int c = 0; c < INSTA_KERNEL_CLASSES; ++c) {
         if (!ctx->prof_pending[c]) continue;
         float ms = 0.0f;
@@ -269,51 +271,130 @@ void prof_collect(insta_ctx* ctx) {
     }
 }
 
-insta_status ensure_workspace(insta_ctx* ctx, uint32_t n_pixels) {
-    if (n_pixels <= ctx->ws_pixels) return INSTA_OK;
-    dev_free(ctx->r_o); dev_free(ctx->r_d); dev_free(ctx->r_rgba); dev_free(ctx->r_depth);
-    dev_free(ctx->r_steps); dev_free(ctx->r_flags); dev_free(ctx->alive[0]); dev_free(ctx->alive[1]);
-    dev_free(ctx->r_off); dev_free(ctx->r_cnt);
-    dev_free(ctx->s_pos); dev_free(ctx->s_cpos); dev_free(ctx->s_cdir); dev_free(ctx->s_out);
-    dev_free(ctx->s_hit);
-    ctx->ws_pixels = 0;
-    const size_t n = n_pixels;
-    // sample buffer: 128 samples per pixel, between 1M and 64M samples (4 GiB)
-    uint64_t cap = (uint64_t)n * 128;
+void free_render_ws(RenderWS& ws) {
+    if (ws.external) {  // carved from memory owned elsewhere: forget the pointers only
+        ws.r_o = ws.r_d = ws.r_rgba = nullptr;
+        ws.r_depth = nullptr;
+        ws.r_steps = ws.r_flags = ws.alive[0] = ws.alive[1] = ws.r_off = ws.r_cnt = nullptr;
+        ws.s_pos = ws.s_cpos = ws.s_cdir = ws.s_out = nullptr;
+        ws.s_hit = nullptr;
+        ws.pixels = 0;
+        ws.samples = 0;
+        ws.bytes = 0;
+        ws.external = false;
+        return;
+    }
+    dev_free(ws.r_o); dev_free(ws.r_d); dev_free(ws.r_rgba); dev_free(ws.r_depth);
+    dev_free(ws.r_steps); dev_free(ws.r_flags); dev_free(ws.alive[0]); dev_free(ws.alive[1]);
+    dev_free(ws.r_off); dev_free(ws.r_cnt);
+    dev_free(ws.s_pos); dev_free(ws.s_cpos); dev_free(ws.s_cdir); dev_free(ws.s_out);
+    dev_free(ws.s_hit);
+    ws.pixels = 0;
+    ws.samples = 0;
+    ws.bytes = 0;
+}
+
+uint64_t ws_sample_cap(uint32_t n_pixels, uint32_t spp) {
+    // samples_per_pixel samples per pixel, between 1M and 64M samples (4 GiB)
+    uint64_t cap = (uint64_t)n_pixels * spp;
     if (cap < (1u << 20)) cap = 1u << 20;
     if (cap > (64ull << 20)) cap = 64ull << 20;
+    return cap;
+}
+
+insta_status ensure_workspace(RenderWS& ws, uint32_t n_pixels) {
+    if (!ws.d_counters) {
+        if (hipMalloc(reinterpret_cast<void**>(&ws.d_counters), kNumCounters * sizeof(uint32_t)) != hipSuccess ||
+            hipHostMalloc(reinterpret_cast<void**>(&ws.h_counters), kNumCounters * sizeof(uint32_t),
+                          hipHostMallocDefault) != hipSuccess) {
+            set_error("render counters allocation failed");
+            return INSTA_ERR_OOM;
+        }
+    }
+    if (n_pixels <= ws.pixels) return INSTA_OK;
+    if (ws.external) {
+        set_error("render: bound workspace smaller than the request");
+        return INSTA_ERR_INVALID;
+    }
+    free_render_ws(ws);
+    const size_t n = n_pixels;
+    const uint64_t cap = ws_sample_cap(n_pixels, ws.samples_per_pixel);
 #define INSTA_ALLOC(ptr, bytes)                                           \
     if (hipMalloc(reinterpret_cast<void**>(&(ptr)), (bytes)) != hipSuccess) { \
+        free_render_ws(ws);                                               \
         set_error("render workspace allocation failed");                 \
         return INSTA_ERR_OOM;                                             \
     }
-    INSTA_ALLOC(ctx->r_o, n * sizeof(float4));
-    INSTA_ALLOC(ctx->r_d, n * sizeof(float4));
-    INSTA_ALLOC(ctx->r_rgba, n * sizeof(float4));
-    INSTA_ALLOC(ctx->r_depth, n * sizeof(float));
-    INSTA_ALLOC(ctx->r_steps, n * sizeof(uint32_t));
-    INSTA_ALLOC(ctx->r_flags, n * sizeof(uint32_t));
-    INSTA_ALLOC(ctx->alive[0], n * sizeof(uint32_t));
-    INSTA_ALLOC(ctx->alive[1], n * sizeof(uint32_t));
-    INSTA_ALLOC(ctx->r_off, n * sizeof(uint32_t));
-    INSTA_ALLOC(ctx->r_cnt, n * sizeof(uint32_t));
-    INSTA_ALLOC(ctx->s_pos, cap * sizeof(float4));
-    INSTA_ALLOC(ctx->s_cpos, cap * sizeof(float4));
-    INSTA_ALLOC(ctx->s_cdir, cap * sizeof(float4));
-    INSTA_ALLOC(ctx->s_out, cap * sizeof(float4));
-    INSTA_ALLOC(ctx->s_hit, cap * sizeof(int32_t));
+    INSTA_ALLOC(ws.r_o, n * sizeof(float4));
+    INSTA_ALLOC(ws.r_d, n * sizeof(float4));
+    INSTA_ALLOC(ws.r_rgba, n * sizeof(float4));
+    INSTA_ALLOC(ws.r_depth, n * sizeof(float));
+    INSTA_ALLOC(ws.r_steps, n * sizeof(uint32_t));
+    INSTA_ALLOC(ws.r_flags, n * sizeof(uint32_t));
+    INSTA_ALLOC(ws.alive[0], n * sizeof(uint32_t));
+    INSTA_ALLOC(ws.alive[1], n * sizeof(uint32_t));
+    INSTA_ALLOC(ws.r_off, n * sizeof(uint32_t));
+    INSTA_ALLOC(ws.r_cnt, n * sizeof(uint32_t));
+    INSTA_ALLOC(ws.s_pos, cap * sizeof(float4));
+    INSTA_ALLOC(ws.s_cpos, cap * sizeof(float4));
+    INSTA_ALLOC(ws.s_cdir, cap * sizeof(float4));
+    INSTA_ALLOC(ws.s_out, cap * sizeof(float4));
+    INSTA_ALLOC(ws.s_hit, cap * sizeof(int32_t));
 #undef INSTA_ALLOC
-    ctx->ws_pixels = n_pixels;
-    ctx->ws_samples = cap;
+    ws.pixels = n_pixels;
+    ws.samples = cap;
+    ws.bytes = render_ws_bytes(n_pixels, ws.samples_per_pixel);
     return INSTA_OK;
 }
 
 }  // namespace
 
-insta_status render(insta_ctx* ctx, const insta_camera* cam, uint32_t width, uint32_t height, int32_t frame,
-                    int32_t occupancy, uint32_t spp, float* rgba, float* depth, uint32_t* steps, hipStream_t stream) {
+constexpr uint64_t kWsAlign = 256;
+constexpr uint64_t kWsBuffers = 15;
+
+uint64_t render_ws_bytes(uint32_t n_pixels, uint32_t samples_per_pixel) {
+    return (uint64_t)n_pixels * (3 * sizeof(float4) + 7 * sizeof(uint32_t)) +
+           ws_sample_cap(n_pixels, samples_per_pixel) * (4 * sizeof(float4) + sizeof(int32_t)) +
+           kWsBuffers * kWsAlign;
+}
+
+void bind_render_ws(RenderWS& ws, char* base, uint32_t n_pixels, uint32_t samples_per_pixel) {
+    const uint64_t cap = ws_sample_cap(n_pixels, samples_per_pixel);
+    const size_t n = n_pixels;
+    uint64_t off = 0;
+    auto carve = [&](auto*& ptr, uint64_t bytes) {
+        ptr = reinterpret_cast<std::remove_reference_t<decltype(ptr)>>(base + off);
+        off += (bytes + kWsAlign - 1) / kWsAlign * kWsAlign;
+    };
+    carve(ws.r_o, n * sizeof(float4));
+    carve(ws.r_d, n * sizeof(float4));
+    carve(ws.r_rgba, n * sizeof(float4));
+    carve(ws.r_depth, n * sizeof(float));
+    carve(ws.r_steps, n * sizeof(uint32_t));
+    carve(ws.r_flags, n * sizeof(uint32_t));
+    carve(ws.alive[0], n * sizeof(uint32_t));
+    carve(ws.alive[1], n * sizeof(uint32_t));
+    carve(ws.r_off, n * sizeof(uint32_t));
+    carve(ws.r_cnt, n * sizeof(uint32_t));
+    carve(ws.s_pos, cap * sizeof(float4));
+    carve(ws.s_cpos, cap * sizeof(float4));
+    carve(ws.s_cdir, cap * sizeof(float4));
+    carve(ws.s_out, cap * sizeof(float4));
+    carve(ws.s_hit, cap * sizeof(int32_t));
+    ws.pixels = n_pixels;
+    ws.samples = cap;
+    ws.samples_per_pixel = samples_per_pixel;
+    ws.bytes = off;
+    ws.external = true;
+}
+
+insta_status render(insta_ctx* ctx, RenderWS& ws, const insta_camera* cam, uint32_t width, uint32_t height,
+                    int32_t frame, int32_t occupancy, uint32_t spp, float* rgba, float* depth, uint32_t* steps,
+                    hipStream_t stream) {
     const uint32_t n_pix = width * height;
-    insta_status st = ensure_workspace(ctx, n_pix);
+    // event timing only for the context's own workspace (one stream at a time)
+    const bool prof = ctx->profiling && &ws == &ctx->ws;
+    insta_status st = ensure_workspace(ws, n_pix);
     if (st != INSTA_OK) return st;
     RenderArgs a;
     for (int i = 0; i < 12; ++i) a.M[i] = cam->matrix[i];
@@ -338,24 +419,24 @@ insta_status render(insta_ctx* ctx, const insta_camera* cam, uint32_t width, uin
         a.hi[i] = ctx->render_hi[i];
     }
     a.grid = ctx->grid_lin(occupancy < 0 ? 0u : (uint32_t)occupancy + 1u);
-    uint32_t* cnt = ctx->d_counters;
+    uint32_t* cnt = ws.d_counters;
     hipLaunchKernelGGL(clear_counters_kernel, dim3(1), dim3(64), 0, stream, cnt, 0u, (uint32_t)kNumCounters);
-    prof_begin(ctx, INSTA_KERNEL_INIT, stream);
+    prof_begin(ctx, prof, INSTA_KERNEL_INIT, stream);
     hipLaunchKernelGGL(init_rays_kernel, dim3((n_pix + kRayThreads - 1) / kRayThreads), dim3(kRayThreads), 0, stream, a,
-                       ctx->r_o, ctx->r_d, ctx->r_rgba, ctx->r_depth, ctx->r_steps, ctx->alive[0], cnt);
+                       ws.r_o, ws.r_d, ws.r_rgba, ws.r_depth, ws.r_steps, ws.alive[0], cnt);
     INSTA_HIP(hipGetLastError());
-    prof_end(ctx, INSTA_KERNEL_INIT, n_pix, stream);
-    INSTA_HIP(hipMemcpyAsync(ctx->h_counters, cnt, kNumCounters * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    prof_end(ctx, prof, INSTA_KERNEL_INIT, n_pix, stream);
+    INSTA_HIP(hipMemcpyAsync(ws.h_counters, cnt, kNumCounters * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
     INSTA_HIP(hipStreamSynchronize(stream));
-    prof_collect(ctx);
-    uint32_t n_alive = ctx->h_counters[kCntAliveA];
-    ctx->last_stats = insta_render_stats{};
-    ctx->last_stats.n_rays = n_pix;
-    ctx->last_stats.n_rays_alive0 = n_alive;
+    prof_collect(ctx, prof);
+    uint32_t n_alive = ws.h_counters[kCntAliveA];
+    ws.stats = insta_render_stats{};
+    ws.stats.n_rays = n_pix;
+    ws.stats.n_rays_alive0 = n_alive;
     uint64_t emitted = 0;
     uint32_t cur = 0, iters = 0;
     while (n_alive > 0) {
-        uint64_t K = ctx->ws_samples / n_alive;
+        uint64_t K = ws.samples / n_alive;
         if (K > ctx->cfg.max_steps) K = ctx->cfg.max_steps;
         if (K < 1) K = 1;
         const uint32_t next = 1u - cur;
@@ -363,50 +444,52 @@ insta_status render(insta_ctx* ctx, const insta_camera* cam, uint32_t width, uin
         hipLaunchKernelGGL(clear_counters_kernel, dim3(1), dim3(64), 0, stream, cnt, (uint32_t)next_slot, 1u);
         hipLaunchKernelGGL(clear_counters_kernel, dim3(1), dim3(64), 0, stream, cnt, (uint32_t)kCntSamples, 1u);
         const dim3 rgrid((n_alive + kRayThreads - 1) / kRayThreads);
-        prof_begin(ctx, INSTA_KERNEL_MARCH, stream);
-        hipLaunchKernelGGL(march_kernel, rgrid, dim3(kRayThreads), 0, stream, a, ctx->alive[cur], n_alive, (uint32_t)K,
-                           ctx->r_o, ctx->r_d, ctx->r_steps, ctx->r_flags, ctx->r_off, ctx->r_cnt,
-                           reinterpret_cast<float*>(ctx->s_hit), ctx->s_pos, cnt);
+        prof_begin(ctx, prof, INSTA_KERNEL_MARCH, stream);
+        hipLaunchKernelGGL(march_kernel, rgrid, dim3(kRayThreads), 0, stream, a, ws.alive[cur], n_alive, (uint32_t)K,
+                           ws.r_o, ws.r_d, ws.r_steps, ws.r_flags, ws.r_off, ws.r_cnt,
+                           reinterpret_cast<float*>(ws.s_hit), ws.s_pos, cnt);
         INSTA_HIP(hipGetLastError());
-        prof_end(ctx, INSTA_KERNEL_MARCH, n_alive, stream);
+        prof_end(ctx, prof, INSTA_KERNEL_MARCH, n_alive, stream);
         const uint64_t n_max64 = (uint64_t)n_alive * K;
-        const uint32_t n_max = (uint32_t)(n_max64 < ctx->ws_samples ? n_max64 : ctx->ws_samples);
-        prof_begin(ctx, INSTA_KERNEL_DEFORM, stream);
-        st = launch_deform_soa(ctx, frame, cnt + kCntSamples, n_max, stream);
+        const uint32_t n_max = (uint32_t)(n_max64 < ws.samples ? n_max64 : ws.samples);
+        prof_begin(ctx, prof, INSTA_KERNEL_DEFORM, stream);
+        st = launch_deform_soa(ctx, ws, frame, cnt + kCntSamples, n_max, stream);
         if (st != INSTA_OK) return st;
-        prof_end(ctx, INSTA_KERNEL_DEFORM, 0, stream);
-        prof_begin(ctx, INSTA_KERNEL_NETWORK, stream);
-        st = launch_network_samples(ctx, cnt + kCntSamples, n_max, ctx->s_cpos, ctx->s_cdir, ctx->s_out, stream);
+        prof_end(ctx, prof, INSTA_KERNEL_DEFORM, 0, stream);
+        prof_begin(ctx, prof, INSTA_KERNEL_NETWORK, stream);
+        st = launch_network_samples(ctx, cnt + kCntSamples, n_max, ws.s_cpos, ws.s_cdir, ws.s_out, stream);
         if (st != INSTA_OK) return st;
-        prof_end(ctx, INSTA_KERNEL_NETWORK, 0, stream);
-        prof_begin(ctx, INSTA_KERNEL_COMPOSITE, stream);
-        hipLaunchKernelGGL(composite_kernel, rgrid, dim3(kRayThreads), 0, stream, a, ctx->alive[cur], n_alive,
-                           ctx->r_off, ctx->r_cnt, ctx->s_pos, ctx->s_out, ctx->r_d, ctx->r_rgba, ctx->r_depth,
-                           ctx->r_steps, ctx->r_flags, ctx->alive[next], cnt, next_slot);
+        prof_end(ctx, prof, INSTA_KERNEL_NETWORK, 0, stream);
+        prof_begin(ctx, prof, INSTA_KERNEL_COMPOSITE, stream);
+        hipLaunchKernelGGL(composite_kernel, rgrid, dim3(kRayThreads), 0, stream, a, ws.alive[cur], n_alive,
+                           ws.r_off, ws.r_cnt, ws.s_pos, ws.s_out, ws.r_d, ws.r_rgba, ws.r_depth,
+                           ws.r_steps, ws.r_flags, ws.alive[next], cnt, next_slot);
         INSTA_HIP(hipGetLastError());
-        prof_end(ctx, INSTA_KERNEL_COMPOSITE, n_alive, stream);
-        INSTA_HIP(hipMemcpyAsync(ctx->h_counters, cnt, kNumCounters * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        prof_end(ctx, prof, INSTA_KERNEL_COMPOSITE, n_alive, stream);
+        INSTA_HIP(hipMemcpyAsync(ws.h_counters, cnt, kNumCounters * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
         INSTA_HIP(hipStreamSynchronize(stream));
         // sample counts are known only now; attribute them to deform/network
-        ctx->prof_pending_items[INSTA_KERNEL_DEFORM] = ctx->h_counters[kCntSamples];
-        ctx->prof_pending_items[INSTA_KERNEL_NETWORK] = ctx->h_counters[kCntSamples];
-        prof_collect(ctx);
-        emitted += ctx->h_counters[kCntSamples];
-        n_alive = ctx->h_counters[next_slot];
+        if (prof) {
+            ctx->prof_pending_items[INSTA_KERNEL_DEFORM] = ws.h_counters[kCntSamples];
+            ctx->prof_pending_items[INSTA_KERNEL_NETWORK] = ws.h_counters[kCntSamples];
+        }
+        prof_collect(ctx, prof);
+        emitted += ws.h_counters[kCntSamples];
+        n_alive = ws.h_counters[next_slot];
         cur = next;
         ++iters;
     }
-    ctx->last_stats.n_samples = ctx->h_counters[kCntComposited];
-    ctx->last_stats.n_samples_emitted = emitted;
-    ctx->last_stats.n_iterations = iters;
-    prof_begin(ctx, INSTA_KERNEL_FINALIZE, stream);
+    ws.stats.n_samples = ws.h_counters[kCntComposited];
+    ws.stats.n_samples_emitted = emitted;
+    ws.stats.n_iterations = iters;
+    prof_begin(ctx, prof, INSTA_KERNEL_FINALIZE, stream);
     hipLaunchKernelGGL(finalize_kernel, dim3((n_pix + kRayThreads - 1) / kRayThreads), dim3(kRayThreads), 0, stream,
-                       n_pix, ctx->r_rgba, ctx->r_depth, ctx->r_steps, reinterpret_cast<float4*>(rgba), depth, steps);
+                       n_pix, ws.r_rgba, ws.r_depth, ws.r_steps, reinterpret_cast<float4*>(rgba), depth, steps);
     INSTA_HIP(hipGetLastError());
-    prof_end(ctx, INSTA_KERNEL_FINALIZE, n_pix, stream);
-    if (ctx->profiling) {
+    prof_end(ctx, prof, INSTA_KERNEL_FINALIZE, n_pix, stream);
+    if (prof) {
         INSTA_HIP(hipStreamSynchronize(stream));
-        prof_collect(ctx);
+        prof_collect(ctx, prof);
     }
     return INSTA_OK;
 }
@@ -429,12 +512,15 @@ void free_workspace(insta_ctx* ctx) {
             (void)hipEventDestroy(ctx->prof_ev[i]);
             ctx->prof_ev[i] = nullptr;
         }
-    dev_free(ctx->r_o); dev_free(ctx->r_d); dev_free(ctx->r_rgba); dev_free(ctx->r_depth);
-    dev_free(ctx->r_steps); dev_free(ctx->r_flags); dev_free(ctx->alive[0]); dev_free(ctx->alive[1]);
-    dev_free(ctx->r_off); dev_free(ctx->r_cnt);
-    dev_free(ctx->s_pos); dev_free(ctx->s_cpos); dev_free(ctx->s_cdir); dev_free(ctx->s_out);
-    dev_free(ctx->s_hit);
-    ctx->ws_pixels = 0;
+    release_render_ws(ctx->ws);
+}
+
+void release_render_ws(RenderWS& ws) {
+    free_render_ws(ws);
+    ws.external = false;
+    dev_free(ws.d_counters);
+    if (ws.h_counters) (void)hipHostFree(ws.h_counters);
+    ws.h_counters = nullptr;
 }
 
 }  // namespace insta

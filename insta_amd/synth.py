@@ -171,10 +171,12 @@ class Avatar:
     bases: np.ndarray | None = None  # [K, V, 3] float32 displacement bases
 
 
-def small_sphere_avatar(seed: int = 0, n_frames: int = 1) -> Avatar:
+def small_sphere_avatar(seed: int = 0, n_frames: int = 1, n_bases: int = 0) -> Avatar:
     """Small workload mesh: 2,048-triangle sphere of radius 0.3, N(0, 0.005)
     vertex noise; each posed frame is a uniform random rotation about the
-    centre plus per-vertex N(0, 0.01) displacement."""
+    centre plus per-vertex N(0, 0.01) displacement.  With n_bases > 0 the
+    avatar also gets that many Gaussian expression bases (sigma 3 mm, width
+    5 cm) for per-request expressions."""
     rng = _rng(seed)
     v, f = octasphere(4)
     v = v * 0.3 + rng.normal(0.0, 0.005, size=v.shape)
@@ -186,7 +188,18 @@ def small_sphere_avatar(seed: int = 0, n_frames: int = 1) -> Avatar:
         p = (canon.astype(np.float64) - CENTER) @ rot.T + CENTER
         p = p + rng.normal(0.0, 0.01, size=p.shape)
         frames.append(p.astype(np.float32))
-    return Avatar(canon, f, masks, frames)
+    bases = gaussian_bases(canon, rng, n_bases=n_bases, width=0.05) if n_bases else None
+    return Avatar(canon, f, masks, frames, bases)
+
+
+def blend_expression_f32(canon: np.ndarray, bases: np.ndarray, coeffs: np.ndarray) -> np.ndarray:
+    """canonical + sum_k coeffs[k] * bases[k] accumulated in float32 in k
+    order, one rounding per multiply and per add (the device blend's order)."""
+    acc = canon.astype(np.float32).copy()
+    c = np.asarray(coeffs, np.float32)
+    for k in range(len(c)):
+        acc = (acc + c[k] * bases[k]).astype(np.float32)
+    return acc
 
 
 def _low_frequency_field(u: np.ndarray, rng: np.random.Generator, amp: float) -> np.ndarray:

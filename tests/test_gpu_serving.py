@@ -1,0 +1,117 @@
+"""The request loop (insta_server_*): a served request equals a direct render
+of the same expression and camera, and the oracle's render; results are the
+same alone and in a concurrent mix; admission respects the HBM budget."""
+import numpy as np
+import pytest
+
+from insta_amd import synth
+from insta_amd.capi import (PRECISION_FP16, PRECISION_FP32, Context, InstaError, Server, default_server_config,
+                            make_camera, small_config)
+from tests.conftest import assert_rel_close, ensure_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _avatar_ctx(seed, n_frames, precision=PRECISION_FP32):
+    import torch
+    av = synth.small_sphere_avatar(seed=seed, n_frames=0, n_bases=50)
+    cfg = small_config(max_frames=n_frames, precision=precision)
+    ctx = Context(cfg)
+    ctx.set_canonical_mesh(av.canonical, av.faces, av.masks)
+    params = synth.network_params(100 + seed, ctx.layout, cfg)
+    ctx.set_params(torch.from_numpy(params).cuda())
+    bases = torch.from_numpy(av.bases).cuda()
+    torch.cuda.synchronize()
+    return av, ctx, bases, params
+
+
+def _request(rng, res, fov=30.0):
+    cam = make_camera(synth.random_camera(rng, 1.0, front_bias=False), synth.focal_from_fov(res, fov))
+    return rng.normal(0.0, 1.0, 50).astype(np.float32), cam
+
+
+def _serve(server, reqs):
+    """reqs: list of (avatar, coeffs, cam, res); returns images (host) in order."""
+    import torch
+    outs, tickets = [], []
+    for aid, coeffs, cam, res in reqs:
+        out = torch.zeros((res * res, 4), dtype=torch.float32, device="cuda")
+        tickets.append(server.submit(aid, coeffs, cam, res, res, out, tag=len(outs)))
+        outs.append(out)
+    results = {}
+    while len(results) < len(reqs):
+        for r in server.poll(wait=True):
+            results[r.user_tag] = r
+    assert all(r.status == 0 for r in results.values())
+    torch.cuda.synchronize()
+    return [o.cpu().numpy() for o in outs], results
+
+
+def test_served_request_matches_direct_render_and_oracle():
+    import torch
+    av, ctx, bases, params = _avatar_ctx(1, n_frames=3)
+    rng = np.random.default_rng(5)
+    coeffs, cam = _request(rng, 48)
+    server = Server(default_server_config(n_slots=2, hbm_budget_bytes=1 << 30))
+    aid = server.add_avatar(ctx, bases)
+    (img,), _ = _serve(server, [(aid, coeffs, cam, 48)])
+    server.close()
+    # direct path on a frame slot the workers do not use
+    ctx.set_frame_expression(2, bases, torch.from_numpy(coeffs).cuda())
+    ctx.build_occupancy(2, 0.05)
+    rgba, _, steps = ctx.render(cam, 48, 48, 2, 2, 0)
+    torch.cuda.synchronize()
+    assert np.array_equal(img, rgba.cpu().numpy())
+    # the oracle on the same blended expression (float32, same order)
+    ensure_oracle()
+    from oracle.oracle import Oracle
+    orc = Oracle(ctx.cfg)
+    orc.set_canonical_mesh(av.canonical, av.faces, av.masks)
+    orc.set_frame(0, synth.blend_expression_f32(av.canonical, av.bases, coeffs))
+    orc.set_params(params)
+    orc.build_occupancy(0, 0.05)
+    o_rgba, _, o_steps, _ = orc.render(cam, 48, 48, 0, 0, 0)
+    same = steps.cpu().numpy() == o_steps.astype(np.int32)
+    assert same.mean() > 0.995
+    assert_rel_close(img[same], o_rgba[same], 1e-5, "served rgba vs oracle")
+
+
+@pytest.mark.parametrize("graphs", [1, 0])
+def test_same_alone_and_in_mix(graphs):
+    avatars = [_avatar_ctx(s, n_frames=4, precision=PRECISION_FP16) for s in (2, 3, 4)]
+    server = Server(default_server_config(n_slots=4, hbm_budget_bytes=2 << 30, use_graphs=graphs))
+    ids = [server.add_avatar(ctx, bases) for (_, ctx, bases, _) in avatars]
+    rng = np.random.default_rng(7)
+    c0, cam0 = _request(rng, 96)
+    (alone,), _ = _serve(server, [(ids[0], c0, cam0, 96)])
+    mix = []
+    for k in range(15):
+        res = int(rng.choice([32, 64, 96, 128]))
+        c, cam = _request(rng, res)
+        mix.append((ids[k % 3], c, cam, res))
+    mix.insert(7, (ids[0], c0, cam0, 96))
+    imgs, results = _serve(server, mix)
+    server.close()
+    assert np.array_equal(imgs[7], alone)
+    # the mix really overlapped: some request was admitted before another finished
+    spans = sorted((r.t_admit, r.t_done) for r in results.values())
+    assert any(b[0] < a[1] for a, b in zip(spans, spans[1:]))
+    assert all(np.isfinite(i).all() for i in imgs)
+
+
+def test_admission_respects_budget():
+    import torch
+    _, ctx, bases, _ = _avatar_ctx(5, n_frames=4, precision=PRECISION_FP16)
+    # a 128x128 request takes the 1M-sample minimum buffer (~70 MB): a 100 MB
+    # budget admits one at a time even with 4 idle slots
+    server = Server(default_server_config(n_slots=4, hbm_budget_bytes=100 << 20))
+    aid = server.add_avatar(ctx, bases)
+    rng = np.random.default_rng(9)
+    reqs = [(aid,) + _request(rng, 128) + (128,) for _ in range(6)]
+    _, results = _serve(server, reqs)
+    spans = sorted((r.t_admit, r.t_done) for r in results.values())
+    assert all(b[0] >= a[1] for a, b in zip(spans, spans[1:]))
+    big = torch.zeros((2048 * 2048, 4), dtype=torch.float32, device="cuda")
+    with pytest.raises(InstaError):
+        server.submit(aid, np.zeros(50, np.float32), reqs[0][2], 2048, 2048, big)
+    server.close()
