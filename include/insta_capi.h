@@ -194,6 +194,10 @@ insta_status insta_build_occupancy(insta_ctx* ctx, int32_t frame, float threshol
 insta_status insta_set_occupancy(insta_ctx* ctx, int32_t slot, const uint8_t* bitfield_dev,
                                  void* stream);
 
+/* Copy a slot's occupancy bitfield out (the density_grid_bitfield the march
+ * reads, testbed_nerf.cu:2863-2878). */
+insta_status insta_get_occupancy(insta_ctx* ctx, int32_t slot, uint8_t* bitfield_dev, void* stream);
+
 /* NerfNetwork::set_params (nerf_network.h:492-525): params_dev holds
  * layout.n_params FP32 values in the declared order; the context keeps FP32
  * master copies and, for reduced precision, converted tables/weights. */
@@ -293,6 +297,36 @@ insta_status insta_get_params(insta_ctx* ctx, float* params_dev, void* stream);
 
 /* Zero the Adam moments and step count. */
 insta_status insta_reset_optimizer(insta_ctx* ctx);
+
+/* warp_random_samples (src/core.cu:528-579): positions_dev [n][3] mapped in
+ * place to canonical space, each through a random posed frame: position i
+ * advances the PCG32 stream seeded by `seed` (then advanced by rng_offset
+ * draws) by i * (n_frames - 1) draws and takes frame floor(u * (n_frames -
+ * 1)), so frame n_frames - 1 is never used (the reference's off-by-one,
+ * kept).  Frames 0 .. n_frames - 2 must be built. */
+insta_status insta_deform_positions(insta_ctx* ctx, float* positions_dev, uint32_t n, uint32_t n_frames,
+                                    uint32_t seed, uint64_t rng_offset, void* stream);
+
+typedef struct insta_density_grid_config {
+    float    decay;            /* EMA decay (reference 0.95) */
+    uint32_t n_uniform;        /* cells sampled at any density */
+    uint32_t n_nonuniform;     /* cells sampled among those above 0.01 */
+    uint32_t seed;             /* PCG32 seed of the cell/position and frame streams */
+} insta_density_grid_config;
+
+/* One density-grid update of training (Testbed::update_density_grid_nerf,
+ * instant-ngp/src/testbed_nerf.cu:2774-2861): sample cells, warp the samples
+ * to canonical space through random frames 0 .. n_frames - 2, splat the
+ * density MLP's optical thickness (max per cell), EMA into the context's
+ * float grid, then threshold at min(0.01, mean) into occupancy slot `slot`
+ * (-1: the canonical slot, f: frame f's slot) for the march.  grid_dev
+ * (nullable) receives the G^3 floats (Morton order); mean (nullable, host)
+ * the grid mean (synchronises). */
+insta_status insta_update_density_grid(insta_ctx* ctx, int32_t slot, uint32_t n_frames,
+                                       const insta_density_grid_config* cfg, float* grid_dev, float* mean,
+                                       void* stream);
+/* Restart the EMA (next update zeroes the grid and reseeds). */
+insta_status insta_reset_density_grid(insta_ctx* ctx);
 
 /* ---- serving -------------------------------------------------------- */
 

@@ -79,6 +79,10 @@ class InstaTrainStats(C.Structure):
                 ("n_samples_used", C.c_uint32), ("loss", C.c_float), ("step", C.c_uint32)]
 
 
+class InstaDensityGridConfig(C.Structure):
+    _fields_ = [("decay", C.c_float), ("n_uniform", C.c_uint32), ("n_nonuniform", C.c_uint32), ("seed", C.c_uint32)]
+
+
 class InstaServerConfig(C.Structure):
     _fields_ = [("n_slots", C.c_uint32), ("hbm_budget_bytes", C.c_uint64), ("samples_per_pixel", C.c_uint32),
                 ("occupancy_threshold", C.c_float), ("use_graphs", C.c_uint32)]
@@ -130,6 +134,11 @@ SIGNATURES = [
     ("insta_get_gradients", C.c_int, [_VP, _VP, _VP]),
     ("insta_get_params", C.c_int, [_VP, _VP, _VP]),
     ("insta_reset_optimizer", C.c_int, [_VP]),
+    ("insta_get_occupancy", C.c_int, [_VP, C.c_int32, _VP, _VP]),
+    ("insta_deform_positions", C.c_int, [_VP, _VP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, _VP]),
+    ("insta_update_density_grid", C.c_int, [_VP, C.c_int32, C.c_uint32, C.POINTER(InstaDensityGridConfig), _VP,
+                                            C.POINTER(C.c_float), _VP]),
+    ("insta_reset_density_grid", C.c_int, [_VP]),
     ("insta_default_server_config", None, [C.POINTER(InstaServerConfig)]),
     ("insta_server_create", C.c_int, [C.POINTER(InstaServerConfig), C.POINTER(_VP)]),
     ("insta_server_add_avatar", C.c_int, [_VP, _VP, _VP, C.c_uint32, C.POINTER(C.c_uint32)]),
@@ -316,6 +325,12 @@ class Context:
               "insta_build_occupancy")
         return bits
 
+    def get_occupancy(self, slot: int):
+        G = self.cfg.grid_size
+        bits = self.torch.empty(G * G * G // 8, dtype=self.torch.uint8, device="cuda")
+        check(self.lib.insta_get_occupancy(self.h, slot, _ptr(bits), self.stream()), "insta_get_occupancy")
+        return bits
+
     def set_occupancy(self, slot: int, bits_dev) -> None:
         check(self.lib.insta_set_occupancy(self.h, slot, _ptr(bits_dev), self.stream()), "insta_set_occupancy")
 
@@ -363,6 +378,25 @@ class Context:
         check(self.lib.insta_get_profile(self.h, C.byref(p)), "insta_get_profile")
         return {name: dict(ms=p.ms[i], launches=p.launches[i], items=p.items[i])
                 for i, name in enumerate(KERNEL_CLASSES)}
+
+    # --- occupancy samples and density grid ---------------------------------
+    def deform_positions(self, pos_dev, n_frames: int, seed: int, rng_offset: int = 0) -> None:
+        check(self.lib.insta_deform_positions(self.h, _ptr(pos_dev), pos_dev.shape[0], n_frames, seed, rng_offset,
+                                              self.stream()), "insta_deform_positions")
+
+    def update_density_grid(self, slot: int, n_frames: int, decay: float = 0.95, n_uniform: int = 0,
+                            n_nonuniform: int = 0, seed: int = 1337, want_grid: bool = False):
+        """One EMA update; returns (grid [G^3] tensor or None, mean)."""
+        cfg = InstaDensityGridConfig(decay, n_uniform, n_nonuniform, seed)
+        G = self.cfg.grid_size
+        grid = self.torch.empty(G ** 3, dtype=self.torch.float32, device="cuda") if want_grid else None
+        mean = C.c_float()
+        check(self.lib.insta_update_density_grid(self.h, slot, n_frames, C.byref(cfg), _ptr(grid), C.byref(mean),
+                                                 self.stream()), "insta_update_density_grid")
+        return grid, mean.value
+
+    def reset_density_grid(self) -> None:
+        check(self.lib.insta_reset_density_grid(self.h), "insta_reset_density_grid")
 
     # --- training ---------------------------------------------------------
     def set_training_set(self, images_dev, cams, frames) -> None:

@@ -246,6 +246,7 @@ void insta_destroy(insta_ctx* ctx) {
     if (!ctx) return;
     insta::free_workspace(ctx);
     insta::free_train(ctx);
+    insta::free_density_grid(ctx);
     using insta::dev_free;
     dev_free(ctx->d_faces); dev_free(ctx->d_masks); dev_free(ctx->d_adj); dev_free(ctx->d_verts);
     dev_free(ctx->d_nodes); dev_free(ctx->d_btris); dev_free(ctx->d_proj); dev_free(ctx->d_proj_inv);
@@ -672,4 +673,70 @@ insta_status insta_reset_optimizer(insta_ctx* ctx) {
     INSTA_CHECK_CTX(ctx);
     if (!ctx->train) return INSTA_OK;
     return insta::reset_optimizer(ctx);
+}
+
+insta_status insta_deform_positions(insta_ctx* ctx, float* positions_dev, uint32_t n, uint32_t n_frames, uint32_t seed,
+                                    uint64_t rng_offset, void* stream) {
+    INSTA_CHECK_CTX(ctx);
+    if ((n && !positions_dev) || n_frames == 0 || n_frames > ctx->cfg.max_frames) {
+        set_error("deform_positions: null pointer or bad frame count");
+        return INSTA_ERR_INVALID;
+    }
+    // frames drawn: 0 .. max(n_frames - 2, 0)
+    for (uint32_t f = 0; f < std::max(n_frames - 1, 1u); ++f)
+        if (!ctx->slot_built[0] || !ctx->slot_built[f + 1]) {
+            set_error("deform_positions: a frame in 0 .. n_frames - 2 is not built");
+            return INSTA_ERR_STATE;
+        }
+    insta::Pcg32 rng = insta::Pcg32::seeded(seed);
+    rng.advance(rng_offset);
+    return insta::launch_deform_positions(ctx, positions_dev, n, n_frames, rng, (hipStream_t)stream);
+}
+
+insta_status insta_update_density_grid(insta_ctx* ctx, int32_t slot, uint32_t n_frames,
+                                       const insta_density_grid_config* cfg, float* grid_dev, float* mean,
+                                       void* stream) {
+    INSTA_CHECK_CTX(ctx);
+    if (!cfg || slot < -1 || slot >= (int32_t)ctx->cfg.max_frames || n_frames == 0 ||
+        n_frames > ctx->cfg.max_frames || (uint64_t)cfg->n_uniform + cfg->n_nonuniform == 0 ||
+        (uint64_t)cfg->n_uniform + cfg->n_nonuniform > (1ull << 26) || !(cfg->decay >= 0.0f && cfg->decay <= 1.0f)) {
+        set_error("update_density_grid: bad slot, frame count or configuration");
+        return INSTA_ERR_INVALID;
+    }
+    if (!ctx->params_set) {
+        set_error("update_density_grid: parameters not set");
+        return INSTA_ERR_STATE;
+    }
+    if (!insta::train_supported(ctx->cfg)) {
+        set_error("update_density_grid: network shape not supported (see insta_set_training_set)");
+        return INSTA_ERR_UNSUPPORTED;
+    }
+    for (uint32_t f = 0; f < std::max(n_frames - 1, 1u); ++f)
+        if (!ctx->slot_built[f + 1]) {
+            set_error("update_density_grid: a frame in 0 .. n_frames - 2 is not built");
+            return INSTA_ERR_STATE;
+        }
+    return insta::update_density_grid(ctx, (uint32_t)(slot + 1), n_frames, *cfg, grid_dev, mean,
+                                      (hipStream_t)stream);
+}
+
+insta_status insta_reset_density_grid(insta_ctx* ctx) {
+    INSTA_CHECK_CTX(ctx);
+    return insta::reset_density_grid(ctx);
+}
+
+insta_status insta_get_occupancy(insta_ctx* ctx, int32_t slot, uint8_t* bitfield_dev, void* stream) {
+    INSTA_CHECK_CTX(ctx);
+    if (slot < -1 || slot >= (int32_t)ctx->cfg.max_frames || !bitfield_dev) {
+        set_error("get_occupancy: bad slot or null pointer");
+        return INSTA_ERR_INVALID;
+    }
+    if (!ctx->grid_built[slot + 1]) {
+        set_error("get_occupancy: slot has no occupancy grid");
+        return INSTA_ERR_STATE;
+    }
+    const size_t G = ctx->cfg.grid_size;
+    INSTA_HIP(hipMemcpyAsync(bitfield_dev, ctx->grid((uint32_t)(slot + 1)), G * G * G / 8, hipMemcpyDeviceToDevice,
+                             (hipStream_t)stream));
+    return INSTA_OK;
 }

@@ -11,6 +11,7 @@
 
 namespace insta {
 struct TrainState;
+struct DensityGridState;
 
 // Buffers of one render in flight: per pixel ray state, per sample the
 // march/deform/network stages, and the round counters.
@@ -87,6 +88,8 @@ struct insta_ctx {
 
     // training state (training set, workspace, gradients, Adam moments)
     insta::TrainState* train = nullptr;
+    // density-grid EMA state (insta_update_density_grid)
+    insta::DensityGridState* density = nullptr;
 
     // optional per-kernel-class event timing of insta_render
     bool profiling = false;
@@ -164,6 +167,9 @@ insta_status launch_deform_aos(insta_ctx* ctx, float* coords, uint32_t n, uint32
                                hipStream_t stream);
 insta_status launch_deform_soa(insta_ctx* ctx, RenderWS& ws, int32_t frame, const uint32_t* n_dev, uint32_t n_max,
                                hipStream_t stream);
+// positions [n][3] in place, each through a random frame of 0..n_frames-2
+insta_status launch_deform_positions(insta_ctx* ctx, float* pos, uint32_t n, uint32_t n_frames, const Pcg32& rng,
+                                     hipStream_t stream);
 // samples s_pos (xyz, ray id bits) with ray directions r_d; frame per ray
 // from ray_frame when non-null
 insta_status launch_deform_samples(insta_ctx* ctx, int32_t frame, const int32_t* ray_frame, const uint32_t* n_dev,
@@ -185,6 +191,11 @@ insta_status set_training_set(insta_ctx* ctx, uint32_t n_images, uint32_t width,
                               const insta_camera* cams, const int32_t* frames, hipStream_t stream);
 insta_status train_step(insta_ctx* ctx, const insta_train_config& tc, insta_train_stats* stats, hipStream_t stream);
 bool train_has_gradients(const insta_ctx* ctx);
+// density grid (density.hip)
+void free_density_grid(insta_ctx* ctx);
+insta_status update_density_grid(insta_ctx* ctx, uint32_t slot, uint32_t n_frames, const insta_density_grid_config& c,
+                                 float* grid_out, float* mean_out, hipStream_t stream);
+insta_status reset_density_grid(insta_ctx* ctx);
 bool train_has_set(const insta_ctx* ctx);
 insta_status get_gradients(insta_ctx* ctx, float* out, hipStream_t stream);
 insta_status reset_optimizer(insta_ctx* ctx);

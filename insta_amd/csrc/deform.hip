@@ -132,16 +132,17 @@ INSTA_DEV bool in_unit_box(V3 p) {
     return p.x >= 0.0f && p.x <= 1.0f && p.y >= 0.0f && p.y <= 1.0f && p.z >= 0.0f && p.z <= 1.0f;
 }
 
-// The transform part of `warp` for a sample whose nearest triangle (BVH-order
-// index `idx` on the frame's BVH) is known.
-INSTA_DEV void warp_from_hit(const MeshSet& ms, uint32_t frame, int32_t idx, V3 p, V3 dir_w, V3& out_p, V3& out_d,
-                             uint32_t& mask) {
+// World-to-triangle (W) and triangle-to-canonical (C) frames of a sample
+// whose nearest triangle is `idx` (BVH order on the frame's BVH): the
+// triangle's own frames, or on the face region their exp-weighted blend with
+// the edge neighbours, the posed blend inverted per sample.
+INSTA_DEV void frames_for_hit(const MeshSet& ms, uint32_t frame, int32_t idx, V3 p, float W[12], float C[12],
+                              uint32_t& mask) {
     const MeshView fr = ms.view(frame + 1);
     const float4 r2 = fr.btris[3 * idx + 2];
     const uint32_t id = __float_as_uint(r2.y);
     mask = __float_as_uint(r2.z);
     const MeshView cn = ms.view(0);
-    float W[12], C[12];
     if (mask == 2u && !ms.disable_interp) {
         const int4 adj = ms.adj[id];
         float Wb[12];
@@ -152,6 +153,13 @@ INSTA_DEV void warp_from_hit(const MeshSet& ms, uint32_t frame, int32_t idx, V3 
         load_rows(fr.proj_inv, id, W);
         load_rows(cn.proj, id, C);
     }
+}
+
+// The transform part of `warp` for a sample whose nearest triangle is known.
+INSTA_DEV void warp_from_hit(const MeshSet& ms, uint32_t frame, int32_t idx, V3 p, V3 dir_w, V3& out_p, V3& out_d,
+                             uint32_t& mask) {
+    float W[12], C[12];
+    frames_for_hit(ms, frame, idx, p, W, C, mask);
     out_p = apply_point(C, apply_point(W, p));
     const V3 d = v3(dir_w.x * 2.0f - 1.0f, dir_w.y * 2.0f - 1.0f, dir_w.z * 2.0f - 1.0f);
     V3 dc = apply_dir(C, apply_dir(W, d));
@@ -255,6 +263,30 @@ __global__ __launch_bounds__(kDeformThreads) void deform_apply_kernel(MeshSet ms
     }
 }
 
+// warp_random_samples (src/core.cu:528-579): positions (stride 3) of
+// occupancy-grid samples mapped to canonical space through a random posed
+// frame each: the PCG32 stream advanced by i * (n_frames - 1) draws gives
+// frame = floor(u * (n_frames - 1)), so the last frame is never chosen (the
+// reference's own off-by-one, kept).  No unit-box test, as in the reference.
+__global__ __launch_bounds__(kDeformThreads) void deform_positions_kernel(MeshSet ms, float* __restrict__ pos,
+                                                                         uint32_t n, uint32_t n_meshes, Pcg32 rng) {
+    extern __shared__ uint32_t stack[];  // [stack_depth(k)][kDeformThreads]
+    const uint32_t i = blockIdx.x * kDeformThreads + threadIdx.x;
+    if (i >= n) return;
+    rng.advance((uint64_t)i * n_meshes);
+    const uint32_t frame = (uint32_t)(rng.next_float() * (float)n_meshes);
+    const V3 p = v3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+    const int32_t h = bvh_closest(ms.view(frame + 1), p, stack + threadIdx.x, kDeformThreads).idx;
+    if (h < 0) return;
+    float W[12], C[12];
+    uint32_t mask;
+    frames_for_hit(ms, frame, h, p, W, C, mask);
+    const V3 c = apply_point(C, apply_point(W, p));
+    pos[3 * i] = c.x;
+    pos[3 * i + 1] = c.y;
+    pos[3 * i + 2] = c.z;
+}
+
 uint32_t blocks_for(uint64_t items) {
     const uint64_t b = (items + kDeformThreads - 1) / kDeformThreads;
     return (uint32_t)(b < kMaxBlocks ? (b ? b : 1) : kMaxBlocks);
@@ -285,6 +317,16 @@ insta_status launch_deform_samples(insta_ctx* ctx, int32_t frame, const int32_t*
     }
     hipLaunchKernelGGL(deform_apply_kernel, dim3(blocks_for(n_max)), dim3(kDeformThreads), 0, stream, ms, frame,
                        ray_frame, n_dev, s_pos, r_d, s_hit, s_cpos, s_cdir);
+    INSTA_HIP(hipGetLastError());
+    return INSTA_OK;
+}
+
+insta_status launch_deform_positions(insta_ctx* ctx, float* pos, uint32_t n, uint32_t n_frames, const Pcg32& rng,
+                                     hipStream_t stream) {
+    if (n == 0) return INSTA_OK;
+    const size_t shmem = (size_t)stack_depth(ctx->bvh_k) * kDeformThreads * sizeof(uint32_t);
+    hipLaunchKernelGGL(deform_positions_kernel, dim3((n + kDeformThreads - 1) / kDeformThreads), dim3(kDeformThreads),
+                       shmem, stream, mesh_set(ctx, ctx->d_built), pos, n, n_frames - 1, rng);
     INSTA_HIP(hipGetLastError());
     return INSTA_OK;
 }

@@ -557,6 +557,10 @@ struct orc_scene {
     std::vector<float> params;
     std::vector<std::vector<uint8_t>> grids;   // slot 0 = canonical, slot f+1 = frame f
     int n_threads = 0;
+    // density-grid EMA (orc_update_density_grid)
+    std::vector<float> dg_grid;
+    uint32_t dg_step = 0;
+    uint64_t dg_rng_state = 0, dg_rng_inc = 0;
     // training set (orc_set_training_set)
     uint32_t tr_n = 0, tr_w = 0, tr_h = 0;
     std::vector<float> tr_images;              // [n][h][w][4]
@@ -1864,6 +1868,175 @@ int orc_train_grads(const orc_scene* s, uint32_t n_rays, uint32_t seed, uint64_t
         counts[1] = (uint32_t)n_samples;
         counts[2] = (uint32_t)n_used;
     }
+    return 0;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// warp_random_samples (src/core.cu:528-579) and the density-grid update of
+// training (Testbed::update_density_grid_nerf, testbed_nerf.cu:2774-2878 with
+// the kernels at 487-577, 585-609), one cascade.
+// ===========================================================================
+namespace {
+
+// frame of position i: PCG32 advanced by i * (n_frames - 1) draws,
+// floor(u * (n_frames - 1)); the last frame is never drawn (reference quirk)
+void deform_position(const orc_scene* s, float* p3, uint32_t i, uint32_t n_frames, const Pcg32& rng0) {
+    Pcg32 rng = rng0;
+    const uint32_t nm = n_frames - 1;
+    rng.advance((uint64_t)i * nm);
+    const uint32_t frame = (uint32_t)(rng.next_float() * (float)nm);
+    const MeshFrame& fr = s->frames[frame];
+    const V3 p = v3(p3[0], p3[1], p3[2]);
+    Hit h = bvh_closest(fr.bvh, p);
+    if (h.idx < 0) return;
+    const uint32_t id = fr.bvh.orig[h.idx];
+    float W[12], C[12];
+    if (s->masks[id] == 2u && !s->cfg.disable_interp) {
+        float Wb[12];
+        blend_frames(s, fr, id, p, Wb);
+        invert_affine_f(Wb, W);
+        blend_frames(s, s->canon, id, p, C);
+    } else {
+        std::memcpy(W, fr.proj_inv[id].m, sizeof(W));
+        std::memcpy(C, s->canon.proj[id].m, sizeof(C));
+    }
+    const V3 c = apply_point(C, apply_point(W, p));
+    p3[0] = c.x;
+    p3[1] = c.y;
+    p3[2] = c.z;
+}
+
+// generate_grid_samples_nerf_nonuniform for one sample
+void grid_sample(uint32_t i, uint32_t n, uint32_t step, float thresh, uint32_t G, const std::vector<float>& grid,
+                 const Pcg32& rng0, float* pos, uint32_t& idx) {
+    const uint32_t cells = G * G * G;
+    Pcg32 rng = rng0;
+    rng.advance((uint64_t)i * 4u);
+    (void)rng.next_float();
+    for (uint32_t j = 0; j < 10; ++j) {
+        idx = ((i + step * n) * 56924617u + j * 19349663u + 96925573u) % cells;
+        if (grid[idx] > thresh) break;
+    }
+    const uint32_t x = compact3(idx), y = compact3(idx >> 1), z = compact3(idx >> 2);
+    const float rx = rng.next_float(), ry = rng.next_float(), rz = rng.next_float();
+    const float g = (float)G;
+    pos[0] = (((float)x + rx) / g - 0.5f) * 1.0f + 0.5f;
+    pos[1] = (((float)y + ry) / g - 0.5f) * 1.0f + 0.5f;
+    pos[2] = (((float)z + rz) / g - 0.5f) * 1.0f + 0.5f;
+}
+
+// density-MLP output 0 (the network's density() path)
+float density_logit(const orc_scene* s, V3 x) {
+    const insta_config& c = s->cfg;
+    const insta_layout& L = s->layout;
+    float din[INSTA_MAX_LEVELS * 8 + 64], h[256], out;
+    encode_one(s, x, din);
+    dense(s->params.data(), din, L.density_in, c.n_neurons, true, h);
+    dense(s->params.data() + (uint64_t)c.n_neurons * L.density_in, h, c.n_neurons, 1, false, &out);
+    return out;
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_deform_positions(const orc_scene* s, float* pos, uint32_t n, uint32_t n_frames, uint32_t seed,
+                         uint64_t rng_offset) {
+    if (n_frames == 0 || n_frames > s->frames.size()) return 1;
+    Pcg32 rng = Pcg32::seeded(seed);
+    rng.advance(rng_offset);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < (int64_t)n; ++i) deform_position(s, pos + 3 * i, (uint32_t)i, n_frames, rng);
+    return 0;
+}
+
+// One update; slot -1 = canonical slot.  grid_out [G^3] (nullable), mean_out
+// (nullable).  The mean is summed as the device does: blocks of 1024 cells
+// reduced by a 256-wide tree, then the block sums by a 256-wide tree.
+int orc_update_density_grid(orc_scene* s, int32_t slot, uint32_t n_frames, float decay, uint32_t n_uniform,
+                            uint32_t n_nonuniform, uint32_t seed, float* grid_out, float* mean_out) {
+    const insta_config& c = s->cfg;
+    const uint32_t G = c.grid_size, cells = G * G * G, n = n_uniform + n_nonuniform;
+    if (s->params.empty() || n_frames == 0) return 1;
+    if (s->dg_step == 0) {
+        s->dg_grid.assign(cells, 0.0f);
+        Pcg32 r = Pcg32::seeded(seed);
+        s->dg_rng_state = r.state;
+        s->dg_rng_inc = r.inc;
+    }
+    const Pcg32 rng{s->dg_rng_state, s->dg_rng_inc};
+    std::vector<float> pos((size_t)n * 3);
+    std::vector<uint32_t> idx(n);
+    for (uint32_t i = 0; i < n_uniform; ++i)
+        grid_sample(i, n_uniform, s->dg_step, -0.01f, G, s->dg_grid, rng, &pos[3 * (size_t)i], idx[i]);
+    for (uint32_t i = 0; i < n_nonuniform; ++i)
+        grid_sample(i, n_nonuniform, s->dg_step, 0.01f, G, s->dg_grid, rng, &pos[3 * ((size_t)n_uniform + i)],
+                    idx[n_uniform + i]);
+    Pcg32 adv = rng;
+    adv.advance(1ull << 32);
+    s->dg_rng_state = adv.state;
+    Pcg32 wr = Pcg32::seeded(seed ^ 0x9e3779b9u);
+    wr.advance((uint64_t)s->dg_step << 32);
+    std::vector<float> thick(n);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        deform_position(s, &pos[3 * i], (uint32_t)i, n_frames, wr);
+        const float logit = density_logit(s, v3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]));
+        thick[i] = std::exp(logit + c.density_logit_bias) * c.dt;
+    }
+    std::vector<float> tmp(cells, 0.0f);
+    for (uint32_t i = 0; i < n; ++i) tmp[idx[i]] = std::max(tmp[idx[i]], thick[i]);
+    for (uint32_t i = 0; i < cells; ++i) {
+        const float prev = s->dg_grid[i];
+        s->dg_grid[i] = prev < 0.0f ? prev : std::fmax(prev * decay, tmp[i]);
+    }
+    // mean, in the device's summation order
+    const uint32_t nb = (cells + 1023) / 1024;
+    const float inv = 1.0f / (float)cells;
+    std::vector<float> part(nb);
+    for (uint32_t b = 0; b < nb; ++b) {
+        float lane[256];
+        for (uint32_t t = 0; t < 256; ++t) {
+            float acc = 0.0f;
+            for (uint32_t k = 0; k < 4; ++k) {
+                const uint32_t i = b * 1024 + k * 256 + t;
+                if (i < cells) acc += std::fmax(s->dg_grid[i], 0.0f) * inv;
+            }
+            lane[t] = acc;
+        }
+        for (uint32_t o = 128; o > 0; o >>= 1)
+            for (uint32_t t = 0; t < o; ++t) lane[t] += lane[t + o];
+        part[b] = lane[0];
+    }
+    float lane[256];
+    for (uint32_t t = 0; t < 256; ++t) {
+        float acc = 0.0f;
+        for (uint32_t i = t; i < nb; i += 256) acc += part[i];
+        lane[t] = acc;
+    }
+    for (uint32_t o = 128; o > 0; o >>= 1)
+        for (uint32_t t = 0; t < o; ++t) lane[t] += lane[t + o];
+    const float mean = lane[0];
+    const float thresh = std::fmin(0.01f, mean);
+    std::vector<uint8_t>& bits = s->grids[slot < 0 ? 0 : slot + 1];
+    bits.assign(cells / 8, 0);
+    for (uint32_t i = 0; i < cells; ++i)
+        if (s->dg_grid[i] > thresh) bits[i / 8] |= (uint8_t)(1u << (i % 8));
+    if (grid_out) std::memcpy(grid_out, s->dg_grid.data(), (size_t)cells * 4);
+    if (mean_out) *mean_out = mean;
+    ++s->dg_step;
+    return 0;
+}
+
+void orc_reset_density_grid(orc_scene* s) { s->dg_step = 0; }
+
+// copy of an occupancy slot's bitfield (slot -1 = canonical); 1 if unset
+int orc_get_occupancy(const orc_scene* s, int32_t slot, uint8_t* out) {
+    const std::vector<uint8_t>& g = s->grids[slot < 0 ? 0 : slot + 1];
+    if (g.empty()) return 1;
+    std::memcpy(out, g.data(), g.size());
     return 0;
 }
 

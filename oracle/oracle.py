@@ -56,6 +56,11 @@ _SIGS = [
     ("orc_pcg32_next", C.c_uint32, [C.c_uint64, C.c_uint64, C.c_uint64]),
     ("orc_pcg32_draws", None, [C.c_uint64, C.c_uint64, C.c_uint32, _VP]),
     ("orc_train_grads", C.c_int, [_VP, C.c_uint32, C.c_uint32, C.c_uint64, _VP, _VP, _VP]),
+    ("orc_deform_positions", C.c_int, [_VP, _VP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64]),
+    ("orc_update_density_grid", C.c_int, [_VP, C.c_int32, C.c_uint32, C.c_float, C.c_uint32, C.c_uint32, C.c_uint32,
+                                          _VP, _VP]),
+    ("orc_reset_density_grid", None, [_VP]),
+    ("orc_get_occupancy", C.c_int, [_VP, C.c_int32, _VP]),
 ]
 
 _lib = None
@@ -163,6 +168,12 @@ class Oracle:
         assert self.L.orc_build_occupancy(self.h, frame, threshold, _p(out)) == 0
         return out
 
+    def get_occupancy(self, slot: int) -> np.ndarray:
+        G = self.cfg.grid_size
+        out = np.zeros(G ** 3 // 8, np.uint8)
+        assert self.L.orc_get_occupancy(self.h, slot, _p(out)) == 0
+        return out
+
     def set_occupancy(self, slot: int, bits) -> None:
         b = _c(bits, np.uint8)
         assert self.L.orc_set_occupancy(self.h, slot, _p(b)) == 0
@@ -228,6 +239,24 @@ def train_grads(orc: Oracle, n_rays: int, seed: int, rays_total: int = 0, want_g
     rc = orc.L.orc_train_grads(orc.h, n_rays, seed, rays_total, _p(g), _p(loss), _p(counts))
     assert rc == 0, rc
     return g, (float(loss[0]), float(loss[1])), tuple(int(c) for c in counts)
+
+
+def deform_positions(orc: Oracle, pos, n_frames: int, seed: int, rng_offset: int = 0) -> np.ndarray:
+    """warp_random_samples on host positions [n, 3] (returns a new array)."""
+    p = _c(pos, np.float32).copy()
+    assert orc.L.orc_deform_positions(orc.h, _p(p), len(p), n_frames, seed, rng_offset) == 0
+    return p
+
+
+def update_density_grid(orc: Oracle, slot: int, n_frames: int, decay: float, n_uniform: int, n_nonuniform: int,
+                        seed: int):
+    """One density-grid update; returns (grid [G^3] float32, mean)."""
+    G = orc.cfg.grid_size
+    grid = np.zeros(G ** 3, np.float32)
+    mean = np.zeros(1, np.float32)
+    assert orc.L.orc_update_density_grid(orc.h, slot, n_frames, decay, n_uniform, n_nonuniform, seed, _p(grid),
+                                         _p(mean)) == 0
+    return grid, float(mean[0])
 
 
 def pcg32_draws(initstate: int, initseq: int, n: int) -> np.ndarray:
