@@ -214,7 +214,7 @@ __global__ __launch_bounds__(kDeformThreads) void deform_aos_kernel(MeshSet ms, 
 //  apply: per sample, the frame transforms of `warp` from the hit; writes
 //         canonical position (+ mask bits) to s_cpos and the warped
 //         direction to s_cdir.
-constexpr uint32_t kMaxBlocks = 8192;
+constexpr uint32_t kMaxBlocks = 65536;
 
 // Frame of a sample: the per-ray table when given (training batches mix
 // frames), else the call's frame; -1 leaves the sample undeformed.
