@@ -71,6 +71,9 @@ constexpr int kCin = 32;             // density outputs + 16 SH
 constexpr uint32_t kTrainThreads = 128;
 constexpr uint32_t kChunk = 1024;    // samples per weight-gradient partial
 constexpr uint32_t kRaySeedStride = 8;
+// backward: wave-level run sums on levels where one marching step covers at
+// most this fraction of a cell (runs of about two or more samples per entry)
+constexpr float kRunMaxStepCells = 0.5f;
 
 // activation rows (feature-major, row stride = sample capacity)
 template <int E>
@@ -352,8 +355,34 @@ __global__ __launch_bounds__(256) void train_loss_kernel(LossArgs a, Pcg32 rng0,
     }
 }
 
+// Sum of v over the run of equal keys that starts at this lane, in the run's
+// first lane (other lanes' sums are partial).  Runs are maximal stretches of
+// consecutive lanes with one key; every lane of the wave is active.
+INSTA_DEV float2 run_sum(uint32_t key, float2 v, bool& head) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t prev = __shfl_up(key, 1u);
+    head = lane == 0u || prev != key;
+    const uint64_t heads = __ballot(head);
+    const uint64_t later = lane == 63u ? 0ull : heads & (~0ull << (lane + 1u));
+    const uint32_t end = later ? (uint32_t)__builtin_ctzll(later) - 1u : 63u;
+#pragma unroll
+    for (uint32_t off = 1; off < 64u; off <<= 1) {
+        const float ox = __shfl_down(v.x, off), oy = __shfl_down(v.y, off);
+        if (lane + off <= end) {
+            v.x = v.x + ox;
+            v.y = v.y + oy;
+        }
+    }
+    return v;
+}
+
 // Backward of one sample: deltas of both MLPs (stored for the weight
-// gradients) and the hash-grid scatter.
+// gradients) and the hash-grid scatter.  The deltas are accumulated as outer
+// products (input-gradient k of a layer is the FMA chain over that layer's
+// outputs in ascending order), so at most one layer's accumulators are live.
+// On the levels of run_mask, the corner gradients of consecutive samples that
+// land on one table entry (coarse levels along one ray) are summed across the
+// wave first and added by one lane.
 template <int L, bool H16>
 __global__ __launch_bounds__(kTrainThreads) void train_backward_kernel(const float* __restrict__ params, GridParams gp,
                                                                        const uint32_t* __restrict__ n_dev,
@@ -363,65 +392,75 @@ __global__ __launch_bounds__(kTrainThreads) void train_backward_kernel(const flo
                                                                        const float4* __restrict__ dout,
                                                                        float* __restrict__ dact, uint32_t ld,
                                                                        float* __restrict__ grad_tab,
-                                                                       __half2* __restrict__ grad_tab_h) {
+                                                                       __half2* __restrict__ grad_tab_h,
+                                                                       uint32_t run_mask) {
     constexpr int E = 2 * L;
     using A = Act<E>;
     using O = Off<E>;
+    const uint32_t n = *n_dev;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *n_dev) return;
-    const float4 d4 = dout[i];
-    const float drgb[3] = {d4.x, d4.y, d4.z};
-    store_rows<3, 3>(dact, ld, Dact::DRGB, i, drgb);
+    if ((i & ~63u) >= n) return;  // whole wave past the batch
+    const bool valid = i < n;
+    const uint32_t ic = valid ? i : n - 1u;  // tail lanes recompute the last sample and store nothing
+    const float4 d4 = valid ? dout[ic] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const float* W4 = params + O::W4;
     const float* W3 = params + O::W3;
     const float* W2 = params + O::W2;
     const float* W1 = params + O::W1;
     const float* W0 = params + O::W0;
-    float dh3[kW];
-#pragma unroll
-    for (int k = 0; k < kW; ++k) {
-        const float s = W4[k] * drgb[0] + W4[kW + k] * drgb[1] + W4[2 * kW + k] * drgb[2];
-        dh3[k] = act[(size_t)(A::H3 + k) * ld + i] > 0.0f ? s : 0.0f;
+    if (valid) {
+        dact[(size_t)(Dact::DRGB + 0) * ld + i] = d4.x;
+        dact[(size_t)(Dact::DRGB + 1) * ld + i] = d4.y;
+        dact[(size_t)(Dact::DRGB + 2) * ld + i] = d4.z;
     }
-    store_rows<kW, kW>(dact, ld, Dact::DH3, i, dh3);
+    // colour layer 2 -> dh3 (one row at a time) -> dh2 accumulators
     float dh2[kW];
 #pragma unroll
-    for (int k = 0; k < kW; ++k) {
-        float s = 0.0f;
+    for (int k = 0; k < kW; ++k) dh2[k] = 0.0f;
 #pragma unroll
-        for (int m = 0; m < kW; ++m) s = __builtin_fmaf(W3[m * kW + k], dh3[m], s);
-        dh2[k] = act[(size_t)(A::H2 + k) * ld + i] > 0.0f ? s : 0.0f;
+    for (int m = 0; m < kW; ++m) {
+        const float s = W4[m] * d4.x + W4[kW + m] * d4.y + W4[2 * kW + m] * d4.z;
+        const float dh3 = act[(size_t)(A::H3 + m) * ld + ic] > 0.0f ? s : 0.0f;
+        if (valid) dact[(size_t)(Dact::DH3 + m) * ld + i] = dh3;
+#pragma unroll
+        for (int k = 0; k < kW; ++k) dh2[k] = __builtin_fmaf(W3[m * kW + k], dh3, dh2[k]);
     }
-    store_rows<kW, kW>(dact, ld, Dact::DH2, i, dh2);
+    // colour layer 0 input gradient restricted to the density outputs
     float dd[kDOut];
 #pragma unroll
-    for (int q = 0; q < kDOut; ++q) {
-        float s = 0.0f;
+    for (int q = 0; q < kDOut; ++q) dd[q] = 0.0f;
 #pragma unroll
-        for (int m = 0; m < kW; ++m) s = __builtin_fmaf(W2[m * kCin + q], dh2[m], s);
-        dd[q] = s;
+    for (int m = 0; m < kW; ++m) {
+        const float g = act[(size_t)(A::H2 + m) * ld + ic] > 0.0f ? dh2[m] : 0.0f;
+        if (valid) dact[(size_t)(Dact::DH2 + m) * ld + i] = g;
+#pragma unroll
+        for (int q = 0; q < kDOut; ++q) dd[q] = __builtin_fmaf(W2[m * kCin + q], g, dd[q]);
     }
     dd[0] = dd[0] + d4.w;  // density logit = density-MLP output 0
-    store_rows<kDOut, kDOut>(dact, ld, Dact::DDENS, i, dd);
-    float dh1[kW];
+    if (valid) {
+#pragma unroll
+        for (int q = 0; q < kDOut; ++q) dact[(size_t)(Dact::DDENS + q) * ld + i] = dd[q];
+    }
+    // density layers: dh1 one row at a time -> dL/dencoding accumulators
+    float denc[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) denc[e] = 0.0f;
 #pragma unroll
     for (int k = 0; k < kW; ++k) {
         float s = 0.0f;
 #pragma unroll
         for (int q = 0; q < kDOut; ++q) s = __builtin_fmaf(W1[q * kW + k], dd[q], s);
-        dh1[k] = act[(size_t)(A::H1 + k) * ld + i] > 0.0f ? s : 0.0f;
-    }
-    store_rows<kW, kW>(dact, ld, Dact::DH1, i, dh1);
-    float denc[E];
+        const float dh1 = act[(size_t)(A::H1 + k) * ld + ic] > 0.0f ? s : 0.0f;
+        if (valid) dact[(size_t)(Dact::DH1 + k) * ld + i] = dh1;
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-        float s = 0.0f;
-#pragma unroll
-        for (int k = 0; k < kW; ++k) s = __builtin_fmaf(W0[k * E + e], dh1[k], s);
-        denc[e] = s;
+        for (int e = 0; e < E; ++e) denc[e] = __builtin_fmaf(W0[k * E + e], dh1, denc[e]);
     }
+    // dL/dencoding through LDS, so that the level loop indexes it at run time
+    __shared__ float s_denc[E][kTrainThreads];
+#pragma unroll
+    for (int e = 0; e < E; ++e) s_denc[e][threadIdx.x] = denc[e];
     // hash-grid scatter (the corner/weight arithmetic of encode_level)
-    const float4 p4 = cpos[c_idx[i]];
+    const float4 p4 = cpos[c_idx[ic]];
     const float xs[3] = {p4.x, p4.y, p4.z};
 #pragma unroll 1
     for (int l = 0; l < L; ++l) {
@@ -429,14 +468,17 @@ __global__ __launch_bounds__(kTrainThreads) void train_backward_kernel(const flo
         float frac[3];
         uint32_t g[3];
 #pragma unroll
-        for (int dd3 = 0; dd3 < 3; ++dd3) {
-            const float pp = xs[dd3] * sc + 0.5f;
+        for (int d = 0; d < 3; ++d) {
+            const float pp = xs[d] * sc + 0.5f;
             const float fl = floorf(pp);
-            frac[dd3] = pp - fl;
-            g[dd3] = (uint32_t)(int32_t)fl;
+            frac[d] = pp - fl;
+            g[d] = (uint32_t)(int32_t)fl;
         }
         const bool dense_l = (gp.dense_mask >> l) & 1u;
+        const bool runs = (run_mask >> l) & 1u;
         const uint32_t res = gp.res[l], size = gp.size[l], off = gp.offset[l];
+        const float e0 = valid ? s_denc[2 * l][threadIdx.x] : 0.0f;
+        const float e1 = valid ? s_denc[2 * l + 1][threadIdx.x] : 0.0f;
 #pragma unroll
         for (uint32_t c = 0; c < 8; ++c) {
             float w = 1.0f;
@@ -448,11 +490,20 @@ __global__ __launch_bounds__(kTrainThreads) void train_backward_kernel(const flo
                 cg[k] = g[k] + (up ? 1u : 0u);
             }
             const uint32_t idx = off + grid_index(cg, dense_l, res, size);
-            if constexpr (H16) {
-                unsafeAtomicAdd(grad_tab_h + idx, __floats2half2_rn(w * denc[2 * l], w * denc[2 * l + 1]));
-            } else {
-                atomicAdd(grad_tab + 2 * (size_t)idx, w * denc[2 * l]);
-                atomicAdd(grad_tab + 2 * (size_t)idx + 1, w * denc[2 * l + 1]);
+            float2 v = make_float2(w * e0, w * e1);
+            bool lead = valid;
+            if (runs) {
+                bool head;
+                v = run_sum(valid ? idx : ~0u, v, head);
+                lead = valid && head;
+            }
+            if (lead) {
+                if constexpr (H16) {
+                    unsafeAtomicAdd(grad_tab_h + idx, __floats2half2_rn(v.x, v.y));
+                } else {
+                    atomicAdd(grad_tab + 2 * (size_t)idx, v.x);
+                    atomicAdd(grad_tab + 2 * (size_t)idx + 1, v.y);
+                }
             }
         }
     }
@@ -616,12 +667,18 @@ insta_status launch_train_backward(insta_ctx* ctx, TrainState* ts, const insta_t
     const uint32_t n_mlp = (uint32_t)ctx->layout.n_mlp_params;
     const dim3 sgrid((cap + kTrainThreads - 1) / kTrainThreads);
     float* grad_tab = ts->grad + n_mlp;
+    // levels whose cells span at least 1 / kRunMaxStepCells marching steps
+    uint32_t run_mask = 0;
+    for (int l = 0; l < L; ++l)
+        if (ctx->cfg.dt * gp.scale[l] <= kRunMaxStepCells) run_mask |= 1u << l;
     if (tc.grid_grad_fp16)
         hipLaunchKernelGGL((train_backward_kernel<L, true>), sgrid, dim3(kTrainThreads), 0, stream, ctx->d_params, gp,
-                           n_dev, ts->c_idx, ts->s_cpos, ts->act, ts->c_dout, ts->dact, cap, grad_tab, ts->grad_h);
+                           n_dev, ts->c_idx, ts->s_cpos, ts->act, ts->c_dout, ts->dact, cap, grad_tab, ts->grad_h,
+                           run_mask);
     else
         hipLaunchKernelGGL((train_backward_kernel<L, false>), sgrid, dim3(kTrainThreads), 0, stream, ctx->d_params, gp,
-                           n_dev, ts->c_idx, ts->s_cpos, ts->act, ts->c_dout, ts->dact, cap, grad_tab, ts->grad_h);
+                           n_dev, ts->c_idx, ts->s_cpos, ts->act, ts->c_dout, ts->dact, cap, grad_tab, ts->grad_h,
+                           run_mask);
     INSTA_HIP(hipGetLastError());
     WArgs wa;
     wa.p[0] = WProduct{Dact::DH1, A::ENC, kW, E, O::W0};
