@@ -104,15 +104,22 @@ __global__ __launch_bounds__(kRayThreads) void march_kernel(RenderArgs a, const 
         const uint32_t done = r_steps[r];
         const uint32_t budget = a.max_steps > done ? a.max_steps - done : 0u;
         const uint32_t kmax = K < budget ? K : budget;
-        V3 pos;
+        // one loop iteration per sample or per empty-voxel skip (the t
+        // sequence of skip_empty + t += dt, without a nested loop whose trip
+        // count differs between lanes)
         while (count < kmax) {
-            if (!skip_empty(a, o, d, idir, t, pos)) {
+            const V3 pos = o + d * t;
+            if (!box_contains(a, pos)) {
                 exited = true;
                 break;
             }
-            t_buf[(size_t)count * n_alive + k] = t;
-            ++count;
-            t += a.dt;
+            if (occupied(a.grid, a.G, pos)) {
+                t_buf[(size_t)count * n_alive + k] = t;
+                ++count;
+                t += a.dt;
+            } else {
+                t = advance_to_next_voxel(t, a.dt, pos, d, idir, a.G);
+            }
         }
     }
     const uint32_t base = wave_reserve(&counters[kCntSamples], count);

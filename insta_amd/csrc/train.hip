@@ -107,6 +107,7 @@ struct TrainState {
     uint32_t* c_idx = nullptr;     // compacted batch: sample index
     float4* c_dout = nullptr;      // compacted batch: dL/d(rgb logits, density logit)
     float *act = nullptr, *dact = nullptr, *part = nullptr;
+    float* t_buf = nullptr;        // ray march scratch, [max_steps][ray_cap]
     // optimizer
     float *grad = nullptr, *adam_m = nullptr, *adam_v = nullptr;
     __half2* grad_h = nullptr;
@@ -181,6 +182,7 @@ __global__ __launch_bounds__(256) void train_rays_kernel(RayArgs a, uint32_t n_r
                                                          uint32_t* __restrict__ ray_base,
                                                          uint32_t* __restrict__ ray_cnt,
                                                          float4* __restrict__ s_pos,
+                                                         float* __restrict__ t_buf,
                                                          uint32_t* __restrict__ counters) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t img = 0, pix = 0, count = 0;
@@ -189,11 +191,19 @@ __global__ __launch_bounds__(256) void train_rays_kernel(RayArgs a, uint32_t n_r
     RenderArgs ra = a.box;
     if (i < n_rays && train_ray(a, i, n_rays, rays_total, rng, img, pix, o, d, t0, ra)) {
         idir = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+        // one iteration per sample or empty-voxel skip; sample t values to
+        // step-major scratch (t_buf[j * n_rays + i])
         float t = t0;
-        V3 pos;
-        while (count < a.box.max_steps && skip_empty(ra, o, d, idir, t, pos)) {
-            ++count;
-            t += a.box.dt;
+        while (count < a.box.max_steps) {
+            const V3 pos = o + d * t;
+            if (!box_contains(ra, pos)) break;
+            if (occupied(ra.grid, ra.G, pos)) {
+                t_buf[(size_t)count * n_rays + i] = t;
+                ++count;
+                t += a.box.dt;
+            } else {
+                t = advance_to_next_voxel(t, a.box.dt, pos, d, idir, ra.G);
+            }
         }
     }
     const uint32_t base = wave_reserve(&counters[kTSamples], count);
@@ -209,12 +219,9 @@ __global__ __launch_bounds__(256) void train_rays_kernel(RayArgs a, uint32_t n_r
     ray_src[r] = i;
     ray_base[r] = base;
     ray_cnt[r] = count;
-    float t = t0;
-    V3 pos;
     for (uint32_t j = 0; j < count; ++j) {
-        skip_empty(ra, o, d, idir, t, pos);
+        const V3 pos = o + d * t_buf[(size_t)j * n_rays + i];
         s_pos[base + j] = make_float4(pos.x, pos.y, pos.z, __uint_as_float(r));
-        t += a.box.dt;
     }
 }
 
@@ -608,6 +615,7 @@ void free_workspace_train(TrainState* ts) {
     dev_free(ts->ray_src); dev_free(ts->ray_base); dev_free(ts->ray_cnt);
     dev_free(ts->s_pos); dev_free(ts->s_cpos); dev_free(ts->s_cdir); dev_free(ts->s_out); dev_free(ts->s_hit);
     dev_free(ts->c_idx); dev_free(ts->c_dout); dev_free(ts->act); dev_free(ts->dact); dev_free(ts->part);
+    dev_free(ts->t_buf);
     ts->ray_cap = ts->sample_cap = ts->compact_cap = 0;
 }
 
@@ -632,6 +640,7 @@ insta_status ensure_train_workspace(insta_ctx* ctx, TrainState* ts, uint32_t n_r
     ok = ok && train_alloc(ts->act, act_rows * cc * sizeof(float));
     ok = ok && train_alloc(ts->dact, (size_t)Dact::N * cc * sizeof(float));
     ok = ok && train_alloc(ts->part, ((cc + kChunk - 1) / kChunk) * n_mlp * sizeof(float));
+    ok = ok && train_alloc(ts->t_buf, (size_t)rc * ctx->cfg.max_steps * sizeof(float));
     if (!ok) {
         free_workspace_train(ts);
         set_error("training workspace allocation failed");
@@ -803,7 +812,7 @@ insta_status train_step(insta_ctx* ctx, const insta_train_config& tc, insta_trai
     ra.max_samples = cap;
     hipLaunchKernelGGL(train_rays_kernel, dim3((n_rays + 255) / 256), dim3(256), 0, stream, ra, n_rays, ts->rays_total,
                        rng, ts->ray_d, ts->ray_frame, ts->ray_img, ts->ray_pix, ts->ray_src, ts->ray_base, ts->ray_cnt,
-                       ts->s_pos, ts->counters);
+                       ts->s_pos, ts->t_buf, ts->counters);
     INSTA_HIP(hipGetLastError());
     hipLaunchKernelGGL(train_count_kernel, dim3(1), dim3(64), 0, stream, ts->counters);
     st = launch_deform_samples(ctx, -1, ts->ray_frame, ts->counters + kTValid, cap, ts->s_pos, ts->ray_d, ts->s_hit,
