@@ -382,6 +382,11 @@ insta_status insta_server_poll(insta_server* srv, insta_result* out, uint32_t ma
                                uint32_t* n_out, int wait);
 /* Seconds since the server was created (the clock of insta_result). */
 double insta_server_clock(insta_server* srv);
+/* With use_graphs: requests whose captured chain (preparation + queued
+ * render rounds) updated the worker's executable graph for that avatar in
+ * place, and those that instantiated one (first request of a worker and
+ * avatar, or a changed graph shape).  No counterpart in the reference. */
+insta_status insta_server_graph_stats(insta_server* srv, uint64_t* updates, uint64_t* builds);
 /* Finishes every queued request, then releases the server. */
 void insta_server_destroy(insta_server* srv);
 

@@ -145,6 +145,7 @@ SIGNATURES = [
     ("insta_server_submit", C.c_int, [_VP, C.POINTER(InstaRequest), C.POINTER(C.c_uint64)]),
     ("insta_server_poll", C.c_int, [_VP, C.POINTER(InstaResult), C.c_uint32, C.POINTER(C.c_uint32), C.c_int]),
     ("insta_server_clock", C.c_double, [_VP]),
+    ("insta_server_graph_stats", C.c_int, [_VP, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ("insta_server_destroy", None, [_VP]),
 ]
 
@@ -480,6 +481,12 @@ class Server:
 
     def clock(self) -> float:
         return self.lib.insta_server_clock(self.h)
+
+    def graph_stats(self) -> tuple[int, int]:
+        """(requests that updated a worker's graph in place, graphs instantiated)."""
+        u, b = C.c_uint64(), C.c_uint64()
+        check(self.lib.insta_server_graph_stats(self.h, C.byref(u), C.byref(b)), "insta_server_graph_stats")
+        return u.value, b.value
 
     def close(self) -> None:
         if self.h:

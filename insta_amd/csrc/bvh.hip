@@ -347,27 +347,30 @@ __global__ __launch_bounds__(kQueryThreads) void closest_kernel(MeshView m, cons
 // mark_flame_density_grid + grid_to_bitfield (testbed_nerf.cu:364-389, 585-609):
 // one thread per byte of 8 Morton-consecutive cells.
 __global__ __launch_bounds__(kQueryThreads) void occupancy_kernel(MeshView m, uint32_t G, float threshold,
-                                                                 uint8_t* __restrict__ bits, uint32_t n_bytes) {
+                                                                 uint8_t* __restrict__ bits, uint32_t n_cells) {
     extern __shared__ uint32_t stack[];  // [stack_depth(k)][kQueryThreads]
-    const uint32_t byte = blockIdx.x * blockDim.x + threadIdx.x;
-    if (byte >= n_bytes) return;
+    // one cell per lane, cells in Morton order; the wave's 64 cell bits are
+    // gathered with a ballot and stored as 8 bytes
+    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
     const float g = (float)G;
     // cells farther than the threshold are never occupied: bound the query
     // a little above threshold^2 so sqrt(bound) >= threshold after rounding
     const float bound = threshold * threshold * (1.0f + 0x1p-10f);
-    uint32_t v = 0;
-    for (uint32_t j = 0; j < 8; ++j) {
-        const uint32_t idx = byte * 8 + j;
+    bool occ = false;
+    if (idx < n_cells) {
         const uint32_t x = compact3(idx), y = compact3(idx >> 1), z = compact3(idx >> 2);
         V3 pos = v3((((float)x + 0.5f) / g - 0.5f) * 1.0f + 0.5f, (((float)y + 0.5f) / g - 0.5f) * 1.0f + 0.5f,
                     (((float)z + 0.5f) / g - 0.5f) * 1.0f + 0.5f);
         Hit h = bvh_closest(m, pos, stack + threadIdx.x, kQueryThreads, bound);
-        if (h.idx < 0) continue;
-        const float dist = sqrtf(h.dist_sq);
-        const uint32_t mask = __float_as_uint(m.btris[3 * h.idx + 2].z);
-        if (dist < threshold && mask != 0u) v |= 1u << j;
+        if (h.idx >= 0) {
+            const float dist = sqrtf(h.dist_sq);
+            const uint32_t mask = __float_as_uint(m.btris[3 * h.idx + 2].z);
+            occ = dist < threshold && mask != 0u;
+        }
     }
-    bits[byte] = (uint8_t)v;
+    const uint64_t word = __ballot(occ);
+    const uint32_t lane = threadIdx.x & 63u;
+    if ((lane & 7u) == 0 && idx < n_cells) bits[idx >> 3] = (uint8_t)(word >> lane);
 }
 
 // Linear-order copy of a Morton-order bitfield: bit x + G(y + G z) of the
@@ -454,10 +457,10 @@ insta_status launch_closest(insta_ctx* ctx, uint32_t slot, const float* pts, uin
 
 insta_status launch_occupancy(insta_ctx* ctx, uint32_t slot, float threshold, hipStream_t stream) {
     const uint32_t G = ctx->cfg.grid_size;
-    const uint32_t n_bytes = G * G * G / 8;
+    const uint32_t n_cells = G * G * G;  // G is a multiple of 8: whole bytes per 8 lanes
     const size_t shmem = (size_t)stack_depth(ctx->bvh_k) * kQueryThreads * sizeof(uint32_t);
-    hipLaunchKernelGGL(occupancy_kernel, dim3((n_bytes + kQueryThreads - 1) / kQueryThreads), dim3(kQueryThreads),
-                       shmem, stream, ctx->mesh(slot), G, threshold, ctx->grid(slot), n_bytes);
+    hipLaunchKernelGGL(occupancy_kernel, dim3((n_cells + kQueryThreads - 1) / kQueryThreads), dim3(kQueryThreads),
+                       shmem, stream, ctx->mesh(slot), G, threshold, ctx->grid(slot), n_cells);
     INSTA_HIP(hipGetLastError());
     return INSTA_OK;
 }

@@ -182,6 +182,17 @@ insta_status launch_set_built(insta_ctx* ctx, uint32_t first, uint32_t n, hipStr
 insta_status render(insta_ctx* ctx, RenderWS& ws, const insta_camera* cam, uint32_t width, uint32_t height,
                     int32_t frame, int32_t occupancy, uint32_t spp, float* rgba, float* depth, uint32_t* steps,
                     hipStream_t stream);
+// The render split at its one host synchronisation: enqueue_render queues ray
+// init, `rounds` march/deform/network/composite rounds (live counts stay on
+// the device) and the final image write, with no host synchronisation, so a
+// stream capture of it is a complete graph; finish_render reads the counters,
+// runs any further rounds the rays still need and rewrites the image.
+insta_status enqueue_render(insta_ctx* ctx, RenderWS& ws, const insta_camera* cam, uint32_t width, uint32_t height,
+                            int32_t frame, int32_t occupancy, uint32_t spp, uint32_t rounds, float* rgba,
+                            float* depth, uint32_t* steps, hipStream_t stream);
+insta_status finish_render(insta_ctx* ctx, RenderWS& ws, const insta_camera* cam, uint32_t width, uint32_t height,
+                           int32_t frame, int32_t occupancy, uint32_t spp, uint32_t rounds, float* rgba, float* depth,
+                           uint32_t* steps, hipStream_t stream);
 void free_workspace(insta_ctx* ctx);
 insta_status set_profiling(insta_ctx* ctx, bool on);
 // training (train.hip)

@@ -5,9 +5,10 @@
 // reference's unit-cube case, testbed_nerf.cu:2684-2691), with this schedule:
 //   init      one thread per pixel: pinhole ray, render-box entry, one-step
 //             ld jitter and empty-space skip; live rays appended to a list
-//   repeat    march: each live ray emits up to K samples (K sized so all fit
-//                    the sample buffer), counted first, then written
-//                    contiguously at a wave-aggregated atomic offset
+//   repeat    march: each live ray emits up to K samples (K = sample
+//                    capacity / live rays, computed on the device), one
+//                    walk with t to scratch, then written contiguously at a
+//                    wave-aggregated atomic offset
 //             deform: posed -> canonical per sample (deform.hip)
 //             network: encode + MLPs per sample (network.hip)
 //             composite: per ray over its samples; rays that neither hit
@@ -15,8 +16,9 @@
 //                    max_steps are appended to the next live list
 //   finalize  one thread per pixel writes RGBA/depth/step counts.
 // Each ray's samples depend only on its own t sequence, so the result does
-// not depend on K or on the order rays are processed in.  The host reads one
-// counter (the live-ray count) per round.
+// not depend on K or on the order rays are processed in.  Live counts stay on
+// the device: the first round is queued with init and finalize, and the host
+// reads the counters once, adding rounds only for rays still live.
 #include <type_traits>
 
 #include "context.hpp"
@@ -34,6 +36,7 @@ enum : uint32_t {
     kCntAlive0 = 3,
     kCntComposited = 4,
     kCntEmitted = 5,
+    kCntRounds = 6,
     kNumCounters = 16
 };
 
@@ -81,14 +84,22 @@ __global__ __launch_bounds__(kRayThreads) void init_rays_kernel(RenderArgs a, fl
 // lane writes its samples from the stored t values (pos = o + d t exactly as
 // during the walk).
 __global__ __launch_bounds__(kRayThreads) void march_kernel(RenderArgs a, const uint32_t* __restrict__ alive,
-                                                            uint32_t n_alive, uint32_t K, float4* __restrict__ r_o,
+                                                            const uint32_t* __restrict__ n_alive_dev,
+                                                            uint64_t sample_cap, float4* __restrict__ r_o,
                                                             const float4* __restrict__ r_d,
                                                             const uint32_t* __restrict__ r_steps,
                                                             uint32_t* __restrict__ r_flags,
                                                             uint32_t* __restrict__ r_off, uint32_t* __restrict__ r_cnt,
                                                             float* __restrict__ t_buf, float4* __restrict__ s_pos,
                                                             uint32_t* __restrict__ counters) {
+    const uint32_t n_alive = *n_alive_dev;
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if ((k & ~63u) >= n_alive) return;  // whole wave past the live list
+    // samples per live ray this round: all fit the sample buffer
+    uint64_t K64 = sample_cap / n_alive;
+    if (K64 > a.max_steps) K64 = a.max_steps;
+    if (K64 < 1) K64 = 1;
+    const uint32_t K = (uint32_t)K64;
     const bool valid = k < n_alive;
     uint32_t r = 0, count = 0;
     bool exited = false;
@@ -136,7 +147,8 @@ __global__ __launch_bounds__(kRayThreads) void march_kernel(RenderArgs a, const 
 }
 
 __global__ __launch_bounds__(kRayThreads) void composite_kernel(RenderArgs a, const uint32_t* __restrict__ alive,
-                                                                uint32_t n_alive, const uint32_t* __restrict__ r_off,
+                                                                const uint32_t* __restrict__ n_alive_dev,
+                                                                const uint32_t* __restrict__ r_off,
                                                                 const uint32_t* __restrict__ r_cnt,
                                                                 const float4* __restrict__ s_pos,
                                                                 const float4* __restrict__ s_out,
@@ -146,7 +158,13 @@ __global__ __launch_bounds__(kRayThreads) void composite_kernel(RenderArgs a, co
                                                                 const uint32_t* __restrict__ r_flags,
                                                                 uint32_t* __restrict__ next_alive,
                                                                 uint32_t* __restrict__ counters, uint32_t next_slot) {
+    const uint32_t n_alive = *n_alive_dev;
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0 && n_alive) {  // round statistics (march of this round is complete)
+        counters[kCntEmitted] += counters[kCntSamples];
+        counters[kCntRounds] += 1;
+    }
+    if ((k & ~63u) >= n_alive) return;
     bool keep = false;
     uint32_t r = 0, used = 0;
     if (k < n_alive) {
@@ -213,6 +231,15 @@ __global__ void finalize_kernel(uint32_t n, const float4* __restrict__ r_rgba, c
 __global__ void clear_counters_kernel(uint32_t* counters, uint32_t first, uint32_t n) {
     const uint32_t i = threadIdx.x;
     if (i < n) counters[first + i] = 0;
+}
+
+// Before a round: clear the next live list's count and the round's sample
+// count; the first round also keeps the initial live count.
+__global__ void round_begin_kernel(uint32_t* counters, uint32_t next_slot, uint32_t first) {
+    if (threadIdx.x != 0) return;
+    if (first) counters[kCntAlive0] = counters[kCntAliveA];
+    counters[next_slot] = 0;
+    counters[kCntSamples] = 0;
 }
 
 // ---- host side of the Sobol pixel offset (random_val.cuh:272-322) --------
@@ -395,14 +422,10 @@ void bind_render_ws(RenderWS& ws, char* base, uint32_t n_pixels, uint32_t sample
     ws.external = true;
 }
 
-insta_status render(insta_ctx* ctx, RenderWS& ws, const insta_camera* cam, uint32_t width, uint32_t height,
-                    int32_t frame, int32_t occupancy, uint32_t spp, float* rgba, float* depth, uint32_t* steps,
-                    hipStream_t stream) {
-    const uint32_t n_pix = width * height;
-    // event timing only for the context's own workspace (one stream at a time)
-    const bool prof = ctx->profiling && &ws == &ctx->ws;
-    insta_status st = ensure_workspace(ws, n_pix);
-    if (st != INSTA_OK) return st;
+namespace {
+
+RenderArgs render_args(const insta_ctx* ctx, const insta_camera* cam, uint32_t width, uint32_t height,
+                       int32_t occupancy, uint32_t spp) {
     RenderArgs a;
     for (int i = 0; i < 12; ++i) a.M[i] = cam->matrix[i];
     a.cx = cam->screen_center[0];
@@ -426,6 +449,74 @@ insta_status render(insta_ctx* ctx, RenderWS& ws, const insta_camera* cam, uint3
         a.hi[i] = ctx->render_hi[i];
     }
     a.grid = ctx->grid_lin(occupancy < 0 ? 0u : (uint32_t)occupancy + 1u);
+    return a;
+}
+
+// One march/deform/network/composite round over live list `cur`, with the
+// live count read on the device.  `max_alive` bounds the live count (it
+// sizes the ray grids).
+insta_status enqueue_round(insta_ctx* ctx, RenderWS& ws, const RenderArgs& a, int32_t frame, uint32_t cur,
+                           bool first, uint32_t max_alive, bool prof, hipStream_t stream) {
+    uint32_t* cnt = ws.d_counters;
+    const uint32_t next = 1u - cur;
+    const uint32_t cur_slot = cur == 0 ? kCntAliveA : kCntAliveB;
+    const uint32_t next_slot = next == 0 ? kCntAliveA : kCntAliveB;
+    hipLaunchKernelGGL(round_begin_kernel, dim3(1), dim3(64), 0, stream, cnt, next_slot, first ? 1u : 0u);
+    const dim3 rgrid((max_alive + kRayThreads - 1) / kRayThreads);
+    prof_begin(ctx, prof, INSTA_KERNEL_MARCH, stream);
+    hipLaunchKernelGGL(march_kernel, rgrid, dim3(kRayThreads), 0, stream, a, ws.alive[cur], cnt + cur_slot,
+                       (uint64_t)ws.samples, ws.r_o, ws.r_d, ws.r_steps, ws.r_flags, ws.r_off, ws.r_cnt,
+                       reinterpret_cast<float*>(ws.s_hit), ws.s_pos, cnt);
+    INSTA_HIP(hipGetLastError());
+    prof_end(ctx, prof, INSTA_KERNEL_MARCH, 0, stream);
+    // at most max_alive * K <= sample capacity samples this round
+    const uint32_t n_max = (uint32_t)ws.samples;
+    prof_begin(ctx, prof, INSTA_KERNEL_DEFORM, stream);
+    insta_status st = launch_deform_soa(ctx, ws, frame, cnt + kCntSamples, n_max, stream);
+    if (st != INSTA_OK) return st;
+    prof_end(ctx, prof, INSTA_KERNEL_DEFORM, 0, stream);
+    prof_begin(ctx, prof, INSTA_KERNEL_NETWORK, stream);
+    st = launch_network_samples(ctx, cnt + kCntSamples, n_max, ws.s_cpos, ws.s_cdir, ws.s_out, stream);
+    if (st != INSTA_OK) return st;
+    prof_end(ctx, prof, INSTA_KERNEL_NETWORK, 0, stream);
+    prof_begin(ctx, prof, INSTA_KERNEL_COMPOSITE, stream);
+    hipLaunchKernelGGL(composite_kernel, rgrid, dim3(kRayThreads), 0, stream, a, ws.alive[cur], cnt + cur_slot,
+                       ws.r_off, ws.r_cnt, ws.s_pos, ws.s_out, ws.r_d, ws.r_rgba, ws.r_depth, ws.r_steps,
+                       ws.r_flags, ws.alive[next], cnt, next_slot);
+    INSTA_HIP(hipGetLastError());
+    prof_end(ctx, prof, INSTA_KERNEL_COMPOSITE, 0, stream);
+    return INSTA_OK;
+}
+
+insta_status enqueue_finalize(insta_ctx* ctx, RenderWS& ws, uint32_t n_pix, float* rgba, float* depth,
+                              uint32_t* steps, bool prof, hipStream_t stream) {
+    prof_begin(ctx, prof, INSTA_KERNEL_FINALIZE, stream);
+    hipLaunchKernelGGL(finalize_kernel, dim3((n_pix + kRayThreads - 1) / kRayThreads), dim3(kRayThreads), 0, stream,
+                       n_pix, ws.r_rgba, ws.r_depth, ws.r_steps, reinterpret_cast<float4*>(rgba), depth, steps);
+    INSTA_HIP(hipGetLastError());
+    prof_end(ctx, prof, INSTA_KERNEL_FINALIZE, n_pix, stream);
+    return INSTA_OK;
+}
+
+// profiled item counts, known once the counters are on the host
+void prof_items(insta_ctx* ctx, bool prof, uint32_t alive, uint32_t samples) {
+    if (!prof) return;
+    ctx->prof_pending_items[INSTA_KERNEL_MARCH] = alive;
+    ctx->prof_pending_items[INSTA_KERNEL_COMPOSITE] = alive;
+    ctx->prof_pending_items[INSTA_KERNEL_DEFORM] = samples;
+    ctx->prof_pending_items[INSTA_KERNEL_NETWORK] = samples;
+}
+
+}  // namespace
+
+insta_status enqueue_render(insta_ctx* ctx, RenderWS& ws, const insta_camera* cam, uint32_t width, uint32_t height,
+                            int32_t frame, int32_t occupancy, uint32_t spp, uint32_t rounds, float* rgba,
+                            float* depth, uint32_t* steps, hipStream_t stream) {
+    const uint32_t n_pix = width * height;
+    const bool prof = ctx->profiling && &ws == &ctx->ws && rounds == 1;
+    insta_status st = ensure_workspace(ws, n_pix);
+    if (st != INSTA_OK) return st;
+    const RenderArgs a = render_args(ctx, cam, width, height, occupancy, spp);
     uint32_t* cnt = ws.d_counters;
     hipLaunchKernelGGL(clear_counters_kernel, dim3(1), dim3(64), 0, stream, cnt, 0u, (uint32_t)kNumCounters);
     prof_begin(ctx, prof, INSTA_KERNEL_INIT, stream);
@@ -433,72 +524,58 @@ insta_status render(insta_ctx* ctx, RenderWS& ws, const insta_camera* cam, uint3
                        ws.r_o, ws.r_d, ws.r_rgba, ws.r_depth, ws.r_steps, ws.alive[0], cnt);
     INSTA_HIP(hipGetLastError());
     prof_end(ctx, prof, INSTA_KERNEL_INIT, n_pix, stream);
+    for (uint32_t r = 0; r < rounds; ++r) {
+        st = enqueue_round(ctx, ws, a, frame, r & 1u, r == 0, n_pix, prof, stream);
+        if (st != INSTA_OK) return st;
+    }
+    return enqueue_finalize(ctx, ws, n_pix, rgba, depth, steps, prof, stream);
+}
+
+insta_status finish_render(insta_ctx* ctx, RenderWS& ws, const insta_camera* cam, uint32_t width, uint32_t height,
+                           int32_t frame, int32_t occupancy, uint32_t spp, uint32_t rounds, float* rgba, float* depth,
+                           uint32_t* steps, hipStream_t stream) {
+    const uint32_t n_pix = width * height;
+    const bool prof = ctx->profiling && &ws == &ctx->ws && rounds == 1;
+    uint32_t* cnt = ws.d_counters;
     INSTA_HIP(hipMemcpyAsync(ws.h_counters, cnt, kNumCounters * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
     INSTA_HIP(hipStreamSynchronize(stream));
+    prof_items(ctx, prof, ws.h_counters[kCntAlive0], ws.h_counters[kCntSamples]);
     prof_collect(ctx, prof);
-    uint32_t n_alive = ws.h_counters[kCntAliveA];
+    uint32_t cur = rounds & 1u;
+    uint32_t n_alive = ws.h_counters[cur == 0 ? kCntAliveA : kCntAliveB];
+    if (n_alive) {
+        // rays left after the enqueued rounds: one round per host check
+        const RenderArgs a = render_args(ctx, cam, width, height, occupancy, spp);
+        while (n_alive) {
+            insta_status st = enqueue_round(ctx, ws, a, frame, cur, false, n_alive, prof, stream);
+            if (st != INSTA_OK) return st;
+            INSTA_HIP(hipMemcpyAsync(ws.h_counters, cnt, kNumCounters * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                     stream));
+            INSTA_HIP(hipStreamSynchronize(stream));
+            prof_items(ctx, prof, n_alive, ws.h_counters[kCntSamples]);
+            prof_collect(ctx, prof);
+            cur = 1u - cur;
+            n_alive = ws.h_counters[cur == 0 ? kCntAliveA : kCntAliveB];
+        }
+        // the image written by the enqueued finalize was partial
+        insta_status st = enqueue_finalize(ctx, ws, n_pix, rgba, depth, steps, false, stream);
+        if (st != INSTA_OK) return st;
+    }
     ws.stats = insta_render_stats{};
     ws.stats.n_rays = n_pix;
-    ws.stats.n_rays_alive0 = n_alive;
-    uint64_t emitted = 0;
-    uint32_t cur = 0, iters = 0;
-    while (n_alive > 0) {
-        uint64_t K = ws.samples / n_alive;
-        if (K > ctx->cfg.max_steps) K = ctx->cfg.max_steps;
-        if (K < 1) K = 1;
-        const uint32_t next = 1u - cur;
-        const uint32_t next_slot = next == 0 ? kCntAliveA : kCntAliveB;
-        hipLaunchKernelGGL(clear_counters_kernel, dim3(1), dim3(64), 0, stream, cnt, (uint32_t)next_slot, 1u);
-        hipLaunchKernelGGL(clear_counters_kernel, dim3(1), dim3(64), 0, stream, cnt, (uint32_t)kCntSamples, 1u);
-        const dim3 rgrid((n_alive + kRayThreads - 1) / kRayThreads);
-        prof_begin(ctx, prof, INSTA_KERNEL_MARCH, stream);
-        hipLaunchKernelGGL(march_kernel, rgrid, dim3(kRayThreads), 0, stream, a, ws.alive[cur], n_alive, (uint32_t)K,
-                           ws.r_o, ws.r_d, ws.r_steps, ws.r_flags, ws.r_off, ws.r_cnt,
-                           reinterpret_cast<float*>(ws.s_hit), ws.s_pos, cnt);
-        INSTA_HIP(hipGetLastError());
-        prof_end(ctx, prof, INSTA_KERNEL_MARCH, n_alive, stream);
-        const uint64_t n_max64 = (uint64_t)n_alive * K;
-        const uint32_t n_max = (uint32_t)(n_max64 < ws.samples ? n_max64 : ws.samples);
-        prof_begin(ctx, prof, INSTA_KERNEL_DEFORM, stream);
-        st = launch_deform_soa(ctx, ws, frame, cnt + kCntSamples, n_max, stream);
-        if (st != INSTA_OK) return st;
-        prof_end(ctx, prof, INSTA_KERNEL_DEFORM, 0, stream);
-        prof_begin(ctx, prof, INSTA_KERNEL_NETWORK, stream);
-        st = launch_network_samples(ctx, cnt + kCntSamples, n_max, ws.s_cpos, ws.s_cdir, ws.s_out, stream);
-        if (st != INSTA_OK) return st;
-        prof_end(ctx, prof, INSTA_KERNEL_NETWORK, 0, stream);
-        prof_begin(ctx, prof, INSTA_KERNEL_COMPOSITE, stream);
-        hipLaunchKernelGGL(composite_kernel, rgrid, dim3(kRayThreads), 0, stream, a, ws.alive[cur], n_alive,
-                           ws.r_off, ws.r_cnt, ws.s_pos, ws.s_out, ws.r_d, ws.r_rgba, ws.r_depth,
-                           ws.r_steps, ws.r_flags, ws.alive[next], cnt, next_slot);
-        INSTA_HIP(hipGetLastError());
-        prof_end(ctx, prof, INSTA_KERNEL_COMPOSITE, n_alive, stream);
-        INSTA_HIP(hipMemcpyAsync(ws.h_counters, cnt, kNumCounters * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-        INSTA_HIP(hipStreamSynchronize(stream));
-        // sample counts are known only now; attribute them to deform/network
-        if (prof) {
-            ctx->prof_pending_items[INSTA_KERNEL_DEFORM] = ws.h_counters[kCntSamples];
-            ctx->prof_pending_items[INSTA_KERNEL_NETWORK] = ws.h_counters[kCntSamples];
-        }
-        prof_collect(ctx, prof);
-        emitted += ws.h_counters[kCntSamples];
-        n_alive = ws.h_counters[next_slot];
-        cur = next;
-        ++iters;
-    }
+    ws.stats.n_rays_alive0 = ws.h_counters[kCntAlive0];
     ws.stats.n_samples = ws.h_counters[kCntComposited];
-    ws.stats.n_samples_emitted = emitted;
-    ws.stats.n_iterations = iters;
-    prof_begin(ctx, prof, INSTA_KERNEL_FINALIZE, stream);
-    hipLaunchKernelGGL(finalize_kernel, dim3((n_pix + kRayThreads - 1) / kRayThreads), dim3(kRayThreads), 0, stream,
-                       n_pix, ws.r_rgba, ws.r_depth, ws.r_steps, reinterpret_cast<float4*>(rgba), depth, steps);
-    INSTA_HIP(hipGetLastError());
-    prof_end(ctx, prof, INSTA_KERNEL_FINALIZE, n_pix, stream);
-    if (prof) {
-        INSTA_HIP(hipStreamSynchronize(stream));
-        prof_collect(ctx, prof);
-    }
+    ws.stats.n_samples_emitted = ws.h_counters[kCntEmitted];
+    ws.stats.n_iterations = ws.h_counters[kCntRounds];
     return INSTA_OK;
+}
+
+insta_status render(insta_ctx* ctx, RenderWS& ws, const insta_camera* cam, uint32_t width, uint32_t height,
+                    int32_t frame, int32_t occupancy, uint32_t spp, float* rgba, float* depth, uint32_t* steps,
+                    hipStream_t stream) {
+    insta_status st = enqueue_render(ctx, ws, cam, width, height, frame, occupancy, spp, 1, rgba, depth, steps, stream);
+    if (st != INSTA_OK) return st;
+    return finish_render(ctx, ws, cam, width, height, frame, occupancy, spp, 1, rgba, depth, steps, stream);
 }
 
 insta_status set_profiling(insta_ctx* ctx, bool on) {

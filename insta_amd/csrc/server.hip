@@ -15,9 +15,13 @@
 // device: their kernels run on different streams, while each worker's host
 // thread waits only on its own stream.  Each request's result depends only on
 // its own inputs (no shared scratch between slots), so it is the same alone
-// and in a mix.  The preparation chain is captured once per (worker, avatar)
-// into a hipGraph and replayed per request; the coefficients reach it
-// through a fixed device buffer.
+// and in a mix.  Each request's whole chain -- preparation, ray init, two
+// march/deform/network/composite rounds with live counts kept on the device,
+// image write -- is stream-captured and moved onto the worker's executable
+// graph for that avatar with hipGraphExecUpdate (same shape, new arguments:
+// camera, resolution, workspace offset, output), then launched as one graph;
+// the host synchronises once per request, and only rays still live after the
+// two rounds take further host-checked rounds.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -98,7 +102,8 @@ struct insta_server {
         insta::RenderWS ws;
         float* d_coeffs = nullptr;
         float* h_coeffs = nullptr;
-        std::vector<hipGraphExec_t> prep;   // per avatar, captured on first use
+        std::vector<hipGraphExec_t> graphs;  // per avatar: preparation + render of the last request
+        uint64_t graph_updates = 0, graph_builds = 0;
         std::thread thread;
     };
 
@@ -141,6 +146,10 @@ insta_status prepare(insta_server* S, insta_server::Slot& sl, const insta_server
     return launch_grid_linear(ctx, slot, stream);
 }
 
+// Render rounds queued with each request (live counts stay on the device);
+// rays still live after them continue on host-checked rounds.
+constexpr uint32_t kQueuedRounds = 2;
+
 insta_status run_request(insta_server* S, insta_server::Slot& sl, Pending& p, char* ws_base) {
     const insta_request& r = p.req;
     const insta_server::Avatar& av = S->avatars[r.avatar];
@@ -148,36 +157,57 @@ insta_status run_request(insta_server* S, insta_server::Slot& sl, Pending& p, ch
     hipStream_t stream = sl.stream;
     std::copy(p.coeffs.begin(), p.coeffs.end(), sl.h_coeffs);
     INSTA_HIP(hipMemcpyAsync(sl.d_coeffs, sl.h_coeffs, av.n_bases * sizeof(float), hipMemcpyHostToDevice, stream));
-    insta_status st;
-    if (S->cfg.use_graphs) {
-        hipGraphExec_t& exec = sl.prep[r.avatar];
-        if (!exec) {
-            hipGraph_t graph = nullptr;
-            INSTA_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-            st = prepare(S, sl, av, stream);
-            const hipError_t ce = hipStreamEndCapture(stream, &graph);
-            if (st != INSTA_OK) {
-                if (graph) (void)hipGraphDestroy(graph);
-                return st;
-            }
-            INSTA_HIP(ce);
-            const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            INSTA_HIP(ie);
-        }
-        INSTA_HIP(hipGraphLaunch(exec, stream));
-    } else {
-        st = prepare(S, sl, av, stream);
-        if (st != INSTA_OK) return st;
-    }
-    // host-side state the replayed kernels established
-    ctx->slot_built[sl.index + 1] = 1;
-    ctx->grid_built[sl.index + 1] = 1;
     bind_render_ws(sl.ws, ws_base, r.width * r.height, S->cfg.samples_per_pixel);
     const int32_t frame = (int32_t)sl.index;
-    st = render(ctx, sl.ws, &r.camera, r.width, r.height, frame, frame, 0, r.rgba_dev, nullptr, nullptr, stream);
+    auto enqueue = [&]() -> insta_status {
+        insta_status st = prepare(S, sl, av, stream);
+        if (st != INSTA_OK) return st;
+        return enqueue_render(ctx, sl.ws, &r.camera, r.width, r.height, frame, frame, 0, kQueuedRounds, r.rgba_dev,
+                              nullptr, nullptr, stream);
+    };
+    insta_status st;
+    if (S->cfg.use_graphs) {
+        // capture this request's chain and move the slot's executable graph
+        // for this avatar onto it (same topology: only arguments change)
+        hipGraph_t graph = nullptr;
+        INSTA_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        st = enqueue();
+        const hipError_t ce = hipStreamEndCapture(stream, &graph);
+        if (st != INSTA_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return st;
+        }
+        INSTA_HIP(ce);
+        hipGraphExec_t& exec = sl.graphs[r.avatar];
+        bool updated = false;
+        if (exec) {
+            hipGraphNode_t err_node = nullptr;
+            hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+            updated = hipGraphExecUpdate(exec, graph, &err_node, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+            if (!updated) {
+                (void)hipGetLastError();
+                (void)hipGraphExecDestroy(exec);
+                exec = nullptr;
+            }
+        }
+        hipError_t ie = hipSuccess;
+        if (!updated) ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        INSTA_HIP(ie);
+        ++(updated ? sl.graph_updates : sl.graph_builds);
+        INSTA_HIP(hipGraphLaunch(exec, stream));
+    } else {
+        st = enqueue();
+        if (st != INSTA_OK) return st;
+    }
+    // host-side state the queued kernels established
+    ctx->slot_built[sl.index + 1] = 1;
+    ctx->grid_built[sl.index + 1] = 1;
+    st = finish_render(ctx, sl.ws, &r.camera, r.width, r.height, frame, frame, 0, kQueuedRounds, r.rgba_dev, nullptr,
+                       nullptr, stream);
+    if (st != INSTA_OK) return st;
     INSTA_HIP(hipStreamSynchronize(stream));
-    return st;
+    return INSTA_OK;
 }
 
 void worker(insta_server* S, insta_server::Slot* sl) {
@@ -300,7 +330,7 @@ insta_status insta_server_add_avatar(insta_server* S, insta_ctx* ctx, const floa
         return INSTA_ERR_STATE;
     }
     S->avatars.push_back({ctx, bases_dev, n_bases});
-    for (auto& sl : S->slots) sl->prep.push_back(nullptr);
+    for (auto& sl : S->slots) sl->graphs.push_back(nullptr);
     *avatar_id = (uint32_t)S->avatars.size() - 1;
     return INSTA_OK;
 }
@@ -351,6 +381,20 @@ insta_status insta_server_poll(insta_server* S, insta_result* out, uint32_t max_
 
 double insta_server_clock(insta_server* S) { return S ? S->now() : 0.0; }
 
+insta_status insta_server_graph_stats(insta_server* S, uint64_t* updates, uint64_t* builds) {
+    if (!S || !updates || !builds) {
+        insta::set_error("server_graph_stats: bad argument");
+        return INSTA_ERR_INVALID;
+    }
+    std::lock_guard<std::mutex> lk(S->mu);
+    *updates = *builds = 0;
+    for (auto& sl : S->slots) {
+        *updates += sl->graph_updates;
+        *builds += sl->graph_builds;
+    }
+    return INSTA_OK;
+}
+
 void insta_server_destroy(insta_server* S) {
     if (!S) return;
     {
@@ -361,7 +405,7 @@ void insta_server_destroy(insta_server* S) {
     for (auto& sl : S->slots)
         if (sl->thread.joinable()) sl->thread.join();
     for (auto& sl : S->slots) {
-        for (hipGraphExec_t e : sl->prep)
+        for (hipGraphExec_t e : sl->graphs)
             if (e) (void)hipGraphExecDestroy(e);
         insta::release_render_ws(sl->ws);
         insta::dev_free(sl->d_coeffs);

@@ -91,6 +91,12 @@ def test_same_alone_and_in_mix(graphs):
         mix.append((ids[k % 3], c, cam, res))
     mix.insert(7, (ids[0], c0, cam0, 96))
     imgs, results = _serve(server, mix)
+    if graphs:
+        # each worker instantiates at most one graph per avatar; every other
+        # request moved that graph onto its own arguments in place
+        updates, builds = server.graph_stats()
+        assert updates + builds == len(mix) + 1
+        assert builds <= 4 * 3
     server.close()
     assert np.array_equal(imgs[7], alone)
     # the mix really overlapped: some request was admitted before another finished
@@ -115,3 +121,27 @@ def test_admission_respects_budget():
     with pytest.raises(InstaError):
         server.submit(aid, np.zeros(50, np.float32), reqs[0][2], 2048, 2048, big)
     server.close()
+
+
+@pytest.mark.parametrize("graphs", [1, 0])
+def test_rounds_beyond_the_queued_ones(graphs):
+    """A 1-sample-per-pixel buffer leaves K = capacity / live rays small, so
+    most rays need rounds after the two queued with the request (host-checked
+    continuation, then the image is rewritten); the image equals the direct
+    render, whose buffer is 128 times larger (fewer, longer rounds)."""
+    import torch
+    av, ctx, bases, _ = _avatar_ctx(6, n_frames=2)
+    rng = np.random.default_rng(11)
+    coeffs, cam = _request(rng, 512)
+    server = Server(default_server_config(n_slots=1, hbm_budget_bytes=1 << 30, samples_per_pixel=1,
+                                          use_graphs=graphs))
+    aid = server.add_avatar(ctx, bases)
+    (img,), results = _serve(server, [(aid, coeffs, cam, 512)])
+    server.close()
+    ctx.set_frame_expression(1, bases, torch.from_numpy(coeffs).cuda())
+    ctx.build_occupancy(1, 0.05)
+    rgba, _, _ = ctx.render(cam, 512, 512, 1, 1, 0)
+    stats = ctx.render_stats()
+    torch.cuda.synchronize()
+    assert results[0].n_samples == stats.n_samples
+    assert np.array_equal(img, rgba.cpu().numpy())
