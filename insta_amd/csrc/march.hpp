@@ -128,4 +128,28 @@ INSTA_DEV uint32_t wave_reserve(uint32_t* counter, uint32_t v) {
     return base + incl - v;
 }
 
+INSTA_DEV uint32_t wave_max(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor(v, o));
+    return v;
+}
+
+// Step-major order within a wave's sample range: sample j of lane l sits at
+// wave_base + (samples of steps < j over the wave's lanes) + (lanes below l
+// with more than j samples).  Lanes are adjacent rays, so neighbouring
+// samples are the same step of neighbouring pixels.  Every lane of the wave
+// calls this with the same j sequence (j < the wave's largest count).
+struct StepMajor {
+    uint32_t running = 0;
+    INSTA_DEV uint32_t next(uint32_t j, uint32_t count, bool& mine) {
+        const uint64_t act = __ballot(j < count);
+        const uint32_t lane = threadIdx.x & 63u;
+        const uint64_t below = act & ((1ull << lane) - 1ull);
+        const uint32_t slot = running + (uint32_t)__popcll(below);
+        running += (uint32_t)__popcll(act);
+        mine = j < count;
+        return slot;
+    }
+};
+
 }  // namespace insta

@@ -7,8 +7,12 @@
 //             ld jitter and empty-space skip; live rays appended to a list
 //   repeat    march: each live ray emits up to K samples (K = sample
 //                    capacity / live rays, computed on the device), one
-//                    walk with t to scratch, then written contiguously at a
-//                    wave-aggregated atomic offset
+//                    walk with t to scratch; a wave's samples take one
+//                    contiguous range (one atomic), step-major inside it:
+//                    step j of the wave's 64 adjacent rays side by side, so
+//                    the deform and network see neighbouring points in
+//                    neighbouring lanes (the reference's sample index is
+//                    likewise step-major, i + j * n_alive)
 //             deform: posed -> canonical per sample (deform.hip)
 //             network: encode + MLPs per sample (network.hip)
 //             composite: per ray over its samples; rays that neither hit
@@ -133,14 +137,20 @@ __global__ __launch_bounds__(kRayThreads) void march_kernel(RenderArgs a, const 
             }
         }
     }
-    const uint32_t base = wave_reserve(&counters[kCntSamples], count);
-    if (!valid) return;
-    for (uint32_t j = 0; j < count; ++j) {
-        const float tj = t_buf[(size_t)j * n_alive + k];
-        const V3 pos = o + d * tj;
-        s_pos[base + j] = make_float4(pos.x, pos.y, pos.z, __uint_as_float(r));
+    // the wave's samples are one contiguous range, written step-major
+    const uint32_t wave_base = __shfl(wave_reserve(&counters[kCntSamples], count), 0);
+    const uint32_t maxc = wave_max(count);
+    StepMajor sm;
+    for (uint32_t j = 0; j < maxc; ++j) {
+        bool mine;
+        const uint32_t slot = sm.next(j, count, mine);
+        if (mine) {
+            const V3 pos = o + d * t_buf[(size_t)j * n_alive + k];
+            s_pos[wave_base + slot] = make_float4(pos.x, pos.y, pos.z, __uint_as_float(r));
+        }
     }
-    r_off[k] = base;
+    if (!valid) return;
+    r_off[k] = wave_base;
     r_cnt[k] = count;
     r_o[r].w = t;
     r_flags[r] = exited ? 2u : 0u;
@@ -165,11 +175,15 @@ __global__ __launch_bounds__(kRayThreads) void composite_kernel(RenderArgs a, co
         counters[kCntRounds] += 1;
     }
     if ((k & ~63u) >= n_alive) return;
+    const bool valid = k < n_alive;
     bool keep = false;
     uint32_t r = 0, used = 0;
-    if (k < n_alive) {
+    // the march's step-major layout of this wave's samples
+    const uint32_t cnt = valid ? r_cnt[k] : 0u;
+    const uint32_t off = r_off[k & ~63u];  // the wave's base (lane 0 is live)
+    const uint32_t maxc = wave_max(cnt);
+    if (valid) {
         r = alive[k];
-        const uint32_t off = r_off[k], cnt = r_cnt[k];
         float4 acc = r_rgba[r];
         float4 rd = r_d[r];
         float depth = r_depth[r];
@@ -177,9 +191,13 @@ __global__ __launch_bounds__(kRayThreads) void composite_kernel(RenderArgs a, co
         const V3 cam_pos = v3(a.M[3], a.M[7], a.M[11]);
         const V3 cam_fwd = v3(a.M[2], a.M[6], a.M[10]);
         bool terminated = false;
-        for (uint32_t j = 0; j < cnt; ++j) {
-            const float4 out = s_out[off + j];
-            const float4 sp = s_pos[off + j];
+        StepMajor sm;
+        for (uint32_t j = 0; j < maxc; ++j) {
+            bool mine;
+            const uint32_t slot = sm.next(j, cnt, mine);
+            if (!mine || terminated) continue;
+            const float4 out = s_out[off + slot];
+            const float4 sp = s_pos[off + slot];
             // composite_kernel_nerf (testbed_nerf.cu:853-999), render mode Shade
             const float T = 1.0f - acc.w;
             const float sigma = expf(out.w + a.density_bias);
@@ -199,7 +217,6 @@ __global__ __launch_bounds__(kRayThreads) void composite_kernel(RenderArgs a, co
                 const float s = acc.w;
                 acc.x /= s; acc.y /= s; acc.z /= s; acc.w /= s;
                 terminated = true;
-                break;
             }
         }
         r_rgba[r] = acc;
