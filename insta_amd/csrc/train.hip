@@ -69,6 +69,7 @@ constexpr int kW = 64;
 constexpr int kDOut = 16;
 constexpr int kCin = 32;             // density outputs + 16 SH
 constexpr uint32_t kTrainThreads = 128;
+constexpr uint32_t kMlpThreads = 256;  // backward: one weight copy in LDS per block
 constexpr uint32_t kChunk = 1024;    // samples per weight-gradient partial
 constexpr uint32_t kRaySeedStride = 8;
 // backward: wave-level run sums on levels where one marching step covers at
@@ -391,7 +392,7 @@ INSTA_DEV float2 run_sum(uint32_t key, float2 v, bool& head) {
 // land on one table entry (coarse levels along one ray) are summed across the
 // wave first and added by one lane.
 template <int L, bool H16>
-__global__ __launch_bounds__(kTrainThreads) void train_backward_kernel(const float* __restrict__ params, GridParams gp,
+__global__ __launch_bounds__(kMlpThreads) void train_backward_kernel(const float* __restrict__ params, GridParams gp,
                                                                        const uint32_t* __restrict__ n_dev,
                                                                        const uint32_t* __restrict__ c_idx,
                                                                        const float4* __restrict__ cpos,
@@ -404,17 +405,20 @@ __global__ __launch_bounds__(kTrainThreads) void train_backward_kernel(const flo
     constexpr int E = 2 * L;
     using A = Act<E>;
     using O = Off<E>;
+    __shared__ float w[O::N];  // both MLPs' weights (broadcast reads)
+    for (uint32_t e = threadIdx.x; e < O::N; e += kMlpThreads) w[e] = params[e];
+    __syncthreads();
     const uint32_t n = *n_dev;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if ((i & ~63u) >= n) return;  // whole wave past the batch
     const bool valid = i < n;
     const uint32_t ic = valid ? i : n - 1u;  // tail lanes recompute the last sample and store nothing
     const float4 d4 = valid ? dout[ic] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    const float* W4 = params + O::W4;
-    const float* W3 = params + O::W3;
-    const float* W2 = params + O::W2;
-    const float* W1 = params + O::W1;
-    const float* W0 = params + O::W0;
+    const float* W4 = w + O::W4;
+    const float* W3 = w + O::W3;
+    const float* W2 = params + O::W2;  // scalar loads: from LDS its fully unrolled loop spills
+    const float* W1 = w + O::W1;
+    const float* W0 = w + O::W0;
     if (valid) {
         dact[(size_t)(Dact::DRGB + 0) * ld + i] = d4.x;
         dact[(size_t)(Dact::DRGB + 1) * ld + i] = d4.y;
@@ -424,7 +428,7 @@ __global__ __launch_bounds__(kTrainThreads) void train_backward_kernel(const flo
     float dh2[kW];
 #pragma unroll
     for (int k = 0; k < kW; ++k) dh2[k] = 0.0f;
-#pragma unroll
+#pragma unroll 1
     for (int m = 0; m < kW; ++m) {
         const float s = W4[m] * d4.x + W4[kW + m] * d4.y + W4[2 * kW + m] * d4.z;
         const float dh3 = act[(size_t)(A::H3 + m) * ld + ic] > 0.0f ? s : 0.0f;
@@ -452,7 +456,7 @@ __global__ __launch_bounds__(kTrainThreads) void train_backward_kernel(const flo
     float denc[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) denc[e] = 0.0f;
-#pragma unroll
+#pragma unroll 1
     for (int k = 0; k < kW; ++k) {
         float s = 0.0f;
 #pragma unroll
@@ -463,7 +467,7 @@ __global__ __launch_bounds__(kTrainThreads) void train_backward_kernel(const flo
         for (int e = 0; e < E; ++e) denc[e] = __builtin_fmaf(W0[k * E + e], dh1, denc[e]);
     }
     // dL/dencoding through LDS, so that the level loop indexes it at run time
-    __shared__ float s_denc[E][kTrainThreads];
+    __shared__ float s_denc[E][kMlpThreads];
 #pragma unroll
     for (int e = 0; e < E; ++e) s_denc[e][threadIdx.x] = denc[e];
     // hash-grid scatter (the corner/weight arithmetic of encode_level)
@@ -674,18 +678,18 @@ insta_status launch_train_backward(insta_ctx* ctx, TrainState* ts, const insta_t
     const uint32_t* n_dev = ts->counters + kTCompactN;
     const GridParams gp = grid_params(ctx);
     const uint32_t n_mlp = (uint32_t)ctx->layout.n_mlp_params;
-    const dim3 sgrid((cap + kTrainThreads - 1) / kTrainThreads);
+    const dim3 sgrid((cap + kMlpThreads - 1) / kMlpThreads);
     float* grad_tab = ts->grad + n_mlp;
     // levels whose cells span at least 1 / kRunMaxStepCells marching steps
     uint32_t run_mask = 0;
     for (int l = 0; l < L; ++l)
         if (ctx->cfg.dt * gp.scale[l] <= kRunMaxStepCells) run_mask |= 1u << l;
     if (tc.grid_grad_fp16)
-        hipLaunchKernelGGL((train_backward_kernel<L, true>), sgrid, dim3(kTrainThreads), 0, stream, ctx->d_params, gp,
+        hipLaunchKernelGGL((train_backward_kernel<L, true>), sgrid, dim3(kMlpThreads), 0, stream, ctx->d_params, gp,
                            n_dev, ts->c_idx, ts->s_cpos, ts->act, ts->c_dout, ts->dact, cap, grad_tab, ts->grad_h,
                            run_mask);
     else
-        hipLaunchKernelGGL((train_backward_kernel<L, false>), sgrid, dim3(kTrainThreads), 0, stream, ctx->d_params, gp,
+        hipLaunchKernelGGL((train_backward_kernel<L, false>), sgrid, dim3(kMlpThreads), 0, stream, ctx->d_params, gp,
                            n_dev, ts->c_idx, ts->s_cpos, ts->act, ts->c_dout, ts->dact, cap, grad_tab, ts->grad_h,
                            run_mask);
     INSTA_HIP(hipGetLastError());
