@@ -410,6 +410,17 @@ def main():
                     per_unit=f"{c['bytes']} B/sample streamed", traffic_source="profiles/render_full_512.json")
     kernels = {k: dict(ms_per_frame=v["ms"] / args.steps, launches=v["launches"], items=v["items"])
                for k, v in prof.items() if v["launches"]}
+    # nearest-triangle work per deformed sample: one more (untimed) frame with
+    # the counting build of the deformation kernel
+    ctx.set_profiling(True, count_traversal=True)
+    ctx.render(cams[0], res, res, 0, 0, 0, out)
+    tc = ctx.traversal_counts()
+    ctx.set_profiling(False)
+    q = max(tc["queries"], 1)
+    traversal = dict(queries=tc["queries"], inner_nodes_per_sample=tc["inner"] / q,
+                     child_boxes_per_sample=4 * tc["inner"] / q, leaves_per_sample=tc["leaves"] / q,
+                     triangles_per_sample=tc["triangles"] / q,
+                     nodes_per_sample=(tc["inner"] + tc["leaves"]) / q)
 
     # precision forms: same workload, other network arithmetic
     forms = {"fp32": dict(frames_per_s=args.steps / elapsed, samples_per_s=value,
@@ -456,6 +467,7 @@ def main():
         "precision_forms": forms,
         "roofline": roof,
         "kernels": kernels,
+        "deform_traversal": traversal,
         "cpu_baseline": cpu,
         "training": train,
         "serving": serve,

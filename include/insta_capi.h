@@ -407,8 +407,14 @@ typedef struct insta_profile {
     double ms[INSTA_KERNEL_CLASSES];         /* summed event time */
     uint64_t launches[INSTA_KERNEL_CLASSES];
     uint64_t items[INSTA_KERNEL_CLASSES];    /* rays or samples processed */
+    /* with enable == 2: nearest-triangle work of the sample deformation
+     * (render and training): queries, inner nodes expanded (4 child boxes
+     * each), leaves visited, triangle distances evaluated */
+    uint64_t bvh_queries, bvh_inner, bvh_leaves, bvh_triangles;
 } insta_profile;
-/* enable != 0 starts (and zeroes) accumulation; 0 stops it. */
+/* enable != 0 starts (and zeroes) accumulation; 0 stops it.  enable == 2 also
+ * counts traversal work, with a counting build of the deformation kernel
+ * (slower: for a separate, untimed render). */
 insta_status insta_set_profiling(insta_ctx* ctx, int enable);
 insta_status insta_get_profile(insta_ctx* ctx, insta_profile* out);
 

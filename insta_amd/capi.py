@@ -99,7 +99,9 @@ class InstaResult(C.Structure):
 
 
 class InstaProfile(C.Structure):
-    _fields_ = [("ms", C.c_double * 8), ("launches", C.c_uint64 * 8), ("items", C.c_uint64 * 8)]
+    _fields_ = [("ms", C.c_double * 8), ("launches", C.c_uint64 * 8), ("items", C.c_uint64 * 8),
+                ("bvh_queries", C.c_uint64), ("bvh_inner", C.c_uint64), ("bvh_leaves", C.c_uint64),
+                ("bvh_triangles", C.c_uint64)]
 
 
 # the C ABI: (name, restype, argtypes); tests check this list against the header
@@ -371,14 +373,21 @@ class Context:
                                     _ptr(depth), _ptr(steps), self.stream()), "insta_render")
         return out
 
-    def set_profiling(self, on: bool) -> None:
-        check(self.lib.insta_set_profiling(self.h, 1 if on else 0), "insta_set_profiling")
+    def set_profiling(self, on, count_traversal: bool = False) -> None:
+        level = (2 if count_traversal else 1) if on else 0
+        check(self.lib.insta_set_profiling(self.h, level), "insta_set_profiling")
 
     def profile(self) -> dict:
         p = InstaProfile()
         check(self.lib.insta_get_profile(self.h, C.byref(p)), "insta_get_profile")
         return {name: dict(ms=p.ms[i], launches=p.launches[i], items=p.items[i])
                 for i, name in enumerate(KERNEL_CLASSES)}
+
+    def traversal_counts(self) -> dict:
+        """Nearest-triangle work counted since set_profiling(True, count_traversal=True)."""
+        p = InstaProfile()
+        check(self.lib.insta_get_profile(self.h, C.byref(p)), "insta_get_profile")
+        return dict(queries=p.bvh_queries, inner=p.bvh_inner, leaves=p.bvh_leaves, triangles=p.bvh_triangles)
 
     # --- occupancy samples and density grid ---------------------------------
     def deform_positions(self, pos_dev, n_frames: int, seed: int, rng_offset: int = 0) -> None:

@@ -564,7 +564,11 @@ insta_status insta_get_render_stats(insta_ctx* ctx, insta_render_stats* out) {
 
 insta_status insta_set_profiling(insta_ctx* ctx, int enable) {
     INSTA_CHECK_CTX(ctx);
-    return insta::set_profiling(ctx, enable != 0);
+    if (enable < 0 || enable > 2) {
+        set_error("set_profiling: enable is 0, 1 or 2");
+        return INSTA_ERR_INVALID;
+    }
+    return insta::set_profiling(ctx, enable);
 }
 
 insta_status insta_get_profile(insta_ctx* ctx, insta_profile* out) {
@@ -574,6 +578,15 @@ insta_status insta_get_profile(insta_ctx* ctx, insta_profile* out) {
         return INSTA_ERR_INVALID;
     }
     *out = ctx->prof;
+    if (ctx->d_trav_counts) {
+        unsigned long long c[4];
+        INSTA_HIP(hipDeviceSynchronize());
+        INSTA_HIP(hipMemcpy(c, ctx->d_trav_counts, sizeof(c), hipMemcpyDeviceToHost));
+        out->bvh_queries = c[0];
+        out->bvh_inner = c[1];
+        out->bvh_leaves = c[2];
+        out->bvh_triangles = c[3];
+    }
     return INSTA_OK;
 }
 

@@ -178,8 +178,14 @@ INSTA_DEV uint32_t pop_live(const uint32_t* stack, int& sp, uint32_t stride, flo
     return kNoNode;
 }
 
-INSTA_DEV Hit bvh_closest(const MeshView& m, V3 p, uint32_t* stack, uint32_t stride,
-                          float max_dist_sq = 3.40282347e38f) {
+// traversal work of one query (counting builds only)
+struct TraversalCount {
+    uint32_t inner = 0, leaves = 0, triangles = 0;
+};
+
+template <bool kCount>
+INSTA_DEV Hit bvh_closest_impl(const MeshView& m, V3 p, uint32_t* stack, uint32_t stride, float max_dist_sq,
+                               TraversalCount* tc) {
     // max_dist_sq (bvh.cuh:48): only triangles closer than this are reported
     float best = max_dist_sq;
     int32_t best_idx = -1;
@@ -190,6 +196,7 @@ INSTA_DEV Hit bvh_closest(const MeshView& m, V3 p, uint32_t* stack, uint32_t str
     uint32_t node = 0;
     for (;;) {
         while (node < m.leaf_base) {
+            if constexpr (kCount) ++tc->inner;
             const uint32_t first = 4u * node + 1u;
             float cd[4];
             uint32_t ci[4];
@@ -225,6 +232,10 @@ INSTA_DEV Hit bvh_closest(const MeshView& m, V3 p, uint32_t* stack, uint32_t str
         const uint32_t j = node - m.leaf_base;
         const uint32_t b = (uint32_t)(((uint64_t)j * m.nf) >> shift);
         const uint32_t e = (uint32_t)(((uint64_t)(j + 1) * m.nf) >> shift);
+        if constexpr (kCount) {
+            ++tc->leaves;
+            tc->triangles += e - b;
+        }
         for (uint32_t t = b; t < e; ++t) {
             const float4 r0 = m.btris[3 * t + 0];
             const float4 r1 = m.btris[3 * t + 1];
@@ -246,6 +257,11 @@ INSTA_DEV Hit bvh_closest(const MeshView& m, V3 p, uint32_t* stack, uint32_t str
     h.orig = best_orig;
     h.dist_sq = best_idx < 0 ? 0.0f : best;
     return h;
+}
+
+INSTA_DEV Hit bvh_closest(const MeshView& m, V3 p, uint32_t* stack, uint32_t stride,
+                          float max_dist_sq = 3.40282347e38f) {
+    return bvh_closest_impl<false>(m, p, stack, stride, max_dist_sq, nullptr);
 }
 
 // Stack entries a query can hold: at most 3 pending siblings per inner level

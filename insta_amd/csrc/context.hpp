@@ -93,6 +93,7 @@ struct insta_ctx {
 
     // optional per-kernel-class event timing of insta_render
     bool profiling = false;
+    unsigned long long* d_trav_counts = nullptr;  // [4] while traversal counting is on
     hipEvent_t prof_ev[2 * INSTA_KERNEL_CLASSES] = {};
     bool prof_pending[INSTA_KERNEL_CLASSES] = {};
     uint64_t prof_pending_items[INSTA_KERNEL_CLASSES] = {};
@@ -194,7 +195,7 @@ insta_status finish_render(insta_ctx* ctx, RenderWS& ws, const insta_camera* cam
                            int32_t frame, int32_t occupancy, uint32_t spp, uint32_t rounds, float* rgba, float* depth,
                            uint32_t* steps, hipStream_t stream);
 void free_workspace(insta_ctx* ctx);
-insta_status set_profiling(insta_ctx* ctx, bool on);
+insta_status set_profiling(insta_ctx* ctx, int level);
 // training (train.hip)
 bool train_supported(const insta_config& cfg);
 void free_train(insta_ctx* ctx);

@@ -222,20 +222,37 @@ INSTA_DEV int32_t sample_frame(int32_t frame, const int32_t* __restrict__ ray_fr
     return ray_frame ? ray_frame[__float_as_uint(sp.w)] : frame;
 }
 
+// kCount: also sum the traversal work into counts[4] (queries, inner nodes,
+// leaves, triangles) -- a separate build for measurement runs
+template <bool kCount>
 __global__ __launch_bounds__(kDeformThreads) void deform_hits_kernel(MeshSet ms, int32_t frame,
                                                                      const int32_t* __restrict__ ray_frame,
                                                                      const uint32_t* __restrict__ n_dev,
                                                                      const float4* __restrict__ s_pos,
-                                                                     int32_t* __restrict__ s_hit) {
+                                                                     int32_t* __restrict__ s_hit,
+                                                                     unsigned long long* __restrict__ counts) {
     extern __shared__ uint32_t stack[];  // [stack_depth(k)][kDeformThreads]
     const uint32_t n = *n_dev;
+    TraversalCount tc;
+    uint32_t queries = 0;
     for (uint32_t i = blockIdx.x * kDeformThreads + threadIdx.x; i < n; i += gridDim.x * kDeformThreads) {
         const float4 sp = s_pos[i];
         const V3 p = v3(sp.x, sp.y, sp.z);
         const int32_t f = sample_frame(frame, ray_frame, sp);
-        s_hit[i] = f >= 0 && in_unit_box(p)
-                       ? bvh_closest(ms.view((uint32_t)f + 1), p, stack + threadIdx.x, kDeformThreads).idx
-                       : -1;
+        const bool q = f >= 0 && in_unit_box(p);
+        if constexpr (kCount) queries += q ? 1u : 0u;
+        s_hit[i] = q ? bvh_closest_impl<kCount>(ms.view((uint32_t)f + 1), p, stack + threadIdx.x, kDeformThreads,
+                                                 3.40282347e38f, &tc).idx
+                     : -1;
+    }
+    if constexpr (kCount) {
+        unsigned long long v[4] = {queries, tc.inner, tc.leaves, tc.triangles};
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) v[c] += __shfl_xor(v[c], o);
+            if ((threadIdx.x & 63u) == 0 && v[c]) atomicAdd(counts + c, v[c]);
+        }
     }
 }
 
@@ -311,8 +328,12 @@ insta_status launch_deform_samples(insta_ctx* ctx, int32_t frame, const int32_t*
     const MeshSet ms = mesh_set(ctx, ctx->d_built);
     if (frame >= 0 || ray_frame) {
         const size_t shmem = (size_t)stack_depth(ctx->bvh_k) * kDeformThreads * sizeof(uint32_t);
-        hipLaunchKernelGGL(deform_hits_kernel, dim3(blocks_for(n_max)), dim3(kDeformThreads), shmem, stream, ms, frame,
-                           ray_frame, n_dev, s_pos, s_hit);
+        if (ctx->d_trav_counts)
+            hipLaunchKernelGGL(deform_hits_kernel<true>, dim3(blocks_for(n_max)), dim3(kDeformThreads), shmem, stream,
+                               ms, frame, ray_frame, n_dev, s_pos, s_hit, ctx->d_trav_counts);
+        else
+            hipLaunchKernelGGL(deform_hits_kernel<false>, dim3(blocks_for(n_max)), dim3(kDeformThreads), shmem,
+                               stream, ms, frame, ray_frame, n_dev, s_pos, s_hit, nullptr);
         INSTA_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(deform_apply_kernel, dim3(blocks_for(n_max)), dim3(kDeformThreads), 0, stream, ms, frame,

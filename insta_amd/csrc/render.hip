@@ -595,7 +595,8 @@ insta_status render(insta_ctx* ctx, RenderWS& ws, const insta_camera* cam, uint3
     return finish_render(ctx, ws, cam, width, height, frame, occupancy, spp, 1, rgba, depth, steps, stream);
 }
 
-insta_status set_profiling(insta_ctx* ctx, bool on) {
+insta_status set_profiling(insta_ctx* ctx, int level) {
+    const bool on = level != 0;
     if (on && !ctx->prof_ev[0]) {
         for (int i = 0; i < 2 * INSTA_KERNEL_CLASSES; ++i) INSTA_HIP(hipEventCreate(&ctx->prof_ev[i]));
     }
@@ -604,10 +605,18 @@ insta_status set_profiling(insta_ctx* ctx, bool on) {
         ctx->prof = insta_profile{};
         for (int c = 0; c < INSTA_KERNEL_CLASSES; ++c) ctx->prof_pending[c] = false;
     }
+    if (level == 2 && !ctx->d_trav_counts) {
+        INSTA_HIP(hipMalloc(reinterpret_cast<void**>(&ctx->d_trav_counts), 4 * sizeof(unsigned long long)));
+        INSTA_HIP(hipMemset(ctx->d_trav_counts, 0, 4 * sizeof(unsigned long long)));
+    } else if (level != 2 && ctx->d_trav_counts) {
+        INSTA_HIP(hipDeviceSynchronize());  // queued counting kernels may still write
+        dev_free(ctx->d_trav_counts);
+    }
     return INSTA_OK;
 }
 
 void free_workspace(insta_ctx* ctx) {
+    dev_free(ctx->d_trav_counts);
     for (int i = 0; i < 2 * INSTA_KERNEL_CLASSES; ++i)
         if (ctx->prof_ev[i]) {
             (void)hipEventDestroy(ctx->prof_ev[i]);

@@ -296,6 +296,33 @@ def test_render_is_deterministic(head):
         assert torch.equal(x, y)
 
 
+def test_traversal_counting_build_renders_the_same(head):
+    """The counting build of the deformation kernel (profiling level 2) gives
+    the same image, and its counts obey the traversal's structure: one query
+    per deformed sample, at least k inner nodes (the first descent) and one
+    leaf per query, 1..4 triangles per leaf."""
+    torch = _torch()
+    ctx, _ = head
+    ctx.build_occupancy(1, 0.015)
+    cam = make_camera(synth.look_at_camera(np.array([0.2, -0.1, 1.0]), 0.7), synth.focal_from_fov(192, 30.0))
+    a = [t.clone() for t in ctx.render(cam, 192, 192, 1, 1)]
+    ctx.set_profiling(True, count_traversal=True)
+    b = ctx.render(cam, 192, 192, 1, 1)
+    st = ctx.render_stats()
+    tc = ctx.traversal_counts()
+    ctx.set_profiling(False)
+    torch.cuda.synchronize()
+    for x, y in zip(a, b):
+        assert torch.equal(x, y)
+    k = int(np.floor(np.log(9976) / np.log(4)))
+    assert 0 < tc["queries"] <= st.n_samples_emitted
+    assert tc["inner"] >= k * tc["queries"]
+    assert tc["leaves"] >= tc["queries"]
+    assert tc["leaves"] <= tc["triangles"] <= 4 * tc["leaves"]
+    # counting is off again: counters are no longer reported
+    assert ctx.traversal_counts()["queries"] == 0
+
+
 # ---------------------------------------------------------------- other shapes and precisions
 def test_network_fallback_shape_matches_oracle(small_avatar):
     """Two density hidden layers: not an MFMA shape, served by the scalar kernel."""
