@@ -390,6 +390,24 @@ insta_status insta_server_graph_stats(insta_server* srv, uint64_t* updates, uint
 /* Finishes every queued request, then releases the server. */
 void insta_server_destroy(insta_server* srv);
 
+/* Snapshot of a trained avatar: Testbed::save_snapshot / load_snapshot
+ * (instant-ngp/src/testbed.cu:2916-3012) for this path -- a msgpack map with
+ * the network configuration ("encoding", "network", "rgb_network",
+ * "dir_encoding", and "insta": every insta_config field) and "snapshot" with
+ * the reference's keys (version, n_params, params_type "float",
+ * params_binary in the insta_set_params order, density_grid_size,
+ * density_grid_binary as FP16 in Morton order -- empty if the density grid
+ * was never updated -- nerf.rgb.rays_per_batch, training_step, aabb,
+ * render_aabb) and, with include_optimizer_state, the Adam moments.  Meshes
+ * and the training set are not part of it (the reference reloads them from
+ * the dataset).  Loading needs a context of the same network shape (see
+ * insta_snapshot_config); the density grid's mean and bitfield are rebuilt
+ * into occupancy slot occupancy_slot (-1 canonical, f posed frame f). */
+insta_status insta_save_snapshot(insta_ctx* ctx, const char* path, int include_optimizer_state, void* stream);
+insta_status insta_load_snapshot(insta_ctx* ctx, const char* path, int32_t occupancy_slot, void* stream);
+/* The configuration stored in a snapshot (host only, no device work). */
+insta_status insta_snapshot_config(const char* path, insta_config* cfg);
+
 /* Per-kernel timing of insta_render, measured with HIP events recorded on the
  * render stream around each kernel class.  No counterpart in the reference
  * (whose only timers are EMA frame times, testbed.cu:1600, 2484-2512); it

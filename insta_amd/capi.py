@@ -128,6 +128,9 @@ SIGNATURES = [
     ("insta_render", C.c_int, [_VP, C.POINTER(InstaCamera), C.c_uint32, C.c_uint32, C.c_int32, C.c_int32,
                                C.c_uint32, _VP, _VP, _VP, _VP]),
     ("insta_get_render_stats", C.c_int, [_VP, C.POINTER(InstaRenderStats)]),
+    ("insta_save_snapshot", C.c_int, [_VP, C.c_char_p, C.c_int, _VP]),
+    ("insta_load_snapshot", C.c_int, [_VP, C.c_char_p, C.c_int32, _VP]),
+    ("insta_snapshot_config", C.c_int, [C.c_char_p, C.POINTER(InstaConfig)]),
     ("insta_set_profiling", C.c_int, [_VP, C.c_int]),
     ("insta_get_profile", C.c_int, [_VP, C.POINTER(InstaProfile)]),
     ("insta_set_training_set", C.c_int, [_VP, C.c_uint32, C.c_uint32, C.c_uint32, _VP, _VP, _VP, _VP]),
@@ -342,6 +345,15 @@ class Context:
         check(self.lib.insta_set_params(self.h, _ptr(params_dev), params_dev.numel(), self.stream()),
               "insta_set_params")
 
+    # --- snapshot -------------------------------------------------------------
+    def save_snapshot(self, path, include_optimizer: bool = False) -> None:
+        check(self.lib.insta_save_snapshot(self.h, str(path).encode(), 1 if include_optimizer else 0, self.stream()),
+              "insta_save_snapshot")
+
+    def load_snapshot(self, path, occupancy_slot: int = -1) -> None:
+        check(self.lib.insta_load_snapshot(self.h, str(path).encode(), occupancy_slot, self.stream()),
+              "insta_load_snapshot")
+
     def encode(self, pos_dev):
         n = pos_dev.shape[0]
         out = self.torch.empty((n, self.cfg.n_levels * self.cfg.n_features_per_level), dtype=self.torch.float32,
@@ -447,6 +459,13 @@ def default_train_config(**kw) -> InstaTrainConfig:
     for k, v in kw.items():
         setattr(tc, k, v)
     return tc
+
+
+def snapshot_config(path) -> InstaConfig:
+    """The configuration stored in a snapshot file (no device work)."""
+    cfg = InstaConfig()
+    check(load_library().insta_snapshot_config(str(path).encode(), C.byref(cfg)), "insta_snapshot_config")
+    return cfg
 
 
 def default_server_config(**kw) -> InstaServerConfig:

@@ -209,6 +209,23 @@ insta_status update_density_grid(insta_ctx* ctx, uint32_t slot, uint32_t n_frame
                                  float* grid_out, float* mean_out, hipStream_t stream);
 insta_status reset_density_grid(insta_ctx* ctx);
 bool train_has_set(const insta_ctx* ctx);
+// snapshot state (snapshot.hip): host copies of the training and density-grid state
+struct TrainSnapshot {
+    bool present = false;
+    uint32_t step = 0, rays_per_batch = 0;
+    uint64_t rays_total = 0;
+    std::vector<float> m, v;  // Adam moments, empty unless saved
+};
+struct DensitySnapshot {
+    std::vector<float> grid;  // [G^3] Morton order, empty if never populated
+    uint32_t step = 0;
+    uint64_t rng_state = 0, rng_inc = 0;
+};
+insta_status train_export(insta_ctx* ctx, bool with_optimizer, TrainSnapshot& out);
+insta_status train_import(insta_ctx* ctx, const TrainSnapshot& in, hipStream_t stream);
+insta_status density_export(insta_ctx* ctx, DensitySnapshot& out);
+// grid, then its mean and bitfield into occupancy slot `slot` (0 canonical, f + 1 frame f)
+insta_status density_import(insta_ctx* ctx, const DensitySnapshot& in, uint32_t slot, hipStream_t stream);
 insta_status get_gradients(insta_ctx* ctx, float* out, hipStream_t stream);
 insta_status reset_optimizer(insta_ctx* ctx);
 insta_status launch_encode(insta_ctx* ctx, const float* pos, uint32_t n, float* out, hipStream_t stream);

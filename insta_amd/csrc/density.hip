@@ -239,6 +239,66 @@ insta_status update_density_grid(insta_ctx* ctx, uint32_t slot, uint32_t n_frame
     return INSTA_OK;
 }
 
+insta_status density_export(insta_ctx* ctx, DensitySnapshot& out) {
+    out = DensitySnapshot{};
+    DensityGridState* d = ctx->density;
+    if (!d || d->step == 0) return INSTA_OK;  // never populated
+    const uint32_t G = ctx->cfg.grid_size, cells = G * G * G;
+    out.grid.resize(cells);
+    INSTA_HIP(hipDeviceSynchronize());
+    INSTA_HIP(hipMemcpy(out.grid.data(), d->grid, (size_t)cells * 4, hipMemcpyDeviceToHost));
+    out.step = d->step;
+    out.rng_state = d->rng.state;
+    out.rng_inc = d->rng.inc;
+    return INSTA_OK;
+}
+
+insta_status density_import(insta_ctx* ctx, const DensitySnapshot& in, uint32_t slot, hipStream_t stream) {
+    const uint32_t G = ctx->cfg.grid_size, cells = G * G * G;
+    if (in.grid.empty()) {
+        if (ctx->density) ctx->density->step = 0;
+        return INSTA_OK;
+    }
+    if (in.grid.size() != cells) {
+        set_error("density grid: size does not match the context's grid");
+        return INSTA_ERR_INVALID;
+    }
+    if (!ctx->density) {
+        DensityGridState* d = new (std::nothrow) DensityGridState();
+        if (!d) {
+            set_error("host allocation failed");
+            return INSTA_ERR_OOM;
+        }
+        ctx->density = d;
+        const uint32_t nb = (cells + kReduce - 1) / kReduce;
+        if (!dg_alloc(&d->grid, cells * 4) || !dg_alloc(&d->tmp, cells * 4) || !dg_alloc(&d->partial, nb * 4) ||
+            !dg_alloc(&d->mean, 4) ||
+            hipHostMalloc(reinterpret_cast<void**>(&d->h_mean), 4, hipHostMallocDefault) != hipSuccess) {
+            free_density_grid(ctx);
+            set_error("density grid allocation failed");
+            return INSTA_ERR_OOM;
+        }
+    }
+    DensityGridState* d = ctx->density;
+    INSTA_HIP(hipMemcpyAsync(d->grid, in.grid.data(), (size_t)cells * 4, hipMemcpyHostToDevice, stream));
+    d->step = in.step ? in.step : 1;
+    d->rng.state = in.rng_state;
+    d->rng.inc = in.rng_inc;
+    // mean and bitfield as after an update (update_density_grid_mean_and_bitfield)
+    const dim3 block(kThreads);
+    const uint32_t nb = (cells + kReduce - 1) / kReduce;
+    hipLaunchKernelGGL(dg_reduce_kernel, dim3(nb), block, 0, stream, d->grid, cells, d->partial);
+    hipLaunchKernelGGL(dg_mean_kernel, dim3(1), block, 0, stream, d->partial, nb, d->mean);
+    hipLaunchKernelGGL(dg_bitfield_kernel, dim3((cells / 8 + kThreads - 1) / kThreads), block, 0, stream, d->grid,
+                       d->mean, ctx->grid(slot), cells / 8);
+    INSTA_HIP(hipGetLastError());
+    insta_status st = launch_grid_linear(ctx, slot, stream);
+    if (st != INSTA_OK) return st;
+    INSTA_HIP(hipStreamSynchronize(stream));  // pageable source
+    ctx->grid_built[slot] = 1;
+    return INSTA_OK;
+}
+
 insta_status reset_density_grid(insta_ctx* ctx) {
     if (ctx->density) ctx->density->step = 0;
     return INSTA_OK;

@@ -728,8 +728,8 @@ void free_train(insta_ctx* ctx) {
     ctx->train = nullptr;
 }
 
-insta_status set_training_set(insta_ctx* ctx, uint32_t n_images, uint32_t width, uint32_t height, const float* rgba,
-                              const insta_camera* cams, const int32_t* frames, hipStream_t stream) {
+// gradients, optimizer moments and counters: created once per context
+static insta_status ensure_train_state(insta_ctx* ctx, hipStream_t stream) {
     if (!ctx->train) {
         TrainState* ts = new (std::nothrow) TrainState();
         if (!ts) {
@@ -752,6 +752,13 @@ insta_status set_training_set(insta_ctx* ctx, uint32_t n_images, uint32_t width,
         INSTA_HIP(hipMemsetAsync(ts->adam_m, 0, np * sizeof(float), stream));
         INSTA_HIP(hipMemsetAsync(ts->adam_v, 0, np * sizeof(float), stream));
     }
+    return INSTA_OK;
+}
+
+insta_status set_training_set(insta_ctx* ctx, uint32_t n_images, uint32_t width, uint32_t height, const float* rgba,
+                              const insta_camera* cams, const int32_t* frames, hipStream_t stream) {
+    insta_status st = ensure_train_state(ctx, stream);
+    if (st != INSTA_OK) return st;
     TrainState* ts = ctx->train;
     dev_free(ts->images);
     dev_free(ts->meta);
@@ -886,6 +893,45 @@ bool train_has_set(const insta_ctx* ctx) { return ctx->train && ctx->train->n_im
 insta_status get_gradients(insta_ctx* ctx, float* out, hipStream_t stream) {
     INSTA_HIP(hipMemcpyAsync(out, ctx->train->grad, ctx->layout.n_params * sizeof(float), hipMemcpyDeviceToDevice,
                              stream));
+    return INSTA_OK;
+}
+
+insta_status train_export(insta_ctx* ctx, bool with_optimizer, TrainSnapshot& out) {
+    out = TrainSnapshot{};
+    TrainState* ts = ctx->train;
+    if (!ts) return INSTA_OK;
+    out.present = true;
+    out.step = ts->step;
+    out.rays_per_batch = ts->rays_per_batch;
+    out.rays_total = ts->rays_total;
+    if (with_optimizer) {
+        const uint64_t np = ctx->layout.n_params;
+        out.m.resize(np);
+        out.v.resize(np);
+        INSTA_HIP(hipDeviceSynchronize());
+        INSTA_HIP(hipMemcpy(out.m.data(), ts->adam_m, np * sizeof(float), hipMemcpyDeviceToHost));
+        INSTA_HIP(hipMemcpy(out.v.data(), ts->adam_v, np * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    return INSTA_OK;
+}
+
+insta_status train_import(insta_ctx* ctx, const TrainSnapshot& in, hipStream_t stream) {
+    insta_status st = ensure_train_state(ctx, stream);
+    if (st != INSTA_OK) return st;
+    TrainState* ts = ctx->train;
+    const uint64_t np = ctx->layout.n_params;
+    ts->step = in.step;
+    ts->rays_per_batch = in.rays_per_batch;
+    ts->rays_total = in.rays_total;
+    if (in.m.size() == np && in.v.size() == np) {
+        INSTA_HIP(hipMemcpyAsync(ts->adam_m, in.m.data(), np * sizeof(float), hipMemcpyHostToDevice, stream));
+        INSTA_HIP(hipMemcpyAsync(ts->adam_v, in.v.data(), np * sizeof(float), hipMemcpyHostToDevice, stream));
+        INSTA_HIP(hipStreamSynchronize(stream));  // pageable sources
+    } else {
+        INSTA_HIP(hipMemsetAsync(ts->adam_m, 0, np * sizeof(float), stream));
+        INSTA_HIP(hipMemsetAsync(ts->adam_v, 0, np * sizeof(float), stream));
+        ts->step = 0;  // fresh moments restart the bias correction
+    }
     return INSTA_OK;
 }
 
