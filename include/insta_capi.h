@@ -77,6 +77,11 @@ typedef struct insta_config {
     uint32_t max_frames;
     uint32_t max_triangles;          /* <= INSTA_MAX_TRIANGLES */
     uint32_t max_vertices;
+    /* nearest-triangle BVH: 4^k count-balanced leaves of bvh_leaf_min ..
+     * 4 * bvh_leaf_min - 1 triangles (the reference's leaves hold 2,
+     * tiny_mesh.cu:28-32); a layout choice, the query result does not
+     * depend on it.  In [1, 64]; default 4 */
+    uint32_t bvh_leaf_min;
 } insta_config;
 
 #define INSTA_MAX_TRIANGLES 16384u

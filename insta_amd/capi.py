@@ -34,6 +34,7 @@ class InstaConfig(C.Structure):
         ("grid_size", C.c_uint32), ("dt", C.c_float), ("max_steps", C.c_uint32), ("min_transmittance", C.c_float),
         ("disable_interp", C.c_uint32), ("precision", C.c_uint32),
         ("max_frames", C.c_uint32), ("max_triangles", C.c_uint32), ("max_vertices", C.c_uint32),
+        ("bvh_leaf_min", C.c_uint32),
     ]
 
 
@@ -203,7 +204,7 @@ def make_config(**kw) -> InstaConfig:
                 density_hidden_layers=1, density_out=16, rgb_hidden_layers=2, sh_degree=4, n_extra_dims=0,
                 density_logit_bias=1.0, grid_size=128, dt=1.73205080757 / 1024.0, max_steps=1024,
                 min_transmittance=0.01, disable_interp=0, precision=PRECISION_FP32, max_frames=16,
-                max_triangles=MAX_TRIANGLES, max_vertices=1 << 16)
+                max_triangles=MAX_TRIANGLES, max_vertices=1 << 16, bvh_leaf_min=4)
     base.update(kw)
     for k, v in base.items():
         setattr(cfg, k, v)
@@ -214,7 +215,7 @@ def small_config(**kw) -> InstaConfig:
     """Small workload: L=8 F=2 T=2^14 N_min=16 b=1.5, 64^3 grid, step 1/256,
     at most 256 steps, MLP 16->64->16 / 32->64->64->3."""
     base = dict(n_levels=8, log2_hashmap_size=14, per_level_scale=1.5, grid_size=64, dt=1.0 / 256.0,
-                max_steps=256, max_frames=4, max_triangles=4096, max_vertices=4096)
+                max_steps=256, max_frames=4, max_triangles=4096, max_vertices=4096, bvh_leaf_min=1)
     base.update(kw)
     return make_config(**base)
 

@@ -29,9 +29,12 @@ __global__ void export_order_kernel(const float4* __restrict__ btris, uint32_t n
     if (i < nf) order[i] = __float_as_uint(btris[3 * i + 2].y);
 }
 
-uint32_t bvh_levels(uint32_t n) {
+// inner levels of the 4-ary tree: the largest k with 4^k * leaf_min <= n, so
+// the 4^k count-balanced leaves hold leaf_min .. 4 * leaf_min - 1 triangles
+// (k = 0 below leaf_min)
+uint32_t bvh_levels(uint32_t n, uint32_t leaf_min) {
     uint32_t k = 0;
-    while ((uint64_t(1) << (2 * (k + 1))) <= n) ++k;
+    while (((uint64_t)leaf_min << (2 * (k + 1))) <= n) ++k;
     return k;
 }
 
@@ -90,6 +93,7 @@ insta_status validate(const insta_config& c) {
     if (c.max_frames == 0 || c.max_frames > 4096) return bad("max_frames in [1, 4096]");
     if (c.max_triangles == 0 || c.max_triangles > INSTA_MAX_TRIANGLES) return bad("max_triangles in [1, 16384]");
     if (c.max_vertices == 0 || c.max_vertices > (1u << 24)) return bad("max_vertices in [1, 2^24]");
+    if (c.bvh_leaf_min == 0 || c.bvh_leaf_min > 64) return bad("bvh_leaf_min in [1, 64]");
     if (c.precision > INSTA_PRECISION_FP32_SCALAR) return bad("unknown precision");
     return INSTA_OK;
 }
@@ -167,6 +171,7 @@ void insta_default_config(insta_config* c) {
     c->max_frames = 16;
     c->max_triangles = INSTA_MAX_TRIANGLES;
     c->max_vertices = 1u << 16;
+    c->bvh_leaf_min = 4;
 }
 
 insta_status insta_query_layout(const insta_config* cfg, insta_layout* out) {
@@ -200,7 +205,7 @@ insta_status insta_create(const insta_config* cfg, insta_ctx** out) {
         set_error("no HIP device");
         return INSTA_ERR_HIP;
     }
-    const uint32_t kmax = insta::bvh_levels(cfg->max_triangles);
+    const uint32_t kmax = insta::bvh_levels(cfg->max_triangles, 1);  // the deepest tree any leaf size needs
     ctx->node_cap = ((1u << (2 * (kmax + 1))) - 1) / 3;
     ctx->n_slots = cfg->max_frames + 1;
     ctx->slot_built.assign(ctx->n_slots, 0);
@@ -317,7 +322,7 @@ insta_status insta_set_canonical_mesh(insta_ctx* ctx, const float* verts, uint32
     }
     ctx->nv = nv;
     ctx->nf = nf;
-    ctx->bvh_k = insta::bvh_levels(nf);
+    ctx->bvh_k = insta::bvh_levels(nf, ctx->cfg.bvh_leaf_min);
     ctx->leaf_base = ((1u << (2 * ctx->bvh_k)) - 1) / 3;
     ctx->n_nodes = ctx->leaf_base + (1u << (2 * ctx->bvh_k));
     insta_status st = insta::mark_built(ctx, 0, ctx->n_slots, 0, stream);

@@ -265,7 +265,7 @@ INSTA_DEV Hit bvh_closest(const MeshView& m, V3 p, uint32_t* stack, uint32_t str
 }
 
 // Stack entries a query can hold: at most 3 pending siblings per inner level
-// of a tree with k inner levels (k <= 7 for 16384 triangles).  Kernels keep
+// of a tree with k inner levels (k <= 6 for 16384 triangles).  Kernels keep
 // the stack in dynamic LDS sized by this, so shallower trees leave room for
 // more resident waves.
 INSTA_HD uint32_t stack_depth(uint32_t k) { return k ? 3u * k : 1u; }

@@ -249,7 +249,7 @@ const CfgField kFields[] = {
     INSTA_U(sh_degree, true), INSTA_U(n_extra_dims, true), INSTA_F(density_logit_bias, false),
     INSTA_U(grid_size, true), INSTA_F(dt, false), INSTA_U(max_steps, false), INSTA_F(min_transmittance, false),
     INSTA_U(disable_interp, false), INSTA_U(precision, false), INSTA_U(max_frames, false),
-    INSTA_U(max_triangles, false), INSTA_U(max_vertices, false),
+    INSTA_U(max_triangles, false), INSTA_U(max_vertices, false), INSTA_U(bvh_leaf_min, false),
 };
 #undef INSTA_U
 #undef INSTA_F

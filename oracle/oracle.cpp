@@ -163,9 +163,11 @@ struct Bvh {
     std::vector<uint32_t> orig;     // BVH order -> original face id
 };
 
-uint32_t bvh_levels(uint32_t n) {
+// inner levels of the 4-ary tree: the largest k with 4^k * leaf_min <= n, so
+// the 4^k count-balanced leaves hold leaf_min .. 4 * leaf_min - 1 triangles
+uint32_t bvh_levels(uint32_t n, uint32_t leaf_min) {
     uint32_t k = 0;
-    while ((uint64_t(1) << (2 * (k + 1))) <= n) ++k;
+    while (((uint64_t)leaf_min << (2 * (k + 1))) <= n) ++k;
     return k;
 }
 
@@ -238,10 +240,10 @@ void bvh_order(const std::vector<Tri>& tris, std::vector<uint32_t>& order, uint3
     order = list[0];
 }
 
-void bvh_build(Bvh& b, const std::vector<Tri>& tris) {
+void bvh_build(Bvh& b, const std::vector<Tri>& tris, uint32_t leaf_min) {
     const uint32_t n = (uint32_t)tris.size();
     b.n = n;
-    b.k = bvh_levels(n);
+    b.k = bvh_levels(n, leaf_min);
     const uint32_t n_leaves = 1u << (2 * b.k);
     b.leaf_base = (n_leaves - 1) / 3;
     b.n_nodes = b.leaf_base + n_leaves;
@@ -603,7 +605,7 @@ void fill_frame(orc_scene* s, MeshFrame& mf, const float* verts, const MeshFrame
             mf.proj_d[12 * (size_t)i + k] = P[k];
         }
     }
-    bvh_build(mf.bvh, mf.tris);
+    bvh_build(mf.bvh, mf.tris, s->cfg.bvh_leaf_min);
     mf.built = true;
 }
 

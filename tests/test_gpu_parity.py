@@ -300,7 +300,7 @@ def test_traversal_counting_build_renders_the_same(head):
     """The counting build of the deformation kernel (profiling level 2) gives
     the same image, and its counts obey the traversal's structure: one query
     per deformed sample, at least k inner nodes (the first descent) and one
-    leaf per query, 1..4 triangles per leaf."""
+    leaf per query, 4..15 triangles per leaf."""
     torch = _torch()
     ctx, _ = head
     ctx.build_occupancy(1, 0.015)
@@ -314,11 +314,11 @@ def test_traversal_counting_build_renders_the_same(head):
     torch.cuda.synchronize()
     for x, y in zip(a, b):
         assert torch.equal(x, y)
-    k = int(np.floor(np.log(9976) / np.log(4)))
+    k = int(np.floor(np.log(9976) / np.log(4))) - 1   # leaves of 4..15 triangles
     assert 0 < tc["queries"] <= st.n_samples_emitted
     assert tc["inner"] >= k * tc["queries"]
     assert tc["leaves"] >= tc["queries"]
-    assert tc["leaves"] <= tc["triangles"] <= 4 * tc["leaves"]
+    assert 4 * tc["leaves"] <= tc["triangles"] <= 15 * tc["leaves"]
     # counting is off again: counters are no longer reported
     assert ctx.traversal_counts()["queries"] == 0
 

@@ -88,7 +88,8 @@ def test_bvh_invariants(small_oracle, small_avatar):
     # leaves partition [0, nf) in order
     lb, le = begin[leaf], end[leaf]
     assert lb[0] == 0 and le[-1] == nf and np.array_equal(lb[1:], le[:-1])
-    assert ((le - lb) >= 1).all() and ((le - lb) <= 4).all()
+    s = small_oracle.cfg.bvh_leaf_min
+    assert ((le - lb) >= s).all() and ((le - lb) <= 4 * s - 1).all()
     # inner nodes enclose their four children
     for i in np.nonzero(~leaf)[0]:
         ch = 4 * i + 1 + np.arange(4)
