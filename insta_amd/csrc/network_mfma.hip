@@ -230,6 +230,28 @@ INSTA_DEV float2 encode_level2(const TT* __restrict__ table, const GridParams& g
 // ---------------------------------------------------------------------------
 constexpr int kF32Stride = 33;  // floats per sample in LDS: 32 features + 1 pad
 
+// XCD-aware tile order for the persistent grid.  Workgroups are dispatched
+// round-robin over the 8 XCDs (block b on XCD b % 8), each with its own L2:
+// the blocks of one XCD walk one contiguous eighth of the samples, so an
+// XCD's L2 holds the table entries of one region of the image rather than
+// of all of it.  Grids that are not a multiple of 8 blocks stride plainly.
+struct XcdTiles {
+    uint32_t tile, end, step;
+    INSTA_DEV explicit XcdTiles(uint32_t n) {
+        const uint32_t tiles = (n + kTile - 1) / kTile;
+        if (gridDim.x % 8u == 0) {
+            const uint32_t xcd = blockIdx.x % 8u, chunk = (tiles + 7u) / 8u;
+            tile = xcd * chunk + blockIdx.x / 8u;
+            end = min(tiles, (xcd + 1u) * chunk);
+            step = gridDim.x / 8u;
+        } else {
+            tile = blockIdx.x;
+            end = tiles;
+            step = gridDim.x;
+        }
+    }
+};
+
 template <int L, class Src>
 __global__ __launch_bounds__(256) void network_f32_kernel(const float* __restrict__ table, GridParams gp,
                                                           const float* __restrict__ frag, Src src) {
@@ -239,7 +261,8 @@ __global__ __launch_bounds__(256) void network_f32_kernel(const float* __restric
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t n = src.count();
     const uint32_t g = lane >> 4, col = lane & 15u;
-    for (uint32_t base = blockIdx.x * kTile; base < n; base += gridDim.x * kTile) {
+    for (XcdTiles it(n); it.tile < it.end; it.tile += it.step) {
+        const uint32_t base = it.tile * kTile;
         __syncthreads();
         {
             const uint32_t i = base + tid;
@@ -370,7 +393,8 @@ __global__ __launch_bounds__(256) void network_x32_kernel(const _Float16* __rest
 #pragma unroll
     for (uint32_t f = 0; f < kFragX32; ++f) A[f] = *reinterpret_cast<const V8*>(frag + ((size_t)f * 64 + lane) * 8);
     const uint32_t n = src.count();
-    for (uint32_t base = blockIdx.x * kTile; base < n; base += gridDim.x * kTile) {
+    for (XcdTiles it(n); it.tile < it.end; it.tile += it.step) {
+        const uint32_t base = it.tile * kTile;
         __syncthreads();
         {
             const uint32_t i = base + tid;
@@ -433,7 +457,8 @@ __global__ __launch_bounds__(256) void network_x32_kernel(const _Float16* __rest
 uint32_t grid_for(uint32_t n_max) {
     const uint32_t tiles = (n_max + kTile - 1) / kTile;
     const uint32_t cap = 256 * 8;
-    return tiles < cap ? (tiles ? tiles : 1) : cap;
+    if (tiles >= cap) return cap;
+    return tiles >= 8 ? (tiles + 7u) / 8u * 8u : (tiles ? tiles : 1);  // whole XCD rounds when possible
 }
 
 }  // namespace

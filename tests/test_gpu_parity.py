@@ -296,6 +296,31 @@ def test_render_is_deterministic(head):
         assert torch.equal(x, y)
 
 
+def test_results_do_not_depend_on_the_bvh_leaf_size(head_avatar):
+    """The nearest-triangle order (dist^2, original id) makes ids, occupancy
+    and renders independent of the tree layout: leaves of 1..3 and 8..31
+    triangles give identical outputs."""
+    torch = _torch()
+    from insta_amd.capi import Context
+    rng = np.random.default_rng(77)
+    pts = torch.from_numpy(_points(rng, 100000, 0.3, 0.7)).cuda()
+    cam = make_camera(synth.look_at_camera(np.array([0.4, 0.1, 1.0]), 0.7), synth.focal_from_fov(128, 30.0))
+    out = []
+    for leaf in (1, 8):
+        cfg = full_config(max_frames=2, bvh_leaf_min=leaf)
+        ctx = Context(cfg)
+        ctx.set_canonical_mesh(head_avatar.canonical, head_avatar.faces, head_avatar.masks)
+        ctx.set_frames(0, torch.from_numpy(np.stack(head_avatar.frames[:2])).cuda())
+        ctx.set_params(torch.from_numpy(synth.network_params(7, ctx.layout, cfg)).cuda())
+        occ = ctx.build_occupancy(1, 0.015)
+        ids, dist = ctx.closest(1, pts)
+        rgba, _, steps = ctx.render(cam, 128, 128, 1, 1)
+        torch.cuda.synchronize()
+        out.append([t.cpu().numpy() for t in (occ, ids, dist, rgba, steps)])
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
 def test_traversal_counting_build_renders_the_same(head):
     """The counting build of the deformation kernel (profiling level 2) gives
     the same image, and its counts obey the traversal's structure: one query
