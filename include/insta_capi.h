@@ -432,7 +432,8 @@ typedef struct insta_profile {
     uint64_t items[INSTA_KERNEL_CLASSES];    /* rays or samples processed */
     /* with enable == 2: nearest-triangle work of the sample deformation
      * (render and training): queries, inner nodes expanded (4 child boxes
-     * each), leaves visited, triangle distances evaluated */
+     * each), leaves visited, exact triangle distances evaluated (those whose
+     * own box passed the bound) */
     uint64_t bvh_queries, bvh_inner, bvh_leaves, bvh_triangles;
 } insta_profile;
 /* enable != 0 starts (and zeroes) accumulation; 0 stops it.  enable == 2 also

@@ -232,20 +232,27 @@ INSTA_DEV Hit bvh_closest_impl(const MeshView& m, V3 p, uint32_t* stack, uint32_
         const uint32_t j = node - m.leaf_base;
         const uint32_t b = (uint32_t)(((uint64_t)j * m.nf) >> shift);
         const uint32_t e = (uint32_t)(((uint64_t)(j + 1) * m.nf) >> shift);
-        if constexpr (kCount) {
-            ++tc->leaves;
-            tc->triangles += e - b;
-        }
+        if constexpr (kCount) ++tc->leaves;
         for (uint32_t t = b; t < e; ++t) {
             const float4 r0 = m.btris[3 * t + 0];
             const float4 r1 = m.btris[3 * t + 1];
             const float4 r2 = m.btris[3 * t + 2];
-            const float d = tri_distance_sq(v3(r0.x, r0.y, r0.z), v3(r0.w, r1.x, r1.y), v3(r1.z, r1.w, r2.x), p);
+            const V3 va = v3(r0.x, r0.y, r0.z), vb = v3(r0.w, r1.x, r1.y), vc = v3(r1.z, r1.w, r2.x);
+            const float4 lo = make_float4(fminf(fminf(va.x, vb.x), vc.x), fminf(fminf(va.y, vb.y), vc.y),
+                                          fminf(fminf(va.z, vb.z), vc.z), 0.0f);
+            const float4 hi = make_float4(fmaxf(fmaxf(va.x, vb.x), vc.x), fmaxf(fmaxf(va.y, vb.y), vc.y),
+                                          fmaxf(fmaxf(va.z, vb.z), vc.z), 0.0f);
+            // the triangle's own box first: cheap, and conservative like the
+            // node test (bound carries the same rounding margin)
+            if (box_distance_sq(lo, hi, p) > bound) continue;
+            if constexpr (kCount) ++tc->triangles;
+            const float d = tri_distance_sq(va, vb, vc, p);
             const uint32_t o = __float_as_uint(r2.y);
             if (d < best || (d == best && o < best_orig)) {
                 best = d;
                 best_idx = (int32_t)t;
                 best_orig = o;
+                bound = prune_bound(best);
             }
         }
         bound = prune_bound(best);

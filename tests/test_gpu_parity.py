@@ -325,7 +325,9 @@ def test_traversal_counting_build_renders_the_same(head):
     """The counting build of the deformation kernel (profiling level 2) gives
     the same image, and its counts obey the traversal's structure: one query
     per deformed sample, at least k inner nodes (the first descent) and one
-    leaf per query, 4..15 triangles per leaf."""
+    leaf per query, at most 15 exact triangle distances per leaf (a
+    triangle whose own box is beyond the bound is skipped) and at least one
+    per query."""
     torch = _torch()
     ctx, _ = head
     ctx.build_occupancy(1, 0.015)
@@ -343,7 +345,7 @@ def test_traversal_counting_build_renders_the_same(head):
     assert 0 < tc["queries"] <= st.n_samples_emitted
     assert tc["inner"] >= k * tc["queries"]
     assert tc["leaves"] >= tc["queries"]
-    assert 4 * tc["leaves"] <= tc["triangles"] <= 15 * tc["leaves"]
+    assert tc["queries"] <= tc["triangles"] <= 15 * tc["leaves"]
     # counting is off again: counters are no longer reported
     assert ctx.traversal_counts()["queries"] == 0
 
