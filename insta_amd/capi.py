@@ -215,7 +215,7 @@ def small_config(**kw) -> InstaConfig:
     """Small workload: L=8 F=2 T=2^14 N_min=16 b=1.5, 64^3 grid, step 1/256,
     at most 256 steps, MLP 16->64->16 / 32->64->64->3."""
     base = dict(n_levels=8, log2_hashmap_size=14, per_level_scale=1.5, grid_size=64, dt=1.0 / 256.0,
-                max_steps=256, max_frames=4, max_triangles=4096, max_vertices=4096, bvh_leaf_min=1)
+                max_steps=256, max_frames=4, max_triangles=4096, max_vertices=4096)
     base.update(kw)
     return make_config(**base)
 
