@@ -395,7 +395,8 @@ def main():
     prof_json = ROOT / "profiles" / "render_full_512.json"
     if prof_json.exists():
         kn = json.loads(prof_json.read_text())["kernels"]
-        names = [k for k in kn if k.startswith(PROFILE_KERNELS.get(dom, "-"))]
+        # (the "<true>" build is the traversal-counting one of an untimed frame)
+        names = [k for k in kn if k.startswith(PROFILE_KERNELS.get(dom, "-")) and "<true>" not in k]
         if names:
             traffic = sum(kn[k]["fetch_bytes"] + kn[k]["write_bytes"] for k in names)
     if dom == "network":
