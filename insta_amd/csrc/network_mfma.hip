@@ -380,7 +380,7 @@ INSTA_DEV typename Vec8<T>::type pack_relu(f4 a, f4 b) {
 }
 
 template <int L, typename T, class Src>
-__global__ __launch_bounds__(256) void network_x32_kernel(const _Float16* __restrict__ table, GridParams gp,
+__global__ __launch_bounds__(256, 4) void network_x32_kernel(const _Float16* __restrict__ table, GridParams gp,
                                                           const T* __restrict__ frag, Src src) {
     typedef typename Vec8<T>::type V8;
     constexpr int DIN = L * 2;
