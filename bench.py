@@ -151,21 +151,29 @@ def cpu_baseline(cfg, av, params, cam, res, budget_s):
                        f"{samples} samples in {secs:.1f} s", frames_per_s=frames / secs)
 
 
-def training_bench(args):
+def training_bench(args, full=False):
     """Training step at the small shapes (L=8, T=2^14, 64^3 grid, 2,048-triangle
-    sphere): rays uniform over 64 posed frames seen by 64 random 512x512
-    cameras with smooth-noise targets, a 2^18-sample training batch per step
-    (adaptive ray count; marched samples are evaluated, the composited ones
-    compacted into the batch), Adam.  FP32 and FP16 grid-gradient accumulation (loss scale
-    128); gradient error of one batch against the FP64 oracle."""
+    sphere), or with full=True at the full-size shapes (L=16, T=2^19, 128^3
+    grid, 9,976-triangle head, <= 1,024 steps): rays uniform over 64 posed
+    frames seen by 64 random 512x512 cameras with smooth-noise targets, a
+    2^18-sample training batch per step (adaptive ray count; marched samples
+    are evaluated, the composited ones compacted into the batch), Adam.  FP32
+    and FP16 grid-gradient accumulation (loss scale 128); gradient error of one
+    batch against the FP64 oracle (4,096 rays; 1,024 at full size)."""
     import torch
     from insta_amd import synth
-    from insta_amd.capi import Context, default_train_config, make_camera, small_config
+    from insta_amd.capi import Context, default_train_config, full_config, make_camera, small_config
     n_frames, res = 64, 512
-    cfg = small_config(max_frames=n_frames)
-    av = synth.small_sphere_avatar(seed=0, n_frames=n_frames)
+    if full:
+        cfg = full_config(max_frames=n_frames)
+        av = synth.head_avatar(seed=0, n_frames=n_frames)
+        radius, thr, n_check = 0.7, 0.015, 1024
+    else:
+        cfg = small_config(max_frames=n_frames)
+        av = synth.small_sphere_avatar(seed=0, n_frames=n_frames)
+        radius, thr, n_check = 1.0, 0.05, 4096
     rng = np.random.default_rng(21)
-    cams = [make_camera(synth.random_camera(rng, 1.0, front_bias=False), synth.focal_from_fov(res, 30.0))
+    cams = [make_camera(synth.random_camera(rng, radius, front_bias=False), synth.focal_from_fov(res, 30.0))
             for _ in range(n_frames)]
     images = synth.smooth_noise_images(5, n_frames, res, res)
     images_dev = torch.from_numpy(images).cuda()
@@ -178,7 +186,7 @@ def training_bench(args):
         params = synth.network_params(7, ctx.layout, cfg)
         ctx.set_params(torch.from_numpy(params).cuda())
         for f in range(n_frames):
-            ctx.build_occupancy(f, 0.05)
+            ctx.build_occupancy(f, thr)
         ctx.set_training_set(images_dev, cams, list(range(n_frames)))
         return ctx, params
 
@@ -209,7 +217,7 @@ def training_bench(args):
     # gradient error of one batch against the FP64 oracle (same rays and samples;
     # a fresh context per variant so both see the batch of step 0)
     from oracle import oracle as O
-    n_rays = 4096
+    n_rays = n_check
     grads = {}
     for name, kw in (("fp32", {}), ("fp16_grid", dict(grid_grad_fp16=1, loss_scale=128.0))):
         ctx, params = make_ctx()
@@ -222,7 +230,7 @@ def training_bench(args):
     orc.set_canonical_mesh(av.canonical, av.faces, av.masks)
     for f in range(n_frames):
         orc.set_frame(f, av.frames[f])
-        orc.build_occupancy(f, 0.05)
+        orc.build_occupancy(f, thr)
     orc.set_params(params)
     O.set_training_set(orc, images, cams, list(range(n_frames)))
     g64, _, counts = O.train_grads(orc, n_rays, 17)
@@ -233,14 +241,16 @@ def training_bench(args):
     out["grad_rel_l2_vs_fp64"] = {"batch": f"{n_rays} rays, {counts[1]} samples"}
     for name, g in grads.items():
         out["grad_rel_l2_vs_fp64"][name] = {n: rel(g[b0:b1], g64[b0:b1]) for n, b0, b1 in zip(names, b[:-1], b[1:])}
-    out["workload"] = ("small shapes (L8 T2^14, 64^3 grid), 64 posed frames x 512x512 random cameras, "
-                       "~2^18 samples/step, Adam(0.9, 0.99, lr 1e-2)")
+    shapes = ("full-size shapes (L16 T2^19, 128^3 grid, 9,976-tri head, <=1024 steps)" if full
+              else "small shapes (L8 T2^14, 64^3 grid, 2,048-tri sphere)")
+    out["workload"] = f"{shapes}, 64 posed frames x 512x512 random cameras, ~2^18 samples/step, Adam(0.9, 0.99, lr 1e-2)"
     return out
 
 
-def serving_bench(args):
-    """Serving form: 8 resident avatars at the small shapes (different seeds for
-    mesh, bases, MLP and tables), FP16 MFMA network, Poisson arrivals at several
+def serving_bench(args, full=False):
+    """Serving form: 8 resident avatars at the small shapes, or with full=True
+    at the full-size shapes (9,976-triangle heads, L16 T2^19 tables, 128^3
+    grids; different seeds for mesh, bases, MLP and tables), FP16 MFMA network, Poisson arrivals at several
     rates; each request = avatar, 50 expression coefficients (device BVH
     rebuild + occupancy), random camera, resolution 128/256/512/1024^2 with
     weights 0.3/0.3/0.3/0.1.  Against sequential plain-FP32 rendering: the
@@ -249,15 +259,20 @@ def serving_bench(args):
     import torch
     from insta_amd import synth
     from insta_amd.capi import (PRECISION_FP16, PRECISION_FP32_SCALAR, Context, Server, default_server_config,
-                                make_camera, small_config)
+                                full_config, make_camera, small_config)
     n_av, n_slots = 8, args.serve_slots
+    radius, thr = (0.7, 0.015) if full else (1.0, 0.05)
     resolutions, weights = np.array([128, 256, 512, 1024]), np.array([0.3, 0.3, 0.3, 0.1])
 
     def avatars(precision):
         out = []
         for a in range(n_av):
-            av = synth.small_sphere_avatar(seed=300 + a, n_frames=0, n_bases=50)
-            cfg = small_config(max_frames=max(n_slots, 1), precision=precision)
+            if full:
+                av = synth.head_avatar(seed=300 + a, n_frames=0, n_bases=50)
+                cfg = full_config(max_frames=max(n_slots, 1), precision=precision)
+            else:
+                av = synth.small_sphere_avatar(seed=300 + a, n_frames=0, n_bases=50)
+                cfg = small_config(max_frames=max(n_slots, 1), precision=precision)
             ctx = Context(cfg)
             ctx.set_canonical_mesh(av.canonical, av.faces, av.masks)
             ctx.set_params(torch.from_numpy(synth.network_params(400 + a, ctx.layout, cfg)).cuda())
@@ -270,7 +285,7 @@ def serving_bench(args):
         reqs = []
         for _ in range(n):
             res = int(rng.choice(resolutions, p=weights))
-            cam = make_camera(synth.random_camera(rng, 1.0, front_bias=False), synth.focal_from_fov(res, 30.0))
+            cam = make_camera(synth.random_camera(rng, radius, front_bias=False), synth.focal_from_fov(res, 30.0))
             reqs.append((int(rng.integers(n_av)), rng.normal(0.0, 1.0, 50).astype(np.float32), cam, res))
         return reqs
 
@@ -278,7 +293,8 @@ def serving_bench(args):
         return float(np.percentile(np.asarray(x) * 1e3, q)) if len(x) else None
 
     fp16 = avatars(PRECISION_FP16)
-    server = Server(default_server_config(n_slots=n_slots, hbm_budget_bytes=args.serve_budget_gib << 30))
+    server = Server(default_server_config(n_slots=n_slots, hbm_budget_bytes=args.serve_budget_gib << 30,
+                                          occupancy_threshold=thr))
     ids = [server.add_avatar(ctx, bases) for ctx, bases in fp16]
     outs = {r: [torch.empty((r * r, 4), dtype=torch.float32, device="cuda") for _ in range(n_slots * 4)]
             for r in resolutions.tolist()}
@@ -325,7 +341,7 @@ def serving_bench(args):
             torch.cuda.synchronize()
             t = time.perf_counter()
             ctx.set_frame_expression(frame, bases, torch.from_numpy(c).cuda())
-            ctx.build_occupancy(frame, 0.05)
+            ctx.build_occupancy(frame, thr)
             ctx.render(cam, res, res, frame, frame, 0, (outs[res][0], None, None))
             torch.cuda.synchronize()
             times.append(time.perf_counter() - t)
@@ -337,6 +353,9 @@ def serving_bench(args):
         svc[str(rate)] = dict(frames_per_s=len(reqs) / (finish - arrivals[0]), p50_ms=pct(lat, 50),
                               p99_ms=pct(lat, 99), mean_service_ms=float(np.mean(times) * 1e3))
     out["sequential_fp32"] = svc
+    out["workload"] = ("8 avatars at the " + ("full-size shapes (9,976-tri heads, L16 T2^19, 128^3 grid)" if full
+                       else "small shapes (2,048-tri spheres, L8 T2^14, 64^3 grid)") +
+                       ", resolutions 128/256/512/1024^2 (0.3/0.3/0.3/0.1), Poisson arrivals")
     return out
 
 
@@ -441,7 +460,9 @@ def main():
             forms["accuracy_vs_fp64"] = accuracy_vs_fp64(ctxs, cfg, av, params, cams[0], res, args.accuracy_pixels)
 
     train = None if args.skip_train else training_bench(args)
+    train_full = None if args.skip_train else training_bench(args, full=True)
     serve = None if args.skip_serve else serving_bench(args)
+    serve_full = None if args.skip_serve else serving_bench(args, full=True)
 
     cpu = None
     if args.cpu_budget > 0:
@@ -471,7 +492,9 @@ def main():
         "deform_traversal": traversal,
         "cpu_baseline": cpu,
         "training": train,
+        "training_full_size": train_full,
         "serving": serve,
+        "serving_full_size": serve_full,
     }
     print(json.dumps(line))
 
