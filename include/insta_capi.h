@@ -265,7 +265,11 @@ typedef struct insta_train_config {
     uint32_t grid_grad_fp16;  /* 1: hash-grid gradients accumulated in FP16 (packed atomics) */
     uint32_t apply_update;    /* 0: gradients only, parameters unchanged */
     uint32_t seed;            /* PCG32 seed of the ray/background stream */
+    int32_t  occupancy_slot;  /* grid the rays march: INSTA_OCCUPANCY_PER_FRAME (each image's
+                                 frame's own grid) or one slot for all rays (-1 canonical,
+                                 f frame f), e.g. the density-grid EMA's */
 } insta_train_config;
+#define INSTA_OCCUPANCY_PER_FRAME (-2)
 
 typedef struct insta_train_stats {
     uint32_t n_rays;          /* rays generated */
@@ -278,8 +282,9 @@ typedef struct insta_train_stats {
 
 /* Defaults: adaptive rays aiming at a 2^18-sample batch, 2^22 marched samples
  * at most, Adam (0.9, 0.99, 1e-15) at learning rate 1e-2, FP32 grid
- * gradients, loss scale 1, seed 1337. */
+ * gradients, loss scale 1, seed 1337, per-frame occupancy grids. */
 void insta_default_train_config(insta_train_config* tc);
+
 
 /* One training step, Testbed::train_nerf_step (testbed_nerf.cu:3176-3400):
  * rays from random training pixels (generate_training_samples_nerf,
@@ -330,6 +335,19 @@ typedef struct insta_density_grid_config {
 insta_status insta_update_density_grid(insta_ctx* ctx, int32_t slot, uint32_t n_frames,
                                        const insta_density_grid_config* cfg, float* grid_dev, float* mean,
                                        void* stream);
+
+/* n_steps of the reference's training loop (Testbed::train +
+ * training_prep_nerf, instant-ngp/src/testbed.cu:2480-2492,
+ * testbed_nerf.cu:3428-3444): before optimizer step s the density grid is
+ * updated when s % clamp(s / 16, 1, 16) == 0 -- G^3 uniform samples while
+ * s < 256, then G^3/4 uniform + G^3/4 above the threshold -- with
+ * dg->decay and dg->seed (its sample counts are ignored), through frames
+ * 0 .. n_frames - 2 (warp_random_samples), into occupancy slot density_slot,
+ * which the step's rays then march (tc->occupancy_slot is replaced by it).
+ * last (nullable) receives the last step's statistics. */
+insta_status insta_train_loop(insta_ctx* ctx, const insta_train_config* tc, uint32_t n_steps,
+                              const insta_density_grid_config* dg, int32_t density_slot, uint32_t n_frames,
+                              insta_train_stats* last, void* stream);
 /* Restart the EMA (next update zeroes the grid and reseeds). */
 insta_status insta_reset_density_grid(insta_ctx* ctx);
 

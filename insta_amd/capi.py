@@ -72,7 +72,7 @@ class InstaTrainConfig(C.Structure):
     _fields_ = [("n_rays", C.c_uint32), ("target_samples", C.c_uint32), ("max_samples", C.c_uint32),
                 ("learning_rate", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("epsilon", C.c_float),
                 ("loss_scale", C.c_float), ("grid_grad_fp16", C.c_uint32), ("apply_update", C.c_uint32),
-                ("seed", C.c_uint32)]
+                ("seed", C.c_uint32), ("occupancy_slot", C.c_int32)]
 
 
 class InstaTrainStats(C.Structure):
@@ -137,6 +137,8 @@ SIGNATURES = [
     ("insta_set_training_set", C.c_int, [_VP, C.c_uint32, C.c_uint32, C.c_uint32, _VP, _VP, _VP, _VP]),
     ("insta_default_train_config", None, [C.POINTER(InstaTrainConfig)]),
     ("insta_train_step", C.c_int, [_VP, C.POINTER(InstaTrainConfig), C.POINTER(InstaTrainStats), _VP]),
+    ("insta_train_loop", C.c_int, [_VP, C.POINTER(InstaTrainConfig), C.c_uint32, C.POINTER(InstaDensityGridConfig),
+                                   C.c_int32, C.c_uint32, C.POINTER(InstaTrainStats), _VP]),
     ("insta_get_gradients", C.c_int, [_VP, _VP, _VP]),
     ("insta_get_params", C.c_int, [_VP, _VP, _VP]),
     ("insta_reset_optimizer", C.c_int, [_VP]),
@@ -433,6 +435,16 @@ class Context:
     def train_step(self, tc: "InstaTrainConfig") -> InstaTrainStats:
         st = InstaTrainStats()
         check(self.lib.insta_train_step(self.h, C.byref(tc), C.byref(st), self.stream()), "insta_train_step")
+        return st
+
+    def train_loop(self, tc: "InstaTrainConfig", n_steps: int, density_slot: int, n_frames: int,
+                   decay: float = 0.95, seed: int = 1337) -> InstaTrainStats:
+        """n_steps of the reference's loop: density-grid EMA on its schedule into
+        density_slot, which the rays march."""
+        st = InstaTrainStats()
+        dg = InstaDensityGridConfig(decay, 0, 0, seed)
+        check(self.lib.insta_train_loop(self.h, C.byref(tc), n_steps, C.byref(dg), density_slot, n_frames, C.byref(st),
+                                        self.stream()), "insta_train_loop")
         return st
 
     def gradients(self):

@@ -610,6 +610,7 @@ void insta_default_train_config(insta_train_config* tc) {
     tc->grid_grad_fp16 = 0;
     tc->apply_update = 1;
     tc->seed = 1337u;
+    tc->occupancy_slot = INSTA_OCCUPANCY_PER_FRAME;
 }
 
 insta_status insta_set_training_set(insta_ctx* ctx, uint32_t n_images, uint32_t width, uint32_t height,
@@ -656,7 +657,45 @@ insta_status insta_train_step(insta_ctx* ctx, const insta_train_config* tc, inst
         set_error("train_step: parameters not set");
         return INSTA_ERR_STATE;
     }
+    if (tc->occupancy_slot != INSTA_OCCUPANCY_PER_FRAME) {
+        if (tc->occupancy_slot < -1 || tc->occupancy_slot >= (int32_t)ctx->cfg.max_frames) {
+            set_error("train_step: bad occupancy slot");
+            return INSTA_ERR_INVALID;
+        }
+        if (!ctx->grid_built[tc->occupancy_slot + 1]) {
+            set_error("train_step: the occupancy slot has no grid");
+            return INSTA_ERR_STATE;
+        }
+    }
     return insta::train_step(ctx, *tc, stats, (hipStream_t)stream);
+}
+
+insta_status insta_train_loop(insta_ctx* ctx, const insta_train_config* tc, uint32_t n_steps,
+                              const insta_density_grid_config* dg, int32_t density_slot, uint32_t n_frames,
+                              insta_train_stats* last, void* stream) {
+    INSTA_CHECK_CTX(ctx);
+    if (!tc || !dg) {
+        set_error("train_loop: null configuration");
+        return INSTA_ERR_INVALID;
+    }
+    const uint64_t g3 = (uint64_t)ctx->cfg.grid_size * ctx->cfg.grid_size * ctx->cfg.grid_size;
+    insta_train_config step_cfg = *tc;
+    step_cfg.occupancy_slot = density_slot;
+    for (uint32_t k = 0; k < n_steps; ++k) {
+        // the optimizer step count plays the reference's m_training_step
+        const uint32_t s = ctx->train ? insta::train_step_count(ctx) : 0u;
+        const uint32_t every = std::min(std::max(s / 16u, 1u), 16u);
+        if (s % every == 0) {
+            insta_density_grid_config c = *dg;
+            c.n_uniform = (uint32_t)(s < 256 ? g3 : g3 / 4);
+            c.n_nonuniform = (uint32_t)(s < 256 ? 0 : g3 / 4);
+            insta_status st = insta_update_density_grid(ctx, density_slot, n_frames, &c, nullptr, nullptr, stream);
+            if (st != INSTA_OK) return st;
+        }
+        insta_status st = insta_train_step(ctx, &step_cfg, last, stream);
+        if (st != INSTA_OK) return st;
+    }
+    return INSTA_OK;
 }
 
 insta_status insta_get_gradients(insta_ctx* ctx, float* grads_dev, void* stream) {

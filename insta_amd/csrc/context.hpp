@@ -209,6 +209,7 @@ insta_status update_density_grid(insta_ctx* ctx, uint32_t slot, uint32_t n_frame
                                  float* grid_out, float* mean_out, hipStream_t stream);
 insta_status reset_density_grid(insta_ctx* ctx);
 bool train_has_set(const insta_ctx* ctx);
+uint32_t train_step_count(const insta_ctx* ctx);  // optimizer steps taken
 // snapshot state (snapshot.hip): host copies of the training and density-grid state
 struct TrainSnapshot {
     bool present = false;

@@ -146,6 +146,7 @@ struct RayArgs {
     const uint8_t* grids_lin;     // [slot][G^3/8]
     size_t grid_bytes;
     uint32_t n_images, width, height, max_samples;
+    int32_t fixed_slot;           // >= -1: every ray marches this slot's grid
 };
 
 INSTA_DEV bool train_ray(const RayArgs& a, uint32_t i, uint32_t n_rays, uint64_t rays_total, Pcg32 rng,
@@ -167,7 +168,7 @@ INSTA_DEV bool train_ray(const RayArgs& a, uint32_t i, uint32_t n_rays, uint64_t
     d = v3(d.x / dn, d.y / dn, d.z / dn);
     o = v3(c.M[3], c.M[7], c.M[11]);
     ra = a.box;
-    ra.grid = a.grids_lin + (size_t)(c.frame + 1) * a.grid_bytes;
+    ra.grid = a.grids_lin + (size_t)((a.fixed_slot >= -1 ? a.fixed_slot : c.frame) + 1) * a.grid_bytes;
     const float tb = box_ray_tmin(ra, o, d);
     if (tb == 3.40282347e38f) return false;
     t0 = fmaxf(tb, 0.0f) + a.box.dt * jitter;
@@ -821,6 +822,7 @@ insta_status train_step(insta_ctx* ctx, const insta_train_config& tc, insta_trai
     ra.width = ts->width;
     ra.height = ts->height;
     ra.max_samples = cap;
+    ra.fixed_slot = tc.occupancy_slot;
     hipLaunchKernelGGL(train_rays_kernel, dim3((n_rays + 255) / 256), dim3(256), 0, stream, ra, n_rays, ts->rays_total,
                        rng, ts->ray_d, ts->ray_frame, ts->ray_img, ts->ray_pix, ts->ray_src, ts->ray_base, ts->ray_cnt,
                        ts->s_pos, ts->t_buf, ts->counters);
@@ -889,6 +891,7 @@ insta_status train_step(insta_ctx* ctx, const insta_train_config& tc, insta_trai
 
 bool train_has_gradients(const insta_ctx* ctx) { return ctx->train && ctx->train->have_grad; }
 bool train_has_set(const insta_ctx* ctx) { return ctx->train && ctx->train->n_images > 0; }
+uint32_t train_step_count(const insta_ctx* ctx) { return ctx->train ? ctx->train->step : 0u; }
 
 insta_status get_gradients(insta_ctx* ctx, float* out, hipStream_t stream) {
     INSTA_HIP(hipMemcpyAsync(out, ctx->train->grad, ctx->layout.n_params * sizeof(float), hipMemcpyDeviceToDevice,

@@ -568,6 +568,7 @@ struct orc_scene {
     std::vector<float> tr_images;              // [n][h][w][4]
     std::vector<insta_camera> tr_cams;
     std::vector<int32_t> tr_frames;
+    int32_t tr_occ_slot = -2;     // >= -1: every training ray marches this slot's grid
 };
 
 namespace {
@@ -1599,7 +1600,7 @@ bool train_ray(const orc_scene* s, uint32_t i, uint32_t n_rays, uint64_t rays_to
     r.samples.clear();
     const float tb = box_ray_tmin(s->render_lo, s->render_hi, o, d);
     if (tb == FLT_MAX) return false;
-    const std::vector<uint8_t>& grid = s->grids[s->tr_frames[r.img] + 1];
+    const std::vector<uint8_t>& grid = s->grids[(s->tr_occ_slot >= -1 ? s->tr_occ_slot : s->tr_frames[r.img]) + 1];
     const uint32_t G = s->cfg.grid_size;
     const float DT = s->cfg.dt;
     const V3 idir = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
@@ -1729,6 +1730,13 @@ void backward_d(const orc_scene* s, const SampleAct& a, const double dout[4], do
 }  // namespace
 
 extern "C" {
+
+// the grid training rays march: slot >= -1 for all rays, -2 each image's frame
+int orc_set_training_occupancy(orc_scene* s, int32_t slot) {
+    if (slot < -2 || slot + 1 >= (int32_t)s->grids.size()) return 1;
+    s->tr_occ_slot = slot;
+    return 0;
+}
 
 int orc_set_training_set(orc_scene* s, uint32_t n_images, uint32_t width, uint32_t height, const float* rgba,
                          const insta_camera* cams, const int32_t* frames) {

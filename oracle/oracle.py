@@ -56,6 +56,7 @@ _SIGS = [
     ("orc_pcg32_next", C.c_uint32, [C.c_uint64, C.c_uint64, C.c_uint64]),
     ("orc_pcg32_draws", None, [C.c_uint64, C.c_uint64, C.c_uint32, _VP]),
     ("orc_train_grads", C.c_int, [_VP, C.c_uint32, C.c_uint32, C.c_uint64, _VP, _VP, _VP]),
+    ("orc_set_training_occupancy", C.c_int, [_VP, C.c_int32]),
     ("orc_deform_positions", C.c_int, [_VP, _VP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64]),
     ("orc_update_density_grid", C.c_int, [_VP, C.c_int32, C.c_uint32, C.c_float, C.c_uint32, C.c_uint32, C.c_uint32,
                                           _VP, _VP]),
@@ -228,6 +229,11 @@ def set_training_set(orc: Oracle, images, cams, frames) -> None:
     arr = (InstaCamera * n)(*cams)
     fr = _c(frames, np.int32)
     assert orc.L.orc_set_training_set(orc.h, n, w, h, _p(im), C.addressof(arr), _p(fr)) == 0
+
+
+def set_training_occupancy(orc: Oracle, slot: int) -> None:
+    """Grid the training rays march: slot >= -1 for every ray, -2 each image's frame."""
+    assert orc.L.orc_set_training_occupancy(orc.h, slot) == 0
 
 
 def train_grads(orc: Oracle, n_rays: int, seed: int, rays_total: int = 0, want_grads: bool = True):
