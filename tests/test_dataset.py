@@ -1,0 +1,114 @@
+"""Dataset ingestion (insta_amd.dataset) on a capture written here: frames
+sorted by mesh path, OBJ meshes with the +0.5 offset, FLAME parameter files,
+camera matrices with the scene offset, focal lengths and principal points
+from the reference's keys, images decoded to RGBA floats."""
+import json
+import math
+
+import numpy as np
+import pytest
+
+from insta_amd import dataset, synth
+
+
+def _write_obj(path, verts, faces, style="plain"):
+    lines = [f"v {x:.9g} {y:.9g} {z:.9g}" for x, y, z in verts]
+    for a, b, c in faces:
+        if style == "slash":
+            lines.append(f"f {a + 1}/{a + 1}/{a + 1} {b + 1}//{b + 1} {c + 1}/{c + 1}")
+        else:
+            lines.append(f"f {a + 1} {b + 1} {c + 1}")
+    path.write_text("\n".join(lines) + "\n")
+
+
+@pytest.fixture
+def capture(tmp_path):
+    from PIL import Image
+    av = synth.small_sphere_avatar(seed=4, n_frames=3)
+    canon = av.canonical - 0.5            # the loader adds 0.5
+    _write_obj(tmp_path / "canonical.obj", canon, av.faces)
+    (tmp_path / "meshes").mkdir()
+    (tmp_path / "exp").mkdir()
+    (tmp_path / "images").mkdir()
+    rng = np.random.default_rng(0)
+    frames = []
+    imgs = []
+    for i, v in enumerate(av.frames):
+        _write_obj(tmp_path / "meshes" / f"{i:05d}.obj", v - 0.5, av.faces, style="slash" if i == 1 else "plain")
+        (tmp_path / "exp" / f"{i:05d}.txt").write_text("\n".join(f"{x:.9g}" for x in rng.normal(size=100)) + "\n")
+        img = rng.integers(0, 256, (24, 32, 4), dtype=np.uint8)
+        Image.fromarray(img, "RGBA").save(tmp_path / "images" / f"{i:05d}.png")
+        imgs.append(img)
+        m = np.eye(4)
+        m[:3, 3] = rng.normal(size=3)
+        frames.append({"file_path": f"images/{i:05d}", "mesh_path": f"meshes/{i:05d}.obj",
+                       "exp_path": f"exp/{i:05d}.txt", "transform_matrix": m.tolist()})
+    frames[2]["camera_angle_x"] = 0.7          # frame-level focal (radians)
+    frames[2]["cx"], frames[2]["w"], frames[2]["cy"], frames[2]["h"] = 8.0, 32, 6.0, 24
+    doc = {"fl_x": 40.0, "fl_y": 41.0, "cx": 16.5, "cy": 11.0, "w": 32, "h": 24,
+           "frames": [frames[2], frames[0], frames[1]]}           # not in mesh-path order
+    (tmp_path / "transforms_train.json").write_text(json.dumps(doc))
+    return tmp_path, av, doc, imgs
+
+
+def test_capture_loads_with_the_reference_conventions(capture):
+    path, av, doc, imgs = capture
+    ds = dataset.load_transforms(path / "transforms_train.json")
+    # written as v - 0.5, read back + 0.5 in float32: equal to rounding
+    assert np.allclose(ds.canonical, av.canonical, atol=1e-6) and np.array_equal(ds.faces, av.faces)
+    assert (ds.masks == synth.MASK_TORSO).all()
+    assert len(ds.frames) == 3 and ds.resolution == (32, 24)
+    for i in range(3):                     # sorted by mesh path: image i is mesh i
+        assert np.allclose(ds.frames[i], av.frames[i], atol=1e-6)
+        assert ds.image_paths[i].name == f"{i:05d}.png"
+        assert ds.expressions[i].shape == (75,)
+    by_mesh = sorted(doc["frames"], key=lambda f: f["mesh_path"])
+    for cam, fr in zip(ds.cameras, by_mesh):
+        m = np.asarray(fr["transform_matrix"])[:3, :4].copy()
+        m[:, 3] += 0.5
+        assert np.allclose(np.frombuffer(bytes(cam.matrix), np.float32).reshape(3, 4), m, atol=1e-6)
+    c0, c2 = ds.cameras[0], ds.cameras[2]
+    assert (c0.focal_px[0], c0.focal_px[1]) == (40.0, 41.0)
+    assert c0.screen_center[0] == pytest.approx(16.5 / 32) and c0.screen_center[1] == pytest.approx(11.0 / 24)
+    assert c2.focal_px[0] == pytest.approx(0.5 * 32 / math.tan(0.35), rel=1e-6) == c2.focal_px[1]
+    assert (c2.screen_center[0], c2.screen_center[1]) == (pytest.approx(0.25), pytest.approx(0.25))
+    got = ds.images()
+    assert np.array_equal(got, np.stack(imgs).astype(np.float32) / 255.0)
+
+
+def test_n_frames_offset_and_aabb(capture):
+    path, _, doc, _ = capture
+    doc = dict(doc, n_frames=2, offset=[0.1, 0.2, 0.3])
+    (path / "t2.json").write_text(json.dumps(doc))
+    ds = dataset.load_transforms(path / "t2.json")
+    assert len(ds.cameras) == 2 and ds.offset == (0.1, 0.2, 0.3)
+    m = np.frombuffer(bytes(ds.cameras[0].matrix), np.float32).reshape(3, 4)
+    t = np.asarray(sorted(doc["frames"], key=lambda f: f["mesh_path"])[0]["transform_matrix"])[:3, 3]
+    assert np.allclose(m[:, 3], t + [0.1, 0.2, 0.3], atol=1e-6)
+    doc3 = dict(doc, aabb=[[-1.0, -2.0, 0.0], [1.0, 2.0, 1.0]])
+    (path / "t3.json").write_text(json.dumps(doc3))
+    ds3 = dataset.load_transforms(path / "t3.json")
+    assert ds3.scale == pytest.approx(0.25) and np.allclose(ds3.offset, [0.5, 0.5, 0.375])
+
+
+def test_obj_polygons_and_negative_indices(tmp_path):
+    p = tmp_path / "q.obj"
+    p.write_text("v 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nf 1 2 3 4\nf -4 -2 -1\n")
+    v, f = dataset.read_obj(p)
+    assert np.allclose(v, np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]]) + 0.5)
+    assert f.tolist() == [[0, 1, 2], [0, 2, 3], [0, 2, 3]]
+    p.write_text("v 0 0 0\nf 1 2 3\n")
+    with pytest.raises(ValueError):
+        dataset.read_obj(p)
+
+
+def test_inconsistent_captures_are_rejected(capture):
+    path, av, doc, _ = capture
+    _write_obj(path / "meshes" / "00001.obj", av.frames[1][:-1] - 0.5, av.faces[:-1])
+    with pytest.raises(ValueError):
+        dataset.load_transforms(path / "transforms_train.json")
+    doc2 = {k: v for k, v in doc.items() if k not in ("fl_x", "fl_y")}
+    doc2["frames"] = [f for f in doc["frames"] if "camera_angle_x" not in f]
+    (path / "nofocal.json").write_text(json.dumps(doc2))
+    with pytest.raises(ValueError):
+        dataset.load_transforms(path / "nofocal.json")
