@@ -64,7 +64,10 @@ typedef struct insta_config {
     uint32_t density_out;            /* 16: rgb input = density_out + SH coefficients */
     uint32_t rgb_hidden_layers;      /* >= 1 */
     uint32_t sh_degree;              /* 4 -> 16 coefficients */
-    uint32_t n_extra_dims;           /* appended to the density input */
+    uint32_t n_extra_dims;           /* appended to the density input: 0, or 16
+                                        expression values per frame on the
+                                        device paths (exp_params of the
+                                        reference's configs/nerf/main.json) */
     float    density_logit_bias;     /* sigma = exp(logit + bias); 0 = reference */
     /* occupancy grid and ray march (cone angle 0, one cascade) */
     uint32_t grid_size;              /* G: occupancy cells per axis over [0,1]^3 */
@@ -162,6 +165,20 @@ insta_status insta_set_frames(insta_ctx* ctx, uint32_t first_frame, uint32_t n_f
  * rebuilt as insta_set_frames does.  Serving path for per-request expressions. */
 insta_status insta_set_frame_expression(insta_ctx* ctx, uint32_t frame, const float* bases_dev,
                                         uint32_t n_bases, const float* coeffs_dev, void* stream);
+
+/* Per-frame extra dims of the density input (n_extra_dims values per frame,
+ * values_dev [n_frames][n_extra_dims] on the device), the expression
+ * conditioning of Core::load_meshes (src/core.cu:251-274: the first
+ * exp_params FLAME values of each frame).  A sample keeps its frame's values
+ * when its nearest triangle is in the lips region (mask 7) or it is not
+ * deformed, and takes 1.0 everywhere else (warp, src/core.cu:514-519); the
+ * density-grid update feeds 1.0 throughout (extract_expressions_random,
+ * nerf_network.h:107-127).  Rendering a frame uses that frame's values
+ * (Core::render_deformed sets extra_dim_idx_for_inference to the frame,
+ * src/core.cu:171).  Frames start at zero; insta_set_frame_expression also
+ * sets them, to its first n_extra_dims coefficients (zero-padded). */
+insta_status insta_set_frame_extra_dims(insta_ctx* ctx, uint32_t first_frame, uint32_t n_frames,
+                                        const float* values_dev, void* stream);
 
 /* Nearest triangle of each point on a frame's mesh (frame = -1: canonical),
  * the query of `closest_triangle`

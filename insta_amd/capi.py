@@ -117,6 +117,7 @@ SIGNATURES = [
     ("insta_set_canonical_mesh", C.c_int, [_VP, _VP, C.c_uint32, _VP, C.c_uint32, _VP, _VP]),
     ("insta_set_frames", C.c_int, [_VP, C.c_uint32, C.c_uint32, _VP, _VP]),
     ("insta_set_frame_expression", C.c_int, [_VP, C.c_uint32, _VP, C.c_uint32, _VP, _VP]),
+    ("insta_set_frame_extra_dims", C.c_int, [_VP, C.c_uint32, C.c_uint32, _VP, _VP]),
     ("insta_closest_triangles", C.c_int, [_VP, C.c_int32, _VP, C.c_uint32, _VP, _VP, _VP]),
     ("insta_export_bvh", C.c_int, [_VP, C.c_int32, _VP, _VP, C.POINTER(C.c_uint32), _VP]),
     ("insta_deform_samples", C.c_int, [_VP, _VP, C.c_uint32, C.c_uint32, C.c_int32, _VP]),
@@ -304,6 +305,13 @@ class Context:
     def set_frame_expression(self, frame: int, bases_dev, coeffs_dev) -> None:
         check(self.lib.insta_set_frame_expression(self.h, frame, _ptr(bases_dev), bases_dev.shape[0],
                                                   _ptr(coeffs_dev), self.stream()), "insta_set_frame_expression")
+
+    def set_frame_extra_dims(self, first: int, values_dev) -> None:
+        """values_dev [n_frames, n_extra_dims] float32 on the device."""
+        v = values_dev.contiguous()
+        n = v.shape[0] if v.dim() == 2 else 1
+        check(self.lib.insta_set_frame_extra_dims(self.h, first, n, _ptr(v), self.stream()),
+              "insta_set_frame_extra_dims")
 
     def closest(self, frame: int, pts_dev):
         n = pts_dev.shape[0]

@@ -86,7 +86,7 @@ insta_status validate(const insta_config& c) {
         return bad("n_neurons in [1, 256], density_out in [1, 48]");
     if (c.density_hidden_layers == 0 || c.rgb_hidden_layers == 0) return bad("at least one hidden layer per MLP");
     if (c.sh_degree != 4) return bad("sh_degree must be 4");
-    if (c.n_extra_dims > 64) return bad("n_extra_dims must be <= 64");
+    if (c.n_extra_dims != 0 && c.n_extra_dims != 16) return bad("n_extra_dims must be 0 or 16");
     if (!is_pow2(c.grid_size) || c.grid_size < 8 || c.grid_size > 1024) return bad("grid_size: power of two in [8, 1024]");
     if (!(c.dt > 0.0f) || c.max_steps == 0) return bad("dt > 0 and max_steps >= 1");
     if (!(c.min_transmittance >= 0.0f && c.min_transmittance < 1.0f)) return bad("min_transmittance in [0, 1)");
@@ -235,6 +235,14 @@ insta_status insta_create(const insta_config* cfg, insta_ctx** out) {
     if (cfg->precision == INSTA_PRECISION_FP16 || cfg->precision == INSTA_PRECISION_BF16)
         alloc(&ctx->d_table_h, ctx->layout.n_grid_params * 2);
     alloc(&ctx->d_scratch, ctx->scratch_bytes);
+    if (cfg->n_extra_dims) {
+        alloc((void**)&ctx->d_frame_extra, (size_t)cfg->max_frames * cfg->n_extra_dims * sizeof(float));
+        alloc((void**)&ctx->d_extra_bias, S * 64 * sizeof(float));
+        if (ok && hipMemset(ctx->d_frame_extra, 0, (size_t)cfg->max_frames * cfg->n_extra_dims * sizeof(float)) !=
+                      hipSuccess)
+            ok = false;
+        if (ok && hipMemset(ctx->d_extra_bias, 0, S * 64 * sizeof(float)) != hipSuccess) ok = false;
+    }
     if (ok && hipMemset(ctx->d_built, 0, S) != hipSuccess) ok = false;
     if (ok && hipMemset(ctx->d_grids, 0, S * (G * G * G / 8)) != hipSuccess) ok = false;
     if (ok && hipMemset(ctx->d_grids_lin, 0, S * (G * G * G / 8)) != hipSuccess) ok = false;
@@ -258,6 +266,7 @@ void insta_destroy(insta_ctx* ctx) {
     dev_free(ctx->d_cen); dev_free(ctx->d_grids); dev_free(ctx->d_grids_lin); dev_free(ctx->d_built); dev_free(ctx->d_params);
     dev_free(ctx->d_build_lists); dev_free(ctx->d_build_q);
     dev_free(ctx->d_scratch); dev_free(ctx->d_wfrag); dev_free(ctx->d_table_h);
+    dev_free(ctx->d_frame_extra); dev_free(ctx->d_extra_bias);
     delete ctx;
 }
 
@@ -374,8 +383,33 @@ insta_status insta_set_frame_expression(insta_ctx* ctx, uint32_t frame, const fl
     }
     insta_status st = insta::launch_blend_expression(ctx, frame + 1, bases_dev, n_bases, coeffs_dev, stream);
     if (st != INSTA_OK) return st;
+    st = insta::launch_extra_from_coeffs(ctx, frame, coeffs_dev, n_bases, stream);
+    if (st != INSTA_OK) return st;
     ctx->grid_built[frame + 1] = 0;
     return insta::mark_built(ctx, frame + 1, 1, 1, stream);
+}
+
+insta_status insta_set_frame_extra_dims(insta_ctx* ctx, uint32_t first_frame, uint32_t n_frames,
+                                        const float* values_dev, void* stream_) {
+    INSTA_CHECK_CTX(ctx);
+    hipStream_t stream = (hipStream_t)stream_;
+    const uint32_t X = ctx->cfg.n_extra_dims;
+    if (X == 0) {
+        set_error("set_frame_extra_dims: n_extra_dims is 0");
+        return INSTA_ERR_STATE;
+    }
+    if (n_frames && !values_dev) {
+        set_error("set_frame_extra_dims: null pointer");
+        return INSTA_ERR_INVALID;
+    }
+    if ((uint64_t)first_frame + n_frames > ctx->cfg.max_frames) {
+        set_error("set_frame_extra_dims: frame range outside [0, max_frames)");
+        return INSTA_ERR_INVALID;
+    }
+    if (n_frames == 0) return INSTA_OK;
+    INSTA_HIP(hipMemcpyAsync(ctx->d_frame_extra + (size_t)first_frame * X, values_dev,
+                             (size_t)n_frames * X * sizeof(float), hipMemcpyDeviceToDevice, stream));
+    return insta::update_extra_rows(ctx, first_frame + 1, n_frames, stream);
 }
 
 insta_status insta_closest_triangles(insta_ctx* ctx, int32_t frame, const float* points_dev, uint32_t n,
@@ -518,9 +552,10 @@ insta_status insta_network_inference(insta_ctx* ctx, const float* coords_dev, ui
         set_error("network_inference: bad stride or null pointer");
         return INSTA_ERR_INVALID;
     }
-    if (insta::mfma_network_supported(ctx->cfg))
+    if (insta::mfma_network_supported(ctx->cfg) && ctx->cfg.n_extra_dims == 0)
         return insta::launch_network_mfma_aos(ctx, coords_dev, n, stride_floats, out_dev, (hipStream_t)stream);
-    if (!insta::network_supported(ctx->cfg) || !insta::scalar_precision(ctx->cfg)) {
+    // per-sample extra values (NerfCoordinate tail) take the FP32 per-sample kernel at any precision
+    if (!insta::network_supported(ctx->cfg) || (!insta::scalar_precision(ctx->cfg) && ctx->cfg.n_extra_dims == 0)) {
         set_error("network_inference: configuration not supported by the fused MLP kernels");
         return INSTA_ERR_UNSUPPORTED;
     }

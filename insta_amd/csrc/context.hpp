@@ -80,6 +80,8 @@ struct insta_ctx {
     float* d_params = nullptr;      // FP32 master copy [n_params]
     void* d_wfrag = nullptr;        // MLP weights arranged as MFMA A fragments
     void* d_table_h = nullptr;      // FP16 copy of the hash tables (reduced precision)
+    float* d_frame_extra = nullptr; // [max_frames][n_extra_dims] per-frame extra dims (n_extra_dims > 0)
+    float* d_extra_bias = nullptr;  // [n_slots][64] W0[:, enc:] . x: row 0 x = 1, row 1 + f frame f's values
     bool params_set = false;
 
     // render workspace of insta_render (grown on demand); the server keeps
@@ -245,6 +247,14 @@ inline bool scalar_precision(const insta_config& c) {
     return c.precision == INSTA_PRECISION_FP32 || c.precision == INSTA_PRECISION_FP32_SCALAR;
 }
 insta_status prepare_network(insta_ctx* ctx, hipStream_t stream);
+// extra-dims table rows [first, first + n) from the current weights and frame values
+insta_status update_extra_rows(insta_ctx* ctx, uint32_t first, uint32_t n, hipStream_t stream);
+// frame's extra dims = first n_extra_dims of coeffs (device), then its table row
+insta_status launch_extra_from_coeffs(insta_ctx* ctx, uint32_t frame, const float* coeffs, uint32_t n_coeffs,
+                                      hipStream_t stream);
+// per-sample extra-dims selector in the canonical direction's w: 0 = all 1.0,
+// 1 + f = frame f's values
+constexpr uint32_t kExtraOnes = 0u;
 insta_status launch_network_mfma_soa(insta_ctx* ctx, const uint32_t* n_dev, uint32_t n_max, const float4* cpos,
                                      const float4* cdir, float4* out, hipStream_t stream);
 insta_status launch_network_mfma_aos(insta_ctx* ctx, const float* coords, uint32_t n, uint32_t stride, float* out,

@@ -24,6 +24,7 @@ struct MeshSet {
     const uint8_t* built;      // [n_slots]
     size_t node_stride, tri_stride, cen_stride;  // float4 units per slot
     uint32_t nf, leaf_base, k, n_slots, disable_interp;
+    uint32_t n_extra;          // extra dims of the density input (0: no selector)
 
     INSTA_HD MeshView view(uint32_t slot) const {
         MeshView m;
@@ -56,6 +57,7 @@ MeshSet mesh_set(const insta_ctx* ctx, const uint8_t* built_dev) {
     s.k = ctx->bvh_k;
     s.n_slots = ctx->n_slots;
     s.disable_interp = ctx->cfg.disable_interp;
+    s.n_extra = ctx->cfg.n_extra_dims;
     return s;
 }
 
@@ -213,7 +215,10 @@ __global__ __launch_bounds__(kDeformThreads) void deform_aos_kernel(MeshSet ms, 
 //         coherence above.)
 //  apply: per sample, the frame transforms of `warp` from the hit; writes
 //         canonical position (+ mask bits) to s_cpos and the warped
-//         direction to s_cdir.
+//         direction to s_cdir, whose w selects the extra dims: all 1.0 when
+//         the sample was deformed off the lips region (mask 7), else the
+//         frame's values (warp only overwrites them in the first case,
+//         src/core.cu:514-519).
 constexpr uint32_t kMaxBlocks = 65536;
 
 // Frame of a sample: the per-ray table when given (training batches mix
@@ -275,8 +280,10 @@ __global__ __launch_bounds__(kDeformThreads) void deform_apply_kernel(MeshSet ms
         const int32_t f = sample_frame(frame, ray_frame, sp);
         const int32_t h = f >= 0 ? s_hit[i] : -1;
         if (h >= 0) warp_from_hit(ms, (uint32_t)f, h, p, dir_w, op, od, mask);
+        const bool frame_values = ms.n_extra && f >= 0 && !(h >= 0 && mask != 7u);
+        const uint32_t sel = frame_values ? (uint32_t)f + 1u : kExtraOnes;
         s_cpos[i] = make_float4(op.x, op.y, op.z, __uint_as_float(mask));
-        s_cdir[i] = make_float4(od.x, od.y, od.z, 0.0f);
+        s_cdir[i] = make_float4(od.x, od.y, od.z, __uint_as_float(sel));
     }
 }
 

@@ -8,7 +8,9 @@
 //             which then advances by 2^32 (pcg32's default jump)
 //   warp      each position to canonical space through a random posed frame
 //             (warp_random_samples, src/core.cu:528-579)
-//   splat     density MLP at the canonical position, optical thickness
+//   splat     density MLP at the canonical position (extra dims all 1.0, as
+//             extract_expressions_random writes, nerf_network.h:107-127),
+//             optical thickness
 //             exp(logit + bias) * dt, atomicMax on the float bits into the
 //             cell (splat_grid_samples_nerf_max_nearest_neighbor, 518-534)
 //   ema       grid = max(grid * decay, new) unless negative
@@ -70,13 +72,13 @@ __global__ void dg_samples_kernel(uint32_t n, uint32_t step, float thresh, uint3
 
 // density MLP output 0 at a canonical position (the FMA chains of the plain
 // inference kernel), optical thickness max-splatted into the sample's cell
-template <int L>
+template <int L, int X>
 __global__ __launch_bounds__(128) void dg_splat_kernel(const float* __restrict__ params,
                                                        const float* __restrict__ table, GridParams gp,
                                                        const float* __restrict__ pos,
                                                        const uint32_t* __restrict__ idx, uint32_t n, float dt,
                                                        float bias, float* __restrict__ tmp) {
-    constexpr int E = 2 * L;
+    constexpr int E = 2 * L + X;
     constexpr int W = 64;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -84,6 +86,8 @@ __global__ __launch_bounds__(128) void dg_splat_kernel(const float* __restrict__
     float enc[E];
 #pragma unroll
     for (int l = 0; l < L; ++l) encode_level<2>(table, gp, (uint32_t)l, x, enc + 2 * l);
+#pragma unroll
+    for (int e = 2 * L; e < E; ++e) enc[e] = 1.0f;
     float h[W];
     dense<E, W, true>(params, enc, h);
     float logit;
@@ -212,12 +216,18 @@ insta_status update_density_grid(insta_ctx* ctx, uint32_t slot, uint32_t n_frame
     const float* table = ctx->d_params + ctx->layout.n_mlp_params;
     const GridParams gp = grid_params(ctx);
     const dim3 sgrid((n + 127) / 128);
-    if (ctx->cfg.n_levels == 8)
-        hipLaunchKernelGGL(dg_splat_kernel<8>, sgrid, dim3(128), 0, stream, ctx->d_params, table, gp, d->pos, d->idx, n,
-                           ctx->cfg.dt, ctx->cfg.density_logit_bias, d->tmp);
-    else
-        hipLaunchKernelGGL(dg_splat_kernel<16>, sgrid, dim3(128), 0, stream, ctx->d_params, table, gp, d->pos, d->idx,
-                           n, ctx->cfg.dt, ctx->cfg.density_logit_bias, d->tmp);
+#define INSTA_SPLAT(L_, X_)                                                                                     \
+    hipLaunchKernelGGL((dg_splat_kernel<L_, X_>), sgrid, dim3(128), 0, stream, ctx->d_params, table, gp, d->pos,    \
+                       d->idx, n, ctx->cfg.dt, ctx->cfg.density_logit_bias, d->tmp)
+    const bool x16 = ctx->cfg.n_extra_dims != 0;
+    if (ctx->cfg.n_levels == 8) {
+        if (x16) INSTA_SPLAT(8, 16);
+        else INSTA_SPLAT(8, 0);
+    } else {
+        if (x16) INSTA_SPLAT(16, 16);
+        else INSTA_SPLAT(16, 0);
+    }
+#undef INSTA_SPLAT
     INSTA_HIP(hipGetLastError());
     hipLaunchKernelGGL(dg_ema_kernel, cgrid, block, 0, stream, d->grid, d->tmp, c.decay, cells);
     const uint32_t nb = (cells + kReduce - 1) / kReduce;

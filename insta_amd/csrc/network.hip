@@ -80,17 +80,20 @@ __global__ __launch_bounds__(kEncThreads) void encode_backward_kernel(GridParams
 struct NetArgs {
     const float* params;   // MLP weights followed by the hash tables
     const float* table;
+    const float* frame_extra;  // [max_frames][X] per-frame extra dims
     GridParams gp;
     uint32_t density_hidden, rgb_hidden;
+    uint32_t n_frames;         // rows of frame_extra
 };
 
 constexpr int kWidth = 64;
 constexpr int kDensityOut = 16;
 constexpr int kRgbIn = kDensityOut + 16;
 
-template <int L, int F>
-INSTA_DEV void network_one(const NetArgs& a, V3 pos, V3 dir_w, float out[4]) {
-    constexpr int DIN = L * F;
+// extra: the sample's X extra dims (null: all 1.0)
+template <int L, int F, int X>
+INSTA_DEV void network_one(const NetArgs& a, V3 pos, V3 dir_w, const float* extra, float out[4]) {
+    constexpr int DIN = L * F + X;
     float x[DIN];
 #pragma unroll
     for (int l = 0; l < L; ++l) {
@@ -99,6 +102,8 @@ INSTA_DEV void network_one(const NetArgs& a, V3 pos, V3 dir_w, float out[4]) {
 #pragma unroll
         for (int f = 0; f < F; ++f) x[l * F + f] = acc[f];
     }
+#pragma unroll
+    for (int e = 0; e < X; ++e) x[L * F + e] = extra ? extra[e] : 1.0f;
     float h[kWidth], g[kWidth];
     const float* W = a.params;
     dense<DIN, kWidth, true>(W, x, h);
@@ -129,7 +134,7 @@ INSTA_DEV void network_one(const NetArgs& a, V3 pos, V3 dir_w, float out[4]) {
     out[3] = r[0];
 }
 
-template <int L, int F>
+template <int L, int F, int X>
 __global__ __launch_bounds__(kNetThreads) void network_aos_kernel(NetArgs a, const float* __restrict__ coords,
                                                                   uint32_t n, uint32_t stride,
                                                                   float* __restrict__ out) {
@@ -137,11 +142,12 @@ __global__ __launch_bounds__(kNetThreads) void network_aos_kernel(NetArgs a, con
     if (i >= n) return;
     const float* c = coords + (size_t)i * stride;
     float o[4];
-    network_one<L, F>(a, v3(c[0], c[1], c[2]), v3(c[8], c[9], c[10]), o);
+    network_one<L, F, X>(a, v3(c[0], c[1], c[2]), v3(c[8], c[9], c[10]), c + stride - X, o);
     reinterpret_cast<float4*>(out)[i] = make_float4(o[0], o[1], o[2], o[3]);
 }
 
-template <int L, int F>
+// the extra dims come from the selector in cdir.w (0: all 1.0, 1 + f: frame f)
+template <int L, int F, int X>
 __global__ __launch_bounds__(kNetThreads) void network_soa_kernel(NetArgs a, const uint32_t* __restrict__ n_dev,
                                                                   const float4* __restrict__ cpos,
                                                                   const float4* __restrict__ cdir,
@@ -149,8 +155,11 @@ __global__ __launch_bounds__(kNetThreads) void network_soa_kernel(NetArgs a, con
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= *n_dev) return;
     const float4 p = cpos[i], d = cdir[i];
+    const uint32_t sel = X ? __float_as_uint(d.w) : kExtraOnes;
+    const float* extra =
+        sel != kExtraOnes && sel <= a.n_frames ? a.frame_extra + (size_t)(sel - 1u) * X : nullptr;
     float o[4];
-    network_one<L, F>(a, v3(p.x, p.y, p.z), v3(d.x, d.y, d.z), o);
+    network_one<L, F, X>(a, v3(p.x, p.y, p.z), v3(d.x, d.y, d.z), extra, o);
     out[i] = make_float4(o[0], o[1], o[2], o[3]);
 }
 
@@ -158,6 +167,8 @@ NetArgs net_args(const insta_ctx* ctx) {
     NetArgs a;
     a.params = ctx->d_params;
     a.table = ctx->d_params + ctx->layout.n_mlp_params;
+    a.frame_extra = ctx->d_frame_extra;
+    a.n_frames = ctx->cfg.max_frames;
     a.gp = grid_params(ctx);
     a.density_hidden = ctx->cfg.density_hidden_layers;
     a.rgb_hidden = ctx->cfg.rgb_hidden_layers;
@@ -179,17 +190,18 @@ GridParams grid_params(const insta_ctx* ctx) {
 }
 
 bool network_supported(const insta_config& c) {
-    const bool shape = c.n_neurons == 64 && c.density_out == 16 && c.sh_degree == 4 && c.n_extra_dims == 0 &&
+    const bool shape = c.n_neurons == 64 && c.density_out == 16 && c.sh_degree == 4 &&
+                       (c.n_extra_dims == 0 || c.n_extra_dims == 16) &&
                        c.density_hidden_layers >= 1 && c.rgb_hidden_layers >= 1;
     const bool grid = c.n_features_per_level == 2 && (c.n_levels == 8 || c.n_levels == 16);
     return shape && grid;
 }
 
-#define INSTA_DISPATCH_LF(L_, F_, KERNEL, ...)                                             \
-    if (ctx->cfg.n_levels == L_ && ctx->cfg.n_features_per_level == F_) {                   \
-        hipLaunchKernelGGL((KERNEL<L_, F_>), __VA_ARGS__);                                  \
-        INSTA_HIP(hipGetLastError());                                                       \
-        return INSTA_OK;                                                                    \
+#define INSTA_DISPATCH_LF(L_, F_, X_, KERNEL, ...)                                                  \
+    if (ctx->cfg.n_levels == L_ && ctx->cfg.n_features_per_level == F_ && ctx->cfg.n_extra_dims == X_) { \
+        hipLaunchKernelGGL((KERNEL<L_, F_, X_>), __VA_ARGS__);                                       \
+        INSTA_HIP(hipGetLastError());                                                                \
+        return INSTA_OK;                                                                             \
     }
 
 insta_status launch_network_aos(insta_ctx* ctx, const float* coords, uint32_t n, uint32_t stride, float* out,
@@ -197,8 +209,10 @@ insta_status launch_network_aos(insta_ctx* ctx, const float* coords, uint32_t n,
     if (n == 0) return INSTA_OK;
     const NetArgs a = net_args(ctx);
     const dim3 grid((n + kNetThreads - 1) / kNetThreads), block(kNetThreads);
-    INSTA_DISPATCH_LF(8, 2, network_aos_kernel, grid, block, 0, stream, a, coords, n, stride, out)
-    INSTA_DISPATCH_LF(16, 2, network_aos_kernel, grid, block, 0, stream, a, coords, n, stride, out)
+    INSTA_DISPATCH_LF(8, 2, 0, network_aos_kernel, grid, block, 0, stream, a, coords, n, stride, out)
+    INSTA_DISPATCH_LF(16, 2, 0, network_aos_kernel, grid, block, 0, stream, a, coords, n, stride, out)
+    INSTA_DISPATCH_LF(8, 2, 16, network_aos_kernel, grid, block, 0, stream, a, coords, n, stride, out)
+    INSTA_DISPATCH_LF(16, 2, 16, network_aos_kernel, grid, block, 0, stream, a, coords, n, stride, out)
     set_error("network: unsupported (n_levels, n_features_per_level)");
     return INSTA_ERR_UNSUPPORTED;
 }
@@ -208,8 +222,10 @@ insta_status launch_network_soa(insta_ctx* ctx, const uint32_t* n_dev, uint32_t 
     if (n_max == 0) return INSTA_OK;
     const NetArgs a = net_args(ctx);
     const dim3 grid((n_max + kNetThreads - 1) / kNetThreads), block(kNetThreads);
-    INSTA_DISPATCH_LF(8, 2, network_soa_kernel, grid, block, 0, stream, a, n_dev, cpos, cdir, out)
-    INSTA_DISPATCH_LF(16, 2, network_soa_kernel, grid, block, 0, stream, a, n_dev, cpos, cdir, out)
+    INSTA_DISPATCH_LF(8, 2, 0, network_soa_kernel, grid, block, 0, stream, a, n_dev, cpos, cdir, out)
+    INSTA_DISPATCH_LF(16, 2, 0, network_soa_kernel, grid, block, 0, stream, a, n_dev, cpos, cdir, out)
+    INSTA_DISPATCH_LF(8, 2, 16, network_soa_kernel, grid, block, 0, stream, a, n_dev, cpos, cdir, out)
+    INSTA_DISPATCH_LF(16, 2, 16, network_soa_kernel, grid, block, 0, stream, a, n_dev, cpos, cdir, out)
     set_error("network: unsupported (n_levels, n_features_per_level)");
     return INSTA_ERR_UNSUPPORTED;
 }

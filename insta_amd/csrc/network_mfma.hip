@@ -19,6 +19,12 @@
 //         tables stored FP16, accumulation FP32
 // Shapes: width 64, density 1 hidden layer -> 16 outputs, colour input 16 +
 // 16 SH, 2 hidden layers -> 3 outputs (the workload MLPs).
+// Extra dims (16 expression values appended to the density input): a
+// sample's 16 values are either all 1.0 or its frame's row, so their share of
+// the first layer is one of n_frames + 1 vectors W0[:, enc:] x, computed in
+// FP32 when the weights or a frame's values change (extra_bias_kernel) and
+// loaded as the first layer's starting accumulator; the per-sample selector
+// (0: ones, 1 + f: frame f) rides in the canonical direction's w.
 #include "context.hpp"
 
 namespace insta {
@@ -50,13 +56,15 @@ constexpr int kDOut = 16;
 // ---------------------------------------------------------------------------
 struct MlpOffsets {
     uint32_t w0, w1, r0, r1, r2;  // float offsets of each matrix in params
-    uint32_t din;
+    uint32_t din;                 // W0 row length: encoding + extra dims
+    uint32_t enc;                 // encoding columns (the MFMA part of W0)
 };
 
 MlpOffsets mlp_offsets(const insta_ctx* ctx) {
     MlpOffsets o;
     const uint32_t din = ctx->layout.density_in;
     o.din = din;
+    o.enc = ctx->cfg.n_levels * ctx->cfg.n_features_per_level;
     o.w0 = 0;
     o.w1 = o.w0 + kW * din;
     o.r0 = o.w1 + kDOut * kW;
@@ -74,12 +82,12 @@ __global__ void prep_f32_kernel(const float* __restrict__ p, MlpOffsets o, float
     const uint32_t f = blockIdx.x, l = threadIdx.x;
     if (f >= n_frag) return;
     const uint32_t m = l % 16, g = l / 16;
-    const uint32_t kb0 = (o.din + 3) / 4;
+    const uint32_t kb0 = (o.enc + 3) / 4;
     uint32_t idx = f;
     float v = 0.0f;
     if (idx < 4 * kb0) {  // density L0: tile t, block kb (natural k order)
         const uint32_t t = idx / kb0, kb = idx % kb0;
-        v = wget(p, o.w0, kW, o.din, 16 * t + m, 4 * kb + g);
+        v = wget(p, o.w0, kW, o.din, 16 * t + m, 4 * kb + g < o.enc ? 4 * kb + g : o.din);
     } else if ((idx -= 4 * kb0) < 16) {  // density out: block (t = idx/4, i = idx%4)
         const uint32_t t = idx / 4, i = idx % 4;
         v = wget(p, o.w1, kDOut, kW, m, 16 * t + 4 * g + i);
@@ -102,14 +110,14 @@ __global__ void prep_x32_kernel(const float* __restrict__ p, MlpOffsets o, T* __
     const uint32_t f = blockIdx.x, l = threadIdx.x;
     if (f >= n_frag) return;
     const uint32_t m = l % 16, g = l / 16;
-    const uint32_t kb0 = (o.din + 31) / 32;
+    const uint32_t kb0 = (o.enc + 31) / 32;
     for (uint32_t j = 0; j < 8; ++j) {
         uint32_t idx = f;
         float v;
         const uint32_t kc = j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4);  // chained k within a 32 block
         if (idx < 4 * kb0) {
-            const uint32_t t = idx / kb0, kb = idx % kb0;
-            v = wget(p, o.w0, kW, o.din, 16 * t + m, 32 * kb + 8 * g + j);
+            const uint32_t t = idx / kb0, kb = idx % kb0, k = 32 * kb + 8 * g + j;
+            v = wget(p, o.w0, kW, o.din, 16 * t + m, k < o.enc ? k : o.din);
         } else if ((idx -= 4 * kb0) < 2) {
             v = wget(p, o.w1, kDOut, kW, m, 32 * idx + kc);
         } else if ((idx -= 2) < 4) {
@@ -125,6 +133,25 @@ __global__ void prep_x32_kernel(const float* __restrict__ p, MlpOffsets o, T* __
 }
 
 constexpr uint32_t kFragX32 = 4 + 2 + 4 + 8 + 2;  // with DIN <= 32
+
+// rows [first, first + n) of the extra-dims table: row 0 = W0[:, enc:] . 1,
+// row 1 + f = W0[:, enc:] . (frame f's values); one thread per output neuron
+__global__ void extra_bias_kernel(const float* __restrict__ p, MlpOffsets o, uint32_t n_extra,
+                                  const float* __restrict__ frame_extra, uint32_t first, float* __restrict__ bias) {
+    const uint32_t r = first + blockIdx.x, j = threadIdx.x;
+    const float* w = p + o.w0 + (size_t)j * o.din + o.enc;
+    const float* x = r ? frame_extra + (size_t)(r - 1) * n_extra : nullptr;
+    float acc = 0.0f;
+    for (uint32_t e = 0; e < n_extra; ++e) acc = __builtin_fmaf(w[e], x ? x[e] : 1.0f, acc);
+    bias[(size_t)r * kW + j] = acc;
+}
+
+// frame values from the first n_extra expression coefficients (zero-padded)
+__global__ void extra_from_coeffs_kernel(const float* __restrict__ coeffs, uint32_t n_coeffs, uint32_t n_extra,
+                                         float* __restrict__ dst) {
+    const uint32_t e = threadIdx.x;
+    if (e < n_extra) dst[e] = e < n_coeffs ? coeffs[e] : 0.0f;
+}
 
 __global__ void table_to_half_kernel(const float* __restrict__ src, _Float16* __restrict__ dst, uint64_t n) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -145,6 +172,7 @@ struct SoaSrc {
         p = v3(a.x, a.y, a.z);
         d = v3(b.x, b.y, b.z);
     }
+    INSTA_DEV uint32_t sel(uint32_t i) const { return __float_as_uint(cdir[i].w); }
 };
 struct AosSrc {
     const float* coords;
@@ -156,6 +184,7 @@ struct AosSrc {
         p = v3(c[0], c[1], c[2]);
         d = v3(c[8], c[9], c[10]);
     }
+    INSTA_DEV uint32_t sel(uint32_t) const { return 0u; }  // extra dims take the scalar kernel
 };
 
 INSTA_DEV void sh16_dev(float x, float y, float z, float* o) {
@@ -252,10 +281,13 @@ struct XcdTiles {
     }
 };
 
-template <int L, class Src>
+template <int L, int X, class Src>
 __global__ __launch_bounds__(256) void network_f32_kernel(const float* __restrict__ table, GridParams gp,
-                                                          const float* __restrict__ frag, Src src) {
+                                                          const float* __restrict__ frag,
+                                                          const float* __restrict__ xbias, uint32_t xrows,
+                                                          Src src) {
     constexpr int DIN = L * 2;
+    static_assert(DIN < kF32Stride, "LDS row holds the features and the extra-dims selector");
     constexpr int KB0 = DIN / 4;
     __shared__ float s_x[kTile * kF32Stride];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
@@ -279,6 +311,10 @@ __global__ __launch_bounds__(256) void network_f32_kernel(const float* __restric
             } else {
                 for (int k = 0; k < DIN; ++k) row[k] = 0.0f;
             }
+            if constexpr (X > 0) {
+                const uint32_t sel = i < n ? src.sel(i) : 0u;
+                row[kF32Stride - 1] = __uint_as_float(sel < xrows ? sel : 0u);  // rows of xbias only
+            }
         }
         __syncthreads();
         for (uint32_t nb = 0; nb < 4; ++nb) {
@@ -296,10 +332,13 @@ __global__ __launch_bounds__(256) void network_f32_kernel(const float* __restric
 #pragma unroll
             for (int b = 0; b < 4; ++b) shg[b] = g == 0 ? sh[b] : (g == 1 ? sh[4 + b] : (g == 2 ? sh[8 + b] : sh[12 + b]));
             const float* W = frag + lane;
+            const float* xb = xbias;
+            if constexpr (X > 0) xb += (size_t)__float_as_uint(xr[kF32Stride - 1]) * kW + 4 * g;
             f4 h[4];
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
                 f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+                if constexpr (X > 0) acc = *reinterpret_cast<const f4*>(xb + 16 * t);
 #pragma unroll
                 for (int kb = 0; kb < KB0; ++kb)
                     acc = __builtin_amdgcn_mfma_f32_16x16x4f32(W[(t * KB0 + kb) * 64], xr[4 * kb + g], acc, 0, 0, 0);
@@ -379,13 +418,14 @@ INSTA_DEV typename Vec8<T>::type pack_relu(f4 a, f4 b) {
     return v;
 }
 
-template <int L, typename T, class Src>
+template <int L, int X, typename T, class Src>
 __global__ __launch_bounds__(256, 4) void network_x32_kernel(const _Float16* __restrict__ table, GridParams gp,
-                                                          const T* __restrict__ frag, Src src) {
+                                                          const T* __restrict__ frag, const float* __restrict__ xbias,
+                                                          uint32_t xrows, Src src) {
     typedef typename Vec8<T>::type V8;
     constexpr int DIN = L * 2;
     static_assert(DIN <= 32, "one K block for the first layer");
-    __shared__ T s_x[kTile * kX32Stride];
+    __shared__ __attribute__((aligned(16))) T s_x[kTile * kX32Stride];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t g = lane >> 4, col = lane & 15u;
     // weight fragments into registers (kFragX32 x 8 values per lane)
@@ -417,6 +457,11 @@ __global__ __launch_bounds__(256, 4) void network_x32_kernel(const _Float16* __r
             } else {
                 for (int k = 0; k < 48; ++k) row[k] = (T)0.0f;
             }
+            // extra-dims selector in the row's padding (halfs 48-49, 4-byte aligned)
+            if constexpr (X > 0) {
+                const uint32_t sel = i < n ? src.sel(i) : 0u;
+                *reinterpret_cast<uint32_t*>(row + 48) = sel < xrows ? sel : 0u;  // rows of xbias only
+            }
         }
         __syncthreads();
         for (uint32_t nb = 0; nb < 4; ++nb) {
@@ -426,8 +471,14 @@ __global__ __launch_bounds__(256, 4) void network_x32_kernel(const _Float16* __r
 #pragma unroll
             for (int j = 0; j < 8; ++j) bx[j] = xr[8 * g + j];
             f4 h[4];
+            if constexpr (X > 0) {
+                const float* xb = xbias + (size_t)*reinterpret_cast<const uint32_t*>(xr + 48) * kW + 4 * g;
 #pragma unroll
-            for (int t = 0; t < 4; ++t) h[t] = Vec8<T>::mfma(A[t], bx, f4{0.0f, 0.0f, 0.0f, 0.0f});
+                for (int t = 0; t < 4; ++t) h[t] = Vec8<T>::mfma(A[t], bx, *reinterpret_cast<const f4*>(xb + 16 * t));
+            } else {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) h[t] = Vec8<T>::mfma(A[t], bx, f4{0.0f, 0.0f, 0.0f, 0.0f});
+            }
             f4 dn = Vec8<T>::mfma(A[4], pack_relu<T>(h[0], h[1]), f4{0.0f, 0.0f, 0.0f, 0.0f});
             dn = Vec8<T>::mfma(A[5], pack_relu<T>(h[2], h[3]), dn);
             V8 rin;
@@ -464,18 +515,39 @@ uint32_t grid_for(uint32_t n_max) {
 }  // namespace
 
 bool mfma_network_supported(const insta_config& c) {
-    return c.precision != INSTA_PRECISION_FP32_SCALAR && c.n_neurons == 64 && c.density_out == 16 && c.sh_degree == 4 && c.n_extra_dims == 0 &&
+    return c.precision != INSTA_PRECISION_FP32_SCALAR && c.n_neurons == 64 && c.density_out == 16 && c.sh_degree == 4 &&
+           (c.n_extra_dims == 0 || c.n_extra_dims == 16) &&
            c.density_hidden_layers == 1 && c.rgb_hidden_layers == 2 && c.n_features_per_level == 2 &&
            (c.n_levels == 8 || c.n_levels == 16);
 }
 
-// Called after the FP32 parameters change: arrange weight fragments and
-// (reduced precision) convert the hash tables.
+insta_status update_extra_rows(insta_ctx* ctx, uint32_t first, uint32_t n, hipStream_t stream) {
+    if (!ctx->d_extra_bias || n == 0) return INSTA_OK;
+    hipLaunchKernelGGL(extra_bias_kernel, dim3(n), dim3(kW), 0, stream, ctx->d_params, mlp_offsets(ctx),
+                       ctx->cfg.n_extra_dims, ctx->d_frame_extra, first, ctx->d_extra_bias);
+    INSTA_HIP(hipGetLastError());
+    return INSTA_OK;
+}
+
+insta_status launch_extra_from_coeffs(insta_ctx* ctx, uint32_t frame, const float* coeffs, uint32_t n_coeffs,
+                                      hipStream_t stream) {
+    if (!ctx->d_frame_extra) return INSTA_OK;
+    const uint32_t X = ctx->cfg.n_extra_dims;
+    hipLaunchKernelGGL(extra_from_coeffs_kernel, dim3(1), dim3(64), 0, stream, coeffs, n_coeffs, X,
+                       ctx->d_frame_extra + (size_t)frame * X);
+    INSTA_HIP(hipGetLastError());
+    return update_extra_rows(ctx, frame + 1, 1, stream);
+}
+
+// Called after the FP32 parameters change: arrange weight fragments,
+// (reduced precision) convert the hash tables, refresh the extra-dims table.
 insta_status prepare_network(insta_ctx* ctx, hipStream_t stream) {
+    insta_status st = update_extra_rows(ctx, 0, ctx->n_slots, stream);
+    if (st != INSTA_OK) return st;
     if (!mfma_network_supported(ctx->cfg)) return INSTA_OK;
     const MlpOffsets o = mlp_offsets(ctx);
     if (ctx->cfg.precision == INSTA_PRECISION_FP32) {
-        const uint32_t nfrag = 4 * ((o.din + 3) / 4) + 16 + 32 + 64 + 16;
+        const uint32_t nfrag = 4 * ((o.enc + 3) / 4) + 16 + 32 + 64 + 16;
         hipLaunchKernelGGL(prep_f32_kernel, dim3(nfrag), dim3(64), 0, stream, ctx->d_params, o,
                            static_cast<float*>(ctx->d_wfrag), nfrag);
     } else {
@@ -493,30 +565,32 @@ insta_status prepare_network(insta_ctx* ctx, hipStream_t stream) {
     return INSTA_OK;
 }
 
-template <class Src>
-static insta_status launch_mfma(insta_ctx* ctx, Src src, uint32_t n_max, hipStream_t stream) {
+template <int X, class Src>
+static insta_status launch_mfma_x(insta_ctx* ctx, Src src, uint32_t n_max, hipStream_t stream) {
     const GridParams gp = grid_params(ctx);
     const dim3 grid(grid_for(n_max)), block(256);
     const float* table = ctx->d_params + ctx->layout.n_mlp_params;
     const _Float16* table_h = static_cast<const _Float16*>(ctx->d_table_h);
+    const float* xb = ctx->d_extra_bias;
+    const uint32_t xr = ctx->n_slots;  // rows of the extra-dims table
     const uint32_t L = ctx->cfg.n_levels;
     switch (ctx->cfg.precision) {
         case INSTA_PRECISION_FP32: {
             const float* fr = static_cast<const float*>(ctx->d_wfrag);
-            if (L == 8) hipLaunchKernelGGL((network_f32_kernel<8, Src>), grid, block, 0, stream, table, gp, fr, src);
-            else hipLaunchKernelGGL((network_f32_kernel<16, Src>), grid, block, 0, stream, table, gp, fr, src);
+            if (L == 8) hipLaunchKernelGGL((network_f32_kernel<8, X, Src>), grid, block, 0, stream, table, gp, fr, xb, xr, src);
+            else hipLaunchKernelGGL((network_f32_kernel<16, X, Src>), grid, block, 0, stream, table, gp, fr, xb, xr, src);
             break;
         }
         case INSTA_PRECISION_FP16: {
             const _Float16* fr = static_cast<const _Float16*>(ctx->d_wfrag);
-            if (L == 8) hipLaunchKernelGGL((network_x32_kernel<8, _Float16, Src>), grid, block, 0, stream, table_h, gp, fr, src);
-            else hipLaunchKernelGGL((network_x32_kernel<16, _Float16, Src>), grid, block, 0, stream, table_h, gp, fr, src);
+            if (L == 8) hipLaunchKernelGGL((network_x32_kernel<8, X, _Float16, Src>), grid, block, 0, stream, table_h, gp, fr, xb, xr, src);
+            else hipLaunchKernelGGL((network_x32_kernel<16, X, _Float16, Src>), grid, block, 0, stream, table_h, gp, fr, xb, xr, src);
             break;
         }
         default: {
             const __bf16* fr = static_cast<const __bf16*>(ctx->d_wfrag);
-            if (L == 8) hipLaunchKernelGGL((network_x32_kernel<8, __bf16, Src>), grid, block, 0, stream, table_h, gp, fr, src);
-            else hipLaunchKernelGGL((network_x32_kernel<16, __bf16, Src>), grid, block, 0, stream, table_h, gp, fr, src);
+            if (L == 8) hipLaunchKernelGGL((network_x32_kernel<8, X, __bf16, Src>), grid, block, 0, stream, table_h, gp, fr, xb, xr, src);
+            else hipLaunchKernelGGL((network_x32_kernel<16, X, __bf16, Src>), grid, block, 0, stream, table_h, gp, fr, xb, xr, src);
             break;
         }
     }
@@ -528,14 +602,18 @@ insta_status launch_network_mfma_soa(insta_ctx* ctx, const uint32_t* n_dev, uint
                                      const float4* cdir, float4* out, hipStream_t stream) {
     if (n_max == 0) return INSTA_OK;
     SoaSrc s{cpos, cdir, out, n_dev};
-    return launch_mfma(ctx, s, n_max, stream);
+    return ctx->cfg.n_extra_dims ? launch_mfma_x<16>(ctx, s, n_max, stream) : launch_mfma_x<0>(ctx, s, n_max, stream);
 }
 
 insta_status launch_network_mfma_aos(insta_ctx* ctx, const float* coords, uint32_t n, uint32_t stride, float* out,
                                      hipStream_t stream) {
     if (n == 0) return INSTA_OK;
+    if (ctx->cfg.n_extra_dims) {
+        set_error("network_inference: extra dims take the per-sample FP32 kernel");
+        return INSTA_ERR_UNSUPPORTED;
+    }
     AosSrc s{coords, stride, n, reinterpret_cast<float4*>(out)};
-    return launch_mfma(ctx, s, n, stream);
+    return launch_mfma_x<0>(ctx, s, n, stream);
 }
 
 }  // namespace insta

@@ -139,6 +139,8 @@ insta_status prepare(insta_server* S, insta_server::Slot& sl, const insta_server
     const uint32_t slot = sl.index + 1;
     insta_status st = launch_blend_expression(ctx, slot, av.bases, av.n_bases, sl.d_coeffs, stream);
     if (st != INSTA_OK) return st;
+    st = launch_extra_from_coeffs(ctx, sl.index, sl.d_coeffs, av.n_bases, stream);  // no-op without extra dims
+    if (st != INSTA_OK) return st;
     st = launch_set_built(ctx, slot, 1, stream);
     if (st != INSTA_OK) return st;
     st = launch_occupancy(ctx, slot, S->cfg.occupancy_threshold, stream);

@@ -30,9 +30,13 @@
 //             sum over chunks: deterministic MLP gradients
 //   adam      Kingma & Ba with bias correction over all parameters, after
 //             removing the loss scale
+// Extra dims (16 per frame) enter the density MLP as 16 more inputs, 1.0 or
+// the ray's frame values by the selector the deformation writes; their
+// columns of W0 get gradients like the encoding's, the values themselves
+// none (latent-code optimisation is off in the reference configuration).
 // Declared simplifications against the reference: no density/colour
-// regularisers, no error map, no extra dims, background not passed through
-// sRGB<->linear, rays never masked out.
+// regularisers, no error map, background not passed through sRGB<->linear,
+// rays never masked out.
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
@@ -236,28 +240,36 @@ INSTA_DEV void store_rows(float* __restrict__ act, uint32_t ld, int row0, uint32
 // Forward with stored activations over the compacted batch; the arithmetic
 // is that of the plain FP32 inference kernel (same FMA chains).  The output
 // layer is not needed: dL/doutput comes from the loss kernel.
-template <int L>
+template <int L, int X>
 __global__ __launch_bounds__(kTrainThreads) void train_forward_kernel(const float* __restrict__ params,
                                                                       const float* __restrict__ table, GridParams gp,
+                                                                      const float* __restrict__ frame_extra,
+                                                                      uint32_t n_frames,
                                                                       const uint32_t* __restrict__ n_dev,
                                                                       const uint32_t* __restrict__ c_idx,
                                                                       const float4* __restrict__ cpos,
                                                                       const float4* __restrict__ cdir,
                                                                       float* __restrict__ act, uint32_t ld) {
-    constexpr int E = 2 * L;
-    using A = Act<E>;
-    using O = Off<E>;
+    constexpr int E = 2 * L, DIN = E + X;
+    using A = Act<DIN>;
+    using O = Off<DIN>;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= *n_dev) return;
     const uint32_t smp = c_idx[i];
     const float4 p4 = cpos[smp], d4 = cdir[smp];
     const V3 x = v3(p4.x, p4.y, p4.z);
-    float enc[E];
+    float enc[DIN];
 #pragma unroll
     for (int l = 0; l < L; ++l) encode_level<2>(table, gp, (uint32_t)l, x, enc + 2 * l);
-    store_rows<E, E>(act, ld, A::ENC, i, enc);
+    if constexpr (X > 0) {
+        const uint32_t sel = __float_as_uint(d4.w);
+        const float* fx = sel != kExtraOnes && sel <= n_frames ? frame_extra + (size_t)(sel - 1u) * X : nullptr;
+#pragma unroll
+        for (int e = 0; e < X; ++e) enc[E + e] = fx ? fx[e] : 1.0f;
+    }
+    store_rows<DIN, DIN>(act, ld, A::ENC, i, enc);
     float h[kW], g[kW], r[kCin];
-    dense<E, kW, true>(params + O::W0, enc, h);
+    dense<DIN, kW, true>(params + O::W0, enc, h);
     store_rows<kW, kW>(act, ld, A::H1, i, h);
     dense<kW, kDOut, false>(params + O::W1, h, r);
     sh16(d4.x * 2.0f - 1.0f, d4.y * 2.0f - 1.0f, d4.z * 2.0f - 1.0f, r + kDOut);
@@ -392,7 +404,7 @@ INSTA_DEV float2 run_sum(uint32_t key, float2 v, bool& head) {
 // On the levels of run_mask, the corner gradients of consecutive samples that
 // land on one table entry (coarse levels along one ray) are summed across the
 // wave first and added by one lane.
-template <int L, bool H16>
+template <int L, bool H16, int X>
 __global__ __launch_bounds__(kMlpThreads) void train_backward_kernel(const float* __restrict__ params, GridParams gp,
                                                                        const uint32_t* __restrict__ n_dev,
                                                                        const uint32_t* __restrict__ c_idx,
@@ -403,9 +415,9 @@ __global__ __launch_bounds__(kMlpThreads) void train_backward_kernel(const float
                                                                        float* __restrict__ grad_tab,
                                                                        __half2* __restrict__ grad_tab_h,
                                                                        uint32_t run_mask) {
-    constexpr int E = 2 * L;
-    using A = Act<E>;
-    using O = Off<E>;
+    constexpr int E = 2 * L, DIN = E + X;  // encoding, density input (+ extra dims)
+    using A = Act<DIN>;
+    using O = Off<DIN>;
     __shared__ float w[O::N];  // both MLPs' weights (broadcast reads)
     for (uint32_t e = threadIdx.x; e < O::N; e += kMlpThreads) w[e] = params[e];
     __syncthreads();
@@ -465,7 +477,7 @@ __global__ __launch_bounds__(kMlpThreads) void train_backward_kernel(const float
         const float dh1 = act[(size_t)(A::H1 + k) * ld + ic] > 0.0f ? s : 0.0f;
         if (valid) dact[(size_t)(Dact::DH1 + k) * ld + i] = dh1;
 #pragma unroll
-        for (int e = 0; e < E; ++e) denc[e] = __builtin_fmaf(W0[k * E + e], dh1, denc[e]);
+        for (int e = 0; e < E; ++e) denc[e] = __builtin_fmaf(W0[k * DIN + e], dh1, denc[e]);
     }
     // dL/dencoding through LDS, so that the level loop indexes it at run time
     __shared__ float s_denc[E][kMlpThreads];
@@ -632,8 +644,8 @@ insta_status ensure_train_workspace(insta_ctx* ctx, TrainState* ts, uint32_t n_r
     const uint32_t rc = std::max(n_rays, ts->ray_cap), sc = std::max(n_samples, ts->sample_cap);
     const uint32_t cc = std::max(n_compact, ts->compact_cap);
     free_workspace_train(ts);
-    const int E = (int)(ctx->cfg.n_levels * 2);
-    const size_t act_rows = (size_t)E + 224, n_mlp = ctx->layout.n_mlp_params;
+    const int DIN = (int)(ctx->cfg.n_levels * 2 + ctx->cfg.n_extra_dims);
+    const size_t act_rows = (size_t)DIN + 224, n_mlp = ctx->layout.n_mlp_params;
     bool ok = true;
     ok = ok && train_alloc(ts->ray_d, rc * sizeof(float4)) && train_alloc(ts->ray_frame, rc * sizeof(int32_t));
     ok = ok && train_alloc(ts->ray_img, rc * 4) && train_alloc(ts->ray_pix, rc * 4) && train_alloc(ts->ray_src, rc * 4);
@@ -657,24 +669,25 @@ insta_status ensure_train_workspace(insta_ctx* ctx, TrainState* ts, uint32_t n_r
     return INSTA_OK;
 }
 
-template <int L>
+template <int L, int X>
 insta_status launch_train_forward(insta_ctx* ctx, TrainState* ts, hipStream_t stream) {
     const uint32_t cap = ts->compact_cap;
     const uint32_t* n_dev = ts->counters + kTCompactN;
     const float* table = ctx->d_params + ctx->layout.n_mlp_params;
     const GridParams gp = grid_params(ctx);
     const dim3 sgrid((cap + kTrainThreads - 1) / kTrainThreads);
-    hipLaunchKernelGGL(train_forward_kernel<L>, sgrid, dim3(kTrainThreads), 0, stream, ctx->d_params, table, gp, n_dev,
-                       ts->c_idx, ts->s_cpos, ts->s_cdir, ts->act, cap);
+    hipLaunchKernelGGL((train_forward_kernel<L, X>), sgrid, dim3(kTrainThreads), 0, stream, ctx->d_params, table, gp,
+                       ctx->d_frame_extra, ctx->cfg.max_frames, n_dev, ts->c_idx, ts->s_cpos, ts->s_cdir, ts->act,
+                       cap);
     INSTA_HIP(hipGetLastError());
     return INSTA_OK;
 }
 
-template <int L>
+template <int L, int X>
 insta_status launch_train_backward(insta_ctx* ctx, TrainState* ts, const insta_train_config& tc, hipStream_t stream) {
-    constexpr int E = 2 * L;
-    using A = Act<E>;
-    using O = Off<E>;
+    constexpr int DIN = 2 * L + X;
+    using A = Act<DIN>;
+    using O = Off<DIN>;
     const uint32_t cap = ts->compact_cap;
     const uint32_t* n_dev = ts->counters + kTCompactN;
     const GridParams gp = grid_params(ctx);
@@ -686,16 +699,16 @@ insta_status launch_train_backward(insta_ctx* ctx, TrainState* ts, const insta_t
     for (int l = 0; l < L; ++l)
         if (ctx->cfg.dt * gp.scale[l] <= kRunMaxStepCells) run_mask |= 1u << l;
     if (tc.grid_grad_fp16)
-        hipLaunchKernelGGL((train_backward_kernel<L, true>), sgrid, dim3(kMlpThreads), 0, stream, ctx->d_params, gp,
+        hipLaunchKernelGGL((train_backward_kernel<L, true, X>), sgrid, dim3(kMlpThreads), 0, stream, ctx->d_params, gp,
                            n_dev, ts->c_idx, ts->s_cpos, ts->act, ts->c_dout, ts->dact, cap, grad_tab, ts->grad_h,
                            run_mask);
     else
-        hipLaunchKernelGGL((train_backward_kernel<L, false>), sgrid, dim3(kMlpThreads), 0, stream, ctx->d_params, gp,
+        hipLaunchKernelGGL((train_backward_kernel<L, false, X>), sgrid, dim3(kMlpThreads), 0, stream, ctx->d_params, gp,
                            n_dev, ts->c_idx, ts->s_cpos, ts->act, ts->c_dout, ts->dact, cap, grad_tab, ts->grad_h,
                            run_mask);
     INSTA_HIP(hipGetLastError());
     WArgs wa;
-    wa.p[0] = WProduct{Dact::DH1, A::ENC, kW, E, O::W0};
+    wa.p[0] = WProduct{Dact::DH1, A::ENC, kW, DIN, O::W0};
     wa.p[1] = WProduct{Dact::DDENS, A::H1, kDOut, kW, O::W1};
     wa.p[2] = WProduct{Dact::DH2, A::DENS, kW, kCin, O::W2};
     wa.p[3] = WProduct{Dact::DH3, A::H2, kW, kW, O::W3};
@@ -713,7 +726,7 @@ insta_status launch_train_backward(insta_ctx* ctx, TrainState* ts, const insta_t
 }  // namespace
 
 bool train_supported(const insta_config& c) {
-    return c.n_neurons == 64 && c.density_out == 16 && c.sh_degree == 4 && c.n_extra_dims == 0 &&
+    return c.n_neurons == 64 && c.density_out == 16 && c.sh_degree == 4 && (c.n_extra_dims == 0 || c.n_extra_dims == 16) &&
            c.density_hidden_layers == 1 && c.rgb_hidden_layers == 2 && c.n_features_per_level == 2 &&
            (c.n_levels == 8 || c.n_levels == 16);
 }
@@ -846,10 +859,13 @@ insta_status train_step(insta_ctx* ctx, const insta_train_config& tc, insta_trai
                        ts->compact_cap, ts->counters);
     INSTA_HIP(hipGetLastError());
     hipLaunchKernelGGL(train_compact_count_kernel, dim3(1), dim3(64), 0, stream, ts->counters, ts->compact_cap);
-    st = ctx->cfg.n_levels == 8 ? launch_train_forward<8>(ctx, ts, stream) : launch_train_forward<16>(ctx, ts, stream);
+    const bool x16 = ctx->cfg.n_extra_dims != 0;
+    st = ctx->cfg.n_levels == 8 ? (x16 ? launch_train_forward<8, 16>(ctx, ts, stream) : launch_train_forward<8, 0>(ctx, ts, stream))
+                                : (x16 ? launch_train_forward<16, 16>(ctx, ts, stream) : launch_train_forward<16, 0>(ctx, ts, stream));
     if (st != INSTA_OK) return st;
-    st = ctx->cfg.n_levels == 8 ? launch_train_backward<8>(ctx, ts, tc, stream)
-                                : launch_train_backward<16>(ctx, ts, tc, stream);
+    st = ctx->cfg.n_levels == 8
+             ? (x16 ? launch_train_backward<8, 16>(ctx, ts, tc, stream) : launch_train_backward<8, 0>(ctx, ts, tc, stream))
+             : (x16 ? launch_train_backward<16, 16>(ctx, ts, tc, stream) : launch_train_backward<16, 0>(ctx, ts, tc, stream));
     if (st != INSTA_OK) return st;
     const uint32_t t = ts->step + (tc.apply_update ? 1u : 0u);
     const float bc1 = tc.apply_update ? 1.0f - (float)std::pow((double)tc.beta1, (double)t) : 1.0f;

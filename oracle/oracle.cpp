@@ -569,6 +569,7 @@ struct orc_scene {
     std::vector<insta_camera> tr_cams;
     std::vector<int32_t> tr_frames;
     int32_t tr_occ_slot = -2;     // >= -1: every training ray marches this slot's grid
+    std::vector<float> frame_extra;  // [max_frames][n_extra_dims] (Core::load_meshes' extra_dims_gpu)
 };
 
 namespace {
@@ -747,6 +748,16 @@ void network_one(const orc_scene* s, V3 pos, V3 dir_w, const float* extra, float
     out4[1] = rgb[1];
     out4[2] = rgb[2];
     out4[3] = rgb_in[0];
+}
+
+// Extra dims of a sample: the frame's values (set per sample by
+// generate_next_nerf_network_inputs / generate_training_samples_nerf,
+// testbed_nerf.cu:788, 1158, 1270) unless warp deformed it off the lips
+// region, which writes 1.0 (src/core.cu:514-519); no frame: 1.0.
+const float* sample_extra(const orc_scene* s, int32_t frame, bool warped, uint32_t mask) {
+    static const std::vector<float> ones(64, 1.0f);
+    if (frame < 0 || (warped && mask != 7u)) return ones.data();
+    return s->frame_extra.data() + (size_t)frame * s->cfg.n_extra_dims;
 }
 
 // ---------------------------------------------------------------------------
@@ -951,13 +962,15 @@ void dense_d(const float* W, const double* x, uint32_t n_in, uint32_t n_out, boo
     }
 }
 
-void network_one_d(const orc_scene* s, D3v pos, D3v dir, double* out4) {
+void network_one_d(const orc_scene* s, D3v pos, D3v dir, const float* extra, double* out4) {
     const insta_config& c = s->cfg;
     const insta_layout& L = s->layout;
     const uint32_t W = c.n_neurons;
     double din[INSTA_MAX_LEVELS * 8 + 64] = {0};
     double h0[256], h1[256], rgb_in[64];
     encode_one_d(s, pos, din);
+    const uint32_t enc = c.n_levels * c.n_features_per_level;
+    for (uint32_t e = 0; e < c.n_extra_dims; ++e) din[enc + e] = extra[e];
     const float* p = s->params.data();
     dense_d(p, din, L.density_in, W, true, h0);
     p += (uint64_t)W * L.density_in;
@@ -1011,6 +1024,7 @@ orc_scene* orc_create(const insta_config* cfg) {
     compute_layout(s->cfg, s->layout);
     s->frames.resize(cfg->max_frames);
     s->grids.resize(cfg->max_frames + 1);
+    s->frame_extra.assign((size_t)cfg->max_frames * cfg->n_extra_dims, 0.0f);
     return s;
 }
 
@@ -1074,6 +1088,13 @@ int orc_set_canonical_mesh(orc_scene* s, const float* verts, uint32_t nv, const 
 int orc_set_frame_vertices(orc_scene* s, uint32_t frame, const float* verts) {
     if (frame >= s->frames.size() || !s->canon.built) return 1;
     fill_frame(s, s->frames[frame], verts, &s->canon);
+    return 0;
+}
+
+int orc_set_frame_extra_dims(orc_scene* s, uint32_t first, uint32_t n, const float* values) {
+    const uint32_t X = s->cfg.n_extra_dims;
+    if (X == 0 || (uint64_t)first + n > s->frames.size()) return 1;
+    std::memcpy(s->frame_extra.data() + (size_t)first * X, values, (size_t)n * X * sizeof(float));
     return 0;
 }
 
@@ -1376,11 +1397,9 @@ int orc_render(const orc_scene* s, const insta_camera* cam, uint32_t width, uint
             // deform (warp) then network on the canonical sample
             V3 cp = pos, cd = dir_w;
             uint32_t mask = 0;
-            warp_one(s, *fr, pos, dir_w, cp, cd, mask);
+            const bool warped = warp_one(s, *fr, pos, dir_w, cp, cd, mask);
             float net[4];
-            float ones[64];
-            for (uint32_t e = 0; e < c.n_extra_dims && e < 64; ++e) ones[e] = (mask != 7u) ? 1.0f : 0.0f;
-            network_one(s, cp, cd, c.n_extra_dims ? ones : nullptr, net);
+            network_one(s, cp, cd, sample_extra(s, frame, warped, mask), net);
             // composite_kernel_nerf body
             float T = 1.0f - acc[3];
             float sigma = std::exp(net[3] + c.density_logit_bias);
@@ -1481,9 +1500,9 @@ int orc_render_fp64(const orc_scene* s, const insta_camera* cam, uint32_t width,
             if (!inside) break;
             D3v cp = dv(pos), cd = dird;
             uint32_t mask = 0;
-            warp_one_d(s, *fr, dv(pos), dird, cp, cd, mask);
+            const bool warped = warp_one_d(s, *fr, dv(pos), dird, cp, cd, mask);
             double net[4];
-            network_one_d(s, cp, cd, net);
+            network_one_d(s, cp, cd, sample_extra(s, frame, warped, mask), net);
             const double T = 1.0 - acc[3];
             const double sigma = std::exp(net[3] + (double)c.density_logit_bias);
             const double alpha = 1.0 - std::exp(-sigma * (double)DT);
@@ -1630,14 +1649,16 @@ struct SampleAct {
 };
 
 // forward of the supported shape (1 density / 2 colour hidden layers, width
-// 64, F = 2) in FP64, keeping the activations
-void forward_d(const orc_scene* s, D3v pos, D3v dir, SampleAct& a) {
-    const uint32_t E = s->cfg.n_levels * 2;
+// 64, F = 2) in FP64, keeping the activations; the density input is the
+// encoding followed by the sample's extra dims
+void forward_d(const orc_scene* s, D3v pos, D3v dir, const float* extra, SampleAct& a) {
+    const uint32_t E = s->cfg.n_levels * 2, DIN = E + s->cfg.n_extra_dims;
     a.pos = pos;
     encode_one_d(s, pos, a.enc);
+    for (uint32_t e = E; e < DIN; ++e) a.enc[e] = extra[e - E];
     const float* p = s->params.data();
-    dense_d(p, a.enc, E, 64, true, a.h1);
-    p += 64 * E;
+    dense_d(p, a.enc, DIN, 64, true, a.h1);
+    p += 64 * DIN;
     dense_d(p, a.h1, 64, 16, false, a.cin);
     p += 16 * 64;
     // SH of the direction (the closed forms of network_one_d)
@@ -1663,8 +1684,8 @@ void forward_d(const orc_scene* s, D3v pos, D3v dir, SampleAct& a) {
 
 // backward of one sample: accumulates dL/dparams into g (FP64)
 void backward_d(const orc_scene* s, const SampleAct& a, const double dout[4], double* g) {
-    const uint32_t E = s->cfg.n_levels * 2;
-    const uint64_t W0 = 0, W1 = W0 + 64 * E, W2 = W1 + 16 * 64, W3 = W2 + 64 * 32, W4 = W3 + 64 * 64;
+    const uint32_t E = s->cfg.n_levels * 2, DIN = E + s->cfg.n_extra_dims;
+    const uint64_t W0 = 0, W1 = W0 + 64 * DIN, W2 = W1 + 16 * 64, W3 = W2 + 64 * 32, W4 = W3 + 64 * 64;
     const float* P = s->params.data();
     double dh3[64], dh2[64], dd[16], dh1[64], denc[64];
     for (int c = 0; c < 3; ++c)
@@ -1697,10 +1718,10 @@ void backward_d(const orc_scene* s, const SampleAct& a, const double dout[4], do
         dh1[k] = a.h1[k] > 0.0 ? acc : 0.0;
     }
     for (int k = 0; k < 64; ++k)
-        for (uint32_t e = 0; e < E; ++e) g[W0 + k * E + e] += dh1[k] * a.enc[e];
-    for (uint32_t e = 0; e < E; ++e) {
+        for (uint32_t e = 0; e < DIN; ++e) g[W0 + k * DIN + e] += dh1[k] * a.enc[e];
+    for (uint32_t e = 0; e < E; ++e) {  // the extra dims get no gradient (no latent-code optimisation)
         double acc = 0.0;
-        for (int k = 0; k < 64; ++k) acc += (double)P[W0 + k * E + e] * dh1[k];
+        for (int k = 0; k < 64; ++k) acc += (double)P[W0 + k * DIN + e] * dh1[k];
         denc[e] = acc;
     }
     // hash-grid scatter with the FP64 trilinear weights of encode_one_d
@@ -1774,7 +1795,8 @@ int orc_train_grads(const orc_scene* s, uint32_t n_rays, uint32_t seed, uint64_t
                     double* loss, uint32_t* counts) {
     const insta_config& c = s->cfg;
     if (s->tr_n == 0 || s->params.empty() || c.n_features_per_level != 2 || c.n_neurons != 64 ||
-        c.density_hidden_layers != 1 || c.rgb_hidden_layers != 2 || c.density_out != 16 || c.n_extra_dims != 0)
+        c.density_hidden_layers != 1 || c.rgb_hidden_layers != 2 || c.density_out != 16 ||
+        c.n_levels * 2 + c.n_extra_dims > 64)
         return 1;
     Pcg32 rng0 = Pcg32::seeded(seed);
     rng0.advance(rays_total * kRaySeedStride);
@@ -1808,9 +1830,9 @@ int orc_train_grads(const orc_scene* s, uint32_t n_rays, uint32_t seed, uint64_t
             // frame); network, loss and backward in FP64 from there
             V3 cp = r.samples[j], cd = dir_w;
             uint32_t mask = 0;
-            warp_one(s, fr, r.samples[j], dir_w, cp, cd, mask);
+            const bool warped = warp_one(s, fr, r.samples[j], dir_w, cp, cd, mask);
             const D3v dird = {(double)cd.x * 2.0 - 1.0, (double)cd.y * 2.0 - 1.0, (double)cd.z * 2.0 - 1.0};
-            forward_d(s, dv(cp), dird, act[j]);
+            forward_d(s, dv(cp), dird, sample_extra(s, s->tr_frames[r.img], warped, mask), act[j]);
         }
         const float* tex = &s->tr_images[((size_t)r.img * s->tr_w * s->tr_h + r.pix) * 4];
         double target[3], bg[3];
@@ -1937,12 +1959,15 @@ void grid_sample(uint32_t i, uint32_t n, uint32_t step, float thresh, uint32_t G
     pos[2] = (((float)z + rz) / g - 0.5f) * 1.0f + 0.5f;
 }
 
-// density-MLP output 0 (the network's density() path)
+// density-MLP output 0 (the network's density() path: extra dims all 1.0,
+// extract_expressions_random, nerf_network.h:107-127)
 float density_logit(const orc_scene* s, V3 x) {
     const insta_config& c = s->cfg;
     const insta_layout& L = s->layout;
     float din[INSTA_MAX_LEVELS * 8 + 64], h[256], out;
     encode_one(s, x, din);
+    const uint32_t enc = c.n_levels * c.n_features_per_level;
+    for (uint32_t e = 0; e < c.n_extra_dims; ++e) din[enc + e] = 1.0f;
     dense(s->params.data(), din, L.density_in, c.n_neurons, true, h);
     dense(s->params.data() + (uint64_t)c.n_neurons * L.density_in, h, c.n_neurons, 1, false, &out);
     return out;

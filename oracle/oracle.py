@@ -25,6 +25,7 @@ _SIGS = [
     ("orc_query_layout", C.c_int, [C.POINTER(InstaConfig), C.POINTER(InstaLayout)]),
     ("orc_set_canonical_mesh", C.c_int, [_VP, _VP, C.c_uint32, _VP, C.c_uint32, _VP]),
     ("orc_set_frame_vertices", C.c_int, [_VP, C.c_uint32, _VP]),
+    ("orc_set_frame_extra_dims", C.c_int, [_VP, C.c_uint32, C.c_uint32, _VP]),
     ("orc_render_box", None, [_VP, _VP]),
     ("orc_adjacency", None, [_VP, _VP]),
     ("orc_frames", C.c_int, [_VP, C.c_int32, _VP, _VP, _VP]),
@@ -118,6 +119,11 @@ class Oracle:
     def set_frame(self, frame: int, verts) -> None:
         v = _c(verts, np.float32)
         assert self.L.orc_set_frame_vertices(self.h, frame, _p(v)) == 0
+
+    def set_frame_extra_dims(self, first: int, values) -> None:
+        v = _c(values, np.float32)
+        v = v.reshape(-1, v.shape[-1]) if v.ndim > 1 else v.reshape(1, -1)
+        assert self.L.orc_set_frame_extra_dims(self.h, first, len(v), _p(v)) == 0
 
     def render_box(self) -> np.ndarray:
         out = np.zeros(6, np.float32)
