@@ -26,8 +26,9 @@
 //             hash tables: FP32 atomics, or packed FP16 atomics (one
 //             global_atomic_pk_add_f16 per corner and level, F = 2)
 //   wgrad     weight gradients dW = sum_n delta_n x_n^T as per-chunk partial
-//             products (LDS tiles, 4x4 register blocks), then a fixed-order
-//             sum over chunks: deterministic MLP gradients
+//             products on FP32 MFMA (samples as the K dimension, operands
+//             staged through LDS), then a fixed-order sum over chunks:
+//             deterministic MLP gradients
 //   adam      Kingma & Ba with bias correction over all parameters, after
 //             removing the loss scale
 // Extra dims (16 per frame) enter the density MLP as 16 more inputs, 1.0 or
@@ -541,47 +542,74 @@ struct WArgs {
 };
 
 // partial[c][out + o*I + i] = sum over samples n of chunk c of
-// dact[d_row + o][n] * act[x_row + i][n]
+// dact[d_row + o][n] * act[x_row + i][n], on v_mfma_f32_16x16x4f32: the
+// samples are the K dimension; 32-sample slices of both operands are staged
+// through LDS (coalesced along the samples), wave w owns input tile i =
+// 16w..16w+15 and the four output tiles o = 0..63 of it, one MFMA per tile
+// and 4 samples (A = 16 outputs x 4 samples, B = 4 samples x 16 inputs).
+typedef float wg_f4 __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(256) void wgrad_kernel(WArgs wa, const uint32_t* __restrict__ n_dev,
                                                     const float* __restrict__ act, const float* __restrict__ dact,
                                                     uint32_t ld, float* __restrict__ part, uint32_t n_mlp) {
-    __shared__ float Ds[64][33];
-    __shared__ float Xs[64][33];
+    constexpr uint32_t kS = 32;   // samples per LDS slice
+    constexpr uint32_t kP = 36;   // LDS row pitch: fragment reads (16 rows x 4 samples) spread over the banks
+    constexpr uint32_t kE = 64u * kS / 256u;  // slice elements per thread and operand
+    __shared__ float Ds[64 * kP];
+    __shared__ float Xs[64 * kP];
     const WProduct pr = wa.p[blockIdx.y];
     const uint32_t n = *n_dev;
     const uint32_t s0 = blockIdx.x * kChunk;
     if (s0 >= n) return;
     const uint32_t s1 = min(s0 + kChunk, n);
-    const uint32_t t = threadIdx.x, to = t / 16u, ti = t % 16u;
-    float acc[4][4] = {};
-    for (uint32_t s = s0; s < s1; s += 32) {
-        for (uint32_t e = t; e < 64u * 32u; e += 256u) {
-            const uint32_t row = e / 32u, col = e % 32u, smp = s + col;
-            Ds[row][col] = (row < pr.O && smp < s1) ? dact[(size_t)(pr.d_row + row) * ld + smp] : 0.0f;
-            Xs[row][col] = (row < pr.I && smp < s1) ? act[(size_t)(pr.x_row + row) * ld + smp] : 0.0f;
+    const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+    const uint32_t m = lane & 15u, g = lane >> 4;   // fragment row / k of this lane
+    const uint32_t n_ot = (pr.O + 15u) / 16u;       // output tiles in use
+    const bool active = 16u * wave < pr.I;          // this wave's input tile exists
+    wg_f4 acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] = wg_f4{0.0f, 0.0f, 0.0f, 0.0f};
+    // next slice in registers while the current one is multiplied
+    float dn[kE], xn[kE];
+    auto fetch = [&](uint32_t s) {
+#pragma unroll
+        for (uint32_t q = 0; q < kE; ++q) {
+            const uint32_t e = t + 256u * q, row = e / kS, smp = s + e % kS;
+            dn[q] = (row < pr.O && smp < s1) ? dact[(size_t)(pr.d_row + row) * ld + smp] : 0.0f;
+            xn[q] = (row < pr.I && smp < s1) ? act[(size_t)(pr.x_row + row) * ld + smp] : 0.0f;
+        }
+    };
+    fetch(s0);
+    for (uint32_t s = s0; s < s1; s += kS) {
+#pragma unroll
+        for (uint32_t q = 0; q < kE; ++q) {
+            const uint32_t e = t + 256u * q;
+            Ds[(e / kS) * kP + e % kS] = dn[q];
+            Xs[(e / kS) * kP + e % kS] = xn[q];
         }
         __syncthreads();
-#pragma unroll 4
-        for (uint32_t col = 0; col < 32u; ++col) {
-            float dv[4], xv[4];
+        if (s + kS < s1) fetch(s + kS);
+        if (active) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                dv[q] = Ds[to + 16u * q][col];
-                xv[q] = Xs[ti + 16u * q][col];
+            for (uint32_t k0 = 0; k0 < kS; k0 += 4u) {
+                const float b = Xs[(16u * wave + m) * kP + k0 + g];
+#pragma unroll
+                for (uint32_t ot = 0; ot < 4u; ++ot)
+                    if (ot < n_ot)
+                        acc[ot] = __builtin_amdgcn_mfma_f32_16x16x4f32(Ds[(16u * ot + m) * kP + k0 + g], b, acc[ot],
+                                                                        0, 0, 0);
             }
-#pragma unroll
-            for (int a2 = 0; a2 < 4; ++a2)
-#pragma unroll
-                for (int b2 = 0; b2 < 4; ++b2) acc[a2][b2] = __builtin_fmaf(dv[a2], xv[b2], acc[a2][b2]);
         }
         __syncthreads();
     }
+    if (!active) return;
+    // lane holds C[o = 16 ot + 4 g + j][i = 16 wave + m]
+    const uint32_t ii = 16u * wave + m;
 #pragma unroll
-    for (int a2 = 0; a2 < 4; ++a2)
+    for (uint32_t ot = 0; ot < 4u; ++ot)
 #pragma unroll
-        for (int b2 = 0; b2 < 4; ++b2) {
-            const uint32_t o = to + 16u * a2, ii = ti + 16u * b2;
-            if (o < pr.O && ii < pr.I) part[(size_t)blockIdx.x * n_mlp + pr.out + o * pr.I + ii] = acc[a2][b2];
+        for (uint32_t j = 0; j < 4u; ++j) {
+            const uint32_t o = 16u * ot + 4u * g + j;
+            if (o < pr.O && ii < pr.I) part[(size_t)blockIdx.x * n_mlp + pr.out + o * pr.I + ii] = acc[ot][j];
         }
 }
 
