@@ -57,11 +57,11 @@ def per_sample_costs(cfg, lay):
     }
 
 
-def setup(args, precision=0, av=None):
+def setup(args, precision=0, av=None, n_extra=0):
     import torch
     from insta_amd import synth
     from insta_amd.capi import Context, full_config, make_camera
-    cfg = full_config(max_frames=16, precision=precision)
+    cfg = full_config(max_frames=16, precision=precision, n_extra_dims=n_extra)
     if av is None:
         av = synth.head_avatar(seed=1, n_frames=16)
     ctx = Context(cfg)
@@ -70,12 +70,20 @@ def setup(args, precision=0, av=None):
     ctx.set_frames(0, frames)
     params = synth.network_params(7, ctx.layout, cfg)
     ctx.set_params(torch.from_numpy(params).cuda())
+    if n_extra:  # per-frame expression values (seeded N(0, 1))
+        vals = np.random.default_rng(77).standard_normal((16, n_extra)).astype(np.float32)
+        ctx.set_frame_extra_dims(0, torch.from_numpy(vals).cuda())
     for f in range(16):
         ctx.build_occupancy(f, 0.015)
     rng = np.random.default_rng(2024)
     cams = [make_camera(synth.random_camera(rng, 0.7), synth.focal_from_fov(args.res, 30.0)) for _ in range(16)]
     torch.cuda.synchronize()
     return ctx, cfg, av, params, cams
+
+
+def phase(name):
+    """One stderr line per finished phase (the JSON result goes to stdout)."""
+    print(f"bench: {name} done", file=sys.stderr, flush=True)
 
 
 def timed_renders(ctx, cams, res, steps, warmup, out):
@@ -454,15 +462,27 @@ def main():
             forms[PRECISION_NAMES[prec]] = dict(frames_per_s=args.steps / e2, samples_per_s=s2 / e2,
                                                 network_ms_per_frame=p2["network"]["ms"] / args.steps)
             ctxs[PRECISION_NAMES[prec]] = c2
+        # the reference's main network shape: 16 expression values on the density input
+        c3, _, _, _, cams3 = setup(args, 1, av, n_extra=16)
+        e3, s3, _, p3 = timed_renders(c3, cams3, res, args.steps, args.warmup, out)
+        forms["fp16_extra_dims"] = dict(frames_per_s=args.steps / e3, samples_per_s=s3 / e3,
+                                        network_ms_per_frame=p3["network"]["ms"] / args.steps,
+                                        note="FP16 with n_extra_dims 16 (per-frame values, 1.0 off the lips)")
+        del c3
         forms["fp16_vs_fp32_frames"] = forms["fp16"]["frames_per_s"] / forms["fp32"]["frames_per_s"]
         forms["fp16_vs_fp32_scalar_frames"] = forms["fp16"]["frames_per_s"] / forms["fp32_scalar"]["frames_per_s"]
         if args.accuracy_pixels > 0:
             forms["accuracy_vs_fp64"] = accuracy_vs_fp64(ctxs, cfg, av, params, cams[0], res, args.accuracy_pixels)
 
+    phase("render and precision forms")
     train = None if args.skip_train else training_bench(args)
+    phase("training (small shapes)")
     train_full = None if args.skip_train else training_bench(args, full=True)
+    phase("training (full size)")
     serve = None if args.skip_serve else serving_bench(args)
+    phase("serving (small shapes)")
     serve_full = None if args.skip_serve else serving_bench(args, full=True)
+    phase("serving (full size)")
 
     cpu = None
     if args.cpu_budget > 0:
