@@ -18,14 +18,19 @@ Follows the reference's loaders for the parts this path uses:
   * read_flame_params (include/rta/helpers.h:30-46): one float per line, at
     most 75;
   * Core::load_meshes (src/core.cu:234-337): the canonical mesh is
-    "canonical.obj" next to the transforms file.
+    "canonical.obj" next to the transforms file;
+  * rta::Masking (src/masking.cpp:120-178): FLAME region masks from the
+    vertex colours of mesh_masks/{boundary, eye_region, face, left_ear, lips,
+    right_ear}.obj (a vertex is in a region when its colour is (>= 1, 0, 0));
+    a face takes the region of the highest FlameMasks id among the regions
+    that hold one of its vertices, else TORSO.
 
 Not covered: camera distortion and rolling shutter, EXR images, alpha /
 dynamic-mask side images, depth and segmentation supervision, ray files,
-dataset caching, the shuffled subset of training images, and the FLAME region
-masks (pass per-face mask ids; by default every face is TORSO, the plain
-per-triangle frame).  Images are returned as stored (value / 255), with no
-sRGB-to-linear conversion, as the training step takes its targets.
+dataset caching and the shuffled subset of training images.  Without a
+mesh_masks folder every face is TORSO (the plain per-triangle frame) unless
+per-face ids are passed.  Images are returned as stored (value / 255), with
+no sRGB-to-linear conversion, as the training step takes its targets.
 """
 from __future__ import annotations
 
@@ -38,6 +43,10 @@ import numpy as np
 
 from .capi import InstaCamera
 from .synth import MASK_TORSO
+
+# mesh_masks files and their FlameMasks ids (include/rta/masking.h:32-50)
+MESH_MASK_REGIONS = (("boundary", 0), ("eye_region", 1), ("face", 2), ("left_ear", 4), ("lips", 7),
+                     ("right_ear", 10))
 
 MESH_SCALE = 1.0
 MESH_OFFSET = 0.5
@@ -65,6 +74,39 @@ def read_obj(path) -> tuple[np.ndarray, np.ndarray]:
     if len(f) and (f.min() < 0 or f.max() >= len(v)):
         raise ValueError(f"{path}: face index out of range")
     return (v * np.float32(MESH_SCALE) + np.float32(MESH_OFFSET)).astype(np.float32), f
+
+
+def read_mask_vertices(path) -> tuple[np.ndarray, int]:
+    """(ids of the vertices coloured (>= 1, 0, 0), vertex count) of an OBJ
+    whose "v" lines carry x y z r g b (Masking::load_mesh_color_mask); a
+    vertex without a colour is not in the region."""
+    ids, n = [], 0
+    for line in Path(path).read_text().splitlines():
+        parts = line.split()
+        if not parts or parts[0] != "v":
+            continue
+        if len(parts) >= 7:
+            r, g, b = float(parts[4]), float(parts[5]), float(parts[6])
+            if r >= 1.0 and g == 0.0 and b == 0.0:
+                ids.append(n)
+        n += 1
+    return np.asarray(ids, dtype=np.int64), n
+
+
+def region_masks(faces: np.ndarray, n_vertices: int, mesh_masks_dir) -> np.ndarray:
+    """Per-face FlameMasks ids from a mesh_masks folder (Masking::mesh_to_mask
+    and triangle_to_mask): the regions are visited in id order and each one
+    that holds a vertex of the face overwrites its id, so the highest wins."""
+    faces = np.asarray(faces, dtype=np.int64)
+    out = np.full(len(faces), MASK_TORSO, np.uint32)
+    for name, rid in MESH_MASK_REGIONS:
+        ids, n = read_mask_vertices(Path(mesh_masks_dir) / f"{name}.obj")
+        if n != n_vertices:
+            raise ValueError(f"{name}.obj: {n} vertices, the canonical mesh has {n_vertices}")
+        member = np.zeros(n_vertices, bool)
+        member[ids] = True
+        out[member[faces].any(axis=1)] = rid
+    return out
 
 
 def read_flame_params(path, max_params: int = N_FLAME_PARAMS) -> np.ndarray:
@@ -135,13 +177,16 @@ def _image_size(path: Path) -> tuple[int, int]:
         return im.size
 
 
-def load_transforms(paths, masks: np.ndarray | None = None) -> Dataset:
+def load_transforms(paths, masks: np.ndarray | None = None, mesh_masks_dir=None) -> Dataset:
     """Read one or more transforms JSON files of one capture (see the module
     docstring for the keys honoured).  Frame i of the result is image i,
-    posed mesh i and expression i."""
+    posed mesh i and expression i.  Region masks: `masks` (one id per face),
+    else those of `mesh_masks_dir`, else TORSO everywhere."""
     paths = [Path(p) for p in ([paths] if isinstance(paths, (str, Path)) else paths)]
     base0 = paths[0].parent
     canon, faces = read_obj(base0 / "canonical.obj")
+    if masks is None and mesh_masks_dir is not None:
+        masks = region_masks(faces, len(canon), mesh_masks_dir)
     if masks is None:
         masks = np.full(len(faces), MASK_TORSO, np.uint32)
     masks = np.asarray(masks, dtype=np.uint32)

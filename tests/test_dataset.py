@@ -112,3 +112,75 @@ def test_inconsistent_captures_are_rejected(capture):
     (path / "nofocal.json").write_text(json.dumps(doc2))
     with pytest.raises(ValueError):
         dataset.load_transforms(path / "nofocal.json")
+
+
+def _write_mask_objs(folder, verts, faces, members, colours=None):
+    """mesh_masks/*.obj: one file per region, red (1, 0, 0) on its member
+    vertices, grey elsewhere; `colours` overrides (region, vertex) -> rgb."""
+    folder.mkdir(parents=True, exist_ok=True)
+    for name, _ in dataset.MESH_MASK_REGIONS:
+        lines = ["# mask"]
+        for i, v in enumerate(verts):
+            rgb = (1.0, 0.0, 0.0) if i in members.get(name, ()) else (0.588235, 0.588235, 0.588235)
+            rgb = (colours or {}).get((name, i), rgb)
+            lines.append("v %f %f %f" % tuple(v) + ("" if rgb is None else " %f %f %f" % tuple(rgb)))
+        lines += ["f %d %d %d" % tuple(f + 1) for f in faces]
+        (folder / f"{name}.obj").write_text("\n".join(lines) + "\n")
+
+
+def _triangle_to_mask_loops(faces, members):
+    """Masking::triangle_to_mask (src/masking.cpp:120-148) as its loops read:
+    regions in FlameMasks order, the first member vertex found in a face sets
+    the face's id, a later region overwrites an earlier one."""
+    out = [15] * len(faces)
+    for fi, face in enumerate(faces):
+        for name, rid in dataset.MESH_MASK_REGIONS:
+            for v in members.get(name, []):
+                if sum(int(face[k] == v) for k in range(3)) > 0:
+                    out[fi] = rid
+                    break
+    return out
+
+
+def test_region_masks_follow_the_reference_rule(tmp_path):
+    verts = np.random.default_rng(0).uniform(-0.1, 0.1, (9, 3))
+    faces = np.array([[0, 1, 2], [2, 3, 4], [5, 6, 7], [6, 7, 8], [1, 3, 5]])
+    members = {"lips": [0], "face": [0, 3], "right_ear": [4], "boundary": [6], "eye_region": [8]}
+    _write_mask_objs(tmp_path / "mesh_masks", verts, faces, members)
+    got = dataset.region_masks(faces, len(verts), tmp_path / "mesh_masks")
+    # lips (7) over face (2); right ear (10) over face; boundary only; eye
+    # region (1) over boundary (0); face (2) through vertex 3
+    assert got.tolist() == [7, 10, 0, 1, 2]
+    # colour threshold of load_mesh_color_mask: r >= 1, g == b == 0
+    colours = {("lips", 0): (0.99, 0.0, 0.0), ("face", 0): (1.0, 0.001, 0.0), ("face", 3): (1.5, 0.0, 0.0),
+               ("right_ear", 4): None, ("boundary", 6): (1.0, 0.0, 0.0)}
+    _write_mask_objs(tmp_path / "m2", verts, faces, members, colours)
+    got = dataset.region_masks(faces, len(verts), tmp_path / "m2")
+    assert got.tolist() == [15, 2, 0, 1, 2]
+
+
+def test_region_masks_match_the_reference_loops_on_a_random_mesh(tmp_path):
+    rng = np.random.default_rng(3)
+    nv = 300
+    verts = rng.uniform(-0.1, 0.1, (nv, 3))
+    faces = np.stack([rng.choice(nv, 3, replace=False) for _ in range(500)])
+    members = {name: sorted(rng.choice(nv, int(rng.integers(5, 60)), replace=False).tolist())
+               for name, _ in dataset.MESH_MASK_REGIONS}
+    _write_mask_objs(tmp_path / "mesh_masks", verts, faces, members)
+    got = dataset.region_masks(faces, nv, tmp_path / "mesh_masks")
+    assert got.tolist() == _triangle_to_mask_loops(faces, members)
+    assert len(set(got.tolist())) >= 5
+
+
+def test_capture_takes_its_masks_from_mesh_masks(capture, tmp_path):
+    path = capture[0] / "transforms_train.json"
+    ds0 = dataset.load_transforms(path)
+    members = {"lips": list(range(0, len(ds0.canonical), 7))}
+    _write_mask_objs(tmp_path / "mesh_masks", ds0.canonical, ds0.faces, members)
+    ds = dataset.load_transforms(path, mesh_masks_dir=tmp_path / "mesh_masks")
+    lips_faces = np.isin(ds0.faces, members["lips"]).any(axis=1)
+    assert np.array_equal(ds.masks == synth.MASK_LIPS, lips_faces)
+    assert np.all(ds.masks[~lips_faces] == synth.MASK_TORSO)
+    with pytest.raises(ValueError):
+        _write_mask_objs(tmp_path / "bad", ds0.canonical[:-1], ds0.faces[:1], members)
+        dataset.load_transforms(path, mesh_masks_dir=tmp_path / "bad")
