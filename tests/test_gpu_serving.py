@@ -12,10 +12,10 @@ from tests.conftest import assert_rel_close, ensure_oracle
 pytestmark = pytest.mark.gpu
 
 
-def _avatar_ctx(seed, n_frames, precision=PRECISION_FP32):
+def _avatar_ctx(seed, n_frames, precision=PRECISION_FP32, n_extra=0):
     import torch
     av = synth.small_sphere_avatar(seed=seed, n_frames=0, n_bases=50)
-    cfg = small_config(max_frames=n_frames, precision=precision)
+    cfg = small_config(max_frames=n_frames, precision=precision, n_extra_dims=n_extra)
     ctx = Context(cfg)
     ctx.set_canonical_mesh(av.canonical, av.faces, av.masks)
     params = synth.network_params(100 + seed, ctx.layout, cfg)
@@ -74,6 +74,42 @@ def test_served_request_matches_direct_render_and_oracle():
     same = steps.cpu().numpy() == o_steps.astype(np.int32)
     assert same.mean() > 0.995
     assert_rel_close(img[same], o_rgba[same], 1e-5, "served rgba vs oracle")
+
+
+@pytest.mark.parametrize("graphs", [1, 0])
+def test_served_request_conditions_on_its_expression(graphs):
+    """16 extra dims: a request's first 16 coefficients become its frame's
+    values (captured into the request's graph when graphs are on)."""
+    import torch
+    av, ctx, bases, params = _avatar_ctx(2, n_frames=3, n_extra=16)
+    res = 48
+    lips = av.masks == synth.MASK_LIPS
+    c = av.canonical[av.faces[lips]].reshape(-1, 3).mean(axis=0) - synth.CENTER
+    cam = make_camera(synth.look_at_camera(c / np.linalg.norm(c), 1.0), synth.focal_from_fov(res, 25.0))
+    coeffs = np.random.default_rng(6).normal(0.0, 1.0, 50).astype(np.float32)
+    server = Server(default_server_config(n_slots=2, hbm_budget_bytes=1 << 30, use_graphs=graphs))
+    aid = server.add_avatar(ctx, bases)
+    (img,), _ = _serve(server, [(aid, coeffs, cam, res)])
+    server.close()
+    ctx.set_frame_expression(2, bases, torch.from_numpy(coeffs).cuda())
+    ctx.build_occupancy(2, 0.05)
+    rgba, _, steps = ctx.render(cam, res, res, 2, 2, 0)
+    torch.cuda.synchronize()
+    assert np.array_equal(img, rgba.cpu().numpy())
+    ensure_oracle()
+    from oracle.oracle import Oracle
+    orc = Oracle(ctx.cfg)
+    orc.set_canonical_mesh(av.canonical, av.faces, av.masks)
+    orc.set_frame(0, synth.blend_expression_f32(av.canonical, av.bases, coeffs))
+    orc.set_params(params)
+    orc.build_occupancy(0, 0.05)
+    orc.set_frame_extra_dims(0, coeffs[:16])
+    o_rgba, _, o_steps, _ = orc.render(cam, res, res, 0, 0, 0)
+    same = steps.cpu().numpy() == o_steps.astype(np.int32)
+    assert same.mean() > 0.995
+    assert_rel_close(img[same], o_rgba[same], 1e-5, "served rgba vs oracle")
+    orc.set_frame_extra_dims(0, np.zeros(16, np.float32))
+    assert np.abs(orc.render(cam, res, res, 0, 0, 0)[0] - o_rgba).max() > 1e-3  # the values matter here
 
 
 @pytest.mark.parametrize("graphs", [1, 0])
