@@ -81,6 +81,8 @@ __device__ inline float wave_max(float v) {
 
 __global__ __launch_bounds__(kBuildThreads) void build_slot_kernel(BuildArgs a) {
     __shared__ uint32_t lds[INSTA_MAX_TRIANGLES];   // 64 KiB: sort keys, then split scratch
+    __shared__ uint16_t sL[INSTA_MAX_TRIANGLES];    // splits: one axis list (a copy of the global one)
+    __shared__ uint16_t sLn[INSTA_MAX_TRIANGLES];   // splits: that list partitioned, copied out
     __shared__ float red[6][kBuildThreads / 64];
     const uint32_t tid = threadIdx.x;
     const uint32_t slot = a.first_slot + blockIdx.x;
@@ -163,8 +165,10 @@ __global__ __launch_bounds__(kBuildThreads) void build_slot_kernel(BuildArgs a) 
     uint16_t* excl = reinterpret_cast<uint16_t*>(lds);
     uint8_t* right = reinterpret_cast<uint8_t*>(lds) + 2 * INSTA_MAX_TRIANGLES;
     uint8_t* seg_axis = reinterpret_cast<uint8_t*>(lds) + 3 * INSTA_MAX_TRIANGLES;
-    constexpr uint32_t kPer = INSTA_MAX_TRIANGLES / kBuildThreads;  // positions per thread
-    const uint32_t p0 = tid * kPer;
+    // positions per thread (a contiguous run: the scans below are blocked),
+    // sized by the mesh so that every wave takes part
+    const uint32_t per = (nf + kBuildThreads - 1) / kBuildThreads;
+    const uint32_t p0 = tid * per;
     uint32_t cur = 0;
     for (uint32_t s = 0; s < 2 * a.k; ++s) {
         const uint16_t* L = lists + (size_t)cur * 3 * a.tri_cap;
@@ -184,26 +188,33 @@ __global__ __launch_bounds__(kBuildThreads) void build_slot_kernel(BuildArgs a) 
             seg_axis[j] = (uint8_t)best_ax;
         }
         __syncthreads();
-        if (p0 < nf) {
-            uint32_t j = seg_of(p0, nf, s);
-            uint32_t e = seg_bound(j + 1, nf, s);
-            for (uint32_t pos = p0; pos < p0 + kPer && pos < nf; ++pos) {
-                while (pos >= e) {
-                    ++j;
-                    e = seg_bound(j + 1, nf, s);
+        // right flags: each position's id in its segment's chosen list; the
+        // lists are read through an LDS copy (coalesced global loads)
+        for (uint32_t ax = 0; ax < 3; ++ax) {
+            const uint16_t* La = L + ax * a.tri_cap;
+            for (uint32_t i = tid; i < nf; i += kBuildThreads) sL[i] = La[i];
+            __syncthreads();
+            if (p0 < nf) {
+                uint32_t j = seg_of(p0, nf, s);
+                uint32_t e = seg_bound(j + 1, nf, s);
+                for (uint32_t pos = p0; pos < p0 + per && pos < nf; ++pos) {
+                    while (pos >= e) {
+                        ++j;
+                        e = seg_bound(j + 1, nf, s);
+                    }
+                    if (seg_axis[j] == ax) right[sL[pos]] = pos >= seg_bound(2 * j + 1, nf, s + 1) ? 1 : 0;
                 }
-                const uint32_t mid = seg_bound(2 * j + 1, nf, s + 1);
-                const uint32_t id = L[seg_axis[j] * a.tri_cap + pos];
-                right[id] = pos >= mid ? 1 : 0;
             }
+            __syncthreads();
         }
-        __syncthreads();
         for (uint32_t ax = 0; ax < 3; ++ax) {
             const uint16_t* La = L + ax * a.tri_cap;
             uint16_t* Lna = Ln + ax * a.tri_cap;
+            for (uint32_t i = tid; i < nf; i += kBuildThreads) sL[i] = La[i];
+            __syncthreads();
             // block-wide exclusive scan of left flags in this list's order
             uint32_t cnt = 0;
-            for (uint32_t pos = p0; pos < p0 + kPer && pos < nf; ++pos) cnt += right[La[pos]] ? 0u : 1u;
+            for (uint32_t pos = p0; pos < p0 + per && pos < nf; ++pos) cnt += right[sL[pos]] ? 0u : 1u;
             uint32_t incl = cnt;
             for (int o = 1; o < 64; o <<= 1) {
                 const uint32_t up = __shfl_up(incl, o);
@@ -213,28 +224,30 @@ __global__ __launch_bounds__(kBuildThreads) void build_slot_kernel(BuildArgs a) 
             __syncthreads();
             uint32_t run = incl - cnt;
             for (uint32_t w = 0; w < wave; ++w) run += __float_as_uint(red[0][w]);
-            for (uint32_t pos = p0; pos < p0 + kPer && pos < nf; ++pos) {
+            for (uint32_t pos = p0; pos < p0 + per && pos < nf; ++pos) {
                 excl[pos] = (uint16_t)run;
-                run += right[La[pos]] ? 0u : 1u;
+                run += right[sL[pos]] ? 0u : 1u;
             }
             __syncthreads();
             if (p0 < nf) {
                 uint32_t j = seg_of(p0, nf, s);
                 uint32_t e = seg_bound(j + 1, nf, s);
-                for (uint32_t pos = p0; pos < p0 + kPer && pos < nf; ++pos) {
+                for (uint32_t pos = p0; pos < p0 + per && pos < nf; ++pos) {
                     while (pos >= e) {
                         ++j;
                         e = seg_bound(j + 1, nf, s);
                     }
                     const uint32_t b = seg_bound(j, nf, s), mid = seg_bound(2 * j + 1, nf, s + 1);
-                    const uint32_t id = La[pos];
+                    const uint32_t id = sL[pos];
                     const uint32_t left_before = (uint32_t)excl[pos] - (uint32_t)excl[b];
                     const uint32_t np = right[id] ? mid + (pos - b) - left_before : b + left_before;
-                    Lna[np] = (uint16_t)id;
+                    sLn[np] = (uint16_t)id;
                 }
             }
             __syncthreads();
+            for (uint32_t i = tid; i < nf; i += kBuildThreads) Lna[i] = sLn[i];
         }
+        __syncthreads();  // the new lists, written by every thread, are read next
         cur ^= 1u;
     }
     const uint16_t* order = lists + (size_t)cur * 3 * a.tri_cap;  // axis-0 list: BVH order
