@@ -79,6 +79,43 @@ __device__ inline float wave_max(float v) {
     return v;
 }
 
+// Bitonic passes with stride j < R of the merge sizes size_lo..size_hi on one
+// thread's block of R consecutive keys (base a multiple of R): every such pair
+// lies inside the block, so these passes run in registers with no barrier;
+// the pairs and directions are those of the LDS passes.
+template <uint32_t R>
+__device__ inline void block_bitonic(uint32_t* keys, uint32_t base, uint32_t size_lo, uint32_t size_hi) {
+    uint32_t k[R];
+#pragma unroll
+    for (uint32_t r = 0; r < R; ++r) k[r] = keys[base + r];
+    for (uint32_t size = size_lo; size <= size_hi; size <<= 1) {
+#pragma unroll
+        for (uint32_t j = R / 2; j > 0; j >>= 1) {
+            if (j >= size) continue;
+#pragma unroll
+            for (uint32_t r = 0; r < R; ++r) {
+                if (r & j) continue;
+                const bool up = ((base + r) & size) == 0;
+                const uint32_t lo = k[r], hi = k[r + j];
+                const bool swap = (lo > hi) == up;
+                k[r] = swap ? hi : lo;
+                k[r + j] = swap ? lo : hi;
+            }
+        }
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < R; ++r) keys[base + r] = k[r];
+}
+
+__device__ inline void block_bitonic_r(uint32_t R, uint32_t* keys, uint32_t tid, uint32_t size_lo, uint32_t size_hi) {
+    switch (R) {
+        case 2: block_bitonic<2>(keys, tid * 2, size_lo, size_hi); break;
+        case 4: block_bitonic<4>(keys, tid * 4, size_lo, size_hi); break;
+        case 8: block_bitonic<8>(keys, tid * 8, size_lo, size_hi); break;
+        default: block_bitonic<16>(keys, tid * 16, size_lo, size_hi); break;
+    }
+}
+
 __global__ __launch_bounds__(kBuildThreads) void build_slot_kernel(BuildArgs a) {
     __shared__ uint32_t lds[INSTA_MAX_TRIANGLES];   // 64 KiB: sort keys, then split scratch
     __shared__ uint16_t sL[INSTA_MAX_TRIANGLES];    // splits: one axis list (a copy of the global one)
@@ -141,8 +178,16 @@ __global__ __launch_bounds__(kBuildThreads) void build_slot_kernel(BuildArgs a) 
             keys[i] = key;
         }
         __syncthreads();
-        for (uint32_t size = 2; size <= npad; size <<= 1) {
-            for (uint32_t j = size >> 1; j > 0; j >>= 1) {
+        // keys per thread for the register passes (1: every pass through LDS)
+        const uint32_t R = npad >= 2 * kBuildThreads ? npad / kBuildThreads : 1u;
+        uint32_t size0 = 2;
+        if (R > 1) {
+            block_bitonic_r(R, keys, tid, 2, R);  // merge sizes 2..R: all their passes
+            __syncthreads();
+            size0 = 2 * R;
+        }
+        for (uint32_t size = size0; size <= npad; size <<= 1) {
+            for (uint32_t j = size >> 1; j >= R; j >>= 1) {
                 for (uint32_t i = tid; i < npad / 2; i += kBuildThreads) {
                     const uint32_t x = ((i & ~(j - 1)) << 1) | (i & (j - 1));
                     const uint32_t y = x + j;
@@ -153,6 +198,10 @@ __global__ __launch_bounds__(kBuildThreads) void build_slot_kernel(BuildArgs a) 
                         keys[y] = kx;
                     }
                 }
+                __syncthreads();
+            }
+            if (R > 1) {
+                block_bitonic_r(R, keys, tid, size, size);  // strides R/2..1
                 __syncthreads();
             }
         }
