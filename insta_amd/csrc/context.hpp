@@ -218,6 +218,9 @@ struct TrainSnapshot {
     uint32_t step = 0, rays_per_batch = 0;
     uint64_t rays_total = 0;
     std::vector<float> m, v;  // Adam moments, empty unless saved
+    // training set cameras: per image the 3x4 matrix, focal (px), principal point
+    uint32_t n_images = 0, width = 0, height = 0;
+    std::vector<float> cams;  // [n_images][16]: matrix[12], fx, fy, cx, cy
 };
 struct DensitySnapshot {
     std::vector<float> grid;  // [G^3] Morton order, empty if never populated

@@ -98,6 +98,7 @@ public:
         be(bits, 4);
     }
     void boolean(bool v) { byte(v ? 0xc3 : 0xc2); }
+    void nil() { byte(0xc0); }
     void key(const char* k) { str(k); }
     void vec3(const float* v) {
         arr(3);
@@ -307,6 +308,55 @@ void write_aabb(MpWriter& w, const float* lo, const float* hi) {
     w.key("max"); w.vec3(hi);
 }
 
+// NerfDataset as json_binding.h:139-162 writes it (Eigen matrices as arrays of
+// rows): the training set's cameras, no file paths (the images are the
+// caller's), the render box, identity local frame, unit scale, offset 0.5
+void write_dataset(MpWriter& w, const TrainSnapshot& ts, const float* lo, const float* hi) {
+    const uint32_t n = ts.n_images;
+    w.map(14);
+    w.key("n_images"); w.uint(n);
+    w.key("paths"); w.arr(n);
+    for (uint32_t i = 0; i < n; ++i) w.str("");
+    w.key("metadata"); w.arr(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* c = ts.cams.data() + (size_t)i * 16;
+        w.map(5);
+        w.key("focal_length"); w.arr(2); w.f32(c[12]); w.f32(c[13]);
+        w.key("camera_distortion"); w.nil();  // ECameraDistortionMode::None writes nothing
+        w.key("principal_point"); w.arr(2); w.f32(c[14]); w.f32(c[15]);
+        w.key("rolling_shutter"); w.arr(4); for (int k = 0; k < 4; ++k) w.f32(0.0f);
+        w.key("resolution"); w.arr(2); w.uint(ts.width); w.uint(ts.height);
+    }
+    w.key("xforms"); w.arr(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* c = ts.cams.data() + (size_t)i * 16;
+        w.map(2);
+        for (const char* k : {"start", "end"}) {
+            w.key(k);
+            w.arr(3);
+            for (int r = 0; r < 3; ++r) {
+                w.arr(4);
+                for (int q = 0; q < 4; ++q) w.f32(c[4 * r + q]);
+            }
+        }
+    }
+    w.key("render_aabb"); write_aabb(w, lo, hi);
+    w.key("render_aabb_to_local"); w.arr(3);
+    for (int r = 0; r < 3; ++r) {
+        w.arr(3);
+        for (int q = 0; q < 3; ++q) w.f32(r == q ? 1.0f : 0.0f);
+    }
+    const float up[3] = {0.0f, 1.0f, 0.0f}, offset[3] = {0.5f, 0.5f, 0.5f};
+    w.key("up"); w.vec3(up);
+    w.key("offset"); w.vec3(offset);
+    w.key("envmap_resolution"); w.arr(2); w.uint(0); w.uint(0);
+    w.key("scale"); w.f32(1.0f);
+    w.key("aabb_scale"); w.uint(1);
+    w.key("from_mitsuba"); w.boolean(false);
+    w.key("is_hdr"); w.boolean(false);
+    w.key("wants_importance_sampling"); w.boolean(true);
+}
+
 bool read_file(const char* path, std::vector<uint8_t>& data) {
     FILE* f = std::fopen(path, "rb");
     if (!f) return false;
@@ -397,13 +447,17 @@ insta_status insta_save_snapshot(insta_ctx* ctx, const char* path, int include_o
     w.key("rng_state"); w.uint(ds.rng_state);
     w.key("rng_inc"); w.uint(ds.rng_inc);
     w.key("nerf");
-    w.map(2);
+    w.map(ts.n_images ? 3 : 2);
     w.key("aabb_scale"); w.uint(1);
     w.key("rgb");
     w.map(3);
     w.key("rays_per_batch"); w.uint(ts.present ? ts.rays_per_batch : 1u << 12);
     w.key("measured_batch_size"); w.uint(0);
     w.key("measured_batch_size_before_compaction"); w.uint(0);
+    if (ts.n_images) {
+        w.key("dataset");
+        write_dataset(w, ts, ctx->render_lo, ctx->render_hi);
+    }
     w.key("training_step"); w.uint(ts.step);
     w.key("rays_total"); w.uint(ts.rays_total);
     const float unit_lo[3] = {0, 0, 0}, unit_hi[3] = {1, 1, 1};

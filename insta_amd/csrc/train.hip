@@ -951,6 +951,23 @@ insta_status train_export(insta_ctx* ctx, bool with_optimizer, TrainSnapshot& ou
     out.step = ts->step;
     out.rays_per_batch = ts->rays_per_batch;
     out.rays_total = ts->rays_total;
+    if (ts->n_images && ts->meta) {
+        std::vector<TrainImage> meta(ts->n_images);
+        INSTA_HIP(hipDeviceSynchronize());
+        INSTA_HIP(hipMemcpy(meta.data(), ts->meta, meta.size() * sizeof(TrainImage), hipMemcpyDeviceToHost));
+        out.n_images = ts->n_images;
+        out.width = ts->width;
+        out.height = ts->height;
+        out.cams.resize((size_t)ts->n_images * 16);
+        for (uint32_t i = 0; i < ts->n_images; ++i) {
+            float* c = out.cams.data() + (size_t)i * 16;
+            std::memcpy(c, meta[i].M, 12 * sizeof(float));
+            c[12] = meta[i].fx;
+            c[13] = meta[i].fy;
+            c[14] = meta[i].cx;
+            c[15] = meta[i].cy;
+        }
+    }
     if (with_optimizer) {
         const uint64_t np = ctx->layout.n_params;
         out.m.resize(np);

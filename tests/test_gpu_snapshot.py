@@ -48,6 +48,20 @@ def test_snapshot_round_trip(tmp_path):
     g16 = np.frombuffer(snap["density_grid_binary"], np.float16)
     assert g16.size == 64 ** 3 and np.array_equal(g16, grid.astype(np.float16))
     assert snap["training_step"] == 3 and snap["optimizer"]["step"] == 3
+    # the training cameras as NerfDataset (json_binding.h): focal length,
+    # principal point, resolution and a 3x4 start/end transform per image
+    cfg_, av_, cams_, images_, _ = scene
+    ds = snap["nerf"]["dataset"]
+    assert ds["n_images"] == N_FRAMES and len(ds["metadata"]) == N_FRAMES and len(ds["xforms"]) == N_FRAMES
+    for i, cam in enumerate(cams_):
+        md = ds["metadata"][i]
+        assert md["resolution"] == [images_.shape[2], images_.shape[1]]
+        assert np.allclose(md["focal_length"], list(cam.focal_px)) and md["camera_distortion"] is None
+        assert np.allclose(md["principal_point"], list(cam.screen_center))
+        m = np.asarray(ds["xforms"][i]["start"], np.float32)
+        assert m.shape == (3, 4) and np.array_equal(m.ravel(), np.asarray(list(cam.matrix), np.float32))
+        assert ds["xforms"][i]["end"] == ds["xforms"][i]["start"]
+    assert set(ds["render_aabb"]) == {"min", "max"} and ds["aabb_scale"] == 1
     cfg2 = snapshot_config(path)
     assert cfg2.n_levels == ctx.cfg.n_levels and cfg2.grid_size == ctx.cfg.grid_size
 
