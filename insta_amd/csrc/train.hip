@@ -18,9 +18,9 @@
 //             dL/d(density logit) = sigma * dt * dL/drgb . (T rgb - suffix),
 //             scaled by loss_scale / n_rays; the composited samples are
 //             compacted into a batch of at most target_samples (1543-1560)
-//   forward   on the compacted batch: hash grid + both MLPs in FP32 (the
-//             plain path's FMA chains), keeping every activation the backward
-//             needs, feature-major [feature][sample]
+//   forward   on the compacted batch: hash grid (FP32, the plain encode)
+//             and the MLPs on FP32 MFMA, keeping every activation the
+//             backward needs, feature-major [feature][sample]
 //   backward  per compacted sample through both MLPs (ReLU masks from the
 //             stored activations) into dL/dencoding, then scattered into the
 //             hash tables: FP32 atomics, or packed FP16 atomics (one
@@ -73,7 +73,6 @@ enum : uint32_t {
 constexpr int kW = 64;
 constexpr int kDOut = 16;
 constexpr int kCin = 32;             // density outputs + 16 SH
-constexpr uint32_t kTrainThreads = 128;
 constexpr uint32_t kMlpThreads = 256;  // backward: one weight copy in LDS per block
 constexpr uint32_t kChunk = 1024;    // samples per weight-gradient partial
 constexpr uint32_t kRaySeedStride = 8;
@@ -238,47 +237,139 @@ INSTA_DEV void store_rows(float* __restrict__ act, uint32_t ld, int row0, uint32
     for (int k = 0; k < NOUT; ++k) act[(size_t)(row0 + k) * ld + i] = v[k];
 }
 
-// Forward with stored activations over the compacted batch; the arithmetic
-// is that of the plain FP32 inference kernel (same FMA chains).  The output
-// layer is not needed: dL/doutput comes from the loss kernel.
+// Forward with stored activations over the compacted batch, on
+// v_mfma_f32_16x16x4f32 (the layout of the FP32 inference kernel): phase A,
+// one thread per sample, encodes (FP32, the scalar encode's arithmetic) into
+// LDS and stores the encoding, extra-dims and SH rows; phase B runs each
+// wave's 64 samples, 16 columns at a time, through the density MLP and the
+// colour MLP's hidden layers with the weights as A operands, storing every
+// layer's rows from the accumulators (lane l holds rows 4(l/16)..+3 of
+// column l%16, contiguous along the batch).  The output layer is not needed:
+// dL/doutput comes from the loss kernel.
+typedef float tf_f4 __attribute__((ext_vector_type(4)));
+constexpr uint32_t kFwdTile = 256;
 template <int L, int X>
-__global__ __launch_bounds__(kTrainThreads) void train_forward_kernel(const float* __restrict__ params,
-                                                                      const float* __restrict__ table, GridParams gp,
-                                                                      const float* __restrict__ frame_extra,
-                                                                      uint32_t n_frames,
-                                                                      const uint32_t* __restrict__ n_dev,
-                                                                      const uint32_t* __restrict__ c_idx,
-                                                                      const float4* __restrict__ cpos,
-                                                                      const float4* __restrict__ cdir,
-                                                                      float* __restrict__ act, uint32_t ld) {
-    constexpr int E = 2 * L, DIN = E + X;
+__global__ __launch_bounds__(256) void train_forward_kernel(const float* __restrict__ params,
+                                                          const float* __restrict__ table, GridParams gp,
+                                                          const float* __restrict__ frame_extra, uint32_t n_frames,
+                                                          const uint32_t* __restrict__ n_dev,
+                                                          const uint32_t* __restrict__ c_idx,
+                                                          const float4* __restrict__ cpos,
+                                                          const float4* __restrict__ cdir,
+                                                          float* __restrict__ act, uint32_t ld) {
+    constexpr int E = 2 * L, DIN = E + X, KB0 = DIN / 4, S = DIN + 1;
+    static_assert(DIN % 4 == 0, "K blocks of 4");
     using A = Act<DIN>;
     using O = Off<DIN>;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *n_dev) return;
-    const uint32_t smp = c_idx[i];
-    const float4 p4 = cpos[smp], d4 = cdir[smp];
-    const V3 x = v3(p4.x, p4.y, p4.z);
-    float enc[DIN];
+    __shared__ float s_x[kFwdTile * S];
+    const uint32_t n = *n_dev;
+    const uint32_t base = blockIdx.x * kFwdTile;
+    if (base >= n) return;  // whole block past the batch
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t m = lane & 15u, g = lane >> 4;
+    {
+        const uint32_t i = base + tid;
+        float* row = s_x + tid * S;
+        if (i < n) {
+            const uint32_t smp = c_idx[i];
+            const float4 p4 = cpos[smp], d4 = cdir[smp];
+            const V3 x = v3(p4.x, p4.y, p4.z);
+            float enc[DIN];
 #pragma unroll
-    for (int l = 0; l < L; ++l) encode_level<2>(table, gp, (uint32_t)l, x, enc + 2 * l);
-    if constexpr (X > 0) {
-        const uint32_t sel = __float_as_uint(d4.w);
-        const float* fx = sel != kExtraOnes && sel <= n_frames ? frame_extra + (size_t)(sel - 1u) * X : nullptr;
+            for (int l = 0; l < L; ++l) encode_level<2>(table, gp, (uint32_t)l, x, enc + 2 * l);
+            if constexpr (X > 0) {
+                const uint32_t sel = __float_as_uint(d4.w);
+                const float* fx = sel != kExtraOnes && sel <= n_frames ? frame_extra + (size_t)(sel - 1u) * X : nullptr;
 #pragma unroll
-        for (int e = 0; e < X; ++e) enc[E + e] = fx ? fx[e] : 1.0f;
+                for (int e = 0; e < X; ++e) enc[E + e] = fx ? fx[e] : 1.0f;
+            }
+#pragma unroll
+            for (int k = 0; k < DIN; ++k) row[k] = enc[k];
+            store_rows<DIN, DIN>(act, ld, A::ENC, i, enc);
+            float sh[16];
+            sh16(d4.x * 2.0f - 1.0f, d4.y * 2.0f - 1.0f, d4.z * 2.0f - 1.0f, sh);
+            store_rows<16, 16>(act, ld, A::SH, i, sh);
+        } else {
+#pragma unroll
+            for (int k = 0; k < DIN; ++k) row[k] = 0.0f;
+        }
     }
-    store_rows<DIN, DIN>(act, ld, A::ENC, i, enc);
-    float h[kW], g[kW], r[kCin];
-    dense<DIN, kW, true>(params + O::W0, enc, h);
-    store_rows<kW, kW>(act, ld, A::H1, i, h);
-    dense<kW, kDOut, false>(params + O::W1, h, r);
-    sh16(d4.x * 2.0f - 1.0f, d4.y * 2.0f - 1.0f, d4.z * 2.0f - 1.0f, r + kDOut);
-    store_rows<kCin, kCin>(act, ld, A::DENS, i, r);
-    dense<kCin, kW, true>(params + O::W2, r, h);
-    store_rows<kW, kW>(act, ld, A::H2, i, h);
-    dense<kW, kW, true>(params + O::W3, h, g);
-    store_rows<kW, kW>(act, ld, A::H3, i, g);
+    __syncthreads();
+    const float* W0 = params + O::W0;
+    const float* W1 = params + O::W1;
+    const float* W2 = params + O::W2;
+    const float* W3 = params + O::W3;
+    for (uint32_t nb = 0; nb < 4; ++nb) {
+        const uint32_t s = wave * 64 + nb * 16 + m;  // column (sample) of this lane
+        const uint32_t i = base + s;
+        const bool live = i < n;
+        const float* xr = s_x + s * S;
+        // SH of the column's sample; lane group g feeds K rows 4g..4g+3
+        float shg[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (live) {
+            const float4 d4 = cdir[c_idx[i]];
+            float sh[16];
+            sh16(d4.x * 2.0f - 1.0f, d4.y * 2.0f - 1.0f, d4.z * 2.0f - 1.0f, sh);
+#pragma unroll
+            for (int b = 0; b < 4; ++b) shg[b] = g == 0 ? sh[b] : (g == 1 ? sh[4 + b] : (g == 2 ? sh[8 + b] : sh[12 + b]));
+        }
+        // density layer 0: H1 = relu(W0 x)
+        tf_f4 h[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            tf_f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int kb = 0; kb < KB0; ++kb)
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(W0[(16 * t + m) * DIN + 4 * kb + g], xr[4 * kb + g], acc, 0, 0, 0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                h[t][q] = fmaxf(acc[q], 0.0f);
+                if (live) act[(size_t)(A::H1 + 16 * t + 4 * g + q) * ld + i] = h[t][q];
+            }
+        }
+        // density output: 16 rows, K = H1 in the order 16t + 4g + b
+        tf_f4 dn = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                dn = __builtin_amdgcn_mfma_f32_16x16x4f32(W1[m * kW + 16 * t + 4 * g + b], h[t][b], dn, 0, 0, 0);
+        if (live)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) act[(size_t)(A::DENS + 4 * g + q) * ld + i] = dn[q];
+        // colour layer 0: K = density outputs (4g + b) then SH (16 + 4g + b)
+        tf_f4 r[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            tf_f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(W2[(16 * t + m) * kCin + 4 * g + b], dn[b], acc, 0, 0, 0);
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(W2[(16 * t + m) * kCin + 16 + 4 * g + b], shg[b], acc, 0, 0,
+                                                           0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                r[t][q] = fmaxf(acc[q], 0.0f);
+                if (live) act[(size_t)(A::H2 + 16 * t + 4 * g + q) * ld + i] = r[t][q];
+            }
+        }
+        // colour layer 1
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            tf_f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int s2 = 0; s2 < 4; ++s2)
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(W3[(16 * t + m) * kW + 16 * s2 + 4 * g + b], r[s2][b],
+                                                               acc, 0, 0, 0);
+            if (live)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) act[(size_t)(A::H3 + 16 * t + 4 * g + q) * ld + i] = fmaxf(acc[q], 0.0f);
+        }
+    }
 }
 
 struct LossArgs {
@@ -703,8 +794,8 @@ insta_status launch_train_forward(insta_ctx* ctx, TrainState* ts, hipStream_t st
     const uint32_t* n_dev = ts->counters + kTCompactN;
     const float* table = ctx->d_params + ctx->layout.n_mlp_params;
     const GridParams gp = grid_params(ctx);
-    const dim3 sgrid((cap + kTrainThreads - 1) / kTrainThreads);
-    hipLaunchKernelGGL((train_forward_kernel<L, X>), sgrid, dim3(kTrainThreads), 0, stream, ctx->d_params, table, gp,
+    const dim3 sgrid((cap + kFwdTile - 1) / kFwdTile);
+    hipLaunchKernelGGL((train_forward_kernel<L, X>), sgrid, dim3(kFwdTile), 0, stream, ctx->d_params, table, gp,
                        ctx->d_frame_extra, ctx->cfg.max_frames, n_dev, ts->c_idx, ts->s_cpos, ts->s_cdir, ts->act,
                        cap);
     INSTA_HIP(hipGetLastError());
