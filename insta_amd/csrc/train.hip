@@ -21,10 +21,11 @@
 //   forward   on the compacted batch: hash grid (FP32, the plain encode)
 //             and the MLPs on FP32 MFMA, keeping every activation the
 //             backward needs, feature-major [feature][sample]
-//   backward  per compacted sample through both MLPs (ReLU masks from the
-//             stored activations) into dL/dencoding, then scattered into the
-//             hash tables: FP32 atomics, or packed FP16 atomics (one
-//             global_atomic_pk_add_f16 per corner and level, F = 2)
+//   backward  both MLPs on FP32 MFMA over the compacted batch (ReLU masks
+//             from the stored activations) into dL/dencoding, then scattered
+//             into the hash tables, one thread per sample: FP32 atomics, or
+//             packed FP16 atomics (one global_atomic_pk_add_f16 per corner
+//             and level, F = 2)
 //   wgrad     weight gradients dW = sum_n delta_n x_n^T as per-chunk partial
 //             products on FP32 MFMA (samples as the K dimension, operands
 //             staged through LDS), then a fixed-order sum over chunks:
@@ -489,13 +490,17 @@ INSTA_DEV float2 run_sum(uint32_t key, float2 v, bool& head) {
     return v;
 }
 
-// Backward of one sample: deltas of both MLPs (stored for the weight
-// gradients) and the hash-grid scatter.  The deltas are accumulated as outer
-// products (input-gradient k of a layer is the FMA chain over that layer's
-// outputs in ascending order), so at most one layer's accumulators are live.
-// On the levels of run_mask, the corner gradients of consecutive samples that
-// land on one table entry (coarse levels along one ray) are summed across the
-// wave first and added by one lane.
+// Backward over a block of 256 compacted samples, in two phases.
+// Phase 1 (FP32 MFMA, the forward kernel's layout, 16 samples per step of a
+// wave): the deltas of both MLPs through the transposed weights (held in
+// LDS), the ReLU masks from the stored activations, each delta's rows stored
+// for the weight gradients, and dL/dencoding into LDS.  Phase 2 (one thread
+// per sample): the hash-grid scatter; on the levels of run_mask the corner
+// gradients of consecutive samples that land on one table entry (coarse
+// levels along one ray) are summed across the wave first and added by one
+// lane.  (A split into a delta kernel and a low-register scatter kernel
+// measured slower: the scatter's extra occupancy does not repay the round
+// trip of dL/dencoding through memory.)
 template <int L, bool H16, int X>
 __global__ __launch_bounds__(kMlpThreads) void train_backward_kernel(const float* __restrict__ params, GridParams gp,
                                                                        const uint32_t* __restrict__ n_dev,
@@ -510,71 +515,109 @@ __global__ __launch_bounds__(kMlpThreads) void train_backward_kernel(const float
     constexpr int E = 2 * L, DIN = E + X;  // encoding, density input (+ extra dims)
     using A = Act<DIN>;
     using O = Off<DIN>;
-    __shared__ float w[O::N];  // both MLPs' weights (broadcast reads)
+    typedef float bf4 __attribute__((ext_vector_type(4)));
+    __shared__ float w[O::N];                  // both MLPs' weights
+    __shared__ float s_denc[E][kMlpThreads];   // dL/dencoding of the block's samples
+    const uint32_t n = *n_dev;
+    const uint32_t base = blockIdx.x * kMlpThreads;
+    if (base >= n) return;  // whole block past the batch
     for (uint32_t e = threadIdx.x; e < O::N; e += kMlpThreads) w[e] = params[e];
     __syncthreads();
-    const uint32_t n = *n_dev;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if ((i & ~63u) >= n) return;  // whole wave past the batch
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t m = lane & 15u, g = lane >> 4;
+    const float* W0 = w + O::W0;
+    const float* W1 = w + O::W1;
+    const float* W2 = w + O::W2;
+    const float* W3 = w + O::W3;
+    const float* W4 = w + O::W4;
+    for (uint32_t nb = 0; nb < 4; ++nb) {
+        const uint32_t s = wave * 64 + nb * 16 + m;  // column (sample) of this lane
+        const uint32_t i = base + s;
+        const bool live = i < n;
+        const float4 d4 = live ? dout[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        // colour output layer: dH3 = relu'(H3) (W4^T dRGB), K = 3 rgb rows (+ one zero)
+        const float b_rgb = g == 0 ? d4.x : (g == 1 ? d4.y : (g == 2 ? d4.z : 0.0f));
+        bf4 dh3[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            bf4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(g < 3 ? W4[g * kW + 16 * t + m] : 0.0f, b_rgb, acc, 0, 0, 0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t row = 16 * t + 4 * g + q;
+                dh3[t][q] = live && act[(size_t)(A::H3 + row) * ld + i] > 0.0f ? acc[q] : 0.0f;
+                if (live) dact[(size_t)(Dact::DH3 + row) * ld + i] = dh3[t][q];
+            }
+        }
+        // colour hidden layer: dH2 = relu'(H2) (W3^T dH3)
+        bf4 dh2[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            bf4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int s2 = 0; s2 < 4; ++s2)
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(W3[(16 * s2 + 4 * g + b) * kW + 16 * t + m], dh3[s2][b],
+                                                               acc, 0, 0, 0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t row = 16 * t + 4 * g + q;
+                dh2[t][q] = live && act[(size_t)(A::H2 + row) * ld + i] > 0.0f ? acc[q] : 0.0f;
+                if (live) dact[(size_t)(Dact::DH2 + row) * ld + i] = dh2[t][q];
+            }
+        }
+        // colour input restricted to the density outputs: dD = (W2^T dH2)[0..15], + dL/d(density logit) on row 0
+        bf4 dd = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                dd = __builtin_amdgcn_mfma_f32_16x16x4f32(W2[(16 * t + 4 * g + b) * kCin + m], dh2[t][b], dd, 0, 0, 0);
+        if (g == 0) dd[0] = dd[0] + d4.w;  // density logit = density-MLP output 0
+        if (live)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) dact[(size_t)(Dact::DDENS + 4 * g + q) * ld + i] = dd[q];
+        // density hidden layer: dH1 = relu'(H1) (W1^T dD)
+        bf4 dh1[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            bf4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(W1[(4 * g + b) * kW + 16 * t + m], dd[b], acc, 0, 0, 0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t row = 16 * t + 4 * g + q;
+                dh1[t][q] = live && act[(size_t)(A::H1 + row) * ld + i] > 0.0f ? acc[q] : 0.0f;
+                if (live) dact[(size_t)(Dact::DH1 + row) * ld + i] = dh1[t][q];
+            }
+        }
+        // dL/dencoding = (W0^T dH1)[0..E), the extra dims' rows not needed
+#pragma unroll
+        for (int te = 0; te < E / 16; ++te) {
+            bf4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int s2 = 0; s2 < 4; ++s2)
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(W0[(16 * s2 + 4 * g + b) * DIN + 16 * te + m],
+                                                               dh1[s2][b], acc, 0, 0, 0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) s_denc[16 * te + 4 * g + q][s] = acc[q];
+        }
+    }
+    __syncthreads();
+    const uint32_t i = base + tid;
+    if ((i & ~63u) >= n) return;  // whole wave past the batch (no barrier follows)
     const bool valid = i < n;
     const uint32_t ic = valid ? i : n - 1u;  // tail lanes recompute the last sample and store nothing
-    const float4 d4 = valid ? dout[ic] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    const float* W4 = w + O::W4;
-    const float* W3 = w + O::W3;
-    const float* W2 = params + O::W2;  // scalar loads: from LDS its fully unrolled loop spills
-    const float* W1 = w + O::W1;
-    const float* W0 = w + O::W0;
     if (valid) {
+        const float4 d4 = dout[i];
         dact[(size_t)(Dact::DRGB + 0) * ld + i] = d4.x;
         dact[(size_t)(Dact::DRGB + 1) * ld + i] = d4.y;
         dact[(size_t)(Dact::DRGB + 2) * ld + i] = d4.z;
     }
-    // colour layer 2 -> dh3 (one row at a time) -> dh2 accumulators
-    float dh2[kW];
-#pragma unroll
-    for (int k = 0; k < kW; ++k) dh2[k] = 0.0f;
-#pragma unroll 1
-    for (int m = 0; m < kW; ++m) {
-        const float s = W4[m] * d4.x + W4[kW + m] * d4.y + W4[2 * kW + m] * d4.z;
-        const float dh3 = act[(size_t)(A::H3 + m) * ld + ic] > 0.0f ? s : 0.0f;
-        if (valid) dact[(size_t)(Dact::DH3 + m) * ld + i] = dh3;
-#pragma unroll
-        for (int k = 0; k < kW; ++k) dh2[k] = __builtin_fmaf(W3[m * kW + k], dh3, dh2[k]);
-    }
-    // colour layer 0 input gradient restricted to the density outputs
-    float dd[kDOut];
-#pragma unroll
-    for (int q = 0; q < kDOut; ++q) dd[q] = 0.0f;
-#pragma unroll
-    for (int m = 0; m < kW; ++m) {
-        const float g = act[(size_t)(A::H2 + m) * ld + ic] > 0.0f ? dh2[m] : 0.0f;
-        if (valid) dact[(size_t)(Dact::DH2 + m) * ld + i] = g;
-#pragma unroll
-        for (int q = 0; q < kDOut; ++q) dd[q] = __builtin_fmaf(W2[m * kCin + q], g, dd[q]);
-    }
-    dd[0] = dd[0] + d4.w;  // density logit = density-MLP output 0
-    if (valid) {
-#pragma unroll
-        for (int q = 0; q < kDOut; ++q) dact[(size_t)(Dact::DDENS + q) * ld + i] = dd[q];
-    }
-    // density layers: dh1 one row at a time -> dL/dencoding accumulators
-    float denc[E];
-#pragma unroll
-    for (int e = 0; e < E; ++e) denc[e] = 0.0f;
-#pragma unroll 1
-    for (int k = 0; k < kW; ++k) {
-        float s = 0.0f;
-#pragma unroll
-        for (int q = 0; q < kDOut; ++q) s = __builtin_fmaf(W1[q * kW + k], dd[q], s);
-        const float dh1 = act[(size_t)(A::H1 + k) * ld + ic] > 0.0f ? s : 0.0f;
-        if (valid) dact[(size_t)(Dact::DH1 + k) * ld + i] = dh1;
-#pragma unroll
-        for (int e = 0; e < E; ++e) denc[e] = __builtin_fmaf(W0[k * DIN + e], dh1, denc[e]);
-    }
-    // dL/dencoding through LDS, so that the level loop indexes it at run time
-    __shared__ float s_denc[E][kMlpThreads];
-#pragma unroll
-    for (int e = 0; e < E; ++e) s_denc[e][threadIdx.x] = denc[e];
     // hash-grid scatter (the corner/weight arithmetic of encode_level)
     const float4 p4 = cpos[c_idx[ic]];
     const float xs[3] = {p4.x, p4.y, p4.z};
@@ -624,6 +667,7 @@ __global__ __launch_bounds__(kMlpThreads) void train_backward_kernel(const float
         }
     }
 }
+
 
 struct WProduct {
     uint32_t d_row, x_row, O, I, out;
