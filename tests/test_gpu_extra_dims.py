@@ -144,7 +144,7 @@ def test_training_gradients_match_fp64(avatar):
 
 
 def test_density_grid_update_matches_oracle(avatar):
-    ctx, orc, O, cfg = None, None, None, small_config(n_extra_dims=X, max_frames=N_FRAMES)
+    cfg = small_config(n_extra_dims=X, max_frames=N_FRAMES)
     ctx, orc, O, _ = _pair(cfg, avatar, _vals(), params_seed=17)
     G = cfg.grid_size
     for k, (nu, nn) in enumerate([(G ** 3, 0), (G ** 3 // 4, G ** 3 // 4)]):
@@ -185,3 +185,23 @@ def test_expression_sets_the_frame_values(n_bases):
     torch.cuda.synchronize()
     assert np.array_equal(r1, r2)
     assert np.abs(r1 - r0).max() > 1e-3
+
+
+def test_full_size_render_matches_oracle():
+    """The full-size shapes (9,976-triangle head, L16 T2^19, 128^3 grid) with
+    the 16 extra dims, FP32 MFMA path, on a view of the lips."""
+    from insta_amd.capi import full_config
+    av = synth.head_avatar(seed=1, n_frames=N_FRAMES)
+    res = 40
+    ctx, orc, _, _ = _pair(full_config(n_extra_dims=X, max_frames=N_FRAMES, precision=0), av, _vals(3))
+    for f in range(N_FRAMES):
+        ctx.build_occupancy(f, 0.015)
+        orc.build_occupancy(f, 0.015)
+    lips = av.masks == synth.MASK_LIPS
+    c = av.frames[0][av.faces[lips]].reshape(-1, 3).mean(axis=0) - synth.CENTER
+    cam = make_camera(synth.look_at_camera(c / np.linalg.norm(c), 0.7), synth.focal_from_fov(res, 25.0))
+    rgba, _, steps = (t.cpu().numpy() for t in ctx.render(cam, res, res, 0, 0, 0))
+    r_rgba, _, r_steps, _ = orc.render(cam, res, res, 0, 0)
+    same = steps.astype(np.uint32) == r_steps
+    assert same.mean() >= 0.99 and (r_steps > 0).sum() > 100 and r_rgba[:, 3].max() > 0.05
+    assert_rel_close(rgba[same], r_rgba[same], RTOL, "full-size rgba with extra dims")
