@@ -62,15 +62,25 @@ def test_invalid_configs_are_rejected():
     lib = capi.load_library()
     lay = capi.InstaLayout()
     for bad in [dict(n_levels=0), dict(n_features_per_level=3), dict(grid_size=100), dict(max_triangles=20000),
-                dict(sh_degree=3), dict(min_transmittance=1.0), dict(dt=0.0)]:
+                dict(sh_degree=3), dict(min_transmittance=1.0), dict(dt=0.0), dict(n_extra_dims=8),
+                dict(n_extra_dims=17)]:
         cfg = capi.full_config(**bad)
         st = lib.insta_query_layout(C.byref(cfg), C.byref(lay))
         assert st == 1, bad
         assert lib.insta_last_error()
 
 
+def test_extra_dims_layout():
+    # 16 expression values appended to the density input: W0 grows by 64 x 16
+    plain, extra = capi.query_layout(capi.full_config()), capi.query_layout(capi.full_config(n_extra_dims=16))
+    assert extra.density_in == plain.density_in + 16 == 48
+    assert extra.n_mlp_params == plain.n_mlp_params + 64 * 16
+    assert extra.n_grid_params == plain.n_grid_params
+
+
 def test_null_context_is_an_error_not_a_crash():
     lib = capi.load_library()
+    assert lib.insta_set_frame_extra_dims(None, 0, 0, None, None) == 1
     assert lib.insta_deform_samples(None, None, 0, 11, -1, None) == 1
     assert lib.insta_render(None, None, 1, 1, 0, 0, 0, None, None, None, None) == 1
     assert b"null" in lib.insta_last_error()

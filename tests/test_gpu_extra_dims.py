@@ -205,3 +205,18 @@ def test_full_size_render_matches_oracle():
     same = steps.astype(np.uint32) == r_steps
     assert same.mean() >= 0.99 and (r_steps > 0).sum() > 100 and r_rgba[:, 3].max() > 0.05
     assert_rel_close(rgba[same], r_rgba[same], RTOL, "full-size rgba with extra dims")
+
+
+def test_frame_extra_dims_errors():
+    import torch
+    from insta_amd.capi import InstaError
+    av = synth.small_sphere_avatar(0, 1)
+    plain = Context(small_config(max_frames=2))
+    with pytest.raises(InstaError):  # no extra dims in this configuration
+        plain.set_frame_extra_dims(0, torch.zeros((1, X), device="cuda"))
+    ctx = Context(small_config(max_frames=2, n_extra_dims=X))
+    ctx.set_canonical_mesh(av.canonical, av.faces, av.masks)
+    with pytest.raises(InstaError):  # frames 1..2 of 2
+        ctx.set_frame_extra_dims(1, torch.zeros((2, X), device="cuda"))
+    ctx.set_frame_extra_dims(1, torch.zeros((1, X), device="cuda"))
+    torch.cuda.synchronize()
