@@ -753,9 +753,17 @@ __global__ void wgrad_reduce_kernel(const float* __restrict__ part, const uint32
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n_mlp) return;
     const uint32_t nc = (*n_dev + kChunk - 1) / kChunk;
-    float s = 0.0f;
-    for (uint32_t c = 0; c < nc; ++c) s += part[(size_t)c * n_mlp + p];
-    grad[p] = s;
+    // eight interleaved running sums (chunk c into sum c mod 8), added in a
+    // fixed order: deterministic, with eight independent loads in flight
+    float s[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    uint32_t c = 0;
+    for (; c + 8 <= nc; c += 8)
+#pragma unroll
+        for (uint32_t r = 0; r < 8; ++r) s[r] += part[(size_t)(c + r) * n_mlp + p];
+#pragma unroll
+    for (uint32_t r = 0; r < 8; ++r)
+        if (c + r < nc) s[r] += part[(size_t)(c + r) * n_mlp + p];
+    grad[p] = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
 }
 
 // Unscale, optionally step Adam.  grad[] receives dL/dparam (unscaled).
