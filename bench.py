@@ -388,7 +388,7 @@ def main():
                     help="run one 256 MiB device copy first (reference kernel for rocprof byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (this build runs on one "
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the library runs on one "
               "GPU; the path would shard by image tiles for rendering and by avatars for serving)", file=sys.stderr)
         sys.exit(2)
 
