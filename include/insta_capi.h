@@ -294,7 +294,10 @@ typedef struct insta_train_stats {
     uint32_t n_samples;       /* samples marched, deformed and evaluated */
     uint32_t n_samples_used;  /* samples composited before transmittance < 1e-4 */
     float    loss;            /* mean Huber loss per ray (unscaled) */
-    uint32_t step;            /* optimizer steps taken so far */
+    uint32_t step;            /* training steps taken (the reference's m_training_step, which
+                                 schedules the density-grid updates of insta_train_loop) */
+    uint32_t adam_step;       /* Adam's step count (bias correction); restarts at 0 with
+                                 fresh moments, e.g. a snapshot loaded without them */
 } insta_train_stats;
 
 /* Defaults: adaptive rays aiming at a 2^18-sample batch, 2^22 marched samples
@@ -322,8 +325,9 @@ insta_status insta_get_gradients(insta_ctx* ctx, float* grads_dev, void* stream)
 /* Current FP32 parameters (layout.n_params), e.g. after training steps. */
 insta_status insta_get_params(insta_ctx* ctx, float* params_dev, void* stream);
 
-/* Zero the Adam moments and step count. */
-insta_status insta_reset_optimizer(insta_ctx* ctx);
+/* Zero the Adam moments and Adam's step count (stream-ordered); the training
+ * step count is kept. */
+insta_status insta_reset_optimizer(insta_ctx* ctx, void* stream);
 
 /* warp_random_samples (src/core.cu:528-579): positions_dev [n][3] mapped in
  * place to canonical space, each through a random posed frame: position i

@@ -77,7 +77,8 @@ class InstaTrainConfig(C.Structure):
 
 class InstaTrainStats(C.Structure):
     _fields_ = [("n_rays", C.c_uint32), ("n_rays_used", C.c_uint32), ("n_samples", C.c_uint32),
-                ("n_samples_used", C.c_uint32), ("loss", C.c_float), ("step", C.c_uint32)]
+                ("n_samples_used", C.c_uint32), ("loss", C.c_float), ("step", C.c_uint32),
+                ("adam_step", C.c_uint32)]
 
 
 class InstaDensityGridConfig(C.Structure):
@@ -142,7 +143,7 @@ SIGNATURES = [
                                    C.c_int32, C.c_uint32, C.POINTER(InstaTrainStats), _VP]),
     ("insta_get_gradients", C.c_int, [_VP, _VP, _VP]),
     ("insta_get_params", C.c_int, [_VP, _VP, _VP]),
-    ("insta_reset_optimizer", C.c_int, [_VP]),
+    ("insta_reset_optimizer", C.c_int, [_VP, _VP]),
     ("insta_get_occupancy", C.c_int, [_VP, C.c_int32, _VP, _VP]),
     ("insta_deform_positions", C.c_int, [_VP, _VP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, _VP]),
     ("insta_update_density_grid", C.c_int, [_VP, C.c_int32, C.c_uint32, C.POINTER(InstaDensityGridConfig), _VP,
@@ -466,7 +467,7 @@ class Context:
         return p
 
     def reset_optimizer(self) -> None:
-        check(self.lib.insta_reset_optimizer(self.h), "insta_reset_optimizer")
+        check(self.lib.insta_reset_optimizer(self.h, self.stream()), "insta_reset_optimizer")
 
     def render_stats(self) -> InstaRenderStats:
         st = InstaRenderStats()

@@ -619,9 +619,13 @@ insta_status insta_get_profile(insta_ctx* ctx, insta_profile* out) {
     }
     *out = ctx->prof;
     if (ctx->d_trav_counts) {
+        // renders and training steps synchronise their stream before they
+        // return, so the counts are final: read them on the context's own
+        // non-blocking stream rather than stalling the device
         unsigned long long c[4];
-        INSTA_HIP(hipDeviceSynchronize());
-        INSTA_HIP(hipMemcpy(c, ctx->d_trav_counts, sizeof(c), hipMemcpyDeviceToHost));
+        if (!ctx->aux_stream) INSTA_HIP(hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking));
+        INSTA_HIP(hipMemcpyAsync(c, ctx->d_trav_counts, sizeof(c), hipMemcpyDeviceToHost, ctx->aux_stream));
+        INSTA_HIP(hipStreamSynchronize(ctx->aux_stream));
         out->bvh_queries = c[0];
         out->bvh_inner = c[1];
         out->bvh_leaves = c[2];
@@ -761,10 +765,10 @@ insta_status insta_get_params(insta_ctx* ctx, float* params_dev, void* stream) {
     return INSTA_OK;
 }
 
-insta_status insta_reset_optimizer(insta_ctx* ctx) {
+insta_status insta_reset_optimizer(insta_ctx* ctx, void* stream) {
     INSTA_CHECK_CTX(ctx);
     if (!ctx->train) return INSTA_OK;
-    return insta::reset_optimizer(ctx);
+    return insta::reset_optimizer(ctx, (hipStream_t)stream);
 }
 
 insta_status insta_deform_positions(insta_ctx* ctx, float* positions_dev, uint32_t n, uint32_t n_frames, uint32_t seed,

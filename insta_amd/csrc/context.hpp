@@ -96,6 +96,7 @@ struct insta_ctx {
     // optional per-kernel-class event timing of insta_render
     bool profiling = false;
     unsigned long long* d_trav_counts = nullptr;  // [4] while traversal counting is on
+    hipStream_t aux_stream = nullptr;             // non-blocking stream for host reads of counters
     hipEvent_t prof_ev[2 * INSTA_KERNEL_CLASSES] = {};
     bool prof_pending[INSTA_KERNEL_CLASSES] = {};
     uint64_t prof_pending_items[INSTA_KERNEL_CLASSES] = {};
@@ -211,11 +212,12 @@ insta_status update_density_grid(insta_ctx* ctx, uint32_t slot, uint32_t n_frame
                                  float* grid_out, float* mean_out, hipStream_t stream);
 insta_status reset_density_grid(insta_ctx* ctx);
 bool train_has_set(const insta_ctx* ctx);
-uint32_t train_step_count(const insta_ctx* ctx);  // optimizer steps taken
+uint32_t train_step_count(const insta_ctx* ctx);  // training steps taken (the reference's m_training_step)
 // snapshot state (snapshot.hip): host copies of the training and density-grid state
 struct TrainSnapshot {
     bool present = false;
-    uint32_t step = 0, rays_per_batch = 0;
+    uint32_t step = 0, rays_per_batch = 0;  // step: training steps taken (snapshot "training_step")
+    uint32_t adam_step = 0;                 // Adam's step (snapshot optimizer "step"), with m and v
     uint64_t rays_total = 0;
     std::vector<float> m, v;  // Adam moments, empty unless saved
     // training set cameras: per image the 3x4 matrix, focal (px), principal point
@@ -227,13 +229,13 @@ struct DensitySnapshot {
     uint32_t step = 0;
     uint64_t rng_state = 0, rng_inc = 0;
 };
-insta_status train_export(insta_ctx* ctx, bool with_optimizer, TrainSnapshot& out);
+insta_status train_export(insta_ctx* ctx, bool with_optimizer, TrainSnapshot& out, hipStream_t stream);
 insta_status train_import(insta_ctx* ctx, const TrainSnapshot& in, hipStream_t stream);
-insta_status density_export(insta_ctx* ctx, DensitySnapshot& out);
+insta_status density_export(insta_ctx* ctx, DensitySnapshot& out, hipStream_t stream);
 // grid, then its mean and bitfield into occupancy slot `slot` (0 canonical, f + 1 frame f)
 insta_status density_import(insta_ctx* ctx, const DensitySnapshot& in, uint32_t slot, hipStream_t stream);
 insta_status get_gradients(insta_ctx* ctx, float* out, hipStream_t stream);
-insta_status reset_optimizer(insta_ctx* ctx);
+insta_status reset_optimizer(insta_ctx* ctx, hipStream_t stream);
 insta_status launch_encode(insta_ctx* ctx, const float* pos, uint32_t n, float* out, hipStream_t stream);
 insta_status launch_encode_backward(insta_ctx* ctx, const float* pos, uint32_t n, const float* dL, float* grad,
                                     hipStream_t stream);

@@ -249,14 +249,16 @@ insta_status update_density_grid(insta_ctx* ctx, uint32_t slot, uint32_t n_frame
     return INSTA_OK;
 }
 
-insta_status density_export(insta_ctx* ctx, DensitySnapshot& out) {
+insta_status density_export(insta_ctx* ctx, DensitySnapshot& out, hipStream_t stream) {
     out = DensitySnapshot{};
     DensityGridState* d = ctx->density;
     if (!d || d->step == 0) return INSTA_OK;  // never populated
     const uint32_t G = ctx->cfg.grid_size, cells = G * G * G;
     out.grid.resize(cells);
-    INSTA_HIP(hipDeviceSynchronize());
-    INSTA_HIP(hipMemcpy(out.grid.data(), d->grid, (size_t)cells * 4, hipMemcpyDeviceToHost));
+    // the grid was last written on the caller's stream: copy there (pageable
+    // destination), without stalling other streams of the device
+    INSTA_HIP(hipMemcpyAsync(out.grid.data(), d->grid, (size_t)cells * 4, hipMemcpyDeviceToHost, stream));
+    INSTA_HIP(hipStreamSynchronize(stream));
     out.step = d->step;
     out.rng_state = d->rng.state;
     out.rng_inc = d->rng.inc;

@@ -44,17 +44,30 @@ enum : uint32_t {
     kNumCounters = 16
 };
 
+// Threads walk the image in 8x8 pixel tiles (one wave per tile), so the live
+// list -- and with it the samples of a march wave -- holds square patches of
+// neighbouring rays: their samples sit close together, the nearest-triangle
+// searches of a wave follow nearly the same BVH path and the hash-grid
+// gathers share cache lines.  Ray state stays indexed by pixel.
+constexpr uint32_t kTile = 8;
+INSTA_HD uint32_t tiled_threads(uint32_t w, uint32_t h) {
+    return ((w + kTile - 1) / kTile) * ((h + kTile - 1) / kTile) * kTile * kTile;
+}
+
 __global__ __launch_bounds__(kRayThreads) void init_rays_kernel(RenderArgs a, float4* __restrict__ r_o,
                                                                 float4* __restrict__ r_d, float4* __restrict__ r_rgba,
                                                                 float* __restrict__ r_depth,
                                                                 uint32_t* __restrict__ r_steps,
                                                                 uint32_t* __restrict__ alive,
                                                                 uint32_t* __restrict__ counters) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t n = a.width * a.height;
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t tiles_x = (a.width + kTile - 1) / kTile;
+    const uint32_t tile = tid / (kTile * kTile), in_tile = tid % (kTile * kTile);
+    const uint32_t x = (tile % tiles_x) * kTile + in_tile % kTile;
+    const uint32_t y = (tile / tiles_x) * kTile + in_tile / kTile;
+    const uint32_t i = y * a.width + x;  // pixel index
     bool live = false;
-    if (i < n) {
-        const uint32_t x = i % a.width, y = i / a.width;
+    if (x < a.width && y < a.height) {
         // pixel_to_ray, pinhole (common_device.cuh:260-315)
         const float u = ((float)x + a.off_x) / (float)a.width;
         const float v = ((float)y + a.off_y) / (float)a.height;
@@ -537,7 +550,8 @@ insta_status enqueue_render(insta_ctx* ctx, RenderWS& ws, const insta_camera* ca
     uint32_t* cnt = ws.d_counters;
     hipLaunchKernelGGL(clear_counters_kernel, dim3(1), dim3(64), 0, stream, cnt, 0u, (uint32_t)kNumCounters);
     prof_begin(ctx, prof, INSTA_KERNEL_INIT, stream);
-    hipLaunchKernelGGL(init_rays_kernel, dim3((n_pix + kRayThreads - 1) / kRayThreads), dim3(kRayThreads), 0, stream, a,
+    hipLaunchKernelGGL(init_rays_kernel, dim3((tiled_threads(width, height) + kRayThreads - 1) / kRayThreads),
+                       dim3(kRayThreads), 0, stream, a,
                        ws.r_o, ws.r_d, ws.r_rgba, ws.r_depth, ws.r_steps, ws.alive[0], cnt);
     INSTA_HIP(hipGetLastError());
     prof_end(ctx, prof, INSTA_KERNEL_INIT, n_pix, stream);
@@ -617,6 +631,8 @@ insta_status set_profiling(insta_ctx* ctx, int level) {
 
 void free_workspace(insta_ctx* ctx) {
     dev_free(ctx->d_trav_counts);
+    if (ctx->aux_stream) (void)hipStreamDestroy(ctx->aux_stream);
+    ctx->aux_stream = nullptr;
     for (int i = 0; i < 2 * INSTA_KERNEL_CLASSES; ++i)
         if (ctx->prof_ev[i]) {
             (void)hipEventDestroy(ctx->prof_ev[i]);

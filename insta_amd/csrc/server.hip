@@ -38,6 +38,7 @@ namespace insta {
 namespace {
 
 constexpr uint64_t kArenaAlign = 256;
+inline uint64_t arena_bytes(uint64_t bytes) { return (bytes + kArenaAlign - 1) / kArenaAlign * kArenaAlign; }
 
 // first-fit allocator over [0, size) with coalescing frees
 class Arena {
@@ -47,7 +48,7 @@ public:
         if (size) free_[0] = size;
     }
     bool alloc(uint64_t bytes, uint64_t& off) {
-        bytes = (bytes + kArenaAlign - 1) / kArenaAlign * kArenaAlign;
+        bytes = arena_bytes(bytes);
         for (auto it = free_.begin(); it != free_.end(); ++it) {
             if (it->second < bytes) continue;
             off = it->first;
@@ -59,7 +60,7 @@ public:
         return false;
     }
     void release(uint64_t off, uint64_t bytes) {
-        bytes = (bytes + kArenaAlign - 1) / kArenaAlign * kArenaAlign;
+        bytes = arena_bytes(bytes);
         auto it = free_.emplace(off, bytes).first;
         auto next = std::next(it);
         if (next != free_.end() && it->first + it->second == next->first) {
@@ -276,13 +277,16 @@ insta_status insta_server_create(const insta_server_config* cfg, insta_server** 
         return INSTA_ERR_OOM;
     }
     S->cfg = *cfg;
+    // the arena hands out 256-byte multiples: a budget that is not one could
+    // accept a request (in submit) that never fits and then blocks the queue
+    S->cfg.hbm_budget_bytes = cfg->hbm_budget_bytes / insta::kArenaAlign * insta::kArenaAlign;
     INSTA_HIP(hipGetDevice(&S->device));
-    if (hipMalloc(reinterpret_cast<void**>(&S->arena_base), cfg->hbm_budget_bytes) != hipSuccess) {
+    if (hipMalloc(reinterpret_cast<void**>(&S->arena_base), S->cfg.hbm_budget_bytes) != hipSuccess) {
         delete S;
         set_error("server_create: cannot reserve the HBM budget");
         return INSTA_ERR_OOM;
     }
-    S->arena.reset(cfg->hbm_budget_bytes);
+    S->arena.reset(S->cfg.hbm_budget_bytes);
     for (uint32_t i = 0; i < cfg->n_slots; ++i) {
         auto sl = std::make_unique<insta_server::Slot>();
         sl->index = i;
@@ -352,7 +356,8 @@ insta_status insta_server_submit(insta_server* S, const insta_request* r, uint64
     p.req = *r;
     p.coeffs.assign(r->coeffs, r->coeffs + S->avatars[r->avatar].n_bases);
     p.bytes = insta::render_ws_bytes(r->width * r->height, S->cfg.samples_per_pixel);
-    if (p.bytes > S->cfg.hbm_budget_bytes) {
+    // compared as the arena will carve it (rounded up to its alignment)
+    if (insta::arena_bytes(p.bytes) > S->cfg.hbm_budget_bytes) {
         set_error("server_submit: the request's workspace exceeds the HBM budget");
         return INSTA_ERR_INVALID;
     }

@@ -407,10 +407,10 @@ insta_status insta_save_snapshot(insta_ctx* ctx, const char* path, int include_o
     INSTA_HIP(hipMemcpyAsync(params.data(), ctx->d_params, np * sizeof(float), hipMemcpyDeviceToHost, s));
     INSTA_HIP(hipStreamSynchronize(s));
     TrainSnapshot ts;
-    insta_status st = train_export(ctx, include_optimizer_state != 0, ts);
+    insta_status st = train_export(ctx, include_optimizer_state != 0, ts, s);
     if (st != INSTA_OK) return st;
     DensitySnapshot ds;
-    st = density_export(ctx, ds);
+    st = density_export(ctx, ds, s);
     if (st != INSTA_OK) return st;
     std::vector<_Float16> grid16(ds.grid.size());
     for (size_t i = 0; i < ds.grid.size(); ++i) grid16[i] = (_Float16)ds.grid[i];
@@ -467,7 +467,7 @@ insta_status insta_save_snapshot(insta_ctx* ctx, const char* path, int include_o
         w.key("optimizer");
         w.map(4);
         w.key("otype"); w.str("Adam");
-        w.key("step"); w.uint(ts.step);
+        w.key("step"); w.uint(ts.adam_step);
         w.key("first_moments_binary"); w.bin(ts.m.data(), ts.m.size() * sizeof(float));
         w.key("second_moments_binary"); w.bin(ts.v.data(), ts.v.size() * sizeof(float));
     }
@@ -590,6 +590,7 @@ insta_status insta_load_snapshot(insta_ctx* ctx, const char* path, int32_t occup
         }
         ts.m.resize(np);
         ts.v.resize(np);
+        if (const MpValue* x = o->get("step"); x && x->uint(v)) ts.adam_step = (uint32_t)v;
         std::memcpy(ts.m.data(), m1->s.data(), np * sizeof(float));
         std::memcpy(ts.v.data(), m2->s.data(), np * sizeof(float));
     }

@@ -36,9 +36,9 @@
 // the ray's frame values by the selector the deformation writes; their
 // columns of W0 get gradients like the encoding's, the values themselves
 // none (latent-code optimisation is off in the reference configuration).
-// Declared simplifications against the reference: no density/colour
-// regularisers, no error map, background not passed through sRGB<->linear,
-// rays never masked out.
+// Training images hold sRGB colour with straight alpha, as the reference's
+// 8-bit images do.  Declared simplifications against the reference: no
+// density/colour regularisers, no error map, rays never masked out.
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
@@ -121,7 +121,12 @@ struct TrainState {
     uint32_t* h_counters = nullptr;
     uint64_t rays_total = 0;
     uint32_t rays_per_batch = 1u << 12;
-    uint32_t step = 0;
+    // two counters, as the reference keeps them: the training step
+    // (Testbed::m_training_step, which times the density-grid updates and
+    // resumes at the saved value + 1 on snapshot load, testbed.cu:3011) and
+    // Adam's own step (bias correction; restarts with fresh moments)
+    uint32_t training_step = 0;
+    uint32_t adam_step = 0;
     bool have_grad = false;
 };
 
@@ -380,6 +385,9 @@ struct LossArgs {
 };
 
 INSTA_DEV float sigmoid(float x) { return 1.0f / (1.0f + expf(-x)); }
+// sRGB transfer functions with the reference's constants (common_device.cuh:31-61)
+INSTA_DEV float srgb_to_linear(float c) { return c <= 0.04045f ? c / 12.92f : powf((c + 0.055f) / 1.055f, 2.4f); }
+INSTA_DEV float linear_to_srgb(float c) { return c < 0.0031308f ? 12.92f * c : 1.055f * powf(c, 0.41666f) - 0.055f; }
 
 __global__ __launch_bounds__(256) void train_loss_kernel(LossArgs a, Pcg32 rng0, const uint32_t* __restrict__ ray_src,
                                                          const uint32_t* __restrict__ ray_img,
@@ -401,10 +409,18 @@ __global__ __launch_bounds__(256) void train_loss_kernel(LossArgs a, Pcg32 rng0,
         rng.advance((uint64_t)ray_src[r] * kRaySeedStride);
         (void)rng.next_float();
         (void)rng.next_float();
-        const float bg[3] = {rng.next_float(), rng.next_float(), rng.next_float()};
+        // linear colour space with sRGB training (testbed_nerf.cu:1419-1443):
+        // the random background goes to linear, is blended behind the
+        // premultiplied linear texel (read_rgba of an 8-bit sRGB image,
+        // common_device.cuh:658-671), then target and background return to sRGB
         const float4 tex = a.images[(size_t)ray_img[r] * a.width * a.height + ray_pix[r]];
-        const float target[3] = {tex.x + (1.0f - tex.w) * bg[0], tex.y + (1.0f - tex.w) * bg[1],
-                                 tex.z + (1.0f - tex.w) * bg[2]};
+        const float texc[3] = {tex.x, tex.y, tex.z};
+        float bg[3], target[3];
+        for (int c = 0; c < 3; ++c) {
+            const float bg_lin = srgb_to_linear(rng.next_float());
+            bg[c] = linear_to_srgb(bg_lin);
+            target[c] = linear_to_srgb(srgb_to_linear(texc[c]) * tex.w + (1.0f - tex.w) * bg_lin);
+        }
         float T = 1.0f;
         for (; used < cnt; ++used) {
             if (T < 1e-4f) break;
@@ -1038,7 +1054,7 @@ insta_status train_step(insta_ctx* ctx, const insta_train_config& tc, insta_trai
              ? (x16 ? launch_train_backward<8, 16>(ctx, ts, tc, stream) : launch_train_backward<8, 0>(ctx, ts, tc, stream))
              : (x16 ? launch_train_backward<16, 16>(ctx, ts, tc, stream) : launch_train_backward<16, 0>(ctx, ts, tc, stream));
     if (st != INSTA_OK) return st;
-    const uint32_t t = ts->step + (tc.apply_update ? 1u : 0u);
+    const uint32_t t = ts->adam_step + (tc.apply_update ? 1u : 0u);
     const float bc1 = tc.apply_update ? 1.0f - (float)std::pow((double)tc.beta1, (double)t) : 1.0f;
     const float bc2 = tc.apply_update ? 1.0f - (float)std::pow((double)tc.beta2, (double)t) : 1.0f;
     hipLaunchKernelGGL(adam_kernel, dim3(2048), dim3(256), 0, stream, ctx->d_params, ts->grad,
@@ -1065,8 +1081,14 @@ insta_status train_step(insta_ctx* ctx, const insta_train_config& tc, insta_trai
     }
     ts->rays_total += n_rays;
     ts->have_grad = true;
-    if (tc.apply_update) ts->step = t;
-    if (stats) stats->step = ts->step;
+    if (tc.apply_update) {
+        ts->adam_step = t;
+        ts->training_step += 1;
+    }
+    if (stats) {
+        stats->step = ts->training_step;
+        stats->adam_step = ts->adam_step;
+    }
     // adaptive ray count (testbed_nerf.cu:3402-3410): rays * target /
     // compacted batch, rounded up to a multiple of 128, at most 2^18 rays
     const uint32_t measured = std::max(std::min(hc[kTUsed], ts->compact_cap), 1u);
@@ -1078,7 +1100,7 @@ insta_status train_step(insta_ctx* ctx, const insta_train_config& tc, insta_trai
 
 bool train_has_gradients(const insta_ctx* ctx) { return ctx->train && ctx->train->have_grad; }
 bool train_has_set(const insta_ctx* ctx) { return ctx->train && ctx->train->n_images > 0; }
-uint32_t train_step_count(const insta_ctx* ctx) { return ctx->train ? ctx->train->step : 0u; }
+uint32_t train_step_count(const insta_ctx* ctx) { return ctx->train ? ctx->train->training_step : 0u; }
 
 insta_status get_gradients(insta_ctx* ctx, float* out, hipStream_t stream) {
     INSTA_HIP(hipMemcpyAsync(out, ctx->train->grad, ctx->layout.n_params * sizeof(float), hipMemcpyDeviceToDevice,
@@ -1086,18 +1108,20 @@ insta_status get_gradients(insta_ctx* ctx, float* out, hipStream_t stream) {
     return INSTA_OK;
 }
 
-insta_status train_export(insta_ctx* ctx, bool with_optimizer, TrainSnapshot& out) {
+insta_status train_export(insta_ctx* ctx, bool with_optimizer, TrainSnapshot& out, hipStream_t stream) {
     out = TrainSnapshot{};
     TrainState* ts = ctx->train;
     if (!ts) return INSTA_OK;
     out.present = true;
-    out.step = ts->step;
+    out.step = ts->training_step;
+    out.adam_step = ts->adam_step;
     out.rays_per_batch = ts->rays_per_batch;
     out.rays_total = ts->rays_total;
     if (ts->n_images && ts->meta) {
         std::vector<TrainImage> meta(ts->n_images);
-        INSTA_HIP(hipDeviceSynchronize());
-        INSTA_HIP(hipMemcpy(meta.data(), ts->meta, meta.size() * sizeof(TrainImage), hipMemcpyDeviceToHost));
+        INSTA_HIP(hipMemcpyAsync(meta.data(), ts->meta, meta.size() * sizeof(TrainImage), hipMemcpyDeviceToHost,
+                                 stream));
+        INSTA_HIP(hipStreamSynchronize(stream));
         out.n_images = ts->n_images;
         out.width = ts->width;
         out.height = ts->height;
@@ -1115,9 +1139,9 @@ insta_status train_export(insta_ctx* ctx, bool with_optimizer, TrainSnapshot& ou
         const uint64_t np = ctx->layout.n_params;
         out.m.resize(np);
         out.v.resize(np);
-        INSTA_HIP(hipDeviceSynchronize());
-        INSTA_HIP(hipMemcpy(out.m.data(), ts->adam_m, np * sizeof(float), hipMemcpyDeviceToHost));
-        INSTA_HIP(hipMemcpy(out.v.data(), ts->adam_v, np * sizeof(float), hipMemcpyDeviceToHost));
+        INSTA_HIP(hipMemcpyAsync(out.m.data(), ts->adam_m, np * sizeof(float), hipMemcpyDeviceToHost, stream));
+        INSTA_HIP(hipMemcpyAsync(out.v.data(), ts->adam_v, np * sizeof(float), hipMemcpyDeviceToHost, stream));
+        INSTA_HIP(hipStreamSynchronize(stream));
     }
     return INSTA_OK;
 }
@@ -1127,28 +1151,31 @@ insta_status train_import(insta_ctx* ctx, const TrainSnapshot& in, hipStream_t s
     if (st != INSTA_OK) return st;
     TrainState* ts = ctx->train;
     const uint64_t np = ctx->layout.n_params;
-    ts->step = in.step;
+    // Testbed::load_snapshot resumes at training_step + 1 (testbed.cu:3011)
+    ts->training_step = in.step + 1;
     ts->rays_per_batch = in.rays_per_batch;
     ts->rays_total = in.rays_total;
     if (in.m.size() == np && in.v.size() == np) {
         INSTA_HIP(hipMemcpyAsync(ts->adam_m, in.m.data(), np * sizeof(float), hipMemcpyHostToDevice, stream));
         INSTA_HIP(hipMemcpyAsync(ts->adam_v, in.v.data(), np * sizeof(float), hipMemcpyHostToDevice, stream));
         INSTA_HIP(hipStreamSynchronize(stream));  // pageable sources
+        ts->adam_step = in.adam_step;
     } else {
         INSTA_HIP(hipMemsetAsync(ts->adam_m, 0, np * sizeof(float), stream));
         INSTA_HIP(hipMemsetAsync(ts->adam_v, 0, np * sizeof(float), stream));
-        ts->step = 0;  // fresh moments restart the bias correction
+        ts->adam_step = 0;  // fresh moments restart the bias correction
     }
     return INSTA_OK;
 }
 
-insta_status reset_optimizer(insta_ctx* ctx) {
+insta_status reset_optimizer(insta_ctx* ctx, hipStream_t stream) {
     TrainState* ts = ctx->train;
     const uint64_t np = ctx->layout.n_params;
-    INSTA_HIP(hipMemset(ts->adam_m, 0, np * sizeof(float)));
-    INSTA_HIP(hipMemset(ts->adam_v, 0, np * sizeof(float)));
-    INSTA_HIP(hipDeviceSynchronize());
-    ts->step = 0;
+    // stream-ordered fill kernels: no device-wide stall, capturable
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(1024), dim3(256), 0, stream, reinterpret_cast<uint32_t*>(ts->adam_m), np);
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(1024), dim3(256), 0, stream, reinterpret_cast<uint32_t*>(ts->adam_v), np);
+    INSTA_HIP(hipGetLastError());
+    ts->adam_step = 0;
     return INSTA_OK;
 }
 

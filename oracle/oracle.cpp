@@ -40,6 +40,20 @@
 #include <omp.h>
 #endif
 
+// Planted mistakes, for showing that the pinning tests are live
+// (tests/test_oracle_pinning.py builds this file with -DORC_MUTANT=k into a
+// scratch directory and expects each pinning check to fail on its mutant):
+//   1  interpolate_transformation: self and neighbour factors swapped (1 <-> 4)
+//   2  spherical harmonics: sign of the first band-1 coefficient flipped
+//   3  Huber loss: threshold 0.1 -> 0.2
+//   4  march: the one-step start jitter of advance_pos_nerf dropped
+//   5  training target: background blended in stored (sRGB) values, not linear
+//   6  render buffer: spp accumulation weights 1/spp instead of 1/(spp + 1)
+// The default build (ORC_MUTANT 0) has none of them.
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+
 namespace {
 
 // ---------------------------------------------------------------------------
@@ -522,7 +536,7 @@ inline void ld_random_pixel_offset(uint32_t spp, float out[2]) {
 void sh16(float x, float y, float z, float* o) {
     const float xy = x * y, xz = x * z, yz = y * z, x2 = x * x, y2 = y * y, z2 = z * z;
     o[0] = 0.28209479177387814f;
-    o[1] = -0.48860251190291987f * y;
+    o[1] = (ORC_MUTANT == 2 ? 0.48860251190291987f : -0.48860251190291987f) * y;
     o[2] = 0.48860251190291987f * z;
     o[3] = -0.48860251190291987f * x;
     o[4] = 1.0925484305920792f * xy;
@@ -614,22 +628,24 @@ void fill_frame(orc_scene* s, MeshFrame& mf, const float* verts, const MeshFrame
 // interpolate_transformation (include/rta/affine.cuh:45-100) without the
 // final inverse: exp-weighted blend of the triangle's own frame (factor 1)
 // and its edge neighbours' frames (factor 4), weights normalised.
+constexpr float kBlendSelf = ORC_MUTANT == 1 ? 4.0f : 1.0f;       // affine.cuh:48
+constexpr float kBlendNeighbour = ORC_MUTANT == 1 ? 1.0f : 4.0f;  // affine.cuh:56, 60
 void blend_frames(const orc_scene* s, const MeshFrame& mf, uint32_t id, V3 p, float out[12]) {
     const int32_t* adj = &s->adjacency[3 * id];
     float sum = 0.0f;
     for (int k = 0; k < 3; ++k)
-        if (adj[k] >= 0) sum += std::exp(-4.0f * norm(mf.cen[adj[k]] - p));
-    sum += std::exp(-1.0f * norm(mf.cen[id] - p));
+        if (adj[k] >= 0) sum += std::exp(-kBlendNeighbour * norm(mf.cen[adj[k]] - p));
+    sum += std::exp(-kBlendSelf * norm(mf.cen[id] - p));
     for (int i = 0; i < 12; ++i) out[i] = 0.0f;
     for (int k = 0; k < 3; ++k) {
         if (adj[k] >= 0) {
-            float a = std::exp(-4.0f * norm(mf.cen[adj[k]] - p));
+            float a = std::exp(-kBlendNeighbour * norm(mf.cen[adj[k]] - p));
             float w = a / sum;
             const float* m = mf.proj[adj[k]].m;
             for (int i = 0; i < 12; ++i) out[i] = out[i] + w * m[i];
         }
     }
-    float a = std::exp(-1.0f * norm(mf.cen[id] - p));
+    float a = std::exp(-kBlendSelf * norm(mf.cen[id] - p));
     float w = a / sum;
     const float* m = mf.proj[id].m;
     for (int i = 0; i < 12; ++i) out[i] = out[i] + w * m[i];
@@ -1364,7 +1380,7 @@ int orc_render(const orc_scene* s, const insta_camera* cam, uint32_t width, uint
         if (!box_contains(lo, hi, o + d * t)) continue;
         const V3 idir = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
         // advance_pos_nerf: jitter by one step, skip empty space
-        t += ld_random_val(spp, i * 786433u, 0) * DT;
+        t += (ORC_MUTANT == 4 ? 0.0f : ld_random_val(spp, i * 786433u, 0)) * DT;
         bool alive = true;
         while (true) {
             V3 pos = o + d * t;
@@ -1576,6 +1592,12 @@ struct Pcg32 {
 };
 
 constexpr uint32_t kRaySeedStride = 8;  // N_MAX_RANDOM_SAMPLES_PER_RAY (testbed_nerf.cu:60)
+constexpr double kHuberDelta = ORC_MUTANT == 3 ? 0.2 : 0.1;  // testbed_nerf.cu:1299
+
+// sRGB transfer functions (common_device.cuh:31-61), the reference's
+// constants (its encoding exponent is 0.41666, not 1/2.4)
+inline double srgb_to_linear_d(double c) { return c <= 0.04045 ? c / 12.92 : std::pow((c + 0.055) / 1.055, 2.4); }
+inline double linear_to_srgb_d(double c) { return c < 0.0031308 ? 12.92 * c : 1.055 * std::pow(c, 0.41666) - 0.055; }
 
 struct TrainRay {
     uint32_t src, img, pix;
@@ -1837,8 +1859,21 @@ int orc_train_grads(const orc_scene* s, uint32_t n_rays, uint32_t seed, uint64_t
         const float* tex = &s->tr_images[((size_t)r.img * s->tr_w * s->tr_h + r.pix) * 4];
         double target[3], bg[3];
         for (int k = 0; k < 3; ++k) {
-            bg[k] = r.bg[k];
-            target[k] = (double)tex[k] + (1.0 - (double)tex[3]) * bg[k];
+            // compute_loss_kernel_train_nerf, linear colour space, sRGB
+            // training (testbed_nerf.cu:1419-1443): the random background is
+            // taken to linear, blended behind the premultiplied linear texel
+            // (read_rgba of an 8-bit image, common_device.cuh:658-671), and
+            // both the target and the background behind the prediction go
+            // back to sRGB
+            const double bg_lin = srgb_to_linear_d(r.bg[k]);
+            const double a = tex[3];
+            if (ORC_MUTANT == 5) {
+                bg[k] = r.bg[k];
+                target[k] = (double)tex[k] + (1.0 - a) * bg[k];
+            } else {
+                bg[k] = linear_to_srgb_d(bg_lin);
+                target[k] = linear_to_srgb_d(srgb_to_linear_d(tex[k]) * a + (1.0 - a) * bg_lin);
+            }
         }
         double T = 1.0, acc[3] = {0, 0, 0};
         uint32_t used = 0;
@@ -1856,9 +1891,10 @@ int orc_train_grads(const orc_scene* s, uint32_t n_rays, uint32_t seed, uint64_t
             for (int k = 0; k < 3; ++k) acc[k] += T * bg[k];
         double grad[3], lsum = 0.0;
         for (int k = 0; k < 3; ++k) {
+            // huber_loss(target, prediction, 0.1) / 5 (testbed_nerf.cu:141-157, 1299)
             const double diff = acc[k] - target[k], ad = std::fabs(diff);
-            const double l = ad > 0.1 ? ad - 0.05 : (0.5 / 0.1) * diff * diff;
-            grad[k] = (ad > 0.1 ? (diff > 0 ? 1.0 : -1.0) : diff / 0.1) / 5.0;
+            const double l = ad > kHuberDelta ? ad - 0.5 * kHuberDelta : (0.5 / kHuberDelta) * diff * diff;
+            grad[k] = (ad > kHuberDelta ? (diff > 0 ? 1.0 : -1.0) : diff / kHuberDelta) / 5.0;
             lsum += l / 5.0;
         }
         rep += lsum / 3.0 * inv_rays;

@@ -65,21 +65,24 @@ _SIGS = [
     ("orc_get_occupancy", C.c_int, [_VP, C.c_int32, _VP]),
 ]
 
-_lib = None
+_libs: dict = {}
 
 
-def lib() -> C.CDLL:
-    global _lib
-    if _lib is None:
-        if not LIB_PATH.exists():
-            raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle`")
-        L = C.CDLL(str(LIB_PATH))
+def lib(path=None) -> C.CDLL:
+    """The oracle library (default: liboracle.so next to this file; the
+    pinning tests pass a build with a planted mistake)."""
+    path = Path(path) if path is not None else LIB_PATH
+    key = str(path)
+    if key not in _libs:
+        if not path.exists():
+            raise RuntimeError(f"{path} missing: run `make -C oracle`")
+        L = C.CDLL(key)
         for name, res, args in _SIGS:
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
-        _lib = L
-    return _lib
+        _libs[key] = L
+    return _libs[key]
 
 
 def _p(a: np.ndarray | None):
@@ -93,8 +96,8 @@ def _c(a, dtype) -> np.ndarray:
 class Oracle:
     """CPU mirror of an insta_ctx: same call sequence, host arrays."""
 
-    def __init__(self, cfg: InstaConfig):
-        self.L = lib()
+    def __init__(self, cfg: InstaConfig, lib_path=None):
+        self.L = lib(lib_path)
         self.cfg = cfg
         self.h = self.L.orc_create(C.byref(cfg))
         if not self.h:
@@ -295,15 +298,15 @@ def render_fp64(orc: Oracle, cam: InstaCamera, width: int, height: int, frame: i
     return rgba
 
 
-def sh16(dirs_w) -> np.ndarray:
+def sh16(dirs_w, lib_path=None) -> np.ndarray:
     d = _c(dirs_w, np.float32)
     out = np.zeros((len(d), 16), np.float32)
-    lib().orc_sh16(_p(d), len(d), _p(out))
+    lib(lib_path).orc_sh16(_p(d), len(d), _p(out))
     return out
 
 
-def ld_random_val(index: int, seed: int, dim: int = 0) -> float:
-    return lib().orc_ld_random_val(index, seed, dim)
+def ld_random_val(index: int, seed: int, dim: int = 0, lib_path=None) -> float:
+    return lib(lib_path).orc_ld_random_val(index, seed, dim)
 
 
 def sobol(index: int, dim: int) -> int:

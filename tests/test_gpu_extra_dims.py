@@ -11,7 +11,7 @@ import pytest
 
 from insta_amd import synth
 from insta_amd.capi import Context, default_train_config, make_camera, small_config
-from tests.conftest import assert_rel_close, ensure_oracle
+from tests.conftest import assert_rel_close, assert_render_matches, ensure_oracle
 
 pytestmark = pytest.mark.gpu
 RTOL = 1e-5
@@ -74,9 +74,8 @@ def test_render_matches_oracle(avatar, prec):
         cam = _lips_camera(avatar, res, frame)
         rgba, depth, steps = (t.cpu().numpy() for t in ctx.render(cam, res, res, frame, frame, 0))
         r_rgba, r_depth, r_steps, _ = orc.render(cam, res, res, frame, frame)
-        same = steps.astype(np.uint32) == r_steps
-        assert same.mean() >= 0.995
-        assert_rel_close(rgba[same], r_rgba[same], RTOL, f"rgba frame {frame} precision {prec}")
+        assert_render_matches(rgba, steps, r_rgba, r_steps, RTOL, ctx.cfg.min_transmittance,
+                              f"rgba frame {frame} precision {prec}")
     # the values reach the image: another frame row changes it
     cam = _lips_camera(avatar, res, 0)
     a = ctx.render(cam, res, res, 0, 0, 0)[0].cpu().numpy()
@@ -202,9 +201,9 @@ def test_full_size_render_matches_oracle():
     cam = make_camera(synth.look_at_camera(c / np.linalg.norm(c), 0.7), synth.focal_from_fov(res, 25.0))
     rgba, _, steps = (t.cpu().numpy() for t in ctx.render(cam, res, res, 0, 0, 0))
     r_rgba, _, r_steps, _ = orc.render(cam, res, res, 0, 0)
-    same = steps.astype(np.uint32) == r_steps
-    assert same.mean() >= 0.99 and (r_steps > 0).sum() > 100 and r_rgba[:, 3].max() > 0.05
-    assert_rel_close(rgba[same], r_rgba[same], RTOL, "full-size rgba with extra dims")
+    assert (r_steps > 0).sum() > 100 and r_rgba[:, 3].max() > 0.05
+    assert_render_matches(rgba, steps, r_rgba, r_steps, RTOL, ctx.cfg.min_transmittance,
+                          "full-size rgba with extra dims", max_mismatch=0.01)
 
 
 def test_frame_extra_dims_errors():

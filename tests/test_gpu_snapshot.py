@@ -95,7 +95,10 @@ def test_snapshot_round_trip(tmp_path):
     s1 = ctx.train_step(tc)
     s2 = ctx2.train_step(tc)
     torch.cuda.synchronize()
-    assert s1.step == s2.step == 4 and s1.n_rays == s2.n_rays and s1.n_samples == s2.n_samples
+    # Testbed::load_snapshot resumes at training_step + 1 (testbed.cu:3011);
+    # Adam continues from the saved moments and step
+    assert s1.step == 4 and s2.step == 5 and s1.adam_step == s2.adam_step == 4
+    assert s1.n_rays == s2.n_rays and s1.n_samples == s2.n_samples
     assert s1.n_samples_used == s2.n_samples_used and s1.loss == pytest.approx(s2.loss, rel=1e-6)
     g1, g2 = ctx.gradients().cpu().numpy(), ctx2.gradients().cpu().numpy()
     nm = ctx.layout.n_mlp_params
@@ -104,6 +107,30 @@ def test_snapshot_round_trip(tmp_path):
     # sign noise of near-zero gradients into full steps, so compare gradients)
     for sl in (slice(0, nm), slice(nm, None)):
         assert np.linalg.norm(g1[sl] - g2[sl]) <= 1e-5 * np.linalg.norm(g1[sl])
+
+
+def test_reload_without_optimizer_state_keeps_the_training_step(tmp_path):
+    """Without saved moments Adam restarts (bias correction from step 1)
+    while the training step -- which times the density-grid updates of the
+    training loop -- resumes at the saved value + 1, as the reference's
+    load_snapshot does whatever the optimizer state (testbed.cu:3011)."""
+    import torch
+    scene, ctx, tc, _ = _trained(steps=4)
+    path = tmp_path / "no_moments.msgpack"
+    ctx.save_snapshot(path)
+    doc = msgpack.unpackb(path.read_bytes(), raw=False)
+    assert doc["snapshot"]["training_step"] == 4 and "optimizer" not in doc["snapshot"]
+    ctx2, _ = _gpu(scene)
+    ctx2.load_snapshot(path, occupancy_slot=-1)
+    s = ctx2.train_step(tc)
+    torch.cuda.synchronize()
+    assert s.step == 6 and s.adam_step == 1
+    # the loop's schedule reads the training step: at step 6 an update is due
+    # every clamp(6 / 16, 1, 16) = 1 steps, past the 256-step warm-up it
+    # would be every 16; the library's own loop keeps counting from there
+    last = ctx2.train_loop(tc, 2, density_slot=-1, n_frames=N_FRAMES)
+    torch.cuda.synchronize()
+    assert last.step == 8 and last.adam_step == 3
 
 
 def test_load_rejects_another_network_shape(tmp_path):

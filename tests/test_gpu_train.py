@@ -7,7 +7,7 @@ import pytest
 
 from insta_amd import synth
 from insta_amd.capi import Context, default_train_config, make_camera, small_config
-from tests.conftest import assert_rel_close, ensure_oracle
+from tests.conftest import assert_rel_close, assert_render_matches, ensure_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -181,9 +181,8 @@ def test_render_after_training_matches_oracle(scene):
     rgba, depth, steps = ctx.render(cam, 48, 48, 1, 1, 0)
     torch.cuda.synchronize()
     o_rgba, o_depth, o_steps, _ = orc.render(cam, 48, 48, 1, 1, 0)
-    assert (steps.cpu().numpy() == o_steps.astype(np.int32)).mean() > 0.995
-    same = steps.cpu().numpy() == o_steps.astype(np.int32)
-    assert_rel_close(rgba.cpu().numpy()[same], o_rgba[same], 1e-4, "rgba after training")
+    assert_render_matches(rgba.cpu().numpy(), steps.cpu().numpy(), o_rgba, o_steps, 1e-4, ctx.cfg.min_transmittance,
+                          "rgba after training")
 
 
 def test_one_occupancy_slot_for_every_ray_matches_fp64(scene):
