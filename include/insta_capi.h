@@ -105,6 +105,9 @@ typedef struct insta_render_stats {
     uint64_t n_samples;        /* samples composited (the deformed-sample count) */
     uint64_t n_samples_emitted;/* samples marched, deformed and evaluated */
     uint32_t n_iterations;     /* march/deform/network/composite rounds */
+    uint32_t n_deform_misses;  /* samples inside the unit cube whose nearest-triangle query
+                                  found nothing and were left undeformed (the reference's
+                                  `id == -1` path of warp, src/core.cu:482-485) */
 } insta_render_stats;
 
 /* Parameter layout: density MLP weights, colour MLP weights, hash tables.
@@ -245,6 +248,35 @@ insta_status insta_encode_backward(insta_ctx* ctx, const float* positions_dev, u
 insta_status insta_network_inference(insta_ctx* ctx, const float* coords_dev, uint32_t n,
                                      uint32_t stride_floats, float* out_dev, void* stream);
 
+/* ---- network plugin (NerfNetwork : tcnn::Network, nerf_network.h:131-652) --
+ * The caller-driven forward/backward of the reference's network on
+ * NerfCoordinate samples (stride_floats = 11 + n_extra_dims; the density
+ * input's extra dims are each sample's last n_extra_dims floats).  They run
+ * the training step's kernels: FP32 hash grid and FP32 MFMA MLPs whatever the
+ * context's inference precision.  Shapes as insta_train_step supports
+ * (F = 2, L = 8 or 16, width 64, 1 density / 2 colour hidden layers). */
+
+/* forward_impl (nerf_network.h:238-294): out_dev [n*4] = rgb logits and the
+ * density logit; keeps the activations the backward needs in the context
+ * (one stored forward per context; insta_train_step overwrites it). */
+insta_status insta_network_forward(insta_ctx* ctx, const float* coords_dev, uint32_t n, uint32_t stride_floats,
+                                   float* out_dev, void* stream);
+/* backward_impl (nerf_network.h:296-383) of the stored forward (same
+ * coordinates and n): dL_doutput_dev [n*4] (rgb logits, density logit) ->
+ * grad_params_dev [layout.n_params] in the insta_set_params order,
+ * overwritten (accumulate = 0, EGradientMode::Overwrite) or added to
+ * (accumulate = 1); dL_dinput_dev (nullable) [n * stride_floats] receives the
+ * position rows 0..2 and the direction rows 8..10 (the input the
+ * direction encoding reads, in [0,1]); other rows are left as they are. */
+insta_status insta_network_backward(insta_ctx* ctx, const float* coords_dev, uint32_t n, uint32_t stride_floats,
+                                    const float* dL_doutput_dev, float* grad_params_dev, int accumulate,
+                                    float* dL_dinput_dev, void* stream);
+/* density (nerf_network.h:385-414): positions_dev [n*3] -> out_dev [n*16],
+ * the density MLP's 16 outputs (output 0 the density logit), extra dims 1.0
+ * (extract_expressions_random, nerf_network.h:107-127). */
+insta_status insta_network_density(insta_ctx* ctx, const float* positions_dev, uint32_t n, float* out_dev,
+                                   void* stream);
+
 /* Testbed::render_frame -> render_nerf -> NerfTracer::{init_rays_from_camera,
  * trace} with the render_deformed hook (testbed_nerf.cu:2030-2244,
  * 2320-2461; core.cu:346-348): renders `frame` through occupancy slot
@@ -258,6 +290,58 @@ insta_status insta_render(insta_ctx* ctx, const insta_camera* cam, uint32_t widt
 
 /* Counters of the last insta_render on this context. */
 insta_status insta_get_render_stats(insta_ctx* ctx, insta_render_stats* out);
+
+/* Samples the AoS deformation entry points (insta_deform_samples,
+ * insta_deform_positions) left undeformed because their nearest-triangle
+ * query found nothing (src/core.cu:482-485, 554-557), summed since the
+ * context was created or last reset (reset != 0 zeroes it after reading).
+ * Stream-ordered read (synchronises `stream`). */
+insta_status insta_get_deform_misses(insta_ctx* ctx, uint64_t* misses, int reset, void* stream);
+
+/* ---- render buffer (Testbed::render_frame's CudaRenderBuffer) -------- */
+
+enum {
+    INSTA_TONEMAP_IDENTITY = 0,  /* ETonemapCurve (render_buffer.h), the reference default */
+    INSTA_TONEMAP_ACES = 1,
+    INSTA_TONEMAP_HABLE = 2,
+    INSTA_TONEMAP_REINHARD = 3
+};
+
+typedef struct insta_frame_config {
+    float    exposure;        /* m_exposure (testbed.h:489): colour * 2^exposure in linear space */
+    float    background[4];   /* m_background_color, sRGB + weight (testbed.h:817: 1,1,1,1) */
+    uint32_t tonemap_curve;   /* INSTA_TONEMAP_* */
+    uint32_t output_srgb;     /* render_frame(to_srgb): 1 sRGB output, 0 linear */
+    uint32_t linear_colors;   /* Nerf::Training::linear_colors: 0 (default) -> shade converts the
+                                 network's sRGB colour to linear before accumulating */
+} insta_frame_config;
+
+/* exposure 0, white background, identity curve, sRGB output, sRGB network colours */
+void insta_default_frame_config(insta_frame_config* fc);
+
+/* The render-buffer stages of Testbed::render_frame
+ * (instant-ngp/src/testbed.cu:2613-2775) for one sample index `spp`, on a
+ * composited RGBA image as insta_render writes it (rgba_dev [n*4], 0 where
+ * alpha <= 0.001, depth_dev [n] nullable): shade_kernel_nerf over a cleared
+ * frame buffer (sRGB -> linear unless linear_colors; depth buffer = depth
+ * where alpha > 0.2, else 0; testbed_nerf.cu:1794-1827), accumulate_kernel
+ * into accum_dev [n*4] (running mean over spp = 0, 1, 2, ...; spp 0 starts
+ * afresh; render_buffer.cu:236-266, 560-578) and tonemap_kernel (background
+ * blend in linear space, exposure, curve, output colour space;
+ * render_buffer.cu:493-522) into out_dev [n*4].  As the reference's
+ * tonemap_kernel does, the tonemapped colour is also written back into the
+ * accumulation buffer, which the next sample index averages with.
+ * depth_out_dev [n] (nullable) receives the depth buffer. */
+insta_status insta_frame_buffer(const float* rgba_dev, const float* depth_dev, uint32_t n, uint32_t spp,
+                                const insta_frame_config* fc, float* accum_dev, float* out_dev,
+                                float* depth_out_dev, void* stream);
+
+/* Testbed::render_frame: insta_render at sample index `spp` followed by
+ * insta_frame_buffer (out_dev receives the tonemapped frame; accum_dev the
+ * caller's accumulation buffer across sample indices). */
+insta_status insta_render_frame(insta_ctx* ctx, const insta_camera* cam, uint32_t width, uint32_t height,
+                                int32_t frame, int32_t occupancy, uint32_t spp, const insta_frame_config* fc,
+                                float* accum_dev, float* out_dev, float* depth_out_dev, void* stream);
 
 /* ---- training step --------------------------------------------------- */
 

@@ -45,7 +45,15 @@ class InstaCamera(C.Structure):
 
 class InstaRenderStats(C.Structure):
     _fields_ = [("n_rays", C.c_uint64), ("n_rays_alive0", C.c_uint64), ("n_samples", C.c_uint64),
-                ("n_samples_emitted", C.c_uint64), ("n_iterations", C.c_uint32)]
+                ("n_samples_emitted", C.c_uint64), ("n_iterations", C.c_uint32), ("n_deform_misses", C.c_uint32)]
+
+
+TONEMAP_IDENTITY, TONEMAP_ACES, TONEMAP_HABLE, TONEMAP_REINHARD = 0, 1, 2, 3
+
+
+class InstaFrameConfig(C.Structure):
+    _fields_ = [("exposure", C.c_float), ("background", C.c_float * 4), ("tonemap_curve", C.c_uint32),
+                ("output_srgb", C.c_uint32), ("linear_colors", C.c_uint32)]
 
 
 class InstaLayout(C.Structure):
@@ -128,9 +136,18 @@ SIGNATURES = [
     ("insta_encode", C.c_int, [_VP, _VP, C.c_uint32, _VP, _VP]),
     ("insta_encode_backward", C.c_int, [_VP, _VP, C.c_uint32, _VP, _VP, _VP]),
     ("insta_network_inference", C.c_int, [_VP, _VP, C.c_uint32, C.c_uint32, _VP, _VP]),
+    ("insta_network_forward", C.c_int, [_VP, _VP, C.c_uint32, C.c_uint32, _VP, _VP]),
+    ("insta_network_backward", C.c_int, [_VP, _VP, C.c_uint32, C.c_uint32, _VP, _VP, C.c_int, _VP, _VP]),
+    ("insta_network_density", C.c_int, [_VP, _VP, C.c_uint32, _VP, _VP]),
     ("insta_render", C.c_int, [_VP, C.POINTER(InstaCamera), C.c_uint32, C.c_uint32, C.c_int32, C.c_int32,
                                C.c_uint32, _VP, _VP, _VP, _VP]),
     ("insta_get_render_stats", C.c_int, [_VP, C.POINTER(InstaRenderStats)]),
+    ("insta_get_deform_misses", C.c_int, [_VP, C.POINTER(C.c_uint64), C.c_int, _VP]),
+    ("insta_default_frame_config", None, [C.POINTER(InstaFrameConfig)]),
+    ("insta_frame_buffer", C.c_int, [_VP, _VP, C.c_uint32, C.c_uint32, C.POINTER(InstaFrameConfig), _VP, _VP, _VP,
+                                     _VP]),
+    ("insta_render_frame", C.c_int, [_VP, C.POINTER(InstaCamera), C.c_uint32, C.c_uint32, C.c_int32, C.c_int32,
+                                     C.c_uint32, C.POINTER(InstaFrameConfig), _VP, _VP, _VP, _VP]),
     ("insta_save_snapshot", C.c_int, [_VP, C.c_char_p, C.c_int, _VP]),
     ("insta_load_snapshot", C.c_int, [_VP, C.c_char_p, C.c_int32, _VP]),
     ("insta_snapshot_config", C.c_int, [C.c_char_p, C.POINTER(InstaConfig)]),
@@ -384,6 +401,33 @@ class Context:
               "insta_network_inference")
         return out
 
+    # --- network plugin (NerfNetwork forward / backward / density) ----------
+    def network_forward(self, coords_dev):
+        n, stride = coords_dev.shape
+        out = self.torch.empty((n, 4), dtype=self.torch.float32, device=coords_dev.device)
+        check(self.lib.insta_network_forward(self.h, _ptr(coords_dev), n, stride, _ptr(out), self.stream()),
+              "insta_network_forward")
+        return out
+
+    def network_backward(self, coords_dev, dL_dout_dev, grads_dev=None, accumulate: bool = False,
+                         dL_dinput_dev=None):
+        """Parameter gradients of the stored forward (a new tensor unless
+        grads_dev is given); dL_dinput_dev [n, stride] receives the position
+        and direction rows when given."""
+        n, stride = coords_dev.shape
+        if grads_dev is None:
+            grads_dev = self.torch.empty(self.layout.n_params, dtype=self.torch.float32, device=coords_dev.device)
+        check(self.lib.insta_network_backward(self.h, _ptr(coords_dev), n, stride, _ptr(dL_dout_dev), _ptr(grads_dev),
+                                              1 if accumulate else 0, _ptr(dL_dinput_dev), self.stream()),
+              "insta_network_backward")
+        return grads_dev
+
+    def network_density(self, pos_dev):
+        out = self.torch.empty((pos_dev.shape[0], 16), dtype=self.torch.float32, device=pos_dev.device)
+        check(self.lib.insta_network_density(self.h, _ptr(pos_dev), pos_dev.shape[0], _ptr(out), self.stream()),
+              "insta_network_density")
+        return out
+
     # --- render -----------------------------------------------------------
     def render(self, cam: InstaCamera, width: int, height: int, frame: int, occupancy: int, spp: int = 0,
                out=None):
@@ -396,6 +440,23 @@ class Context:
         check(self.lib.insta_render(self.h, C.byref(cam), width, height, frame, occupancy, spp, _ptr(rgba),
                                     _ptr(depth), _ptr(steps), self.stream()), "insta_render")
         return out
+
+    def render_frame(self, cam: InstaCamera, width: int, height: int, frame: int, occupancy: int, spp: int,
+                     fc: "InstaFrameConfig", accum, out=None, depth=None):
+        """Testbed::render_frame at sample index spp: render, shade,
+        accumulate into `accum` [h*w, 4], tonemap into `out` (returned)."""
+        torch = self.torch
+        if out is None:
+            out = torch.empty((height * width, 4), dtype=torch.float32, device="cuda")
+        check(self.lib.insta_render_frame(self.h, C.byref(cam), width, height, frame, occupancy, spp, C.byref(fc),
+                                          _ptr(accum), _ptr(out), _ptr(depth), self.stream()), "insta_render_frame")
+        return out
+
+    def deform_misses(self, reset: bool = False) -> int:
+        v = C.c_uint64()
+        check(self.lib.insta_get_deform_misses(self.h, C.byref(v), 1 if reset else 0, self.stream()),
+              "insta_get_deform_misses")
+        return int(v.value)
 
     def set_profiling(self, on, count_traversal: bool = False) -> None:
         level = (2 if count_traversal else 1) if on else 0
@@ -473,6 +534,26 @@ class Context:
         st = InstaRenderStats()
         check(self.lib.insta_get_render_stats(self.h, C.byref(st)), "insta_get_render_stats")
         return st
+
+
+def default_frame_config(**kw) -> InstaFrameConfig:
+    fc = InstaFrameConfig()
+    load_library().insta_default_frame_config(C.byref(fc))
+    for k, v in kw.items():
+        if k == "background":
+            for i in range(4):
+                fc.background[i] = float(v[i])
+        else:
+            setattr(fc, k, v)
+    return fc
+
+
+def frame_buffer(rgba, depth, spp: int, fc: InstaFrameConfig, accum, out, depth_out=None, stream=None) -> None:
+    """insta_frame_buffer on device tensors (rgba [n, 4], accum/out [n, 4])."""
+    import torch
+    s = stream if stream is not None else C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    check(load_library().insta_frame_buffer(_ptr(rgba), _ptr(depth), rgba.shape[0], spp, C.byref(fc), _ptr(accum),
+                                            _ptr(out), _ptr(depth_out), s), "insta_frame_buffer")
 
 
 def default_train_config(**kw) -> InstaTrainConfig:

@@ -235,6 +235,8 @@ insta_status insta_create(const insta_config* cfg, insta_ctx** out) {
     if (cfg->precision == INSTA_PRECISION_FP16 || cfg->precision == INSTA_PRECISION_BF16)
         alloc(&ctx->d_table_h, ctx->layout.n_grid_params * 2);
     alloc(&ctx->d_scratch, ctx->scratch_bytes);
+    alloc((void**)&ctx->d_misses, sizeof(uint32_t));
+    if (ok && hipMemset(ctx->d_misses, 0, sizeof(uint32_t)) != hipSuccess) ok = false;
     if (cfg->n_extra_dims) {
         alloc((void**)&ctx->d_frame_extra, (size_t)cfg->max_frames * cfg->n_extra_dims * sizeof(float));
         alloc((void**)&ctx->d_extra_bias, S * 64 * sizeof(float));
@@ -266,7 +268,7 @@ void insta_destroy(insta_ctx* ctx) {
     dev_free(ctx->d_cen); dev_free(ctx->d_grids); dev_free(ctx->d_grids_lin); dev_free(ctx->d_built); dev_free(ctx->d_params);
     dev_free(ctx->d_build_lists); dev_free(ctx->d_build_q);
     dev_free(ctx->d_scratch); dev_free(ctx->d_wfrag); dev_free(ctx->d_table_h);
-    dev_free(ctx->d_frame_extra); dev_free(ctx->d_extra_bias);
+    dev_free(ctx->d_frame_extra); dev_free(ctx->d_extra_bias); dev_free(ctx->d_misses);
     delete ctx;
 }
 
@@ -592,6 +594,111 @@ insta_status insta_render(insta_ctx* ctx, const insta_camera* cam, uint32_t widt
                          (hipStream_t)stream);
 }
 
+namespace {
+bool frame_config_ok(const insta_frame_config* fc) {
+    return fc && fc->tonemap_curve <= INSTA_TONEMAP_REINHARD && std::isfinite(fc->exposure) &&
+           std::isfinite(fc->background[0]) && std::isfinite(fc->background[1]) &&
+           std::isfinite(fc->background[2]) && std::isfinite(fc->background[3]);
+}
+}  // namespace
+
+insta_status insta_network_forward(insta_ctx* ctx, const float* coords_dev, uint32_t n, uint32_t stride_floats,
+                                   float* out_dev, void* stream) {
+    INSTA_CHECK_CTX(ctx);
+    if (!coords_dev || !out_dev || n == 0 || stride_floats != 11 + ctx->cfg.n_extra_dims) {
+        set_error("network_forward: null pointer, empty batch or stride other than 11 + n_extra_dims");
+        return INSTA_ERR_INVALID;
+    }
+    if (!ctx->params_set) {
+        set_error("network_forward: parameters not set");
+        return INSTA_ERR_STATE;
+    }
+    if (!insta::train_supported(ctx->cfg)) {
+        set_error("network_forward: network shape not supported by the training kernels");
+        return INSTA_ERR_UNSUPPORTED;
+    }
+    return insta::network_forward(ctx, coords_dev, n, stride_floats, out_dev, (hipStream_t)stream);
+}
+
+insta_status insta_network_backward(insta_ctx* ctx, const float* coords_dev, uint32_t n, uint32_t stride_floats,
+                                    const float* dL_doutput_dev, float* grad_params_dev, int accumulate,
+                                    float* dL_dinput_dev, void* stream) {
+    INSTA_CHECK_CTX(ctx);
+    if (!coords_dev || !dL_doutput_dev || !grad_params_dev || stride_floats != 11 + ctx->cfg.n_extra_dims) {
+        set_error("network_backward: null pointer or stride other than 11 + n_extra_dims");
+        return INSTA_ERR_INVALID;
+    }
+    if (n == 0 || insta::network_forward_batch(ctx) != n) {
+        set_error("network_backward: no stored forward of this batch size (insta_network_forward first)");
+        return INSTA_ERR_STATE;
+    }
+    return insta::network_backward(ctx, n, stride_floats, dL_doutput_dev, grad_params_dev, accumulate != 0,
+                                   dL_dinput_dev, (hipStream_t)stream);
+}
+
+insta_status insta_network_density(insta_ctx* ctx, const float* positions_dev, uint32_t n, float* out_dev,
+                                   void* stream) {
+    INSTA_CHECK_CTX(ctx);
+    if (!positions_dev || !out_dev) {
+        set_error("network_density: null pointer");
+        return INSTA_ERR_INVALID;
+    }
+    if (!ctx->params_set) {
+        set_error("network_density: parameters not set");
+        return INSTA_ERR_STATE;
+    }
+    if (!insta::train_supported(ctx->cfg)) {
+        set_error("network_density: network shape not supported");
+        return INSTA_ERR_UNSUPPORTED;
+    }
+    return insta::launch_density_outputs(ctx, positions_dev, n, out_dev, (hipStream_t)stream);
+}
+
+insta_status insta_frame_buffer(const float* rgba_dev, const float* depth_dev, uint32_t n, uint32_t spp,
+                                const insta_frame_config* fc, float* accum_dev, float* out_dev,
+                                float* depth_out_dev, void* stream) {
+    if (!rgba_dev || !accum_dev || !out_dev || !frame_config_ok(fc)) {
+        set_error("frame_buffer: null buffer or bad frame configuration");
+        return INSTA_ERR_INVALID;
+    }
+    return insta::launch_frame_buffer(reinterpret_cast<const float4*>(rgba_dev), depth_dev, n, spp, *fc,
+                                      reinterpret_cast<float4*>(accum_dev), reinterpret_cast<float4*>(out_dev),
+                                      depth_out_dev, (hipStream_t)stream);
+}
+
+insta_status insta_render_frame(insta_ctx* ctx, const insta_camera* cam, uint32_t width, uint32_t height,
+                                int32_t frame, int32_t occupancy, uint32_t spp, const insta_frame_config* fc,
+                                float* accum_dev, float* out_dev, float* depth_out_dev, void* stream) {
+    INSTA_CHECK_CTX(ctx);
+    if (!accum_dev || !out_dev || !frame_config_ok(fc)) {
+        set_error("render_frame: null buffer or bad frame configuration");
+        return INSTA_ERR_INVALID;
+    }
+    // the composited image goes through out_dev (shaded in place), the
+    // composited depth through depth_out_dev
+    insta_status st = insta_render(ctx, cam, width, height, frame, occupancy, spp, out_dev, depth_out_dev, nullptr,
+                                   stream);
+    if (st != INSTA_OK) return st;
+    return insta::launch_frame_buffer(reinterpret_cast<const float4*>(out_dev), depth_out_dev, width * height, spp,
+                                      *fc, reinterpret_cast<float4*>(accum_dev), reinterpret_cast<float4*>(out_dev),
+                                      depth_out_dev, (hipStream_t)stream);
+}
+
+insta_status insta_get_deform_misses(insta_ctx* ctx, uint64_t* misses, int reset, void* stream) {
+    INSTA_CHECK_CTX(ctx);
+    if (!misses) {
+        set_error("get_deform_misses: null pointer");
+        return INSTA_ERR_INVALID;
+    }
+    const hipStream_t s = (hipStream_t)stream;
+    uint32_t h = 0;
+    INSTA_HIP(hipMemcpyAsync(&h, ctx->d_misses, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    if (reset) INSTA_HIP(hipMemsetAsync(ctx->d_misses, 0, sizeof(uint32_t), s));
+    INSTA_HIP(hipStreamSynchronize(s));
+    *misses = h;
+    return INSTA_OK;
+}
+
 insta_status insta_get_render_stats(insta_ctx* ctx, insta_render_stats* out) {
     INSTA_CHECK_CTX(ctx);
     if (!out) {
@@ -635,6 +742,16 @@ insta_status insta_get_profile(insta_ctx* ctx, insta_profile* out) {
 }
 
 }  // extern "C"
+
+void insta_default_frame_config(insta_frame_config* fc) {
+    if (!fc) return;
+    *fc = insta_frame_config{};
+    fc->exposure = 0.0f;
+    for (int c = 0; c < 4; ++c) fc->background[c] = 1.0f;
+    fc->tonemap_curve = INSTA_TONEMAP_IDENTITY;
+    fc->output_srgb = 1;
+    fc->linear_colors = 0;
+}
 
 void insta_default_train_config(insta_train_config* tc) {
     if (!tc) return;

@@ -102,6 +102,10 @@ struct insta_ctx {
     uint64_t prof_pending_items[INSTA_KERNEL_CLASSES] = {};
     insta_profile prof{};
 
+    // samples the AoS deformation entry points left undeformed because the
+    // nearest-triangle query found nothing (src/core.cu:482-485)
+    uint32_t* d_misses = nullptr;
+
     // misc scratch for AoS entry points
     void* d_scratch = nullptr;
     size_t scratch_bytes = 0;
@@ -170,15 +174,16 @@ insta_status launch_closest(insta_ctx* ctx, uint32_t slot, const float* pts, uin
 insta_status launch_deform_aos(insta_ctx* ctx, float* coords, uint32_t n, uint32_t stride, int32_t frame,
                                hipStream_t stream);
 insta_status launch_deform_soa(insta_ctx* ctx, RenderWS& ws, int32_t frame, const uint32_t* n_dev, uint32_t n_max,
-                               hipStream_t stream);
+                               uint32_t* miss_counter, hipStream_t stream);
 // positions [n][3] in place, each through a random frame of 0..n_frames-2
 insta_status launch_deform_positions(insta_ctx* ctx, float* pos, uint32_t n, uint32_t n_frames, const Pcg32& rng,
                                      hipStream_t stream);
 // samples s_pos (xyz, ray id bits) with ray directions r_d; frame per ray
 // from ray_frame when non-null
+// miss_counter (nullable) counts in-box samples whose query found no triangle
 insta_status launch_deform_samples(insta_ctx* ctx, int32_t frame, const int32_t* ray_frame, const uint32_t* n_dev,
                                    uint32_t n_max, const float4* s_pos, const float4* r_d, int32_t* s_hit,
-                                   float4* s_cpos, float4* s_cdir, hipStream_t stream);
+                                   float4* s_cpos, float4* s_cdir, uint32_t* miss_counter, hipStream_t stream);
 insta_status launch_occupancy(insta_ctx* ctx, uint32_t slot, float threshold, hipStream_t stream);
 insta_status launch_grid_linear(insta_ctx* ctx, uint32_t slot, hipStream_t stream);
 insta_status launch_set_built(insta_ctx* ctx, uint32_t first, uint32_t n, hipStream_t stream);
@@ -198,6 +203,10 @@ insta_status finish_render(insta_ctx* ctx, RenderWS& ws, const insta_camera* cam
                            int32_t frame, int32_t occupancy, uint32_t spp, uint32_t rounds, float* rgba, float* depth,
                            uint32_t* steps, hipStream_t stream);
 void free_workspace(insta_ctx* ctx);
+// render buffer stages (shade, accumulate, tonemap) on a composited image
+insta_status launch_frame_buffer(const float4* rgba, const float* depth, uint32_t n, uint32_t spp,
+                                 const insta_frame_config& fc, float4* accum, float4* out, float* depth_out,
+                                 hipStream_t stream);
 insta_status set_profiling(insta_ctx* ctx, int level);
 // training (train.hip)
 bool train_supported(const insta_config& cfg);
@@ -235,6 +244,13 @@ insta_status density_export(insta_ctx* ctx, DensitySnapshot& out, hipStream_t st
 // grid, then its mean and bitfield into occupancy slot `slot` (0 canonical, f + 1 frame f)
 insta_status density_import(insta_ctx* ctx, const DensitySnapshot& in, uint32_t slot, hipStream_t stream);
 insta_status get_gradients(insta_ctx* ctx, float* out, hipStream_t stream);
+// network plugin (train.hip, density.hip)
+insta_status network_forward(insta_ctx* ctx, const float* coords, uint32_t n, uint32_t stride, float* out,
+                             hipStream_t stream);
+uint32_t network_forward_batch(const insta_ctx* ctx);  // samples of the stored forward (0: none)
+insta_status network_backward(insta_ctx* ctx, uint32_t n, uint32_t stride, const float* dL_dout, float* grad_out,
+                              bool accumulate, float* dL_dinput, hipStream_t stream);
+insta_status launch_density_outputs(insta_ctx* ctx, const float* pos, uint32_t n, float* out, hipStream_t stream);
 insta_status reset_optimizer(insta_ctx* ctx, hipStream_t stream);
 insta_status launch_encode(insta_ctx* ctx, const float* pos, uint32_t n, float* out, hipStream_t stream);
 insta_status launch_encode_backward(insta_ctx* ctx, const float* pos, uint32_t n, const float* dL, float* grad,

@@ -170,12 +170,17 @@ INSTA_DEV void warp_from_hit(const MeshSet& ms, uint32_t frame, int32_t idx, V3 
     out_d = v3((dc.x + 1.0f) * 0.5f, (dc.y + 1.0f) * 0.5f, (dc.z + 1.0f) * 0.5f);
 }
 
-// Body of `warp` for one sample; false leaves the sample untouched.
+// Body of `warp` for one sample; false leaves the sample untouched.  A
+// sample in the unit box whose query finds no triangle is counted (the
+// reference prints it and returns, src/core.cu:482-485).
 INSTA_DEV bool warp_one(const MeshSet& ms, uint32_t frame, V3 p, V3 dir_w, V3& out_p, V3& out_d, uint32_t& mask,
-                        uint32_t* stack, uint32_t stride) {
+                        uint32_t* stack, uint32_t stride, uint32_t* misses) {
     if (!in_unit_box(p)) return false;
     const Hit h = bvh_closest(ms.view(frame + 1), p, stack, stride);
-    if (h.idx < 0) return false;
+    if (h.idx < 0) {
+        atomicAdd(misses, 1u);
+        return false;
+    }
     warp_from_hit(ms, frame, h.idx, p, dir_w, out_p, out_d, mask);
     return true;
 }
@@ -183,7 +188,7 @@ INSTA_DEV bool warp_one(const MeshSet& ms, uint32_t frame, V3 p, V3 dir_w, V3& o
 // NerfCoordinate samples in place (stride floats per sample)
 __global__ __launch_bounds__(kDeformThreads) void deform_aos_kernel(MeshSet ms, float* __restrict__ coords, uint32_t n,
                                                                     uint32_t stride, int32_t frame_arg,
-                                                                    uint32_t n_extra) {
+                                                                    uint32_t n_extra, uint32_t* __restrict__ misses) {
     extern __shared__ uint32_t stack[];  // [stack_depth(k)][kDeformThreads]
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -195,7 +200,7 @@ __global__ __launch_bounds__(kDeformThreads) void deform_aos_kernel(MeshSet ms, 
     V3 op, od;
     uint32_t mask = 0;
     if (!warp_one(ms, frame, v3(c[0], c[1], c[2]), v3(c[8], c[9], c[10]), op, od, mask, stack + threadIdx.x,
-                  kDeformThreads))
+                  kDeformThreads, misses))
         return;
     c[0] = op.x; c[1] = op.y; c[2] = op.z;
     c[8] = od.x; c[9] = od.y; c[10] = od.z;
@@ -235,20 +240,28 @@ __global__ __launch_bounds__(kDeformThreads) void deform_hits_kernel(MeshSet ms,
                                                                      const uint32_t* __restrict__ n_dev,
                                                                      const float4* __restrict__ s_pos,
                                                                      int32_t* __restrict__ s_hit,
-                                                                     unsigned long long* __restrict__ counts) {
+                                                                     unsigned long long* __restrict__ counts,
+                                                                     uint32_t* __restrict__ misses) {
     extern __shared__ uint32_t stack[];  // [stack_depth(k)][kDeformThreads]
     const uint32_t n = *n_dev;
     TraversalCount tc;
-    uint32_t queries = 0;
+    uint32_t queries = 0, missed = 0;
     for (uint32_t i = blockIdx.x * kDeformThreads + threadIdx.x; i < n; i += gridDim.x * kDeformThreads) {
         const float4 sp = s_pos[i];
         const V3 p = v3(sp.x, sp.y, sp.z);
         const int32_t f = sample_frame(frame, ray_frame, sp);
         const bool q = f >= 0 && in_unit_box(p);
         if constexpr (kCount) queries += q ? 1u : 0u;
-        s_hit[i] = q ? bvh_closest_impl<kCount>(ms.view((uint32_t)f + 1), p, stack + threadIdx.x, kDeformThreads,
-                                                 3.40282347e38f, &tc).idx
-                     : -1;
+        const int32_t h = q ? bvh_closest_impl<kCount>(ms.view((uint32_t)f + 1), p, stack + threadIdx.x,
+                                                        kDeformThreads, 3.40282347e38f, &tc).idx
+                            : -1;
+        missed += (q && h < 0) ? 1u : 0u;
+        s_hit[i] = h;
+    }
+    if (misses) {  // one atomic per wave, only when a lane missed
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) missed += __shfl_xor(missed, o);
+        if ((threadIdx.x & 63u) == 0 && missed) atomicAdd(misses, missed);
     }
     if constexpr (kCount) {
         unsigned long long v[4] = {queries, tc.inner, tc.leaves, tc.triangles};
@@ -293,7 +306,8 @@ __global__ __launch_bounds__(kDeformThreads) void deform_apply_kernel(MeshSet ms
 // frame = floor(u * (n_frames - 1)), so the last frame is never chosen (the
 // reference's own off-by-one, kept).  No unit-box test, as in the reference.
 __global__ __launch_bounds__(kDeformThreads) void deform_positions_kernel(MeshSet ms, float* __restrict__ pos,
-                                                                         uint32_t n, uint32_t n_meshes, Pcg32 rng) {
+                                                                         uint32_t n, uint32_t n_meshes, Pcg32 rng,
+                                                                         uint32_t* __restrict__ misses) {
     extern __shared__ uint32_t stack[];  // [stack_depth(k)][kDeformThreads]
     const uint32_t i = blockIdx.x * kDeformThreads + threadIdx.x;
     if (i >= n) return;
@@ -301,7 +315,10 @@ __global__ __launch_bounds__(kDeformThreads) void deform_positions_kernel(MeshSe
     const uint32_t frame = (uint32_t)(rng.next_float() * (float)n_meshes);
     const V3 p = v3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
     const int32_t h = bvh_closest(ms.view(frame + 1), p, stack + threadIdx.x, kDeformThreads).idx;
-    if (h < 0) return;
+    if (h < 0) {  // src/core.cu:554-557
+        atomicAdd(misses, 1u);
+        return;
+    }
     float W[12], C[12];
     uint32_t mask;
     frames_for_hit(ms, frame, h, p, W, C, mask);
@@ -323,24 +340,25 @@ insta_status launch_deform_aos(insta_ctx* ctx, float* coords, uint32_t n, uint32
     if (n == 0) return INSTA_OK;
     const size_t shmem = (size_t)stack_depth(ctx->bvh_k) * kDeformThreads * sizeof(uint32_t);
     hipLaunchKernelGGL(deform_aos_kernel, dim3((n + kDeformThreads - 1) / kDeformThreads), dim3(kDeformThreads), shmem,
-                       stream, mesh_set(ctx, ctx->d_built), coords, n, stride, frame, ctx->cfg.n_extra_dims);
+                       stream, mesh_set(ctx, ctx->d_built), coords, n, stride, frame, ctx->cfg.n_extra_dims,
+                       ctx->d_misses);
     INSTA_HIP(hipGetLastError());
     return INSTA_OK;
 }
 
 insta_status launch_deform_samples(insta_ctx* ctx, int32_t frame, const int32_t* ray_frame, const uint32_t* n_dev,
                                    uint32_t n_max, const float4* s_pos, const float4* r_d, int32_t* s_hit,
-                                   float4* s_cpos, float4* s_cdir, hipStream_t stream) {
+                                   float4* s_cpos, float4* s_cdir, uint32_t* miss_counter, hipStream_t stream) {
     if (n_max == 0) return INSTA_OK;
     const MeshSet ms = mesh_set(ctx, ctx->d_built);
     if (frame >= 0 || ray_frame) {
         const size_t shmem = (size_t)stack_depth(ctx->bvh_k) * kDeformThreads * sizeof(uint32_t);
         if (ctx->d_trav_counts)
             hipLaunchKernelGGL(deform_hits_kernel<true>, dim3(blocks_for(n_max)), dim3(kDeformThreads), shmem, stream,
-                               ms, frame, ray_frame, n_dev, s_pos, s_hit, ctx->d_trav_counts);
+                               ms, frame, ray_frame, n_dev, s_pos, s_hit, ctx->d_trav_counts, miss_counter);
         else
             hipLaunchKernelGGL(deform_hits_kernel<false>, dim3(blocks_for(n_max)), dim3(kDeformThreads), shmem,
-                               stream, ms, frame, ray_frame, n_dev, s_pos, s_hit, nullptr);
+                               stream, ms, frame, ray_frame, n_dev, s_pos, s_hit, nullptr, miss_counter);
         INSTA_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(deform_apply_kernel, dim3(blocks_for(n_max)), dim3(kDeformThreads), 0, stream, ms, frame,
@@ -354,15 +372,15 @@ insta_status launch_deform_positions(insta_ctx* ctx, float* pos, uint32_t n, uin
     if (n == 0) return INSTA_OK;
     const size_t shmem = (size_t)stack_depth(ctx->bvh_k) * kDeformThreads * sizeof(uint32_t);
     hipLaunchKernelGGL(deform_positions_kernel, dim3((n + kDeformThreads - 1) / kDeformThreads), dim3(kDeformThreads),
-                       shmem, stream, mesh_set(ctx, ctx->d_built), pos, n, n_frames - 1, rng);
+                       shmem, stream, mesh_set(ctx, ctx->d_built), pos, n, n_frames - 1, rng, ctx->d_misses);
     INSTA_HIP(hipGetLastError());
     return INSTA_OK;
 }
 
 insta_status launch_deform_soa(insta_ctx* ctx, RenderWS& ws, int32_t frame, const uint32_t* n_dev, uint32_t n_max,
-                               hipStream_t stream) {
+                               uint32_t* miss_counter, hipStream_t stream) {
     return launch_deform_samples(ctx, frame, nullptr, n_dev, n_max, ws.s_pos, ws.r_d, ws.s_hit, ws.s_cpos, ws.s_cdir,
-                                 stream);
+                                 miss_counter, stream);
 }
 
 }  // namespace insta

@@ -96,6 +96,32 @@ __global__ __launch_bounds__(128) void dg_splat_kernel(const float* __restrict__
     atomicMax(reinterpret_cast<uint32_t*>(tmp) + idx[i], __float_as_uint(thickness));
 }
 
+// NerfNetwork::density (nerf_network.h:385-414): the density MLP's 16
+// outputs at canonical positions, extra dims 1.0 (extract_expressions_random
+// writes 1.0, nerf_network.h:107-127); out [n][16], output 0 the density logit
+template <int L, int X>
+__global__ __launch_bounds__(128) void density_outputs_kernel(const float* __restrict__ params,
+                                                              const float* __restrict__ table, GridParams gp,
+                                                              const float* __restrict__ pos, uint32_t n,
+                                                              float* __restrict__ out) {
+    constexpr int E = 2 * L + X;
+    constexpr int W = 64;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const V3 x = v3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+    float enc[E];
+#pragma unroll
+    for (int l = 0; l < L; ++l) encode_level<2>(table, gp, (uint32_t)l, x, enc + 2 * l);
+#pragma unroll
+    for (int e = 2 * L; e < E; ++e) enc[e] = 1.0f;
+    float h[W];
+    dense<E, W, true>(params, enc, h);
+    float o[16];
+    dense<W, 16, false>(params + W * E, h, o);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) out[(size_t)i * 16 + k] = o[k];
+}
+
 __global__ void dg_ema_kernel(float* __restrict__ grid, const float* __restrict__ tmp, float decay, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -246,6 +272,22 @@ insta_status update_density_grid(insta_ctx* ctx, uint32_t slot, uint32_t n_frame
     }
     ctx->grid_built[slot] = 1;
     ++d->step;
+    return INSTA_OK;
+}
+
+insta_status launch_density_outputs(insta_ctx* ctx, const float* pos, uint32_t n, float* out, hipStream_t stream) {
+    if (n == 0) return INSTA_OK;
+    const GridParams gp = grid_params(ctx);
+    const float* table = ctx->d_params + ctx->layout.n_mlp_params;
+    const dim3 grid((n + 127) / 128), block(128);
+    const bool x16 = ctx->cfg.n_extra_dims == 16;
+    if (ctx->cfg.n_levels == 8)
+        hipLaunchKernelGGL((x16 ? density_outputs_kernel<8, 16> : density_outputs_kernel<8, 0>), grid, block, 0, stream,
+                           ctx->d_params, table, gp, pos, n, out);
+    else
+        hipLaunchKernelGGL((x16 ? density_outputs_kernel<16, 16> : density_outputs_kernel<16, 0>), grid, block, 0,
+                           stream, ctx->d_params, table, gp, pos, n, out);
+    INSTA_HIP(hipGetLastError());
     return INSTA_OK;
 }
 

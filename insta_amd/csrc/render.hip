@@ -23,6 +23,7 @@
 // not depend on K or on the order rays are processed in.  Live counts stay on
 // the device: the first round is queued with init and finalize, and the host
 // reads the counters once, adding rounds only for rays still live.
+#include <cmath>
 #include <type_traits>
 
 #include "context.hpp"
@@ -41,6 +42,7 @@ enum : uint32_t {
     kCntComposited = 4,
     kCntEmitted = 5,
     kCntRounds = 6,
+    kCntMisses = 7,   // in-box samples whose nearest-triangle query found nothing
     kNumCounters = 16
 };
 
@@ -256,6 +258,68 @@ __global__ void finalize_kernel(uint32_t n, const float4* __restrict__ r_rgba, c
     rgba[i] = acc.w > 0.001f ? acc : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (depth) depth[i] = acc.w > 0.2f ? r_depth[i] : 1e10f;
     if (steps) steps[i] = r_steps[i];
+}
+
+// ---- render buffer: shade, accumulate, tonemap ----------------------------
+// (testbed_nerf.cu:1794-1827; render_buffer.cu:236-266, 264-350, 493-522),
+// one thread per pixel, fused: the frame buffer of one sample index exists
+// only in registers.
+struct FrameArgs {
+    float sample_count;     // spp: the accumulation's running-mean weight
+    float exposure_scale;   // 2^exposure
+    float bg[4];            // background, rgb already linear, weight
+    float k[6];             // rational tonemap coefficients (ACES, Hable)
+    uint32_t curve, output_srgb, linear_colors;
+};
+
+INSTA_DEV float srgb_to_linear(float c) { return c <= 0.04045f ? c / 12.92f : powf((c + 0.055f) / 1.055f, 2.4f); }
+INSTA_DEV float linear_to_srgb(float c) { return c < 0.0031308f ? 12.92f * c : 1.055f * powf(c, 0.41666f) - 0.055f; }
+
+INSTA_DEV V3 tonemap_curve(V3 x, const FrameArgs& a) {
+    if (a.curve == INSTA_TONEMAP_IDENTITY) return x;
+    x = v3(fmaxf(x.x, 0.0f), fmaxf(x.y, 0.0f), fmaxf(x.z, 0.0f));
+    if (a.curve == INSTA_TONEMAP_REINHARD) {
+        const float Y = 0.2126f * x.x + 0.7152f * x.y + 0.0722f * x.z;
+        const float s = 1.0f / (Y + 1.0f);
+        return v3(x.x * s, x.y * s, x.z * s);
+    }
+    float c[3] = {x.x, x.y, x.z};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float sq = c[i] * c[i];
+        const float nom = sq * a.k[0] + a.k[1] * c[i] + a.k[2];
+        const float den = a.k[3] * sq + a.k[4] * c[i] + a.k[5];
+        c[i] = nom / den;
+    }
+    return v3(c[0], c[1], c[2]);
+}
+
+// rgba/out and depth/depth_out may be the same buffers (shaded in place)
+__global__ void frame_buffer_kernel(uint32_t n, FrameArgs a, const float4* rgba, const float* depth,
+                                    float4* __restrict__ accum, float4* out, float* depth_out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    // shade_kernel_nerf into a cleared frame buffer: tmp + 0 * (1 - tmp.w)
+    float4 f = rgba[i];
+    if (!a.linear_colors) f = make_float4(srgb_to_linear(f.x), srgb_to_linear(f.y), srgb_to_linear(f.z), f.w);
+    if (depth_out) depth_out[i] = f.w > 0.2f && depth ? depth[i] : 0.0f;
+    // accumulate_kernel, linear colour space (spp 0 starts from a cleared buffer)
+    const float s = a.sample_count;
+    float4 t = s == 0.0f ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : accum[i];
+    t.x = (t.x * s + f.x) / (s + 1.0f);
+    t.y = (t.y * s + f.y) / (s + 1.0f);
+    t.z = (t.z * s + f.z) / (s + 1.0f);
+    t.w = (t.w * s + f.w) / (s + 1.0f);
+    // tonemap_kernel: background behind, exposure, curve, output colour space
+    const float w = (1.0f - t.w) * a.bg[3];
+    V3 c = v3(t.x + a.bg[0] * w, t.y + a.bg[1] * w, t.z + a.bg[2] * w);
+    t.w = t.w + w;
+    c = v3(c.x * a.exposure_scale, c.y * a.exposure_scale, c.z * a.exposure_scale);
+    c = tonemap_curve(c, a);
+    if (a.output_srgb) c = v3(linear_to_srgb(c.x), linear_to_srgb(c.y), linear_to_srgb(c.z));
+    const float4 o = make_float4(c.x, c.y, c.z, t.w);
+    accum[i] = o;  // the reference's tonemap_kernel writes its result back
+    out[i] = o;
 }
 
 __global__ void clear_counters_kernel(uint32_t* counters, uint32_t first, uint32_t n) {
@@ -502,7 +566,7 @@ insta_status enqueue_round(insta_ctx* ctx, RenderWS& ws, const RenderArgs& a, in
     // at most max_alive * K <= sample capacity samples this round
     const uint32_t n_max = (uint32_t)ws.samples;
     prof_begin(ctx, prof, INSTA_KERNEL_DEFORM, stream);
-    insta_status st = launch_deform_soa(ctx, ws, frame, cnt + kCntSamples, n_max, stream);
+    insta_status st = launch_deform_soa(ctx, ws, frame, cnt + kCntSamples, n_max, cnt + kCntMisses, stream);
     if (st != INSTA_OK) return st;
     prof_end(ctx, prof, INSTA_KERNEL_DEFORM, 0, stream);
     prof_begin(ctx, prof, INSTA_KERNEL_NETWORK, stream);
@@ -598,6 +662,7 @@ insta_status finish_render(insta_ctx* ctx, RenderWS& ws, const insta_camera* cam
     ws.stats.n_samples = ws.h_counters[kCntComposited];
     ws.stats.n_samples_emitted = ws.h_counters[kCntEmitted];
     ws.stats.n_iterations = ws.h_counters[kCntRounds];
+    ws.stats.n_deform_misses = ws.h_counters[kCntMisses];
     return INSTA_OK;
 }
 
@@ -626,6 +691,45 @@ insta_status set_profiling(insta_ctx* ctx, int level) {
         INSTA_HIP(hipDeviceSynchronize());  // queued counting kernels may still write
         dev_free(ctx->d_trav_counts);
     }
+    return INSTA_OK;
+}
+
+// host half of the render buffer: the per-call constants of
+// frame_buffer_kernel (background to linear, exposure scale and the
+// tonemap curve's rational coefficients, render_buffer.cu:264-325, 493-506)
+namespace {
+float h_srgb_to_linear(float c) { return c <= 0.04045f ? c / 12.92f : std::pow((c + 0.055f) / 1.055f, 2.4f); }
+}
+
+insta_status launch_frame_buffer(const float4* rgba, const float* depth, uint32_t n, uint32_t spp,
+                                 const insta_frame_config& fc, float4* accum, float4* out, float* depth_out,
+                                 hipStream_t stream) {
+    if (n == 0) return INSTA_OK;
+    FrameArgs a{};
+    a.sample_count = (float)spp;
+    a.exposure_scale = std::pow(2.0f, fc.exposure);
+    for (int c = 0; c < 3; ++c) a.bg[c] = h_srgb_to_linear(fc.background[c]);
+    a.bg[3] = fc.background[3];
+    a.curve = fc.tonemap_curve;
+    a.output_srgb = fc.output_srgb;
+    a.linear_colors = fc.linear_colors;
+    if (fc.tonemap_curve == INSTA_TONEMAP_ACES) {
+        // Narkowicz's ACES fit with the 0.6 pre-exposure folded in
+        a.k[0] = 0.6f * 0.6f * 2.51f; a.k[1] = 0.6f * 0.03f; a.k[2] = 0.0f;
+        a.k[3] = 0.6f * 0.6f * 2.43f; a.k[4] = 0.6f * 0.59f; a.k[5] = 0.14f;
+    } else if (fc.tonemap_curve == INSTA_TONEMAP_HABLE) {
+        // Hable's filmic curve, white point 11.2, exposure bias 2 folded in
+        const float A = 0.15f, B = 0.50f, C = 0.10f, D = 0.20f, E = 0.02f, F = 0.30f, W = 11.2f;
+        float k0 = A * F - A * E, k1 = C * B * F - B * E, k2 = 0.0f, k3 = A * F, k4 = B * F, k5 = D * F * F;
+        const float nom = k0 * (W * W) + k1 * W + k2;
+        const float den = k3 * (W * W) + k4 * W + k5;
+        const float white = den / nom;
+        a.k[0] = 4.0f * k0 * white; a.k[1] = 2.0f * k1 * white; a.k[2] = k2 * white;
+        a.k[3] = 4.0f * k3; a.k[4] = 2.0f * k4; a.k[5] = k5;
+    }
+    hipLaunchKernelGGL(frame_buffer_kernel, dim3((n + kRayThreads - 1) / kRayThreads), dim3(kRayThreads), 0, stream,
+                       n, a, rgba, depth, accum, out, depth_out);
+    INSTA_HIP(hipGetLastError());
     return INSTA_OK;
 }
 

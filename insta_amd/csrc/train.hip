@@ -128,6 +128,11 @@ struct TrainState {
     uint32_t training_step = 0;
     uint32_t adam_step = 0;
     bool have_grad = false;
+    // network plugin: per-sample extra dims of the last forward ([cap][X]) and
+    // its batch size (0: no stored forward; a training step overwrites it)
+    float* net_extra = nullptr;
+    uint32_t net_extra_cap = 0;
+    uint32_t net_n = 0;
 };
 
 namespace {
@@ -250,8 +255,11 @@ INSTA_DEV void store_rows(float* __restrict__ act, uint32_t ld, int row0, uint32
 // wave's 64 samples, 16 columns at a time, through the density MLP and the
 // colour MLP's hidden layers with the weights as A operands, storing every
 // layer's rows from the accumulators (lane l holds rows 4(l/16)..+3 of
-// column l%16, contiguous along the batch).  The output layer is not needed:
-// dL/doutput comes from the loss kernel.
+// column l%16, contiguous along the batch).  The training step needs no
+// output layer (dL/doutput comes from the loss kernel); the network plugin's
+// forward asks for it through `out` (rgb logits and the density logit per
+// batch entry).  Extra dims come from the frame table by the selector, or
+// per sample from x_extra [n][X] when given.
 typedef float tf_f4 __attribute__((ext_vector_type(4)));
 constexpr uint32_t kFwdTile = 256;
 template <int L, int X>
@@ -262,7 +270,9 @@ __global__ __launch_bounds__(256) void train_forward_kernel(const float* __restr
                                                           const uint32_t* __restrict__ c_idx,
                                                           const float4* __restrict__ cpos,
                                                           const float4* __restrict__ cdir,
-                                                          float* __restrict__ act, uint32_t ld) {
+                                                          const float* __restrict__ x_extra,
+                                                          float* __restrict__ act, uint32_t ld,
+                                                          float4* __restrict__ out) {
     constexpr int E = 2 * L, DIN = E + X, KB0 = DIN / 4, S = DIN + 1;
     static_assert(DIN % 4 == 0, "K blocks of 4");
     using A = Act<DIN>;
@@ -285,7 +295,9 @@ __global__ __launch_bounds__(256) void train_forward_kernel(const float* __restr
             for (int l = 0; l < L; ++l) encode_level<2>(table, gp, (uint32_t)l, x, enc + 2 * l);
             if constexpr (X > 0) {
                 const uint32_t sel = __float_as_uint(d4.w);
-                const float* fx = sel != kExtraOnes && sel <= n_frames ? frame_extra + (size_t)(sel - 1u) * X : nullptr;
+                const float* fx = x_extra ? x_extra + (size_t)smp * X
+                                          : (sel != kExtraOnes && sel <= n_frames ? frame_extra + (size_t)(sel - 1u) * X
+                                                                                  : nullptr);
 #pragma unroll
                 for (int e = 0; e < X; ++e) enc[E + e] = fx ? fx[e] : 1.0f;
             }
@@ -362,6 +374,7 @@ __global__ __launch_bounds__(256) void train_forward_kernel(const float* __restr
             }
         }
         // colour layer 1
+        tf_f4 h3[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             tf_f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -371,9 +384,24 @@ __global__ __launch_bounds__(256) void train_forward_kernel(const float* __restr
                 for (int b = 0; b < 4; ++b)
                     acc = __builtin_amdgcn_mfma_f32_16x16x4f32(W3[(16 * t + m) * kW + 16 * s2 + 4 * g + b], r[s2][b],
                                                                acc, 0, 0, 0);
-            if (live)
 #pragma unroll
-                for (int q = 0; q < 4; ++q) act[(size_t)(A::H3 + 16 * t + 4 * g + q) * ld + i] = fmaxf(acc[q], 0.0f);
+            for (int q = 0; q < 4; ++q) {
+                h3[t][q] = fmaxf(acc[q], 0.0f);
+                if (live) act[(size_t)(A::H3 + 16 * t + 4 * g + q) * ld + i] = h3[t][q];
+            }
+        }
+        if (out) {
+            // colour output (3 rows of W4, padded to 16): lane group 0 holds
+            // rows 0..3 of its column; the density logit is density output 0
+            const float* W4 = params + O::W4;
+            tf_f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int s2 = 0; s2 < 4; ++s2)
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(m < 3 ? W4[m * kW + 16 * s2 + 4 * g + b] : 0.0f,
+                                                               h3[s2][b], acc, 0, 0, 0);
+            if (live && g == 0) out[i] = make_float4(acc[0], acc[1], acc[2], dn[0]);
         }
     }
 }
@@ -864,8 +892,8 @@ insta_status launch_train_forward(insta_ctx* ctx, TrainState* ts, hipStream_t st
     const GridParams gp = grid_params(ctx);
     const dim3 sgrid((cap + kFwdTile - 1) / kFwdTile);
     hipLaunchKernelGGL((train_forward_kernel<L, X>), sgrid, dim3(kFwdTile), 0, stream, ctx->d_params, table, gp,
-                       ctx->d_frame_extra, ctx->cfg.max_frames, n_dev, ts->c_idx, ts->s_cpos, ts->s_cdir, ts->act,
-                       cap);
+                       ctx->d_frame_extra, ctx->cfg.max_frames, n_dev, ts->c_idx, ts->s_cpos, ts->s_cdir,
+                       static_cast<const float*>(nullptr), ts->act, cap, static_cast<float4*>(nullptr));
     INSTA_HIP(hipGetLastError());
     return INSTA_OK;
 }
@@ -910,6 +938,155 @@ insta_status launch_train_backward(insta_ctx* ctx, TrainState* ts, const insta_t
     return INSTA_OK;
 }
 
+// ---- NerfNetwork plugin (nerf_network.h:238-383): forward with stored
+// activations and backward on caller-given samples, through the training
+// step's kernels; the batch is the caller's samples in order (c_idx = i).
+__global__ void net_prepare_kernel(const float* __restrict__ coords, uint32_t n, uint32_t stride, uint32_t X,
+                                   float4* __restrict__ cpos, float4* __restrict__ cdir, uint32_t* __restrict__ c_idx,
+                                   float* __restrict__ x_extra, uint32_t* __restrict__ counters) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) counters[kTCompactN] = n;
+    if (i >= n) return;
+    const float* c = coords + (size_t)i * stride;
+    cpos[i] = make_float4(c[0], c[1], c[2], 0.0f);
+    cdir[i] = make_float4(c[8], c[9], c[10], __uint_as_float(kExtraOnes));
+    c_idx[i] = i;
+    // the density input's extra dims are the coordinate's last X floats
+    // (forward_impl's extract_expressions, nerf_network.h:253-262)
+    for (uint32_t e = 0; e < X; ++e) x_extra[(size_t)i * X + e] = c[stride - X + e];
+}
+
+// param_gradients_mode Overwrite / Accumulate
+__global__ void net_grad_out_kernel(const float* __restrict__ g, float* __restrict__ out, uint64_t n,
+                                    uint32_t accumulate) {
+    for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < n; p += (uint64_t)gridDim.x * blockDim.x)
+        out[p] = accumulate ? out[p] + g[p] : g[p];
+}
+
+// Input gradients of the backward (dL_dinput rows 0..2 and the direction rows
+// 8..10; the reference backpropagates through the position and direction
+// encodings only, nerf_network.h:318-380): dL/dencoding = W0^T dH1 and
+// dL/dSH = W2[:, 16:32]^T dH2 from the stored deltas; the position through
+// the derivative of each level's trilinear weights (scale_l per axis) times
+// the corner features, the direction (input in [0,1], d = 2 x - 1) through
+// the derivatives of the 16 SH polynomials.
+INSTA_DEV void sh16_grad(float x, float y, float z, const float* g, float out[3]) {
+    const float C1 = 0.48860251190291987f, C2 = 1.0925484305920792f, C3 = 0.94617469575755997f;
+    const float C4 = 0.54627421529603959f, C5 = 0.59004358992664352f, C6 = 2.8906114426405538f;
+    const float C7 = 0.45704579946446572f, C8 = 0.3731763325901154f, C9 = 1.4453057213202769f;
+    float dx = 0.0f, dy = 0.0f, dz = 0.0f;
+    dy += g[1] * -C1;
+    dz += g[2] * C1;
+    dx += g[3] * -C1;
+    dx += g[4] * (C2 * y); dy += g[4] * (C2 * x);
+    dy += g[5] * (-C2 * z); dz += g[5] * (-C2 * y);
+    dz += g[6] * (2.0f * C3 * z);
+    dx += g[7] * (-C2 * z); dz += g[7] * (-C2 * x);
+    dx += g[8] * (2.0f * C4 * x); dy += g[8] * (-2.0f * C4 * y);
+    dx += g[9] * (-6.0f * C5 * x * y); dy += g[9] * (C5 * (3.0f * y * y - 3.0f * x * x));
+    dx += g[10] * (C6 * y * z); dy += g[10] * (C6 * x * z); dz += g[10] * (C6 * x * y);
+    dy += g[11] * (C7 * (1.0f - 5.0f * z * z)); dz += g[11] * (-10.0f * C7 * y * z);
+    dz += g[12] * (C8 * (15.0f * z * z - 3.0f));
+    dx += g[13] * (C7 * (1.0f - 5.0f * z * z)); dz += g[13] * (-10.0f * C7 * x * z);
+    dx += g[14] * (2.0f * C9 * x * z); dy += g[14] * (-2.0f * C9 * y * z); dz += g[14] * (C9 * (x * x - y * y));
+    dx += g[15] * (C5 * (3.0f * y * y - 3.0f * x * x)); dy += g[15] * (6.0f * C5 * x * y);
+    out[0] = dx;
+    out[1] = dy;
+    out[2] = dz;
+}
+
+template <int L, int X>
+__global__ __launch_bounds__(128) void net_input_grad_kernel(const float* __restrict__ params,
+                                                             const float* __restrict__ table, GridParams gp,
+                                                             uint32_t n, const float4* __restrict__ cpos,
+                                                             const float4* __restrict__ cdir,
+                                                             const float* __restrict__ dact, uint32_t ld,
+                                                             float* __restrict__ dL_dinput, uint32_t stride) {
+    constexpr int E = 2 * L, DIN = E + X;
+    using O = Off<DIN>;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float denc[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) denc[e] = 0.0f;
+    float dsh[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) dsh[j] = 0.0f;
+    for (int k = 0; k < kW; ++k) {
+        const float d1 = dact[(size_t)(Dact::DH1 + k) * ld + i];
+        const float d2 = dact[(size_t)(Dact::DH2 + k) * ld + i];
+#pragma unroll
+        for (int e = 0; e < E; ++e) denc[e] += params[O::W0 + k * DIN + e] * d1;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) dsh[j] += params[O::W2 + k * kCin + 16 + j] * d2;
+    }
+    const float4 p4 = cpos[i];
+    const float xs[3] = {p4.x, p4.y, p4.z};
+    float gx[3] = {0.0f, 0.0f, 0.0f};
+    for (int l = 0; l < L; ++l) {
+        const float sc = gp.scale[l];
+        float frac[3];
+        uint32_t g0[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const float pp = xs[d] * sc + 0.5f;
+            const float fl = floorf(pp);
+            frac[d] = pp - fl;
+            g0[d] = (uint32_t)(int32_t)fl;
+        }
+        const bool dense_l = (gp.dense_mask >> l) & 1u;
+#pragma unroll
+        for (uint32_t c = 0; c < 8; ++c) {
+            uint32_t cg[3];
+            float wk[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const bool up = (c >> k) & 1u;
+                wk[k] = up ? frac[k] : 1.0f - frac[k];
+                cg[k] = g0[k] + (up ? 1u : 0u);
+            }
+            const uint32_t idx = gp.offset[l] + grid_index(cg, dense_l, gp.res[l], gp.size[l]);
+            const float s = denc[2 * l] * table[2 * (size_t)idx] + denc[2 * l + 1] * table[2 * (size_t)idx + 1];
+            gx[0] += s * (((c & 1u) ? sc : -sc) * wk[1] * wk[2]);
+            gx[1] += s * (((c & 2u) ? sc : -sc) * wk[0] * wk[2]);
+            gx[2] += s * (((c & 4u) ? sc : -sc) * wk[0] * wk[1]);
+        }
+    }
+    const float4 d4 = cdir[i];
+    float gd[3];
+    sh16_grad(d4.x * 2.0f - 1.0f, d4.y * 2.0f - 1.0f, d4.z * 2.0f - 1.0f, dsh, gd);
+    float* o = dL_dinput + (size_t)i * stride;
+    o[0] = gx[0];
+    o[1] = gx[1];
+    o[2] = gx[2];
+    o[8] = 2.0f * gd[0];
+    o[9] = 2.0f * gd[1];
+    o[10] = 2.0f * gd[2];
+}
+
+template <int L, int X>
+insta_status launch_net_forward(insta_ctx* ctx, TrainState* ts, float4* out, hipStream_t stream) {
+    const uint32_t cap = ts->compact_cap;
+    const float* table = ctx->d_params + ctx->layout.n_mlp_params;
+    hipLaunchKernelGGL((train_forward_kernel<L, X>), dim3((cap + kFwdTile - 1) / kFwdTile), dim3(kFwdTile), 0, stream,
+                       ctx->d_params, table, grid_params(ctx), ctx->d_frame_extra, ctx->cfg.max_frames,
+                       ts->counters + kTCompactN, ts->c_idx, ts->s_cpos, ts->s_cdir,
+                       X ? static_cast<const float*>(ts->net_extra) : nullptr, ts->act, cap, out);
+    INSTA_HIP(hipGetLastError());
+    return INSTA_OK;
+}
+
+template <int L, int X>
+insta_status launch_net_input_grad(insta_ctx* ctx, TrainState* ts, uint32_t n, float* dL_dinput, uint32_t stride,
+                                   hipStream_t stream) {
+    const float* table = ctx->d_params + ctx->layout.n_mlp_params;
+    hipLaunchKernelGGL((net_input_grad_kernel<L, X>), dim3((n + 127) / 128), dim3(128), 0, stream, ctx->d_params,
+                       table, grid_params(ctx), n, ts->s_cpos, ts->s_cdir, ts->dact, ts->compact_cap, dL_dinput,
+                       stride);
+    INSTA_HIP(hipGetLastError());
+    return INSTA_OK;
+}
+
 }  // namespace
 
 bool train_supported(const insta_config& c) {
@@ -923,7 +1100,7 @@ void free_train(insta_ctx* ctx) {
     if (!ts) return;
     free_workspace_train(ts);
     dev_free(ts->images); dev_free(ts->meta); dev_free(ts->grad); dev_free(ts->grad_h);
-    dev_free(ts->adam_m); dev_free(ts->adam_v); dev_free(ts->counters);
+    dev_free(ts->adam_m); dev_free(ts->adam_v); dev_free(ts->counters); dev_free(ts->net_extra);
     if (ts->h_counters) (void)hipHostFree(ts->h_counters);
     delete ts;
     ctx->train = nullptr;
@@ -992,6 +1169,7 @@ insta_status set_training_set(insta_ctx* ctx, uint32_t n_images, uint32_t width,
 
 insta_status train_step(insta_ctx* ctx, const insta_train_config& tc, insta_train_stats* stats, hipStream_t stream) {
     TrainState* ts = ctx->train;
+    ts->net_n = 0;  // the step reuses the network plugin's activation buffers
     const uint32_t n_rays = tc.n_rays ? tc.n_rays : ts->rays_per_batch;
     insta_status st = ensure_train_workspace(ctx, ts, n_rays, tc.max_samples, tc.target_samples);
     if (st != INSTA_OK) return st;
@@ -1029,7 +1207,7 @@ insta_status train_step(insta_ctx* ctx, const insta_train_config& tc, insta_trai
     INSTA_HIP(hipGetLastError());
     hipLaunchKernelGGL(train_count_kernel, dim3(1), dim3(64), 0, stream, ts->counters);
     st = launch_deform_samples(ctx, -1, ts->ray_frame, ts->counters + kTValid, cap, ts->s_pos, ts->ray_d, ts->s_hit,
-                               ts->s_cpos, ts->s_cdir, stream);
+                               ts->s_cpos, ts->s_cdir, ctx->d_misses, stream);
     if (st != INSTA_OK) return st;
     st = launch_network_samples(ctx, ts->counters + kTValid, cap, ts->s_cpos, ts->s_cdir, ts->s_out, stream);
     if (st != INSTA_OK) return st;
@@ -1176,6 +1354,78 @@ insta_status reset_optimizer(insta_ctx* ctx, hipStream_t stream) {
     hipLaunchKernelGGL(fill_zero_kernel, dim3(1024), dim3(256), 0, stream, reinterpret_cast<uint32_t*>(ts->adam_v), np);
     INSTA_HIP(hipGetLastError());
     ts->adam_step = 0;
+    return INSTA_OK;
+}
+
+// ---- NerfNetwork plugin entry points --------------------------------------
+namespace {
+insta_status ensure_net_workspace(insta_ctx* ctx, uint32_t n, hipStream_t stream) {
+    insta_status st = ensure_train_state(ctx, stream);
+    if (st != INSTA_OK) return st;
+    TrainState* ts = ctx->train;
+    st = ensure_train_workspace(ctx, ts, 0, n, n);
+    if (st != INSTA_OK) return st;
+    const uint32_t X = ctx->cfg.n_extra_dims;
+    if (X && ts->net_extra_cap < ts->compact_cap) {
+        dev_free(ts->net_extra);
+        ts->net_extra_cap = 0;
+        if (hipMalloc(reinterpret_cast<void**>(&ts->net_extra), (size_t)ts->compact_cap * X * sizeof(float)) !=
+            hipSuccess) {
+            set_error("network plugin workspace allocation failed");
+            return INSTA_ERR_OOM;
+        }
+        ts->net_extra_cap = ts->compact_cap;
+    }
+    return INSTA_OK;
+}
+}  // namespace
+
+insta_status network_forward(insta_ctx* ctx, const float* coords, uint32_t n, uint32_t stride, float* out,
+                             hipStream_t stream) {
+    insta_status st = ensure_net_workspace(ctx, n, stream);
+    if (st != INSTA_OK) return st;
+    TrainState* ts = ctx->train;
+    const uint32_t X = ctx->cfg.n_extra_dims;
+    hipLaunchKernelGGL(net_prepare_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, coords, n, stride, X, ts->s_cpos,
+                       ts->s_cdir, ts->c_idx, ts->net_extra, ts->counters);
+    INSTA_HIP(hipGetLastError());
+    float4* o = reinterpret_cast<float4*>(out);
+    const bool x16 = X != 0;
+    st = ctx->cfg.n_levels == 8 ? (x16 ? launch_net_forward<8, 16>(ctx, ts, o, stream) : launch_net_forward<8, 0>(ctx, ts, o, stream))
+                                : (x16 ? launch_net_forward<16, 16>(ctx, ts, o, stream) : launch_net_forward<16, 0>(ctx, ts, o, stream));
+    if (st != INSTA_OK) return st;
+    ts->net_n = n;
+    return INSTA_OK;
+}
+
+uint32_t network_forward_batch(const insta_ctx* ctx) { return ctx->train ? ctx->train->net_n : 0u; }
+
+insta_status network_backward(insta_ctx* ctx, uint32_t n, uint32_t stride, const float* dL_dout, float* grad_out,
+                              bool accumulate, float* dL_dinput, hipStream_t stream) {
+    TrainState* ts = ctx->train;
+    const uint64_t np = ctx->layout.n_params;
+    INSTA_HIP(hipMemcpyAsync(ts->c_dout, dL_dout, (size_t)n * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(1024), dim3(256), 0, stream, reinterpret_cast<uint32_t*>(ts->grad), np);
+    INSTA_HIP(hipGetLastError());
+    insta_train_config tc{};
+    tc.grid_grad_fp16 = 0;
+    const bool x16 = ctx->cfg.n_extra_dims != 0;
+    insta_status st =
+        ctx->cfg.n_levels == 8
+            ? (x16 ? launch_train_backward<8, 16>(ctx, ts, tc, stream) : launch_train_backward<8, 0>(ctx, ts, tc, stream))
+            : (x16 ? launch_train_backward<16, 16>(ctx, ts, tc, stream) : launch_train_backward<16, 0>(ctx, ts, tc, stream));
+    if (st != INSTA_OK) return st;
+    hipLaunchKernelGGL(net_grad_out_kernel, dim3(1024), dim3(256), 0, stream, ts->grad, grad_out, np,
+                       accumulate ? 1u : 0u);
+    INSTA_HIP(hipGetLastError());
+    if (dL_dinput) {
+        st = ctx->cfg.n_levels == 8
+                 ? (x16 ? launch_net_input_grad<8, 16>(ctx, ts, n, dL_dinput, stride, stream)
+                        : launch_net_input_grad<8, 0>(ctx, ts, n, dL_dinput, stride, stream))
+                 : (x16 ? launch_net_input_grad<16, 16>(ctx, ts, n, dL_dinput, stride, stream)
+                        : launch_net_input_grad<16, 0>(ctx, ts, n, dL_dinput, stride, stream));
+        if (st != INSTA_OK) return st;
+    }
     return INSTA_OK;
 }
 

@@ -1451,6 +1451,71 @@ int orc_render(const orc_scene* s, const insta_camera* cam, uint32_t width, uint
     return 0;
 }
 
+// Render-buffer stages of Testbed::render_frame for one sample index:
+// shade_kernel_nerf into a cleared frame buffer (testbed_nerf.cu:1794-1827),
+// accumulate_kernel in the linear colour space (render_buffer.cu:236-266,
+// 560-578) and tonemap_kernel (render_buffer.cu:264-350, 493-522), whose
+// result is also written back into the accumulation buffer.
+namespace {
+float srgb_to_linear_f(float c) { return c <= 0.04045f ? c / 12.92f : std::pow((c + 0.055f) / 1.055f, 2.4f); }
+float linear_to_srgb_f(float c) { return c < 0.0031308f ? 12.92f * c : 1.055f * std::pow(c, 0.41666f) - 0.055f; }
+}  // namespace
+
+int orc_frame_buffer(const float* rgba, const float* depth, uint32_t n, uint32_t spp, const insta_frame_config* fc,
+                     float* accum, float* out, float* depth_out) {
+    if (!fc || fc->tonemap_curve > INSTA_TONEMAP_REINHARD) return 1;
+    const float s = (float)spp;
+    const float acc_w = ORC_MUTANT == 6 ? s : s + 1.0f;
+    const float expo = std::pow(2.0f, fc->exposure);
+    float bg[4];
+    for (int c = 0; c < 3; ++c) bg[c] = srgb_to_linear_f(fc->background[c]);
+    bg[3] = fc->background[3];
+    float k[6] = {0, 0, 0, 0, 0, 0};
+    if (fc->tonemap_curve == INSTA_TONEMAP_ACES) {
+        k[0] = 0.6f * 0.6f * 2.51f; k[1] = 0.6f * 0.03f; k[2] = 0.0f;
+        k[3] = 0.6f * 0.6f * 2.43f; k[4] = 0.6f * 0.59f; k[5] = 0.14f;
+    } else if (fc->tonemap_curve == INSTA_TONEMAP_HABLE) {
+        const float A = 0.15f, B = 0.50f, C = 0.10f, D = 0.20f, E = 0.02f, F = 0.30f, W = 11.2f;
+        float k0 = A * F - A * E, k1 = C * B * F - B * E, k2 = 0.0f, k3 = A * F, k4 = B * F, k5 = D * F * F;
+        const float nom = k0 * (W * W) + k1 * W + k2, den = k3 * (W * W) + k4 * W + k5, white = den / nom;
+        k[0] = 4.0f * k0 * white; k[1] = 2.0f * k1 * white; k[2] = k2 * white;
+        k[3] = 4.0f * k3; k[4] = 2.0f * k4; k[5] = k5;
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        float f[4] = {rgba[4 * i], rgba[4 * i + 1], rgba[4 * i + 2], rgba[4 * i + 3]};
+        if (!fc->linear_colors)
+            for (int c = 0; c < 3; ++c) f[c] = srgb_to_linear_f(f[c]);
+        if (depth_out) depth_out[i] = f[3] > 0.2f && depth ? depth[i] : 0.0f;
+        float t[4] = {0, 0, 0, 0};
+        if (spp != 0)
+            for (int c = 0; c < 4; ++c) t[c] = accum[4 * i + c];
+        for (int c = 0; c < 4; ++c) t[c] = (t[c] * s + f[c]) / acc_w;
+        const float w = (1.0f - t[3]) * bg[3];
+        float col[3];
+        for (int c = 0; c < 3; ++c) col[c] = t[c] + bg[c] * w;
+        t[3] = t[3] + w;
+        for (int c = 0; c < 3; ++c) col[c] = col[c] * expo;
+        if (fc->tonemap_curve != INSTA_TONEMAP_IDENTITY) {
+            for (int c = 0; c < 3; ++c) col[c] = std::fmax(col[c], 0.0f);
+            if (fc->tonemap_curve == INSTA_TONEMAP_REINHARD) {
+                const float Y = 0.2126f * col[0] + 0.7152f * col[1] + 0.0722f * col[2];
+                const float sc = 1.0f / (Y + 1.0f);
+                for (int c = 0; c < 3; ++c) col[c] = col[c] * sc;
+            } else {
+                for (int c = 0; c < 3; ++c) {
+                    const float sq = col[c] * col[c];
+                    col[c] = (sq * k[0] + k[1] * col[c] + k[2]) / (k[3] * sq + k[4] * col[c] + k[5]);
+                }
+            }
+        }
+        if (fc->output_srgb)
+            for (int c = 0; c < 3; ++c) col[c] = linear_to_srgb_f(col[c]);
+        for (int c = 0; c < 3; ++c) accum[4 * i + c] = out[4 * i + c] = col[c];
+        accum[4 * i + 3] = out[4 * i + 3] = t[3];
+    }
+    return 0;
+}
+
 // FP64 reference render: the FP32 march of orc_render (identical samples),
 // then deformation, encoding, MLPs and compositing in double.  rgba [n*4].
 int orc_render_fp64(const orc_scene* s, const insta_camera* cam, uint32_t width, uint32_t height, int32_t frame,
@@ -1667,7 +1732,7 @@ bool train_ray(const orc_scene* s, uint32_t i, uint32_t n_rays, uint64_t rays_to
 
 struct SampleAct {
     double enc[64], h1[64], cin[32], h2[64], h3[64], out[4];
-    D3v pos;
+    D3v pos, dir;
 };
 
 // forward of the supported shape (1 density / 2 colour hidden layers, width
@@ -1676,6 +1741,7 @@ struct SampleAct {
 void forward_d(const orc_scene* s, D3v pos, D3v dir, const float* extra, SampleAct& a) {
     const uint32_t E = s->cfg.n_levels * 2, DIN = E + s->cfg.n_extra_dims;
     a.pos = pos;
+    a.dir = dir;
     encode_one_d(s, pos, a.enc);
     for (uint32_t e = E; e < DIN; ++e) a.enc[e] = extra[e - E];
     const float* p = s->params.data();
@@ -1704,8 +1770,11 @@ void forward_d(const orc_scene* s, D3v pos, D3v dir, const float* extra, SampleA
     a.out[3] = a.cin[0];
 }
 
-// backward of one sample: accumulates dL/dparams into g (FP64)
-void backward_d(const orc_scene* s, const SampleAct& a, const double dout[4], double* g) {
+// backward of one sample: accumulates dL/dparams into g (FP64); with
+// dinput (6 values) also the input gradients of NerfNetwork::backward_impl:
+// the position through each level's trilinear weights, the direction (input
+// x in [0,1], d = 2x - 1) through the SH polynomials (nerf_network.h:318-380)
+void backward_d(const orc_scene* s, const SampleAct& a, const double dout[4], double* g, double* dinput = nullptr) {
     const uint32_t E = s->cfg.n_levels * 2, DIN = E + s->cfg.n_extra_dims;
     const uint64_t W0 = 0, W1 = W0 + 64 * DIN, W2 = W1 + 16 * 64, W3 = W2 + 64 * 32, W4 = W3 + 64 * 64;
     const float* P = s->params.data();
@@ -1768,11 +1837,125 @@ void backward_d(const orc_scene* s, const SampleAct& a, const double dout[4], do
             gt[e + 1] += w * denc[2 * l + 1];
         }
     }
+    if (!dinput) return;
+    // d/dx of the trilinear interpolation: per corner, the derivative of its
+    // weight along each axis (scale_l times the other two factors)
+    const float* table = P + L.n_mlp_params;
+    double gx[3] = {0, 0, 0};
+    for (uint32_t l = 0; l < s->cfg.n_levels; ++l) {
+        const double sc = (double)L.level_scale[l];
+        double frac[3];
+        uint32_t gi[3];
+        for (int d = 0; d < 3; ++d) {
+            double pp = xs[d] * sc + 0.5;
+            double fl = std::floor(pp);
+            frac[d] = pp - fl;
+            gi[d] = (uint32_t)(int32_t)fl;
+        }
+        for (uint32_t c = 0; c < 8; ++c) {
+            double f[3];
+            for (int d = 0; d < 3; ++d) f[d] = ((c >> d) & 1u) ? frac[d] : 1.0 - frac[d];
+            uint32_t idx = grid_index(L, l, gi[0] + (c & 1u), gi[1] + ((c >> 1) & 1u), gi[2] + ((c >> 2) & 1u));
+            const uint64_t e = (uint64_t(L.level_offset[l]) + idx) * 2;
+            const double sfe = denc[2 * l] * (double)table[e] + denc[2 * l + 1] * (double)table[e + 1];
+            for (int d = 0; d < 3; ++d) {
+                const double dw = (((c >> d) & 1u) ? sc : -sc) * f[(d + 1) % 3] * f[(d + 2) % 3];
+                gx[d] += sfe * dw;
+            }
+        }
+    }
+    // dL/dSH = W2[:, 16:32]^T dH2, through the derivatives of the 16 closed forms
+    double dsh[16];
+    for (int q = 0; q < 16; ++q) {
+        double acc = 0.0;
+        for (int m = 0; m < 64; ++m) acc += (double)P[W2 + m * 32 + 16 + q] * dh2[m];
+        dsh[q] = acc;
+    }
+    const double x = a.dir.x, y = a.dir.y, z = a.dir.z, pi = 3.14159265358979323846;
+    const double c1 = std::sqrt(3.0 / (4 * pi)), c2 = 0.5 * std::sqrt(15.0 / pi), c3 = 0.25 * std::sqrt(5.0 / pi),
+                 c4 = 0.25 * std::sqrt(15.0 / pi), c5 = 0.25 * std::sqrt(35.0 / (2 * pi)), c6 = 0.5 * std::sqrt(105.0 / pi),
+                 c7 = 0.25 * std::sqrt(21.0 / (2 * pi)), c8 = 0.25 * std::sqrt(7.0 / pi), c9 = 0.25 * std::sqrt(105.0 / pi);
+    // rows: d sh_q / d(x, y, z)
+    const double J[16][3] = {{0, 0, 0},
+                             {0, -c1, 0},
+                             {0, 0, c1},
+                             {-c1, 0, 0},
+                             {c2 * y, c2 * x, 0},
+                             {0, -c2 * z, -c2 * y},
+                             {0, 0, 6 * c3 * z},
+                             {-c2 * z, 0, -c2 * x},
+                             {2 * c4 * x, -2 * c4 * y, 0},
+                             {-6 * c5 * x * y, c5 * (3 * y * y - 3 * x * x), 0},
+                             {c6 * y * z, c6 * x * z, c6 * x * y},
+                             {0, c7 * (1 - 5 * z * z), -10 * c7 * y * z},
+                             {0, 0, c8 * (15 * z * z - 3)},
+                             {c7 * (1 - 5 * z * z), 0, -10 * c7 * x * z},
+                             {2 * c9 * x * z, -2 * c9 * y * z, c9 * (x * x - y * y)},
+                             {c5 * (3 * y * y - 3 * x * x), 6 * c5 * x * y, 0}};
+    double gd[3] = {0, 0, 0};
+    for (int q = 0; q < 16; ++q)
+        for (int d = 0; d < 3; ++d) gd[d] += dsh[q] * J[q][d];
+    for (int d = 0; d < 3; ++d) {
+        dinput[d] = gx[d];
+        dinput[3 + d] = 2.0 * gd[d];
+    }
 }
 
 }  // namespace
 
 extern "C" {
+
+// NerfNetwork plugin in FP64 (nerf_network.h:238-414): forward on
+// NerfCoordinate samples given in double (stride floats each; extra dims the
+// last n_extra_dims), out [n*4] = rgb logits and density logit; with dL_dout,
+// grads [n_params] (overwritten) and dinput [n*6] (position, then the
+// direction input in [0,1]).
+int orc_network_plugin_d(const orc_scene* s, const double* coords, uint32_t n, uint32_t stride, const double* dL_dout,
+                         double* out, double* grads, double* dinput) {
+    const insta_config& c = s->cfg;
+    if (s->params.empty() || c.n_features_per_level != 2 || c.n_neurons != 64 || c.density_hidden_layers != 1 ||
+        c.rgb_hidden_layers != 2 || c.density_out != 16 || c.n_levels * 2 + c.n_extra_dims > 64 ||
+        stride != 11 + c.n_extra_dims)
+        return 1;
+    const uint64_t np = s->layout.n_params;
+    if (grads) std::fill(grads, grads + np, 0.0);
+    std::vector<double> g(grads ? np : 0, 0.0);
+    for (uint32_t i = 0; i < n; ++i) {
+        const double* co = coords + (size_t)i * stride;
+        float extra[64];
+        for (uint32_t e = 0; e < c.n_extra_dims; ++e) extra[e] = (float)co[stride - c.n_extra_dims + e];
+        SampleAct a;
+        forward_d(s, {co[0], co[1], co[2]}, {co[8] * 2.0 - 1.0, co[9] * 2.0 - 1.0, co[10] * 2.0 - 1.0}, extra, a);
+        if (out)
+            for (int k = 0; k < 4; ++k) out[4 * (size_t)i + k] = a.out[k];
+        if (dL_dout && (grads || dinput)) {
+            const double* d = dL_dout + 4 * (size_t)i;
+            std::vector<double> tmp;
+            double* gg = grads;
+            if (!gg) {
+                tmp.assign(np, 0.0);
+                gg = tmp.data();
+            }
+            backward_d(s, a, d, gg, dinput ? dinput + 6 * (size_t)i : nullptr);
+        }
+    }
+    return 0;
+}
+
+// NerfNetwork::density in FP64: the density MLP's 16 outputs, extra dims 1.0
+int orc_network_density_d(const orc_scene* s, const double* pos, uint32_t n, double* out) {
+    const insta_config& c = s->cfg;
+    if (s->params.empty() || c.density_hidden_layers != 1 || c.density_out != 16) return 1;
+    const uint32_t E = c.n_levels * c.n_features_per_level, DIN = E + c.n_extra_dims, W = c.n_neurons;
+    for (uint32_t i = 0; i < n; ++i) {
+        double din[INSTA_MAX_LEVELS * 8 + 64] = {0}, h[256];
+        encode_one_d(s, {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]}, din);
+        for (uint32_t e = E; e < DIN; ++e) din[e] = 1.0;
+        dense_d(s->params.data(), din, DIN, W, true, h);
+        dense_d(s->params.data() + (uint64_t)W * DIN, h, W, 16, false, out + 16 * (size_t)i);
+    }
+    return 0;
+}
 
 // the grid training rays march: slot >= -1 for all rays, -2 each image's frame
 int orc_set_training_occupancy(orc_scene* s, int32_t slot) {

@@ -11,7 +11,7 @@ from pathlib import Path
 
 import numpy as np
 
-from insta_amd.capi import InstaCamera, InstaConfig, InstaLayout
+from insta_amd.capi import InstaCamera, InstaConfig, InstaFrameConfig, InstaLayout
 
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "liboracle.so"
@@ -51,6 +51,9 @@ _SIGS = [
     ("orc_pixel_offset", None, [C.c_uint32, _VP]),
     ("orc_render", C.c_int, [_VP, C.POINTER(InstaCamera), C.c_uint32, C.c_uint32, C.c_int32, C.c_int32,
                              C.c_uint32, _VP, C.c_uint32, _VP, _VP, _VP, _VP]),
+    ("orc_network_plugin_d", C.c_int, [_VP, _VP, C.c_uint32, C.c_uint32, _VP, _VP, _VP, _VP]),
+    ("orc_network_density_d", C.c_int, [_VP, _VP, C.c_uint32, _VP]),
+    ("orc_frame_buffer", C.c_int, [_VP, _VP, C.c_uint32, C.c_uint32, C.POINTER(InstaFrameConfig), _VP, _VP, _VP]),
     ("orc_render_fp64", C.c_int, [_VP, C.POINTER(InstaCamera), C.c_uint32, C.c_uint32, C.c_int32, C.c_int32,
                                   C.c_uint32, _VP, C.c_uint32, _VP]),
     ("orc_set_training_set", C.c_int, [_VP, C.c_uint32, C.c_uint32, C.c_uint32, _VP, _VP, _VP]),
@@ -296,6 +299,39 @@ def render_fp64(orc: Oracle, cam: InstaCamera, width: int, height: int, frame: i
                                0 if sub is None else len(sub), _p(rgba))
     assert rc == 0, rc
     return rgba
+
+
+def network_plugin_d(orc: "Oracle", coords, dL_dout=None):
+    """NerfNetwork forward (and backward with dL_dout) in FP64 on NerfCoordinate
+    samples [n, stride]: (out [n, 4], grads [n_params] or None, dinput [n, 6] or
+    None) -- dinput holds d/dposition then d/d(direction input in [0,1])."""
+    c = _c(coords, np.float64)
+    n, stride = c.shape
+    out = np.zeros((n, 4), np.float64)
+    d = None if dL_dout is None else _c(dL_dout, np.float64)
+    g = None if d is None else np.zeros(orc.layout.n_params, np.float64)
+    di = None if d is None else np.zeros((n, 6), np.float64)
+    assert orc.L.orc_network_plugin_d(orc.h, _p(c), n, stride, _p(d), _p(out), _p(g), _p(di)) == 0
+    return out, g, di
+
+
+def network_density_d(orc: "Oracle", pos) -> np.ndarray:
+    p = _c(pos, np.float64)
+    out = np.zeros((len(p), 16), np.float64)
+    assert orc.L.orc_network_density_d(orc.h, _p(p), len(p), _p(out)) == 0
+    return out
+
+
+def frame_buffer(rgba, depth, spp: int, fc: InstaFrameConfig, accum, lib_path=None):
+    """Render-buffer stages (shade, accumulate, tonemap) on a composited
+    image; `accum` [n, 4] float32 is updated in place.  Returns (out, depth)."""
+    c = _c(rgba, np.float32).reshape(-1, 4)
+    d = None if depth is None else _c(depth, np.float32)
+    assert accum.dtype == np.float32 and accum.flags.c_contiguous and accum.size == c.size
+    out = np.zeros_like(c)
+    dout = np.zeros(len(c), np.float32)
+    assert lib(lib_path).orc_frame_buffer(_p(c), _p(d), len(c), spp, C.byref(fc), _p(accum), _p(out), _p(dout)) == 0
+    return out, dout
 
 
 def sh16(dirs_w, lib_path=None) -> np.ndarray:

@@ -400,3 +400,57 @@ def test_render_reduced_precision_psnr(head_avatar, prec):
     p = _psnr(rgba[:, :3], r_rgba[:, :3])
     print(f"precision {prec}: PSNR vs FP32 oracle {p:.1f} dB")
     assert p >= 45.0
+
+
+# ---------------------------------------------------------------- render buffer
+@pytest.mark.parametrize("curve,srgb", [(0, 1), (1, 1), (2, 0), (3, 1)])
+def test_render_frame_accumulates_like_the_oracle(small_avatar, curve, srgb):
+    """Testbed::render_frame over 4 sample indices (jittered cameras): the
+    shaded, accumulated and tonemapped frame against the oracle's render of
+    each sample index followed by its render-buffer stages.  No early
+    termination, so every ray's samples are pure march geometry."""
+    torch = _torch()
+    from insta_amd.capi import default_frame_config
+    from oracle import oracle as O
+    ctx, orc = _scene(small_config(min_transmittance=0.0), small_avatar)
+    ctx.build_occupancy(0, 0.05)
+    orc.build_occupancy(0, 0.05)
+    res = 48
+    cam = make_camera(synth.look_at_camera(np.array([0.2, -0.1, 1.0]), 1.0), synth.focal_from_fov(res, 30.0),
+                      snap=False)
+    fc = default_frame_config(exposure=0.3, background=(0.2, 0.5, 0.9, 1.0), tonemap_curve=curve, output_srgb=srgb)
+    accum = torch.zeros((res * res, 4), dtype=torch.float32, device="cuda")
+    depth = torch.empty(res * res, dtype=torch.float32, device="cuda")
+    r_accum = np.zeros((res * res, 4), np.float32)
+    for spp in range(4):
+        out = ctx.render_frame(cam, res, res, 0, 0, spp, fc, accum, depth=depth)
+        torch.cuda.synchronize()
+        r_rgba, r_depth, _, _ = orc.render(cam, res, res, 0, 0, spp)
+        r_out, r_dout = O.frame_buffer(r_rgba, r_depth, spp, fc, r_accum)
+        assert_rel_close(out.cpu().numpy(), r_out, RTOL, f"frame spp {spp}")
+        assert_rel_close(accum.cpu().numpy(), r_accum, RTOL, f"accumulation buffer spp {spp}")
+        d, rd = depth.cpu().numpy(), r_dout
+        assert np.array_equal(d == 0, rd == 0)
+        assert_rel_close(d[rd > 0], rd[rd > 0], RTOL, "depth buffer")
+
+
+def test_frame_buffer_stage_on_identical_inputs():
+    """insta_frame_buffer on one composited image sequence: the same values as
+    the oracle's stage to rounding of pow (GPU and host libm)."""
+    torch = _torch()
+    from insta_amd.capi import default_frame_config, frame_buffer
+    from oracle import oracle as O
+    ensure_oracle()
+    rng = np.random.default_rng(12)
+    n = 5000
+    fc = default_frame_config(exposure=-0.5, background=(0.9, 0.1, 0.3, 0.6), tonemap_curve=2)
+    accum = torch.zeros((n, 4), dtype=torch.float32, device="cuda")
+    out = torch.empty_like(accum)
+    r_accum = np.zeros((n, 4), np.float32)
+    for spp in range(5):
+        a = rng.uniform(0, 1, n).astype(np.float32)
+        rgba = np.concatenate([rng.uniform(0, 1, (n, 3)).astype(np.float32) * a[:, None], a[:, None]], axis=1)
+        frame_buffer(torch.from_numpy(rgba).cuda(), None, spp, fc, accum, out)
+        torch.cuda.synchronize()
+        r_out, _ = O.frame_buffer(rgba, None, spp, fc, r_accum)
+        assert_rel_close(out.cpu().numpy(), r_out, 2e-6, f"frame buffer spp {spp}")

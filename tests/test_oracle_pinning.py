@@ -21,6 +21,12 @@ check that every pin is live: the oracle is rebuilt with a planted mistake
                   (pinhole ray, slab entry, max(t, 0.05) + 1e-6, one-step
                   jitter, t += dt while inside the render box), compared
                   exactly
+  render buffer   shade (sRGB -> linear), running mean over 4 sample
+                  indices, background blend, exposure, the four tonemap
+                  curves and the output transfer function, with the
+                  reference's write-back of the tonemapped colour into the
+                  accumulation buffer (testbed_nerf.cu:1794-1827,
+                  render_buffer.cu:236-350, 493-522), in FP64 numpy
 """
 import math
 import subprocess
@@ -298,8 +304,90 @@ def test_march_count_fp32_restated_exactly():
     check_march()
 
 
+# ---------------------------------------------------------------- render buffer
+def _tonemap_np(x, curve):
+    if curve == 0:
+        return x
+    x = np.maximum(x, 0.0)
+    if curve == 3:   # Reinhard on luminance
+        Y = x @ np.array([0.2126, 0.7152, 0.0722])
+        return x / (Y + 1.0)[:, None]
+    if curve == 1:   # ACES (Narkowicz), 0.6 pre-exposure
+        a, b, c, d, e = 2.51, 0.03, 2.43, 0.59, 0.14
+        x = 0.6 * x
+        return (x * (a * x + b)) / (x * (c * x + d) + e)
+    # Hable, white 11.2, exposure bias 2
+    A, B, C, D, E, F, W = 0.15, 0.50, 0.10, 0.20, 0.02, 0.30, 11.2
+    h = lambda v: (v * (A * v + C * B) + D * E) / (v * (A * v + B) + D * F) - E / F
+    return h(2.0 * x) / h(W)
+
+
+def _srgb_to_linear_np(c):
+    return np.where(c <= 0.04045, c / 12.92, ((c + 0.055) / 1.055) ** 2.4)
+
+
+def _linear_to_srgb_np(c):
+    return np.where(c < 0.0031308, 12.92 * c, 1.055 * np.maximum(c, 0.0031308) ** 0.41666 - 0.055)
+
+
+def check_frame_buffer(lib_path=None):
+    from insta_amd.capi import default_frame_config
+    from oracle import oracle as O
+    rng = np.random.default_rng(4)
+    n = 500
+    frames = []
+    for _ in range(4):
+        a = rng.uniform(0.0, 1.0, n)
+        a[rng.uniform(size=n) < 0.2] = 0.0
+        rgb = rng.uniform(0.0, 1.0, (n, 3)) * a[:, None]
+        frames.append(np.concatenate([rgb, a[:, None]], axis=1).astype(np.float32))
+    depth = rng.uniform(0.1, 2.0, n).astype(np.float32)
+    for curve in (0, 1, 2, 3):
+        for srgb in (1, 0):
+            bg = (0.3, 0.8, 0.5, 0.75)
+            fc = default_frame_config(exposure=0.4, background=bg, tonemap_curve=curve, output_srgb=srgb)
+            accum = np.zeros((n, 4), np.float32)
+            ref_acc = np.zeros((n, 4))
+            bg_lin = _srgb_to_linear_np(np.array(bg[:3]))
+            for k, f in enumerate(frames):
+                out, dout = O.frame_buffer(f, depth, k, fc, accum, lib_path)
+                fr = f.astype(np.float64)
+                lin = np.concatenate([_srgb_to_linear_np(fr[:, :3]), fr[:, 3:]], axis=1)
+                t = (ref_acc * k + lin) / (k + 1)
+                w = (1.0 - t[:, 3]) * bg[3]
+                col = (t[:, :3] + bg_lin[None] * w[:, None]) * 2.0 ** 0.4
+                col = _tonemap_np(col, curve)
+                if srgb:
+                    col = _linear_to_srgb_np(col)
+                ref_acc = np.concatenate([col, (t[:, 3] + w)[:, None]], axis=1)   # written back
+                assert np.abs(out - ref_acc).max() < 2e-5, (curve, srgb, k, np.abs(out - ref_acc).max())
+                assert np.array_equal(out, accum)
+                assert np.array_equal(dout, np.where(f[:, 3] > 0.2, depth, 0.0).astype(np.float32))
+
+
+def test_frame_buffer_by_hand():
+    ensure_oracle()
+    check_frame_buffer()
+
+
+def test_frame_buffer_identical_frames_keep_their_mean():
+    """Linear colours, identity curve, linear output, no exposure: the running
+    mean of one image is that image with the background behind it."""
+    from insta_amd.capi import default_frame_config
+    from oracle import oracle as O
+    ensure_oracle()
+    rng = np.random.default_rng(5)
+    a = rng.uniform(0.0, 1.0, 64)
+    f = np.concatenate([rng.uniform(0, 1, (64, 3)) * a[:, None], a[:, None]], axis=1).astype(np.float32)
+    fc = default_frame_config(output_srgb=0, linear_colors=1, background=(0.2, 0.4, 0.6, 0.0))
+    accum = np.zeros((64, 4), np.float32)
+    for k in range(6):
+        out, _ = O.frame_buffer(f, None, k, fc, accum)
+        assert np.abs(out - f).max() < 1e-6, k
+
+
 # ---------------------------------------------------------------- liveness
-PINS = {1: check_blend, 2: check_sh, 3: check_loss, 4: check_march, 5: check_loss}
+PINS = {1: check_blend, 2: check_sh, 3: check_loss, 4: check_march, 5: check_loss, 6: check_frame_buffer}
 
 
 @pytest.fixture(scope="session")
