@@ -132,6 +132,36 @@ const char* insta_version(void);
 /* Default configuration (the full-size workload shapes). */
 void insta_default_config(insta_config* cfg);
 
+/* Run settings of the reference's JSON configuration that are not network
+ * shape: the optimizer (configs/nerf/main.json:21-32: Ema over Adam) and
+ * INSTA's run keys (src/core.cu:199-221). */
+typedef struct insta_run_config {
+    float    learning_rate, beta1, beta2, epsilon;  /* Adam */
+    float    l2_reg;              /* Adam's L2 regularisation (gradient += l2_reg * param) */
+    float    ema_decay;           /* Ema wrapper decay; 0 = no Ema */
+    uint32_t exp_params;          /* expression parameters = the density input's extra dims */
+    uint32_t geo_params, use_eyes, optimize_latent_code;
+    uint32_t max_training_steps;  /* "max_steps": training iterations of a run */
+    float    depth_weight;        /* depth supervision lambda */
+    float    seg_weight;          /* segmentation supervision lambda */
+    uint32_t max_cached_bvh, max_images_gpu;
+} insta_run_config;
+
+void insta_default_run_config(insta_run_config* run);
+
+/* Testbed::load_network_config / reload_network_from_file
+ * (instant-ngp/src/testbed.cu:77-152) with the readers of reset_network
+ * (testbed.cu:2181-2300), NerfNetwork (nerf_network.h:135-152) and
+ * rta::Core::load_training_data (src/core.cu:199-221): reads a JSON file
+ * (comments allowed), merges its "parent" chain (the parent, relative to the
+ * child's directory, merge-patched with the child, RFC 7386), and fills cfg
+ * (insta_default_config for absent keys; encoding, network, rgb_network,
+ * dir_encoding, exp_params -> n_extra_dims, disable_interp; per_level_scale
+ * from the finest resolution 2048 when absent) and run (nullable).  Returns
+ * INSTA_ERR_UNSUPPORTED for an encoding, network, activation, direction
+ * encoding, loss or optimizer this library does not implement.  Host only. */
+insta_status insta_load_config_json(const char* path, insta_config* cfg, insta_run_config* run);
+
 /* Parameter/table layout for a configuration (host-only, no device). */
 insta_status insta_query_layout(const insta_config* cfg, insta_layout* out);
 

@@ -51,6 +51,14 @@ class InstaRenderStats(C.Structure):
 TONEMAP_IDENTITY, TONEMAP_ACES, TONEMAP_HABLE, TONEMAP_REINHARD = 0, 1, 2, 3
 
 
+class InstaRunConfig(C.Structure):
+    _fields_ = [("learning_rate", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("epsilon", C.c_float),
+                ("l2_reg", C.c_float), ("ema_decay", C.c_float), ("exp_params", C.c_uint32), ("geo_params", C.c_uint32),
+                ("use_eyes", C.c_uint32), ("optimize_latent_code", C.c_uint32), ("max_training_steps", C.c_uint32),
+                ("depth_weight", C.c_float), ("seg_weight", C.c_float), ("max_cached_bvh", C.c_uint32),
+                ("max_images_gpu", C.c_uint32)]
+
+
 class InstaFrameConfig(C.Structure):
     _fields_ = [("exposure", C.c_float), ("background", C.c_float * 4), ("tonemap_curve", C.c_uint32),
                 ("output_srgb", C.c_uint32), ("linear_colors", C.c_uint32)]
@@ -121,6 +129,8 @@ SIGNATURES = [
     ("insta_version", C.c_char_p, []),
     ("insta_default_config", None, [C.POINTER(InstaConfig)]),
     ("insta_query_layout", C.c_int, [C.POINTER(InstaConfig), C.POINTER(InstaLayout)]),
+    ("insta_default_run_config", None, [C.POINTER(InstaRunConfig)]),
+    ("insta_load_config_json", C.c_int, [C.c_char_p, C.POINTER(InstaConfig), C.POINTER(InstaRunConfig)]),
     ("insta_create", C.c_int, [C.POINTER(InstaConfig), C.POINTER(_VP)]),
     ("insta_destroy", None, [_VP]),
     ("insta_set_canonical_mesh", C.c_int, [_VP, _VP, C.c_uint32, _VP, C.c_uint32, _VP, _VP]),
@@ -534,6 +544,14 @@ class Context:
         st = InstaRenderStats()
         check(self.lib.insta_get_render_stats(self.h, C.byref(st)), "insta_get_render_stats")
         return st
+
+
+def load_config_json(path) -> tuple[InstaConfig, InstaRunConfig]:
+    """The reference's JSON network configuration ("parent" chain merged)."""
+    cfg, run = InstaConfig(), InstaRunConfig()
+    check(load_library().insta_load_config_json(str(path).encode(), C.byref(cfg), C.byref(run)),
+          "insta_load_config_json")
+    return cfg, run
 
 
 def default_frame_config(**kw) -> InstaFrameConfig:

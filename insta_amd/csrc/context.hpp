@@ -146,6 +146,7 @@ inline void dev_free(T*& p) {
 }
 
 void set_error(const std::string& msg);
+insta_status load_config_json(const char* path, insta_config* cfg, insta_run_config* run);
 insta_status hip_fail(hipError_t e, const char* what);
 
 #define INSTA_HIP(call)                                              \

@@ -193,7 +193,10 @@ bool network_supported(const insta_config& c) {
     const bool shape = c.n_neurons == 64 && c.density_out == 16 && c.sh_degree == 4 &&
                        (c.n_extra_dims == 0 || c.n_extra_dims == 16) &&
                        c.density_hidden_layers >= 1 && c.rgb_hidden_layers >= 1;
-    const bool grid = c.n_features_per_level == 2 && (c.n_levels == 8 || c.n_levels == 16);
+    // F = 2 (the workload shapes) or the reference's configs/nerf/main.json
+    // grid (16 levels of 8 features) on this per-sample FP32 path
+    const bool grid = (c.n_features_per_level == 2 && (c.n_levels == 8 || c.n_levels == 16)) ||
+                      (c.n_features_per_level == 8 && c.n_levels == 16);
     return shape && grid;
 }
 
@@ -213,6 +216,8 @@ insta_status launch_network_aos(insta_ctx* ctx, const float* coords, uint32_t n,
     INSTA_DISPATCH_LF(16, 2, 0, network_aos_kernel, grid, block, 0, stream, a, coords, n, stride, out)
     INSTA_DISPATCH_LF(8, 2, 16, network_aos_kernel, grid, block, 0, stream, a, coords, n, stride, out)
     INSTA_DISPATCH_LF(16, 2, 16, network_aos_kernel, grid, block, 0, stream, a, coords, n, stride, out)
+    INSTA_DISPATCH_LF(16, 8, 0, network_aos_kernel, grid, block, 0, stream, a, coords, n, stride, out)
+    INSTA_DISPATCH_LF(16, 8, 16, network_aos_kernel, grid, block, 0, stream, a, coords, n, stride, out)
     set_error("network: unsupported (n_levels, n_features_per_level)");
     return INSTA_ERR_UNSUPPORTED;
 }
@@ -226,6 +231,8 @@ insta_status launch_network_soa(insta_ctx* ctx, const uint32_t* n_dev, uint32_t 
     INSTA_DISPATCH_LF(16, 2, 0, network_soa_kernel, grid, block, 0, stream, a, n_dev, cpos, cdir, out)
     INSTA_DISPATCH_LF(8, 2, 16, network_soa_kernel, grid, block, 0, stream, a, n_dev, cpos, cdir, out)
     INSTA_DISPATCH_LF(16, 2, 16, network_soa_kernel, grid, block, 0, stream, a, n_dev, cpos, cdir, out)
+    INSTA_DISPATCH_LF(16, 8, 0, network_soa_kernel, grid, block, 0, stream, a, n_dev, cpos, cdir, out)
+    INSTA_DISPATCH_LF(16, 8, 16, network_soa_kernel, grid, block, 0, stream, a, n_dev, cpos, cdir, out)
     set_error("network: unsupported (n_levels, n_features_per_level)");
     return INSTA_ERR_UNSUPPORTED;
 }

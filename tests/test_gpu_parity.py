@@ -454,3 +454,32 @@ def test_frame_buffer_stage_on_identical_inputs():
         torch.cuda.synchronize()
         r_out, _ = O.frame_buffer(rgba, None, spp, fc, r_accum)
         assert_rel_close(out.cpu().numpy(), r_out, 2e-6, f"frame buffer spp {spp}")
+
+
+# ---------------------------------------------------------------- reference configuration
+def test_render_with_the_reference_main_config(small_avatar, tmp_path):
+    """A configs/nerf/main.json-shaped network (hash grid 16 levels x 8
+    features, T = 2^18; density MLP with 2 hidden layers, colour MLP with 3;
+    16 expression values) read through insta_load_config_json renders on the
+    per-sample FP32 path like the oracle."""
+    torch = _torch()
+    import json
+    from insta_amd.capi import load_config_json
+    from tests.test_config_json import MAIN
+    path = tmp_path / "main.json"
+    path.write_text(json.dumps(MAIN))
+    cfg, _ = load_config_json(path)
+    for k, v in dict(grid_size=64, dt=1.0 / 256.0, max_steps=256, max_frames=2, max_triangles=4096,
+                     max_vertices=4096).items():
+        setattr(cfg, k, v)
+    assert cfg.n_features_per_level == 8 and cfg.density_hidden_layers == 2 and cfg.n_extra_dims == 16
+    ctx, orc = _scene(cfg, small_avatar)
+    vals = np.random.default_rng(3).standard_normal((2, 16)).astype(np.float32)
+    ctx.set_frame_extra_dims(0, torch.from_numpy(vals).cuda())
+    orc.set_frame_extra_dims(0, vals)
+    ctx.build_occupancy(0, 0.05)
+    orc.build_occupancy(0, 0.05)
+    cam = make_camera(synth.look_at_camera(np.array([0.3, 0.2, 1.0]), 1.0), synth.focal_from_fov(32, 30.0))
+    (rgba, depth, steps), (r_rgba, r_depth, r_steps), _ = _render_pair(ctx, orc, cam, 32, 32, 0, 0)
+    assert steps.sum() > 1000
+    assert_render_matches(rgba, steps, r_rgba, r_steps, RTOL, cfg.min_transmittance, "main.json network")
