@@ -91,22 +91,56 @@ constexpr int kDensityOut = 16;
 constexpr int kRgbIn = kDensityOut + 16;
 
 // extra: the sample's X extra dims (null: all 1.0)
+// First density layer, level by level: each level's F features go straight
+// into the 64 accumulators, so a wide input (16 levels x 8 features) never
+// lives as one unrolled vector.  Every output still sums its inputs in index
+// order with one FMA each, so the result equals dense<DIN, 64, true> bit for bit.
 template <int L, int F, int X>
-INSTA_DEV void network_one(const NetArgs& a, V3 pos, V3 dir_w, const float* extra, float out[4]) {
+INSTA_DEV void first_layer_by_level(const NetArgs& a, V3 pos, const float* extra, float h[kWidth]) {
     constexpr int DIN = L * F + X;
-    float x[DIN];
+    const float* W = a.params;
 #pragma unroll
+    for (int o = 0; o < kWidth; ++o) h[o] = 0.0f;
+#pragma unroll 1
     for (int l = 0; l < L; ++l) {
         float acc[F];
         encode_level<F>(a.table, a.gp, (uint32_t)l, pos, acc);
 #pragma unroll
-        for (int f = 0; f < F; ++f) x[l * F + f] = acc[f];
+        for (int o = 0; o < kWidth; ++o) {
+#pragma unroll
+            for (int f = 0; f < F; ++f) h[o] = __builtin_fmaf(W[o * DIN + l * F + f], acc[f], h[o]);
+        }
     }
 #pragma unroll
-    for (int e = 0; e < X; ++e) x[L * F + e] = extra ? extra[e] : 1.0f;
+    for (int e = 0; e < X; ++e) {
+        const float xe = extra ? extra[e] : 1.0f;
+#pragma unroll
+        for (int o = 0; o < kWidth; ++o) h[o] = __builtin_fmaf(W[o * DIN + L * F + e], xe, h[o]);
+    }
+#pragma unroll
+    for (int o = 0; o < kWidth; ++o) h[o] = h[o] > 0.0f ? h[o] : 0.0f;
+}
+
+template <int L, int F, int X>
+INSTA_DEV void network_one(const NetArgs& a, V3 pos, V3 dir_w, const float* extra, float out[4]) {
+    constexpr int DIN = L * F + X;
     float h[kWidth], g[kWidth];
     const float* W = a.params;
-    dense<DIN, kWidth, true>(W, x, h);
+    if constexpr (F >= 8) {
+        first_layer_by_level<L, F, X>(a, pos, extra, h);
+    } else {
+        float x[DIN];
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            float acc[F];
+            encode_level<F>(a.table, a.gp, (uint32_t)l, pos, acc);
+#pragma unroll
+            for (int f = 0; f < F; ++f) x[l * F + f] = acc[f];
+        }
+#pragma unroll
+        for (int e = 0; e < X; ++e) x[L * F + e] = extra ? extra[e] : 1.0f;
+        dense<DIN, kWidth, true>(W, x, h);
+    }
     W += kWidth * DIN;
     for (uint32_t k = 1; k < a.density_hidden; ++k) {
         dense<kWidth, kWidth, true>(W, h, g);

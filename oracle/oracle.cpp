@@ -1957,6 +1957,29 @@ int orc_network_density_d(const orc_scene* s, const double* pos, uint32_t n, dou
     return 0;
 }
 
+// NerfNetwork::density in FP32 (nerf_network.h:385-414): the density MLP's
+// 16 outputs, extra dims 1.0, each layer summed in input order
+int orc_network_density(const orc_scene* s, const float* pos, uint32_t n, float* out) {
+    const insta_config& c = s->cfg;
+    if (s->params.empty() || c.density_out != 16) return 1;
+    const uint32_t E = c.n_levels * c.n_features_per_level, DIN = E + c.n_extra_dims, W = c.n_neurons;
+    for (uint32_t i = 0; i < n; ++i) {
+        float din[INSTA_MAX_LEVELS * 8 + 64], h0[256], h1[256];
+        encode_one(s, {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]}, din);
+        for (uint32_t e = E; e < DIN; ++e) din[e] = 1.0f;
+        const float* p = s->params.data();
+        dense(p, din, DIN, W, true, h0);
+        p += (uint64_t)W * DIN;
+        for (uint32_t k = 1; k < c.density_hidden_layers; ++k) {
+            dense(p, h0, W, W, true, h1);
+            std::memcpy(h0, h1, W * sizeof(float));
+            p += (uint64_t)W * W;
+        }
+        dense(p, h0, W, 16, false, out + 16 * (size_t)i);
+    }
+    return 0;
+}
+
 // the grid training rays march: slot >= -1 for all rays, -2 each image's frame
 int orc_set_training_occupancy(orc_scene* s, int32_t slot) {
     if (slot < -2 || slot + 1 >= (int32_t)s->grids.size()) return 1;

@@ -53,6 +53,7 @@ _SIGS = [
                              C.c_uint32, _VP, C.c_uint32, _VP, _VP, _VP, _VP]),
     ("orc_network_plugin_d", C.c_int, [_VP, _VP, C.c_uint32, C.c_uint32, _VP, _VP, _VP, _VP]),
     ("orc_network_density_d", C.c_int, [_VP, _VP, C.c_uint32, _VP]),
+    ("orc_network_density", C.c_int, [_VP, _VP, C.c_uint32, _VP]),
     ("orc_frame_buffer", C.c_int, [_VP, _VP, C.c_uint32, C.c_uint32, C.POINTER(InstaFrameConfig), _VP, _VP, _VP]),
     ("orc_render_fp64", C.c_int, [_VP, C.POINTER(InstaCamera), C.c_uint32, C.c_uint32, C.c_int32, C.c_int32,
                                   C.c_uint32, _VP, C.c_uint32, _VP]),
@@ -313,6 +314,14 @@ def network_plugin_d(orc: "Oracle", coords, dL_dout=None):
     di = None if d is None else np.zeros((n, 6), np.float64)
     assert orc.L.orc_network_plugin_d(orc.h, _p(c), n, stride, _p(d), _p(out), _p(g), _p(di)) == 0
     return out, g, di
+
+
+def network_density(orc: "Oracle", pos) -> np.ndarray:
+    """NerfNetwork::density in FP32: [n, 16] density MLP outputs."""
+    p = _c(pos, np.float32)
+    out = np.zeros((len(p), 16), np.float32)
+    assert orc.L.orc_network_density(orc.h, _p(p), len(p), _p(out)) == 0
+    return out
 
 
 def network_density_d(orc: "Oracle", pos) -> np.ndarray:

@@ -110,3 +110,21 @@ def test_density_is_the_forward_density_with_extras_one():
     dens = O.network_density_d(o, c[:, 0:3])
     assert dens.shape == (200, 16)
     assert np.abs(dens[:, 0] - out[:, 3]).max() < 1e-12
+
+
+@pytest.mark.parametrize("n_extra", [0, 16])
+def test_fp32_density_is_the_fp32_network_density(n_extra):
+    """The FP32 density restatement: column 0 is the FP32 network's density
+    logit bit for bit (same sums, same order; extra dims 1.0), and every
+    column is the FP64 density to FP32 rounding."""
+    from oracle import oracle as O
+    o, cfg = _oracle(n_extra, table_scale=0.05)
+    c = _coords(np.random.default_rng(5), 2000, n_extra).astype(np.float32)
+    if n_extra:
+        c[:, 11:] = 1.0
+    dens = O.network_density(o, c[:, 0:3])
+    assert dens.dtype == np.float32 and dens.shape == (2000, 16)
+    assert np.array_equal(dens[:, 0], o.network(c)[:, 3])
+    ref = O.network_density_d(o, c[:, 0:3].astype(np.float64))
+    for k in range(16):
+        assert np.linalg.norm(dens[:, k] - ref[:, k]) <= 1e-5 * np.linalg.norm(ref[:, k])
