@@ -61,7 +61,10 @@ def setup(args, precision=0, av=None, n_extra=0):
     import torch
     from insta_amd import synth
     from insta_amd.capi import Context, full_config, make_camera
-    cfg = full_config(max_frames=16, precision=precision, n_extra_dims=n_extra)
+    kw = {} if args.cell_split is None else dict(cell_split=args.cell_split)
+    if args.cell_list_entries:
+        kw["cell_list_entries"] = args.cell_list_entries
+    cfg = full_config(max_frames=16, precision=precision, n_extra_dims=n_extra, **kw)
     if av is None:
         av = synth.head_avatar(seed=1, n_frames=16)
     ctx = Context(cfg)
@@ -73,11 +76,16 @@ def setup(args, precision=0, av=None, n_extra=0):
     if n_extra:  # per-frame expression values (seeded N(0, 1))
         vals = np.random.default_rng(77).standard_normal((16, n_extra)).astype(np.float32)
         ctx.set_frame_extra_dims(0, torch.from_numpy(vals).cuda())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
     for f in range(16):
         ctx.build_occupancy(f, 0.015)
+    torch.cuda.synchronize()
+    setup_ms = (time.perf_counter() - t0) * 1e3 / 16
     rng = np.random.default_rng(2024)
     cams = [make_camera(synth.random_camera(rng, 0.7), synth.focal_from_fov(args.res, 30.0)) for _ in range(16)]
     torch.cuda.synchronize()
+    ctx.occupancy_ms_per_frame = setup_ms
     return ctx, cfg, av, params, cams
 
 
@@ -87,16 +95,25 @@ def phase(name):
 
 
 def timed_renders(ctx, cams, res, steps, warmup, out):
-    """Render `steps` frames (after `warmup`), cycling the 16 posed frames;
-    returns (seconds, samples composited, samples emitted, kernel profile)."""
+    """Render `steps` frames (after `warmup`), cycling the 16 posed frames.
+    The timed pass runs with profiling off and reads nothing back between
+    frames (each insta_render still waits for its own frame's counters, as
+    the API returns a finished image).  A second, untimed pass over the same
+    frames (renders are deterministic: same camera, frame and sample index)
+    collects the sample counts and the per-kernel HIP-event breakdown.
+    Returns (seconds, samples composited, samples emitted, kernel profile)."""
     import torch
     for s in range(warmup):
         ctx.render(cams[s % 16], res, res, s % 16, s % 16, 0, out)
     torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        f = s % 16
+        ctx.render(cams[f], res, res, f, f, 0, out)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
     ctx.set_profiling(True)
     samples = emitted = rounds = rays0 = 0
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
     for s in range(steps):
         f = s % 16
         ctx.render(cams[f], res, res, f, f, 0, out)
@@ -106,7 +123,6 @@ def timed_renders(ctx, cams, res, steps, warmup, out):
         rounds += st.n_iterations
         rays0 += st.n_rays_alive0
     torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
     prof = ctx.profile()
     prof["_rounds_per_frame"] = rounds / steps
     prof["_live_rays_per_frame"] = rays0 / steps
@@ -137,26 +153,35 @@ def accuracy_vs_fp64(ctxs, cfg, av, params, cam, res, n_pix):
     return out
 
 
-def cpu_baseline(cfg, av, params, cam, res, budget_s):
-    """The oracle on the host cores: whole 512x512 frames of posed frame 0 at
-    successive sample indices (new jitter each), until about budget_s."""
+def cpu_baseline(cfg, av, params, cam, res, budget_s, runs=5):
+    """The oracle on the host cores (OpenMP over rays): one warm-up run, then
+    the median of `runs` timed runs of the same work, on a bounded pixel
+    subset of the 512x512 frame of posed frame 0 (a fixed random subset of
+    pixels, sized from the warm-up so the timed runs take about budget_s)."""
     from oracle.oracle import Oracle
     orc = Oracle(cfg)
     orc.set_canonical_mesh(av.canonical, av.faces, av.masks)
     orc.set_frame(0, av.frames[0])
     orc.set_params(params)
     orc.build_occupancy(0, 0.015)
-    samples, secs, frames = 0, 0.0, 0
-    while secs < budget_s:
+    order = np.random.default_rng(5).permutation(res * res).astype(np.uint32)
+    n_pix = min(res * res, 16384)
+    t0 = time.perf_counter()
+    orc.render(cam, res, res, 0, 0, 0, pixels=order[:n_pix])       # warm-up
+    warm = time.perf_counter() - t0
+    n_pix = int(min(res * res, max(1024, n_pix * (budget_s / runs) / max(warm, 1e-3))))
+    pix = order[:n_pix]
+    times, samples = [], 0
+    for _ in range(runs):
         t0 = time.perf_counter()
-        _, _, _, st = orc.render(cam, res, res, 0, 0, frames)
-        secs += time.perf_counter() - t0
-        samples += int(st[0])
-        frames += 1
+        samples = int(orc.render(cam, res, res, 0, 0, 0, pixels=pix)[3][0])
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return dict(value=samples / secs, unit="samples/s", cores=threads, kind="port",
-                sample=f"{frames} full {res}x{res} frames of posed frame 0 (spp 0..{frames - 1}), "
-                       f"{samples} samples in {secs:.1f} s", frames_per_s=frames / secs)
+    return dict(value=samples / med, unit="samples/s", cores=threads, kind="port",
+                sample=f"{n_pix} random pixels of a {res}x{res} frame of posed frame 0 ({samples} samples); "
+                       f"1 warm-up, median of {runs} runs ({', '.join(f'{t:.2f}' for t in times)} s)",
+                frames_per_s=(samples / med) / (samples * res * res / n_pix))
 
 
 def training_bench(args, full=False):
@@ -384,6 +409,9 @@ def main():
     ap.add_argument("--serve-budget-gib", type=int, default=24)
     ap.add_argument("--train-steps", type=int, default=40)
     ap.add_argument("--train-warmup", type=int, default=8)
+    ap.add_argument("--cell-split", type=int, default=None,
+                    help="candidate-list split of the render contexts (default: the library's)")
+    ap.add_argument("--cell-list-entries", type=int, default=0, help="list entries per frame (0: the library's)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 256 MiB device copy first (reference kernel for rocprof byte counters)")
     args = ap.parse_args()
@@ -445,10 +473,14 @@ def main():
     tc = ctx.traversal_counts()
     ctx.set_profiling(False)
     q = max(tc["queries"], 1)
-    traversal = dict(queries=tc["queries"], inner_nodes_per_sample=tc["inner"] / q,
+    traversal = dict(queries=tc["queries"], list_queries=tc["list_queries"], cell_split=cfg.cell_split,
+                     list_entries_per_sample=tc["list_entries"] / q,
+                     inner_nodes_per_sample=tc["inner"] / q,
                      child_boxes_per_sample=4 * tc["inner"] / q, leaves_per_sample=tc["leaves"] / q,
                      triangles_per_sample=tc["triangles"] / q,
-                     nodes_per_sample=(tc["inner"] + tc["leaves"]) / q)
+                     nodes_per_sample=(tc["inner"] + tc["leaves"]) / q,
+                     occupancy_and_lists_ms_per_frame=ctx.occupancy_ms_per_frame,
+                     list_info_frame0=ctx.cell_list_info(0))
 
     # precision forms: same workload, other network arithmetic
     forms = {"fp32": dict(frames_per_s=args.steps / elapsed, samples_per_s=value,

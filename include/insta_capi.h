@@ -85,6 +85,16 @@ typedef struct insta_config {
      * tiny_mesh.cu:28-32); a layout choice, the query result does not
      * depend on it.  In [1, 64]; default 4 */
     uint32_t bvh_leaf_min;
+    /* nearest-triangle candidate lists: insta_build_occupancy also splits
+     * each occupied cell of the frame's mesh occupancy grid into
+     * cell_split^3 sub-cells and keeps, per sub-cell, every triangle that can
+     * be the nearest one to a point inside it (its box within the sub-cell's
+     * largest possible nearest distance), sorted by box distance.  Queries in
+     * such a sub-cell scan the list; all others traverse the BVH.  The result
+     * is the same either way.  0 = BVH only, else 1, 2 or 4 */
+    uint32_t cell_split;
+    uint32_t cell_list_cells;        /* occupied cells per frame that get lists (cap) */
+    uint32_t cell_list_entries;      /* list entries per frame (cap; sub-cells past it use the BVH) */
 } insta_config;
 
 #define INSTA_MAX_TRIANGLES 16384u
@@ -220,6 +230,21 @@ insta_status insta_set_frame_extra_dims(insta_ctx* ctx, uint32_t first_frame, ui
  * dist_dev [n] (nullable) receives the Euclidean distance. */
 insta_status insta_closest_triangles(insta_ctx* ctx, int32_t frame, const float* points_dev,
                                      uint32_t n, int32_t* ids_dev, float* dist_dev, void* stream);
+
+/* Candidate lists of a frame (see insta_config.cell_split), as left by its
+ * last insta_build_occupancy: occupied cells, entries used, whether the lists
+ * answer queries (0 when cell_split is 0, before the first build, after the
+ * frame's mesh changed, or when the occupied cells exceeded cell_list_cells),
+ * and sub-cells that fell back to the BVH because cell_list_entries ran out.
+ * Waits for the stream.  No counterpart in the reference (its queries always
+ * walk the BVH, bvh.cuh:48-96). */
+typedef struct insta_cell_list_info {
+    uint32_t occupied_cells;
+    uint32_t entries;
+    uint32_t valid;
+    uint32_t subcells_without_list;
+} insta_cell_list_info;
+insta_status insta_get_cell_list_info(insta_ctx* ctx, int32_t frame, insta_cell_list_info* out, void* stream);
 
 /* Export a frame's BVH (frame = -1: canonical) for inspection: nodes_dev
  * [n_nodes*8] floats (min xyz, begin, max xyz, end; begin/end are int32 bit
@@ -586,8 +611,11 @@ typedef struct insta_profile {
     /* with enable == 2: nearest-triangle work of the sample deformation
      * (render and training): queries, inner nodes expanded (4 child boxes
      * each), leaves visited, exact triangle distances evaluated (those whose
-     * own box passed the bound) */
+     * own box, and on the list path also whose plane, passed the bound) */
     uint64_t bvh_queries, bvh_inner, bvh_leaves, bvh_triangles;
+    /* of bvh_queries, those answered from a candidate list, and the list
+     * entries they scanned (their exact distances are in bvh_triangles) */
+    uint64_t list_queries, list_entries;
 } insta_profile;
 /* enable != 0 starts (and zeroes) accumulation; 0 stops it.  enable == 2 also
  * counts traversal work, with a counting build of the deformation kernel

@@ -34,8 +34,14 @@ class InstaConfig(C.Structure):
         ("grid_size", C.c_uint32), ("dt", C.c_float), ("max_steps", C.c_uint32), ("min_transmittance", C.c_float),
         ("disable_interp", C.c_uint32), ("precision", C.c_uint32),
         ("max_frames", C.c_uint32), ("max_triangles", C.c_uint32), ("max_vertices", C.c_uint32),
-        ("bvh_leaf_min", C.c_uint32),
+        ("bvh_leaf_min", C.c_uint32), ("cell_split", C.c_uint32), ("cell_list_cells", C.c_uint32),
+        ("cell_list_entries", C.c_uint32),
     ]
+
+
+class InstaCellListInfo(C.Structure):
+    _fields_ = [("occupied_cells", C.c_uint32), ("entries", C.c_uint32), ("valid", C.c_uint32),
+                ("subcells_without_list", C.c_uint32)]
 
 
 class InstaCamera(C.Structure):
@@ -119,7 +125,7 @@ class InstaResult(C.Structure):
 class InstaProfile(C.Structure):
     _fields_ = [("ms", C.c_double * 8), ("launches", C.c_uint64 * 8), ("items", C.c_uint64 * 8),
                 ("bvh_queries", C.c_uint64), ("bvh_inner", C.c_uint64), ("bvh_leaves", C.c_uint64),
-                ("bvh_triangles", C.c_uint64)]
+                ("bvh_triangles", C.c_uint64), ("list_queries", C.c_uint64), ("list_entries", C.c_uint64)]
 
 
 # the C ABI: (name, restype, argtypes); tests check this list against the header
@@ -138,6 +144,7 @@ SIGNATURES = [
     ("insta_set_frame_expression", C.c_int, [_VP, C.c_uint32, _VP, C.c_uint32, _VP, _VP]),
     ("insta_set_frame_extra_dims", C.c_int, [_VP, C.c_uint32, C.c_uint32, _VP, _VP]),
     ("insta_closest_triangles", C.c_int, [_VP, C.c_int32, _VP, C.c_uint32, _VP, _VP, _VP]),
+    ("insta_get_cell_list_info", C.c_int, [_VP, C.c_int32, C.POINTER(InstaCellListInfo), _VP]),
     ("insta_export_bvh", C.c_int, [_VP, C.c_int32, _VP, _VP, C.POINTER(C.c_uint32), _VP]),
     ("insta_deform_samples", C.c_int, [_VP, _VP, C.c_uint32, C.c_uint32, C.c_int32, _VP]),
     ("insta_build_occupancy", C.c_int, [_VP, C.c_int32, C.c_float, _VP, _VP]),
@@ -235,7 +242,8 @@ def make_config(**kw) -> InstaConfig:
                 density_hidden_layers=1, density_out=16, rgb_hidden_layers=2, sh_degree=4, n_extra_dims=0,
                 density_logit_bias=1.0, grid_size=128, dt=1.73205080757 / 1024.0, max_steps=1024,
                 min_transmittance=0.01, disable_interp=0, precision=PRECISION_FP32, max_frames=16,
-                max_triangles=MAX_TRIANGLES, max_vertices=1 << 16, bvh_leaf_min=4)
+                max_triangles=MAX_TRIANGLES, max_vertices=1 << 16, bvh_leaf_min=4, cell_split=4,
+                cell_list_cells=1 << 16, cell_list_entries=1 << 25)
     base.update(kw)
     for k, v in base.items():
         setattr(cfg, k, v)
@@ -348,6 +356,12 @@ class Context:
         check(self.lib.insta_closest_triangles(self.h, frame, _ptr(pts_dev), n, _ptr(ids), _ptr(dist),
                                                self.stream()), "insta_closest_triangles")
         return ids, dist
+
+    def cell_list_info(self, frame: int) -> dict:
+        info = InstaCellListInfo()
+        check(self.lib.insta_get_cell_list_info(self.h, frame, C.byref(info), self.stream()),
+              "insta_get_cell_list_info")
+        return {name: getattr(info, name) for name, _ in InstaCellListInfo._fields_}
 
     def export_bvh(self, frame: int):
         nn = C.c_uint32()
@@ -482,7 +496,8 @@ class Context:
         """Nearest-triangle work counted since set_profiling(True, count_traversal=True)."""
         p = InstaProfile()
         check(self.lib.insta_get_profile(self.h, C.byref(p)), "insta_get_profile")
-        return dict(queries=p.bvh_queries, inner=p.bvh_inner, leaves=p.bvh_leaves, triangles=p.bvh_triangles)
+        return dict(queries=p.bvh_queries, inner=p.bvh_inner, leaves=p.bvh_leaves, triangles=p.bvh_triangles,
+                    list_queries=p.list_queries, list_entries=p.list_entries)
 
     # --- occupancy samples and density grid ---------------------------------
     def deform_positions(self, pos_dev, n_frames: int, seed: int, rng_offset: int = 0) -> None:

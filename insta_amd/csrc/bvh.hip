@@ -395,15 +395,33 @@ __global__ void blend_expression_kernel(const float* __restrict__ canon, const f
 
 constexpr uint32_t kQueryThreads = 256;
 
-__global__ __launch_bounds__(kQueryThreads) void closest_kernel(MeshView m, const float* __restrict__ pts, uint32_t n,
-                                                               int32_t* __restrict__ ids, float* __restrict__ dist) {
+// nearest triangle per point: the frame's candidate lists where they cover
+// the point (use_lists), else the BVH.  kCount: also sum the query work into
+// counts[kTravCounters] (a separate build for measurement runs)
+template <bool kCount>
+__global__ __launch_bounds__(kQueryThreads) void closest_kernel(MeshView m, CellListView cl, const float* __restrict__ pts,
+                                                               uint32_t n, int32_t* __restrict__ ids,
+                                                               float* __restrict__ dist,
+                                                               unsigned long long* __restrict__ counts) {
     extern __shared__ uint32_t stack[];  // [stack_depth(k)][kQueryThreads]
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    V3 p = v3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
-    Hit h = bvh_closest(m, p, stack + threadIdx.x, kQueryThreads);
-    ids[i] = h.idx < 0 ? -1 : (int32_t)h.orig;
-    if (dist) dist[i] = sqrtf(h.dist_sq);
+    TraversalCount tc;
+    if (i < n) {
+        V3 p = v3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+        Hit h = closest_impl<kCount>(m, cl, p, stack + threadIdx.x, kQueryThreads, &tc);
+        ids[i] = h.idx < 0 ? -1 : (int32_t)h.orig;
+        if (dist) dist[i] = sqrtf(h.dist_sq);
+    }
+    if constexpr (kCount) {
+        unsigned long long v[kTravCounters] = {i < n ? 1ull : 0ull, tc.inner, tc.leaves, tc.triangles, tc.lists,
+                                               tc.entries};
+#pragma unroll
+        for (int c = 0; c < kTravCounters; ++c) {
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) v[c] += __shfl_xor(v[c], o);
+            if ((threadIdx.x & 63u) == 0 && v[c]) atomicAdd(counts + c, v[c]);
+        }
+    }
 }
 
 // mark_flame_density_grid + grid_to_bitfield (testbed_nerf.cu:364-389, 585-609):
@@ -489,6 +507,9 @@ insta_status launch_build_slots(insta_ctx* ctx, uint32_t first_slot, uint32_t n_
         set_error("build: mesh size or slot range out of bounds");
         return INSTA_ERR_INVALID;
     }
+    // the slots' candidate lists describe the old meshes until rebuilt
+    insta_status st = launch_cell_lists_invalidate(ctx, first_slot, n_slots, stream);
+    if (st != INSTA_OK) return st;
     hipLaunchKernelGGL(build_slot_kernel, dim3(n_slots), dim3(kBuildThreads), 0, stream, a);
     INSTA_HIP(hipGetLastError());
     return INSTA_OK;
@@ -508,11 +529,17 @@ insta_status launch_blend_expression(insta_ctx* ctx, uint32_t slot, const float*
 }
 
 insta_status launch_closest(insta_ctx* ctx, uint32_t slot, const float* pts, uint32_t n, int32_t* ids, float* dist,
-                            hipStream_t stream) {
+                            bool use_lists, hipStream_t stream) {
     if (n == 0) return INSTA_OK;
     const size_t shmem = (size_t)stack_depth(ctx->bvh_k) * kQueryThreads * sizeof(uint32_t);
-    hipLaunchKernelGGL(closest_kernel, dim3((n + kQueryThreads - 1) / kQueryThreads), dim3(kQueryThreads), shmem,
-                       stream, ctx->mesh(slot), pts, n, ids, dist);
+    const CellListView cl = use_lists ? ctx->cells(slot) : CellListView{};
+    const dim3 grid((n + kQueryThreads - 1) / kQueryThreads);
+    if (ctx->d_trav_counts)
+        hipLaunchKernelGGL(closest_kernel<true>, grid, dim3(kQueryThreads), shmem, stream, ctx->mesh(slot), cl, pts, n,
+                           ids, dist, ctx->d_trav_counts);
+    else
+        hipLaunchKernelGGL(closest_kernel<false>, grid, dim3(kQueryThreads), shmem, stream, ctx->mesh(slot), cl, pts, n,
+                           ids, dist, nullptr);
     INSTA_HIP(hipGetLastError());
     return INSTA_OK;
 }

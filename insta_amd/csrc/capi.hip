@@ -95,6 +95,11 @@ insta_status validate(const insta_config& c) {
     if (c.max_vertices == 0 || c.max_vertices > (1u << 24)) return bad("max_vertices in [1, 2^24]");
     if (c.bvh_leaf_min == 0 || c.bvh_leaf_min > 64) return bad("bvh_leaf_min in [1, 64]");
     if (c.precision > INSTA_PRECISION_FP32_SCALAR) return bad("unknown precision");
+    if (!(c.cell_split == 0 || c.cell_split == 1 || c.cell_split == 2 || c.cell_split == 4))
+        return bad("cell_split must be 0, 1, 2 or 4");
+    if (c.cell_split && (c.cell_list_cells == 0 || c.cell_list_cells > c.grid_size * c.grid_size * c.grid_size ||
+                         c.cell_list_entries == 0 || c.cell_list_entries > (1u << 28)))
+        return bad("cell_list_cells in [1, grid_size^3], cell_list_entries in [1, 2^28]");
     return INSTA_OK;
 }
 
@@ -130,6 +135,32 @@ bool slot_of(const insta_ctx* ctx, int32_t frame, uint32_t& slot) {
 }
 
 }  // namespace
+
+insta_status check_render_args(const insta_ctx* ctx, const insta_camera* cam, uint32_t width, uint32_t height,
+                               int32_t frame, int32_t occupancy, const float* rgba) {
+    if (!cam || !rgba || width == 0 || height == 0 || (uint64_t)width * height > (1ull << 26)) {
+        set_error("render: bad camera, output pointer or resolution");
+        return INSTA_ERR_INVALID;
+    }
+    if (!ctx->params_set) {
+        set_error("render: parameters not set");
+        return INSTA_ERR_STATE;
+    }
+    if (!mfma_network_supported(ctx->cfg) && (!network_supported(ctx->cfg) || !scalar_precision(ctx->cfg))) {
+        set_error("render: configuration not supported by the fused MLP kernels");
+        return INSTA_ERR_UNSUPPORTED;
+    }
+    uint32_t slot;
+    if (!slot_of(ctx, frame, slot)) {
+        set_error("render: frame not built");
+        return INSTA_ERR_STATE;
+    }
+    if (occupancy < -1 || occupancy >= (int32_t)ctx->cfg.max_frames || !ctx->grid_built[occupancy + 1]) {
+        set_error("render: occupancy slot not built");
+        return INSTA_ERR_STATE;
+    }
+    return INSTA_OK;
+}
 
 void set_error(const std::string& msg) { g_error = msg; }
 
@@ -172,6 +203,9 @@ void insta_default_config(insta_config* c) {
     c->max_triangles = INSTA_MAX_TRIANGLES;
     c->max_vertices = 1u << 16;
     c->bvh_leaf_min = 4;
+    c->cell_split = 4;
+    c->cell_list_cells = 1u << 16;
+    c->cell_list_entries = 1u << 25;
 }
 
 insta_status insta_query_layout(const insta_config* cfg, insta_layout* out) {
@@ -248,6 +282,12 @@ insta_status insta_create(const insta_config* cfg, insta_ctx** out) {
     if (ok && hipMemset(ctx->d_built, 0, S) != hipSuccess) ok = false;
     if (ok && hipMemset(ctx->d_grids, 0, S * (G * G * G / 8)) != hipSuccess) ok = false;
     if (ok && hipMemset(ctx->d_grids_lin, 0, S * (G * G * G / 8)) != hipSuccess) ok = false;
+    if (ok && insta::alloc_cell_lists(ctx) != INSTA_OK) {
+        const std::string msg = insta_last_error();
+        insta_destroy(ctx);
+        set_error(msg);
+        return INSTA_ERR_OOM;
+    }
     if (!ok) {
         insta_destroy(ctx);
         set_error("device allocation failed in insta_create");
@@ -269,6 +309,7 @@ void insta_destroy(insta_ctx* ctx) {
     dev_free(ctx->d_build_lists); dev_free(ctx->d_build_q);
     dev_free(ctx->d_scratch); dev_free(ctx->d_wfrag); dev_free(ctx->d_table_h);
     dev_free(ctx->d_frame_extra); dev_free(ctx->d_extra_bias); dev_free(ctx->d_misses);
+    insta::free_cell_lists(ctx);
     delete ctx;
 }
 
@@ -426,7 +467,27 @@ insta_status insta_closest_triangles(insta_ctx* ctx, int32_t frame, const float*
         set_error("closest_triangles: null pointer");
         return INSTA_ERR_INVALID;
     }
-    return insta::launch_closest(ctx, slot, points_dev, n, ids_dev, dist_dev, (hipStream_t)stream);
+    return insta::launch_closest(ctx, slot, points_dev, n, ids_dev, dist_dev, true, (hipStream_t)stream);
+}
+
+insta_status insta_get_cell_list_info(insta_ctx* ctx, int32_t frame, insta_cell_list_info* out, void* stream) {
+    INSTA_CHECK_CTX(ctx);
+    if (!out || frame < -1 || frame >= (int32_t)ctx->cfg.max_frames) {
+        set_error("get_cell_list_info: bad frame or null pointer");
+        return INSTA_ERR_INVALID;
+    }
+    *out = insta_cell_list_info{};
+    if (!ctx->d_cl_state) return INSTA_OK;
+    uint32_t st[insta::kClState];
+    const uint32_t slot = (uint32_t)(frame + 1);
+    INSTA_HIP(hipMemcpyAsync(st, ctx->d_cl_state + insta::kClState * (size_t)slot, sizeof(st), hipMemcpyDeviceToHost,
+                             (hipStream_t)stream));
+    INSTA_HIP(hipStreamSynchronize((hipStream_t)stream));
+    out->occupied_cells = st[0];
+    out->entries = st[1];
+    out->valid = st[2];
+    out->subcells_without_list = st[4];
+    return INSTA_OK;
 }
 
 insta_status insta_export_bvh(insta_ctx* ctx, int32_t frame, float* nodes_dev, uint32_t* order_dev,
@@ -478,9 +539,7 @@ insta_status insta_build_occupancy(insta_ctx* ctx, int32_t frame, float threshol
         set_error("build_occupancy: frame not built");
         return INSTA_ERR_STATE;
     }
-    insta_status st = insta::launch_occupancy(ctx, slot, threshold, stream);
-    if (st != INSTA_OK) return st;
-    st = insta::launch_grid_linear(ctx, slot, stream);
+    insta_status st = insta::launch_mesh_occupancy(ctx, slot, threshold, stream);
     if (st != INSTA_OK) return st;
     ctx->grid_built[slot] = 1;
     const size_t G = ctx->cfg.grid_size;
@@ -568,28 +627,8 @@ insta_status insta_render(insta_ctx* ctx, const insta_camera* cam, uint32_t widt
                           int32_t occupancy, uint32_t spp, float* rgba_dev, float* depth_dev, uint32_t* steps_dev,
                           void* stream) {
     INSTA_CHECK_CTX(ctx);
-    if (!cam || !rgba_dev || width == 0 || height == 0 || (uint64_t)width * height > (1ull << 26)) {
-        set_error("render: bad camera, output pointer or resolution");
-        return INSTA_ERR_INVALID;
-    }
-    if (!ctx->params_set) {
-        set_error("render: parameters not set");
-        return INSTA_ERR_STATE;
-    }
-    if (!insta::mfma_network_supported(ctx->cfg) &&
-        (!insta::network_supported(ctx->cfg) || !insta::scalar_precision(ctx->cfg))) {
-        set_error("render: configuration not supported by the fused MLP kernels");
-        return INSTA_ERR_UNSUPPORTED;
-    }
-    uint32_t slot;
-    if (!insta::slot_of(ctx, frame, slot)) {
-        set_error("render: frame not built");
-        return INSTA_ERR_STATE;
-    }
-    if (occupancy < -1 || occupancy >= (int32_t)ctx->cfg.max_frames || !ctx->grid_built[occupancy + 1]) {
-        set_error("render: occupancy slot not built");
-        return INSTA_ERR_STATE;
-    }
+    const insta_status st = insta::check_render_args(ctx, cam, width, height, frame, occupancy, rgba_dev);
+    if (st != INSTA_OK) return st;
     return insta::render(ctx, ctx->ws, cam, width, height, frame, occupancy, spp, rgba_dev, depth_dev, steps_dev,
                          (hipStream_t)stream);
 }
@@ -729,7 +768,7 @@ insta_status insta_get_profile(insta_ctx* ctx, insta_profile* out) {
         // renders and training steps synchronise their stream before they
         // return, so the counts are final: read them on the context's own
         // non-blocking stream rather than stalling the device
-        unsigned long long c[4];
+        unsigned long long c[insta::kTravCounters];
         if (!ctx->aux_stream) INSTA_HIP(hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking));
         INSTA_HIP(hipMemcpyAsync(c, ctx->d_trav_counts, sizeof(c), hipMemcpyDeviceToHost, ctx->aux_stream));
         INSTA_HIP(hipStreamSynchronize(ctx->aux_stream));
@@ -737,6 +776,8 @@ insta_status insta_get_profile(insta_ctx* ctx, insta_profile* out) {
         out->bvh_inner = c[1];
         out->bvh_leaves = c[2];
         out->bvh_triangles = c[3];
+        out->list_queries = c[4];
+        out->list_entries = c[5];
     }
     return INSTA_OK;
 }

@@ -180,8 +180,11 @@ INSTA_DEV uint32_t pop_live(const uint32_t* stack, int& sp, uint32_t stride, flo
 
 // traversal work of one query (counting builds only)
 struct TraversalCount {
-    uint32_t inner = 0, leaves = 0, triangles = 0;
+    uint32_t inner = 0, leaves = 0, triangles = 0, lists = 0, entries = 0;
 };
+// device counters of a counting build: queries, inner nodes, leaves, exact
+// triangle distances, list queries, list entries scanned
+constexpr int kTravCounters = 6;
 
 template <bool kCount>
 INSTA_DEV Hit bvh_closest_impl(const MeshView& m, V3 p, uint32_t* stack, uint32_t stride, float max_dist_sq,
@@ -269,6 +272,149 @@ INSTA_DEV Hit bvh_closest_impl(const MeshView& m, V3 p, uint32_t* stack, uint32_
 INSTA_DEV Hit bvh_closest(const MeshView& m, V3 p, uint32_t* stack, uint32_t stride,
                           float max_dist_sq = 3.40282347e38f) {
     return bvh_closest_impl<false>(m, p, stack, stride, max_dist_sq, nullptr);
+}
+
+// ---------------------------------------------------------------------------
+// Per-cell candidate lists.  For a frame, each occupied cell of its mesh
+// occupancy grid (G^3, linear order x + G(y + G z)) is split into S^3
+// sub-cells.  A sub-cell's list holds every triangle whose box lies within U
+// of the sub-cell's box (grown by kCellEps), where U^2 bounds the squared
+// distance from any point of the box to the triangle nearest to the box
+// centre (the largest over the 8 corners: squared distance to a convex set
+// is convex), with a relative margin.  A triangle left out is therefore
+// farther from every point of the sub-cell than that triangle, and the
+// nearest triangle of a point in the sub-cell is always on its list.
+//   bits   [G^3/64] 64-bit words of the occupancy bits the lists were built on
+//   rank   [G^3/64] occupied cells before each word
+//   head   [cells * S^3] (first entry, count | seed << 16); kNoList: no list
+//   ent    entries: (box distance^2 truncated to its upper 16 bits) << 16 |
+//          triangle (BVH order); the seed (nearest at the centre) first, the
+//          rest ascending
+//   state  [kClState] occupied cells, entries used, valid (1: lists answer
+//          queries), occupied cells beyond the cap, sub-cells without a list
+// ---------------------------------------------------------------------------
+constexpr uint32_t kNoList = 0xffffffffu;
+constexpr uint32_t kClState = 8;
+constexpr float kCellEps = 1e-5f;
+
+struct CellListView {
+    const uint64_t* bits;
+    const uint32_t* rank;
+    const uint2* head;
+    const uint32_t* ent;
+    const uint32_t* state;
+    uint32_t G, S, cap;
+};
+
+// Sub-cell list of a point in [0,1]^3: false when the lists are not valid,
+// the point's cell is not occupied, or its sub-cell overflowed.
+INSTA_DEV bool cell_list_find(const CellListView& cl, V3 p, uint2& hd) {
+    if (!cl.state || cl.state[2] == 0u) return false;
+    // the cell the march's occupancy lookup gives the point (march.hpp), so
+    // every marched sample finds its cell; the point lies in that cell's
+    // box up to rounding far below kCellEps
+    const float g = (float)cl.G;
+    const float gx = ((p.x - 0.5f) * 1.0f + 0.5f) * g, gy = ((p.y - 0.5f) * 1.0f + 0.5f) * g,
+                gz = ((p.z - 0.5f) * 1.0f + 0.5f) * g;
+    const uint32_t hi = cl.G - 1u;
+    const uint32_t cx = min((uint32_t)gx, hi), cy = min((uint32_t)gy, hi), cz = min((uint32_t)gz, hi);
+    const uint32_t cell = cx + cl.G * (cy + cl.G * cz);
+    const uint64_t w = cl.bits[cell >> 6];
+    const uint32_t b = cell & 63u;
+    if (!((w >> b) & 1ull)) return false;
+    const uint32_t r = cl.rank[cell >> 6] + (uint32_t)__popcll(w & ((1ull << b) - 1ull));
+    if (r >= cl.cap) return false;
+    const float s = (float)cl.S;
+    const uint32_t sm = cl.S - 1u;
+    const uint32_t sx = min((uint32_t)((gx - (float)cx) * s), sm);
+    const uint32_t sy = min((uint32_t)((gy - (float)cy) * s), sm);
+    const uint32_t sz = min((uint32_t)((gz - (float)cz) * s), sm);
+    hd = cl.head[(size_t)r * (cl.S * cl.S * cl.S) + sx + cl.S * (sy + cl.S * sz)];
+    return hd.y != kNoList;
+}
+
+// Nearest triangle among a sub-cell's list, exact under the same
+// (dist_sq, original id) order and prune margin as bvh_closest_impl: the scan
+// stops at the first entry whose box distance exceeds the bound (entries
+// ascend), and a triangle is measured only when both its own box and its
+// plane lie within the bound (the distance to a triangle is at least the
+// distance to its plane).  The scan is latency-bound (each entry is a load of
+// the entry, then of its triangle), so entries go in batches of kListBatch:
+// the batch's entry words and triangles are loaded together, then tested in
+// order; loads past the stopping entry are wasted, never used.
+constexpr uint32_t kListBatch = 2;
+
+template <bool kCount>
+INSTA_DEV Hit list_closest_impl(const MeshView& m, const uint32_t* __restrict__ ent, uint2 hd, V3 p,
+                                TraversalCount* tc) {
+    float best = 3.40282347e38f;
+    int32_t best_idx = -1;
+    uint32_t best_orig = 0xffffffffu;
+    float bound = prune_bound(best);
+    const uint32_t n = hd.y & 0xffffu;
+    const uint32_t* e = ent + hd.x;
+    if constexpr (kCount) ++tc->lists;
+    bool done = false;
+    for (uint32_t k = 0; k < n && !done; k += kListBatch) {
+        uint32_t w[kListBatch];
+        float4 R[kListBatch][3];
+#pragma unroll
+        for (uint32_t j = 0; j < kListBatch; ++j) w[j] = e[min(k + j, n - 1u)];
+#pragma unroll
+        for (uint32_t j = 0; j < kListBatch; ++j) {
+            const uint32_t t = w[j] & 0xffffu;
+            R[j][0] = m.btris[3 * t + 0];
+            R[j][1] = m.btris[3 * t + 1];
+            R[j][2] = m.btris[3 * t + 2];
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kListBatch; ++j) {
+            if (done || k + j >= n || __uint_as_float(w[j] & 0xffff0000u) > bound) {
+                done = true;
+                continue;
+            }
+            if constexpr (kCount) ++tc->entries;
+            const float4 r0 = R[j][0], r1 = R[j][1], r2 = R[j][2];
+            const V3 va = v3(r0.x, r0.y, r0.z), vb = v3(r0.w, r1.x, r1.y), vc = v3(r1.z, r1.w, r2.x);
+            const float4 lo = make_float4(fminf(fminf(va.x, vb.x), vc.x), fminf(fminf(va.y, vb.y), vc.y),
+                                          fminf(fminf(va.z, vb.z), vc.z), 0.0f);
+            const float4 hi = make_float4(fmaxf(fmaxf(va.x, vb.x), vc.x), fmaxf(fmaxf(va.y, vb.y), vc.y),
+                                          fmaxf(fmaxf(va.z, vb.z), vc.z), 0.0f);
+            if (box_distance_sq(lo, hi, p) > bound) continue;
+            const V3 nor = cross(vb - va, va - vc);
+            const float pd = dot(nor, p - va);
+            if (pd * pd > bound * sqnorm(nor) * (1.0f + 0x1p-8f)) continue;
+            if constexpr (kCount) ++tc->triangles;
+            const float d = tri_distance_sq(va, vb, vc, p);
+            const uint32_t o = __float_as_uint(r2.y);
+            if (d < best || (d == best && o < best_orig)) {
+                best = d;
+                best_idx = (int32_t)(w[j] & 0xffffu);
+                best_orig = o;
+                bound = prune_bound(best);
+            }
+        }
+    }
+    Hit h;
+    h.idx = best_idx;
+    h.orig = best_orig;
+    h.dist_sq = best_idx < 0 ? 0.0f : best;
+    return h;
+}
+
+// Nearest triangle of a point: its sub-cell's list when there is one, else
+// the BVH (same result).
+template <bool kCount>
+INSTA_DEV Hit closest_impl(const MeshView& m, const CellListView& cl, V3 p, uint32_t* stack, uint32_t stride,
+                           TraversalCount* tc) {
+    uint2 hd;
+    const bool in_box = p.x >= 0.0f && p.x <= 1.0f && p.y >= 0.0f && p.y <= 1.0f && p.z >= 0.0f && p.z <= 1.0f;
+    if (in_box && cell_list_find(cl, p, hd)) return list_closest_impl<kCount>(m, cl.ent, hd, p, tc);
+    return bvh_closest_impl<kCount>(m, p, stack, stride, 3.40282347e38f, tc);
+}
+
+INSTA_DEV Hit closest(const MeshView& m, const CellListView& cl, V3 p, uint32_t* stack, uint32_t stride) {
+    return closest_impl<false>(m, cl, p, stack, stride, nullptr);
 }
 
 // Stack entries a query can hold: at most 3 pending siblings per inner level

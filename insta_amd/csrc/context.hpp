@@ -39,8 +39,13 @@ struct RenderWS {
     uint64_t bytes = 0;             // device bytes held
     bool external = false;          // buffers carved from caller memory (bind_render_ws)
 };
-// device bytes a workspace for n_pixels needs (with spp samples per pixel)
+// words of a render's round counters (RenderWS::d_counters)
+constexpr uint32_t kRenderCounterWords = 16;
+// device bytes a workspace for n_pixels needs (with spp samples per pixel,
+// or an explicit sample capacity per round)
 uint64_t render_ws_bytes(uint32_t n_pixels, uint32_t samples_per_pixel);
+uint64_t render_ws_bytes_cap(uint32_t n_pixels, uint64_t sample_cap);
+void bind_render_ws_cap(RenderWS& ws, char* base, uint32_t n_pixels, uint64_t sample_cap);
 void release_render_ws(RenderWS& ws);
 // point ws's buffers into [base, base + render_ws_bytes(n_pixels, spp)) (no
 // allocation; the counters must already exist)
@@ -76,6 +81,15 @@ struct insta_ctx {
     uint32_t* d_build_q = nullptr;      // [n_slots][3][max_triangles]
     float render_lo[3] = {0, 0, 0}, render_hi[3] = {1, 1, 1};
 
+    // nearest-triangle candidate lists per slot (cfg.cell_split > 0; layout
+    // in common.hpp), sized at creation from cell_list_cells / _entries
+    uint64_t* d_cl_bits = nullptr;  // [n_slots][G^3/64]
+    uint32_t* d_cl_rank = nullptr;  // [n_slots][G^3/64]
+    uint32_t* d_cl_cells = nullptr; // [n_slots][cell_list_cells] occupied cells (linear index)
+    uint2* d_cl_head = nullptr;     // [n_slots][cell_list_cells * S^3]
+    uint32_t* d_cl_ent = nullptr;   // [n_slots][cell_list_entries]
+    uint32_t* d_cl_state = nullptr; // [n_slots][kClState]
+
     // parameters
     float* d_params = nullptr;      // FP32 master copy [n_params]
     void* d_wfrag = nullptr;        // MLP weights arranged as MFMA A fragments
@@ -95,7 +109,7 @@ struct insta_ctx {
 
     // optional per-kernel-class event timing of insta_render
     bool profiling = false;
-    unsigned long long* d_trav_counts = nullptr;  // [4] while traversal counting is on
+    unsigned long long* d_trav_counts = nullptr;  // [kTravCounters] while traversal counting is on
     hipStream_t aux_stream = nullptr;             // non-blocking stream for host reads of counters
     hipEvent_t prof_ev[2 * INSTA_KERNEL_CLASSES] = {};
     bool prof_pending[INSTA_KERNEL_CLASSES] = {};
@@ -135,6 +149,21 @@ struct insta_ctx {
         const size_t G = cfg.grid_size;
         return d_grids_lin + (size_t)slot * (G * G * G / 8);
     }
+    // candidate lists of a slot (all null when cell_split is 0)
+    insta::CellListView cells(uint32_t slot) const {
+        insta::CellListView v{};
+        if (!d_cl_state) return v;
+        const size_t G = cfg.grid_size, S = cfg.cell_split, words = G * G * G / 64;
+        v.bits = d_cl_bits + slot * words;
+        v.rank = d_cl_rank + slot * words;
+        v.head = d_cl_head + slot * (size_t)cfg.cell_list_cells * S * S * S;
+        v.ent = d_cl_ent + slot * (size_t)cfg.cell_list_entries;
+        v.state = d_cl_state + insta::kClState * (size_t)slot;
+        v.G = cfg.grid_size;
+        v.S = cfg.cell_split;
+        v.cap = cfg.cell_list_cells;
+        return v;
+    }
 };
 
 namespace insta {
@@ -171,7 +200,7 @@ insta_status launch_build_slots(insta_ctx* ctx, uint32_t first_slot, uint32_t n_
 insta_status launch_blend_expression(insta_ctx* ctx, uint32_t slot, const float* bases, uint32_t n_bases,
                                      const float* coeffs, hipStream_t stream);
 insta_status launch_closest(insta_ctx* ctx, uint32_t slot, const float* pts, uint32_t n, int32_t* ids, float* dist,
-                            hipStream_t stream);
+                            bool use_lists, hipStream_t stream);
 insta_status launch_deform_aos(insta_ctx* ctx, float* coords, uint32_t n, uint32_t stride, int32_t frame,
                                hipStream_t stream);
 insta_status launch_deform_soa(insta_ctx* ctx, RenderWS& ws, int32_t frame, const uint32_t* n_dev, uint32_t n_max,
@@ -186,6 +215,14 @@ insta_status launch_deform_samples(insta_ctx* ctx, int32_t frame, const int32_t*
                                    uint32_t n_max, const float4* s_pos, const float4* r_d, int32_t* s_hit,
                                    float4* s_cpos, float4* s_cdir, uint32_t* miss_counter, hipStream_t stream);
 insta_status launch_occupancy(insta_ctx* ctx, uint32_t slot, float threshold, hipStream_t stream);
+// candidate lists (celllist.hip): device buffers, the build over a slot's
+// current linear occupancy bits, and the invalidation when its mesh changes
+insta_status alloc_cell_lists(insta_ctx* ctx);
+void free_cell_lists(insta_ctx* ctx);
+insta_status launch_cell_lists(insta_ctx* ctx, uint32_t slot, hipStream_t stream);
+insta_status launch_cell_lists_invalidate(insta_ctx* ctx, uint32_t first, uint32_t n, hipStream_t stream);
+// occupancy of the slot's posed mesh, its linear copy and its candidate lists
+insta_status launch_mesh_occupancy(insta_ctx* ctx, uint32_t slot, float threshold, hipStream_t stream);
 insta_status launch_grid_linear(insta_ctx* ctx, uint32_t slot, hipStream_t stream);
 insta_status launch_set_built(insta_ctx* ctx, uint32_t first, uint32_t n, hipStream_t stream);
 // render pipeline (render.hip) on an explicit workspace
@@ -209,6 +246,9 @@ insta_status launch_frame_buffer(const float4* rgba, const float* depth, uint32_
                                  const insta_frame_config& fc, float4* accum, float4* out, float* depth_out,
                                  hipStream_t stream);
 insta_status set_profiling(insta_ctx* ctx, int level);
+// the argument and state checks of insta_render
+insta_status check_render_args(const insta_ctx* ctx, const insta_camera* cam, uint32_t width, uint32_t height,
+                               int32_t frame, int32_t occupancy, const float* rgba);
 // training (train.hip)
 bool train_supported(const insta_config& cfg);
 void free_train(insta_ctx* ctx);

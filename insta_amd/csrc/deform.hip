@@ -25,6 +25,19 @@ struct MeshSet {
     size_t node_stride, tri_stride, cen_stride;  // float4 units per slot
     uint32_t nf, leaf_base, k, n_slots, disable_interp;
     uint32_t n_extra;          // extra dims of the density input (0: no selector)
+    CellListView cl0;          // candidate lists of slot 0 (null when off)
+    size_t cl_words, cl_heads, cl_ents;  // per-slot strides of the list arrays
+
+    INSTA_HD CellListView cells(uint32_t slot) const {
+        CellListView c = cl0;
+        if (!c.state) return c;
+        c.bits += slot * cl_words;
+        c.rank += slot * cl_words;
+        c.head += slot * cl_heads;
+        c.ent += slot * cl_ents;
+        c.state += kClState * (size_t)slot;
+        return c;
+    }
 
     INSTA_HD MeshView view(uint32_t slot) const {
         MeshView m;
@@ -58,6 +71,11 @@ MeshSet mesh_set(const insta_ctx* ctx, const uint8_t* built_dev) {
     s.n_slots = ctx->n_slots;
     s.disable_interp = ctx->cfg.disable_interp;
     s.n_extra = ctx->cfg.n_extra_dims;
+    s.cl0 = ctx->cells(0);
+    const size_t G = ctx->cfg.grid_size, S = ctx->cfg.cell_split;
+    s.cl_words = G * G * G / 64;
+    s.cl_heads = (size_t)ctx->cfg.cell_list_cells * S * S * S;
+    s.cl_ents = ctx->cfg.cell_list_entries;
     return s;
 }
 
@@ -176,7 +194,7 @@ INSTA_DEV void warp_from_hit(const MeshSet& ms, uint32_t frame, int32_t idx, V3 
 INSTA_DEV bool warp_one(const MeshSet& ms, uint32_t frame, V3 p, V3 dir_w, V3& out_p, V3& out_d, uint32_t& mask,
                         uint32_t* stack, uint32_t stride, uint32_t* misses) {
     if (!in_unit_box(p)) return false;
-    const Hit h = bvh_closest(ms.view(frame + 1), p, stack, stride);
+    const Hit h = closest(ms.view(frame + 1), ms.cells(frame + 1), p, stack, stride);
     if (h.idx < 0) {
         atomicAdd(misses, 1u);
         return false;
@@ -210,8 +228,10 @@ __global__ __launch_bounds__(kDeformThreads) void deform_aos_kernel(MeshSet ms, 
 
 // Render samples (SoA) in two kernels:
 //  hits:  nearest triangle per posed sample (s_pos) into s_hit, one sample
-//         per lane: neighbouring lanes take neighbouring samples of a ray,
-//         so a wave's traversals follow nearly the same node sequence.  Kept
+//         per lane, from the sample's sub-cell candidate list when the frame
+//         has lists (celllist.hip), else by BVH traversal.  A wave's lanes
+//         take the same step of neighbouring rays (an 8x8 pixel tile), so
+//         they mostly share a sub-cell and scan the same entries.  Kept
 //         apart from the transforms so the traversal runs at low register
 //         count, i.e. with many waves resident to cover its chain of
 //         dependent node loads.  (Seeding a query with the previous sample's
@@ -252,8 +272,8 @@ __global__ __launch_bounds__(kDeformThreads) void deform_hits_kernel(MeshSet ms,
         const int32_t f = sample_frame(frame, ray_frame, sp);
         const bool q = f >= 0 && in_unit_box(p);
         if constexpr (kCount) queries += q ? 1u : 0u;
-        const int32_t h = q ? bvh_closest_impl<kCount>(ms.view((uint32_t)f + 1), p, stack + threadIdx.x,
-                                                        kDeformThreads, 3.40282347e38f, &tc).idx
+        const int32_t h = q ? closest_impl<kCount>(ms.view((uint32_t)f + 1), ms.cells((uint32_t)f + 1), p,
+                                                   stack + threadIdx.x, kDeformThreads, &tc).idx
                             : -1;
         missed += (q && h < 0) ? 1u : 0u;
         s_hit[i] = h;
@@ -264,9 +284,9 @@ __global__ __launch_bounds__(kDeformThreads) void deform_hits_kernel(MeshSet ms,
         if ((threadIdx.x & 63u) == 0 && missed) atomicAdd(misses, missed);
     }
     if constexpr (kCount) {
-        unsigned long long v[4] = {queries, tc.inner, tc.leaves, tc.triangles};
+        unsigned long long v[kTravCounters] = {queries, tc.inner, tc.leaves, tc.triangles, tc.lists, tc.entries};
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
+        for (int c = 0; c < kTravCounters; ++c) {
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) v[c] += __shfl_xor(v[c], o);
             if ((threadIdx.x & 63u) == 0 && v[c]) atomicAdd(counts + c, v[c]);
@@ -314,7 +334,7 @@ __global__ __launch_bounds__(kDeformThreads) void deform_positions_kernel(MeshSe
     rng.advance((uint64_t)i * n_meshes);
     const uint32_t frame = (uint32_t)(rng.next_float() * (float)n_meshes);
     const V3 p = v3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
-    const int32_t h = bvh_closest(ms.view(frame + 1), p, stack + threadIdx.x, kDeformThreads).idx;
+    const int32_t h = closest(ms.view(frame + 1), ms.cells(frame + 1), p, stack + threadIdx.x, kDeformThreads).idx;
     if (h < 0) {  // src/core.cu:554-557
         atomicAdd(misses, 1u);
         return;

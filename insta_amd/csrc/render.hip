@@ -480,14 +480,24 @@ insta_status ensure_workspace(RenderWS& ws, uint32_t n_pixels) {
 constexpr uint64_t kWsAlign = 256;
 constexpr uint64_t kWsBuffers = 15;
 
-uint64_t render_ws_bytes(uint32_t n_pixels, uint32_t samples_per_pixel) {
+static_assert(kNumCounters == kRenderCounterWords, "render counter words");
+
+uint64_t render_ws_bytes_cap(uint32_t n_pixels, uint64_t sample_cap) {
     return (uint64_t)n_pixels * (3 * sizeof(float4) + 7 * sizeof(uint32_t)) +
-           ws_sample_cap(n_pixels, samples_per_pixel) * (4 * sizeof(float4) + sizeof(int32_t)) +
-           kWsBuffers * kWsAlign;
+           sample_cap * (4 * sizeof(float4) + sizeof(int32_t)) + kWsBuffers * kWsAlign;
+}
+
+uint64_t render_ws_bytes(uint32_t n_pixels, uint32_t samples_per_pixel) {
+    return render_ws_bytes_cap(n_pixels, ws_sample_cap(n_pixels, samples_per_pixel));
 }
 
 void bind_render_ws(RenderWS& ws, char* base, uint32_t n_pixels, uint32_t samples_per_pixel) {
-    const uint64_t cap = ws_sample_cap(n_pixels, samples_per_pixel);
+    bind_render_ws_cap(ws, base, n_pixels, ws_sample_cap(n_pixels, samples_per_pixel));
+    ws.samples_per_pixel = samples_per_pixel;
+}
+
+void bind_render_ws_cap(RenderWS& ws, char* base, uint32_t n_pixels, uint64_t sample_cap) {
+    const uint64_t cap = sample_cap;
     const size_t n = n_pixels;
     uint64_t off = 0;
     auto carve = [&](auto*& ptr, uint64_t bytes) {
@@ -511,7 +521,6 @@ void bind_render_ws(RenderWS& ws, char* base, uint32_t n_pixels, uint32_t sample
     carve(ws.s_hit, cap * sizeof(int32_t));
     ws.pixels = n_pixels;
     ws.samples = cap;
-    ws.samples_per_pixel = samples_per_pixel;
     ws.bytes = off;
     ws.external = true;
 }
@@ -685,8 +694,8 @@ insta_status set_profiling(insta_ctx* ctx, int level) {
         for (int c = 0; c < INSTA_KERNEL_CLASSES; ++c) ctx->prof_pending[c] = false;
     }
     if (level == 2 && !ctx->d_trav_counts) {
-        INSTA_HIP(hipMalloc(reinterpret_cast<void**>(&ctx->d_trav_counts), 4 * sizeof(unsigned long long)));
-        INSTA_HIP(hipMemset(ctx->d_trav_counts, 0, 4 * sizeof(unsigned long long)));
+        INSTA_HIP(hipMalloc(reinterpret_cast<void**>(&ctx->d_trav_counts), kTravCounters * sizeof(unsigned long long)));
+        INSTA_HIP(hipMemset(ctx->d_trav_counts, 0, kTravCounters * sizeof(unsigned long long)));
     } else if (level != 2 && ctx->d_trav_counts) {
         INSTA_HIP(hipDeviceSynchronize());  // queued counting kernels may still write
         dev_free(ctx->d_trav_counts);

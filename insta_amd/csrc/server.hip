@@ -144,9 +144,7 @@ insta_status prepare(insta_server* S, insta_server::Slot& sl, const insta_server
     if (st != INSTA_OK) return st;
     st = launch_set_built(ctx, slot, 1, stream);
     if (st != INSTA_OK) return st;
-    st = launch_occupancy(ctx, slot, S->cfg.occupancy_threshold, stream);
-    if (st != INSTA_OK) return st;
-    return launch_grid_linear(ctx, slot, stream);
+    return launch_mesh_occupancy(ctx, slot, S->cfg.occupancy_threshold, stream);
 }
 
 // Render rounds queued with each request (live counts stay on the device);

@@ -239,10 +239,12 @@ struct CfgField {
     const char* name;
     bool is_float;
     size_t offset;
-    bool shape;  // must match for parameters to load
+    bool shape;     // must match for parameters to load
+    bool optional;  // absent from older snapshots: the default then
 };
-#define INSTA_U(f, s) {#f, false, offsetof(insta_config, f), s}
-#define INSTA_F(f, s) {#f, true, offsetof(insta_config, f), s}
+#define INSTA_U(f, s) {#f, false, offsetof(insta_config, f), s, false}
+#define INSTA_F(f, s) {#f, true, offsetof(insta_config, f), s, false}
+#define INSTA_UO(f) {#f, false, offsetof(insta_config, f), false, true}
 const CfgField kFields[] = {
     INSTA_U(n_levels, true), INSTA_U(n_features_per_level, true), INSTA_U(log2_hashmap_size, true),
     INSTA_U(base_resolution, true), INSTA_F(per_level_scale, true), INSTA_U(n_neurons, true),
@@ -251,7 +253,9 @@ const CfgField kFields[] = {
     INSTA_U(grid_size, true), INSTA_F(dt, false), INSTA_U(max_steps, false), INSTA_F(min_transmittance, false),
     INSTA_U(disable_interp, false), INSTA_U(precision, false), INSTA_U(max_frames, false),
     INSTA_U(max_triangles, false), INSTA_U(max_vertices, false), INSTA_U(bvh_leaf_min, false),
+    INSTA_UO(cell_split), INSTA_UO(cell_list_cells), INSTA_UO(cell_list_entries),
 };
+#undef INSTA_UO
 #undef INSTA_U
 #undef INSTA_F
 
@@ -274,10 +278,16 @@ void write_config(MpWriter& w, const insta_config& c) {
 
 bool read_config(const MpValue& m, insta_config& c) {
     if (m.kind != MpValue::Map) return false;
+    insta_config defaults;
+    insta_default_config(&defaults);
     for (const CfgField& f : kFields) {
         const MpValue* v = m.get(f.name);
-        if (!v) return false;
         char* base = reinterpret_cast<char*>(&c) + f.offset;
+        if (!v && f.optional) {
+            std::memcpy(base, reinterpret_cast<const char*>(&defaults) + f.offset, 4);
+            continue;
+        }
+        if (!v) return false;
         if (f.is_float) {
             double d;
             if (!v->number(d)) return false;

@@ -319,15 +319,21 @@ def test_results_do_not_depend_on_the_bvh_leaf_size(head_avatar):
         assert np.array_equal(a, b)
 
 
-def test_traversal_counting_build_renders_the_same(head):
+@pytest.mark.parametrize("split", [2, 0])
+def test_traversal_counting_build_renders_the_same(head, head_avatar, split):
     """The counting build of the deformation kernel (profiling level 2) gives
-    the same image, and its counts obey the traversal's structure: one query
-    per deformed sample, at least k inner nodes (the first descent) and one
-    leaf per query, at most 15 exact triangle distances per leaf (a
-    triangle whose own box is beyond the bound is skipped) and at least one
-    per query."""
+    the same image, and its counts obey the query's structure.  BVH only
+    (cell_split 0): one query per deformed sample, at least k inner nodes (the
+    first descent) and one leaf per query, at most 15 exact triangle
+    distances per leaf (a triangle whose own box is beyond the bound is
+    skipped) and at least one per query.  With candidate lists: every query
+    takes its sub-cell's list, no node is visited, and each list query scans
+    at least its seed entry and measures at least that triangle."""
     torch = _torch()
-    ctx, _ = head
+    if split:
+        ctx, _ = head
+    else:
+        ctx, _ = _scene(full_config(max_frames=4, cell_split=0), head_avatar)
     ctx.build_occupancy(1, 0.015)
     cam = make_camera(synth.look_at_camera(np.array([0.2, -0.1, 1.0]), 0.7), synth.focal_from_fov(192, 30.0))
     a = [t.clone() for t in ctx.render(cam, 192, 192, 1, 1)]
@@ -339,11 +345,17 @@ def test_traversal_counting_build_renders_the_same(head):
     torch.cuda.synchronize()
     for x, y in zip(a, b):
         assert torch.equal(x, y)
-    k = int(np.floor(np.log(9976) / np.log(4))) - 1   # leaves of 4..15 triangles
     assert 0 < tc["queries"] <= st.n_samples_emitted
-    assert tc["inner"] >= k * tc["queries"]
-    assert tc["leaves"] >= tc["queries"]
-    assert tc["queries"] <= tc["triangles"] <= 15 * tc["leaves"]
+    if split:
+        assert tc["list_queries"] == tc["queries"]
+        assert tc["inner"] == 0 and tc["leaves"] == 0
+        assert tc["queries"] <= tc["triangles"] <= tc["list_entries"]
+    else:
+        k = int(np.floor(np.log(9976) / np.log(4))) - 1   # leaves of 4..15 triangles
+        assert tc["list_queries"] == 0 and tc["list_entries"] == 0
+        assert tc["inner"] >= k * tc["queries"]
+        assert tc["leaves"] >= tc["queries"]
+        assert tc["queries"] <= tc["triangles"] <= 15 * tc["leaves"]
     # counting is off again: counters are no longer reported
     assert ctx.traversal_counts()["queries"] == 0
 
