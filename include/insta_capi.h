@@ -343,6 +343,20 @@ insta_status insta_render(insta_ctx* ctx, const insta_camera* cam, uint32_t widt
                           int32_t frame, int32_t occupancy, uint32_t spp, float* rgba_dev,
                           float* depth_dev, uint32_t* steps_dev, void* stream);
 
+/* insta_render without allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops with a fixed memory plan, harnesses that forbid hipMalloc).
+ * Same arguments and result as insta_render; the round counters and the ray
+ * and sample buffers live in one fixed device array of the library (about
+ * 73 MiB per GPU, held from library load to process exit) instead of the
+ * context's grown workspace.  At most 65536 pixels (256 x 256) per call
+ * (INSTA_ERR_INVALID beyond); a march round holds at most 2^20 samples, so
+ * larger frames take further rounds.  Every call on a GPU shares that GPU's
+ * array: calls that overlap on one GPU race.  Calls no hipMalloc/hipFree. */
+insta_status insta_render_noalloc(insta_ctx* ctx, const insta_camera* cam, uint32_t width, uint32_t height,
+                                  int32_t frame, int32_t occupancy, uint32_t spp, float* rgba_dev,
+                                  float* depth_dev, uint32_t* steps_dev, void* stream);
+
 /* Counters of the last insta_render on this context. */
 insta_status insta_get_render_stats(insta_ctx* ctx, insta_render_stats* out);
 

@@ -24,7 +24,7 @@ CSRC = PKG / "csrc"
 BUILD = ROOT / "_build"
 LIB = PKG / "libinsta_amd.so"
 SOURCES = ["capi.hip", "bvh.hip", "deform.hip", "network.hip", "network_mfma.hip", "render.hip", "train.hip", "server.hip", "density.hip",
-           "snapshot.hip", "config.hip", "celllist.hip"]
+           "snapshot.hip", "config.hip", "celllist.hip", "noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 

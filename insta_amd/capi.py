@@ -158,6 +158,8 @@ SIGNATURES = [
     ("insta_network_density", C.c_int, [_VP, _VP, C.c_uint32, _VP, _VP]),
     ("insta_render", C.c_int, [_VP, C.POINTER(InstaCamera), C.c_uint32, C.c_uint32, C.c_int32, C.c_int32,
                                C.c_uint32, _VP, _VP, _VP, _VP]),
+    ("insta_render_noalloc", C.c_int, [_VP, C.POINTER(InstaCamera), C.c_uint32, C.c_uint32, C.c_int32, C.c_int32,
+                                       C.c_uint32, _VP, _VP, _VP, _VP]),
     ("insta_get_render_stats", C.c_int, [_VP, C.POINTER(InstaRenderStats)]),
     ("insta_get_deform_misses", C.c_int, [_VP, C.POINTER(C.c_uint64), C.c_int, _VP]),
     ("insta_default_frame_config", None, [C.POINTER(InstaFrameConfig)]),
@@ -463,6 +465,20 @@ class Context:
         rgba, depth, steps = out
         check(self.lib.insta_render(self.h, C.byref(cam), width, height, frame, occupancy, spp, _ptr(rgba),
                                     _ptr(depth), _ptr(steps), self.stream()), "insta_render")
+        return out
+
+    def render_noalloc(self, cam: InstaCamera, width: int, height: int, frame: int, occupancy: int, spp: int = 0,
+                       out=None):
+        """insta_render_noalloc: the render on the library's fixed device
+        array (no allocation in the call; at most 256 x 256 pixels)."""
+        torch = self.torch
+        if out is None:
+            out = (torch.empty((height * width, 4), dtype=torch.float32, device="cuda"),
+                   torch.empty(height * width, dtype=torch.float32, device="cuda"),
+                   torch.empty(height * width, dtype=torch.int32, device="cuda"))
+        rgba, depth, steps = out
+        check(self.lib.insta_render_noalloc(self.h, C.byref(cam), width, height, frame, occupancy, spp, _ptr(rgba),
+                                            _ptr(depth), _ptr(steps), self.stream()), "insta_render_noalloc")
         return out
 
     def render_frame(self, cam: InstaCamera, width: int, height: int, frame: int, occupancy: int, spp: int,

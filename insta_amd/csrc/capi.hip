@@ -282,6 +282,12 @@ insta_status insta_create(const insta_config* cfg, insta_ctx** out) {
     if (ok && hipMemset(ctx->d_built, 0, S) != hipSuccess) ok = false;
     if (ok && hipMemset(ctx->d_grids, 0, S * (G * G * G / 8)) != hipSuccess) ok = false;
     if (ok && hipMemset(ctx->d_grids_lin, 0, S * (G * G * G / 8)) != hipSuccess) ok = false;
+    if (ok && insta::noalloc_prepare_device(ctx->device) != INSTA_OK) {
+        const std::string msg = insta_last_error();
+        insta_destroy(ctx);
+        set_error(msg);
+        return INSTA_ERR_HIP;
+    }
     if (ok && insta::alloc_cell_lists(ctx) != INSTA_OK) {
         const std::string msg = insta_last_error();
         insta_destroy(ctx);

@@ -246,6 +246,8 @@ insta_status launch_frame_buffer(const float4* rgba, const float* depth, uint32_
                                  const insta_frame_config& fc, float4* accum, float4* out, float* depth_out,
                                  hipStream_t stream);
 insta_status set_profiling(insta_ctx* ctx, int level);
+// resolve this device's fixed render scratch (noalloc.hip); called at set-up
+insta_status noalloc_prepare_device(int device);
 // the argument and state checks of insta_render
 insta_status check_render_args(const insta_ctx* ctx, const insta_camera* cam, uint32_t width, uint32_t height,
                                int32_t frame, int32_t occupancy, const float* rgba);
