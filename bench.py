@@ -184,6 +184,91 @@ def cpu_baseline(cfg, av, params, cam, res, budget_s, runs=5):
                 frames_per_s=(samples / med) / (samples * res * res / n_pix))
 
 
+def precision_bench(args, n_views=8, res=64, launch=1 << 20, reps=20):
+    """Precision form at the small shapes (2,048-triangle sphere, L=8 F=2
+    T=2^14, 64^3 grid, 64x64 images, <= 256 steps of 1/256) with
+    trained-scale hash tables (U(-0.5, 0.5); Xavier MLPs), so that the colour
+    depends on the encoding and the arithmetic shows: per-pixel RGB error of
+    each network arithmetic against the FP64 oracle render over `n_views`
+    random cameras (max, RMS, PSNR), and the network alone on batches of
+    2^20 samples per launch (HIP events on the library's stream): time,
+    achieved TFLOP/s on the algorithmic FLOP and its fraction of the peak of
+    the units it runs on (MFMA FP32 157.3 TFLOP/s, FP16/BF16 2.5 PFLOP/s;
+    the per-sample FP32 path runs on the vector ALUs, same FP32 peak)."""
+    import torch
+    from insta_amd import synth
+    from insta_amd.capi import PRECISION_NAMES, Context, make_camera, small_config
+    from oracle.oracle import Oracle, render_fp64
+    av = synth.small_sphere_avatar(seed=0, n_frames=1)
+    base = small_config(max_frames=1)
+    orc = Oracle(base)
+    lay = orc.layout
+    params = synth.network_params(7, lay, base)
+    params[lay.n_mlp_params:] = np.random.default_rng(31).uniform(-0.5, 0.5, lay.n_grid_params).astype(np.float32)
+    orc.set_canonical_mesh(av.canonical, av.faces, av.masks)
+    orc.set_frame(0, av.frames[0])
+    orc.set_params(params)
+    orc.build_occupancy(0, 0.05)
+    rng = np.random.default_rng(32)
+    cams = [make_camera(synth.random_camera(rng, 1.0, front_bias=False), synth.focal_from_fov(res, 30.0))
+            for _ in range(n_views)]
+    refs = [render_fp64(orc, cam, res, res, 0, 0, 0)[:, :3] for cam in cams]
+    # 2^20 network inputs: canonical positions near the surface, random directions
+    tri = av.canonical[av.faces].astype(np.float64)
+    k = rng.integers(0, len(av.faces), launch)
+    pos = np.einsum("nk,nkd->nd", rng.dirichlet([1, 1, 1], launch), tri[k]) + 0.03 * rng.standard_normal((launch, 3))
+    d = rng.standard_normal((launch, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    coords = np.zeros((launch, 11), np.float32)
+    coords[:, 0:3] = np.clip(pos, 0.0, 1.0)
+    coords[:, 8:11] = (d + 1.0) * 0.5
+    coords_dev = torch.from_numpy(coords).cuda()
+    flops = per_sample_costs(base, lay)["network"]["flops"]
+    out = {"workload": f"small shapes, {n_views} random {res}x{res} views, hash tables U(-0.5, 0.5), "
+                       f"network batches of {launch} samples"}
+    for prec in (0, 3, 1, 2):
+        name = PRECISION_NAMES[prec]
+        cfg = small_config(max_frames=1, precision=prec)
+        ctx = Context(cfg)
+        ctx.set_canonical_mesh(av.canonical, av.faces, av.masks)
+        ctx.set_frames(0, torch.from_numpy(av.frames[0][None]).cuda())
+        ctx.set_params(torch.from_numpy(params).cuda())
+        ctx.build_occupancy(0, 0.05)
+        errs, imgs = [], []
+        for cam, ref in zip(cams, refs):
+            rgba, _, _ = ctx.render(cam, res, res, 0, 0, 0)
+            img = rgba.cpu().numpy()[:, :3].astype(np.float64)
+            imgs.append(img)
+            errs.append(img - ref)
+        e = np.concatenate(errs)
+        mse = float(np.mean(e ** 2))
+        if prec == 0:
+            fp32_imgs = imgs
+        e32 = np.concatenate([a - b for a, b in zip(imgs, fp32_imgs)])
+        mse32 = float(np.mean(e32 ** 2))
+        stream = torch.cuda.current_stream()
+        for _ in range(3):
+            ctx.network_inference(coords_dev)
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        for _ in range(reps):
+            ctx.network_inference(coords_dev)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        ms = ev0.elapsed_time(ev1) / reps
+        peak = 2500.0 if prec in (1, 2) else FP32_PEAK_TFLOPS
+        tflops = flops * launch / (ms * 1e-3) / 1e12
+        out[name] = dict(max_abs=float(np.abs(e).max()), rms=math.sqrt(mse),
+                         psnr_db=(float("inf") if mse == 0 else 10 * math.log10(1.0 / mse)),
+                         vs_fp32_render=dict(max_abs=float(np.abs(e32).max()), rms=math.sqrt(mse32),
+                                             psnr_db=(float("inf") if mse32 == 0 else 10 * math.log10(1.0 / mse32))),
+                         network_ms_per_launch=ms, samples_per_s=launch / (ms * 1e-3), tflops=tflops,
+                         peak_tflops=peak, utilisation=tflops / peak,
+                         units="matrix cores" if prec != 3 else "vector ALUs")
+        del ctx
+    return out
+
+
 def training_bench(args, full=False):
     """Training step at the small shapes (L=8, T=2^14, 64^3 grid, 2,048-triangle
     sphere), or with full=True at the full-size shapes (L=16, T=2^19, 128^3
@@ -224,13 +309,21 @@ def training_bench(args, full=False):
         return ctx, params
 
     out = {}
+    target = 1 << 18
     for name, kw in (("fp32_grid_grads", {}), ("fp16_grid_grads", dict(grid_grad_fp16=1, loss_scale=128.0))):
         ctx, _ = make_ctx()
-        tc = default_train_config(seed=3, **kw)
+        # marched samples per step are not capped below what 2^18 composited
+        # samples need (rays stop within a few samples once trained)
+        tc = default_train_config(seed=3, max_samples=1 << 24, target_samples=target, **kw)
         first = None
-        for _ in range(args.train_warmup):
+        # warm up until the adaptive ray count holds the batch at the target
+        # (three steps in a row within 2 %), at least train_warmup steps
+        full, warm = 0, 0
+        while warm < args.train_warmup or (full < 3 and warm < args.train_warmup_max):
             st = ctx.train_step(tc)
             first = st.loss if first is None else first
+            full = full + 1 if min(st.n_samples_used, target) >= 0.98 * target else 0
+            warm += 1
         torch.cuda.synchronize()
         samples = rays = batch = 0
         t0 = time.perf_counter()
@@ -244,7 +337,7 @@ def training_bench(args, full=False):
         k = args.train_steps
         out[name] = dict(steps_per_s=k / dt, ms_per_step=dt / k * 1e3, batch_samples_per_step=batch / k,
                          marched_samples_per_step=samples / k, rays_per_step=rays / k,
-                         batch_samples_per_s=batch / dt, loss_first=first, loss_last=st.loss)
+                         batch_samples_per_s=batch / dt, loss_first=first, loss_last=st.loss, warmup_steps=warm)
         del ctx
 
     # gradient error of one batch against the FP64 oracle (same rays and samples;
@@ -276,7 +369,9 @@ def training_bench(args, full=False):
         out["grad_rel_l2_vs_fp64"][name] = {n: rel(g[b0:b1], g64[b0:b1]) for n, b0, b1 in zip(names, b[:-1], b[1:])}
     shapes = ("full-size shapes (L16 T2^19, 128^3 grid, 9,976-tri head, <=1024 steps)" if full
               else "small shapes (L8 T2^14, 64^3 grid, 2,048-tri sphere)")
-    out["workload"] = f"{shapes}, 64 posed frames x 512x512 random cameras, ~2^18 samples/step, Adam(0.9, 0.99, lr 1e-2)"
+    out["workload"] = (f"{shapes}, 64 posed frames x 512x512 random cameras, a batch of up to 2^18 composited "
+                       "samples per step (adaptive ray count, at most 2^18 rays; timed after it holds), "
+                       "Adam(0.9, 0.99, lr 1e-2)")
     return out
 
 
@@ -409,6 +504,8 @@ def main():
     ap.add_argument("--serve-budget-gib", type=int, default=24)
     ap.add_argument("--train-steps", type=int, default=40)
     ap.add_argument("--train-warmup", type=int, default=8)
+    ap.add_argument("--train-warmup-max", type=int, default=120,
+                    help="warm-up steps allowed for the batch to reach 2^18 samples")
     ap.add_argument("--cell-split", type=int, default=None,
                     help="candidate-list split of the render contexts (default: the library's)")
     ap.add_argument("--cell-list-entries", type=int, default=0, help="list entries per frame (0: the library's)")
@@ -506,6 +603,8 @@ def main():
         if args.accuracy_pixels > 0:
             forms["accuracy_vs_fp64"] = accuracy_vs_fp64(ctxs, cfg, av, params, cams[0], res, args.accuracy_pixels)
 
+    if not args.skip_forms:
+        forms["small_shapes"] = precision_bench(args)
     phase("render and precision forms")
     train = None if args.skip_train else training_bench(args)
     phase("training (small shapes)")

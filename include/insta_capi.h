@@ -332,6 +332,21 @@ insta_status insta_network_backward(insta_ctx* ctx, const float* coords_dev, uin
 insta_status insta_network_density(insta_ctx* ctx, const float* positions_dev, uint32_t n, float* out_dev,
                                    void* stream);
 
+/* Workspace reservation.  insta_render, insta_render_frame, insta_train_step,
+ * insta_train_loop and the network plugin entry points never allocate: their
+ * buffers are sized here, ahead of the calls (the reference sizes its
+ * scratch arenas per call, testbed_nerf.cu:2246-2271, 3176-3202).  Each field
+ * grows its workspace to at least that size (0 leaves it as it is; nothing
+ * shrinks).  A call that needs more than the reservation fails with
+ * INSTA_ERR_STATE and names this entry point. */
+typedef struct insta_reserve_config {
+    uint32_t render_pixels;   /* width * height of the largest render */
+    uint32_t train_rays;      /* rays per training step (1 << 18 covers the adaptive count) */
+    uint32_t train_samples;   /* marched samples per step (insta_train_config.max_samples) */
+    uint32_t train_batch;     /* composited batch (target_samples); also the network plugin's batch */
+} insta_reserve_config;
+insta_status insta_reserve(insta_ctx* ctx, const insta_reserve_config* r, void* stream);
+
 /* Testbed::render_frame -> render_nerf -> NerfTracer::{init_rays_from_camera,
  * trace} with the render_deformed hook (testbed_nerf.cu:2030-2244,
  * 2320-2461; core.cu:346-348): renders `frame` through occupancy slot

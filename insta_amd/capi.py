@@ -44,6 +44,11 @@ class InstaCellListInfo(C.Structure):
                 ("subcells_without_list", C.c_uint32)]
 
 
+class InstaReserveConfig(C.Structure):
+    _fields_ = [("render_pixels", C.c_uint32), ("train_rays", C.c_uint32), ("train_samples", C.c_uint32),
+                ("train_batch", C.c_uint32)]
+
+
 class InstaCamera(C.Structure):
     _fields_ = [("matrix", C.c_float * 12), ("focal_px", C.c_float * 2), ("screen_center", C.c_float * 2),
                 ("snap_to_pixel_centers", C.c_uint32)]
@@ -138,6 +143,7 @@ SIGNATURES = [
     ("insta_default_run_config", None, [C.POINTER(InstaRunConfig)]),
     ("insta_load_config_json", C.c_int, [C.c_char_p, C.POINTER(InstaConfig), C.POINTER(InstaRunConfig)]),
     ("insta_create", C.c_int, [C.POINTER(InstaConfig), C.POINTER(_VP)]),
+    ("insta_reserve", C.c_int, [_VP, C.POINTER(InstaReserveConfig), _VP]),
     ("insta_destroy", None, [_VP]),
     ("insta_set_canonical_mesh", C.c_int, [_VP, _VP, C.c_uint32, _VP, C.c_uint32, _VP, _VP]),
     ("insta_set_frames", C.c_int, [_VP, C.c_uint32, C.c_uint32, _VP, _VP]),
@@ -305,6 +311,22 @@ class Context:
         check(self.lib.insta_create(C.byref(cfg), C.byref(h)), "insta_create")
         self.h = h
         self.nv = self.nf = 0
+        self.reserved = dict(render_pixels=0, train_rays=0, train_samples=0, train_batch=0)
+
+    def reserve(self, **kw) -> None:
+        """insta_reserve: grow the named workspaces (render_pixels,
+        train_rays, train_samples, train_batch) to at least the given sizes."""
+        r = InstaReserveConfig(**kw)
+        check(self.lib.insta_reserve(self.h, C.byref(r), self.stream()), "insta_reserve")
+        for k, v in kw.items():
+            self.reserved[k] = max(self.reserved[k], v)
+
+    def _reserve_for(self, **need) -> None:
+        """Reserve ahead of a call whatever it needs beyond the reservation so
+        far (the calls themselves never allocate)."""
+        grow = {k: v for k, v in need.items() if v > self.reserved[k]}
+        if grow:
+            self.reserve(**grow)
 
     def close(self) -> None:
         if self.h:
@@ -430,6 +452,7 @@ class Context:
     # --- network plugin (NerfNetwork forward / backward / density) ----------
     def network_forward(self, coords_dev):
         n, stride = coords_dev.shape
+        self._reserve_for(train_samples=n, train_batch=n)
         out = self.torch.empty((n, 4), dtype=self.torch.float32, device=coords_dev.device)
         check(self.lib.insta_network_forward(self.h, _ptr(coords_dev), n, stride, _ptr(out), self.stream()),
               "insta_network_forward")
@@ -463,6 +486,7 @@ class Context:
                    torch.empty(height * width, dtype=torch.float32, device="cuda"),
                    torch.empty(height * width, dtype=torch.int32, device="cuda"))
         rgba, depth, steps = out
+        self._reserve_for(render_pixels=width * height)
         check(self.lib.insta_render(self.h, C.byref(cam), width, height, frame, occupancy, spp, _ptr(rgba),
                                     _ptr(depth), _ptr(steps), self.stream()), "insta_render")
         return out
@@ -488,6 +512,7 @@ class Context:
         torch = self.torch
         if out is None:
             out = torch.empty((height * width, 4), dtype=torch.float32, device="cuda")
+        self._reserve_for(render_pixels=width * height)
         check(self.lib.insta_render_frame(self.h, C.byref(cam), width, height, frame, occupancy, spp, C.byref(fc),
                                           _ptr(accum), _ptr(out), _ptr(depth), self.stream()), "insta_render_frame")
         return out
@@ -543,7 +568,13 @@ class Context:
         check(self.lib.insta_set_training_set(self.h, n, w, h, _ptr(images_dev.contiguous()), C.addressof(arr),
                                               fr.ctypes.data, self.stream()), "insta_set_training_set")
 
+    def _reserve_training(self, tc: "InstaTrainConfig") -> None:
+        # the adaptive ray count stays below 2^18 rays
+        self._reserve_for(train_rays=tc.n_rays or (1 << 18), train_samples=tc.max_samples,
+                          train_batch=tc.target_samples)
+
     def train_step(self, tc: "InstaTrainConfig") -> InstaTrainStats:
+        self._reserve_training(tc)
         st = InstaTrainStats()
         check(self.lib.insta_train_step(self.h, C.byref(tc), C.byref(st), self.stream()), "insta_train_step")
         return st
@@ -552,6 +583,7 @@ class Context:
                    decay: float = 0.95, seed: int = 1337) -> InstaTrainStats:
         """n_steps of the reference's loop: density-grid EMA on its schedule into
         density_slot, which the rays march."""
+        self._reserve_training(tc)
         st = InstaTrainStats()
         dg = InstaDensityGridConfig(decay, 0, 0, seed)
         check(self.lib.insta_train_loop(self.h, C.byref(tc), n_steps, C.byref(dg), density_slot, n_frames, C.byref(st),

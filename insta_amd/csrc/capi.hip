@@ -844,6 +844,25 @@ insta_status insta_set_training_set(insta_ctx* ctx, uint32_t n_images, uint32_t 
     return insta::set_training_set(ctx, n_images, width, height, rgba_dev, cameras, frames, (hipStream_t)stream);
 }
 
+insta_status insta_reserve(insta_ctx* ctx, const insta_reserve_config* r, void* stream) {
+    INSTA_CHECK_CTX(ctx);
+    if (!r || (uint64_t)r->render_pixels > (1ull << 26) || r->train_rays > (1u << 24) ||
+        r->train_samples > (1u << 26) || r->train_batch > (1u << 24)) {
+        set_error("reserve: null pointer or size beyond the limits of render and train_step");
+        return INSTA_ERR_INVALID;
+    }
+    if (r->render_pixels) {
+        const insta_status st = insta::reserve_render_ws(ctx->ws, r->render_pixels);
+        if (st != INSTA_OK) return st;
+    }
+    if (r->train_rays || r->train_samples || r->train_batch) {
+        const insta_status st =
+            insta::reserve_training(ctx, r->train_rays, r->train_samples, r->train_batch, (hipStream_t)stream);
+        if (st != INSTA_OK) return st;
+    }
+    return INSTA_OK;
+}
+
 insta_status insta_train_step(insta_ctx* ctx, const insta_train_config* tc, insta_train_stats* stats, void* stream) {
     INSTA_CHECK_CTX(ctx);
     if (!tc || tc->max_samples == 0 || tc->max_samples > (1u << 26) || tc->n_rays > (1u << 24) ||

@@ -47,6 +47,8 @@ uint64_t render_ws_bytes(uint32_t n_pixels, uint32_t samples_per_pixel);
 uint64_t render_ws_bytes_cap(uint32_t n_pixels, uint64_t sample_cap);
 void bind_render_ws_cap(RenderWS& ws, char* base, uint32_t n_pixels, uint64_t sample_cap);
 void release_render_ws(RenderWS& ws);
+// allocate ws (and its counters) for n_pixels (insta_reserve)
+insta_status reserve_render_ws(RenderWS& ws, uint32_t n_pixels);
 // point ws's buffers into [base, base + render_ws_bytes(n_pixels, spp)) (no
 // allocation; the counters must already exist)
 void bind_render_ws(RenderWS& ws, char* base, uint32_t n_pixels, uint32_t samples_per_pixel);
@@ -257,6 +259,9 @@ void free_train(insta_ctx* ctx);
 insta_status set_training_set(insta_ctx* ctx, uint32_t n_images, uint32_t width, uint32_t height, const float* rgba,
                               const insta_camera* cams, const int32_t* frames, hipStream_t stream);
 insta_status train_step(insta_ctx* ctx, const insta_train_config& tc, insta_train_stats* stats, hipStream_t stream);
+// training state and workspace for up to `rays` rays, `samples` marched
+// samples and a compacted batch of `batch` (insta_reserve)
+insta_status reserve_training(insta_ctx* ctx, uint32_t rays, uint32_t samples, uint32_t batch, hipStream_t stream);
 bool train_has_gradients(const insta_ctx* ctx);
 // density grid (density.hip)
 void free_density_grid(insta_ctx* ctx);

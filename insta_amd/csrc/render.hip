@@ -430,7 +430,18 @@ uint64_t ws_sample_cap(uint32_t n_pixels, uint32_t spp) {
     return cap;
 }
 
+// a render's workspace must already hold n_pixels (insta_reserve, or a
+// binding onto caller memory): nothing is allocated per render
 insta_status ensure_workspace(RenderWS& ws, uint32_t n_pixels) {
+    if (ws.d_counters && n_pixels <= ws.pixels) return INSTA_OK;
+    set_error("render: the workspace holds " + std::to_string(ws.pixels) + " pixels, the frame needs " +
+              std::to_string(n_pixels) + ": reserve them with insta_reserve (render_pixels)");
+    return INSTA_ERR_STATE;
+}
+
+}  // namespace
+
+insta_status reserve_render_ws(RenderWS& ws, uint32_t n_pixels) {
     if (!ws.d_counters) {
         if (hipMalloc(reinterpret_cast<void**>(&ws.d_counters), kNumCounters * sizeof(uint32_t)) != hipSuccess ||
             hipHostMalloc(reinterpret_cast<void**>(&ws.h_counters), kNumCounters * sizeof(uint32_t),
@@ -474,8 +485,6 @@ insta_status ensure_workspace(RenderWS& ws, uint32_t n_pixels) {
     ws.bytes = render_ws_bytes(n_pixels, ws.samples_per_pixel);
     return INSTA_OK;
 }
-
-}  // namespace
 
 constexpr uint64_t kWsAlign = 256;
 constexpr uint64_t kWsBuffers = 15;

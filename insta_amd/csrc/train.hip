@@ -853,8 +853,10 @@ void free_workspace_train(TrainState* ts) {
 
 // rays, marched samples and the compacted batch grow independently to the
 // largest request seen
-insta_status ensure_train_workspace(insta_ctx* ctx, TrainState* ts, uint32_t n_rays, uint32_t n_samples,
-                                    uint32_t n_compact) {
+// grow the training workspace to hold n_rays rays, n_samples marched samples
+// and a compacted batch of n_compact (insta_reserve only)
+insta_status grow_train_workspace(insta_ctx* ctx, TrainState* ts, uint32_t n_rays, uint32_t n_samples,
+                                  uint32_t n_compact) {
     if (n_rays <= ts->ray_cap && n_samples <= ts->sample_cap && n_compact <= ts->compact_cap) return INSTA_OK;
     const uint32_t rc = std::max(n_rays, ts->ray_cap), sc = std::max(n_samples, ts->sample_cap);
     const uint32_t cc = std::max(n_compact, ts->compact_cap);
@@ -882,6 +884,17 @@ insta_status ensure_train_workspace(insta_ctx* ctx, TrainState* ts, uint32_t n_r
     ts->sample_cap = sc;
     ts->compact_cap = cc;
     return INSTA_OK;
+}
+
+// a step or plugin call must fit the reserved workspace: nothing is allocated per call
+insta_status check_train_workspace(const TrainState* ts, uint32_t n_rays, uint32_t n_samples, uint32_t n_compact,
+                                   const char* who) {
+    if (n_rays <= ts->ray_cap && n_samples <= ts->sample_cap && n_compact <= ts->compact_cap) return INSTA_OK;
+    set_error(std::string(who) + ": needs " + std::to_string(n_rays) + " rays, " + std::to_string(n_samples) +
+              " samples, a batch of " + std::to_string(n_compact) + "; reserved " + std::to_string(ts->ray_cap) +
+              " / " + std::to_string(ts->sample_cap) + " / " + std::to_string(ts->compact_cap) +
+              ": reserve them with insta_reserve (train_rays, train_samples, train_batch)");
+    return INSTA_ERR_STATE;
 }
 
 template <int L, int X>
@@ -1171,10 +1184,11 @@ insta_status train_step(insta_ctx* ctx, const insta_train_config& tc, insta_trai
     TrainState* ts = ctx->train;
     ts->net_n = 0;  // the step reuses the network plugin's activation buffers
     const uint32_t n_rays = tc.n_rays ? tc.n_rays : ts->rays_per_batch;
-    insta_status st = ensure_train_workspace(ctx, ts, n_rays, tc.max_samples, tc.target_samples);
+    insta_status st = check_train_workspace(ts, n_rays, tc.max_samples, tc.target_samples, "train_step");
     if (st != INSTA_OK) return st;
     const uint64_t np = ctx->layout.n_params, n_mlp = ctx->layout.n_mlp_params;
-    const uint32_t cap = ts->sample_cap;
+    // the marched-sample cap is the step's (a larger reservation changes nothing)
+    const uint32_t cap = tc.max_samples;
     // zero gradients with a fill kernel (capturable, unlike memset nodes)
     hipLaunchKernelGGL(fill_zero_kernel, dim3(1024), dim3(256), 0, stream, reinterpret_cast<uint32_t*>(ts->grad), np);
     if (tc.grid_grad_fp16)
@@ -1359,11 +1373,27 @@ insta_status reset_optimizer(insta_ctx* ctx, hipStream_t stream) {
 
 // ---- NerfNetwork plugin entry points --------------------------------------
 namespace {
-insta_status ensure_net_workspace(insta_ctx* ctx, uint32_t n, hipStream_t stream) {
+insta_status ensure_net_workspace(insta_ctx* ctx, uint32_t n, hipStream_t) {
+    if (!ctx->train) {
+        set_error("network plugin: no workspace: reserve one with insta_reserve (train_samples, train_batch)");
+        return INSTA_ERR_STATE;
+    }
+    TrainState* ts = ctx->train;
+    insta_status st = check_train_workspace(ts, 0, n, n, "network plugin");
+    if (st != INSTA_OK) return st;
+    if (ctx->cfg.n_extra_dims && ts->net_extra_cap < n) {
+        set_error("network plugin: extra-dims workspace smaller than the batch: reserve with insta_reserve");
+        return INSTA_ERR_STATE;
+    }
+    return INSTA_OK;
+}
+}  // namespace
+
+insta_status reserve_training(insta_ctx* ctx, uint32_t rays, uint32_t samples, uint32_t batch, hipStream_t stream) {
     insta_status st = ensure_train_state(ctx, stream);
     if (st != INSTA_OK) return st;
     TrainState* ts = ctx->train;
-    st = ensure_train_workspace(ctx, ts, 0, n, n);
+    st = grow_train_workspace(ctx, ts, rays, samples, batch);
     if (st != INSTA_OK) return st;
     const uint32_t X = ctx->cfg.n_extra_dims;
     if (X && ts->net_extra_cap < ts->compact_cap) {
@@ -1378,7 +1408,6 @@ insta_status ensure_net_workspace(insta_ctx* ctx, uint32_t n, hipStream_t stream
     }
     return INSTA_OK;
 }
-}  // namespace
 
 insta_status network_forward(insta_ctx* ctx, const float* coords, uint32_t n, uint32_t stride, float* out,
                              hipStream_t stream) {
