@@ -325,18 +325,20 @@ def training_bench(args, full=False):
             full = full + 1 if min(st.n_samples_used, target) >= 0.98 * target else 0
             warm += 1
         torch.cuda.synchronize()
-        samples = rays = batch = 0
+        samples = rays = batch = evaluated = 0
         t0 = time.perf_counter()
         for _ in range(args.train_steps):
             st = ctx.train_step(tc)
             samples += st.n_samples
+            evaluated += st.n_samples_evaluated
             rays += st.n_rays
             batch += min(st.n_samples_used, tc.target_samples)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         k = args.train_steps
         out[name] = dict(steps_per_s=k / dt, ms_per_step=dt / k * 1e3, batch_samples_per_step=batch / k,
-                         marched_samples_per_step=samples / k, rays_per_step=rays / k,
+                         marched_samples_per_step=samples / k, evaluated_samples_per_step=evaluated / k,
+                         rays_per_step=rays / k,
                          batch_samples_per_s=batch / dt, loss_first=first, loss_last=st.loss, warmup_steps=warm)
         del ctx
 
@@ -503,6 +505,7 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=6)
     ap.add_argument("--serve-budget-gib", type=int, default=24)
     ap.add_argument("--train-steps", type=int, default=40)
+    ap.add_argument("--train-sizes", choices=["both", "small", "full"], default="both")
     ap.add_argument("--train-warmup", type=int, default=8)
     ap.add_argument("--train-warmup-max", type=int, default=120,
                     help="warm-up steps allowed for the batch to reach 2^18 samples")
@@ -606,9 +609,9 @@ def main():
     if not args.skip_forms:
         forms["small_shapes"] = precision_bench(args)
     phase("render and precision forms")
-    train = None if args.skip_train else training_bench(args)
+    train = None if args.skip_train or args.train_sizes == "full" else training_bench(args)
     phase("training (small shapes)")
-    train_full = None if args.skip_train else training_bench(args, full=True)
+    train_full = None if args.skip_train or args.train_sizes == "small" else training_bench(args, full=True)
     phase("training (full size)")
     serve = None if args.skip_serve else serving_bench(args)
     phase("serving (small shapes)")

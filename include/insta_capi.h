@@ -459,13 +459,18 @@ typedef struct insta_train_config {
 typedef struct insta_train_stats {
     uint32_t n_rays;          /* rays generated */
     uint32_t n_rays_used;     /* rays that collected samples (and fit the cap) */
-    uint32_t n_samples;       /* samples marched, deformed and evaluated */
+    uint32_t n_samples;       /* samples marched */
     uint32_t n_samples_used;  /* samples composited before transmittance < 1e-4 */
     float    loss;            /* mean Huber loss per ray (unscaled) */
     uint32_t step;            /* training steps taken (the reference's m_training_step, which
                                  schedules the density-grid updates of insta_train_loop) */
     uint32_t adam_step;       /* Adam's step count (bias correction); restarts at 0 with
                                  fresh moments, e.g. a snapshot loaded without them */
+    uint32_t n_samples_evaluated; /* of n_samples, those deformed and evaluated: the
+                                 step marches every ray in full but evaluates its
+                                 samples in windows, and only while the ray's
+                                 transmittance is still >= 1e-4 (the loss reads
+                                 no sample past that) */
 } insta_train_stats;
 
 /* Defaults: adaptive rays aiming at a 2^18-sample batch, 2^22 marched samples

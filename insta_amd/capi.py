@@ -105,7 +105,7 @@ class InstaTrainConfig(C.Structure):
 class InstaTrainStats(C.Structure):
     _fields_ = [("n_rays", C.c_uint32), ("n_rays_used", C.c_uint32), ("n_samples", C.c_uint32),
                 ("n_samples_used", C.c_uint32), ("loss", C.c_float), ("step", C.c_uint32),
-                ("adam_step", C.c_uint32)]
+                ("adam_step", C.c_uint32), ("n_samples_evaluated", C.c_uint32)]
 
 
 class InstaDensityGridConfig(C.Structure):

@@ -260,10 +260,12 @@ __global__ __launch_bounds__(kClThreads) void cl_sort_kernel(ClArgs a) {
             key[q] = x < n ? e[x] : 0xffffffffu;
             rank[q] = 0u;
         }
+        const uint32_t nq = (n + 63u) / 64u;  // key slots in use (wave-uniform)
         for (uint32_t j = 0; j < n; ++j) {
             const uint32_t v = e[j];
 #pragma unroll
-            for (uint32_t q = 0; q < kSortPerLane; ++q) rank[q] += v < key[q] ? 1u : 0u;
+            for (uint32_t q = 0; q < kSortPerLane; ++q)
+                if (q < nq) rank[q] += v < key[q] ? 1u : 0u;
         }
 #pragma unroll
         for (uint32_t q = 0; q < kSortPerLane; ++q)

@@ -68,8 +68,19 @@ enum : uint32_t {
     kTLoss = 5,      // float bits: sum of per-ray loss / n_rays
     kTCompactRes = 6,  // compacted samples reserved
     kTCompactN = 7,    // compacted samples kept (<= compact capacity)
-    kTCounters = 8
+    kTRound0 = 8,      // samples evaluated in each window (kRounds counters)
+    kTCounters = 12
 };
+
+// Evaluation windows over each ray's marched samples: samples [kWin[w],
+// kWin[w + 1]) of the rays still alive are deformed and evaluated in round
+// w, then each ray's transmittance advances over them (the loss kernel's
+// arithmetic); a ray whose transmittance fell below 1e-4, or whose samples
+// ran out, takes no further round.  The loss reads exactly the samples the
+// rounds evaluated, so the step computes what one pass over every marched
+// sample would.
+constexpr uint32_t kRounds = 4;
+constexpr uint32_t kWin[kRounds + 1] = {0u, 4u, 16u, 64u, 0xffffffffu};
 
 constexpr int kW = 64;
 constexpr int kDOut = 16;
@@ -109,6 +120,11 @@ struct TrainState {
     int32_t* ray_frame = nullptr;
     uint32_t *ray_img = nullptr, *ray_pix = nullptr, *ray_src = nullptr, *ray_base = nullptr, *ray_cnt = nullptr;
     float4 *s_pos = nullptr, *s_cpos = nullptr, *s_cdir = nullptr, *s_out = nullptr, *s_dout = nullptr;
+    // evaluation rounds: per ray transmittance, alive flag and the first slot
+    // of its samples in the round buffers (r_*: one window's samples, packed)
+    float* ray_T = nullptr;
+    uint32_t *ray_alive = nullptr, *ray_gbase = nullptr;
+    float4 *r_pos = nullptr, *r_cpos = nullptr, *r_cdir = nullptr, *r_out = nullptr;
     int32_t* s_hit = nullptr;
     uint32_t* c_idx = nullptr;     // compacted batch: sample index
     float4* c_dout = nullptr;      // compacted batch: dL/d(rgb logits, density logit)
@@ -200,6 +216,8 @@ __global__ __launch_bounds__(256) void train_rays_kernel(RayArgs a, uint32_t n_r
                                                          uint32_t* __restrict__ ray_cnt,
                                                          float4* __restrict__ s_pos,
                                                          float* __restrict__ t_buf,
+                                                         float* __restrict__ ray_T,
+                                                         uint32_t* __restrict__ ray_alive,
                                                          uint32_t* __restrict__ counters) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t img = 0, pix = 0, count = 0;
@@ -236,6 +254,8 @@ __global__ __launch_bounds__(256) void train_rays_kernel(RayArgs a, uint32_t n_r
     ray_src[r] = i;
     ray_base[r] = base;
     ray_cnt[r] = count;
+    ray_T[r] = 1.0f;
+    ray_alive[r] = 1u;
     for (uint32_t j = 0; j < count; ++j) {
         const V3 pos = o + d * t_buf[(size_t)j * n_rays + i];
         s_pos[base + j] = make_float4(pos.x, pos.y, pos.z, __uint_as_float(r));
@@ -413,6 +433,72 @@ struct LossArgs {
 };
 
 INSTA_DEV float sigmoid(float x) { return 1.0f / (1.0f + expf(-x)); }
+
+// one sample's transmittance step, exactly as the loss kernel composites
+INSTA_DEV float transmit(float T, float density_logit, float density_bias, float dt) {
+    const float sigma = expf(density_logit + density_bias);
+    const float alpha = 1.0f - expf(-sigma * dt);
+    return T * (1.0f - alpha);
+}
+
+// Round `w` gather: the window's samples of every alive ray, packed into the
+// round buffers (each ray's samples contiguous from ray_gbase).
+__global__ __launch_bounds__(256) void train_gather_kernel(uint32_t lo, uint32_t hi,
+                                                           const uint32_t* __restrict__ ray_base,
+                                                           const uint32_t* __restrict__ ray_cnt,
+                                                           const uint32_t* __restrict__ ray_alive,
+                                                           uint32_t* __restrict__ ray_gbase,
+                                                           const float4* __restrict__ s_pos,
+                                                           float4* __restrict__ r_pos,
+                                                           uint32_t* __restrict__ counters, uint32_t round_slot) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t n = 0, base = 0;
+    if (r < counters[kTRays] && ray_alive[r]) {
+        const uint32_t cnt = ray_cnt[r];
+        n = cnt > lo ? min(cnt, hi) - lo : 0u;
+        base = ray_base[r];
+    }
+    const uint32_t g = wave_reserve(&counters[round_slot], n);
+    if (n == 0) return;
+    ray_gbase[r] = g;
+    for (uint32_t k = 0; k < n; ++k) r_pos[g + k] = s_pos[base + lo + k];
+}
+
+// Round `w` scatter and transmittance: each alive ray's evaluated samples go
+// back to their marched slots, and its transmittance advances over them; the
+// ray stays alive only while it is >= 1e-4 with samples left past the window.
+__global__ __launch_bounds__(256) void train_round_kernel(uint32_t lo, uint32_t hi, float density_bias, float dt,
+                                                          const uint32_t* __restrict__ ray_base,
+                                                          const uint32_t* __restrict__ ray_cnt,
+                                                          uint32_t* __restrict__ ray_alive,
+                                                          const uint32_t* __restrict__ ray_gbase,
+                                                          float* __restrict__ ray_T,
+                                                          const float4* __restrict__ r_cpos,
+                                                          const float4* __restrict__ r_cdir,
+                                                          const float4* __restrict__ r_out,
+                                                          float4* __restrict__ s_cpos, float4* __restrict__ s_cdir,
+                                                          float4* __restrict__ s_out,
+                                                          const uint32_t* __restrict__ counters) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= counters[kTRays] || !ray_alive[r]) return;
+    const uint32_t cnt = ray_cnt[r];
+    const uint32_t n = cnt > lo ? min(cnt, hi) - lo : 0u;
+    const uint32_t base = ray_base[r] + lo, g = ray_gbase[r];
+    float T = ray_T[r];
+    bool alive = n > 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        const float4 o = r_out[g + k];
+        s_cpos[base + k] = r_cpos[g + k];
+        s_cdir[base + k] = r_cdir[g + k];
+        s_out[base + k] = o;
+        if (alive) {
+            T = transmit(T, o.w, density_bias, dt);
+            if (T < 1e-4f) alive = false;
+        }
+    }
+    ray_T[r] = T;
+    ray_alive[r] = alive && cnt > hi ? 1u : 0u;
+}
 // sRGB transfer functions with the reference's constants (common_device.cuh:31-61)
 INSTA_DEV float srgb_to_linear(float c) { return c <= 0.04045f ? c / 12.92f : powf((c + 0.055f) / 1.055f, 2.4f); }
 INSTA_DEV float linear_to_srgb(float c) { return c < 0.0031308f ? 12.92f * c : 1.055f * powf(c, 0.41666f) - 0.055f; }
@@ -459,7 +545,7 @@ __global__ __launch_bounds__(256) void train_loss_kernel(LossArgs a, Pcg32 rng0,
             acc[0] = acc[0] + w * sigmoid(o.x);
             acc[1] = acc[1] + w * sigmoid(o.y);
             acc[2] = acc[2] + w * sigmoid(o.z);
-            T = T * (1.0f - alpha);
+            T = T * (1.0f - alpha);  // = transmit(T, o.w, ...), which the rounds use
         }
         if (used == cnt) {
             for (int c = 0; c < 3; ++c) acc[c] = acc[c] + T * bg[c];
@@ -848,6 +934,8 @@ void free_workspace_train(TrainState* ts) {
     dev_free(ts->s_pos); dev_free(ts->s_cpos); dev_free(ts->s_cdir); dev_free(ts->s_out); dev_free(ts->s_hit);
     dev_free(ts->c_idx); dev_free(ts->c_dout); dev_free(ts->act); dev_free(ts->dact); dev_free(ts->part);
     dev_free(ts->t_buf);
+    dev_free(ts->ray_T); dev_free(ts->ray_alive); dev_free(ts->ray_gbase);
+    dev_free(ts->r_pos); dev_free(ts->r_cpos); dev_free(ts->r_cdir); dev_free(ts->r_out);
     ts->ray_cap = ts->sample_cap = ts->compact_cap = 0;
 }
 
@@ -870,6 +958,10 @@ insta_status grow_train_workspace(insta_ctx* ctx, TrainState* ts, uint32_t n_ray
     ok = ok && train_alloc(ts->s_pos, sc * sizeof(float4)) && train_alloc(ts->s_cpos, sc * sizeof(float4));
     ok = ok && train_alloc(ts->s_cdir, sc * sizeof(float4)) && train_alloc(ts->s_out, sc * sizeof(float4));
     ok = ok && train_alloc(ts->s_hit, sc * sizeof(int32_t));
+    ok = ok && train_alloc(ts->ray_T, rc * sizeof(float)) && train_alloc(ts->ray_alive, rc * 4) &&
+         train_alloc(ts->ray_gbase, rc * 4);
+    ok = ok && train_alloc(ts->r_pos, sc * sizeof(float4)) && train_alloc(ts->r_cpos, sc * sizeof(float4));
+    ok = ok && train_alloc(ts->r_cdir, sc * sizeof(float4)) && train_alloc(ts->r_out, sc * sizeof(float4));
     ok = ok && train_alloc(ts->c_idx, cc * sizeof(uint32_t)) && train_alloc(ts->c_dout, cc * sizeof(float4));
     ok = ok && train_alloc(ts->act, act_rows * cc * sizeof(float));
     ok = ok && train_alloc(ts->dact, (size_t)Dact::N * cc * sizeof(float));
@@ -1217,14 +1309,28 @@ insta_status train_step(insta_ctx* ctx, const insta_train_config& tc, insta_trai
     ra.fixed_slot = tc.occupancy_slot;
     hipLaunchKernelGGL(train_rays_kernel, dim3((n_rays + 255) / 256), dim3(256), 0, stream, ra, n_rays, ts->rays_total,
                        rng, ts->ray_d, ts->ray_frame, ts->ray_img, ts->ray_pix, ts->ray_src, ts->ray_base, ts->ray_cnt,
-                       ts->s_pos, ts->t_buf, ts->counters);
+                       ts->s_pos, ts->t_buf, ts->ray_T, ts->ray_alive, ts->counters);
     INSTA_HIP(hipGetLastError());
     hipLaunchKernelGGL(train_count_kernel, dim3(1), dim3(64), 0, stream, ts->counters);
-    st = launch_deform_samples(ctx, -1, ts->ray_frame, ts->counters + kTValid, cap, ts->s_pos, ts->ray_d, ts->s_hit,
-                               ts->s_cpos, ts->s_cdir, ctx->d_misses, stream);
-    if (st != INSTA_OK) return st;
-    st = launch_network_samples(ctx, ts->counters + kTValid, cap, ts->s_cpos, ts->s_cdir, ts->s_out, stream);
-    if (st != INSTA_OK) return st;
+    // deform and evaluate the marched samples window by window, only while
+    // their rays are still alive
+    const dim3 rgrid((n_rays + 255) / 256);
+    for (uint32_t w = 0; w < kRounds; ++w) {
+        const uint32_t lo = kWin[w], hi = kWin[w + 1];
+        hipLaunchKernelGGL(train_gather_kernel, rgrid, dim3(256), 0, stream, lo, hi, ts->ray_base, ts->ray_cnt,
+                           ts->ray_alive, ts->ray_gbase, ts->s_pos, ts->r_pos, ts->counters, kTRound0 + w);
+        INSTA_HIP(hipGetLastError());
+        const uint32_t* n_round = ts->counters + kTRound0 + w;
+        st = launch_deform_samples(ctx, -1, ts->ray_frame, n_round, cap, ts->r_pos, ts->ray_d, ts->s_hit, ts->r_cpos,
+                                   ts->r_cdir, ctx->d_misses, stream);
+        if (st != INSTA_OK) return st;
+        st = launch_network_samples(ctx, n_round, cap, ts->r_cpos, ts->r_cdir, ts->r_out, stream);
+        if (st != INSTA_OK) return st;
+        hipLaunchKernelGGL(train_round_kernel, rgrid, dim3(256), 0, stream, lo, hi, ctx->cfg.density_logit_bias,
+                           ctx->cfg.dt, ts->ray_base, ts->ray_cnt, ts->ray_alive, ts->ray_gbase, ts->ray_T,
+                           ts->r_cpos, ts->r_cdir, ts->r_out, ts->s_cpos, ts->s_cdir, ts->s_out, ts->counters);
+        INSTA_HIP(hipGetLastError());
+    }
     LossArgs la;
     la.images = ts->images;
     la.width = ts->width;
@@ -1267,6 +1373,8 @@ insta_status train_step(insta_ctx* ctx, const insta_train_config& tc, insta_trai
         stats->n_rays_used = hc[kTRays];
         stats->n_samples = hc[kTValid];
         stats->n_samples_used = hc[kTUsed];
+        stats->n_samples_evaluated = 0;
+        for (uint32_t w = 0; w < kRounds; ++w) stats->n_samples_evaluated += hc[kTRound0 + w];
         float loss;
         std::memcpy(&loss, &hc[kTLoss], 4);
         stats->loss = loss;
