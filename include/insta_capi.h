@@ -332,6 +332,12 @@ insta_status insta_network_backward(insta_ctx* ctx, const float* coords_dev, uin
 insta_status insta_network_density(insta_ctx* ctx, const float* positions_dev, uint32_t n, float* out_dev,
                                    void* stream);
 
+/* The parameters renders and insta_network_inference use (n_params floats,
+ * insta_get_params' order): the training weights, or their debiased average
+ * while training runs with ema_decay > 0.  No counterpart beyond the
+ * reference trainer's inference parameters (tcnn Trainer::params_inference). */
+insta_status insta_get_inference_params(insta_ctx* ctx, float* params_dev, void* stream);
+
 /* Workspace reservation.  insta_render, insta_render_frame, insta_train_step,
  * insta_train_loop and the network plugin entry points never allocate: their
  * buffers are sized here, ahead of the calls (the reference sizes its
@@ -453,6 +459,14 @@ typedef struct insta_train_config {
     int32_t  occupancy_slot;  /* grid the rays march: INSTA_OCCUPANCY_PER_FRAME (each image's
                                  frame's own grid) or one slot for all rays (-1 canonical,
                                  f frame f), e.g. the density-grid EMA's */
+    float    l2_reg;          /* Adam's L2 term: l2_reg * w joins every parameter's gradient in
+                                 the moment updates (configs/nerf/main.json "l2_reg" 1e-6) */
+    float    ema_decay;       /* 0: off.  Otherwise the Ema optimizer around Adam
+                                 (main.json "Ema", decay 0.95): after each update the
+                                 average a = decay * a + (1 - decay) * w is kept, and renders,
+                                 insta_network_inference and insta_get_inference_params use
+                                 a / (1 - decay^t) (t updates since it started); training
+                                 itself keeps using w */
 } insta_train_config;
 #define INSTA_OCCUPANCY_PER_FRAME (-2)
 

@@ -99,7 +99,7 @@ class InstaTrainConfig(C.Structure):
     _fields_ = [("n_rays", C.c_uint32), ("target_samples", C.c_uint32), ("max_samples", C.c_uint32),
                 ("learning_rate", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("epsilon", C.c_float),
                 ("loss_scale", C.c_float), ("grid_grad_fp16", C.c_uint32), ("apply_update", C.c_uint32),
-                ("seed", C.c_uint32), ("occupancy_slot", C.c_int32)]
+                ("seed", C.c_uint32), ("occupancy_slot", C.c_int32), ("l2_reg", C.c_float), ("ema_decay", C.c_float)]
 
 
 class InstaTrainStats(C.Structure):
@@ -144,6 +144,7 @@ SIGNATURES = [
     ("insta_load_config_json", C.c_int, [C.c_char_p, C.POINTER(InstaConfig), C.POINTER(InstaRunConfig)]),
     ("insta_create", C.c_int, [C.POINTER(InstaConfig), C.POINTER(_VP)]),
     ("insta_reserve", C.c_int, [_VP, C.POINTER(InstaReserveConfig), _VP]),
+    ("insta_get_inference_params", C.c_int, [_VP, _VP, _VP]),
     ("insta_destroy", None, [_VP]),
     ("insta_set_canonical_mesh", C.c_int, [_VP, _VP, C.c_uint32, _VP, C.c_uint32, _VP, _VP]),
     ("insta_set_frames", C.c_int, [_VP, C.c_uint32, C.c_uint32, _VP, _VP]),
@@ -598,6 +599,11 @@ class Context:
     def params(self):
         p = self.torch.empty(self.layout.n_params, dtype=self.torch.float32, device="cuda")
         check(self.lib.insta_get_params(self.h, _ptr(p), self.stream()), "insta_get_params")
+        return p
+
+    def inference_params(self):
+        p = self.torch.empty(self.layout.n_params, dtype=self.torch.float32, device="cuda")
+        check(self.lib.insta_get_inference_params(self.h, _ptr(p), self.stream()), "insta_get_inference_params")
         return p
 
     def reset_optimizer(self) -> None:

@@ -578,6 +578,7 @@ insta_status insta_set_params(insta_ctx* ctx, const float* params_dev, uint64_t 
     }
     INSTA_HIP(hipMemcpyAsync(ctx->d_params, params_dev, n_params * sizeof(float), hipMemcpyDeviceToDevice,
                              (hipStream_t)stream));
+    insta::reset_inference_average(ctx);
     insta_status st = insta::prepare_network(ctx, (hipStream_t)stream);
     if (st != INSTA_OK) return st;
     ctx->params_set = true;
@@ -814,6 +815,8 @@ void insta_default_train_config(insta_train_config* tc) {
     tc->apply_update = 1;
     tc->seed = 1337u;
     tc->occupancy_slot = INSTA_OCCUPANCY_PER_FRAME;
+    tc->l2_reg = 0.0f;
+    tc->ema_decay = 0.0f;
 }
 
 insta_status insta_set_training_set(insta_ctx* ctx, uint32_t n_images, uint32_t width, uint32_t height,
@@ -844,6 +847,17 @@ insta_status insta_set_training_set(insta_ctx* ctx, uint32_t n_images, uint32_t 
     return insta::set_training_set(ctx, n_images, width, height, rgba_dev, cameras, frames, (hipStream_t)stream);
 }
 
+insta_status insta_get_inference_params(insta_ctx* ctx, float* params_dev, void* stream) {
+    INSTA_CHECK_CTX(ctx);
+    if (!params_dev || !ctx->params_set) {
+        set_error("get_inference_params: null pointer or parameters not set");
+        return !params_dev ? INSTA_ERR_INVALID : INSTA_ERR_STATE;
+    }
+    INSTA_HIP(hipMemcpyAsync(params_dev, insta::net_params(ctx), ctx->layout.n_params * sizeof(float),
+                             hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    return INSTA_OK;
+}
+
 insta_status insta_reserve(insta_ctx* ctx, const insta_reserve_config* r, void* stream) {
     INSTA_CHECK_CTX(ctx);
     if (!r || (uint64_t)r->render_pixels > (1ull << 26) || r->train_rays > (1u << 24) ||
@@ -867,7 +881,7 @@ insta_status insta_train_step(insta_ctx* ctx, const insta_train_config* tc, inst
     INSTA_CHECK_CTX(ctx);
     if (!tc || tc->max_samples == 0 || tc->max_samples > (1u << 26) || tc->n_rays > (1u << 24) ||
         tc->target_samples == 0 || tc->target_samples > (1u << 24) || !(tc->loss_scale > 0.0f) ||
-        !(tc->learning_rate >= 0.0f)) {
+        !(tc->learning_rate >= 0.0f) || !(tc->l2_reg >= 0.0f) || !(tc->ema_decay >= 0.0f && tc->ema_decay < 1.0f)) {
         set_error("train_step: bad training configuration");
         return INSTA_ERR_INVALID;
     }

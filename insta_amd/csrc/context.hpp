@@ -93,7 +93,9 @@ struct insta_ctx {
     uint32_t* d_cl_state = nullptr; // [n_slots][kClState]
 
     // parameters
-    float* d_params = nullptr;      // FP32 master copy [n_params]
+    float* d_params = nullptr;      // FP32 master copy [n_params] (the weights training updates)
+    const float* net_src = nullptr; // the parameters the inference network is prepared from:
+                                    // d_params, or the debiased weight average under an EMA optimizer
     void* d_wfrag = nullptr;        // MLP weights arranged as MFMA A fragments
     void* d_table_h = nullptr;      // FP16 copy of the hash tables (reduced precision)
     float* d_frame_extra = nullptr; // [max_frames][n_extra_dims] per-frame extra dims (n_extra_dims > 0)
@@ -169,6 +171,9 @@ struct insta_ctx {
 };
 
 namespace insta {
+
+// the parameters the inference network reads (see insta_ctx::net_src)
+inline const float* net_params(const insta_ctx* c) { return c->net_src ? c->net_src : c->d_params; }
 
 template <class T>
 inline void dev_free(T*& p) {
@@ -263,6 +268,8 @@ insta_status train_step(insta_ctx* ctx, const insta_train_config& tc, insta_trai
 // samples and a compacted batch of `batch` (insta_reserve)
 insta_status reserve_training(insta_ctx* ctx, uint32_t rays, uint32_t samples, uint32_t batch, hipStream_t stream);
 bool train_has_gradients(const insta_ctx* ctx);
+// new weights: inference reads them directly, the Ema average restarts
+void reset_inference_average(insta_ctx* ctx);
 // density grid (density.hip)
 void free_density_grid(insta_ctx* ctx);
 insta_status update_density_grid(insta_ctx* ctx, uint32_t slot, uint32_t n_frames, const insta_density_grid_config& c,

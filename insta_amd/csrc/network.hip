@@ -199,8 +199,8 @@ __global__ __launch_bounds__(kNetThreads) void network_soa_kernel(NetArgs a, con
 
 NetArgs net_args(const insta_ctx* ctx) {
     NetArgs a;
-    a.params = ctx->d_params;
-    a.table = ctx->d_params + ctx->layout.n_mlp_params;
+    a.params = net_params(ctx);
+    a.table = net_params(ctx) + ctx->layout.n_mlp_params;
     a.frame_extra = ctx->d_frame_extra;
     a.n_frames = ctx->cfg.max_frames;
     a.gp = grid_params(ctx);

@@ -523,7 +523,7 @@ bool mfma_network_supported(const insta_config& c) {
 
 insta_status update_extra_rows(insta_ctx* ctx, uint32_t first, uint32_t n, hipStream_t stream) {
     if (!ctx->d_extra_bias || n == 0) return INSTA_OK;
-    hipLaunchKernelGGL(extra_bias_kernel, dim3(n), dim3(kW), 0, stream, ctx->d_params, mlp_offsets(ctx),
+    hipLaunchKernelGGL(extra_bias_kernel, dim3(n), dim3(kW), 0, stream, net_params(ctx), mlp_offsets(ctx),
                        ctx->cfg.n_extra_dims, ctx->d_frame_extra, first, ctx->d_extra_bias);
     INSTA_HIP(hipGetLastError());
     return INSTA_OK;
@@ -548,18 +548,18 @@ insta_status prepare_network(insta_ctx* ctx, hipStream_t stream) {
     const MlpOffsets o = mlp_offsets(ctx);
     if (ctx->cfg.precision == INSTA_PRECISION_FP32) {
         const uint32_t nfrag = 4 * ((o.enc + 3) / 4) + 16 + 32 + 64 + 16;
-        hipLaunchKernelGGL(prep_f32_kernel, dim3(nfrag), dim3(64), 0, stream, ctx->d_params, o,
+        hipLaunchKernelGGL(prep_f32_kernel, dim3(nfrag), dim3(64), 0, stream, net_params(ctx), o,
                            static_cast<float*>(ctx->d_wfrag), nfrag);
     } else {
         if (ctx->cfg.precision == INSTA_PRECISION_FP16)
-            hipLaunchKernelGGL(prep_x32_kernel<_Float16>, dim3(kFragX32), dim3(64), 0, stream, ctx->d_params, o,
+            hipLaunchKernelGGL(prep_x32_kernel<_Float16>, dim3(kFragX32), dim3(64), 0, stream, net_params(ctx), o,
                                static_cast<_Float16*>(ctx->d_wfrag), kFragX32);
         else
-            hipLaunchKernelGGL(prep_x32_kernel<__bf16>, dim3(kFragX32), dim3(64), 0, stream, ctx->d_params, o,
+            hipLaunchKernelGGL(prep_x32_kernel<__bf16>, dim3(kFragX32), dim3(64), 0, stream, net_params(ctx), o,
                                static_cast<__bf16*>(ctx->d_wfrag), kFragX32);
         const uint64_t nt = ctx->layout.n_grid_params;
         hipLaunchKernelGGL(table_to_half_kernel, dim3((uint32_t)((nt + 255) / 256)), dim3(256), 0, stream,
-                           ctx->d_params + ctx->layout.n_mlp_params, static_cast<_Float16*>(ctx->d_table_h), nt);
+                           net_params(ctx) + ctx->layout.n_mlp_params, static_cast<_Float16*>(ctx->d_table_h), nt);
     }
     INSTA_HIP(hipGetLastError());
     return INSTA_OK;
@@ -569,7 +569,7 @@ template <int X, class Src>
 static insta_status launch_mfma_x(insta_ctx* ctx, Src src, uint32_t n_max, hipStream_t stream) {
     const GridParams gp = grid_params(ctx);
     const dim3 grid(grid_for(n_max)), block(256);
-    const float* table = ctx->d_params + ctx->layout.n_mlp_params;
+    const float* table = net_params(ctx) + ctx->layout.n_mlp_params;
     const _Float16* table_h = static_cast<const _Float16*>(ctx->d_table_h);
     const float* xb = ctx->d_extra_bias;
     const uint32_t xr = ctx->n_slots;  // rows of the extra-dims table

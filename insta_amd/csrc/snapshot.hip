@@ -609,6 +609,7 @@ insta_status insta_load_snapshot(insta_ctx* ctx, const char* path, int32_t occup
     std::memcpy(params.data(), pb->s.data(), np * sizeof(float));
     INSTA_HIP(hipMemcpyAsync(ctx->d_params, params.data(), np * sizeof(float), hipMemcpyHostToDevice, s));
     INSTA_HIP(hipStreamSynchronize(s));  // pageable source
+    reset_inference_average(ctx);
     st = prepare_network(ctx, s);
     if (st != INSTA_OK) return st;
     ctx->params_set = true;

@@ -143,6 +143,9 @@ struct TrainState {
     // Adam's own step (bias correction; restarts with fresh moments)
     uint32_t training_step = 0;
     uint32_t adam_step = 0;
+    // Ema optimizer: running average of the weights and its debiased copy
+    float *ema = nullptr, *ema_infer = nullptr;
+    uint32_t ema_step = 0;
     bool have_grad = false;
     // network plugin: per-sample extra dims of the last forward ([cap][X]) and
     // its batch size (0: no stored forward; a training step overwrites it)
@@ -900,7 +903,7 @@ __global__ void wgrad_reduce_kernel(const float* __restrict__ part, const uint32
 __global__ void adam_kernel(float* __restrict__ params, float* __restrict__ grad, const __half2* __restrict__ grad_h,
                             float* __restrict__ m, float* __restrict__ v, uint64_t n_params, uint64_t n_mlp,
                             float inv_scale, float lr, float b1, float b2, float eps, float bc1, float bc2,
-                            uint32_t apply) {
+                            float l2_reg, uint32_t apply) {
     for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < n_params;
          p += (uint64_t)gridDim.x * blockDim.x) {
         float g;
@@ -914,12 +917,26 @@ __global__ void adam_kernel(float* __restrict__ params, float* __restrict__ grad
         g = g * inv_scale;
         grad[p] = g;
         if (apply) {
+            g = g + l2_reg * params[p];
             const float mm = b1 * m[p] + (1.0f - b1) * g;
             const float vv = b2 * v[p] + (1.0f - b2) * (g * g);
             m[p] = mm;
             v[p] = vv;
             params[p] = params[p] - lr * (mm / bc1) / (sqrtf(vv / bc2) + eps);
         }
+    }
+}
+
+// Ema around the optimizer: a = decay * a + (1 - decay) * w (a starts at 0
+// with the first update), and the debiased average a * debias, debias =
+// 1 / (1 - decay^t), for inference
+__global__ void ema_kernel(const float* __restrict__ w, float* __restrict__ a, float* __restrict__ infer,
+                           uint64_t n, float decay, float debias, uint32_t first) {
+    for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < n; p += (uint64_t)gridDim.x * blockDim.x) {
+        const float prev = first ? 0.0f : a[p];
+        const float v = decay * prev + (1.0f - decay) * w[p];
+        a[p] = v;
+        infer[p] = v * debias;
     }
 }
 
@@ -1206,6 +1223,8 @@ void free_train(insta_ctx* ctx) {
     free_workspace_train(ts);
     dev_free(ts->images); dev_free(ts->meta); dev_free(ts->grad); dev_free(ts->grad_h);
     dev_free(ts->adam_m); dev_free(ts->adam_v); dev_free(ts->counters); dev_free(ts->net_extra);
+    if (ctx->net_src == ts->ema_infer) ctx->net_src = nullptr;
+    dev_free(ts->ema); dev_free(ts->ema_infer);
     if (ts->h_counters) (void)hipHostFree(ts->h_counters);
     delete ts;
     ctx->train = nullptr;
@@ -1222,7 +1241,8 @@ static insta_status ensure_train_state(insta_ctx* ctx, hipStream_t stream) {
         ctx->train = ts;
         const uint64_t np = ctx->layout.n_params;
         bool ok = train_alloc(ts->grad, np * sizeof(float)) && train_alloc(ts->adam_m, np * sizeof(float)) &&
-                  train_alloc(ts->adam_v, np * sizeof(float)) &&
+                  train_alloc(ts->adam_v, np * sizeof(float)) && train_alloc(ts->ema, np * sizeof(float)) &&
+                  train_alloc(ts->ema_infer, np * sizeof(float)) &&
                   train_alloc(ts->grad_h, (ctx->layout.n_grid_params / 2 + 1) * sizeof(__half2)) &&
                   train_alloc(ts->counters, kTCounters * sizeof(uint32_t)) &&
                   hipHostMalloc((void**)&ts->h_counters, kTCounters * sizeof(uint32_t), hipHostMallocDefault) ==
@@ -1278,6 +1298,14 @@ insta_status train_step(insta_ctx* ctx, const insta_train_config& tc, insta_trai
     const uint32_t n_rays = tc.n_rays ? tc.n_rays : ts->rays_per_batch;
     insta_status st = check_train_workspace(ts, n_rays, tc.max_samples, tc.target_samples, "train_step");
     if (st != INSTA_OK) return st;
+    // the step evaluates its samples with the training weights (under an Ema
+    // optimizer the inference network holds the average between steps)
+    const float* inference_src = net_params(ctx);
+    if (inference_src != ctx->d_params) {
+        ctx->net_src = nullptr;
+        st = prepare_network(ctx, stream);
+        if (st != INSTA_OK) return st;
+    }
     const uint64_t np = ctx->layout.n_params, n_mlp = ctx->layout.n_mlp_params;
     // the marched-sample cap is the step's (a larger reservation changes nothing)
     const uint32_t cap = tc.max_samples;
@@ -1358,9 +1386,25 @@ insta_status train_step(insta_ctx* ctx, const insta_train_config& tc, insta_trai
     hipLaunchKernelGGL(adam_kernel, dim3(2048), dim3(256), 0, stream, ctx->d_params, ts->grad,
                        tc.grid_grad_fp16 ? ts->grad_h : nullptr, ts->adam_m, ts->adam_v, np, n_mlp,
                        1.0f / tc.loss_scale, tc.learning_rate, tc.beta1, tc.beta2, tc.epsilon, bc1, bc2,
-                       tc.apply_update);
+                       tc.l2_reg, tc.apply_update);
     INSTA_HIP(hipGetLastError());
     if (tc.apply_update) {
+        if (tc.ema_decay > 0.0f) {
+            // the weight average renders use; training keeps its own weights
+            ts->ema_step += 1;
+            const float debias = (float)(1.0 / (1.0 - std::pow((double)tc.ema_decay, (double)ts->ema_step)));
+            hipLaunchKernelGGL(ema_kernel, dim3(2048), dim3(256), 0, stream, ctx->d_params, ts->ema, ts->ema_infer,
+                               np, tc.ema_decay, debias, ts->ema_step == 1 ? 1u : 0u);
+            INSTA_HIP(hipGetLastError());
+            ctx->net_src = ts->ema_infer;
+        } else {
+            ts->ema_step = 0;
+            ctx->net_src = nullptr;
+        }
+        st = prepare_network(ctx, stream);
+        if (st != INSTA_OK) return st;
+    } else if (inference_src != net_params(ctx)) {
+        ctx->net_src = inference_src == ctx->d_params ? nullptr : inference_src;  // back to what renders used
         st = prepare_network(ctx, stream);
         if (st != INSTA_OK) return st;
     }
@@ -1396,6 +1440,11 @@ insta_status train_step(insta_ctx* ctx, const insta_train_config& tc, insta_trai
     next = std::min(std::max(next, 128.0), (double)(1u << 18));
     ts->rays_per_batch = ((uint32_t)next + 127u) & ~127u;
     return INSTA_OK;
+}
+
+void reset_inference_average(insta_ctx* ctx) {
+    if (ctx->train) ctx->train->ema_step = 0;
+    ctx->net_src = nullptr;
 }
 
 bool train_has_gradients(const insta_ctx* ctx) { return ctx->train && ctx->train->have_grad; }

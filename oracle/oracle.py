@@ -380,3 +380,24 @@ def triangle_distance_sq(tri9, p) -> float:
     t = _c(tri9, np.float32)
     q = _c(p, np.float32)
     return lib().orc_triangle_distance_sq(_p(t), _p(q))
+
+
+def adam_update_d(w, g, m, v, t, lr, beta1, beta2, eps, l2_reg=0.0):
+    """One Adam update in FP64 (Kingma & Ba 2015, with bias correction), the
+    L2 term l2_reg * w joined to the gradient in the moment updates (tcnn's
+    Adam "l2_reg", configs/nerf/main.json:21-32).  Returns (w, m, v)."""
+    w, g, m, v = (np.asarray(x, np.float64) for x in (w, g, m, v))
+    gg = g + l2_reg * w
+    m = beta1 * m + (1.0 - beta1) * gg
+    v = beta2 * v + (1.0 - beta2) * gg * gg
+    mh = m / (1.0 - beta1 ** t)
+    vh = v / (1.0 - beta2 ** t)
+    return w - lr * mh / (np.sqrt(vh) + eps), m, v
+
+
+def ema_update_d(a, w, decay, t):
+    """The Ema optimizer's weight average in FP64 (tcnn "Ema", main.json
+    decay 0.95): a_t = decay * a_{t-1} + (1 - decay) * w_t from a_0 = 0, and
+    the debiased a_t / (1 - decay^t) that inference uses.  Returns (a, debiased)."""
+    a = decay * np.asarray(a, np.float64) + (1.0 - decay) * np.asarray(w, np.float64)
+    return a, a / (1.0 - decay ** t)

@@ -238,3 +238,72 @@ def test_train_loop_follows_the_reference_schedule(scene):
     occ_a, occ_b = a.get_occupancy(-1).cpu().numpy(), b.get_occupancy(-1).cpu().numpy()
     assert (np.unpackbits(occ_a) == np.unpackbits(occ_b)).mean() > 0.999
     assert np.unpackbits(occ_a).sum() > 0
+
+
+def test_adam_l2_term_against_the_fp64_restatement(scene):
+    """Two steps with l2_reg: the parameters follow the oracle's FP64 Adam
+    with the L2 term (moments carried over), given the step's own loss
+    gradients (which exclude the L2 term)."""
+    import torch
+    from oracle.oracle import adam_update_d
+    ctx, params = _gpu(scene)
+    cfg = scene[0]
+    tc = default_train_config(n_rays=1024, max_samples=1024 * cfg.max_steps, target_samples=1024 * cfg.max_steps,
+                              seed=8, l2_reg=0.05)
+    w = params.astype(np.float64)
+    m = v = np.zeros_like(w)
+    gs = []
+    for t in (1, 2):
+        ctx.train_step(tc)
+        g = ctx.gradients().cpu().numpy().astype(np.float64)
+        gs.append(g)
+        got = ctx.params().cpu().numpy().astype(np.float64)
+        torch.cuda.synchronize()
+        w, m, v = adam_update_d(w, g, m, v, t, tc.learning_rate, tc.beta1, tc.beta2, tc.epsilon, tc.l2_reg)
+        # FP32 arithmetic against FP64 on the same gradients
+        np.testing.assert_allclose(got, w, rtol=1e-5, atol=1e-7)
+        w = got  # continue from the device's weights
+    # the L2 term moved weights whose loss gradient was zero (untouched table entries)
+    idle = (gs[0] == 0) & (gs[1] == 0) & (np.abs(params) > 0.05)
+    assert idle.any() and (np.abs(got[idle]) < np.abs(params[idle])).all()
+
+
+def test_ema_average_is_what_inference_uses(scene):
+    """Three steps under the Ema optimizer: the inference parameters are the
+    FP64 debiased average of the training weights after each step, the
+    training weights stay Adam's, and a render uses the average (it equals
+    the render of a context given the average as its parameters)."""
+    import torch
+    from insta_amd.capi import Context
+    from oracle.oracle import ema_update_d
+    ctx, params = _gpu(scene)
+    cfg = scene[0]
+    decay = 0.9
+    tc = default_train_config(n_rays=1024, max_samples=1024 * cfg.max_steps, target_samples=1024 * cfg.max_steps,
+                              seed=9, ema_decay=decay)
+    a = np.zeros(ctx.layout.n_params)
+    for t in (1, 2, 3):
+        ctx.train_step(tc)
+        w = ctx.params().cpu().numpy().astype(np.float64)
+        inf = ctx.inference_params().cpu().numpy().astype(np.float64)
+        torch.cuda.synchronize()
+        a, deb = ema_update_d(a, w, decay, t)
+        np.testing.assert_allclose(inf, deb, rtol=1e-5, atol=1e-7)
+        assert not np.array_equal(inf, w)
+    # a render through the average, against a context holding it as weights
+    av = scene[1]
+    cam = make_camera(synth.look_at_camera(np.array([0.2, 0.1, 1.0]), 1.0), synth.focal_from_fov(48, 30.0))
+    ctx.build_occupancy(0, 0.05)
+    r1 = [x.clone() for x in ctx.render(cam, 48, 48, 0, 0)]
+    ref = Context(cfg)
+    ref.set_canonical_mesh(av.canonical, av.faces, av.masks)
+    ref.set_frames(0, torch.from_numpy(np.stack(av.frames)).cuda())
+    ref.set_params(ctx.inference_params())
+    ref.build_occupancy(0, 0.05)
+    r2 = ref.render(cam, 48, 48, 0, 0)
+    torch.cuda.synchronize()
+    for x, y in zip(r1, r2):
+        assert torch.equal(x, y)
+    # new weights end the averaging: inference reads them directly
+    ctx.set_params(torch.from_numpy(params).cuda())
+    assert torch.equal(ctx.inference_params().cpu(), torch.from_numpy(params))
