@@ -438,13 +438,30 @@ insta_status insta_render_frame(insta_ctx* ctx, const insta_camera* cam, uint32_
 /* Training set: n_images RGBA float images, rgba_dev [n][height][width][4]
  * (colour in [0,1]; alpha 1 = opaque, else composited over the random
  * background), image i seen by cameras[i] (host array) through posed frame
- * frames[i] (host array; that frame's mesh and occupancy slot must be built
- * before training).  The context keeps its own copy of the images.  Stands
+ * frames[i] (host array; that frame's mesh must be built, and its occupancy
+ * grid too before a step whose rays march their frames' grids).  The context keeps its own copy of the images.  Stands
  * for the NeRF dataset load + Testbed::load_training_data
  * (instant-ngp/src/nerf_loader.cu:405-540, testbed.cu:2951-2980). */
 insta_status insta_set_training_set(insta_ctx* ctx, uint32_t n_images, uint32_t width, uint32_t height,
                                     const float* rgba_dev, const insta_camera* cameras,
                                     const int32_t* frames, void* stream);
+
+/* Segmentation masks and depth images of the training set last given to
+ * insta_set_training_set (which drops any earlier ones), either NULL:
+ * seg_dev [n][height][width] uint8 mask ids (the first channel of the
+ * reference's RGBA mask image), depth_dev [n][height][width] float depth along
+ * the camera axis in scene units (the reference's depth texel times its depth
+ * scale), 0 = no depth at that pixel.  The context keeps copies.  In the
+ * loss (compute_loss_kernel_train_nerf, testbed_nerf.cu:1476-1497, 1555-1586):
+ * rays whose mask id is 120, 130 or 140 have their colour gradient scaled by
+ * seg_weight; with masks set, ids <= 90 or > 250 and texels of alpha < 0.001
+ * get no depth term; otherwise, where the depth texel is positive, the L1 loss
+ * depth_weight * |D - |d| z| joins the density gradient, D the composited
+ * distance |x - o| of the samples, |d| the unnormalised ray direction's length,
+ * z the depth texel.  Stands for the seg_mask_path / depth_path loads of
+ * NerfDataset (instant-ngp/src/nerf_loader.cu:773-800, 992-1012). */
+insta_status insta_set_training_supervision(insta_ctx* ctx, const uint8_t* seg_dev, const float* depth_dev,
+                                            void* stream);
 
 typedef struct insta_train_config {
     uint32_t n_rays;          /* rays this step; 0 = adaptive count (aims at target_samples) */
@@ -467,6 +484,11 @@ typedef struct insta_train_config {
                                  insta_network_inference and insta_get_inference_params use
                                  a / (1 - decay^t) (t updates since it started); training
                                  itself keeps using w */
+    float    seg_weight;      /* colour-gradient weight of rays whose mask id is 120, 130 or 140
+                                 (seg_mask_supervision_lambda, testbed.h:623, default 1;
+                                 main.json "seg_weight" 40); used with insta_set_training_supervision */
+    float    depth_weight;    /* weight of the L1 depth term (depth_supervision_lambda,
+                                 testbed.h:622, default 1.5; main.json "depth_weight" 1.25) */
 } insta_train_config;
 #define INSTA_OCCUPANCY_PER_FRAME (-2)
 
@@ -539,7 +561,10 @@ typedef struct insta_density_grid_config {
  * float grid, then threshold at min(0.01, mean) into occupancy slot `slot`
  * (-1: the canonical slot, f: frame f's slot) for the march.  grid_dev
  * (nullable) receives the G^3 floats (Morton order); mean (nullable, host)
- * the grid mean (synchronises). */
+ * the grid mean (synchronises).  With a training set loaded, the first update
+ * (and any after the number of training images changed) first marks the
+ * cells no training camera sees as untrained, -1: never sampled, never
+ * occupied (mark_untrained_density_grid, testbed_nerf.cu:391-431, 2795-2806). */
 insta_status insta_update_density_grid(insta_ctx* ctx, int32_t slot, uint32_t n_frames,
                                        const insta_density_grid_config* cfg, float* grid_dev, float* mean,
                                        void* stream);

@@ -99,7 +99,8 @@ class InstaTrainConfig(C.Structure):
     _fields_ = [("n_rays", C.c_uint32), ("target_samples", C.c_uint32), ("max_samples", C.c_uint32),
                 ("learning_rate", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("epsilon", C.c_float),
                 ("loss_scale", C.c_float), ("grid_grad_fp16", C.c_uint32), ("apply_update", C.c_uint32),
-                ("seed", C.c_uint32), ("occupancy_slot", C.c_int32), ("l2_reg", C.c_float), ("ema_decay", C.c_float)]
+                ("seed", C.c_uint32), ("occupancy_slot", C.c_int32), ("l2_reg", C.c_float), ("ema_decay", C.c_float),
+                ("seg_weight", C.c_float), ("depth_weight", C.c_float)]
 
 
 class InstaTrainStats(C.Structure):
@@ -180,6 +181,7 @@ SIGNATURES = [
     ("insta_set_profiling", C.c_int, [_VP, C.c_int]),
     ("insta_get_profile", C.c_int, [_VP, C.POINTER(InstaProfile)]),
     ("insta_set_training_set", C.c_int, [_VP, C.c_uint32, C.c_uint32, C.c_uint32, _VP, _VP, _VP, _VP]),
+    ("insta_set_training_supervision", C.c_int, [_VP, _VP, _VP, _VP]),
     ("insta_default_train_config", None, [C.POINTER(InstaTrainConfig)]),
     ("insta_train_step", C.c_int, [_VP, C.POINTER(InstaTrainConfig), C.POINTER(InstaTrainStats), _VP]),
     ("insta_train_loop", C.c_int, [_VP, C.POINTER(InstaTrainConfig), C.c_uint32, C.POINTER(InstaDensityGridConfig),
@@ -568,6 +570,15 @@ class Context:
         fr = np.ascontiguousarray(frames, dtype=np.int32)
         check(self.lib.insta_set_training_set(self.h, n, w, h, _ptr(images_dev.contiguous()), C.addressof(arr),
                                               fr.ctypes.data, self.stream()), "insta_set_training_set")
+
+    def set_training_supervision(self, seg_dev=None, depth_dev=None) -> None:
+        """seg_dev [n, h, w] uint8 mask ids and depth_dev [n, h, w] float32
+        camera-axis depth (0 = none) on the device, either None, for the
+        training set set last."""
+        check(self.lib.insta_set_training_supervision(
+            self.h, None if seg_dev is None else _ptr(seg_dev.contiguous()),
+            None if depth_dev is None else _ptr(depth_dev.contiguous()), self.stream()),
+            "insta_set_training_supervision")
 
     def _reserve_training(self, tc: "InstaTrainConfig") -> None:
         # the adaptive ray count stays below 2^18 rays

@@ -817,6 +817,8 @@ void insta_default_train_config(insta_train_config* tc) {
     tc->occupancy_slot = INSTA_OCCUPANCY_PER_FRAME;
     tc->l2_reg = 0.0f;
     tc->ema_decay = 0.0f;
+    tc->seg_weight = 1.0f;     // seg_mask_supervision_lambda (testbed.h:623)
+    tc->depth_weight = 1.5f;   // depth_supervision_lambda (testbed.h:622)
 }
 
 insta_status insta_set_training_set(insta_ctx* ctx, uint32_t n_images, uint32_t width, uint32_t height,
@@ -830,8 +832,8 @@ insta_status insta_set_training_set(insta_ctx* ctx, uint32_t n_images, uint32_t 
     }
     for (uint32_t i = 0; i < n_images; ++i) {
         const int32_t f = frames[i];
-        if (f < 0 || f >= (int32_t)ctx->cfg.max_frames || !ctx->slot_built[f + 1] || !ctx->grid_built[f + 1]) {
-            set_error("set_training_set: an image refers to a frame whose mesh or occupancy grid is not built");
+        if (f < 0 || f >= (int32_t)ctx->cfg.max_frames || !ctx->slot_built[f + 1]) {
+            set_error("set_training_set: an image refers to a frame whose mesh is not built");
             return INSTA_ERR_STATE;
         }
         if (!(cameras[i].focal_px[0] > 0.0f) || !(cameras[i].focal_px[1] > 0.0f)) {
@@ -845,6 +847,16 @@ insta_status insta_set_training_set(insta_ctx* ctx, uint32_t n_images, uint32_t 
         return INSTA_ERR_UNSUPPORTED;
     }
     return insta::set_training_set(ctx, n_images, width, height, rgba_dev, cameras, frames, (hipStream_t)stream);
+}
+
+insta_status insta_set_training_supervision(insta_ctx* ctx, const uint8_t* seg_dev, const float* depth_dev,
+                                            void* stream) {
+    INSTA_CHECK_CTX(ctx);
+    if (!insta::train_has_set(ctx)) {
+        set_error("set_training_supervision: no training set");
+        return INSTA_ERR_STATE;
+    }
+    return insta::set_training_supervision(ctx, seg_dev, depth_dev, (hipStream_t)stream);
 }
 
 insta_status insta_get_inference_params(insta_ctx* ctx, float* params_dev, void* stream) {
@@ -881,7 +893,8 @@ insta_status insta_train_step(insta_ctx* ctx, const insta_train_config* tc, inst
     INSTA_CHECK_CTX(ctx);
     if (!tc || tc->max_samples == 0 || tc->max_samples > (1u << 26) || tc->n_rays > (1u << 24) ||
         tc->target_samples == 0 || tc->target_samples > (1u << 24) || !(tc->loss_scale > 0.0f) ||
-        !(tc->learning_rate >= 0.0f) || !(tc->l2_reg >= 0.0f) || !(tc->ema_decay >= 0.0f && tc->ema_decay < 1.0f)) {
+        !(tc->learning_rate >= 0.0f) || !(tc->l2_reg >= 0.0f) || !(tc->ema_decay >= 0.0f && tc->ema_decay < 1.0f) ||
+        !(tc->seg_weight >= 0.0f && tc->seg_weight <= 1e6f) || !(tc->depth_weight >= 0.0f && tc->depth_weight <= 1e6f)) {
         set_error("train_step: bad training configuration");
         return INSTA_ERR_INVALID;
     }
@@ -901,6 +914,14 @@ insta_status insta_train_step(insta_ctx* ctx, const insta_train_config* tc, inst
         if (!ctx->grid_built[tc->occupancy_slot + 1]) {
             set_error("train_step: the occupancy slot has no grid");
             return INSTA_ERR_STATE;
+        }
+    } else {
+        for (const int32_t f : insta::train_frames(ctx)) {
+            if (!ctx->slot_built[f + 1] || !ctx->grid_built[f + 1]) {
+                set_error("train_step: frame " + std::to_string(f) +
+                          " of the training set has no mesh or occupancy grid (rays march their frame's grid)");
+                return INSTA_ERR_STATE;
+            }
         }
     }
     return insta::train_step(ctx, *tc, stats, (hipStream_t)stream);

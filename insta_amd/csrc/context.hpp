@@ -276,6 +276,22 @@ insta_status update_density_grid(insta_ctx* ctx, uint32_t slot, uint32_t n_frame
                                  float* grid_out, float* mean_out, hipStream_t stream);
 insta_status reset_density_grid(insta_ctx* ctx);
 bool train_has_set(const insta_ctx* ctx);
+// one training image's camera on the device: 3x4 matrix (columns right, down,
+// forward, position; row-major), focal lengths and principal point in
+// pixels / image fractions, its posed frame
+struct TrainImage {
+    float M[12];
+    float fx, fy, cx, cy;
+    int32_t frame;
+    uint32_t pad[3];
+};
+struct TrainSetView {
+    const TrainImage* meta = nullptr;  // device [n_images]
+    uint32_t n_images = 0, width = 0, height = 0;
+};
+TrainSetView train_set_view(const insta_ctx* ctx);
+const std::vector<int32_t>& train_frames(const insta_ctx* ctx);  // frame of each training image
+insta_status set_training_supervision(insta_ctx* ctx, const uint8_t* seg, const float* depth, hipStream_t stream);
 uint32_t train_step_count(const insta_ctx* ctx);  // training steps taken (the reference's m_training_step)
 // snapshot state (snapshot.hip): host copies of the training and density-grid state
 struct TrainSnapshot {

@@ -35,6 +35,7 @@ struct DensityGridState {
     float* h_mean = nullptr;    // pinned mirror
     uint32_t cap = 0;
     uint32_t step = 0;
+    uint32_t marked_n = 0;      // training images at the last untrained-cell marking
     Pcg32 rng{0, 0};
 };
 
@@ -47,6 +48,32 @@ constexpr float kMinOpticalThickness = 0.01f;  // NERF_MIN_OPTICAL_THICKNESS (te
 __global__ void dg_fill_kernel(float* p, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.0f;
+}
+
+// mark_untrained_density_grid (testbed_nerf.cu:391-431): -1 for a cell that
+// no training camera sees (centre widened by the half-diagonal against each
+// camera's centred frustum), 0 for a seen cell that was untrained (or always
+// with `clear`); one thread per cell, Morton order, one cascade
+__global__ void dg_mark_kernel(float* __restrict__ grid, uint32_t G, const TrainImage* __restrict__ meta,
+                               uint32_t n_images, float half_x, float half_y, uint32_t clear) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= G * G * G) return;
+    const float x = (float)compact3(i), y = (float)compact3(i >> 1), z = (float)compact3(i >> 2);
+    const float px = ((x + 0.5f) / (float)G - 0.5f) * 1.0f + 0.5f;
+    const float py = ((y + 0.5f) / (float)G - 0.5f) * 1.0f + 0.5f;
+    const float pz = ((z + 0.5f) / (float)G - 0.5f) * 1.0f + 0.5f;
+    const float radius = 0.5f * 1.73205080757f * 1.0f / (float)G;
+    bool seen = false;
+    for (uint32_t j = 0; j < n_images && !seen; ++j) {
+        const TrainImage& c = meta[j];
+        const float lx = px - c.M[3], ly = py - c.M[7], lz = pz - c.M[11];
+        const float cx = lx * c.M[0] + ly * c.M[4] + lz * c.M[8];
+        const float cy = lx * c.M[1] + ly * c.M[5] + lz * c.M[9];
+        const float cz = lx * c.M[2] + ly * c.M[6] + lz * c.M[10];
+        seen = cz > 0.0f && fabsf(cx) - radius < cz / c.fx * half_x && fabsf(cy) - radius < cz / c.fy * half_y;
+    }
+    const float g = grid[i];
+    if (clear || (g < 0.0f) != !seen) grid[i] = seen ? 0.0f : -1.0f;
 }
 
 __global__ void dg_samples_kernel(uint32_t n, uint32_t step, float thresh, uint32_t G, const float* __restrict__ grid,
@@ -223,6 +250,15 @@ insta_status update_density_grid(insta_ctx* ctx, uint32_t slot, uint32_t n_frame
     if (d->step == 0) {
         d->rng = Pcg32::seeded(c.seed);
         hipLaunchKernelGGL(dg_fill_kernel, cgrid, block, 0, stream, d->grid, cells);
+    }
+    // cells no training camera sees, on the first update and whenever the
+    // number of training images changed (the reference keys this to
+    // training step 0, where its first update runs, testbed_nerf.cu:2795-2806)
+    const TrainSetView tv = train_set_view(ctx);
+    if (tv.n_images > 0 && (d->step == 0 || tv.n_images != d->marked_n)) {
+        hipLaunchKernelGGL(dg_mark_kernel, cgrid, block, 0, stream, d->grid, G, tv.meta, tv.n_images,
+                           (float)tv.width * 0.5f, (float)tv.height * 0.5f, d->step == 0 ? 1u : 0u);
+        d->marked_n = tv.n_images;
     }
     hipLaunchKernelGGL(dg_fill_kernel, cgrid, block, 0, stream, d->tmp, cells);
     if (c.n_uniform)

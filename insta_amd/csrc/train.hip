@@ -52,13 +52,6 @@
 
 namespace insta {
 
-struct TrainImage {
-    float M[12];
-    float fx, fy, cx, cy;
-    int32_t frame;
-    uint32_t pad[3];
-};
-
 enum : uint32_t {
     kTRays = 0,      // rays kept (ray list length)
     kTSamples = 1,   // samples reserved
@@ -113,10 +106,15 @@ struct TrainState {
     uint32_t n_images = 0, width = 0, height = 0;
     float4* images = nullptr;
     TrainImage* meta = nullptr;
+    // supervision of the training set (nullptr: none): mask id and
+    // camera-axis depth per pixel, [n][h][w]
+    uint8_t* seg = nullptr;
+    float* depth = nullptr;
     std::vector<int32_t> frames;
     // workspace
     uint32_t ray_cap = 0, sample_cap = 0, compact_cap = 0;
     float4* ray_d = nullptr;
+    float4* ray_o = nullptr;       // origin, w = length of the unnormalised direction
     int32_t* ray_frame = nullptr;
     uint32_t *ray_img = nullptr, *ray_pix = nullptr, *ray_src = nullptr, *ray_base = nullptr, *ray_cnt = nullptr;
     float4 *s_pos = nullptr, *s_cpos = nullptr, *s_cdir = nullptr, *s_out = nullptr, *s_dout = nullptr;
@@ -184,7 +182,7 @@ struct RayArgs {
 };
 
 INSTA_DEV bool train_ray(const RayArgs& a, uint32_t i, uint32_t n_rays, uint64_t rays_total, Pcg32 rng,
-                         uint32_t& img, uint32_t& pix, V3& o, V3& d, float& t0, RenderArgs& ra) {
+                         uint32_t& img, uint32_t& pix, V3& o, V3& d, float& dn, float& t0, RenderArgs& ra) {
     img = (uint32_t)((((uint64_t)i + rays_total) * a.n_images / n_rays) % a.n_images);
     rng.advance((uint64_t)i * kRaySeedStride);
     const float x = rng.next_float();
@@ -198,7 +196,7 @@ INSTA_DEV bool train_ray(const RayArgs& a, uint32_t i, uint32_t n_rays, uint64_t
     const V3 d0 = v3((x - c.cx) * (float)a.width / c.fx, (y - c.cy) * (float)a.height / c.fy, 1.0f);
     d = v3(c.M[0] * d0.x + c.M[1] * d0.y + c.M[2] * d0.z, c.M[4] * d0.x + c.M[5] * d0.y + c.M[6] * d0.z,
            c.M[8] * d0.x + c.M[9] * d0.y + c.M[10] * d0.z);
-    const float dn = norm(d);
+    dn = norm(d);
     d = v3(d.x / dn, d.y / dn, d.z / dn);
     o = v3(c.M[3], c.M[7], c.M[11]);
     ra = a.box;
@@ -211,6 +209,7 @@ INSTA_DEV bool train_ray(const RayArgs& a, uint32_t i, uint32_t n_rays, uint64_t
 
 __global__ __launch_bounds__(256) void train_rays_kernel(RayArgs a, uint32_t n_rays, uint64_t rays_total,
                                                          Pcg32 rng, float4* __restrict__ ray_d,
+                                                         float4* __restrict__ ray_o,
                                                          int32_t* __restrict__ ray_frame,
                                                          uint32_t* __restrict__ ray_img,
                                                          uint32_t* __restrict__ ray_pix,
@@ -225,9 +224,9 @@ __global__ __launch_bounds__(256) void train_rays_kernel(RayArgs a, uint32_t n_r
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t img = 0, pix = 0, count = 0;
     V3 o = v3(0, 0, 0), d = v3(0, 0, 1), idir = v3(0, 0, 1);
-    float t0 = 0.0f;
+    float t0 = 0.0f, dn = 1.0f;
     RenderArgs ra = a.box;
-    if (i < n_rays && train_ray(a, i, n_rays, rays_total, rng, img, pix, o, d, t0, ra)) {
+    if (i < n_rays && train_ray(a, i, n_rays, rays_total, rng, img, pix, o, d, dn, t0, ra)) {
         idir = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
         // one iteration per sample or empty-voxel skip; sample t values to
         // step-major scratch (t_buf[j * n_rays + i])
@@ -251,6 +250,7 @@ __global__ __launch_bounds__(256) void train_rays_kernel(RayArgs a, uint32_t n_r
     const uint32_t r = wave_append(&counters[kTRays], keep);
     if (!keep) return;
     ray_d[r] = make_float4(d.x, d.y, d.z, 0.0f);
+    ray_o[r] = make_float4(o.x, o.y, o.z, dn);
     ray_frame[r] = a.meta[img].frame;
     ray_img[r] = img;
     ray_pix[r] = pix;
@@ -431,8 +431,11 @@ __global__ __launch_bounds__(256) void train_forward_kernel(const float* __restr
 
 struct LossArgs {
     const float4* images;
+    const uint8_t* seg;   // mask ids (nullptr: no masks)
+    const float* depth;   // camera-axis depth, 0 = none (nullptr: no depth images)
     uint32_t width, height;
     float dt, density_bias, loss_scale_per_ray, inv_rays;
+    float seg_weight, depth_weight;
 };
 
 INSTA_DEV float sigmoid(float x) { return 1.0f / (1.0f + expf(-x)); }
@@ -511,6 +514,8 @@ __global__ __launch_bounds__(256) void train_loss_kernel(LossArgs a, Pcg32 rng0,
                                                          const uint32_t* __restrict__ ray_pix,
                                                          const uint32_t* __restrict__ ray_base,
                                                          const uint32_t* __restrict__ ray_cnt,
+                                                         const float4* __restrict__ ray_o,
+                                                         const float4* __restrict__ s_pos,
                                                          const float4* __restrict__ s_out,
                                                          uint32_t* __restrict__ c_idx, float4* __restrict__ c_dout,
                                                          uint32_t compact_cap, uint32_t* __restrict__ counters) {
@@ -519,6 +524,9 @@ __global__ __launch_bounds__(256) void train_loss_kernel(LossArgs a, Pcg32 rng0,
     float ray_loss = 0.0f;
     uint32_t used = 0, base = 0;
     float acc[3] = {0.0f, 0.0f, 0.0f}, grad[3] = {0.0f, 0.0f, 0.0f};
+    // depth supervision: the composited depth and dL/dD (0: none for this ray)
+    float depth_ray = 0.0f, dgrad = 0.0f;
+    V3 org = v3(0.0f, 0.0f, 0.0f);
     if (valid) {
         base = ray_base[r];
         const uint32_t cnt = ray_cnt[r];
@@ -530,7 +538,8 @@ __global__ __launch_bounds__(256) void train_loss_kernel(LossArgs a, Pcg32 rng0,
         // the random background goes to linear, is blended behind the
         // premultiplied linear texel (read_rgba of an 8-bit sRGB image,
         // common_device.cuh:658-671), then target and background return to sRGB
-        const float4 tex = a.images[(size_t)ray_img[r] * a.width * a.height + ray_pix[r]];
+        const size_t texel = (size_t)ray_img[r] * a.width * a.height + ray_pix[r];
+        const float4 tex = a.images[texel];
         const float texc[3] = {tex.x, tex.y, tex.z};
         float bg[3], target[3];
         for (int c = 0; c < 3; ++c) {
@@ -538,6 +547,21 @@ __global__ __launch_bounds__(256) void train_loss_kernel(LossArgs a, Pcg32 rng0,
             bg[c] = linear_to_srgb(bg_lin);
             target[c] = linear_to_srgb(srgb_to_linear(texc[c]) * tex.w + (1.0f - tex.w) * bg_lin);
         }
+        // segmentation and depth supervision (testbed_nerf.cu:1476-1497): mask
+        // ids 120/130/140 weight the colour gradient; ids <= 90 or > 250 and
+        // near-transparent texels get no depth term; the target is the depth
+        // texel times the unnormalised direction's length (0 texel: none)
+        float seg_w = 1.0f, lam = a.depth_weight;
+        if (a.seg) {
+            const uint32_t id = a.seg[texel];
+            if (id == 120u || id == 130u || id == 140u) seg_w = a.seg_weight;
+            if (id <= 90u || id > 250u) lam = 0.0f;
+            if (tex.w < 0.001f) lam = 0.0f;
+        }
+        const float4 ro = ray_o[r];
+        org = v3(ro.x, ro.y, ro.z);
+        const float target_depth = ro.w * ((lam > 0.0f && a.depth) ? a.depth[texel] : -1.0f);
+        const bool supervise_depth = target_depth > 0.0f;
         float T = 1.0f;
         for (; used < cnt; ++used) {
             if (T < 1e-4f) break;
@@ -548,8 +572,14 @@ __global__ __launch_bounds__(256) void train_loss_kernel(LossArgs a, Pcg32 rng0,
             acc[0] = acc[0] + w * sigmoid(o.x);
             acc[1] = acc[1] + w * sigmoid(o.y);
             acc[2] = acc[2] + w * sigmoid(o.z);
+            if (supervise_depth) {
+                const float4 p = s_pos[base + used];
+                depth_ray = depth_ray + w * norm(v3(p.x, p.y, p.z) - org);
+            }
             T = T * (1.0f - alpha);  // = transmit(T, o.w, ...), which the rounds use
         }
+        // L1 depth loss (depth_loss_type's default, testbed_nerf.cu:133-139)
+        if (supervise_depth) dgrad = copysignf(1.0f, depth_ray - target_depth) * lam;
         if (used == cnt) {
             for (int c = 0; c < 3; ++c) acc[c] = acc[c] + T * bg[c];
         }
@@ -558,7 +588,7 @@ __global__ __launch_bounds__(256) void train_loss_kernel(LossArgs a, Pcg32 rng0,
             const float diff = acc[c] - target[c];
             const float ad = fabsf(diff);
             const float l = ad > 0.1f ? ad - 0.05f : (0.5f / 0.1f) * diff * diff;
-            grad[c] = (ad > 0.1f ? (diff > 0.0f ? 1.0f : -1.0f) : diff / 0.1f) / 5.0f;
+            grad[c] = (ad > 0.1f ? (diff > 0.0f ? 1.0f : -1.0f) : diff / 0.1f) / 5.0f * seg_w;
             ray_loss += l / 5.0f;
         }
         ray_loss = ray_loss / 3.0f * a.inv_rays;
@@ -569,7 +599,7 @@ __global__ __launch_bounds__(256) void train_loss_kernel(LossArgs a, Pcg32 rng0,
     if (valid) {
         const float ls = a.loss_scale_per_ray;
         float T = 1.0f;
-        float acc2[3] = {0.0f, 0.0f, 0.0f};
+        float acc2[3] = {0.0f, 0.0f, 0.0f}, depth2 = 0.0f;
         for (uint32_t j = 0; j < n_c; ++j) {
             const float4 o = s_out[base + j];
             const float sg[3] = {sigmoid(o.x), sigmoid(o.y), sigmoid(o.z)};
@@ -583,6 +613,13 @@ __global__ __launch_bounds__(256) void train_loss_kernel(LossArgs a, Pcg32 rng0,
             for (int c = 0; c < 3; ++c) {
                 drgb[c] = ls * (w * grad[c] * (sg[c] * (1.0f - sg[c])));
                 dsig = dsig + grad[c] * (T * sg[c] - (acc[c] - acc2[c]));
+            }
+            if (dgrad != 0.0f) {
+                // dD/dsigma_j / dt = T_{j+1} depth_j - (depth suffix after j)
+                const float4 p = s_pos[base + j];
+                const float dj = norm(v3(p.x, p.y, p.z) - org);
+                depth2 = depth2 + w * dj;
+                dsig = dsig + dgrad * (T * dj - (depth_ray - depth2));
             }
             c_idx[cb + j] = base + j;
             c_dout[cb + j] = make_float4(drgb[0], drgb[1], drgb[2], ls * (sigma * (a.dt * dsig)));
@@ -946,7 +983,7 @@ bool train_alloc(T*& p, size_t bytes) {
 }
 
 void free_workspace_train(TrainState* ts) {
-    dev_free(ts->ray_d); dev_free(ts->ray_frame); dev_free(ts->ray_img); dev_free(ts->ray_pix);
+    dev_free(ts->ray_d); dev_free(ts->ray_o); dev_free(ts->ray_frame); dev_free(ts->ray_img); dev_free(ts->ray_pix);
     dev_free(ts->ray_src); dev_free(ts->ray_base); dev_free(ts->ray_cnt);
     dev_free(ts->s_pos); dev_free(ts->s_cpos); dev_free(ts->s_cdir); dev_free(ts->s_out); dev_free(ts->s_hit);
     dev_free(ts->c_idx); dev_free(ts->c_dout); dev_free(ts->act); dev_free(ts->dact); dev_free(ts->part);
@@ -969,7 +1006,8 @@ insta_status grow_train_workspace(insta_ctx* ctx, TrainState* ts, uint32_t n_ray
     const int DIN = (int)(ctx->cfg.n_levels * 2 + ctx->cfg.n_extra_dims);
     const size_t act_rows = (size_t)DIN + 224, n_mlp = ctx->layout.n_mlp_params;
     bool ok = true;
-    ok = ok && train_alloc(ts->ray_d, rc * sizeof(float4)) && train_alloc(ts->ray_frame, rc * sizeof(int32_t));
+    ok = ok && train_alloc(ts->ray_d, rc * sizeof(float4)) && train_alloc(ts->ray_o, rc * sizeof(float4)) &&
+         train_alloc(ts->ray_frame, rc * sizeof(int32_t));
     ok = ok && train_alloc(ts->ray_img, rc * 4) && train_alloc(ts->ray_pix, rc * 4) && train_alloc(ts->ray_src, rc * 4);
     ok = ok && train_alloc(ts->ray_base, rc * 4) && train_alloc(ts->ray_cnt, rc * 4);
     ok = ok && train_alloc(ts->s_pos, sc * sizeof(float4)) && train_alloc(ts->s_cpos, sc * sizeof(float4));
@@ -1221,7 +1259,8 @@ void free_train(insta_ctx* ctx) {
     TrainState* ts = ctx->train;
     if (!ts) return;
     free_workspace_train(ts);
-    dev_free(ts->images); dev_free(ts->meta); dev_free(ts->grad); dev_free(ts->grad_h);
+    dev_free(ts->images); dev_free(ts->meta); dev_free(ts->seg); dev_free(ts->depth);
+    dev_free(ts->grad); dev_free(ts->grad_h);
     dev_free(ts->adam_m); dev_free(ts->adam_v); dev_free(ts->counters); dev_free(ts->net_extra);
     if (ctx->net_src == ts->ema_infer) ctx->net_src = nullptr;
     dev_free(ts->ema); dev_free(ts->ema_infer);
@@ -1265,6 +1304,8 @@ insta_status set_training_set(insta_ctx* ctx, uint32_t n_images, uint32_t width,
     TrainState* ts = ctx->train;
     dev_free(ts->images);
     dev_free(ts->meta);
+    dev_free(ts->seg);    // a new set starts without supervision
+    dev_free(ts->depth);
     const size_t px = (size_t)n_images * width * height;
     if (!train_alloc(ts->images, px * sizeof(float4)) || !train_alloc(ts->meta, n_images * sizeof(TrainImage))) {
         set_error("training image allocation failed");
@@ -1289,6 +1330,22 @@ insta_status set_training_set(insta_ctx* ctx, uint32_t n_images, uint32_t width,
     ts->width = width;
     ts->height = height;
     ts->rays_total = 0;
+    return INSTA_OK;
+}
+
+insta_status set_training_supervision(insta_ctx* ctx, const uint8_t* seg, const float* depth, hipStream_t stream) {
+    TrainState* ts = ctx->train;
+    const size_t px = (size_t)ts->n_images * ts->width * ts->height;
+    dev_free(ts->seg);
+    dev_free(ts->depth);
+    if ((seg && !train_alloc(ts->seg, px)) || (depth && !train_alloc(ts->depth, px * sizeof(float)))) {
+        dev_free(ts->seg);
+        dev_free(ts->depth);
+        set_error("training supervision allocation failed");
+        return INSTA_ERR_OOM;
+    }
+    if (seg) INSTA_HIP(hipMemcpyAsync(ts->seg, seg, px, hipMemcpyDeviceToDevice, stream));
+    if (depth) INSTA_HIP(hipMemcpyAsync(ts->depth, depth, px * sizeof(float), hipMemcpyDeviceToDevice, stream));
     return INSTA_OK;
 }
 
@@ -1336,7 +1393,8 @@ insta_status train_step(insta_ctx* ctx, const insta_train_config& tc, insta_trai
     ra.max_samples = cap;
     ra.fixed_slot = tc.occupancy_slot;
     hipLaunchKernelGGL(train_rays_kernel, dim3((n_rays + 255) / 256), dim3(256), 0, stream, ra, n_rays, ts->rays_total,
-                       rng, ts->ray_d, ts->ray_frame, ts->ray_img, ts->ray_pix, ts->ray_src, ts->ray_base, ts->ray_cnt,
+                       rng, ts->ray_d, ts->ray_o, ts->ray_frame, ts->ray_img, ts->ray_pix, ts->ray_src, ts->ray_base,
+                       ts->ray_cnt,
                        ts->s_pos, ts->t_buf, ts->ray_T, ts->ray_alive, ts->counters);
     INSTA_HIP(hipGetLastError());
     hipLaunchKernelGGL(train_count_kernel, dim3(1), dim3(64), 0, stream, ts->counters);
@@ -1361,6 +1419,10 @@ insta_status train_step(insta_ctx* ctx, const insta_train_config& tc, insta_trai
     }
     LossArgs la;
     la.images = ts->images;
+    la.seg = ts->seg;
+    la.depth = ts->depth;
+    la.seg_weight = tc.seg_weight;
+    la.depth_weight = tc.depth_weight;
     la.width = ts->width;
     la.height = ts->height;
     la.dt = ctx->cfg.dt;
@@ -1368,7 +1430,8 @@ insta_status train_step(insta_ctx* ctx, const insta_train_config& tc, insta_trai
     la.loss_scale_per_ray = tc.loss_scale / (float)n_rays;
     la.inv_rays = 1.0f / (float)n_rays;
     hipLaunchKernelGGL(train_loss_kernel, dim3((n_rays + 255) / 256), dim3(256), 0, stream, la, rng, ts->ray_src,
-                       ts->ray_img, ts->ray_pix, ts->ray_base, ts->ray_cnt, ts->s_out, ts->c_idx, ts->c_dout,
+                       ts->ray_img, ts->ray_pix, ts->ray_base, ts->ray_cnt, ts->ray_o, ts->s_pos, ts->s_out,
+                       ts->c_idx, ts->c_dout,
                        ts->compact_cap, ts->counters);
     INSTA_HIP(hipGetLastError());
     hipLaunchKernelGGL(train_compact_count_kernel, dim3(1), dim3(64), 0, stream, ts->counters, ts->compact_cap);
@@ -1449,6 +1512,12 @@ void reset_inference_average(insta_ctx* ctx) {
 
 bool train_has_gradients(const insta_ctx* ctx) { return ctx->train && ctx->train->have_grad; }
 bool train_has_set(const insta_ctx* ctx) { return ctx->train && ctx->train->n_images > 0; }
+const std::vector<int32_t>& train_frames(const insta_ctx* ctx) { return ctx->train->frames; }
+TrainSetView train_set_view(const insta_ctx* ctx) {
+    const TrainState* ts = ctx->train;
+    if (!ts) return TrainSetView{};
+    return TrainSetView{ts->meta, ts->n_images, ts->width, ts->height};
+}
 uint32_t train_step_count(const insta_ctx* ctx) { return ctx->train ? ctx->train->training_step : 0u; }
 
 insta_status get_gradients(insta_ctx* ctx, float* out, hipStream_t stream) {

@@ -25,9 +25,15 @@ Follows the reference's loaders for the parts this path uses:
     a face takes the region of the highest FlameMasks id among the regions
     that hold one of its vertices, else TORSO.
 
+  * segmentation masks and depth images (nerf_loader.cu:773-823, 992-1012):
+    a frame's "seg_mask_path" image (its first channel is the mask id); with
+    "integer_depth_scale" > 0 at the file level, a frame's "depth_path" image
+    (16-bit PNG, or .bin: one uint32 per pixel kept as uint16) times that
+    scale times the scene scale; 0 where a frame has none.
+
 Not covered: camera distortion and rolling shutter, EXR images, alpha /
-dynamic-mask side images, depth and segmentation supervision, ray files,
-dataset caching and the shuffled subset of training images.  Without a
+dynamic-mask side images, ray files and dataset caching.  The shuffled
+subset of training images and its periodic reload are insta_amd.training's.  Without a
 mesh_masks folder every face is TORSO (the plain per-triangle frame) unless
 per-face ids are passed.  Images are returned as stored (value / 255), with
 no sRGB-to-linear conversion, as the training step takes its targets.
@@ -153,22 +159,90 @@ class Dataset:
     expressions: list = field(default_factory=list)   # per image: FLAME parameters (float32)
     cameras: list = field(default_factory=list)       # per image: InstaCamera
     image_paths: list = field(default_factory=list)
+    seg_paths: list = field(default_factory=list)     # per image: mask image path or None
+    depth_paths: list = field(default_factory=list)   # per image: depth image path or None
+    depth_scales: list = field(default_factory=list)  # per image: metres-to-scene factor, < 0: no depth
+    file_counts: list = field(default_factory=list)   # images per transforms file, in order
     resolution: tuple = (0, 0)         # (width, height)
     offset: tuple = (0.5, 0.5, 0.5)
     scale: float = 1.0
 
-    def images(self) -> np.ndarray:
-        """[n, h, w, 4] float32 RGBA in [0, 1] (PIL decodes the files)."""
+    def _ids(self, ids):
+        return range(len(self.image_paths)) if ids is None else list(ids)
+
+    def images(self, ids=None) -> np.ndarray:
+        """[n, h, w, 4] float32 RGBA in [0, 1] of images `ids` (default all;
+        PIL decodes the files)."""
         from PIL import Image
         w, h = self.resolution
-        out = np.empty((len(self.image_paths), h, w, 4), np.float32)
-        for i, p in enumerate(self.image_paths):
+        ids = self._ids(ids)
+        out = np.empty((len(ids), h, w, 4), np.float32)
+        for k, i in enumerate(ids):
+            p = self.image_paths[i]
             with Image.open(p) as im:
                 a = np.asarray(im.convert("RGBA"), dtype=np.float32)
             if a.shape[:2] != (h, w):
                 raise ValueError(f"{p}: {a.shape[1]}x{a.shape[0]}, expected {w}x{h}")
-            out[i] = a / 255.0
+            out[k] = a / 255.0
         return out
+
+    def seg_masks(self, ids=None) -> np.ndarray | None:
+        """[n, h, w] uint8 mask ids of images `ids` (the first channel of each
+        mask image), None when none of them has a mask.  The device takes
+        masks for every image of a training set or for none."""
+        from PIL import Image
+        ids = self._ids(ids)
+        have = [self.seg_paths[i] is not None for i in ids]
+        if not any(have):
+            return None
+        if not all(have):
+            raise ValueError("segmentation masks for some images of the set only")
+        w, h = self.resolution
+        out = np.empty((len(ids), h, w), np.uint8)
+        for k, i in enumerate(ids):
+            with Image.open(self.seg_paths[i]) as im:
+                a = np.asarray(im.convert("RGBA"), dtype=np.uint8)
+            if a.shape[:2] != (h, w):
+                raise ValueError(f"{self.seg_paths[i]}: wrong resolution")
+            out[k] = a[..., 0]
+        return out
+
+    def depth_images(self, ids=None) -> np.ndarray | None:
+        """[n, h, w] float32 camera-axis depth in scene units of images `ids`
+        (raw value x integer_depth_scale x scene scale), 0 where an image has
+        no depth file; None when no image's file sets integer_depth_scale."""
+        ids = self._ids(ids)
+        if not any(self.depth_scales[i] >= 0.0 for i in ids):
+            return None
+        w, h = self.resolution
+        out = np.zeros((len(ids), h, w), np.float32)
+        for k, i in enumerate(ids):
+            if self.depth_paths[i] is not None and self.depth_scales[i] > 0.0:
+                out[k] = read_depth(self.depth_paths[i], w, h).astype(np.float32) * np.float32(self.depth_scales[i])
+        return out
+
+
+def read_depth(path, width: int, height: int) -> np.ndarray:
+    """[h, w] uint16 raw depth: a .bin file of one little-endian uint32 per
+    pixel kept modulo 2^16 (nerf_loader.cu:794-806), else an image read as
+    one 16-bit channel (stbi_load_16: 8-bit values v become 257 v)."""
+    path = Path(path)
+    if ".bin" in path.name:
+        raw = np.fromfile(path, dtype="<u4", count=width * height)
+        if raw.size != width * height:
+            raise ValueError(f"{path}: expected {width * height} pixels")
+        return (raw & 0xFFFF).astype(np.uint16).reshape(height, width)
+    from PIL import Image
+    with Image.open(path) as im:
+        if im.mode in ("I;16", "I;16L", "I;16B", "I"):
+            a = np.asarray(im).astype(np.int64)
+        elif im.mode == "L":
+            a = np.asarray(im).astype(np.int64) * 257
+        else:
+            raise ValueError(f"{path}: depth images are one 8- or 16-bit channel, not {im.mode}")
+    if a.shape != (height, width):
+        raise ValueError(f"{path}: depth image has the wrong resolution")
+    return np.clip(a, 0, 65535).astype(np.uint16)
 
 
 def _image_size(path: Path) -> tuple[int, int]:
@@ -215,6 +289,10 @@ def load_transforms(paths, masks: np.ndarray | None = None, mesh_masks_dir=None)
             offset = (hi + lo) * 0.5 * -scale + 0.5
         ds.scale, ds.offset = scale, tuple(float(x) for x in offset)
         pp_file = _principal_point(doc, (0.5, 0.5))
+        # the reference multiplies by the scene scale (m.depth_scale * result.scale)
+        depth_scale = float(doc["integer_depth_scale"]) * scale if "integer_depth_scale" in doc else -1.0
+        depth_on = bool(doc.get("enable_depth_loading", True))
+        n_before = len(ds.cameras)
         for fr in frames:
             img = base / fr["file_path"]
             if img.suffix == "":
@@ -243,6 +321,12 @@ def load_transforms(paths, masks: np.ndarray | None = None, mesh_masks_dir=None)
                                   else np.zeros(0, np.float32))
             ds.cameras.append(cam)
             ds.image_paths.append(img)
+            seg = base / fr["seg_mask_path"] if "seg_mask_path" in fr else None
+            ds.seg_paths.append(seg if seg is not None and seg.exists() else None)
+            dp = base / fr["depth_path"] if "depth_path" in fr and depth_on and depth_scale > 0.0 else None
+            ds.depth_paths.append(dp if dp is not None and dp.exists() else None)
+            ds.depth_scales.append(depth_scale)
+        ds.file_counts.append(len(ds.cameras) - n_before)
     if not ds.cameras:
         raise ValueError("no frames found in the transforms files")
     return ds

@@ -49,6 +49,8 @@
 //   4  march: the one-step start jitter of advance_pos_nerf dropped
 //   5  training target: background blended in stored (sRGB) values, not linear
 //   6  render buffer: spp accumulation weights 1/spp instead of 1/(spp + 1)
+//   7  depth supervision: target depth not scaled by the unnormalised ray direction's length
+//   8  depth supervision: the depth suffix left out of the density gradient
 // The default build (ORC_MUTANT 0) has none of them.
 #ifndef ORC_MUTANT
 #define ORC_MUTANT 0
@@ -576,6 +578,7 @@ struct orc_scene {
     // density-grid EMA (orc_update_density_grid)
     std::vector<float> dg_grid;
     uint32_t dg_step = 0;
+    uint32_t dg_marked_n = 0;     // training images at the last untrained-cell marking
     uint64_t dg_rng_state = 0, dg_rng_inc = 0;
     // training set (orc_set_training_set)
     uint32_t tr_n = 0, tr_w = 0, tr_h = 0;
@@ -583,6 +586,9 @@ struct orc_scene {
     std::vector<insta_camera> tr_cams;
     std::vector<int32_t> tr_frames;
     int32_t tr_occ_slot = -2;     // >= -1: every training ray marches this slot's grid
+    std::vector<uint8_t> tr_seg;  // [n][h][w] mask ids, empty: no masks (orc_set_training_supervision)
+    std::vector<float> tr_depth;  // [n][h][w] camera-axis depth, 0 = none; empty: no depth images
+    double tr_seg_weight = 1.0, tr_depth_weight = 1.5;
     std::vector<float> frame_extra;  // [max_frames][n_extra_dims] (Core::load_meshes' extra_dims_gpu)
 };
 
@@ -1611,10 +1617,17 @@ int orc_render_fp64(const orc_scene* s, const insta_camera* cam, uint32_t width,
 // 3176-3400): the FP32 ray generation, march and deformation of the device
 // kernels (identical canonical samples), then network, loss and the backward
 // pass in FP64.  Gradients are those of the objective
-//   L = (1 / n_rays) * sum_rays sum_channels Huber_0.1(rgb - target) / 5,
+//   L = (1 / n_rays) * sum_rays [ s_r * sum_channels Huber_0.1(rgb - target) / 5
+//                                 + l_r * |D_r - D*_r| ],
 // i.e. the per-channel gradient of compute_loss_kernel_train_nerf
 // (1305-1643) without the loss scale; the reported loss is the per-ray
-// channel mean, as the reference reports it.
+// channel mean of the colour term without s_r, as the reference reports it.
+// With segmentation masks and depth images (1476-1497): s_r is the seg
+// weight for mask ids 120, 130, 140, else 1; l_r is the depth weight, 0 for
+// mask ids <= 90 or > 250 or texel alpha < 0.001 (only where masks are set),
+// and 0 where the target D*_r = |unnormalised direction| * depth texel is not
+// positive; D_r = sum_j w_j |x_j - o| over the composited samples (posed
+// positions x_j, ray origin o), L1 (the default depth_loss_type).
 // ===========================================================================
 namespace {
 
@@ -1667,6 +1680,7 @@ inline double linear_to_srgb_d(double c) { return c < 0.0031308 ? 12.92 * c : 1.
 struct TrainRay {
     uint32_t src, img, pix;
     V3 o, d;
+    float dn;  // length of the unnormalised direction (camera z component 1)
     std::vector<V3> samples;
     float bg[3];
 };
@@ -1703,6 +1717,7 @@ bool train_ray(const orc_scene* s, uint32_t i, uint32_t n_rays, uint64_t rays_to
     const V3 o = v3(M[3], M[7], M[11]);
     r.o = o;
     r.d = d;
+    r.dn = dn;
     r.samples.clear();
     const float tb = box_ray_tmin(s->render_lo, s->render_hi, o, d);
     if (tb == FLT_MAX) return false;
@@ -1990,15 +2005,31 @@ int orc_set_training_occupancy(orc_scene* s, int32_t slot) {
 int orc_set_training_set(orc_scene* s, uint32_t n_images, uint32_t width, uint32_t height, const float* rgba,
                          const insta_camera* cams, const int32_t* frames) {
     for (uint32_t i = 0; i < n_images; ++i)
-        if (frames[i] < 0 || (uint32_t)frames[i] >= s->frames.size() || !s->frames[frames[i]].built ||
-            s->grids[frames[i] + 1].empty())
-            return 1;
+        if (frames[i] < 0 || (uint32_t)frames[i] >= s->frames.size() || !s->frames[frames[i]].built) return 1;
     s->tr_n = n_images;
     s->tr_w = width;
     s->tr_h = height;
     s->tr_images.assign(rgba, rgba + (size_t)n_images * width * height * 4);
     s->tr_cams.assign(cams, cams + n_images);
     s->tr_frames.assign(frames, frames + n_images);
+    s->tr_seg.clear();
+    s->tr_depth.clear();
+    return 0;
+}
+
+// Segmentation masks (mask id per pixel: the red channel of the reference's
+// RGBA mask image) and depth images (depth along the camera axis in scene
+// units, 0 = none) of the training set, either nullable; weights are
+// seg_mask_supervision_lambda and depth_supervision_lambda (testbed.h:622-623,
+// src/core.cu:187, 208)
+int orc_set_training_supervision(orc_scene* s, const uint8_t* seg, const float* depth, double seg_weight,
+                                 double depth_weight) {
+    if (s->tr_n == 0) return 1;
+    const size_t n = (size_t)s->tr_n * s->tr_w * s->tr_h;
+    if (seg) s->tr_seg.assign(seg, seg + n); else s->tr_seg.clear();
+    if (depth) s->tr_depth.assign(depth, depth + n); else s->tr_depth.clear();
+    s->tr_seg_weight = seg_weight;
+    s->tr_depth_weight = depth_weight;
     return 0;
 }
 
@@ -2026,6 +2057,13 @@ int orc_train_grads(const orc_scene* s, uint32_t n_rays, uint32_t seed, uint64_t
         c.density_hidden_layers != 1 || c.rgb_hidden_layers != 2 || c.density_out != 16 ||
         c.n_levels * 2 + c.n_extra_dims > 64)
         return 1;
+    // the grid(s) the rays march must exist
+    if (s->tr_occ_slot >= -1) {
+        if (s->grids[s->tr_occ_slot + 1].empty()) return 1;
+    } else {
+        for (const int32_t f : s->tr_frames)
+            if (s->grids[f + 1].empty()) return 1;
+    }
     Pcg32 rng0 = Pcg32::seeded(seed);
     rng0.advance(rays_total * kRaySeedStride);
     const uint64_t np = s->layout.n_params;
@@ -2095,19 +2133,46 @@ int orc_train_grads(const orc_scene* s, uint32_t n_rays, uint32_t seed, uint64_t
         n_used += used;
         if (used == cnt)
             for (int k = 0; k < 3; ++k) acc[k] += T * bg[k];
+        // segmentation and depth supervision (testbed_nerf.cu:1476-1497)
+        const size_t texel = (size_t)r.img * s->tr_w * s->tr_h + r.pix;
+        double seg_w = 1.0, lam = s->tr_depth_weight;
+        if (!s->tr_seg.empty()) {
+            const int id = s->tr_seg[texel];
+            if (id == 120 || id == 130 || id == 140) seg_w = s->tr_seg_weight;
+            if (id <= 90 || id > 250) lam = 0.0;
+            if (tex[3] < 0.001f) lam = 0.0;
+        }
+        const double dscale = ORC_MUTANT == 7 ? 1.0 : (double)r.dn;
+        const double target_depth = dscale * ((lam > 0.0 && !s->tr_depth.empty()) ? (double)s->tr_depth[texel] : -1.0);
+        // per-sample depth |x_j - o| and the composited depth
+        std::vector<double> depth(target_depth > 0.0 ? used : 0);
+        double depth_ray = 0.0;
+        if (target_depth > 0.0) {
+            double Td = 1.0;
+            for (uint32_t j = 0; j < used; ++j) {
+                const double dx = (double)r.samples[j].x - r.o.x, dy = (double)r.samples[j].y - r.o.y,
+                             dz = (double)r.samples[j].z - r.o.z;
+                depth[j] = std::sqrt(dx * dx + dy * dy + dz * dz);
+                const double alpha = 1.0 - std::exp(-std::exp(act[j].out[3] + bias) * dt);
+                depth_ray += alpha * Td * depth[j];
+                Td *= 1.0 - alpha;
+            }
+        }
+        // L1 (testbed_nerf.cu:133-139): gradient copysign(1, D - D*)
+        const double dgrad = target_depth > 0.0 ? lam * std::copysign(1.0, depth_ray - target_depth) : 0.0;
         double grad[3], lsum = 0.0;
         for (int k = 0; k < 3; ++k) {
             // huber_loss(target, prediction, 0.1) / 5 (testbed_nerf.cu:141-157, 1299)
             const double diff = acc[k] - target[k], ad = std::fabs(diff);
             const double l = ad > kHuberDelta ? ad - 0.5 * kHuberDelta : (0.5 / kHuberDelta) * diff * diff;
-            grad[k] = (ad > kHuberDelta ? (diff > 0 ? 1.0 : -1.0) : diff / kHuberDelta) / 5.0;
+            grad[k] = seg_w * (ad > kHuberDelta ? (diff > 0 ? 1.0 : -1.0) : diff / kHuberDelta) / 5.0;
             lsum += l / 5.0;
         }
         rep += lsum / 3.0 * inv_rays;
-        obj += lsum * inv_rays;
+        obj += (seg_w * lsum + (target_depth > 0.0 ? lam * std::fabs(depth_ray - target_depth) : 0.0)) * inv_rays;
         if (!grads) continue;
         T = 1.0;
-        double acc2[3] = {0, 0, 0};
+        double acc2[3] = {0, 0, 0}, depth2 = 0.0;
         for (uint32_t j = 0; j < used; ++j) {
             const double* o = act[j].out;
             double sg[3];
@@ -2121,6 +2186,11 @@ int orc_train_grads(const orc_scene* s, uint32_t n_rays, uint32_t seed, uint64_t
             for (int k = 0; k < 3; ++k) {
                 dout[k] = inv_rays * w * grad[k] * sg[k] * (1.0 - sg[k]);
                 dsig += grad[k] * (T * sg[k] - (acc[k] - acc2[k]));
+            }
+            if (dgrad != 0.0) {
+                // d D / d sigma_j = dt (T_{j+1} depth_j - suffix_j)
+                depth2 += w * depth[j];
+                dsig += dgrad * (T * depth[j] - (ORC_MUTANT == 8 ? 0.0 : depth_ray - depth2));
             }
             dout[3] = inv_rays * sigma * dt * dsig;
             backward_d(s, act[j], dout, tg[tid].data());
@@ -2215,6 +2285,39 @@ float density_logit(const orc_scene* s, V3 x) {
     return out;
 }
 
+// mark_untrained_density_grid (testbed_nerf.cu:391-431, called at 2795-2806):
+// a cell no training camera sees (its centre, widened by the cell's
+// half-diagonal, outside every camera's centred frustum of half-extent
+// resolution / 2 / focal) is -1 (untrained: never sampled, never occupied);
+// a seen cell is 0 if it was untrained, or always when `clear`.  One cascade
+// (level 0).
+void mark_untrained(orc_scene* s, bool clear) {
+    const uint32_t G = s->cfg.grid_size, cells = G * G * G;
+    const float radius = 0.5f * 1.73205080757f * 1.0f / (float)G;
+    const float half_x = (float)s->tr_w * 0.5f, half_y = (float)s->tr_h * 0.5f;
+    for (uint32_t i = 0; i < cells; ++i) {
+        const float x = (float)compact3(i), y = (float)compact3(i >> 1), z = (float)compact3(i >> 2);
+        const float px = ((x + 0.5f) / (float)G - 0.5f) * 1.0f + 0.5f;
+        const float py = ((y + 0.5f) / (float)G - 0.5f) * 1.0f + 0.5f;
+        const float pz = ((z + 0.5f) / (float)G - 0.5f) * 1.0f + 0.5f;
+        int count = 0;
+        for (uint32_t j = 0; j < s->tr_n; ++j) {
+            const float* M = s->tr_cams[j].matrix;  // columns right, down, forward, position
+            const float lx = px - M[3], ly = py - M[7], lz = pz - M[11];
+            const float cx = lx * M[0] + ly * M[4] + lz * M[8];
+            const float cy = lx * M[1] + ly * M[5] + lz * M[9];
+            const float cz = lx * M[2] + ly * M[6] + lz * M[10];
+            if (cz > 0.0f && std::fabs(cx) - radius < cz / s->tr_cams[j].focal_px[0] * half_x &&
+                std::fabs(cy) - radius < cz / s->tr_cams[j].focal_px[1] * half_y) {
+                count = 1;
+                break;
+            }
+        }
+        float& g = s->dg_grid[i];
+        if (clear || (g < 0.0f) != (count <= 0)) g = count > 0 ? 0.0f : -1.0f;
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -2242,6 +2345,13 @@ int orc_update_density_grid(orc_scene* s, int32_t slot, uint32_t n_frames, float
         Pcg32 r = Pcg32::seeded(seed);
         s->dg_rng_state = r.state;
         s->dg_rng_inc = r.inc;
+    }
+    // cells no training camera sees, on the first update and whenever the
+    // number of training images changed (the reference keys this to training
+    // step 0, where its first update runs)
+    if (s->tr_n > 0 && (s->dg_step == 0 || s->tr_n != s->dg_marked_n)) {
+        mark_untrained(s, s->dg_step == 0);
+        s->dg_marked_n = s->tr_n;
     }
     const Pcg32 rng{s->dg_rng_state, s->dg_rng_inc};
     std::vector<float> pos((size_t)n * 3);

@@ -62,6 +62,7 @@ _SIGS = [
     ("orc_pcg32_draws", None, [C.c_uint64, C.c_uint64, C.c_uint32, _VP]),
     ("orc_train_grads", C.c_int, [_VP, C.c_uint32, C.c_uint32, C.c_uint64, _VP, _VP, _VP]),
     ("orc_set_training_occupancy", C.c_int, [_VP, C.c_int32]),
+    ("orc_set_training_supervision", C.c_int, [_VP, _VP, _VP, C.c_double, C.c_double]),
     ("orc_deform_positions", C.c_int, [_VP, _VP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64]),
     ("orc_update_density_grid", C.c_int, [_VP, C.c_int32, C.c_uint32, C.c_float, C.c_uint32, C.c_uint32, C.c_uint32,
                                           _VP, _VP]),
@@ -242,6 +243,16 @@ def set_training_set(orc: Oracle, images, cams, frames) -> None:
     arr = (InstaCamera * n)(*cams)
     fr = _c(frames, np.int32)
     assert orc.L.orc_set_training_set(orc.h, n, w, h, _p(im), C.addressof(arr), _p(fr)) == 0
+
+
+def set_training_supervision(orc: Oracle, seg=None, depth=None, seg_weight: float = 1.0,
+                             depth_weight: float = 1.5) -> None:
+    """Segmentation masks seg [n, h, w] uint8 (mask ids) and depth images
+    depth [n, h, w] float32 (camera-axis depth, 0 = none) of the training set
+    set last, either None; the weights of the segmentation and depth terms."""
+    sg = None if seg is None else _c(seg, np.uint8)
+    dp = None if depth is None else _c(depth, np.float32)
+    assert orc.L.orc_set_training_supervision(orc.h, _p(sg), _p(dp), float(seg_weight), float(depth_weight)) == 0
 
 
 def set_training_occupancy(orc: Oracle, slot: int) -> None:

@@ -91,6 +91,78 @@ def test_n_frames_offset_and_aabb(capture):
     assert ds3.scale == pytest.approx(0.25) and np.allclose(ds3.offset, [0.5, 0.5, 0.375])
 
 
+def _add_supervision(path, doc, rng, depth_files=("png", "bin", None)):
+    """Segmentation masks for every frame and depth files per frame (16-bit
+    PNG, .bin of uint32, or none); returns the written ids and raw depths by
+    mesh order."""
+    from PIL import Image
+    (path / "seg").mkdir(exist_ok=True)
+    (path / "depth").mkdir(exist_ok=True)
+    ids, raws = [], []
+    by_mesh = sorted(doc["frames"], key=lambda f: f["mesh_path"])
+    for i, fr in enumerate(by_mesh):
+        seg = rng.integers(0, 256, (24, 32, 4), dtype=np.uint8)
+        Image.fromarray(seg, "RGBA").save(path / "seg" / f"{i:05d}.png")
+        fr["seg_mask_path"] = f"seg/{i:05d}.png"
+        ids.append(seg[..., 0])
+        raw = rng.integers(0, 65536, (24, 32)).astype(np.uint16)
+        kind = depth_files[i]
+        if kind == "png":
+            Image.fromarray(raw).save(path / "depth" / f"{i:05d}.png")
+            fr["depth_path"] = f"depth/{i:05d}.png"
+        elif kind == "bin":
+            # the upper 16 bits are dropped by the reader (uint32 -> uint16)
+            (raw.astype("<u4") | (rng.integers(0, 3, raw.shape).astype("<u4") << 16)).tofile(
+                path / "depth" / f"{i:05d}.bin")
+            fr["depth_path"] = f"depth/{i:05d}.bin"
+        else:
+            raw = np.zeros_like(raw)
+        raws.append(raw)
+    return ids, raws
+
+
+def test_segmentation_masks_and_depth_images(capture):
+    path, _, doc, _ = capture
+    rng = np.random.default_rng(7)
+    doc = json.loads(json.dumps(doc))
+    ids, raws = _add_supervision(path, doc, rng)
+    doc["integer_depth_scale"] = 0.001
+    doc["scale"] = 1.5
+    (path / "sup.json").write_text(json.dumps(doc))
+    ds = dataset.load_transforms(path / "sup.json")
+    seg = ds.seg_masks()
+    assert seg.dtype == np.uint8 and np.array_equal(seg, np.stack(ids))
+    depth = ds.depth_images()
+    expect = np.stack(raws).astype(np.float32) * np.float32(0.001 * 1.5)
+    assert depth.dtype == np.float32 and np.array_equal(depth, expect)
+    assert (depth[2] == 0).all() and (depth[0] > 0).mean() > 0.99
+    assert np.array_equal(ds.seg_masks([2, 0]), seg[[2, 0]])
+    assert np.array_equal(ds.depth_images([1]), depth[[1]])
+    # no integer_depth_scale: no depth at all, as the reference skips the files
+    del doc["integer_depth_scale"]
+    (path / "nodepth.json").write_text(json.dumps(doc))
+    assert dataset.load_transforms(path / "nodepth.json").depth_images() is None
+    # masks for only some images of a set are refused (the device takes all or none)
+    for fr in doc["frames"]:
+        if fr["mesh_path"].endswith("00001.obj"):
+            del fr["seg_mask_path"]
+    (path / "someseg.json").write_text(json.dumps(doc))
+    ds2 = dataset.load_transforms(path / "someseg.json")
+    assert ds2.seg_masks([0, 2]) is not None
+    with pytest.raises(ValueError):
+        ds2.seg_masks()
+
+
+def test_8_bit_depth_images_scale_as_stbi_load_16(tmp_path):
+    from PIL import Image
+    a = np.arange(12, dtype=np.uint8).reshape(3, 4) * 20
+    Image.fromarray(a, "L").save(tmp_path / "d.png")
+    assert np.array_equal(dataset.read_depth(tmp_path / "d.png", 4, 3), a.astype(np.uint16) * 257)
+    Image.fromarray(np.zeros((3, 4, 3), np.uint8), "RGB").save(tmp_path / "rgb.png")
+    with pytest.raises(ValueError):
+        dataset.read_depth(tmp_path / "rgb.png", 4, 3)
+
+
 def test_obj_polygons_and_negative_indices(tmp_path):
     p = tmp_path / "q.obj"
     p.write_text("v 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nf 1 2 3 4\nf -4 -2 -1\n")

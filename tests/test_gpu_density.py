@@ -67,3 +67,31 @@ def test_density_grid_updates_match_oracle():
     rgba, _, steps = ctx.render(cam, 32, 32, -1, -1, 0)
     torch.cuda.synchronize()
     assert int(steps.sum().item()) > 0
+
+
+def test_untrained_cells_match_oracle():
+    """With a training set, the first update marks the cells no training
+    camera sees (-1, mark_untrained_density_grid) exactly as the oracle;
+    they stay out of the samples, the EMA and the bitfield, and a training
+    set with another image count re-marks."""
+    import torch
+    from tests.test_oracle_density import _untrained_cameras
+    ctx, orc, O, cfg = _pair()
+    G = cfg.grid_size
+    cams, W, H = _untrained_cameras()
+    imgs = np.full((2, H, W, 4), 0.5, np.float32)
+    for n in (1, 2):
+        ctx.set_training_set(torch.from_numpy(imgs[:n]).cuda(), cams[:n], list(range(n)))
+        O.set_training_set(orc, imgs[:n], cams[:n], list(range(n)))
+        for k, (nu, nn) in enumerate([(G ** 3, 0), (G ** 3 // 4, G ** 3 // 4)]):
+            grid, mean = ctx.update_density_grid(-1, 3, decay=0.95, n_uniform=nu, n_nonuniform=nn, seed=31,
+                                                 want_grid=True)
+            g = grid.cpu().numpy()
+            rg, rmean = O.update_density_grid(orc, -1, 3, 0.95, nu, nn, 31)
+            assert np.array_equal(g < 0, rg < 0), (n, k)
+            assert (rg < 0).sum() > 1000 and (rg >= 0).sum() > 1000
+            assert np.array_equal(g == 0, rg == 0), (n, k)
+            assert_rel_close(g, rg, 1e-4, f"grid {n} {k}")
+            assert abs(mean - rmean) <= 1e-4 * rmean
+            bits = np.unpackbits(ctx.get_occupancy(-1).cpu().numpy(), bitorder="little").astype(bool)
+            assert not bits[g < 0].any()

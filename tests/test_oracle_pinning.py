@@ -17,6 +17,13 @@ check that every pin is live: the oracle is rebuilt with a planted mistake
                   background (PCG32 draws 2-4 of the ray) taken through the
                   linear-space blend of testbed_nerf.cu:1419-1443 on a
                   partial-alpha texel
+  supervision     segmentation and depth terms (testbed_nerf.cu:1476-1497,
+                  1555-1586): the objective of a two-sample ray by hand for
+                  each mask-id rule (120/130/140 weighted, <= 90 and > 250
+                  and alpha < 0.001 without depth, depth 0 = none) with the
+                  target |unnormalised direction| * depth texel; and the
+                  gradient of the supervised objective against central
+                  differences
   march count     the full-grid march restated with FP32 scalar arithmetic
                   (pinhole ray, slab entry, max(t, 0.05) + 1e-6, one-step
                   jitter, t += dt while inside the render box), compared
@@ -222,6 +229,163 @@ def test_loss_case_covers_both_huber_branches():
     assert min(diffs) < 0.1 < max(diffs), diffs
 
 
+# ---------------------------------------------------------------- supervision
+def _supervision_ray(seed, cam, W, H, lo, hi, dt, n_steps):
+    """The ray's unnormalised-direction length and its samples' distances
+    |x_j - o|: pinhole ray, slab entry, jitter (draw 3), t += dt, in float32."""
+    u = _pcg32_floats(seed, 5)
+    x, y, jitter = f32(u[0]), f32(u[1]), f32(u[3])
+    M = np.array(cam.matrix, np.float32).reshape(3, 4)
+    o = M[:, 3]
+    d0 = (((x - f32(cam.screen_center[0])) * f32(W)) / f32(cam.focal_px[0]),
+          ((y - f32(cam.screen_center[1])) * f32(H)) / f32(cam.focal_px[1]), f32(1.0))
+    d = [M[r, 0] * d0[0] + M[r, 1] * d0[1] + M[r, 2] * d0[2] for r in range(3)]
+    dn = np.sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2])
+    d = [d[0] / dn, d[1] / dn, d[2] / dn]
+    t = max(_box_tmin(lo, hi, o, d), f32(0.0)) + f32(dt) * jitter
+    dist = []
+    for _ in range(n_steps):
+        pos = [o[k] + d[k] * t for k in range(3)]
+        dist.append(math.sqrt(sum((float(pos[k]) - float(o[k])) ** 2 for k in range(3))))
+        t = t + f32(dt)
+    return float(dn), dist
+
+
+def check_supervision(lib_path=None):
+    """Objective of a two-sample ray with a segmentation mask and a depth
+    image, by hand: s * sum Huber/5 + l * |D - |d| z| with s the seg weight for
+    mask ids 120/130/140, l the depth weight unless the id is <= 90 or > 250
+    or the texel's alpha < 0.001, D = a z_0 + a (1 - a) z_1 (testbed_nerf.cu:
+    1476-1497, 1555-1586)."""
+    from oracle import oracle as O
+    av = synth.small_sphere_avatar(0, 1)
+    bias, seed, n_steps = 5.0, 11, 2
+    cfg = small_config(max_steps=n_steps, density_logit_bias=bias)
+    o = O.Oracle(cfg, lib_path)
+    o.set_canonical_mesh(av.canonical, av.faces, av.masks)
+    o.set_frame(0, av.canonical)
+    o.set_occupancy(0, np.full(cfg.grid_size ** 3 // 8, 0xFF, np.uint8))
+    o.set_params(np.zeros(o.layout.n_params, np.float32))
+    W = H = 4
+    # a wide field of view: the unnormalised direction is clearly longer than 1
+    cam = make_camera(synth.look_at_camera(np.array([0.2, 0.1, 1.0]), 1.0), synth.focal_from_fov(W, 70.0))
+    box = o.render_box()
+    dn, dist = _supervision_ray(seed, cam, W, H, box[:3].astype(np.float32), box[3:].astype(np.float32),
+                                cfg.dt, n_steps)
+    assert dn > 1.05, dn
+    a = 1.0 - math.exp(-math.exp(bias) * cfg.dt)
+    D = a * dist[0] + a * (1.0 - a) * dist[1]
+    T = (1.0 - a) ** n_steps
+    bgu = _pcg32_floats(seed, 5)[2:5]
+    colour = (0.9, 0.25, 0.6)
+
+    def huber_sum(alpha_tex):
+        tot = 0.0
+        for k in range(3):
+            bg_lin = _srgb_to_linear(bgu[k])
+            pred = 0.5 * (1.0 - T) + T * _linear_to_srgb(bg_lin)
+            target = _linear_to_srgb(_srgb_to_linear(colour[k]) * alpha_tex + (1 - alpha_tex) * bg_lin)
+            tot += _huber5(pred - target)
+        return tot
+
+    z_above, z_below = 1.3 * D, 0.6 * D    # camera-axis depth; the target is dn * z
+    # (mask id or None, texel alpha, depth texel, seg weight, depth weight) -> (seg factor, depth factor)
+    cases = [(None, 0.6, z_below, 3.0, 1.5, 1.0, 1.5), (None, 0.6, z_above, 3.0, 1.5, 1.0, 1.5),
+             (120, 0.6, z_below, 3.0, 1.25, 3.0, 1.25), (130, 0.6, z_above, 40.0, 1.25, 40.0, 1.25),
+             (140, 0.6, z_below, 2.0, 1.0, 2.0, 1.0), (90, 0.6, z_below, 3.0, 1.5, 1.0, 0.0),
+             (91, 0.6, z_below, 3.0, 1.5, 1.0, 1.5), (250, 0.6, z_above, 3.0, 1.5, 1.0, 1.5),
+             (251, 0.6, z_below, 3.0, 1.5, 1.0, 0.0), (200, 0.0005, z_below, 3.0, 1.5, 1.0, 0.0),
+             (200, 0.6, 0.0, 3.0, 1.5, 1.0, 0.0), (121, 0.6, z_below, 3.0, 1.5, 1.0, 1.5)]
+    for mask, alpha_tex, z, sw, dw, s_f, d_f in cases:
+        img = np.zeros((1, H, W, 4), np.float32)
+        img[..., :3] = colour
+        img[..., 3] = alpha_tex
+        O.set_training_set(o, img, [cam], [0])
+        seg = None if mask is None else np.full((1, H, W), mask, np.uint8)
+        O.set_training_supervision(o, seg, np.full((1, H, W), z, np.float32), sw, dw)
+        _, (reported, objective), counts = O.train_grads(o, 1, seed, want_grads=False)
+        assert counts == (1, n_steps, n_steps), counts
+        h = huber_sum(alpha_tex)
+        expect = s_f * h + (d_f * abs(D - dn * float(np.float32(z))) if z > 0 else 0.0)
+        assert math.isclose(objective, expect, rel_tol=1e-6), (mask, alpha_tex, z, objective, expect)
+        assert math.isclose(reported, h / 3.0, rel_tol=1e-6), (mask, reported, h / 3.0)
+
+
+def test_supervision_objective_by_hand():
+    ensure_oracle()
+    check_supervision()
+
+
+def supervised_train_oracle(lib_path=None, n_frames=2, res=32, seed=3):
+    """Two posed frames of the sphere with smooth images, random mask ids
+    (supervised and unsupervised, weighted and not) and depth images around
+    the sphere's distance, some texels without depth."""
+    from oracle import oracle as O
+    cfg = small_config(max_frames=n_frames)
+    av = synth.small_sphere_avatar(seed=seed, n_frames=n_frames)
+    orc = O.Oracle(cfg, lib_path)
+    orc.set_canonical_mesh(av.canonical, av.faces, av.masks)
+    for f, v in enumerate(av.frames):
+        orc.set_frame(f, v)
+        orc.build_occupancy(f, 0.05)
+    params = synth.network_params(11, orc.layout, cfg)
+    rng = np.random.default_rng(5)
+    nm = orc.layout.n_mlp_params
+    params[nm:] = rng.uniform(-0.5, 0.5, size=params.size - nm).astype(np.float32)
+    orc.set_params(params)
+    rng = np.random.default_rng(seed)
+    cams = [make_camera(synth.random_camera(rng, 1.0, front_bias=False), synth.focal_from_fov(res, 30.0))
+            for _ in range(n_frames)]
+    O.set_training_set(orc, synth.smooth_noise_images(seed, n_frames, res, res), cams, list(range(n_frames)))
+    seg = rng.choice(np.array([0, 90, 100, 120, 130, 140, 200, 251], np.uint8), size=(n_frames, res, res))
+    depth = rng.uniform(0.5, 1.2, (n_frames, res, res)).astype(np.float32)
+    depth[rng.uniform(size=depth.shape) < 0.2] = 0.0
+    O.set_training_supervision(orc, seg, depth, 4.0, 2.0)
+    return orc, cfg, params
+
+
+def fd_check(orc, cfg, params, n_rays, seed, picks_per_block=5, tol=1e-5):
+    """Central differences of the objective against the oracle's gradient for
+    parameters with a clearly non-zero gradient in every weight block and in
+    the tables; the worst relative error (best of three step sizes each)."""
+    from oracle import oracle as O
+    g, _, _ = O.train_grads(orc, n_rays, seed)
+    lay = orc.layout
+    rng = np.random.default_rng(0)
+    bounds = np.cumsum([0] + [o * i for (o, i) in lay.mlp_shapes(cfg)] + [lay.n_grid_params])
+    picks = []
+    for b0, b1 in zip(bounds[:-1], bounds[1:]):
+        blk = np.arange(b0, b1)
+        big = blk[np.abs(g[blk]) > 0.05 * np.abs(g[blk]).max()]
+        picks += list(rng.choice(big, size=min(picks_per_block, len(big)), replace=False))
+    worst = 0.0
+    for p in picks:
+        errs = []
+        for h in (2.0 ** -17, 2.0 ** -19, 2.0 ** -21):
+            pp, pm = params.copy(), params.copy()
+            pp[p] = params[p] + np.float32(h)
+            pm[p] = params[p] - np.float32(h)
+            orc.set_params(pp)
+            _, (_, lp), _ = O.train_grads(orc, n_rays, seed, want_grads=False)
+            orc.set_params(pm)
+            _, (_, lm), _ = O.train_grads(orc, n_rays, seed, want_grads=False)
+            fd = (lp - lm) / (float(pp[p]) - float(pm[p]))
+            errs.append(abs(fd - g[p]) / (abs(g[p]) + 1e-12))
+        worst = max(worst, min(errs))
+    orc.set_params(params)
+    return worst
+
+
+def check_supervision_gradient(lib_path=None):
+    orc, cfg, params = supervised_train_oracle(lib_path)
+    assert fd_check(orc, cfg, params, 48, 9, picks_per_block=3) < 1e-5
+
+
+def test_supervision_gradient_matches_finite_differences():
+    ensure_oracle()
+    check_supervision_gradient()
+
+
 # ---------------------------------------------------------------- march count
 def _box_tmin(lo, hi, o, d):
     """Slab entry distance in float32, in the march's operation order."""
@@ -387,7 +551,8 @@ def test_frame_buffer_identical_frames_keep_their_mean():
 
 
 # ---------------------------------------------------------------- liveness
-PINS = {1: check_blend, 2: check_sh, 3: check_loss, 4: check_march, 5: check_loss, 6: check_frame_buffer}
+PINS = {1: check_blend, 2: check_sh, 3: check_loss, 4: check_march, 5: check_loss, 6: check_frame_buffer,
+        7: check_supervision, 8: check_supervision_gradient}
 
 
 @pytest.fixture(scope="session")
