@@ -394,15 +394,17 @@ def serving_bench(args, full=False):
     radius, thr = (0.7, 0.015) if full else (1.0, 0.05)
     resolutions, weights = np.array([128, 256, 512, 1024]), np.array([0.3, 0.3, 0.3, 0.1])
 
+    kw = {} if args.serve_cell_split is None else dict(cell_split=args.serve_cell_split)
+
     def avatars(precision):
         out = []
         for a in range(n_av):
             if full:
                 av = synth.head_avatar(seed=300 + a, n_frames=0, n_bases=50)
-                cfg = full_config(max_frames=max(n_slots, 1), precision=precision)
+                cfg = full_config(max_frames=max(n_slots, 1), precision=precision, **kw)
             else:
                 av = synth.small_sphere_avatar(seed=300 + a, n_frames=0, n_bases=50)
-                cfg = small_config(max_frames=max(n_slots, 1), precision=precision)
+                cfg = small_config(max_frames=max(n_slots, 1), precision=precision, **kw)
             ctx = Context(cfg)
             ctx.set_canonical_mesh(av.canonical, av.faces, av.masks)
             ctx.set_params(torch.from_numpy(synth.network_params(400 + a, ctx.layout, cfg)).cuda())
@@ -504,6 +506,9 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=120)
     ap.add_argument("--serve-slots", type=int, default=6)
     ap.add_argument("--serve-budget-gib", type=int, default=24)
+    ap.add_argument("--serve-sizes", choices=["both", "small", "full"], default="both")
+    ap.add_argument("--serve-cell-split", type=int, default=None,
+                    help="candidate-list split of the served avatars (default: the library's)")
     ap.add_argument("--train-steps", type=int, default=40)
     ap.add_argument("--train-sizes", choices=["both", "small", "full"], default="both")
     ap.add_argument("--train-warmup", type=int, default=8)
@@ -613,9 +618,9 @@ def main():
     phase("training (small shapes)")
     train_full = None if args.skip_train or args.train_sizes == "small" else training_bench(args, full=True)
     phase("training (full size)")
-    serve = None if args.skip_serve else serving_bench(args)
+    serve = None if args.skip_serve or args.serve_sizes == "full" else serving_bench(args)
     phase("serving (small shapes)")
-    serve_full = None if args.skip_serve else serving_bench(args, full=True)
+    serve_full = None if args.skip_serve or args.serve_sizes == "small" else serving_bench(args, full=True)
     phase("serving (full size)")
 
     cpu = None

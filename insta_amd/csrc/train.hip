@@ -695,12 +695,18 @@ __global__ __launch_bounds__(kMlpThreads) void train_backward_kernel(const float
     __syncthreads();
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t m = lane & 15u, g = lane >> 4;
-    const float* W0 = w + O::W0;
-    const float* W1 = w + O::W1;
-    const float* W2 = w + O::W2;
-    const float* W3 = w + O::W3;
-    const float* W4 = w + O::W4;
     for (uint32_t nb = 0; nb < 4; ++nb) {
+        // Opaque copies of 0 and ld: the weight reads (LDS) and the row
+        // addresses of act/dact are formed inside this loop.  Loop-invariant,
+        // they were otherwise hoisted out of it into ~360 registers (1 wave
+        // per SIMD); recomputed per block of 16 samples, 248 (2 waves).
+        uint32_t z = 0, ldz = ld;
+        asm volatile("" : "+s"(z), "+s"(ldz));
+        const float* W0 = w + O::W0 + z;
+        const float* W1 = w + O::W1 + z;
+        const float* W2 = w + O::W2 + z;
+        const float* W3 = w + O::W3 + z;
+        const float* W4 = w + O::W4 + z;
         const uint32_t s = wave * 64 + nb * 16 + m;  // column (sample) of this lane
         const uint32_t i = base + s;
         const bool live = i < n;
@@ -715,8 +721,8 @@ __global__ __launch_bounds__(kMlpThreads) void train_backward_kernel(const float
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const uint32_t row = 16 * t + 4 * g + q;
-                dh3[t][q] = live && act[(size_t)(A::H3 + row) * ld + i] > 0.0f ? acc[q] : 0.0f;
-                if (live) dact[(size_t)(Dact::DH3 + row) * ld + i] = dh3[t][q];
+                dh3[t][q] = live && act[(size_t)(A::H3 + row) * ldz + i] > 0.0f ? acc[q] : 0.0f;
+                if (live) dact[(size_t)(Dact::DH3 + row) * ldz + i] = dh3[t][q];
             }
         }
         // colour hidden layer: dH2 = relu'(H2) (W3^T dH3)
@@ -733,8 +739,8 @@ __global__ __launch_bounds__(kMlpThreads) void train_backward_kernel(const float
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const uint32_t row = 16 * t + 4 * g + q;
-                dh2[t][q] = live && act[(size_t)(A::H2 + row) * ld + i] > 0.0f ? acc[q] : 0.0f;
-                if (live) dact[(size_t)(Dact::DH2 + row) * ld + i] = dh2[t][q];
+                dh2[t][q] = live && act[(size_t)(A::H2 + row) * ldz + i] > 0.0f ? acc[q] : 0.0f;
+                if (live) dact[(size_t)(Dact::DH2 + row) * ldz + i] = dh2[t][q];
             }
         }
         // colour input restricted to the density outputs: dD = (W2^T dH2)[0..15], + dL/d(density logit) on row 0
@@ -747,7 +753,7 @@ __global__ __launch_bounds__(kMlpThreads) void train_backward_kernel(const float
         if (g == 0) dd[0] = dd[0] + d4.w;  // density logit = density-MLP output 0
         if (live)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) dact[(size_t)(Dact::DDENS + 4 * g + q) * ld + i] = dd[q];
+            for (int q = 0; q < 4; ++q) dact[(size_t)(Dact::DDENS + 4 * g + q) * ldz + i] = dd[q];
         // density hidden layer: dH1 = relu'(H1) (W1^T dD)
         bf4 dh1[4];
 #pragma unroll
@@ -759,8 +765,8 @@ __global__ __launch_bounds__(kMlpThreads) void train_backward_kernel(const float
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const uint32_t row = 16 * t + 4 * g + q;
-                dh1[t][q] = live && act[(size_t)(A::H1 + row) * ld + i] > 0.0f ? acc[q] : 0.0f;
-                if (live) dact[(size_t)(Dact::DH1 + row) * ld + i] = dh1[t][q];
+                dh1[t][q] = live && act[(size_t)(A::H1 + row) * ldz + i] > 0.0f ? acc[q] : 0.0f;
+                if (live) dact[(size_t)(Dact::DH1 + row) * ldz + i] = dh1[t][q];
             }
         }
         // dL/dencoding = (W0^T dH1)[0..E), the extra dims' rows not needed
