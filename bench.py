@@ -318,11 +318,11 @@ def training_bench(args, full=False):
         first = None
         # warm up until the adaptive ray count holds the batch at the target
         # (three steps in a row within 2 %), at least train_warmup steps
-        full, warm = 0, 0
-        while warm < args.train_warmup or (full < 3 and warm < args.train_warmup_max):
+        held, warm = 0, 0
+        while warm < args.train_warmup or (held < 3 and warm < args.train_warmup_max):
             st = ctx.train_step(tc)
             first = st.loss if first is None else first
-            full = full + 1 if min(st.n_samples_used, target) >= 0.98 * target else 0
+            held = held + 1 if min(st.n_samples_used, target) >= 0.98 * target else 0
             warm += 1
         torch.cuda.synchronize()
         samples = rays = batch = evaluated = 0
@@ -394,7 +394,7 @@ def serving_bench(args, full=False):
     radius, thr = (0.7, 0.015) if full else (1.0, 0.05)
     resolutions, weights = np.array([128, 256, 512, 1024]), np.array([0.3, 0.3, 0.3, 0.1])
 
-    kw = {} if args.serve_cell_split is None else dict(cell_split=args.serve_cell_split)
+    kw = dict(cell_split=args.serve_cell_split)
 
     def avatars(precision):
         out = []
@@ -487,7 +487,8 @@ def serving_bench(args, full=False):
     out["sequential_fp32"] = svc
     out["workload"] = ("8 avatars at the " + ("full-size shapes (9,976-tri heads, L16 T2^19, 128^3 grid)" if full
                        else "small shapes (2,048-tri spheres, L8 T2^14, 64^3 grid)") +
-                       ", resolutions 128/256/512/1024^2 (0.3/0.3/0.3/0.1), Poisson arrivals")
+                       ", resolutions 128/256/512/1024^2 (0.3/0.3/0.3/0.1), Poisson arrivals, candidate lists "
+                       f"split {args.serve_cell_split}")
     return out
 
 
@@ -507,8 +508,9 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=6)
     ap.add_argument("--serve-budget-gib", type=int, default=24)
     ap.add_argument("--serve-sizes", choices=["both", "small", "full"], default="both")
-    ap.add_argument("--serve-cell-split", type=int, default=None,
-                    help="candidate-list split of the served avatars (default: the library's)")
+    ap.add_argument("--serve-cell-split", type=int, default=2,
+                    help="candidate-list split of the served avatars' contexts (each request builds its "
+                         "frame's lists for one render; 2 measured best for both avatar sizes)")
     ap.add_argument("--train-steps", type=int, default=40)
     ap.add_argument("--train-sizes", choices=["both", "small", "full"], default="both")
     ap.add_argument("--train-warmup", type=int, default=8)

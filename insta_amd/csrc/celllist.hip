@@ -241,8 +241,11 @@ __global__ __launch_bounds__(kClThreads) void cl_fill_kernel(ClArgs a) {
 // Sort each list's entries after the seed by key (truncated box distance,
 // then triangle): one wave per sub-cell, each lane ranks up to kSortPerLane
 // keys by counting the smaller ones (keys are distinct), then writes them to
-// their ranks.  The loop over the list is wave-uniform, and every store
-// depends on all of its loads, so no entry is overwritten before it is read.
+// their ranks.  The keys are read from memory once, into registers (key q of
+// lane l is entry l + 64 q); the counting pass broadcasts them from there
+// lane by lane (readlane), so it issues no memory reads.  Every store
+// follows all of the wave's loads, so no entry is overwritten before it is
+// read.
 __global__ __launch_bounds__(kClThreads) void cl_sort_kernel(ClArgs a) {
     const uint32_t S3 = a.S * a.S * a.S;
     const uint32_t n_sub = min(a.state[0], a.cap) * S3;
@@ -261,11 +264,16 @@ __global__ __launch_bounds__(kClThreads) void cl_sort_kernel(ClArgs a) {
             rank[q] = 0u;
         }
         const uint32_t nq = (n + 63u) / 64u;  // key slots in use (wave-uniform)
-        for (uint32_t j = 0; j < n; ++j) {
-            const uint32_t v = e[j];
 #pragma unroll
-            for (uint32_t q = 0; q < kSortPerLane; ++q)
-                if (q < nq) rank[q] += v < key[q] ? 1u : 0u;
+        for (uint32_t qs = 0; qs < kSortPerLane; ++qs) {
+            if (qs >= nq) break;
+            const uint32_t cnt = min(64u, n - 64u * qs);
+            for (uint32_t l = 0; l < cnt; ++l) {
+                const uint32_t v = (uint32_t)__builtin_amdgcn_readlane((int)key[qs], (int)l);
+#pragma unroll
+                for (uint32_t q = 0; q < kSortPerLane; ++q)
+                    if (q < nq) rank[q] += v < key[q] ? 1u : 0u;
+            }
         }
 #pragma unroll
         for (uint32_t q = 0; q < kSortPerLane; ++q)

@@ -240,6 +240,36 @@ def test_train_loop_follows_the_reference_schedule(scene):
     assert np.unpackbits(occ_a).sum() > 0
 
 
+def test_train_loop_density_schedule_matches_oracle(scene):
+    """insta_train_loop's density-grid updates against the oracle's update
+    called on the reference schedule by hand (before step s when
+    s % clamp(s / 16, 1, 16) == 0, G^3 uniform samples below step 256).
+    Learning rate 0 keeps the parameters fixed, so the grid depends only on
+    which updates ran, in which order and with which sample counts: the
+    march slot's bits agree with the oracle's."""
+    import torch
+    n_steps = 40
+    cfg = scene[0]
+    G3 = cfg.grid_size ** 3
+    ctx, params = _gpu(scene)
+    tc = default_train_config(n_rays=1024, max_samples=1 << 18, target_samples=1 << 18, seed=21, learning_rate=0.0)
+    st = ctx.train_loop(tc, n_steps, density_slot=-1, n_frames=N_FRAMES, decay=0.95, seed=5)
+    torch.cuda.synchronize()
+    assert st.step == n_steps
+    assert np.array_equal(ctx.params().cpu().numpy(), params)
+    orc, O = _oracle(scene, params)
+    updates = 0
+    for s in range(n_steps):
+        if s % min(max(s // 16, 1), 16) == 0:
+            O.update_density_grid(orc, -1, N_FRAMES, 0.95, G3, 0, 5)
+            updates += 1
+    assert updates == 36
+    got = np.unpackbits(ctx.get_occupancy(-1).cpu().numpy(), bitorder="little").astype(bool)
+    ref = np.unpackbits(orc.get_occupancy(-1), bitorder="little").astype(bool)
+    assert ref.sum() > 1000 and (~ref).sum() > 1000
+    assert (got == ref).mean() > 0.999
+
+
 def test_adam_l2_term_against_the_fp64_restatement(scene):
     """Two steps with l2_reg: the parameters follow the oracle's FP64 Adam
     with the L2 term (moments carried over), given the step's own loss
