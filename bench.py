@@ -70,7 +70,11 @@ def setup(args, precision=0, av=None, n_extra=0):
     ctx = Context(cfg)
     ctx.set_canonical_mesh(av.canonical, av.faces, av.masks)
     frames = torch.from_numpy(np.stack(av.frames)).cuda()
-    ctx.set_frames(0, frames)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.set_frames(0, frames)   # 16 device BVH builds (with FP64 frames), one workgroup each
+    torch.cuda.synchronize()
+    ctx.bvh_ms_per_frame = (time.perf_counter() - t0) * 1e3 / 16
     params = synth.network_params(7, ctx.layout, cfg)
     ctx.set_params(torch.from_numpy(params).cuda())
     if n_extra:  # per-frame expression values (seeded N(0, 1))
@@ -182,6 +186,66 @@ def cpu_baseline(cfg, av, params, cam, res, budget_s, runs=5):
                 sample=f"{n_pix} random pixels of a {res}x{res} frame of posed frame 0 ({samples} samples); "
                        f"1 warm-up, median of {runs} runs ({', '.join(f'{t:.2f}' for t in times)} s)",
                 frames_per_s=(samples / med) / (samples * res * res / n_pix))
+
+
+def small_scalar_bench(runs=5):
+    """The small CPU-scalar workload (2,048-triangle sphere, one posed frame,
+    L=8 F=2 T=2^14, 64^3 grid at distance 0.05, a 64x64 render from a camera
+    at radius 1.0 with a 30 degree field of view, <= 256 steps of 1/256):
+    the oracle on one host thread and on all of them, and the library on the
+    GPU (FP32), each 1 warm-up and the median of `runs`; the GPU image
+    against the oracle's (largest per-pixel RGBA difference)."""
+    import torch
+    from insta_amd import synth
+    from insta_amd.capi import Context, make_camera, small_config
+    from oracle.oracle import Oracle
+    cfg = small_config(max_frames=1)
+    av = synth.small_sphere_avatar(seed=0, n_frames=1)
+    res = 64
+    cam = make_camera(synth.random_camera(np.random.default_rng(5), 1.0), synth.focal_from_fov(res, 30.0))
+    ctx = Context(cfg)
+    ctx.set_canonical_mesh(av.canonical, av.faces, av.masks)
+    ctx.set_frames(0, torch.from_numpy(av.frames[0][None].astype(np.float32)).cuda())
+    params = synth.network_params(7, ctx.layout, cfg)
+    ctx.set_params(torch.from_numpy(params).cuda())
+    ctx.build_occupancy(0, 0.05)
+    orc = Oracle(cfg)
+    orc.set_canonical_mesh(av.canonical, av.faces, av.masks)
+    orc.set_frame(0, av.frames[0])
+    orc.set_params(params)
+    orc.build_occupancy(0, 0.05)
+
+    def median_time(fn):
+        fn()
+        ts = []
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts))
+
+    out = {}
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    ref = None
+    for name, nt in (("cpu_1_thread", 1), ("cpu_all_threads", threads)):
+        orc.set_threads(nt)
+        ref = orc.render(cam, res, res, 0, 0)
+        samples = int(ref[3][0])
+        t = median_time(lambda: orc.render(cam, res, res, 0, 0))
+        out[name] = dict(threads=nt, samples_per_s=samples / t, ms_per_frame=t * 1e3)
+    rgba = None
+
+    def gpu():
+        nonlocal rgba
+        rgba = ctx.render(cam, res, res, 0, 0)[0]
+        torch.cuda.synchronize()
+    t = median_time(gpu)
+    out["gpu_fp32"] = dict(samples_per_s=int(ref[3][0]) / t, ms_per_frame=t * 1e3)
+    out["max_abs_rgba_vs_oracle"] = float(np.abs(rgba.cpu().numpy() - ref[0]).max())
+    out["samples"] = int(ref[3][0])
+    out["workload"] = ("small CPU-scalar workload: 2,048-tri sphere, L8 T2^14, 64^3 grid, 64x64 render, "
+                       "<= 256 steps of 1/256; 1 warm-up, median of %d" % runs)
+    return out
 
 
 def precision_bench(args, n_views=8, res=64, launch=1 << 20, reps=20):
@@ -587,6 +651,7 @@ def main():
                      triangles_per_sample=tc["triangles"] / q,
                      nodes_per_sample=(tc["inner"] + tc["leaves"]) / q,
                      occupancy_and_lists_ms_per_frame=ctx.occupancy_ms_per_frame,
+                     bvh_ms_per_frame=ctx.bvh_ms_per_frame,
                      list_info_frame0=ctx.cell_list_info(0))
 
     # precision forms: same workload, other network arithmetic
@@ -626,8 +691,10 @@ def main():
     phase("serving (full size)")
 
     cpu = None
+    small = None
     if args.cpu_budget > 0:
         cpu = cpu_baseline(cfg, av, params, cams[0], res, args.cpu_budget)
+        small = small_scalar_bench()
 
     line = {
         "metric": METRIC,
@@ -651,7 +718,15 @@ def main():
         "roofline": roof,
         "kernels": kernels,
         "deform_traversal": traversal,
+        # a posed frame's set-up (device BVH + occupancy grid + candidate
+        # lists) is outside the timed renders, which reuse 16 resident frames;
+        # the rate when every frame is a new pose rendered once includes it
+        "posed_frame_setup": dict(
+            bvh_ms=ctx.bvh_ms_per_frame, occupancy_and_lists_ms=ctx.occupancy_ms_per_frame,
+            frames_per_s_one_render_per_pose=1e3 / (elapsed / args.steps * 1e3 + ctx.bvh_ms_per_frame
+                                                    + ctx.occupancy_ms_per_frame)),
         "cpu_baseline": cpu,
+        "small_scalar": small,
         "training": train,
         "training_full_size": train_full,
         "serving": serve,

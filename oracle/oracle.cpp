@@ -1364,7 +1364,11 @@ int orc_render(const orc_scene* s, const insta_camera* cam, uint32_t width, uint
     ld_random_pixel_offset(cam->snap_to_pixel_centers ? 0u : spp, offset);
     const uint32_t n_pix = pixel_subset ? n_subset : width * height;
     uint64_t total_samples = 0, alive0 = 0;
-#pragma omp parallel for schedule(dynamic, 64) reduction(+ : total_samples, alive0)
+    int nt = 1;
+#ifdef _OPENMP
+    nt = s->n_threads > 0 ? s->n_threads : omp_get_max_threads();
+#endif
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : total_samples, alive0) num_threads(nt)
     for (int64_t q = 0; q < (int64_t)n_pix; ++q) {
         const uint32_t i = pixel_subset ? pixel_subset[q] : (uint32_t)q;
         const uint32_t x = i % width, y = i / width;
