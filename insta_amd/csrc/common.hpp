@@ -340,9 +340,10 @@ INSTA_DEV bool cell_list_find(const CellListView& cl, V3 p, uint2& hd) {
 // plane lie within the bound (the distance to a triangle is at least the
 // distance to its plane).  The scan is latency-bound (each entry is a load of
 // the entry, then of its triangle), so entries go in batches of kListBatch:
-// the batch's entry words and triangles are loaded together, then tested in
-// order; loads past the stopping entry are wasted, never used.
-constexpr uint32_t kListBatch = 2;
+// the batch's triangles are loaded together, with the next batch's entry
+// words already in flight, then tested in order; loads past the stopping
+// entry are wasted, never used.
+constexpr uint32_t kListBatch = 3;
 
 template <bool kCount>
 INSTA_DEV Hit list_closest_impl(const MeshView& m, const uint32_t* __restrict__ ent, uint2 hd, V3 p,
@@ -355,11 +356,17 @@ INSTA_DEV Hit list_closest_impl(const MeshView& m, const uint32_t* __restrict__ 
     const uint32_t* e = ent + hd.x;
     if constexpr (kCount) ++tc->lists;
     bool done = false;
+    uint32_t wn[kListBatch];
+#pragma unroll
+    for (uint32_t j = 0; j < kListBatch; ++j) wn[j] = e[min(j, n - 1u)];
     for (uint32_t k = 0; k < n && !done; k += kListBatch) {
         uint32_t w[kListBatch];
         float4 R[kListBatch][3];
 #pragma unroll
-        for (uint32_t j = 0; j < kListBatch; ++j) w[j] = e[min(k + j, n - 1u)];
+        for (uint32_t j = 0; j < kListBatch; ++j) w[j] = wn[j];
+        // the next batch's entry words, in flight with this batch's triangles
+#pragma unroll
+        for (uint32_t j = 0; j < kListBatch; ++j) wn[j] = e[min(k + kListBatch + j, n - 1u)];
 #pragma unroll
         for (uint32_t j = 0; j < kListBatch; ++j) {
             const uint32_t t = w[j] & 0xffffu;
