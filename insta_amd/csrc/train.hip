@@ -102,6 +102,9 @@ struct Off {
                               N = W4 + 3 * kW;
 };
 
+// XCDs of the device: workgroups of one XCD share its L2
+constexpr uint32_t kXcds = 8;
+
 struct TrainState {
     uint32_t n_images = 0, width = 0, height = 0;
     float4* images = nullptr;
@@ -130,7 +133,10 @@ struct TrainState {
     float* t_buf = nullptr;        // ray march scratch, [max_steps][ray_cap]
     // optimizer
     float *grad = nullptr, *adam_m = nullptr, *adam_v = nullptr;
-    __half2* grad_h = nullptr;
+    // hash-table gradients during the backward: one copy per XCD (FP32 or
+    // packed FP16), summed into grad afterwards
+    float* grad_x = nullptr;
+    __half2* grad_xh = nullptr;
     uint32_t* counters = nullptr;
     uint32_t* h_counters = nullptr;
     uint64_t rays_total = 0;
@@ -639,6 +645,40 @@ __global__ __launch_bounds__(256) void train_loss_kernel(LossArgs a, Pcg32 rng0,
     }
 }
 
+// The XCD (accelerator complex die) this wave runs on, 0 .. kXcds - 1: the
+// hardware register XCC_ID (s_getreg_b32, register 20).
+INSTA_DEV uint32_t xcd_id() { return (uint32_t)__builtin_amdgcn_s_getreg((31 << 11) | 20) % kXcds; }
+
+// Sum of the per-XCD copies of the table gradients into grad_tab (FP32; the
+// copies summed in a fixed order); the copies are cleared before each
+// backward.
+__global__ void grad_copies_reduce_kernel(const float* __restrict__ grad_x, const __half2* __restrict__ grad_xh,
+                                          uint64_t entries, uint32_t fp16, float* __restrict__ grad_tab,
+                                          uint32_t accumulate) {
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < entries;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        float a = 0.0f, b = 0.0f;
+#pragma unroll
+        for (uint32_t x = 0; x < kXcds; ++x) {
+            if (fp16) {
+                const __half2 h = grad_xh[x * entries + e];
+                a = a + __low2float(h);
+                b = b + __high2float(h);
+            } else {
+                const float2 v = *reinterpret_cast<const float2*>(grad_x + 2 * (x * entries + e));
+                a = a + v.x;
+                b = b + v.y;
+            }
+        }
+        if (accumulate) {
+            a = a + grad_tab[2 * e];
+            b = b + grad_tab[2 * e + 1];
+        }
+        grad_tab[2 * e] = a;
+        grad_tab[2 * e + 1] = b;
+    }
+}
+
 // Sum of v over the run of equal keys that starts at this lane, in the run's
 // first lane (other lanes' sums are partial).  Runs are maximal stretches of
 // consecutive lanes with one key; every lane of the wave is active.
@@ -679,9 +719,9 @@ __global__ __launch_bounds__(kMlpThreads) void train_backward_kernel(const float
                                                                        const float* __restrict__ act,
                                                                        const float4* __restrict__ dout,
                                                                        float* __restrict__ dact, uint32_t ld,
-                                                                       float* __restrict__ grad_tab,
-                                                                       __half2* __restrict__ grad_tab_h,
-                                                                       uint32_t run_mask) {
+                                                                       float* __restrict__ grad_x,
+                                                                       __half2* __restrict__ grad_xh,
+                                                                       uint64_t entries, uint32_t run_mask) {
     constexpr int E = 2 * L, DIN = E + X;  // encoding, density input (+ extra dims)
     using A = Act<DIN>;
     using O = Off<DIN>;
@@ -794,7 +834,13 @@ __global__ __launch_bounds__(kMlpThreads) void train_backward_kernel(const float
         dact[(size_t)(Dact::DRGB + 1) * ld + i] = d4.y;
         dact[(size_t)(Dact::DRGB + 2) * ld + i] = d4.z;
     }
-    // hash-grid scatter (the corner/weight arithmetic of encode_level)
+    // hash-grid scatter (the corner/weight arithmetic of encode_level) into
+    // this XCD's copy of the table gradients: a copy's lines stay in one
+    // XCD's L2, instead of moving between the L2s of the eight XCDs that
+    // would otherwise add to the same entries
+    const uint32_t xcd = xcd_id();
+    float* __restrict__ grad_tab = grad_x + (size_t)xcd * 2u * entries;
+    __half2* __restrict__ grad_tab_h = grad_xh + (size_t)xcd * entries;
     const float4 p4 = cpos[c_idx[ic]];
     const float xs[3] = {p4.x, p4.y, p4.z};
 #pragma unroll 1
@@ -1075,18 +1121,29 @@ insta_status launch_train_backward(insta_ctx* ctx, TrainState* ts, const insta_t
     const uint32_t n_mlp = (uint32_t)ctx->layout.n_mlp_params;
     const dim3 sgrid((cap + kMlpThreads - 1) / kMlpThreads);
     float* grad_tab = ts->grad + n_mlp;
+    const uint64_t entries = ctx->layout.n_grid_params / 2;
+    // clear this backward's copies (a fill kernel: capturable)
+    if (tc.grid_grad_fp16)
+        hipLaunchKernelGGL(fill_zero_kernel, dim3(2048), dim3(256), 0, stream, reinterpret_cast<uint32_t*>(ts->grad_xh),
+                           kXcds * entries);
+    else
+        hipLaunchKernelGGL(fill_zero_kernel, dim3(2048), dim3(256), 0, stream, reinterpret_cast<uint32_t*>(ts->grad_x),
+                           kXcds * 2 * entries);
     // levels whose cells span at least 1 / kRunMaxStepCells marching steps
     uint32_t run_mask = 0;
     for (int l = 0; l < L; ++l)
         if (ctx->cfg.dt * gp.scale[l] <= kRunMaxStepCells) run_mask |= 1u << l;
     if (tc.grid_grad_fp16)
         hipLaunchKernelGGL((train_backward_kernel<L, true, X>), sgrid, dim3(kMlpThreads), 0, stream, ctx->d_params, gp,
-                           n_dev, ts->c_idx, ts->s_cpos, ts->act, ts->c_dout, ts->dact, cap, grad_tab, ts->grad_h,
-                           run_mask);
+                           n_dev, ts->c_idx, ts->s_cpos, ts->act, ts->c_dout, ts->dact, cap, ts->grad_x, ts->grad_xh,
+                           entries, run_mask);
     else
         hipLaunchKernelGGL((train_backward_kernel<L, false, X>), sgrid, dim3(kMlpThreads), 0, stream, ctx->d_params, gp,
-                           n_dev, ts->c_idx, ts->s_cpos, ts->act, ts->c_dout, ts->dact, cap, grad_tab, ts->grad_h,
-                           run_mask);
+                           n_dev, ts->c_idx, ts->s_cpos, ts->act, ts->c_dout, ts->dact, cap, ts->grad_x, ts->grad_xh,
+                           entries, run_mask);
+    INSTA_HIP(hipGetLastError());
+    hipLaunchKernelGGL(grad_copies_reduce_kernel, dim3(2048), dim3(256), 0, stream, ts->grad_x, ts->grad_xh, entries,
+                       tc.grid_grad_fp16 ? 1u : 0u, grad_tab, 0u);
     INSTA_HIP(hipGetLastError());
     WArgs wa;
     wa.p[0] = WProduct{Dact::DH1, A::ENC, kW, DIN, O::W0};
@@ -1266,7 +1323,7 @@ void free_train(insta_ctx* ctx) {
     if (!ts) return;
     free_workspace_train(ts);
     dev_free(ts->images); dev_free(ts->meta); dev_free(ts->seg); dev_free(ts->depth);
-    dev_free(ts->grad); dev_free(ts->grad_h);
+    dev_free(ts->grad); dev_free(ts->grad_x); dev_free(ts->grad_xh);
     dev_free(ts->adam_m); dev_free(ts->adam_v); dev_free(ts->counters); dev_free(ts->net_extra);
     if (ctx->net_src == ts->ema_infer) ctx->net_src = nullptr;
     dev_free(ts->ema); dev_free(ts->ema_infer);
@@ -1288,7 +1345,8 @@ static insta_status ensure_train_state(insta_ctx* ctx, hipStream_t stream) {
         bool ok = train_alloc(ts->grad, np * sizeof(float)) && train_alloc(ts->adam_m, np * sizeof(float)) &&
                   train_alloc(ts->adam_v, np * sizeof(float)) && train_alloc(ts->ema, np * sizeof(float)) &&
                   train_alloc(ts->ema_infer, np * sizeof(float)) &&
-                  train_alloc(ts->grad_h, (ctx->layout.n_grid_params / 2 + 1) * sizeof(__half2)) &&
+                  train_alloc(ts->grad_x, kXcds * ctx->layout.n_grid_params * sizeof(float)) &&
+                  train_alloc(ts->grad_xh, kXcds * (ctx->layout.n_grid_params / 2) * sizeof(__half2)) &&
                   train_alloc(ts->counters, kTCounters * sizeof(uint32_t)) &&
                   hipHostMalloc((void**)&ts->h_counters, kTCounters * sizeof(uint32_t), hipHostMallocDefault) ==
                       hipSuccess;
@@ -1374,9 +1432,6 @@ insta_status train_step(insta_ctx* ctx, const insta_train_config& tc, insta_trai
     const uint32_t cap = tc.max_samples;
     // zero gradients with a fill kernel (capturable, unlike memset nodes)
     hipLaunchKernelGGL(fill_zero_kernel, dim3(1024), dim3(256), 0, stream, reinterpret_cast<uint32_t*>(ts->grad), np);
-    if (tc.grid_grad_fp16)
-        hipLaunchKernelGGL(fill_zero_kernel, dim3(1024), dim3(256), 0, stream,
-                           reinterpret_cast<uint32_t*>(ts->grad_h), ctx->layout.n_grid_params / 2);
     hipLaunchKernelGGL(train_clear_kernel, dim3(1), dim3(64), 0, stream, ts->counters, cap);
     INSTA_HIP(hipGetLastError());
 
@@ -1453,7 +1508,7 @@ insta_status train_step(insta_ctx* ctx, const insta_train_config& tc, insta_trai
     const float bc1 = tc.apply_update ? 1.0f - (float)std::pow((double)tc.beta1, (double)t) : 1.0f;
     const float bc2 = tc.apply_update ? 1.0f - (float)std::pow((double)tc.beta2, (double)t) : 1.0f;
     hipLaunchKernelGGL(adam_kernel, dim3(2048), dim3(256), 0, stream, ctx->d_params, ts->grad,
-                       tc.grid_grad_fp16 ? ts->grad_h : nullptr, ts->adam_m, ts->adam_v, np, n_mlp,
+                       nullptr, ts->adam_m, ts->adam_v, np, n_mlp,
                        1.0f / tc.loss_scale, tc.learning_rate, tc.beta1, tc.beta2, tc.epsilon, bc1, bc2,
                        tc.l2_reg, tc.apply_update);
     INSTA_HIP(hipGetLastError());
