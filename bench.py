@@ -436,7 +436,9 @@ def training_bench(args, full=False):
     shapes = ("full-size shapes (L16 T2^19, 128^3 grid, 9,976-tri head, <=1024 steps)" if full
               else "small shapes (L8 T2^14, 64^3 grid, 2,048-tri sphere)")
     out["workload"] = (f"{shapes}, 64 posed frames x 512x512 random cameras, a batch of up to 2^18 composited "
-                       "samples per step (adaptive ray count, at most 2^18 rays; timed after it holds), "
+                       "samples per step (adaptive ray count, at most 2^18 rays; timed after 60 warm-up steps, "
+                       "once it holds the batch; the rays per step keep growing as training terminates rays "
+                       "earlier), "
                        "Adam(0.9, 0.99, lr 1e-2)")
     return out
 
@@ -577,7 +579,9 @@ def main():
                          "frame's lists for one render; 2 measured best for both avatar sizes)")
     ap.add_argument("--train-steps", type=int, default=40)
     ap.add_argument("--train-sizes", choices=["both", "small", "full"], default="both")
-    ap.add_argument("--train-warmup", type=int, default=8)
+    ap.add_argument("--train-warmup", type=int, default=60,
+                    help="training steps before the timed ones (the adaptive ray count then holds the batch "
+                         "at 2^18 samples)")
     ap.add_argument("--train-warmup-max", type=int, default=120,
                     help="warm-up steps allowed for the batch to reach 2^18 samples")
     ap.add_argument("--cell-split", type=int, default=None,
