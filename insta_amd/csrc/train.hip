@@ -137,6 +137,7 @@ struct TrainState {
     // packed FP16), summed into grad afterwards
     float* grad_x = nullptr;
     __half2* grad_xh = nullptr;
+    float* denc = nullptr;         // dL/dencoding of the batch [32][batch cap], for the LDS scatter
     uint32_t* counters = nullptr;
     uint32_t* h_counters = nullptr;
     uint64_t rays_total = 0;
@@ -650,32 +651,100 @@ __global__ __launch_bounds__(256) void train_loss_kernel(LossArgs a, Pcg32 rng0,
 INSTA_DEV uint32_t xcd_id() { return (uint32_t)__builtin_amdgcn_s_getreg((31 << 11) | 20) % kXcds; }
 
 // Sum of the per-XCD copies of the table gradients into grad_tab (FP32; the
-// copies summed in a fixed order); the copies are cleared before each
-// backward.
+// copies summed in a fixed order, the FP32 ones then the FP16 ones, each set
+// when in use); the copies are cleared before each backward.
 __global__ void grad_copies_reduce_kernel(const float* __restrict__ grad_x, const __half2* __restrict__ grad_xh,
-                                          uint64_t entries, uint32_t fp16, float* __restrict__ grad_tab,
-                                          uint32_t accumulate) {
+                                          uint64_t entries, uint32_t use_x, uint32_t use_xh,
+                                          float* __restrict__ grad_tab, uint32_t accumulate) {
     for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < entries;
          e += (uint64_t)gridDim.x * blockDim.x) {
         float a = 0.0f, b = 0.0f;
+        if (use_x)
 #pragma unroll
-        for (uint32_t x = 0; x < kXcds; ++x) {
-            if (fp16) {
-                const __half2 h = grad_xh[x * entries + e];
-                a = a + __low2float(h);
-                b = b + __high2float(h);
-            } else {
+            for (uint32_t x = 0; x < kXcds; ++x) {
                 const float2 v = *reinterpret_cast<const float2*>(grad_x + 2 * (x * entries + e));
                 a = a + v.x;
                 b = b + v.y;
             }
-        }
+        if (use_xh)
+#pragma unroll
+            for (uint32_t x = 0; x < kXcds; ++x) {
+                const __half2 h = grad_xh[x * entries + e];
+                a = a + __low2float(h);
+                b = b + __high2float(h);
+            }
         if (accumulate) {
             a = a + grad_tab[2 * e];
             b = b + grad_tab[2 * e + 1];
         }
         grad_tab[2 * e] = a;
         grad_tab[2 * e + 1] = b;
+    }
+}
+
+// Hash-grid scatter of the levels whose table fits in LDS (at most
+// kLdsEntries entries): one workgroup per (level, chunk of the batch) adds
+// its samples' corner gradients into an LDS copy of the level's table (LDS
+// float atomics), then adds the table's non-zero entries to this XCD's copy
+// of the gradients.  Global atomics drop from eight corners x two features
+// per sample to two per touched entry per chunk.  Same corner and weight
+// arithmetic as the per-sample scatter.
+constexpr uint32_t kLdsEntries = 16384;
+constexpr uint32_t kScatterThreads = 1024;
+constexpr uint32_t kScatterChunk = 8192;  // samples per workgroup
+
+__global__ __launch_bounds__(kScatterThreads) void scatter_lds_kernel(GridParams gp, const uint32_t* __restrict__ n_dev,
+                                                                      const uint32_t* __restrict__ c_idx,
+                                                                      const float4* __restrict__ cpos,
+                                                                      const float* __restrict__ denc_g, uint32_t ld,
+                                                                      float* __restrict__ grad_x, uint64_t entries,
+                                                                      uint32_t lds_mask) {
+    __shared__ float tab[2 * kLdsEntries];
+    const uint32_t l = blockIdx.y;
+    if (!((lds_mask >> l) & 1u)) return;  // whole workgroup
+    const uint32_t n = *n_dev;
+    const uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    const uint32_t s0 = blockIdx.x * chunk, s1 = min(n, s0 + chunk);
+    if (s0 >= s1) return;  // whole workgroup
+    const uint32_t size = gp.size[l], off = gp.offset[l], res = gp.res[l];
+    const bool dense_l = (gp.dense_mask >> l) & 1u;
+    const float sc = gp.scale[l];
+    for (uint32_t j = threadIdx.x; j < 2 * size; j += kScatterThreads) tab[j] = 0.0f;
+    __syncthreads();
+    for (uint32_t i = s0 + threadIdx.x; i < s1; i += kScatterThreads) {
+        const float4 p4 = cpos[c_idx[i]];
+        const float xs[3] = {p4.x, p4.y, p4.z};
+        const float e0 = denc_g[(size_t)(2 * l) * ld + i], e1 = denc_g[(size_t)(2 * l + 1) * ld + i];
+        float frac[3];
+        uint32_t g[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const float pp = xs[d] * sc + 0.5f;
+            const float fl = floorf(pp);
+            frac[d] = pp - fl;
+            g[d] = (uint32_t)(int32_t)fl;
+        }
+#pragma unroll
+        for (uint32_t c = 0; c < 8; ++c) {
+            float w = 1.0f;
+            uint32_t cg[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const bool up = (c >> k) & 1u;
+                w = w * (up ? frac[k] : 1.0f - frac[k]);
+                cg[k] = g[k] + (up ? 1u : 0u);
+            }
+            const uint32_t idx = grid_index(cg, dense_l, res, size);
+            atomicAdd(&tab[2 * idx], w * e0);
+            atomicAdd(&tab[2 * idx + 1], w * e1);
+        }
+    }
+    __syncthreads();
+    float* __restrict__ grad_tab = grad_x + (size_t)xcd_id() * 2u * entries + 2u * (size_t)off;
+    for (uint32_t j = threadIdx.x; j < size; j += kScatterThreads) {
+        const float a = tab[2 * j], b = tab[2 * j + 1];
+        if (a != 0.0f) atomicAdd(grad_tab + 2 * j, a);
+        if (b != 0.0f) atomicAdd(grad_tab + 2 * j + 1, b);
     }
 }
 
@@ -721,7 +790,9 @@ __global__ __launch_bounds__(kMlpThreads) void train_backward_kernel(const float
                                                                        float* __restrict__ dact, uint32_t ld,
                                                                        float* __restrict__ grad_x,
                                                                        __half2* __restrict__ grad_xh,
-                                                                       uint64_t entries, uint32_t run_mask) {
+                                                                       uint64_t entries, uint32_t run_mask,
+                                                                       float* __restrict__ denc_g,
+                                                                       uint32_t lds_mask) {
     constexpr int E = 2 * L, DIN = E + X;  // encoding, density input (+ extra dims)
     using A = Act<DIN>;
     using O = Off<DIN>;
@@ -845,6 +916,14 @@ __global__ __launch_bounds__(kMlpThreads) void train_backward_kernel(const float
     const float xs[3] = {p4.x, p4.y, p4.z};
 #pragma unroll 1
     for (int l = 0; l < L; ++l) {
+        if ((lds_mask >> l) & 1u) {
+            // scattered by scatter_lds_kernel, which reads dL/dencoding from memory
+            if (valid) {
+                denc_g[(size_t)(2 * l) * ld + i] = s_denc[2 * l][threadIdx.x];
+                denc_g[(size_t)(2 * l + 1) * ld + i] = s_denc[2 * l + 1][threadIdx.x];
+            }
+            continue;
+        }
         const float sc = gp.scale[l];
         float frac[3];
         uint32_t g[3];
@@ -1040,6 +1119,7 @@ void free_workspace_train(TrainState* ts) {
     dev_free(ts->s_pos); dev_free(ts->s_cpos); dev_free(ts->s_cdir); dev_free(ts->s_out); dev_free(ts->s_hit);
     dev_free(ts->c_idx); dev_free(ts->c_dout); dev_free(ts->act); dev_free(ts->dact); dev_free(ts->part);
     dev_free(ts->t_buf);
+    dev_free(ts->denc);
     dev_free(ts->ray_T); dev_free(ts->ray_alive); dev_free(ts->ray_gbase);
     dev_free(ts->r_pos); dev_free(ts->r_cpos); dev_free(ts->r_cdir); dev_free(ts->r_out);
     ts->ray_cap = ts->sample_cap = ts->compact_cap = 0;
@@ -1074,6 +1154,7 @@ insta_status grow_train_workspace(insta_ctx* ctx, TrainState* ts, uint32_t n_ray
     ok = ok && train_alloc(ts->dact, (size_t)Dact::N * cc * sizeof(float));
     ok = ok && train_alloc(ts->part, ((cc + kChunk - 1) / kChunk) * n_mlp * sizeof(float));
     ok = ok && train_alloc(ts->t_buf, (size_t)rc * ctx->cfg.max_steps * sizeof(float));
+    ok = ok && train_alloc(ts->denc, (size_t)2 * INSTA_MAX_LEVELS * cc * sizeof(float));
     if (!ok) {
         free_workspace_train(ts);
         set_error("training workspace allocation failed");
@@ -1122,11 +1203,18 @@ insta_status launch_train_backward(insta_ctx* ctx, TrainState* ts, const insta_t
     const dim3 sgrid((cap + kMlpThreads - 1) / kMlpThreads);
     float* grad_tab = ts->grad + n_mlp;
     const uint64_t entries = ctx->layout.n_grid_params / 2;
-    // clear this backward's copies (a fill kernel: capturable)
-    if (tc.grid_grad_fp16)
+    // levels whose tables fit in LDS are scattered by scatter_lds_kernel,
+    // the others per sample (FP32, or packed FP16 copies)
+    uint32_t lds_mask = 0;
+    for (int l = 0; l < L; ++l)
+        if (gp.size[l] <= kLdsEntries) lds_mask |= 1u << l;
+    const bool use_xh = tc.grid_grad_fp16 && lds_mask != (1u << L) - 1u;
+    const bool use_x = !tc.grid_grad_fp16 || lds_mask != 0;
+    // clear this backward's copies (fill kernels: capturable)
+    if (use_xh)
         hipLaunchKernelGGL(fill_zero_kernel, dim3(2048), dim3(256), 0, stream, reinterpret_cast<uint32_t*>(ts->grad_xh),
                            kXcds * entries);
-    else
+    if (use_x)
         hipLaunchKernelGGL(fill_zero_kernel, dim3(2048), dim3(256), 0, stream, reinterpret_cast<uint32_t*>(ts->grad_x),
                            kXcds * 2 * entries);
     // levels whose cells span at least 1 / kRunMaxStepCells marching steps
@@ -1136,14 +1224,20 @@ insta_status launch_train_backward(insta_ctx* ctx, TrainState* ts, const insta_t
     if (tc.grid_grad_fp16)
         hipLaunchKernelGGL((train_backward_kernel<L, true, X>), sgrid, dim3(kMlpThreads), 0, stream, ctx->d_params, gp,
                            n_dev, ts->c_idx, ts->s_cpos, ts->act, ts->c_dout, ts->dact, cap, ts->grad_x, ts->grad_xh,
-                           entries, run_mask);
+                           entries, run_mask, ts->denc, lds_mask);
     else
         hipLaunchKernelGGL((train_backward_kernel<L, false, X>), sgrid, dim3(kMlpThreads), 0, stream, ctx->d_params, gp,
                            n_dev, ts->c_idx, ts->s_cpos, ts->act, ts->c_dout, ts->dact, cap, ts->grad_x, ts->grad_xh,
-                           entries, run_mask);
+                           entries, run_mask, ts->denc, lds_mask);
     INSTA_HIP(hipGetLastError());
+    if (lds_mask) {
+        const uint32_t chunks = std::min<uint32_t>(std::max<uint32_t>((cap + kScatterChunk - 1) / kScatterChunk, 1u), 64u);
+        hipLaunchKernelGGL(scatter_lds_kernel, dim3(chunks, L), dim3(kScatterThreads), 0, stream, gp, n_dev, ts->c_idx,
+                           ts->s_cpos, ts->denc, cap, ts->grad_x, entries, lds_mask);
+        INSTA_HIP(hipGetLastError());
+    }
     hipLaunchKernelGGL(grad_copies_reduce_kernel, dim3(2048), dim3(256), 0, stream, ts->grad_x, ts->grad_xh, entries,
-                       tc.grid_grad_fp16 ? 1u : 0u, grad_tab, 0u);
+                       use_x ? 1u : 0u, use_xh ? 1u : 0u, grad_tab, 0u);
     INSTA_HIP(hipGetLastError());
     WArgs wa;
     wa.p[0] = WProduct{Dact::DH1, A::ENC, kW, DIN, O::W0};
