@@ -45,6 +45,7 @@
 #include <cmath>
 #include <cstring>
 #include <vector>
+#include <type_traits>
 
 #include "context.hpp"
 #include "encoding.hpp"
@@ -120,7 +121,7 @@ struct TrainState {
     float4* ray_o = nullptr;       // origin, w = length of the unnormalised direction
     int32_t* ray_frame = nullptr;
     uint32_t *ray_img = nullptr, *ray_pix = nullptr, *ray_src = nullptr, *ray_base = nullptr, *ray_cnt = nullptr;
-    float4 *s_pos = nullptr, *s_cpos = nullptr, *s_cdir = nullptr, *s_out = nullptr, *s_dout = nullptr;
+    float4 *s_cpos = nullptr, *s_cdir = nullptr, *s_out = nullptr, *s_dout = nullptr;
     // evaluation rounds: per ray transmittance, alive flag and the first slot
     // of its samples in the round buffers (r_*: one window's samples, packed)
     float* ray_T = nullptr;
@@ -130,7 +131,7 @@ struct TrainState {
     uint32_t* c_idx = nullptr;     // compacted batch: sample index
     float4* c_dout = nullptr;      // compacted batch: dL/d(rgb logits, density logit)
     float *act = nullptr, *dact = nullptr, *part = nullptr;
-    float* t_buf = nullptr;        // ray march scratch, [max_steps][ray_cap]
+    float* t_buf = nullptr;        // march t values, step-major [max_steps][rays of the step]: the rounds and the loss read them
     // optimizer
     float *grad = nullptr, *adam_m = nullptr, *adam_v = nullptr;
     // hash-table gradients during the backward: one copy per XCD (FP32 or
@@ -138,6 +139,10 @@ struct TrainState {
     float* grad_x = nullptr;
     __half2* grad_xh = nullptr;
     float* denc = nullptr;         // dL/dencoding of the batch [32][batch cap], for the LDS scatter
+    // LDS-scattered levels (a prefix of the levels, entries [0, lds_end)):
+    // one partial table per chunk of the batch, [kScatterMaxChunks][2 lds_end]
+    float* grad_p = nullptr;
+    uint32_t lds_levels = 0, lds_end = 0;
     uint32_t* counters = nullptr;
     uint32_t* h_counters = nullptr;
     uint64_t rays_total = 0;
@@ -223,7 +228,6 @@ __global__ __launch_bounds__(256) void train_rays_kernel(RayArgs a, uint32_t n_r
                                                          uint32_t* __restrict__ ray_src,
                                                          uint32_t* __restrict__ ray_base,
                                                          uint32_t* __restrict__ ray_cnt,
-                                                         float4* __restrict__ s_pos,
                                                          float* __restrict__ t_buf,
                                                          float* __restrict__ ray_T,
                                                          uint32_t* __restrict__ ray_alive,
@@ -266,10 +270,6 @@ __global__ __launch_bounds__(256) void train_rays_kernel(RayArgs a, uint32_t n_r
     ray_cnt[r] = count;
     ray_T[r] = 1.0f;
     ray_alive[r] = 1u;
-    for (uint32_t j = 0; j < count; ++j) {
-        const V3 pos = o + d * t_buf[(size_t)j * n_rays + i];
-        s_pos[base + j] = make_float4(pos.x, pos.y, pos.z, __uint_as_float(r));
-    }
 }
 
 template <int NIN, int NOUT>
@@ -461,20 +461,29 @@ __global__ __launch_bounds__(256) void train_gather_kernel(uint32_t lo, uint32_t
                                                            const uint32_t* __restrict__ ray_cnt,
                                                            const uint32_t* __restrict__ ray_alive,
                                                            uint32_t* __restrict__ ray_gbase,
-                                                           const float4* __restrict__ s_pos,
+                                                           const float4* __restrict__ ray_o,
+                                                           const float4* __restrict__ ray_d,
+                                                           const uint32_t* __restrict__ ray_src,
+                                                           const float* __restrict__ t_buf, uint32_t t_stride,
                                                            float4* __restrict__ r_pos,
                                                            uint32_t* __restrict__ counters, uint32_t round_slot) {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t n = 0, base = 0;
+    uint32_t n = 0;
     if (r < counters[kTRays] && ray_alive[r]) {
         const uint32_t cnt = ray_cnt[r];
         n = cnt > lo ? min(cnt, hi) - lo : 0u;
-        base = ray_base[r];
     }
     const uint32_t g = wave_reserve(&counters[round_slot], n);
     if (n == 0) return;
     ray_gbase[r] = g;
-    for (uint32_t k = 0; k < n; ++k) r_pos[g + k] = s_pos[base + lo + k];
+    // posed positions from the march's t values (the march's own arithmetic)
+    const float4 o4 = ray_o[r], d4 = ray_d[r];
+    const V3 o = v3(o4.x, o4.y, o4.z), d = v3(d4.x, d4.y, d4.z);
+    const float* __restrict__ tr = t_buf + ray_src[r];
+    for (uint32_t k = 0; k < n; ++k) {
+        const V3 pos = o + d * tr[(size_t)(lo + k) * t_stride];
+        r_pos[g + k] = make_float4(pos.x, pos.y, pos.z, __uint_as_float(r));
+    }
 }
 
 // Round `w` scatter and transmittance: each alive ray's evaluated samples go
@@ -522,7 +531,8 @@ __global__ __launch_bounds__(256) void train_loss_kernel(LossArgs a, Pcg32 rng0,
                                                          const uint32_t* __restrict__ ray_base,
                                                          const uint32_t* __restrict__ ray_cnt,
                                                          const float4* __restrict__ ray_o,
-                                                         const float4* __restrict__ s_pos,
+                                                         const float4* __restrict__ ray_d,
+                                                         const float* __restrict__ t_buf, uint32_t t_stride,
                                                          const float4* __restrict__ s_out,
                                                          uint32_t* __restrict__ c_idx, float4* __restrict__ c_dout,
                                                          uint32_t compact_cap, uint32_t* __restrict__ counters) {
@@ -533,9 +543,11 @@ __global__ __launch_bounds__(256) void train_loss_kernel(LossArgs a, Pcg32 rng0,
     float acc[3] = {0.0f, 0.0f, 0.0f}, grad[3] = {0.0f, 0.0f, 0.0f};
     // depth supervision: the composited depth and dL/dD (0: none for this ray)
     float depth_ray = 0.0f, dgrad = 0.0f;
-    V3 org = v3(0.0f, 0.0f, 0.0f);
+    V3 org = v3(0.0f, 0.0f, 0.0f), dir = v3(0.0f, 0.0f, 1.0f);
+    const float* __restrict__ tr = t_buf;  // this ray's march t values, stride t_stride
     if (valid) {
         base = ray_base[r];
+        tr = t_buf + ray_src[r];
         const uint32_t cnt = ray_cnt[r];
         Pcg32 rng = rng0;
         rng.advance((uint64_t)ray_src[r] * kRaySeedStride);
@@ -565,8 +577,9 @@ __global__ __launch_bounds__(256) void train_loss_kernel(LossArgs a, Pcg32 rng0,
             if (id <= 90u || id > 250u) lam = 0.0f;
             if (tex.w < 0.001f) lam = 0.0f;
         }
-        const float4 ro = ray_o[r];
+        const float4 ro = ray_o[r], rd = ray_d[r];
         org = v3(ro.x, ro.y, ro.z);
+        dir = v3(rd.x, rd.y, rd.z);
         const float target_depth = ro.w * ((lam > 0.0f && a.depth) ? a.depth[texel] : -1.0f);
         const bool supervise_depth = target_depth > 0.0f;
         float T = 1.0f;
@@ -580,8 +593,8 @@ __global__ __launch_bounds__(256) void train_loss_kernel(LossArgs a, Pcg32 rng0,
             acc[1] = acc[1] + w * sigmoid(o.y);
             acc[2] = acc[2] + w * sigmoid(o.z);
             if (supervise_depth) {
-                const float4 p = s_pos[base + used];
-                depth_ray = depth_ray + w * norm(v3(p.x, p.y, p.z) - org);
+                const V3 p = org + dir * tr[(size_t)used * t_stride];
+                depth_ray = depth_ray + w * norm(p - org);
             }
             T = T * (1.0f - alpha);  // = transmit(T, o.w, ...), which the rounds use
         }
@@ -623,8 +636,8 @@ __global__ __launch_bounds__(256) void train_loss_kernel(LossArgs a, Pcg32 rng0,
             }
             if (dgrad != 0.0f) {
                 // dD/dsigma_j / dt = T_{j+1} depth_j - (depth suffix after j)
-                const float4 p = s_pos[base + j];
-                const float dj = norm(v3(p.x, p.y, p.z) - org);
+                const V3 p = org + dir * tr[(size_t)j * t_stride];
+                const float dj = norm(p - org);
                 depth2 = depth2 + w * dj;
                 dsig = dsig + dgrad * (T * dj - (depth_ray - depth2));
             }
@@ -650,16 +663,24 @@ __global__ __launch_bounds__(256) void train_loss_kernel(LossArgs a, Pcg32 rng0,
 // hardware register XCC_ID (s_getreg_b32, register 20).
 INSTA_DEV uint32_t xcd_id() { return (uint32_t)__builtin_amdgcn_s_getreg((31 << 11) | 20) % kXcds; }
 
-// Sum of the per-XCD copies of the table gradients into grad_tab (FP32; the
-// copies summed in a fixed order, the FP32 ones then the FP16 ones, each set
-// when in use); the copies are cleared before each backward.
+// Table gradients into grad_tab (FP32): entries below lds_end are the sum of
+// the LDS scatter's chunk partials (chunk order), the others the sum of the
+// per-XCD copies (a fixed order, the FP32 ones then the FP16 ones, each set
+// when in use; the copies are cleared before each backward).
 __global__ void grad_copies_reduce_kernel(const float* __restrict__ grad_x, const __half2* __restrict__ grad_xh,
                                           uint64_t entries, uint32_t use_x, uint32_t use_xh,
+                                          const float* __restrict__ grad_p, uint32_t lds_end, uint32_t chunks,
                                           float* __restrict__ grad_tab, uint32_t accumulate) {
     for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < entries;
          e += (uint64_t)gridDim.x * blockDim.x) {
         float a = 0.0f, b = 0.0f;
-        if (use_x)
+        if (e < lds_end) {
+            for (uint32_t c = 0; c < chunks; ++c) {
+                const float2 v = *reinterpret_cast<const float2*>(grad_p + 2 * ((uint64_t)c * lds_end + e));
+                a = a + v.x;
+                b = b + v.y;
+            }
+        } else if (use_x)
 #pragma unroll
             for (uint32_t x = 0; x < kXcds; ++x) {
                 const float2 v = *reinterpret_cast<const float2*>(grad_x + 2 * (x * entries + e));
@@ -682,39 +703,92 @@ __global__ void grad_copies_reduce_kernel(const float* __restrict__ grad_x, cons
     }
 }
 
-// Hash-grid scatter of the levels whose table fits in LDS (at most
-// kLdsEntries entries): one workgroup per (level, chunk of the batch) adds
-// its samples' corner gradients into an LDS copy of the level's table (LDS
-// float atomics), then adds the table's non-zero entries to this XCD's copy
-// of the gradients.  Global atomics drop from eight corners x two features
-// per sample to two per touched entry per chunk.  Same corner and weight
-// arithmetic as the per-sample scatter.
+// Sum of v over the run of equal keys that starts at this lane, in the run's
+// first lane (other lanes' sums are partial).  Runs are maximal stretches of
+// consecutive lanes with one key; every lane of the wave is active.
+INSTA_DEV float2 run_sum(uint32_t key, float2 v, bool& head) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t prev = __shfl_up(key, 1u);
+    head = lane == 0u || prev != key;
+    const uint64_t heads = __ballot(head);
+    const uint64_t later = lane == 63u ? 0ull : heads & (~0ull << (lane + 1u));
+    const uint32_t end = later ? (uint32_t)__builtin_ctzll(later) - 1u : 63u;
+#pragma unroll
+    for (uint32_t off = 1; off < 64u; off <<= 1) {
+        const float ox = __shfl_down(v.x, off), oy = __shfl_down(v.y, off);
+        if (lane + off <= end) {
+            v.x = v.x + ox;
+            v.y = v.y + oy;
+        }
+    }
+    return v;
+}
+
+// Hash-grid scatter of the leading levels whose tables fit in LDS (at most
+// kLdsEntries entries each): one workgroup per (level, chunk of the batch)
+// adds its samples' corner gradients into an LDS copy of the level's table
+// (LDS float atomics; on levels in run_mask a wave first sums runs of
+// consecutive samples with one corner, as the per-sample scatter does), then
+// stores the whole table as its chunk's partial (plain coalesced stores, no
+// global atomics); grad_copies_reduce_kernel sums the partials in chunk
+// order.  Same corner and weight arithmetic as the per-sample scatter.
 constexpr uint32_t kLdsEntries = 16384;
 constexpr uint32_t kScatterThreads = 1024;
-constexpr uint32_t kScatterChunk = 8192;  // samples per workgroup
+constexpr uint32_t kScatterChunk = 8192;    // samples per workgroup
+constexpr uint32_t kScatterMaxChunks = 64;  // partial tables
+
+// The leading levels whose tables fit in LDS, as a mask, and the first entry
+// after them.
+uint32_t lds_scatter_levels(const GridParams& gp, int n_levels, uint32_t* end) {
+    uint32_t mask = 0, e = 0;
+    for (int l = 0; l < n_levels && gp.size[l] <= kLdsEntries; ++l) {
+        mask |= 1u << l;
+        e = gp.offset[l] + gp.size[l];
+    }
+    *end = e;
+    return mask;
+}
 
 __global__ __launch_bounds__(kScatterThreads) void scatter_lds_kernel(GridParams gp, const uint32_t* __restrict__ n_dev,
                                                                       const uint32_t* __restrict__ c_idx,
                                                                       const float4* __restrict__ cpos,
                                                                       const float* __restrict__ denc_g, uint32_t ld,
-                                                                      float* __restrict__ grad_x, uint64_t entries,
-                                                                      uint32_t lds_mask) {
+                                                                      uint32_t run_mask, float* __restrict__ grad_p,
+                                                                      uint32_t lds_end) {
     __shared__ float tab[2 * kLdsEntries];
-    const uint32_t l = blockIdx.y;
-    if (!((lds_mask >> l) & 1u)) return;  // whole workgroup
+    const uint32_t l = blockIdx.y;  // grid.y: the LDS-scattered levels
     const uint32_t n = *n_dev;
     const uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
-    const uint32_t s0 = blockIdx.x * chunk, s1 = min(n, s0 + chunk);
-    if (s0 >= s1) return;  // whole workgroup
+    // an empty chunk still stores its (zero) partial
+    const uint32_t s0 = min(n, blockIdx.x * chunk), s1 = min(n, s0 + chunk);
     const uint32_t size = gp.size[l], off = gp.offset[l], res = gp.res[l];
     const bool dense_l = (gp.dense_mask >> l) & 1u;
     const float sc = gp.scale[l];
     for (uint32_t j = threadIdx.x; j < 2 * size; j += kScatterThreads) tab[j] = 0.0f;
     __syncthreads();
-    for (uint32_t i = s0 + threadIdx.x; i < s1; i += kScatterThreads) {
-        const float4 p4 = cpos[c_idx[i]];
-        const float xs[3] = {p4.x, p4.y, p4.z};
-        const float e0 = denc_g[(size_t)(2 * l) * ld + i], e1 = denc_g[(size_t)(2 * l + 1) * ld + i];
+    const bool runs = (run_mask >> l) & 1u;
+    // every thread runs every round (run_sum needs whole waves); the next
+    // round's sample is loaded before this round's corners are added
+    const uint32_t rounds = (s1 - s0 + kScatterThreads - 1) / kScatterThreads;
+    auto load = [&](uint32_t q, float4& p4, float& e0, float& e1) {
+        const uint32_t i = s0 + q * kScatterThreads + threadIdx.x;
+        if (i < s1) {
+            p4 = cpos[c_idx[i]];
+            e0 = denc_g[(size_t)(2 * l) * ld + i];
+            e1 = denc_g[(size_t)(2 * l + 1) * ld + i];
+        } else {
+            p4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            e0 = e1 = 0.0f;
+        }
+    };
+    float4 pn;
+    float en0 = 0.0f, en1 = 0.0f;
+    if (rounds) load(0, pn, en0, en1);
+    for (uint32_t q = 0; q < rounds; ++q) {
+        const bool valid = s0 + q * kScatterThreads + threadIdx.x < s1;
+        const float xs[3] = {pn.x, pn.y, pn.z};
+        const float e0 = en0, e1 = en1;
+        if (q + 1 < rounds) load(q + 1, pn, en0, en1);
         float frac[3];
         uint32_t g[3];
 #pragma unroll
@@ -735,38 +809,22 @@ __global__ __launch_bounds__(kScatterThreads) void scatter_lds_kernel(GridParams
                 cg[k] = g[k] + (up ? 1u : 0u);
             }
             const uint32_t idx = grid_index(cg, dense_l, res, size);
-            atomicAdd(&tab[2 * idx], w * e0);
-            atomicAdd(&tab[2 * idx + 1], w * e1);
+            float2 v = make_float2(w * e0, w * e1);
+            bool lead = valid;
+            if (runs) {
+                bool head;
+                v = run_sum(valid ? idx : ~0u, v, head);
+                lead = valid && head;
+            }
+            if (lead) {
+                atomicAdd(&tab[2 * idx], v.x);
+                atomicAdd(&tab[2 * idx + 1], v.y);
+            }
         }
     }
     __syncthreads();
-    float* __restrict__ grad_tab = grad_x + (size_t)xcd_id() * 2u * entries + 2u * (size_t)off;
-    for (uint32_t j = threadIdx.x; j < size; j += kScatterThreads) {
-        const float a = tab[2 * j], b = tab[2 * j + 1];
-        if (a != 0.0f) atomicAdd(grad_tab + 2 * j, a);
-        if (b != 0.0f) atomicAdd(grad_tab + 2 * j + 1, b);
-    }
-}
-
-// Sum of v over the run of equal keys that starts at this lane, in the run's
-// first lane (other lanes' sums are partial).  Runs are maximal stretches of
-// consecutive lanes with one key; every lane of the wave is active.
-INSTA_DEV float2 run_sum(uint32_t key, float2 v, bool& head) {
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t prev = __shfl_up(key, 1u);
-    head = lane == 0u || prev != key;
-    const uint64_t heads = __ballot(head);
-    const uint64_t later = lane == 63u ? 0ull : heads & (~0ull << (lane + 1u));
-    const uint32_t end = later ? (uint32_t)__builtin_ctzll(later) - 1u : 63u;
-#pragma unroll
-    for (uint32_t off = 1; off < 64u; off <<= 1) {
-        const float ox = __shfl_down(v.x, off), oy = __shfl_down(v.y, off);
-        if (lane + off <= end) {
-            v.x = v.x + ox;
-            v.y = v.y + oy;
-        }
-    }
-    return v;
+    float* __restrict__ dst = grad_p + 2u * ((size_t)blockIdx.x * lds_end + off);
+    for (uint32_t j = threadIdx.x; j < 2 * size; j += kScatterThreads) dst[j] = tab[j];
 }
 
 // Backward over a block of 256 compacted samples, in two phases.
@@ -979,17 +1037,20 @@ struct WArgs {
 
 // partial[c][out + o*I + i] = sum over samples n of chunk c of
 // dact[d_row + o][n] * act[x_row + i][n], on v_mfma_f32_16x16x4f32: the
-// samples are the K dimension; 32-sample slices of both operands are staged
-// through LDS (coalesced along the samples), wave w owns input tile i =
-// 16w..16w+15 and the four output tiles o = 0..63 of it, one MFMA per tile
-// and 4 samples (A = 16 outputs x 4 samples, B = 4 samples x 16 inputs).
+// samples are the K dimension; kS-sample slices of both operands are staged
+// through LDS (each thread loads 4 consecutive samples of kPass rows of
+// each operand as float4, coalesced along the samples), wave w owns input
+// tile i = 16w..16w+15 and the four output tiles o = 0..63 of it, one MFMA
+// per tile and 4 samples (A = 16 outputs x 4 samples, B = 4 samples x 16
+// inputs).
 typedef float wg_f4 __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(256) void wgrad_kernel(WArgs wa, const uint32_t* __restrict__ n_dev,
                                                     const float* __restrict__ act, const float* __restrict__ dact,
                                                     uint32_t ld, float* __restrict__ part, uint32_t n_mlp) {
-    constexpr uint32_t kS = 32;   // samples per LDS slice
-    constexpr uint32_t kP = 36;   // LDS row pitch: fragment reads (16 rows x 4 samples) spread over the banks
-    constexpr uint32_t kE = 64u * kS / 256u;  // slice elements per thread and operand
+    constexpr uint32_t kS = 32;            // samples per LDS slice
+    constexpr uint32_t kP = kS + 4;        // LDS row pitch: fragment reads (16 rows x 4 samples) spread over the banks
+    constexpr uint32_t kRowsPass = 1024u / kS;  // rows one pass of 256 threads covers (kS / 4 threads per row)
+    constexpr uint32_t kPass = 64u / kRowsPass;
     __shared__ float Ds[64 * kP];
     __shared__ float Xs[64 * kP];
     const WProduct pr = wa.p[blockIdx.y];
@@ -1001,39 +1062,62 @@ __global__ __launch_bounds__(256) void wgrad_kernel(WArgs wa, const uint32_t* __
     const uint32_t m = lane & 15u, g = lane >> 4;   // fragment row / k of this lane
     const uint32_t n_ot = (pr.O + 15u) / 16u;       // output tiles in use
     const bool active = 16u * wave < pr.I;          // this wave's input tile exists
+    const bool vec = (ld & 3u) == 0u;               // rows 16-byte aligned
+    const uint32_t c4 = (t % (kS / 4u)) * 4u;       // this thread's first sample in a slice
+    const uint32_t r0 = t / (kS / 4u);              // its rows r0 + kRowsPass p
     wg_f4 acc[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) acc[q] = wg_f4{0.0f, 0.0f, 0.0f, 0.0f};
+    const float* dp[kPass];
+    const float* xp[kPass];
+#pragma unroll
+    for (uint32_t q = 0; q < kPass; ++q) {
+        const uint32_t row = r0 + kRowsPass * q;
+        dp[q] = row < pr.O ? dact + (size_t)(pr.d_row + row) * ld : nullptr;
+        xp[q] = row < pr.I ? act + (size_t)(pr.x_row + row) * ld : nullptr;
+    }
+    auto ld4 = [&](const float* p, uint32_t smp) -> float4 {
+        if (!p) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (vec && smp + 4u <= s1) return *reinterpret_cast<const float4*>(p + smp);
+        return make_float4(smp < s1 ? p[smp] : 0.0f, smp + 1u < s1 ? p[smp + 1u] : 0.0f,
+                           smp + 2u < s1 ? p[smp + 2u] : 0.0f, smp + 3u < s1 ? p[smp + 3u] : 0.0f);
+    };
     // next slice in registers while the current one is multiplied
-    float dn[kE], xn[kE];
+    float4 dn[kPass], xn[kPass];
     auto fetch = [&](uint32_t s) {
 #pragma unroll
-        for (uint32_t q = 0; q < kE; ++q) {
-            const uint32_t e = t + 256u * q, row = e / kS, smp = s + e % kS;
-            dn[q] = (row < pr.O && smp < s1) ? dact[(size_t)(pr.d_row + row) * ld + smp] : 0.0f;
-            xn[q] = (row < pr.I && smp < s1) ? act[(size_t)(pr.x_row + row) * ld + smp] : 0.0f;
+        for (uint32_t q = 0; q < kPass; ++q) {
+            dn[q] = ld4(dp[q], s + c4);
+            xn[q] = ld4(xp[q], s + c4);
         }
     };
     fetch(s0);
     for (uint32_t s = s0; s < s1; s += kS) {
 #pragma unroll
-        for (uint32_t q = 0; q < kE; ++q) {
-            const uint32_t e = t + 256u * q;
-            Ds[(e / kS) * kP + e % kS] = dn[q];
-            Xs[(e / kS) * kP + e % kS] = xn[q];
+        for (uint32_t q = 0; q < kPass; ++q) {
+            const uint32_t row = r0 + kRowsPass * q;
+            *reinterpret_cast<float4*>(&Ds[row * kP + c4]) = dn[q];
+            *reinterpret_cast<float4*>(&Xs[row * kP + c4]) = xn[q];
         }
         __syncthreads();
         if (s + kS < s1) fetch(s + kS);
-        if (active) {
+        // one output tile (O <= 16) or all four (rows past O are zero):
+        // straight-line MFMAs, the accumulators stay in place
+        auto mma = [&](auto tiles) {
 #pragma unroll
             for (uint32_t k0 = 0; k0 < kS; k0 += 4u) {
                 const float b = Xs[(16u * wave + m) * kP + k0 + g];
 #pragma unroll
-                for (uint32_t ot = 0; ot < 4u; ++ot)
-                    if (ot < n_ot)
-                        acc[ot] = __builtin_amdgcn_mfma_f32_16x16x4f32(Ds[(16u * ot + m) * kP + k0 + g], b, acc[ot],
-                                                                        0, 0, 0);
+                for (uint32_t ot = 0; ot < decltype(tiles)::value; ++ot)
+                    acc[ot] = __builtin_amdgcn_mfma_f32_16x16x4f32(Ds[(16u * ot + m) * kP + k0 + g], b, acc[ot],
+                                                                    0, 0, 0);
             }
+        };
+        if (active) {
+            if (n_ot <= 1u)
+                mma(std::integral_constant<uint32_t, 1>{});
+            else
+                mma(std::integral_constant<uint32_t, 4>{});
         }
         __syncthreads();
     }
@@ -1116,7 +1200,7 @@ bool train_alloc(T*& p, size_t bytes) {
 void free_workspace_train(TrainState* ts) {
     dev_free(ts->ray_d); dev_free(ts->ray_o); dev_free(ts->ray_frame); dev_free(ts->ray_img); dev_free(ts->ray_pix);
     dev_free(ts->ray_src); dev_free(ts->ray_base); dev_free(ts->ray_cnt);
-    dev_free(ts->s_pos); dev_free(ts->s_cpos); dev_free(ts->s_cdir); dev_free(ts->s_out); dev_free(ts->s_hit);
+    dev_free(ts->s_cpos); dev_free(ts->s_cdir); dev_free(ts->s_out); dev_free(ts->s_hit);
     dev_free(ts->c_idx); dev_free(ts->c_dout); dev_free(ts->act); dev_free(ts->dact); dev_free(ts->part);
     dev_free(ts->t_buf);
     dev_free(ts->denc);
@@ -1142,7 +1226,7 @@ insta_status grow_train_workspace(insta_ctx* ctx, TrainState* ts, uint32_t n_ray
          train_alloc(ts->ray_frame, rc * sizeof(int32_t));
     ok = ok && train_alloc(ts->ray_img, rc * 4) && train_alloc(ts->ray_pix, rc * 4) && train_alloc(ts->ray_src, rc * 4);
     ok = ok && train_alloc(ts->ray_base, rc * 4) && train_alloc(ts->ray_cnt, rc * 4);
-    ok = ok && train_alloc(ts->s_pos, sc * sizeof(float4)) && train_alloc(ts->s_cpos, sc * sizeof(float4));
+    ok = ok && train_alloc(ts->s_cpos, sc * sizeof(float4));
     ok = ok && train_alloc(ts->s_cdir, sc * sizeof(float4)) && train_alloc(ts->s_out, sc * sizeof(float4));
     ok = ok && train_alloc(ts->s_hit, sc * sizeof(int32_t));
     ok = ok && train_alloc(ts->ray_T, rc * sizeof(float)) && train_alloc(ts->ray_alive, rc * 4) &&
@@ -1205,11 +1289,10 @@ insta_status launch_train_backward(insta_ctx* ctx, TrainState* ts, const insta_t
     const uint64_t entries = ctx->layout.n_grid_params / 2;
     // levels whose tables fit in LDS are scattered by scatter_lds_kernel,
     // the others per sample (FP32, or packed FP16 copies)
-    uint32_t lds_mask = 0;
-    for (int l = 0; l < L; ++l)
-        if (gp.size[l] <= kLdsEntries) lds_mask |= 1u << l;
-    const bool use_xh = tc.grid_grad_fp16 && lds_mask != (1u << L) - 1u;
-    const bool use_x = !tc.grid_grad_fp16 || lds_mask != 0;
+    const uint32_t lds_mask = ts->lds_levels, lds_end = ts->lds_end;
+    const bool per_sample = lds_mask != (1u << L) - 1u;
+    const bool use_xh = tc.grid_grad_fp16 && per_sample;
+    const bool use_x = !tc.grid_grad_fp16 && per_sample;
     // clear this backward's copies (fill kernels: capturable)
     if (use_xh)
         hipLaunchKernelGGL(fill_zero_kernel, dim3(2048), dim3(256), 0, stream, reinterpret_cast<uint32_t*>(ts->grad_xh),
@@ -1230,14 +1313,15 @@ insta_status launch_train_backward(insta_ctx* ctx, TrainState* ts, const insta_t
                            n_dev, ts->c_idx, ts->s_cpos, ts->act, ts->c_dout, ts->dact, cap, ts->grad_x, ts->grad_xh,
                            entries, run_mask, ts->denc, lds_mask);
     INSTA_HIP(hipGetLastError());
+    const uint32_t chunks =
+        std::min<uint32_t>(std::max<uint32_t>((cap + kScatterChunk - 1) / kScatterChunk, 1u), kScatterMaxChunks);
     if (lds_mask) {
-        const uint32_t chunks = std::min<uint32_t>(std::max<uint32_t>((cap + kScatterChunk - 1) / kScatterChunk, 1u), 64u);
-        hipLaunchKernelGGL(scatter_lds_kernel, dim3(chunks, L), dim3(kScatterThreads), 0, stream, gp, n_dev, ts->c_idx,
-                           ts->s_cpos, ts->denc, cap, ts->grad_x, entries, lds_mask);
+        hipLaunchKernelGGL(scatter_lds_kernel, dim3(chunks, __builtin_popcount(lds_mask)), dim3(kScatterThreads), 0,
+                           stream, gp, n_dev, ts->c_idx, ts->s_cpos, ts->denc, cap, run_mask, ts->grad_p, lds_end);
         INSTA_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(grad_copies_reduce_kernel, dim3(2048), dim3(256), 0, stream, ts->grad_x, ts->grad_xh, entries,
-                       use_x ? 1u : 0u, use_xh ? 1u : 0u, grad_tab, 0u);
+                       use_x ? 1u : 0u, use_xh ? 1u : 0u, ts->grad_p, lds_end, chunks, grad_tab, 0u);
     INSTA_HIP(hipGetLastError());
     WArgs wa;
     wa.p[0] = WProduct{Dact::DH1, A::ENC, kW, DIN, O::W0};
@@ -1417,7 +1501,7 @@ void free_train(insta_ctx* ctx) {
     if (!ts) return;
     free_workspace_train(ts);
     dev_free(ts->images); dev_free(ts->meta); dev_free(ts->seg); dev_free(ts->depth);
-    dev_free(ts->grad); dev_free(ts->grad_x); dev_free(ts->grad_xh);
+    dev_free(ts->grad); dev_free(ts->grad_x); dev_free(ts->grad_xh); dev_free(ts->grad_p);
     dev_free(ts->adam_m); dev_free(ts->adam_v); dev_free(ts->counters); dev_free(ts->net_extra);
     if (ctx->net_src == ts->ema_infer) ctx->net_src = nullptr;
     dev_free(ts->ema); dev_free(ts->ema_infer);
@@ -1436,11 +1520,14 @@ static insta_status ensure_train_state(insta_ctx* ctx, hipStream_t stream) {
         }
         ctx->train = ts;
         const uint64_t np = ctx->layout.n_params;
+        ts->lds_levels = lds_scatter_levels(grid_params(ctx), (int)ctx->cfg.n_levels, &ts->lds_end);
         bool ok = train_alloc(ts->grad, np * sizeof(float)) && train_alloc(ts->adam_m, np * sizeof(float)) &&
                   train_alloc(ts->adam_v, np * sizeof(float)) && train_alloc(ts->ema, np * sizeof(float)) &&
                   train_alloc(ts->ema_infer, np * sizeof(float)) &&
                   train_alloc(ts->grad_x, kXcds * ctx->layout.n_grid_params * sizeof(float)) &&
                   train_alloc(ts->grad_xh, kXcds * (ctx->layout.n_grid_params / 2) * sizeof(__half2)) &&
+                  (ts->lds_end == 0 ||
+                   train_alloc(ts->grad_p, (size_t)kScatterMaxChunks * 2 * ts->lds_end * sizeof(float))) &&
                   train_alloc(ts->counters, kTCounters * sizeof(uint32_t)) &&
                   hipHostMalloc((void**)&ts->h_counters, kTCounters * sizeof(uint32_t), hipHostMallocDefault) ==
                       hipSuccess;
@@ -1549,8 +1636,7 @@ insta_status train_step(insta_ctx* ctx, const insta_train_config& tc, insta_trai
     ra.fixed_slot = tc.occupancy_slot;
     hipLaunchKernelGGL(train_rays_kernel, dim3((n_rays + 255) / 256), dim3(256), 0, stream, ra, n_rays, ts->rays_total,
                        rng, ts->ray_d, ts->ray_o, ts->ray_frame, ts->ray_img, ts->ray_pix, ts->ray_src, ts->ray_base,
-                       ts->ray_cnt,
-                       ts->s_pos, ts->t_buf, ts->ray_T, ts->ray_alive, ts->counters);
+                       ts->ray_cnt, ts->t_buf, ts->ray_T, ts->ray_alive, ts->counters);
     INSTA_HIP(hipGetLastError());
     hipLaunchKernelGGL(train_count_kernel, dim3(1), dim3(64), 0, stream, ts->counters);
     // deform and evaluate the marched samples window by window, only while
@@ -1559,7 +1645,8 @@ insta_status train_step(insta_ctx* ctx, const insta_train_config& tc, insta_trai
     for (uint32_t w = 0; w < kRounds; ++w) {
         const uint32_t lo = kWin[w], hi = kWin[w + 1];
         hipLaunchKernelGGL(train_gather_kernel, rgrid, dim3(256), 0, stream, lo, hi, ts->ray_base, ts->ray_cnt,
-                           ts->ray_alive, ts->ray_gbase, ts->s_pos, ts->r_pos, ts->counters, kTRound0 + w);
+                           ts->ray_alive, ts->ray_gbase, ts->ray_o, ts->ray_d, ts->ray_src, ts->t_buf, n_rays,
+                           ts->r_pos, ts->counters, kTRound0 + w);
         INSTA_HIP(hipGetLastError());
         const uint32_t* n_round = ts->counters + kTRound0 + w;
         st = launch_deform_samples(ctx, -1, ts->ray_frame, n_round, cap, ts->r_pos, ts->ray_d, ts->s_hit, ts->r_cpos,
@@ -1585,7 +1672,8 @@ insta_status train_step(insta_ctx* ctx, const insta_train_config& tc, insta_trai
     la.loss_scale_per_ray = tc.loss_scale / (float)n_rays;
     la.inv_rays = 1.0f / (float)n_rays;
     hipLaunchKernelGGL(train_loss_kernel, dim3((n_rays + 255) / 256), dim3(256), 0, stream, la, rng, ts->ray_src,
-                       ts->ray_img, ts->ray_pix, ts->ray_base, ts->ray_cnt, ts->ray_o, ts->s_pos, ts->s_out,
+                       ts->ray_img, ts->ray_pix, ts->ray_base, ts->ray_cnt, ts->ray_o, ts->ray_d, ts->t_buf,
+                       n_rays, ts->s_out,
                        ts->c_idx, ts->c_dout,
                        ts->compact_cap, ts->counters);
     INSTA_HIP(hipGetLastError());
