@@ -727,7 +727,8 @@ INSTA_DEV float2 run_sum(uint32_t key, float2 v, bool& head) {
 // Hash-grid scatter of the leading levels whose tables fit in LDS (at most
 // kLdsEntries entries each): one workgroup per (level, chunk of the batch)
 // adds its samples' corner gradients into an LDS copy of the level's table
-// (LDS float atomics; on levels in run_mask a wave first sums runs of
+// (LDS float atomics, or packed FP16 atomics under H16, one per corner
+// instead of two; on levels in run_mask a wave first sums runs of
 // consecutive samples with one corner, as the per-sample scatter does), then
 // stores the whole table as its chunk's partial (plain coalesced stores, no
 // global atomics); grad_copies_reduce_kernel sums the partials in chunk
@@ -749,13 +750,16 @@ uint32_t lds_scatter_levels(const GridParams& gp, int n_levels, uint32_t* end) {
     return mask;
 }
 
+template <bool H16>
 __global__ __launch_bounds__(kScatterThreads) void scatter_lds_kernel(GridParams gp, const uint32_t* __restrict__ n_dev,
                                                                       const uint32_t* __restrict__ c_idx,
                                                                       const float4* __restrict__ cpos,
                                                                       const float* __restrict__ denc_g, uint32_t ld,
                                                                       uint32_t run_mask, float* __restrict__ grad_p,
                                                                       uint32_t lds_end) {
-    __shared__ float tab[2 * kLdsEntries];
+    // FP32 pairs (128 KB, one workgroup per CU) or packed FP16 (64 KB, two)
+    using Entry = std::conditional_t<H16, __half2, float2>;
+    __shared__ Entry tab[kLdsEntries];
     const uint32_t l = blockIdx.y;  // grid.y: the LDS-scattered levels
     const uint32_t n = *n_dev;
     const uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
@@ -764,7 +768,12 @@ __global__ __launch_bounds__(kScatterThreads) void scatter_lds_kernel(GridParams
     const uint32_t size = gp.size[l], off = gp.offset[l], res = gp.res[l];
     const bool dense_l = (gp.dense_mask >> l) & 1u;
     const float sc = gp.scale[l];
-    for (uint32_t j = threadIdx.x; j < 2 * size; j += kScatterThreads) tab[j] = 0.0f;
+    for (uint32_t j = threadIdx.x; j < size; j += kScatterThreads) {
+        if constexpr (H16)
+            tab[j] = __floats2half2_rn(0.0f, 0.0f);
+        else
+            tab[j] = make_float2(0.0f, 0.0f);
+    }
     __syncthreads();
     const bool runs = (run_mask >> l) & 1u;
     // every thread runs every round (run_sum needs whole waves); the next
@@ -817,14 +826,24 @@ __global__ __launch_bounds__(kScatterThreads) void scatter_lds_kernel(GridParams
                 lead = valid && head;
             }
             if (lead) {
-                atomicAdd(&tab[2 * idx], v.x);
-                atomicAdd(&tab[2 * idx + 1], v.y);
+                if constexpr (H16) {
+                    unsafeAtomicAdd(&tab[idx], __floats2half2_rn(v.x, v.y));
+                } else {
+                    atomicAdd(&tab[idx].x, v.x);
+                    atomicAdd(&tab[idx].y, v.y);
+                }
             }
         }
     }
     __syncthreads();
     float* __restrict__ dst = grad_p + 2u * ((size_t)blockIdx.x * lds_end + off);
-    for (uint32_t j = threadIdx.x; j < 2 * size; j += kScatterThreads) dst[j] = tab[j];
+    float2* __restrict__ dst2 = reinterpret_cast<float2*>(dst);
+    for (uint32_t j = threadIdx.x; j < size; j += kScatterThreads) {
+        if constexpr (H16)
+            dst2[j] = __half22float2(tab[j]);
+        else
+            dst2[j] = tab[j];
+    }
 }
 
 // Backward over a block of 256 compacted samples, in two phases.
@@ -1316,8 +1335,13 @@ insta_status launch_train_backward(insta_ctx* ctx, TrainState* ts, const insta_t
     const uint32_t chunks =
         std::min<uint32_t>(std::max<uint32_t>((cap + kScatterChunk - 1) / kScatterChunk, 1u), kScatterMaxChunks);
     if (lds_mask) {
-        hipLaunchKernelGGL(scatter_lds_kernel, dim3(chunks, __builtin_popcount(lds_mask)), dim3(kScatterThreads), 0,
-                           stream, gp, n_dev, ts->c_idx, ts->s_cpos, ts->denc, cap, run_mask, ts->grad_p, lds_end);
+        const dim3 grid(chunks, __builtin_popcount(lds_mask));
+        if (tc.grid_grad_fp16)
+            hipLaunchKernelGGL(scatter_lds_kernel<true>, grid, dim3(kScatterThreads), 0, stream, gp, n_dev, ts->c_idx,
+                               ts->s_cpos, ts->denc, cap, run_mask, ts->grad_p, lds_end);
+        else
+            hipLaunchKernelGGL(scatter_lds_kernel<false>, grid, dim3(kScatterThreads), 0, stream, gp, n_dev, ts->c_idx,
+                               ts->s_cpos, ts->denc, cap, run_mask, ts->grad_p, lds_end);
         INSTA_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(grad_copies_reduce_kernel, dim3(2048), dim3(256), 0, stream, ts->grad_x, ts->grad_xh, entries,

@@ -122,9 +122,9 @@ def test_fp16_grid_gradients_with_loss_scale(scene):
         print(f"  {name}: rel L2 err {e:.3e}")
         # MLP gradients stay FP32 (loss scale divides out exactly: a power of
         # two); hash-grid gradients carry FP16 accumulation error: each add
-        # rounds to 11 bits of the running sum, and coarse-level entries take
-        # thousands of adds (observed 3.2e-2)
-        assert e < (5e-2 if name == "tables" else 1e-3), name
+        # rounds to 11 bits of the running sum of its chunk of the batch
+        # (8192 samples per partial table; observed 1.4e-3)
+        assert e < (1e-2 if name == "tables" else 1e-3), name
 
 
 def test_adam_first_step_closed_form(scene):
