@@ -1673,8 +1673,13 @@ insta_status train_step(insta_ctx* ctx, const insta_train_config& tc, insta_trai
                            ts->r_pos, ts->counters, kTRound0 + w);
         INSTA_HIP(hipGetLastError());
         const uint32_t* n_round = ts->counters + kTRound0 + w;
-        st = launch_deform_samples(ctx, -1, ts->ray_frame, n_round, cap, ts->r_pos, ts->ray_d, ts->s_hit, ts->r_cpos,
-                                   ts->r_cdir, ctx->d_misses, stream);
+        // the deformation kernels stride over the round's samples: a grid for
+        // at most the window's samples and 2^20 threads, not for the sample
+        // capacity (empty workgroups of a 2^24-thread grid cost ~15 us a launch)
+        const uint32_t round_grid =
+            (uint32_t)std::min<uint64_t>(std::min<uint64_t>(cap, (uint64_t)n_rays * (hi - lo)), 1u << 20);
+        st = launch_deform_samples(ctx, -1, ts->ray_frame, n_round, round_grid, ts->r_pos, ts->ray_d, ts->s_hit,
+                                   ts->r_cpos, ts->r_cdir, ctx->d_misses, stream);
         if (st != INSTA_OK) return st;
         st = launch_network_samples(ctx, n_round, cap, ts->r_cpos, ts->r_cdir, ts->r_out, stream);
         if (st != INSTA_OK) return st;
