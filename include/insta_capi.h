@@ -474,9 +474,10 @@ typedef struct insta_train_config {
     float    learning_rate;   /* Adam */
     float    beta1, beta2, epsilon;
     float    loss_scale;      /* gradients are computed scaled by this, then unscaled */
-    uint32_t grid_grad_fp16;  /* 1: hash-grid gradients of the levels with more than 16384 entries
-                                 accumulated in FP16 (packed atomics); smaller levels are always
-                                 accumulated in FP32 */
+    uint32_t grid_grad_fp16;  /* 1: hash-grid gradients accumulated in FP16 (packed atomics): per
+                                 chunk of 8192 batch entries in LDS for levels of at most 16384
+                                 entries (the chunks then summed in FP32), per XCD copy for larger
+                                 levels; 0: FP32 throughout */
     uint32_t apply_update;    /* 0: gradients only, parameters unchanged */
     uint32_t seed;            /* PCG32 seed of the ray/background stream */
     int32_t  occupancy_slot;  /* grid the rays march: INSTA_OCCUPANCY_PER_FRAME (each image's
