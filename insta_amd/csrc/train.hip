@@ -72,9 +72,11 @@ enum : uint32_t {
 // arithmetic); a ray whose transmittance fell below 1e-4, or whose samples
 // ran out, takes no further round.  The loss reads exactly the samples the
 // rounds evaluated, so the step computes what one pass over every marched
-// sample would.
+// sample would.  Once trained, most rays stop within a sample or two of the
+// surface, so the first window holds one sample (a first window of 4 or 8
+// measured 10-90 % slower rounds).
 constexpr uint32_t kRounds = 4;
-constexpr uint32_t kWin[kRounds + 1] = {0u, 4u, 16u, 64u, 0xffffffffu};
+constexpr uint32_t kWin[kRounds + 1] = {0u, 1u, 4u, 16u, 0xffffffffu};
 
 constexpr int kW = 64;
 constexpr int kDOut = 16;
